@@ -1,0 +1,245 @@
+/*
+ * restir_amd — C ABI of the MI355X (gfx950) ReSTIR GI/DI hot path.
+ *
+ * The reference renderer (pkonecki/restir) has no plugin API: each pass is a
+ * bgfx compute program whose inputs are bound by stage number and dispatched
+ * from Renderer::render (src/renderer.cpp:626-833). Every entry point below
+ * replaces one such dispatch, keeping its inputs and outputs (G-buffer planes,
+ * reservoir planes, the hash grid buffer and the uniforms) as caller-owned
+ * device buffers with explicit formats. Calls are stream ordered and
+ * asynchronous; `stream` is a hipStream_t (NULL: the legacy default stream).
+ * Nothing crosses the ABI but plain pointers, sizes and POD structs.
+ *
+ * Conventions (what the reference leaves to bgfx's shader library and GPU):
+ *   - Matrices are 16 floats, column major (element [col * 4 + row]), and
+ *     applied as M * v.
+ *   - A texel fetch at texture coordinate uv of a W x H plane reads texel
+ *     (clamp(floor(uv.x * W), 0, W - 1), clamp(floor(uv.y * H), 0, H - 1)),
+ *     row-major, row 0 first: point sampling with clamp, as the reference's
+ *     samplers are set up (renderer.cpp:1324-1328).
+ *   - RGBA8 planes are uint32 words, red in the low byte; a channel reads as
+ *     byte / 255. RGBA16F planes are 4 IEEE binary16 values per texel, written
+ *     with round-to-nearest-even. RGBA32F planes are 4 floats per texel.
+ *   - toClipSpaceDepth(d) is d * 2 - 1 with homogeneous_depth, else d;
+ *     toClipSpaceY(y) is y with origin_bottom_left, else -y.
+ *
+ * Return value of every int function: RGI_OK, or a negative RGI_E_* code
+ * whose text rgi_last_error() returns (per thread).
+ */
+#ifndef RESTIR_AMD_H
+#define RESTIR_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RGI_ABI_VERSION 1
+
+#define RGI_OK 0
+#define RGI_E_INVALID (-1)   /* an argument is out of range or inconsistent */
+#define RGI_E_DEVICE (-2)    /* a HIP call failed */
+#define RGI_E_NOMEM (-3)     /* a device allocation failed */
+
+/* Ray modes, RAY_MODE_* in shaders/bvh.sh:17-19 (RayMode, settings.h:58-65). */
+#define RGI_RAY_SCREEN 0
+#define RGI_RAY_HYBRID 1
+#define RGI_RAY_BVH 2
+
+/* Grid insert modes (gridInsert, shaders/hashgrid.sh:182-200). */
+#define RGI_GRID_ATOMIC 0        /* the reference's atomics, in whatever order the GPU runs them */
+#define RGI_GRID_DETERMINISTIC 1 /* one fixed linearization of those atomics (see rgi_grid_insert) */
+
+/* One frame's uniforms: the u_* vec4/mat4 uniforms of the hot-path passes,
+ * one field each (restir.sh:57-71, hashgrid.sh:49-50, bvh.sh:14, di.sh:35-39,
+ * shadow.sh:23-25, sky.sh:11-13, cs_di_temporal.sc:28-30), packed by the
+ * reference at renderer.cpp:497-531, 614-621, 632-640, 693-731. Integers the
+ * reference passes as floats are integers here. */
+typedef struct rgi_frame {
+    float inv_view_proj[16];      /* u_sceneInvViewProj */
+    float view_proj[16];          /* u_sceneViewProj */
+    float prev_view_proj[16];     /* u_previousViewProj */
+    float camera_position[4];     /* u_cameraPosition; u_gridCamera of insert and GI spatial */
+    float prev_camera_position[4];/* u_diPrevious; u_gridCamera of GI temporal (renderer.cpp:813) */
+    float light_position[4];      /* u_lightPosition (main point light) */
+    float light_color[4];         /* u_lightColor */
+    uint32_t frame_index;         /* u_frameInfo.x (frame number & 0xffffff) */
+    int32_t history_valid;        /* u_frameInfo.y */
+    int32_t origin_bottom_left;   /* u_frameInfo.z */
+    int32_t multiple_bounces;     /* u_frameInfo.w */
+    int32_t homogeneous_depth;    /* bgfx caps homogeneousDepth (toClipSpaceDepth) */
+    int32_t gi_width;             /* u_screenInfo.x: GI target size */
+    int32_t gi_height;            /* u_screenInfo.y */
+    int32_t width;                /* u_screenSize.x: full-resolution size (DI passes) */
+    int32_t height;               /* u_screenSize.y */
+    float scene_scale;            /* u_sceneInfo.x */
+    float boiling_filter;         /* u_sceneInfo.y (0: off) */
+    float jacobian_clamp;         /* u_sceneInfo.z (0: off) */
+    float outlier_clamp;          /* u_sceneInfo.w (0: off) */
+    float lighting_surface;       /* u_lightingInfo.x: scale of the surface light plane */
+    float lighting_emission;      /* u_lightingInfo.y: scale of off-screen triangle emission */
+    int32_t initial_candidates;   /* u_restirOptions.x */
+    int32_t temporal_taps;        /* u_restirOptions.y */
+    int32_t spatial_taps;         /* u_restirOptions.z */
+    int32_t trace_steps;          /* u_restirOptions.w */
+    float grid_min_cell;          /* u_gridInfo.x: smallest cell size */
+    float grid_pixel_size;        /* u_gridInfo.y: world size of a GI pixel at unit distance */
+    float grid_cell_pixels;       /* u_gridInfo.z: cell size in GI pixels */
+    int32_t world_space_reuse;    /* u_gridInfo.w */
+    uint32_t grid_table_size;     /* kGridTableSize (hashgrid.sh:53); any power of two >= 64 */
+    int32_t ray_mode;             /* u_bvhInfo.w: RGI_RAY_* */
+    float sky_zenith[4];          /* u_skyZenith (w: 1 when the sky is on) */
+    float sky_horizon[4];         /* u_skyHorizon */
+    float sky_ground[4];          /* u_skyGround */
+    float shadow_view_proj[96];   /* u_shadowViewProj[6] */
+    float shadow_info[4];         /* u_shadowInfo: far plane, tan(half fov), 1 / face size, origin bottom-left */
+    int32_t shadows_enabled;      /* u_shadowOptions.x */
+    int32_t shadow_width;         /* shadow atlas size in texels (3 x 2 faces) */
+    int32_t shadow_height;
+    int32_t di_light_count;       /* u_diInfo.x */
+    int32_t di_candidates;        /* u_diInfo.y */
+    int32_t di_spatial_taps;      /* u_diInfo.z */
+    int32_t di_history_valid;     /* u_diInfo.w */
+    int32_t reserved[3];
+} rgi_frame;
+
+/* The scene: the BVH buffer in the layout of src/bvh.h:9-22 (nodes of 2 vec4,
+ * then triangle records of 4 vec4, indices stored as float values) and the
+ * light list in the layout of src/lights.h:7-14 (4 vec4 per light). */
+typedef struct rgi_scene {
+    const float* bvh;             /* b_scene: (2 * node_count + 4 * triangle_count) float4 */
+    uint32_t node_count;          /* u_bvhInfo.z (0: no BVH) */
+    uint32_t triangle_offset;     /* u_bvhInfo.x, in float4 (= 2 * node_count) */
+    uint32_t triangle_count;
+    uint32_t light_count;
+    const float* lights;          /* b_lights: 4 * light_count float4 */
+} rgi_scene;
+
+/* G-buffer planes at one resolution (the GI size for the GI passes and the
+ * grid insert, the full size for the DI passes). Planes a pass does not read
+ * may be NULL. */
+typedef struct rgi_gbuffer {
+    int32_t width;
+    int32_t height;
+    const float* depth;           /* D32F window depth; >= 0.99999 is the sky */
+    const uint32_t* albedo;       /* RGBA8: base colour, a: roughness */
+    const uint32_t* normal;       /* RGBA8: normal * 0.5 + 0.5, a: metallic */
+    const uint32_t* shading_normal; /* RGBA8: shading normal * 0.5 + 0.5 (DI) */
+    const uint16_t* emission;     /* RGBA16F: emitted radiance (GI-only lighting's surface light) */
+} rgi_gbuffer;
+
+/* One frame's GI reservoirs: three RGBA32F planes at the GI size
+ * (restir.sh:11-15): sample position + M, receiver position + W, receiver
+ * normal + sample id. */
+typedef struct rgi_gi_reservoirs {
+    float* sample;
+    float* receiver;
+    float* normal;
+} rgi_gi_reservoirs;
+
+/* What the GI passes read besides the G-buffer (setGiPassInputs,
+ * renderer.cpp:769-798). */
+typedef struct rgi_gi_inputs {
+    const float* history_sample;   /* last frame's reservoirs (stage 3 of the temporal pass) */
+    const float* history_receiver; /* stage 9 */
+    const float* history_normal;   /* stage 6 */
+    const uint16_t* history_color; /* RGBA16F, stage 5: last frame's (filtered) demodulated GI */
+    const uint16_t* history_moments; /* RGBA16F, stage 8 (spatial pass) */
+    const uint16_t* surface_light; /* RGBA16F, stage 14: DI diffuse, or the G-buffer emission */
+    const float* shadow_map;       /* R32F atlas, stage 7 (NULL when shadows are off) */
+    const uint32_t* grid;          /* stage 11: last frame's grid (temporal), this frame's (spatial) */
+} rgi_gi_inputs;
+
+/* DI reservoirs at full resolution (di.sh:12-16): sample RGBA32F (point on
+ * the light, light index), receiver RGBA32F (position, W), normal RGBA16F
+ * (receiver normal, M). */
+typedef struct rgi_di_reservoirs {
+    float* sample;
+    float* receiver;
+    uint16_t* normal;
+} rgi_di_reservoirs;
+
+const char* rgi_last_error(void);
+int rgi_abi_version(void);
+/* sizeof of the ABI structs (0: rgi_frame, 1: rgi_scene, 2: rgi_gbuffer,
+ * 3: rgi_gi_reservoirs, 4: rgi_gi_inputs, 5: rgi_di_reservoirs), so that
+ * bindings can check their mirrors. */
+uint32_t rgi_abi_sizeof(int which);
+
+/* Library state: scratch buffers of the deterministic grid insert and of the
+ * traversal queue, grown on demand. One context per host thread. */
+typedef struct rgi_context rgi_context;
+int rgi_create(rgi_context** out_context);
+void rgi_destroy(rgi_context* context);
+
+/* cs_grid_clear (shaders/cs_grid_clear.sc:12-21, dispatched at
+ * renderer.cpp:735-738): zeroes the checksums and counts of `grid`, a buffer
+ * of (2 + 32) * table_size uint32 laid out [checksums | counts | entries]. */
+int rgi_grid_clear(uint32_t* grid, uint32_t table_size, void* stream);
+
+/* cs_grid_insert (shaders/cs_grid_insert.sc:40-73, dispatched at
+ * renderer.cpp:740-750): lists every GI pixel of `gbuffer` (depth and normal,
+ * at the GI size) in the cell of its jittered receiver. The grid camera is
+ * frame->camera_position. RGI_GRID_ATOMIC runs gridInsert's atomics as the GPU
+ * schedules them. RGI_GRID_DETERMINISTIC gives the result of one legal order
+ * of them: probe round by probe round, pixels in ascending index within a
+ * round; a cell's count is every pixel resolved to it, and its entries are
+ * its 32 lowest pixel indices, ascending. */
+int rgi_grid_insert(rgi_context* context, const rgi_frame* frame, const rgi_gbuffer* gbuffer,
+                    uint32_t* grid, int mode, void* stream);
+
+/* cs_restir_temporal (shaders/cs_restir_temporal.sc:57-297, dispatched at
+ * renderer.cpp:807-817): fresh candidates, temporal reuse through last
+ * frame's grid (inputs->grid), boiling filter; writes this frame's GI
+ * reservoirs. */
+int rgi_gi_temporal(rgi_context* context, const rgi_frame* frame, const rgi_scene* scene,
+                    const rgi_gbuffer* gbuffer, const rgi_gi_inputs* inputs,
+                    const rgi_gi_reservoirs* out, void* stream);
+
+/* cs_restir_spatial (shaders/cs_restir_spatial.sc:95-246, dispatched at
+ * renderer.cpp:821-833): spatial reuse through this frame's grid
+ * (inputs->grid) over this frame's reservoirs, shading, temporal
+ * accumulation. Writes RGBA16F color (demodulated estimate, history length)
+ * and moments (mean luminance, mean squared luminance, history length, 0). */
+int rgi_gi_spatial(rgi_context* context, const rgi_frame* frame, const rgi_scene* scene,
+                   const rgi_gbuffer* gbuffer, const rgi_gi_inputs* inputs,
+                   const rgi_gi_reservoirs* reservoirs, uint16_t* out_color, uint16_t* out_moments,
+                   void* stream);
+
+/* cs_di_temporal (shaders/cs_di_temporal.sc:32-180, dispatched at
+ * renderer.cpp:660-671), at full resolution. */
+int rgi_di_temporal(rgi_context* context, const rgi_frame* frame, const rgi_scene* scene,
+                    const rgi_gbuffer* gbuffer, const rgi_di_reservoirs* previous,
+                    const rgi_di_reservoirs* out, void* stream);
+
+/* cs_di_spatial (shaders/cs_di_spatial.sc:30-169, dispatched at
+ * renderer.cpp:673-681): writes RGBA16F diffuse and specular radiance. */
+int rgi_di_spatial(rgi_context* context, const rgi_frame* frame, const rgi_scene* scene,
+                   const rgi_gbuffer* gbuffer, const rgi_di_reservoirs* reservoirs,
+                   uint16_t* out_diffuse, uint16_t* out_specular, void* stream);
+
+/* traceBvh (shaders/bvh.sh:298-319) over `count` rays, each 8 floats:
+ * origin.xyz, max distance, direction.xyz, unused. Closest hit replays
+ * traceBvhSharedStack's visit order (bvh.sh:216-295); any hit is
+ * traceBvhStackless (bvh.sh:140-174). Writes the triangle (BVH order, -1 for
+ * a miss) and the distance (the max distance on a miss). */
+int rgi_trace(rgi_context* context, const rgi_scene* scene, const float* rays, uint32_t count,
+              int32_t* hit_triangle, float* hit_distance, int any_hit, void* stream);
+
+/* The G-buffer of the scene seen through `frame` (inv_view_proj,
+ * camera_position, homogeneous_depth, origin_bottom_left, width x height), one
+ * primary ray per pixel centre through the BVH: the producer of the GI
+ * passes' inputs, standing in for fs_gbuffer.sc. `materials` holds one float4
+ * per material index (roughness, metallic, 0, 0). Writes depth, albedo,
+ * normal, shading normal (= normal) and emission planes. */
+int rgi_gbuffer_primary(rgi_context* context, const rgi_frame* frame, const rgi_scene* scene,
+                        const float* materials, uint32_t material_count, float* depth,
+                        uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal,
+                        uint16_t* emission, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RESTIR_AMD_H */

@@ -1,0 +1,79 @@
+/*
+ * Host oracle of the ReSTIR GI/DI hot path — TEST INFRASTRUCTURE ONLY.
+ *
+ * A clean-room C++ restatement of the reference's shader logic
+ * (shaders/hashgrid.sh, cs_grid_clear.sc, cs_grid_insert.sc, compute.sh,
+ * bvh.sh, restir.sh, cs_restir_temporal.sc, cs_restir_spatial.sc, di.sh,
+ * cs_di_temporal.sc, cs_di_spatial.sc, brdf.sh, shadow.sh, sky.sh), run on the
+ * CPU over host copies of the same buffers the GPU passes take. It is the
+ * checker of the HIP kernels and the CPU baseline of the benchmark; nothing
+ * in the product links or calls it. Every function cites the reference lines
+ * it follows.
+ *
+ * All pointers in the rgi_* structs are HOST pointers here. Deterministic
+ * only: the grid insert runs the fixed linearization of the reference's
+ * atomics documented at rgi_grid_insert (include/restir_amd.h).
+ */
+#ifndef RESTIR_ORACLE_H
+#define RESTIR_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/restir_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Number of OpenMP threads the oracle uses (0: the OpenMP default). */
+void orc_set_threads(int threads);
+int orc_get_threads(void);
+
+/* Scalar primitives, for known-answer tests. */
+uint32_t orc_pcg(uint32_t value);                       /* gridHash / pcg, hashgrid.sh:61-67 */
+uint32_t orc_checksum_hash(uint32_t value);             /* gridChecksumHash, hashgrid.sh:69-78 */
+float orc_next_random(uint32_t* state);                 /* nextRandom, restir.sh:131-135 */
+float orc_grid_cell_size(const rgi_frame* frame, const float* position3, const float* camera3); /* hashgrid.sh:82-87 */
+uint32_t orc_grid_normal_bucket(const float* normal3);  /* hashgrid.sh:89-103 */
+void orc_grid_key(const rgi_frame* frame, const float* position3, const float* normal3, float cell_size,
+                  uint32_t* slot, uint32_t* checksum);  /* hashgrid.sh:116-127 */
+void orc_cosine_hemisphere(const float* normal3, float r0, float r1, float* out3); /* restir.sh:201-212 */
+void orc_sincos(float x, float* s, float* c);
+uint16_t orc_half_bits(float value);
+
+/* Per-pixel keys of cs_grid_insert (cs_grid_insert.sc:40-73): slot, checksum
+ * (0 for sky pixels, which insert nothing) and the jittered cell size. */
+void orc_grid_keys(const rgi_frame* frame, const rgi_gbuffer* gbuffer, uint32_t* out_slot,
+                   uint32_t* out_checksum, float* out_cell_size);
+
+void orc_grid_clear(uint32_t* grid, uint32_t table_size);          /* cs_grid_clear.sc:12-21 */
+void orc_grid_insert(const rgi_frame* frame, const rgi_gbuffer* gbuffer, uint32_t* grid); /* deterministic */
+uint32_t orc_grid_pick_entry(const rgi_frame* frame, const uint32_t* grid, const float* camera3,
+                             const float* position3, const float* normal3, float r0, float r1,
+                             float r2);                               /* hashgrid.sh:155-177 */
+
+/* traceBvh over rays of 8 floats (origin, tmax, direction, unused), as
+ * rgi_trace. */
+void orc_trace(const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle,
+               float* hit_distance, int any_hit);
+
+void orc_gi_temporal(const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
+                     const rgi_gi_inputs* inputs, const rgi_gi_reservoirs* out);
+void orc_gi_spatial(const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
+                    const rgi_gi_inputs* inputs, const rgi_gi_reservoirs* reservoirs,
+                    uint16_t* out_color, uint16_t* out_moments);
+void orc_di_temporal(const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
+                     const rgi_di_reservoirs* previous, const rgi_di_reservoirs* out);
+void orc_di_spatial(const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
+                    const rgi_di_reservoirs* reservoirs, uint16_t* out_diffuse, uint16_t* out_specular);
+
+/* The G-buffer producer of rgi_gbuffer_primary, on the CPU. */
+void orc_gbuffer_primary(const rgi_frame* frame, const rgi_scene* scene, const float* materials,
+                         uint32_t material_count, float* depth, uint32_t* albedo, uint32_t* normal,
+                         uint32_t* shading_normal, uint16_t* emission);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

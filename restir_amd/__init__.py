@@ -1,0 +1,315 @@
+"""restir_amd: the ReSTIR GI/DI hot path on MI355X (gfx950), behind a C ABI.
+
+This package is the Python side of the C ABI declared in include/restir_amd.h
+and include/restir_amd_scene.h: ctypes mirrors of the ABI structs, a loader
+for the in-tree librestir_amd.so (it fails loudly when the library is
+missing; there is no fallback path), the host-side scene helpers, and
+`Settings`/`make_frame`, which pack the per-frame uniforms the way the
+reference renderer does (src/renderer.cpp:497-531, 614-621, 632-640,
+693-731; defaults from src/settings.h:183-227).
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librestir_amd.so")
+
+RGI_OK = 0
+RGI_RAY_SCREEN, RGI_RAY_HYBRID, RGI_RAY_BVH = 0, 1, 2
+RGI_GRID_ATOMIC, RGI_GRID_DETERMINISTIC = 0, 1
+RGI_SCENE_CORNELL, RGI_SCENE_ATRIUM, RGI_SCENE_BOXFIELD = 0, 1, 2
+
+F, I32, U32, P = ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p
+
+
+class Frame(ctypes.Structure):
+    """rgi_frame (include/restir_amd.h)."""
+    _fields_ = [
+        ("inv_view_proj", F * 16), ("view_proj", F * 16), ("prev_view_proj", F * 16),
+        ("camera_position", F * 4), ("prev_camera_position", F * 4),
+        ("light_position", F * 4), ("light_color", F * 4),
+        ("frame_index", U32), ("history_valid", I32), ("origin_bottom_left", I32),
+        ("multiple_bounces", I32), ("homogeneous_depth", I32),
+        ("gi_width", I32), ("gi_height", I32), ("width", I32), ("height", I32),
+        ("scene_scale", F), ("boiling_filter", F), ("jacobian_clamp", F), ("outlier_clamp", F),
+        ("lighting_surface", F), ("lighting_emission", F),
+        ("initial_candidates", I32), ("temporal_taps", I32), ("spatial_taps", I32), ("trace_steps", I32),
+        ("grid_min_cell", F), ("grid_pixel_size", F), ("grid_cell_pixels", F), ("world_space_reuse", I32),
+        ("grid_table_size", U32), ("ray_mode", I32),
+        ("sky_zenith", F * 4), ("sky_horizon", F * 4), ("sky_ground", F * 4),
+        ("shadow_view_proj", F * 96), ("shadow_info", F * 4), ("shadows_enabled", I32),
+        ("shadow_width", I32), ("shadow_height", I32),
+        ("di_light_count", I32), ("di_candidates", I32), ("di_spatial_taps", I32), ("di_history_valid", I32),
+        ("reserved", I32 * 3),
+    ]
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [("bvh", P), ("node_count", U32), ("triangle_offset", U32), ("triangle_count", U32),
+                ("light_count", U32), ("lights", P)]
+
+
+class GBuffer(ctypes.Structure):
+    _fields_ = [("width", I32), ("height", I32), ("depth", P), ("albedo", P), ("normal", P),
+                ("shading_normal", P), ("emission", P)]
+
+
+class GiReservoirs(ctypes.Structure):
+    _fields_ = [("sample", P), ("receiver", P), ("normal", P)]
+
+
+class GiInputs(ctypes.Structure):
+    _fields_ = [("history_sample", P), ("history_receiver", P), ("history_normal", P), ("history_color", P),
+                ("history_moments", P), ("surface_light", P), ("shadow_map", P), ("grid", P)]
+
+
+class DiReservoirs(ctypes.Structure):
+    _fields_ = [("sample", P), ("receiver", P), ("normal", P)]
+
+
+class Mesh(ctypes.Structure):
+    _fields_ = [("triangle_count", U32), ("positions", P), ("albedo", P), ("emission", P), ("material", P),
+                ("material_count", U32), ("materials", P)]
+
+
+ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs]
+
+
+class RestirError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded librestir_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RestirError("librestir_amd.so is not built (%s): run python -m restir_amd.build" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.rgi_last_error.restype = ctypes.c_char_p
+    L.rgi_abi_sizeof.restype = U32
+    L.rgi_abi_sizeof.argtypes = [ctypes.c_int]
+    L.rgi_create.argtypes = [ctypes.POINTER(P)]
+    L.rgi_destroy.argtypes = [P]
+    L.rgi_destroy.restype = None
+    L.rgi_grid_clear.argtypes = [P, U32, P]
+    L.rgi_grid_insert.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, ctypes.c_int, P]
+    L.rgi_gi_temporal.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), ctypes.POINTER(GBuffer),
+                                  ctypes.POINTER(GiInputs), ctypes.POINTER(GiReservoirs), P]
+    L.rgi_gi_spatial.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), ctypes.POINTER(GBuffer),
+                                 ctypes.POINTER(GiInputs), ctypes.POINTER(GiReservoirs), P, P, P]
+    L.rgi_di_temporal.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), ctypes.POINTER(GBuffer),
+                                  ctypes.POINTER(DiReservoirs), ctypes.POINTER(DiReservoirs), P]
+    L.rgi_di_spatial.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), ctypes.POINTER(GBuffer),
+                                 ctypes.POINTER(DiReservoirs), P, P, P]
+    L.rgi_trace.argtypes = [P, ctypes.POINTER(Scene), P, U32, P, P, ctypes.c_int, P]
+    L.rgi_gbuffer_primary.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), P, U32, P, P, P, P, P, P]
+    L.rgi_scene_generate.restype = ctypes.c_int64
+    L.rgi_scene_generate.argtypes = [ctypes.c_int, U32, U32, ctypes.POINTER(Mesh), U32]
+    L.rgi_mesh_bounds.restype = None
+    L.rgi_mesh_bounds.argtypes = [ctypes.POINTER(Mesh), P, P, P, P]
+    L.rgi_bvh_capacity.restype = ctypes.c_uint64
+    L.rgi_bvh_capacity.argtypes = [U32]
+    L.rgi_bvh_build.restype = ctypes.c_int64
+    L.rgi_bvh_build.argtypes = [ctypes.POINTER(Mesh), P, P]
+    L.rgi_lights_build.restype = ctypes.c_int64
+    L.rgi_lights_build.argtypes = [P, U32, U32, I32, F, U32, P, P, F, P, U32]
+    L.rgi_camera_matrices.restype = None
+    L.rgi_camera_matrices.argtypes = [P, P, F, F, F, F, ctypes.c_int, P, P, P, P]
+    for i, cls in enumerate(ABI_STRUCTS):
+        size = L.rgi_abi_sizeof(i)
+        if size != ctypes.sizeof(cls):
+            raise RestirError("ABI mismatch: %s is %d bytes in C, %d in Python" % (cls.__name__, size, ctypes.sizeof(cls)))
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != RGI_OK:
+        raise RestirError("restir_amd error %d: %s" % (status, lib().rgi_last_error().decode()))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------------ scene
+
+class HostScene:
+    """A procedural scene on the host: mesh, BVH (bvh.h layout), light list."""
+
+    def __init__(self, kind=RGI_SCENE_CORNELL, seed=1, size=0, extra_lights=0, extra_intensity=0.3, light_seed=1):
+        L = lib()
+        count = L.rgi_scene_generate(kind, seed, size, None, 0)
+        if count < 0:
+            raise RestirError("unknown scene kind %d" % kind)
+        self.positions = np.zeros((count, 9), np.float32)
+        self.albedo = np.zeros((count, 3), np.float32)
+        self.emission = np.zeros((count, 3), np.float32)
+        self.material = np.zeros(count, np.uint32)
+        self.materials = np.zeros((16, 4), np.float32)
+        mesh = Mesh(0, self.positions.ctypes.data, self.albedo.ctypes.data, self.emission.ctypes.data,
+                    self.material.ctypes.data, 0, self.materials.ctypes.data)
+        got = L.rgi_scene_generate(kind, seed, size, ctypes.byref(mesh), count)
+        assert got == count
+        self.material_count = min(mesh.material_count, 16)
+        self.triangle_count = int(count)
+        self.bounds_min = np.zeros(3, np.float32)
+        self.bounds_max = np.zeros(3, np.float32)
+        self.center = np.zeros(3, np.float32)
+        radius = ctypes.c_float()
+        L.rgi_mesh_bounds(ctypes.byref(mesh), _ptr(self.bounds_min), _ptr(self.bounds_max), _ptr(self.center),
+                          ctypes.byref(radius))
+        self.radius = float(radius.value)
+        cap = L.rgi_bvh_capacity(count)
+        buf = np.zeros(cap, np.float32)
+        depth = ctypes.c_uint32()
+        nodes = L.rgi_bvh_build(ctypes.byref(mesh), _ptr(buf), ctypes.byref(depth))
+        if nodes < 0:
+            raise RestirError("BVH build failed")
+        self.node_count = int(nodes)
+        self.triangle_offset = 2 * self.node_count
+        self.bvh_depth = int(depth.value)
+        self.bvh = buf[: (2 * self.node_count + 4 * count) * 4].copy()
+        # buildLightList's scale of an intensity of 1: the squared scene scale (renderer.cpp:497-503).
+        self.scene_scale = self.radius / 1.75
+        nl = L.rgi_lights_build(_ptr(self.bvh), self.node_count, count, extra_lights, extra_intensity, light_seed,
+                                _ptr(self.bounds_min), _ptr(self.bounds_max), self.scene_scale ** 2, None, 0)
+        self.lights = np.zeros((max(nl, 1), 16), np.float32)
+        L.rgi_lights_build(_ptr(self.bvh), self.node_count, count, extra_lights, extra_intensity, light_seed,
+                           _ptr(self.bounds_min), _ptr(self.bounds_max), self.scene_scale ** 2,
+                           _ptr(self.lights), max(nl, 1))
+        self.light_count = int(nl)
+        self.lights = self.lights[: max(nl, 1)]
+
+    def rgi_scene(self, bvh_ptr, lights_ptr):
+        return Scene(bvh_ptr, self.node_count, self.triangle_offset, self.triangle_count, self.light_count, lights_ptr)
+
+
+def camera_matrices(eye, target, fovy=48.0, aspect=1.0, near=0.05, far=10.0, homogeneous_depth=False):
+    L = lib()
+    e = np.asarray(eye, np.float32)
+    t = np.asarray(target, np.float32)
+    view, proj, vp, inv = (np.zeros(16, np.float32) for _ in range(4))
+    L.rgi_camera_matrices(_ptr(e), _ptr(t), fovy, aspect, near, far, int(homogeneous_depth), _ptr(view), _ptr(proj),
+                          _ptr(vp), _ptr(inv))
+    return view, proj, vp, inv
+
+
+def orbit_camera(scene, frame=0, width=320, height=240, yaw=0.64, pitch=0.255, distance=3.17, yaw_step=0.0,
+                 homogeneous_depth=False):
+    """The reference's starting framing (main.cpp:60-62, camera.cpp:112-122): `distance`
+    scene radii from the centre at (yaw, pitch), fovy 48; yaw advances by yaw_step per frame."""
+    y = yaw + yaw_step * frame
+    r = distance * scene.radius
+    c = scene.center
+    eye = np.array([c[0] + r * math.cos(pitch) * math.sin(y), c[1] + r * math.sin(pitch),
+                    c[2] + r * math.cos(pitch) * math.cos(y)], np.float32)
+    far = max(float(np.linalg.norm(eye - c)) + 2.0 * scene.radius, 2.5 * scene.radius)
+    view, proj, vp, inv = camera_matrices(eye, c, 48.0, width / height, 0.05 * scene.radius, far, homogeneous_depth)
+    return eye, vp, inv
+
+
+# --------------------------------------------------------------- settings
+
+class Settings:
+    """The hot path's part of the reference's Settings (settings.h:183-227)."""
+
+    def __init__(self, **kw):
+        self.initial_candidates = 2
+        self.temporal_taps = 2
+        self.spatial_taps = 4
+        self.trace_steps = 32
+        self.di_candidates = 8
+        self.di_spatial_taps = 3
+        self.ray_mode = RGI_RAY_HYBRID
+        self.world_space_reuse = True
+        self.multiple_bounces = True
+        self.clamp_jacobian = True
+        self.clamp_outliers = True
+        self.boiling_filter = True
+        self.restir_di = True          # DirectLighting::ReStirDi (else GiOnly)
+        self.emissive_lighting = True
+        self.sky = False               # off for the Cornell box (main.cpp:80)
+        self.sky_zenith = (0.30, 0.50, 0.95)
+        self.sky_horizon = (0.85, 0.88, 0.92)
+        self.sky_ground = (0.32, 0.29, 0.26)
+        self.sky_intensity = 1.0
+        self.light_color = (0.0, 0.0, 0.0)  # main point light off
+        self.light_position = (0.0, 1.7, 0.0)
+        self.shadows = False
+        self.grid_table_size = 65536
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(k)
+            setattr(self, k, v)
+
+    def reference_mode(self):
+        """Effective settings while the reference accumulates (renderer.cpp:437-452)."""
+        s = Settings(**self.__dict__)
+        s.temporal_taps = 0
+        s.spatial_taps = 0
+        s.clamp_jacobian = False
+        s.clamp_outliers = False
+        s.boiling_filter = False
+        s.di_spatial_taps = 0
+        return s
+
+
+def make_frame(settings, scene, frame_index, eye, vp, inv_vp, prev_eye, prev_vp, width, height, history_valid,
+               di_history_valid=False, gi_divisor=1, origin_bottom_left=False, homogeneous_depth=False):
+    """Packs rgi_frame as Renderer::render packs its uniforms."""
+    s = settings
+    f = Frame()
+    f.inv_view_proj[:] = [float(v) for v in inv_vp]
+    f.view_proj[:] = [float(v) for v in vp]
+    f.prev_view_proj[:] = [float(v) for v in prev_vp]
+    f.camera_position[:] = [float(eye[0]), float(eye[1]), float(eye[2]), 1.0]
+    f.prev_camera_position[:] = [float(prev_eye[0]), float(prev_eye[1]), float(prev_eye[2]), 1.0]
+    f.light_position[:] = [float(v) for v in s.light_position] + [1.0]
+    scale = scene.scene_scale
+    f.light_color[:] = [c * scale * scale for c in s.light_color] + [1.0]
+    f.frame_index = frame_index & 0xFFFFFF
+    f.history_valid = int(history_valid)
+    f.origin_bottom_left = int(origin_bottom_left)
+    f.multiple_bounces = int(s.multiple_bounces)
+    f.homogeneous_depth = int(homogeneous_depth)
+    gw, gh = max(1, width // gi_divisor), max(1, height // gi_divisor)
+    f.gi_width, f.gi_height, f.width, f.height = gw, gh, width, height
+    f.scene_scale = scale
+    f.boiling_filter = 4.0 if s.boiling_filter else 0.0
+    f.jacobian_clamp = 10.0 if s.clamp_jacobian else 0.0
+    f.outlier_clamp = 8.0 if s.clamp_outliers else 0.0
+    f.lighting_surface = 1.0 if s.restir_di else (1.0 if s.emissive_lighting else 0.0)
+    f.lighting_emission = 0.0 if s.restir_di else (1.0 if s.emissive_lighting else 0.0)
+    f.initial_candidates = s.initial_candidates
+    f.temporal_taps = s.temporal_taps
+    f.spatial_taps = s.spatial_taps
+    f.trace_steps = s.trace_steps
+    f.grid_min_cell = 0.02 * scale
+    f.grid_pixel_size = 2.0 * math.tan(0.5 * math.radians(48.0)) / gh
+    f.grid_cell_pixels = 6.0
+    f.world_space_reuse = int(s.world_space_reuse)
+    f.grid_table_size = s.grid_table_size
+    f.ray_mode = s.ray_mode
+    w = 1.0 if s.sky else 0.0
+    f.sky_zenith[:] = [c * s.sky_intensity for c in s.sky_zenith] + [w]
+    f.sky_horizon[:] = [c * s.sky_intensity for c in s.sky_horizon] + [0.0]
+    f.sky_ground[:] = [c * s.sky_intensity for c in s.sky_ground] + [0.0]
+    f.shadows_enabled = int(s.shadows)
+    f.di_light_count = scene.light_count if s.restir_di else 0
+    f.di_candidates = s.di_candidates
+    f.di_spatial_taps = s.di_spatial_taps
+    f.di_history_valid = int(di_history_valid)
+    return f
+
+
+def frame_bytes(frame):
+    return bytes(frame)

@@ -1,0 +1,253 @@
+// The C ABI (include/restir_amd.h): argument checks, error text, the
+// context's scratch memory, and dispatch to the kernel files.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "internal.h"
+
+namespace rgi {
+
+static thread_local char g_error[512] = "";
+
+int fail(int code, const char* fmt, ...)
+{
+    va_list args;
+    va_start(args, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, args);
+    va_end(args);
+    return code;
+}
+
+int check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(RGI_E_DEVICE, "%s: launch failed: %s", what, hipGetErrorString(e));
+    return RGI_OK;
+}
+
+void* scratch(rgi_context* ctx, size_t bytes)
+{
+    if (ctx->scratch_bytes >= bytes) return ctx->scratch;
+    if (ctx->scratch) {
+        // Work queued on any stream may still use the old buffer.
+        hipDeviceSynchronize();
+        hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    size_t grown = bytes + bytes / 4;
+    if (hipMalloc(&ctx->scratch, grown) != hipSuccess) {
+        ctx->scratch = nullptr;
+        fail(RGI_E_NOMEM, "scratch allocation of %zu bytes failed", grown);
+        return nullptr;
+    }
+    ctx->scratch_bytes = grown;
+    return ctx->scratch;
+}
+
+static bool pow2(uint32_t v) { return v != 0 && (v & (v - 1)) == 0; }
+static bool aligned(const void* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; }
+
+static int check_frame(const rgi_frame* f)
+{
+    if (!f) return fail(RGI_E_INVALID, "frame is NULL");
+    if (!pow2(f->grid_table_size) || f->grid_table_size < 64u || f->grid_table_size > (1u << 26))
+        return fail(RGI_E_INVALID, "grid_table_size %u is not a power of two in [64, 2^26]", f->grid_table_size);
+    if (f->ray_mode < RGI_RAY_SCREEN || f->ray_mode > RGI_RAY_BVH)
+        return fail(RGI_E_INVALID, "ray_mode %d is not one of RGI_RAY_*", f->ray_mode);
+    if (f->initial_candidates < 0 || f->temporal_taps < 0 || f->spatial_taps < 0 || f->trace_steps < 0 ||
+        f->di_candidates < 0 || f->di_spatial_taps < 0 || f->di_light_count < 0)
+        return fail(RGI_E_INVALID, "negative sample counts in frame");
+    return RGI_OK;
+}
+
+static int check_gbuffer(const rgi_gbuffer* g, bool need_albedo, bool need_shading)
+{
+    if (!g) return fail(RGI_E_INVALID, "gbuffer is NULL");
+    if (g->width <= 0 || g->height <= 0 || (int64_t)g->width * g->height >= (1ll << 29))
+        return fail(RGI_E_INVALID, "gbuffer size %d x %d out of range", g->width, g->height);
+    if (!g->depth || !g->normal) return fail(RGI_E_INVALID, "gbuffer depth and normal planes are required");
+    if (need_albedo && !g->albedo) return fail(RGI_E_INVALID, "gbuffer albedo plane is required");
+    if (need_shading && !g->shading_normal) return fail(RGI_E_INVALID, "gbuffer shading_normal plane is required");
+    return RGI_OK;
+}
+
+static int check_scene(const rgi_scene* s, bool need_lights)
+{
+    if (!s) return fail(RGI_E_INVALID, "scene is NULL");
+    if (s->node_count > 0 && !s->bvh) return fail(RGI_E_INVALID, "scene has nodes but no bvh buffer");
+    if (s->node_count > 0 && s->triangle_offset != 2u * s->node_count)
+        return fail(RGI_E_INVALID, "triangle_offset %u != 2 * node_count %u", s->triangle_offset, s->node_count);
+    if (s->bvh && !aligned(s->bvh, 16)) return fail(RGI_E_INVALID, "bvh buffer must be 16-byte aligned");
+    if (need_lights && s->light_count > 0 && (!s->lights || !aligned(s->lights, 16)))
+        return fail(RGI_E_INVALID, "lights buffer missing or not 16-byte aligned");
+    return RGI_OK;
+}
+
+} // namespace rgi
+
+using namespace rgi;
+
+extern "C" {
+
+const char* rgi_last_error(void) { return g_error; }
+int rgi_abi_version(void) { return RGI_ABI_VERSION; }
+
+uint32_t rgi_abi_sizeof(int which)
+{
+    switch (which) {
+    case 0: return (uint32_t)sizeof(rgi_frame);
+    case 1: return (uint32_t)sizeof(rgi_scene);
+    case 2: return (uint32_t)sizeof(rgi_gbuffer);
+    case 3: return (uint32_t)sizeof(rgi_gi_reservoirs);
+    case 4: return (uint32_t)sizeof(rgi_gi_inputs);
+    case 5: return (uint32_t)sizeof(rgi_di_reservoirs);
+    default: return 0;
+    }
+}
+
+int rgi_create(rgi_context** out)
+{
+    if (!out) return fail(RGI_E_INVALID, "out_context is NULL");
+    rgi_context* ctx = new (std::nothrow) rgi_context();
+    if (!ctx) return fail(RGI_E_NOMEM, "context allocation failed");
+    if (hipGetDevice(&ctx->device) != hipSuccess) {
+        delete ctx;
+        return fail(RGI_E_DEVICE, "no HIP device: %s", hipGetErrorString(hipGetLastError()));
+    }
+    *out = ctx;
+    return RGI_OK;
+}
+
+void rgi_destroy(rgi_context* ctx)
+{
+    if (!ctx) return;
+    if (ctx->scratch) {
+        hipDeviceSynchronize();
+        hipFree(ctx->scratch);
+    }
+    delete ctx;
+}
+
+int rgi_grid_clear(uint32_t* grid, uint32_t table_size, void* stream)
+{
+    if (!grid) return fail(RGI_E_INVALID, "grid is NULL");
+    if (!aligned(grid, 16)) return fail(RGI_E_INVALID, "grid must be 16-byte aligned");
+    if (table_size < 64u || (table_size & (table_size - 1u))) return fail(RGI_E_INVALID, "table_size must be a power of two >= 64");
+    return grid_clear(grid, table_size, as_stream(stream));
+}
+
+int rgi_grid_insert(rgi_context* ctx, const rgi_frame* frame, const rgi_gbuffer* g, uint32_t* grid, int mode,
+                    void* stream)
+{
+    int e;
+    if ((e = check_frame(frame)) != RGI_OK) return e;
+    if ((e = check_gbuffer(g, false, false)) != RGI_OK) return e;
+    if (!grid) return fail(RGI_E_INVALID, "grid is NULL");
+    if (mode == RGI_GRID_ATOMIC) return grid_insert_atomic(*frame, *g, grid, as_stream(stream));
+    if (mode == RGI_GRID_DETERMINISTIC) {
+        if (!ctx) return fail(RGI_E_INVALID, "the deterministic insert needs a context");
+        return grid_insert_deterministic(ctx, *frame, *g, grid, as_stream(stream));
+    }
+    return fail(RGI_E_INVALID, "unknown grid insert mode %d", mode);
+}
+
+int rgi_gi_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
+                    const rgi_gi_inputs* in, const rgi_gi_reservoirs* out, void* stream)
+{
+    (void)ctx;
+    int e;
+    if ((e = check_frame(frame)) != RGI_OK) return e;
+    if ((e = check_gbuffer(g, true, false)) != RGI_OK) return e;
+    if ((e = check_scene(scene, false)) != RGI_OK) return e;
+    if (!in || !out) return fail(RGI_E_INVALID, "inputs and outputs are required");
+    if (!in->history_sample || !in->history_receiver || !in->history_normal || !in->history_color || !in->surface_light ||
+        !in->grid)
+        return fail(RGI_E_INVALID, "gi temporal: history planes, surface light and previous grid are required");
+    if (frame->shadows_enabled && !in->shadow_map) return fail(RGI_E_INVALID, "shadows enabled but shadow_map is NULL");
+    if (!out->sample || !out->receiver || !out->normal) return fail(RGI_E_INVALID, "gi temporal: output planes are required");
+    if (frame->gi_width != g->width || frame->gi_height != g->height)
+        return fail(RGI_E_INVALID, "frame GI size %d x %d != gbuffer %d x %d", frame->gi_width, frame->gi_height, g->width, g->height);
+    return gi_temporal(*frame, *scene, *g, *in, *out, as_stream(stream));
+}
+
+int rgi_gi_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
+                   const rgi_gi_inputs* in, const rgi_gi_reservoirs* res, uint16_t* out_color, uint16_t* out_moments,
+                   void* stream)
+{
+    (void)ctx;
+    int e;
+    if ((e = check_frame(frame)) != RGI_OK) return e;
+    if ((e = check_gbuffer(g, true, false)) != RGI_OK) return e;
+    if ((e = check_scene(scene, false)) != RGI_OK) return e;
+    if (!in || !res || !out_color || !out_moments) return fail(RGI_E_INVALID, "gi spatial: inputs and outputs are required");
+    if (!in->history_receiver || !in->history_normal || !in->history_color || !in->history_moments || !in->surface_light ||
+        !in->grid)
+        return fail(RGI_E_INVALID, "gi spatial: history planes, surface light and grid are required");
+    if (frame->shadows_enabled && !in->shadow_map) return fail(RGI_E_INVALID, "shadows enabled but shadow_map is NULL");
+    if (!res->sample || !res->receiver || !res->normal) return fail(RGI_E_INVALID, "gi spatial: reservoir planes are required");
+    if (frame->gi_width != g->width || frame->gi_height != g->height)
+        return fail(RGI_E_INVALID, "frame GI size %d x %d != gbuffer %d x %d", frame->gi_width, frame->gi_height, g->width, g->height);
+    return gi_spatial(*frame, *scene, *g, *in, *res, out_color, out_moments, as_stream(stream));
+}
+
+int rgi_di_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
+                    const rgi_di_reservoirs* prev, const rgi_di_reservoirs* out, void* stream)
+{
+    (void)ctx;
+    int e;
+    if ((e = check_frame(frame)) != RGI_OK) return e;
+    if ((e = check_gbuffer(g, true, true)) != RGI_OK) return e;
+    if ((e = check_scene(scene, true)) != RGI_OK) return e;
+    if ((uint32_t)frame->di_light_count > scene->light_count)
+        return fail(RGI_E_INVALID, "di_light_count %d exceeds scene light_count %u", frame->di_light_count, scene->light_count);
+    if (!prev || !out || !prev->sample || !prev->receiver || !prev->normal || !out->sample || !out->receiver || !out->normal)
+        return fail(RGI_E_INVALID, "di temporal: reservoir planes are required");
+    return di_temporal(*frame, *scene, *g, *prev, *out, as_stream(stream));
+}
+
+int rgi_di_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
+                   const rgi_di_reservoirs* res, uint16_t* out_diffuse, uint16_t* out_specular, void* stream)
+{
+    (void)ctx;
+    int e;
+    if ((e = check_frame(frame)) != RGI_OK) return e;
+    if ((e = check_gbuffer(g, true, true)) != RGI_OK) return e;
+    if ((e = check_scene(scene, true)) != RGI_OK) return e;
+    if ((uint32_t)frame->di_light_count > scene->light_count)
+        return fail(RGI_E_INVALID, "di_light_count %d exceeds scene light_count %u", frame->di_light_count, scene->light_count);
+    if (!res || !res->sample || !res->receiver || !res->normal || !out_diffuse || !out_specular)
+        return fail(RGI_E_INVALID, "di spatial: reservoir and output planes are required");
+    return di_spatial(*frame, *scene, *g, *res, out_diffuse, out_specular, as_stream(stream));
+}
+
+int rgi_trace(rgi_context* ctx, const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle,
+              float* hit_distance, int any_hit, void* stream)
+{
+    (void)ctx;
+    int e;
+    if ((e = check_scene(scene, false)) != RGI_OK) return e;
+    if (count > 0 && (!rays || !hit_triangle || !hit_distance)) return fail(RGI_E_INVALID, "trace: buffers are required");
+    if (rays && !aligned(rays, 16)) return fail(RGI_E_INVALID, "rays must be 16-byte aligned");
+    return trace_rays(*scene, rays, count, hit_triangle, hit_distance, any_hit, as_stream(stream));
+}
+
+int rgi_gbuffer_primary(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const float* materials,
+                        uint32_t material_count, float* depth, uint32_t* albedo, uint32_t* normal,
+                        uint32_t* shading_normal, uint16_t* emission, void* stream)
+{
+    (void)ctx;
+    int e;
+    if (!frame) return fail(RGI_E_INVALID, "frame is NULL");
+    if ((e = check_scene(scene, false)) != RGI_OK) return e;
+    if (frame->width <= 0 || frame->height <= 0) return fail(RGI_E_INVALID, "frame width/height must be positive");
+    if (!depth || !albedo || !normal || !shading_normal || !emission) return fail(RGI_E_INVALID, "gbuffer outputs are required");
+    if (material_count > 0 && (!materials || !aligned(materials, 16))) return fail(RGI_E_INVALID, "materials missing or misaligned");
+    return gbuffer_primary(*frame, *scene, materials, material_count, depth, albedo, normal, shading_normal, emission,
+                           as_stream(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,415 @@
+// ReSTIR DI passes on gfx950 (di.sh, brdf.sh, cs_di_temporal.sc,
+// cs_di_spatial.sc) at full resolution: fresh light candidates by power with
+// the generalized balance heuristic against last frame's reservoir, then
+// pairwise-MIS spatial reuse and shading. Shadow rays are any-hit walks of
+// the BVH. One pixel per lane, 16 x 16 pixels per 256-thread block.
+#include "bvh_device.h"
+#include "internal.h"
+
+namespace rgi {
+namespace {
+
+struct DiPass {
+    const rgi_frame* f;
+    SceneView v;
+    rgi_gbuffer g;
+    const float4* lights;
+    F2 texel;
+    float sc;
+};
+
+struct Surf {
+    F3 position, normal, surface_normal, albedo;
+    float roughness, metallic;
+    F3 eye;
+    bool valid;
+};
+
+RGI_DEV Surf load_surface(const DiPass& p, F2 uv)
+{
+    Surf s;
+    uint32_t t = texel_index(uv, p.g.width, p.g.height);
+    float depth = p.g.depth[t];
+    F4 ar = unorm8(p.g.albedo[t]);
+    F4 nm = unorm8(p.g.normal[t]);
+    F4 sn = unorm8(p.g.shading_normal[t]);
+    s.valid = depth < kSkyDepth;
+    s.position = reconstruct_position(*p.f, uv, minf_(depth, kSkyDepth));
+    s.surface_normal = normalize(F3{nm.x * 2.0f - 1.0f, nm.y * 2.0f - 1.0f, nm.z * 2.0f - 1.0f});
+    s.normal = normalize(F3{sn.x * 2.0f - 1.0f, sn.y * 2.0f - 1.0f, sn.z * 2.0f - 1.0f});
+    s.albedo = F3{ar.x, ar.y, ar.z};
+    s.roughness = ar.w;
+    s.metallic = nm.w;
+    s.eye = F3{p.f->camera_position[0], p.f->camera_position[1], p.f->camera_position[2]};
+    return s;
+}
+
+RGI_DEV F4 light_row(const DiPass& p, int light, int row)
+{
+    float4 q = p.lights[light * 4 + row];
+    return F4{q.x, q.y, q.z, q.w};
+}
+
+RGI_DEV bool point_light(const DiPass& p, int light) { return light_row(p, light, 0).w > 0.5f; }
+
+// Power-proportional choice: binary search of the CDF (di.sh:123-142).
+RGI_DEV int pick_light(const DiPass& p, float random, float& pmf)
+{
+    int low = 0, high = p.f->di_light_count - 1;
+    for (int step = 0; step < 24 && low < high; ++step) {
+        int middle = (low + high) >> 1;
+        if (light_row(p, middle, 2).w < random) low = middle + 1;
+        else high = middle;
+    }
+    pmf = light_row(p, low, 3).w;
+    return low;
+}
+
+RGI_DEV float light_area(const DiPass& p, int light)
+{
+    F3 v0 = xyz(light_row(p, light, 0));
+    return 0.5f * length(cross(xyz(light_row(p, light, 1)) - v0, xyz(light_row(p, light, 2)) - v0));
+}
+
+RGI_DEV F3 light_point(const DiPass& p, int light, float r0, float r1, float& density)
+{
+    F3 v0 = xyz(light_row(p, light, 0));
+    if (point_light(p, light)) {
+        density = 1.0f;
+        return v0;
+    }
+    float root = sqrtf(r0);
+    float u = 1.0f - root;
+    float v = r1 * root;
+    density = 1.0f / maxf_(light_area(p, light), 1e-12f);
+    return (v0 + (xyz(light_row(p, light, 1)) - v0) * u) + (xyz(light_row(p, light, 2)) - v0) * v;
+}
+
+RGI_DEV float ggx_d(F3 n, F3 h, float roughness)
+{
+    float alpha = roughness * roughness;
+    float a2 = alpha * alpha;
+    float nh = maxf_(dot(n, h), 0.0f);
+    float den = (nh * nh) * (a2 - 1.0f) + 1.0f;
+    return a2 / maxf_((kPi * den) * den, 0.0001f);
+}
+
+RGI_DEV float smith_g1(float nd, float roughness)
+{
+    float k = ((roughness + 1.0f) * (roughness + 1.0f)) / 8.0f;
+    return nd / maxf_(nd * (1.0f - k) + k, 0.0001f);
+}
+
+RGI_DEV void brdf(F3 n, F3 v, F3 l, F3 albedo, float roughness_in, float metallic, F3& diffuse, F3& specular)
+{
+    float nl = maxf_(dot(n, l), 0.0f);
+    float nv = maxf_(dot(n, v), 0.0f);
+    F3 h = normalize(v + l);
+    F3 f0 = mix3(splat3(0.04f), albedo, metallic);
+    float c = clampf_(dot(h, v), 0.0f, 1.0f);
+    float x = 1.0f - c;
+    float w = ((x * x) * (x * x)) * x;
+    F3 fr = f0 + (splat3(1.0f) - f0) * w;
+    float roughness = maxf_(roughness_in, 0.05f);
+    float d = ggx_d(n, h, roughness);
+    float g = smith_g1(nl, roughness) * smith_g1(nv, roughness);
+    specular = ((fr * (d * g)) / maxf_((4.0f * nl) * nv, 0.001f)) * nl;
+    diffuse = (((splat3(1.0f) - fr) * (1.0f - metallic)) * albedo / kPi) * nl;
+}
+
+RGI_DEV void light_sample(const DiPass& p, const Surf& s, int light, F3 point, F3& diffuse, F3& specular)
+{
+    diffuse = splat3(0.0f);
+    specular = splat3(0.0f);
+    if (light < 0 || light >= p.f->di_light_count) return;
+    F3 to_light = point - s.position;
+    float d2 = maxf_(dot(to_light, to_light), (0.0001f * p.sc) * p.sc);
+    F3 dir = to_light * rsqrt_(d2);
+    if (dot(s.surface_normal, dir) <= 0.0f) return;
+    F3 radiance = xyz(light_row(p, light, 3)) / d2;
+    if (!point_light(p, light)) {
+        F3 v0 = xyz(light_row(p, light, 0));
+        F3 ln = normalize(cross(xyz(light_row(p, light, 1)) - v0, xyz(light_row(p, light, 2)) - v0));
+        radiance = radiance * fabsf(dot(ln, dir));
+    }
+    F3 d, sp;
+    brdf(s.normal, normalize(s.eye - s.position), dir, s.albedo, s.roughness, s.metallic, d, sp);
+    diffuse = d * radiance;
+    specular = sp * radiance;
+}
+
+RGI_DEV float light_target(const DiPass& p, const Surf& s, int light, F3 point)
+{
+    F3 d, sp;
+    light_sample(p, s, light, point, d, sp);
+    return luminance(d + sp);
+}
+
+RGI_DEV bool light_visible(const DiPass& p, const Surf& s, F3 point)
+{
+    float bias = 0.01f * p.sc;
+    F3 origin = s.position + s.surface_normal * bias;
+    F3 to_point = point - origin;
+    float pd = length(to_point);
+    return trace_any(p.v, origin, to_point / maxf_(pd, 0.0001f), pd * 0.999f - bias) < 0;
+}
+
+RGI_DEV bool same_surface(const DiPass& p, F3 position, F3 n, F3 other_position, F3 other_normal, float radius)
+{
+    F3 off = other_position - position;
+    return dot(other_normal, other_normal) > 0.5f && dot(off, off) <= radius * radius &&
+           fabsf(dot(off, n)) <= 0.03f * p.sc && dot(normalize(other_normal), n) >= 0.9f;
+}
+
+RGI_DEV void block_pixel(int& x, int& y)
+{
+    uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    x = (int)(blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u));
+    y = (int)(blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3));
+}
+
+RGI_DEV DiPass make_pass(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, F2 texel)
+{
+    DiPass p;
+    p.f = &f;
+    p.v = scene_view(s);
+    p.g = g;
+    p.lights = reinterpret_cast<const float4*>(s.lights);
+    p.texel = texel;
+    p.sc = f.scene_scale;
+    return p;
+}
+
+__global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs prev,
+                                                     rgi_di_reservoirs out, F2 texel)
+{
+    DiPass p = make_pass(f, s, g, texel);
+    int x, y;
+    block_pixel(x, y);
+    if (x >= g.width || y >= g.height) return;
+    uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+    Surf surface = load_surface(p, uv);
+    if (!surface.valid || f.di_light_count == 0) {
+        store_f4(out.sample, o, F4{0.0f, 0.0f, 0.0f, -1.0f});
+        store_f4(out.receiver, o, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        store_h4(out.normal, o, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        return;
+    }
+    uint32_t rng = pcg((uint32_t)x + pcg((uint32_t)y + pcg(f.frame_index ^ 0x3c6ef372u)));
+    int candidates = f.di_candidates;
+    float wsum = 0.0f;
+    F3 chosen_point = splat3(0.0f);
+    int chosen_light = -1;
+    float chosen_target = 0.0f;
+    for (int c = 0; c < candidates; ++c) {
+        float pmf;
+        float rl = next_random(rng);
+        int light = pick_light(p, rl, pmf);
+        float density;
+        float r0 = next_random(rng);
+        float r1 = next_random(rng);
+        F3 lp = light_point(p, light, r0, r1, density);
+        float target = light_target(p, surface, light, lp);
+        float source_pdf = pmf * density;
+        float w = source_pdf > 0.0f ? target / source_pdf : 0.0f;
+        wsum += w;
+        float rsel = next_random(rng);
+        if (rsel * wsum < w) {
+            chosen_point = lp;
+            chosen_light = light;
+            chosen_target = target;
+        }
+    }
+    float ccount = (float)candidates;
+    float cweight = chosen_target > 0.0f ? wsum / (ccount * chosen_target) : 0.0f;
+    F3 cpoint = chosen_point;
+    int clight = chosen_light;
+    float ctarget = chosen_target;
+
+    float tcount = 0.0f, tweight = 0.0f;
+    F3 tpoint = splat3(0.0f);
+    int tlight = -1;
+    Surf previous = surface;
+    F4 clip = mul(f.prev_view_proj, F4{surface.position.x, surface.position.y, surface.position.z, 1.0f});
+    if (f.di_history_valid && clip.w > 0.0001f) {
+        F2 puv = F2{(clip.x / clip.w) * 0.5f + 0.5f, ((f.origin_bottom_left ? clip.y : -clip.y) / clip.w) * 0.5f + 0.5f};
+        if (puv.x > 0.0f && puv.x < 1.0f && puv.y > 0.0f && puv.y < 1.0f) {
+            uint32_t t = texel_index(puv, g.width, g.height);
+            F4 sd = load_f4(prev.sample, t);
+            F4 rc = load_f4(prev.receiver, t);
+            F4 nm = load_h4(prev.normal, t);
+            int light = to_int(sd.w + 0.5f);
+            if (sd.w >= 0.0f && light < f.di_light_count &&
+                same_surface(p, surface.position, surface.normal, xyz(rc), xyz(nm), 0.1f * p.sc)) {
+                tcount = minf_(nm.w, 20.0f);
+                tweight = rc.w;
+                tpoint = xyz(sd);
+                tlight = light;
+                previous.position = xyz(rc);
+                previous.normal = normalize(xyz(nm));
+                previous.eye = F3{f.prev_camera_position[0], f.prev_camera_position[1], f.prev_camera_position[2]};
+            }
+        }
+    }
+    wsum = 0.0f;
+    chosen_light = -1;
+    chosen_target = 0.0f;
+    if (ctarget > 0.0f) {
+        float ptarget = 0.0f;
+        if (tcount > 0.0f) {
+            if (light_visible(p, previous, cpoint)) ptarget = light_target(p, previous, clight, cpoint);
+        }
+        float den = ccount * ctarget + tcount * ptarget;
+        float w = (((ccount * ctarget) / den) * ctarget) * cweight;
+        wsum += w;
+        chosen_point = cpoint;
+        chosen_light = clight;
+        chosen_target = ctarget;
+    }
+    if (tcount > 0.0f && tweight > 0.0f) {
+        float target = light_target(p, surface, tlight, tpoint);
+        float ptarget = light_target(p, previous, tlight, tpoint);
+        float den = tcount * ptarget + ccount * target;
+        float w = den > 0.0f ? (((tcount * ptarget) / den) * target) * tweight : 0.0f;
+        wsum += w;
+        if (w > 0.0f) {
+            float rsel = next_random(rng);
+            if (rsel * wsum < w) {
+                chosen_point = tpoint;
+                chosen_light = tlight;
+                chosen_target = target;
+            }
+        }
+    }
+    float cw = chosen_target > 0.0f ? wsum / chosen_target : 0.0f;
+    if (cw > 0.0f) {
+        if (!light_visible(p, surface, chosen_point)) cw = 0.0f;
+    }
+    store_f4(out.sample, o, F4{chosen_point.x, chosen_point.y, chosen_point.z, (float)chosen_light});
+    store_f4(out.receiver, o, F4{surface.position.x, surface.position.y, surface.position.z, cw});
+    store_h4(out.normal, o, F4{surface.normal.x, surface.normal.y, surface.normal.z, ccount + tcount});
+}
+
+__global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs res,
+                                                    uint16_t* out_diffuse, uint16_t* out_specular, F2 texel)
+{
+    DiPass p = make_pass(f, s, g, texel);
+    int x, y;
+    block_pixel(x, y);
+    if (x >= g.width || y >= g.height) return;
+    uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+    Surf surface = load_surface(p, uv);
+    uint32_t t0 = texel_index(uv, g.width, g.height);
+    F4 own_sample = load_f4(res.sample, t0);
+    F4 own_receiver = load_f4(res.receiver, t0);
+    F4 own_normal = load_h4(res.normal, t0);
+    if (!surface.valid || f.di_light_count == 0) {
+        store_h4(out_diffuse, o, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        store_h4(out_specular, o, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        return;
+    }
+    uint32_t rng = pcg((uint32_t)x + pcg((uint32_t)y + pcg(f.frame_index ^ 0x9e3779b9u)));
+    int taps = f.di_spatial_taps;
+    float ccount = maxf_(minf_(own_normal.w, 20.0f), 1.0f);
+    int clight = own_sample.w >= 0.0f ? to_int(own_sample.w + 0.5f) : -1;
+    F3 cpoint = xyz(own_sample);
+    float ctarget = clight >= 0 && own_receiver.w > 0.0f ? light_target(p, surface, clight, cpoint) : 0.0f;
+    float pair_c = ccount / (float)(taps > 1 ? taps : 1);
+    float total = ccount;
+    float share = taps > 0 ? 0.0f : ccount;
+    float wsum = 0.0f;
+    F3 chosen_point = cpoint;
+    int chosen_light = -1;
+    float chosen_target = 0.0f;
+    bool neighbour_selected = false;
+    for (int tap = 0; tap < taps; ++tap) {
+        float ra = next_random(rng);
+        float radius = 16.0f * sqrtf(ra);
+        float rb = next_random(rng);
+        float angle = (2.0f * kPi) * rb;
+        float rsel = next_random(rng);
+        float sa, ca;
+        rgi_sincos(angle, &sa, &ca);
+        F2 nuv = F2{uv.x + (ca * radius) * texel.x, uv.y + (sa * radius) * texel.y};
+        Surf nb = load_surface(p, nuv);
+        uint32_t tn = texel_index(nuv, g.width, g.height);
+        F4 sd = load_f4(res.sample, tn);
+        F4 rc = load_f4(res.receiver, tn);
+        F4 nm = load_h4(res.normal, tn);
+        int light = sd.w >= 0.0f ? to_int(sd.w + 0.5f) : -1;
+        if (nuv.x <= 0.0f || nuv.x >= 1.0f || nuv.y <= 0.0f || nuv.y >= 1.0f || !nb.valid ||
+            !same_surface(p, surface.position, surface.normal, nb.position, nb.normal, 4.0f * (0.1f * p.sc))) {
+            share += pair_c;
+            continue;
+        }
+        float ncount = minf_(nm.w, 20.0f);
+        float pair_weight = ncount + pair_c;
+        total += ncount;
+        if (ctarget > 0.0f) {
+            float ntarget = 0.0f;
+            if (light_visible(p, nb, cpoint)) ntarget = light_target(p, nb, clight, cpoint);
+            float cd = pair_c * ctarget;
+            float den = ncount * ntarget + cd;
+            share += (pair_weight * cd) / den;
+        }
+        if (light >= 0 && light < f.di_light_count && rc.w > 0.0f) {
+            float target = light_target(p, surface, light, xyz(sd));
+            float ntarget = light_target(p, nb, light, xyz(sd));
+            float nd = ncount * ntarget;
+            float den = nd + pair_c * target;
+            float w = den > 0.0f ? (((pair_weight * nd) / den) * target) * rc.w : 0.0f;
+            wsum += w;
+            if (w > 0.0f && rsel * wsum < w) {
+                chosen_point = xyz(sd);
+                chosen_light = light;
+                chosen_target = target;
+                neighbour_selected = true;
+            }
+        }
+    }
+    if (ctarget > 0.0f) {
+        float w = (share * ctarget) * own_receiver.w;
+        wsum += w;
+        float rsel = next_random(rng);
+        if (rsel * wsum < w) {
+            chosen_point = cpoint;
+            chosen_light = clight;
+            chosen_target = ctarget;
+            neighbour_selected = false;
+        }
+    }
+    float cw = chosen_target > 0.0f ? wsum / (total * chosen_target) : 0.0f;
+    if (cw > 0.0f && neighbour_selected) {
+        if (!light_visible(p, surface, chosen_point)) cw = 0.0f;
+    }
+    F3 d, sp;
+    light_sample(p, surface, chosen_light, chosen_point, d, sp);
+    d = d * cw;
+    sp = sp * cw;
+    store_h4(out_diffuse, o, F4{d.x, d.y, d.z, 1.0f});
+    store_h4(out_specular, o, F4{sp.x, sp.y, sp.z, 1.0f});
+}
+
+} // namespace
+
+int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
+                const rgi_di_reservoirs& out, hipStream_t stream)
+{
+    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
+    F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    hipLaunchKernelGGL(k_di_temporal, blocks, dim3(256), 0, stream, f, s, g, prev, out, texel);
+    return check_launch("di temporal");
+}
+
+int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
+               uint16_t* diffuse, uint16_t* specular, hipStream_t stream)
+{
+    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
+    F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    hipLaunchKernelGGL(k_di_spatial, blocks, dim3(256), 0, stream, f, s, g, res, diffuse, specular, texel);
+    return check_launch("di spatial");
+}
+
+} // namespace rgi

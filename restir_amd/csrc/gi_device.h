@@ -1,0 +1,447 @@
+// ReSTIR GI per-pixel logic on gfx950 (restir.sh, shadow.sh, sky.sh): sample
+// shading, the depth-buffer march, visibility, the reconnection Jacobian and
+// pairwise MIS, evaluated per lane in the same order as the reference's
+// shader so results are bit-exact.
+#pragma once
+
+#include "bvh_device.h"
+
+namespace rgi {
+
+// Everything a GI pass reads besides its per-pixel state.
+struct GiPass {
+    const rgi_frame* f;      // kernel-argument frame
+    SceneView v;
+    rgi_gbuffer g;
+    rgi_gi_inputs in;
+    F2 texel;                // 1 / GI size
+    F2 size;                 // GI size
+    F3 grid_camera;          // u_gridCamera of this pass
+    float sc;                // scene scale
+};
+
+RGI_DEV uint32_t tix(const GiPass& p, F2 uv) { return texel_index(uv, p.g.width, p.g.height); }
+RGI_DEV float fetch_depth(const GiPass& p, F2 uv) { return p.g.depth[tix(p, uv)]; }
+
+RGI_DEV F3 decode_normal(const GiPass& p, F2 uv)
+{
+    F4 n = unorm8(p.g.normal[tix(p, uv)]);
+    return normalize(F3{n.x * 2.0f - 1.0f, n.y * 2.0f - 1.0f, n.z * 2.0f - 1.0f});
+}
+
+RGI_DEV F3 diffuse_albedo(const GiPass& p, F2 uv)
+{
+    uint32_t t = tix(p, uv);
+    F4 a = unorm8(p.g.albedo[t]);
+    float metallic = unorm8(p.g.normal[t]).w;
+    return F3{a.x, a.y, a.z} * (1.0f - metallic);
+}
+
+// ------------------------------------------------------------ shadow, sky
+
+RGI_DEV int shadow_face(F3 d)
+{
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    if (ax >= ay && ax >= az) return d.x >= 0.0f ? 0 : 1;
+    if (ay >= az) return d.y >= 0.0f ? 2 : 3;
+    return d.z >= 0.0f ? 4 : 5;
+}
+
+RGI_DEV float light_visibility_hard(const GiPass& p, F3 position, F3 n)
+{
+    const rgi_frame& f = *p.f;
+    if (!f.shadows_enabled || p.in.shadow_map == nullptr) return 1.0f;
+    F3 light = F3{f.light_position[0], f.light_position[1], f.light_position[2]};
+    float light_distance = length(light - position);
+    float texel_size = ((2.0f * light_distance) * f.shadow_info[1]) * f.shadow_info[2];
+    F3 pos = position + n * (texel_size * 1.5f);
+    F3 from_light = pos - light;
+    int face = shadow_face(from_light);
+    F4 clip = mul(f.shadow_view_proj + face * 16, F4{pos.x, pos.y, pos.z, 1.0f});
+    float ndx = clip.x / clip.w, ndy = clip.y / clip.w;
+    bool obl = f.shadow_info[3] > 0.5f;
+    float lx = ndx * 0.5f + 0.5f;
+    float ly = obl ? ndy * 0.5f + 0.5f : 0.5f - ndy * 0.5f;
+    float row = face >= 3 ? 1.0f : 0.0f;
+    float column = (float)face - 3.0f * row;
+    if (obl) row = 1.0f - row;
+    F2 uv = F2{(column + lx) * (1.0f / 3.0f), (row + ly) * 0.5f};
+    float cosine = clampf_(dot(n, -normalize(from_light)), 0.0f, 1.0f);
+    float slope = minf_(sqrtf(1.0f - cosine * cosine) / maxf_(cosine, 0.0001f), 8.0f);
+    float bias = texel_size * (1.0f + 0.5f * slope);
+    float reference = (length(from_light) - bias) / f.shadow_info[0];
+    float stored = p.in.shadow_map[texel_index(uv, f.shadow_width, f.shadow_height)];
+    return reference <= stored ? 1.0f : 0.0f;
+}
+
+RGI_DEV bool sky_on(const rgi_frame& f) { return f.sky_zenith[3] > 0.5f; }
+
+RGI_DEV F3 sky_radiance(const rgi_frame& f, F3 d)
+{
+    if (!sky_on(f)) return splat3(0.0f);
+    F3 horizon = F3{f.sky_horizon[0], f.sky_horizon[1], f.sky_horizon[2]};
+    float h = clampf_(d.y, -1.0f, 1.0f);
+    if (h >= 0.0f) return mix3(horizon, F3{f.sky_zenith[0], f.sky_zenith[1], f.sky_zenith[2]}, sqrtf(h));
+    return mix3(horizon, F3{f.sky_ground[0], f.sky_ground[1], f.sky_ground[2]}, sqrtf(-h));
+}
+
+// ------------------------------------------------------------ shading
+
+RGI_DEV F3 cosine_lobe(F3 n, float r0, float r1)
+{
+    float radius = sqrtf(r0);
+    float angle = (2.0f * kPi) * r1;
+    float sa, ca;
+    rgi_sincos(angle, &sa, &ca);
+    float lx = radius * ca, ly = radius * sa, lz = sqrtf(maxf_(1.0f - r0, 0.0f));
+    F3 t, b;
+    tangent_frame(n, t, b);
+    return normalize((t * lx + b * ly) + n * lz);
+}
+
+RGI_DEV F3 point_light_direct(const GiPass& p, F3 albedo, F3 position, F3 n)
+{
+    const rgi_frame& f = *p.f;
+    F3 to_light = F3{f.light_position[0], f.light_position[1], f.light_position[2]} - position;
+    float d2 = maxf_(dot(to_light, to_light), (0.01f * p.sc) * p.sc);
+    float nl = maxf_(dot(n, to_light * rsqrt_(d2)), 0.0f);
+    F3 direct = splat3(0.0f);
+    if (nl > 0.0f) {
+        F3 color = F3{f.light_color[0], f.light_color[1], f.light_color[2]};
+        direct = ((albedo * (color / d2)) * (nl / kPi)) * light_visibility_hard(p, position, n);
+    }
+    return direct;
+}
+
+RGI_DEV bool is_sky(float id) { return id > kSkySampleId * 0.5f; }
+
+RGI_DEV float sample_id_of(const SceneView& v, int tri, F3 dir)
+{
+    F3 v0 = vertex(v, tri, 0);
+    F3 pn = cross(vertex(v, tri, 1) - v0, vertex(v, tri, 2) - v0);
+    return dot(pn, dir) <= 0.0f ? (float)(tri + 1) : -(float)(tri + 1);
+}
+
+RGI_DEV F4 fetch_f4(const float* plane, const GiPass& p, F2 uv) { return load_f4(plane, tix(p, uv)); }
+RGI_DEV F4 fetch_h4(const uint16_t* plane, const GiPass& p, F2 uv) { return load_h4(plane, tix(p, uv)); }
+
+// Whether history texel uv held the same surface as (position, n).
+RGI_DEV bool same_history_surface(const GiPass& p, F2 uv, F3 position, F3 n)
+{
+    F3 rp = xyz(fetch_f4(p.in.history_receiver, p, uv));
+    F3 rn = xyz(fetch_f4(p.in.history_normal, p, uv));
+    F3 off = rp - position;
+    float bounce = 0.1f * p.sc;
+    return !(dot(rn, rn) < 0.5f || dot(off, off) > bounce * bounce || fabsf(dot(off, n)) > 0.03f * p.sc ||
+             dot(normalize(rn), n) < 0.9f);
+}
+
+RGI_DEV F3 previous_indirect(const GiPass& p, F3 position, F3 n, F3 albedo)
+{
+    const rgi_frame& f = *p.f;
+    if (!f.history_valid || !f.multiple_bounces) return splat3(0.0f);
+    F4 clip = mul(f.prev_view_proj, F4{position.x, position.y, position.z, 1.0f});
+    if (clip.w <= 0.0001f) return splat3(0.0f);
+    F2 uv = clip_to_uv(f, clip);
+    if (!on_screen(uv)) return splat3(0.0f);
+    if (!same_history_surface(p, uv, position, n)) return splat3(0.0f);
+    return xyz(fetch_h4(p.in.history_color, p, uv)) * albedo;
+}
+
+RGI_DEV float depth_behind(const GiPass& p, F3 ray_position, F3& surface, bool& visible)
+{
+    float ray_depth;
+    F2 uv;
+    surface = splat3(0.0f);
+    visible = project_to_screen(*p.f, ray_position, uv, ray_depth);
+    if (!visible) return -1.0f;
+    uint32_t t = tix(p, uv);
+    float depth = p.g.depth[t];
+    if (depth >= kSkyDepth) return -1.0f;
+    surface = reconstruct_position(*p.f, uv, depth);
+    float behind = ray_depth - mul_w(p.f->view_proj, surface);
+    if (behind > 0.0f) {
+        F4 nr = unorm8(p.g.normal[t]);
+        F3 sn = normalize(F3{nr.x * 2.0f - 1.0f, nr.y * 2.0f - 1.0f, nr.z * 2.0f - 1.0f});
+        if (dot(ray_position - surface, sn) > -(0.01f * p.sc)) return -1.0f;
+    }
+    return behind;
+}
+
+// The depth-buffer march of traceScreenSpaceRay: quadratic steps, then five
+// bisection steps on the first crossing.
+RGI_DEV bool march(const GiPass& p, F3 origin, F3 dir, float max_distance, F3& hit_position)
+{
+    hit_position = splat3(0.0f);
+    float previous = 0.0f;
+    bool previous_front = true;
+    int steps = p.f->trace_steps;
+    float step_scale = 32.0f / (float)steps;
+    for (int i = 1; i <= steps; ++i) {
+        if (previous >= max_distance) break;
+        float sp = (float)i * step_scale;
+        float distance = minf_((0.02f + (sp * sp) * 0.0035f) * p.sc, max_distance);
+        F3 surface;
+        bool visible;
+        float behind = depth_behind(p, origin + dir * distance, surface, visible);
+        if (!visible) return false;
+        bool front = behind <= 0.0f;
+        if (!front && previous_front) {
+            float near_d = previous, far_d = distance;
+            for (int r = 0; r < 5; ++r) {
+                float middle = 0.5f * (near_d + far_d);
+                F3 ms;
+                bool mv;
+                float mb = depth_behind(p, origin + dir * middle, ms, mv);
+                if (mv && mb > 0.0f) {
+                    far_d = middle;
+                    behind = mb;
+                    surface = ms;
+                } else {
+                    near_d = middle;
+                }
+            }
+            if (behind < 0.03f * p.sc + 0.03f * far_d) {
+                hit_position = surface;
+                return true;
+            }
+        }
+        previous_front = front;
+        previous = distance;
+    }
+    return false;
+}
+
+// evaluateSample: the integrand of a sample point at a receiver.
+RGI_DEV bool shade_sample(const GiPass& p, F3 rpos, F3 rn, F3 ralbedo, F3 spos, float sid, F3& contribution, F3& snormal)
+{
+    const rgi_frame& f = *p.f;
+    contribution = splat3(0.0f);
+    snormal = F3{0.0f, 0.0f, 1.0f};
+    if (is_sky(sid)) {
+        snormal = -spos;
+        contribution = (ralbedo * sky_radiance(f, spos)) * (maxf_(dot(rn, spos), 0.0f) / kPi);
+        return sky_on(f);
+    }
+    bool visible_surface = false;
+    F2 suv;
+    float sview;
+    if (project_to_screen(f, spos, suv, sview)) {
+        float sdepth = fetch_depth(p, suv);
+        visible_surface = sdepth < kSkyDepth && length(reconstruct_position(f, suv, sdepth) - spos) <= 0.1f * p.sc;
+    }
+    int tri = to_int(fabsf(sid) + 0.5f) - 1;
+    if (!visible_surface && (tri < 0 || !bvh_ready(p.v))) return false;
+    if (visible_surface) {
+        snormal = decode_normal(p, suv);
+    } else {
+        F3 v0 = vertex(p.v, tri, 0);
+        snormal = normalize(cross(vertex(p.v, tri, 1) - v0, vertex(p.v, tri, 2) - v0)) * (sid < 0.0f ? -1.0f : 1.0f);
+    }
+    F3 to_receiver = rpos - spos;
+    float d2 = dot(to_receiver, to_receiver);
+    if (d2 < ((0.0001f * p.sc) * p.sc) * 0.01f) return true;
+    F3 dir = to_receiver * rsqrt_(d2);
+    float scos = dot(snormal, dir);
+    float rcos = -dot(rn, dir);
+    if (scos <= 0.0f || rcos <= 0.0f) return true;
+    F3 radiance;
+    if (visible_surface) {
+        F3 albedo = diffuse_albedo(p, suv);
+        radiance = (point_light_direct(p, albedo, spos, snormal) + previous_indirect(p, spos, snormal, albedo)) +
+                   xyz(fetch_h4(p.in.surface_light, p, suv)) * f.lighting_surface;
+    } else {
+        F3 albedo = tri_albedo(p.v, tri);
+        radiance = (point_light_direct(p, albedo, spos, snormal) + previous_indirect(p, spos, snormal, albedo)) +
+                   tri_emission(p.v, tri) * f.lighting_emission;
+    }
+    contribution = (ralbedo * radiance) * (rcos / kPi);
+    return true;
+}
+
+RGI_DEV float geometry_term(const GiPass& p, F3 rpos, F3 rn, F3 spos, F3 sn, float sid)
+{
+    if (is_sky(sid)) return maxf_(dot(rn, spos), 0.0f);
+    F3 off = spos - rpos;
+    float d2 = dot(off, off);
+    if (d2 < ((0.0001f * p.sc) * p.sc) * 0.01f) return 0.0f;
+    F3 dir = off * rsqrt_(d2);
+    float rc = dot(rn, dir);
+    float scos = -dot(sn, dir);
+    return rc > 0.0f && scos > 0.0f ? (rc * scos) / d2 : 0.0f;
+}
+
+RGI_DEV bool visible_between(const GiPass& p, F3 rpos, F3 rn, F3 spos, float sid)
+{
+    float bias = 0.02f * p.sc;
+    float max_bvh = 4.0f * p.sc;
+    F3 origin = rpos + rn * bias;
+    bool sky = is_sky(sid);
+    F3 to_sample = sky ? spos * max_bvh : spos - origin;
+    float sd = length(to_sample);
+    F3 dir = to_sample / maxf_(sd, 0.0001f);
+    bool visible = true;
+    if (sky) {
+        visible = false;
+        if (bvh_ready(p.v)) visible = trace_any(p.v, origin, dir, max_bvh) < 0;
+    } else if (p.f->ray_mode == RGI_RAY_BVH && bvh_ready(p.v)) {
+        visible = trace_any(p.v, origin, dir, sd - bias) < 0;
+    } else {
+        F3 occ;
+        visible = !march(p, origin, dir, sd * 0.95f - 0.05f * p.sc, occ);
+    }
+    return visible;
+}
+
+RGI_DEV float jacobian(const GiPass& p, F3 here, F3 there, F3 spos, F3 sn, float sid)
+{
+    if (is_sky(sid)) return 1.0f;
+    F3 th = here - spos;
+    F3 tt = there - spos;
+    float hs = dot(th, th), ts = dot(tt, tt);
+    float lim = ((0.0001f * p.sc) * p.sc) * 0.01f;
+    if (hs < lim || ts < lim) return 0.0f;
+    float ch = dot(sn, th) * rsqrt_(hs);
+    float ct = dot(sn, tt) * rsqrt_(ts);
+    if (ct <= 0.001f) return 0.0f;
+    float j = maxf_((ch / ct) * (ts / hs), 0.0f);
+    float clamp_value = p.f->jacobian_clamp;
+    return clamp_value > 0.0f ? minf_(j, clamp_value) : j;
+}
+
+struct Res {
+    F3 contribution, position, normal;
+    float id, target, wsum, count;
+};
+
+RGI_DEV Res empty_res()
+{
+    Res r;
+    r.contribution = splat3(0.0f);
+    r.position = splat3(0.0f);
+    r.normal = F3{0.0f, 0.0f, 1.0f};
+    r.id = 0.0f;
+    r.target = 0.0f;
+    r.wsum = 0.0f;
+    r.count = 0.0f;
+    return r;
+}
+
+RGI_DEV bool stream_in(Res& r, F3 contribution, F3 position, F3 n, float id, float target, float weight, float random)
+{
+    r.wsum += weight;
+    if (random * r.wsum < weight) {
+        r.contribution = contribution;
+        r.position = position;
+        r.normal = n;
+        r.id = id;
+        r.target = target;
+        return true;
+    }
+    return false;
+}
+
+struct Nb {
+    F3 sample_position;
+    float sample_id;
+    F3 receiver_position, receiver_normal;
+    float count, weight;
+};
+
+RGI_DEV bool accept(const GiPass& p, F4 sample, F4 receiver, F4 rnormal, F3 position, F3 n, float radius, Nb& nb)
+{
+    nb.sample_position = xyz(sample);
+    nb.sample_id = rnormal.w;
+    nb.receiver_position = xyz(receiver);
+    nb.receiver_normal = F3{0.0f, 0.0f, 1.0f};
+    nb.count = minf_(sample.w, 20.0f);
+    nb.weight = receiver.w;
+    F3 rn = xyz(rnormal);
+    if (sample.w <= 0.0f || dot(rn, rn) < 0.5f) return false;
+    nb.receiver_normal = normalize(rn);
+    F3 off = xyz(receiver) - position;
+    return dot(off, off) <= radius * radius && fabsf(dot(off, n)) <= 0.03f * p.sc && dot(nb.receiver_normal, n) >= 0.9f;
+}
+
+RGI_DEV float reuse_radius(const GiPass& p, F3 position)
+{
+    float r = 0.4f * p.sc;
+    return p.f->world_space_reuse ? maxf_(r, 2.0f * grid_cell_size(*p.f, p.grid_camera, position)) : r;
+}
+
+struct Pairwise {
+    Res combined;
+    F3 c_position, c_normal;
+    float c_id, pair_c, c_geometry, c_share, total;
+    bool has_c, neighbour_selected;
+};
+
+RGI_DEV Pairwise begin_pairwise(const GiPass& p, F3 position, F3 n, const Res& c, float ccount, int taps)
+{
+    Pairwise m;
+    m.combined = empty_res();
+    m.c_position = c.position;
+    m.c_normal = c.normal;
+    m.c_id = c.id;
+    m.pair_c = ccount / (float)(taps > 1 ? taps : 1);
+    m.total = ccount;
+    m.has_c = c.target > 0.000001f;
+    m.c_geometry = m.has_c ? geometry_term(p, position, n, c.position, c.normal, c.id) : 0.0f;
+    m.c_share = taps > 0 ? 0.0f : ccount;
+    m.neighbour_selected = false;
+    return m;
+}
+
+RGI_DEV void add_pair(const GiPass& p, Pairwise& m, F3 position, F3 n, F3 albedo, const Nb& nb, float random)
+{
+    float ncount = nb.count;
+    float pair_weight = ncount + m.pair_c;
+    m.total += ncount;
+    if (m.has_c) {
+        float ng = geometry_term(p, nb.receiver_position, nb.receiver_normal, m.c_position, m.c_normal, m.c_id);
+        if (ng > 0.0f) {
+            if (!visible_between(p, nb.receiver_position, nb.receiver_normal, m.c_position, m.c_id)) ng = 0.0f;
+        }
+        float cd = m.pair_c * m.c_geometry;
+        float den = ncount * ng + cd;
+        m.c_share += den > 0.0f ? (pair_weight * cd) / den : pair_weight;
+    }
+    F3 contribution = splat3(0.0f);
+    F3 sn = F3{0.0f, 0.0f, 1.0f};
+    bool shifted = false;
+    if (nb.weight > 0.0f) shifted = shade_sample(p, position, n, albedo, nb.sample_position, nb.sample_id, contribution, sn);
+    if (shifted) {
+        float target = luminance(contribution);
+        float j = jacobian(p, position, nb.receiver_position, nb.sample_position, sn, nb.sample_id);
+        float hg = geometry_term(p, position, n, nb.sample_position, sn, nb.sample_id);
+        float tg = geometry_term(p, nb.receiver_position, nb.receiver_normal, nb.sample_position, sn, nb.sample_id);
+        float nd = ncount * tg;
+        float den = nd + m.pair_c * hg;
+        float mis = den > 0.0f ? (pair_weight * nd) / den : 0.0f;
+        float w = ((mis * target) * nb.weight) * j;
+        if (w > 0.0f && stream_in(m.combined, contribution, nb.sample_position, sn, nb.sample_id, target, w, random))
+            m.neighbour_selected = true;
+    }
+}
+
+RGI_DEV float finish_pairwise(const GiPass& p, Pairwise& m, F3 position, F3 n, const Res& c, float cweight, float random)
+{
+    if (m.has_c) {
+        float w = (m.c_share * c.target) * cweight;
+        if (stream_in(m.combined, c.contribution, c.position, c.normal, c.id, c.target, w, random))
+            m.neighbour_selected = false;
+    }
+    float cw = m.combined.target > 0.000001f ? m.combined.wsum / (m.total * m.combined.target) : 0.0f;
+    if (cw > 0.0f && m.neighbour_selected) {
+        if (!visible_between(p, position, n, m.combined.position, m.combined.id)) cw = 0.0f;
+    }
+    return cw;
+}
+
+RGI_DEV F2 pixel_uv(const GiPass& p, uint32_t index)
+{
+    uint32_t w = (uint32_t)p.g.width;
+    return F2{((float)(index % w) + 0.5f) * p.texel.x, ((float)(index / w) + 0.5f) * p.texel.y};
+}
+
+} // namespace rgi

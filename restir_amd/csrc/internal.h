@@ -1,0 +1,46 @@
+// Library internals shared by the kernel files: the context (scratch
+// buffers), error reporting and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/restir_amd.h"
+
+struct rgi_context {
+    int device = 0;
+    void* scratch = nullptr;       // grid insert (deterministic) and other per-call scratch
+    size_t scratch_bytes = 0;
+};
+
+namespace rgi {
+
+int fail(int code, const char* fmt, ...);
+int check_launch(const char* what);
+// Scratch of at least `bytes`, grown (after a device sync) when too small.
+void* scratch(rgi_context* ctx, size_t bytes);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// Kernel-file entry points, called by the C ABI after argument checks.
+int grid_clear(uint32_t* grid, uint32_t table_size, hipStream_t stream);
+int grid_insert_atomic(const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid, hipStream_t stream);
+int grid_insert_deterministic(rgi_context* ctx, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
+                              hipStream_t stream);
+int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* hit, float* dist, int any_hit,
+               hipStream_t stream);
+int gbuffer_primary(const rgi_frame& f, const rgi_scene& s, const float* materials, uint32_t material_count,
+                    float* depth, uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal, uint16_t* emission,
+                    hipStream_t stream);
+int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+                const rgi_gi_reservoirs& out, hipStream_t stream);
+int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+               const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, hipStream_t stream);
+int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
+                const rgi_di_reservoirs& out, hipStream_t stream);
+int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
+               uint16_t* diffuse, uint16_t* specular, hipStream_t stream);
+
+} // namespace rgi

@@ -1,0 +1,149 @@
+// Ray batches through the BVH (rgi_trace) and the primary-ray G-buffer
+// producer (rgi_gbuffer_primary).
+//
+// rgi_trace keeps every lane busy with a persistent loop: each wave takes the
+// next 64 rays from a global counter when all its lanes are done, so waves
+// that drew short rays move on instead of idling behind long ones. Closest-hit
+// rays keep their 8-entry ring stack in LDS, laid out slot-major so that the
+// 64 lanes of a wave hit 64 different banks.
+#include "bvh_device.h"
+#include "internal.h"
+
+namespace rgi {
+namespace {
+
+constexpr int kTraceBlock = 256;
+
+__global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4* rays, uint32_t count, int32_t* hit,
+                                                       float* dist, int any_hit)
+{
+    __shared__ int ring[kStackRing * kTraceBlock];
+    SceneView v = scene_view(s);
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float4 a = rays[2u * i], b = rays[2u * i + 1u];
+    F3 o = F3{a.x, a.y, a.z}, d = F3{b.x, b.y, b.z};
+    float t = a.w;
+    int h;
+    if (any_hit) {
+        h = trace_any(v, o, d, a.w);
+    } else {
+        h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock);
+    }
+    hit[i] = h;
+    dist[i] = t;
+}
+
+// Persistent form: waves pull 64-ray batches until the counter passes count.
+__global__ void __launch_bounds__(kTraceBlock) k_trace_persistent(rgi_scene s, const float4* rays, uint32_t count,
+                                                                  int32_t* hit, float* dist, int any_hit,
+                                                                  uint32_t* next_batch)
+{
+    __shared__ int ring[kStackRing * kTraceBlock];
+    SceneView v = scene_view(s);
+    uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        uint32_t batch = 0;
+        if (lane == 0) batch = atomicAdd(next_batch, 64u);
+        batch = __builtin_amdgcn_readfirstlane(batch);
+        if (batch >= count) break;
+        uint32_t i = batch + lane;
+        if (i < count) {
+            float4 a = rays[2u * i], b = rays[2u * i + 1u];
+            F3 o = F3{a.x, a.y, a.z}, d = F3{b.x, b.y, b.z};
+            float t = a.w;
+            int h;
+            if (any_hit) h = trace_any(v, o, d, a.w);
+            else h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock);
+            hit[i] = h;
+            dist[i] = t;
+        }
+    }
+}
+
+RGI_DEV uint32_t unorm_byte(float v)
+{
+    float q = floorf(clampf_(v, 0.0f, 1.0f) * 255.0f + 0.5f);
+    return (uint32_t)to_int(q);
+}
+
+__global__ void __launch_bounds__(256) k_gbuffer(rgi_frame f, rgi_scene s, const float4* materials,
+                                                 uint32_t material_count, F2 texel, float* depth, uint32_t* albedo,
+                                                 uint32_t* normal, uint32_t* shading_normal, uint16_t* emission)
+{
+    __shared__ int ring[kStackRing * 256];
+    SceneView v = scene_view(s);
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = (uint32_t)f.width * (uint32_t)f.height;
+    if (i >= n) return;
+    int x = (int)(i % (uint32_t)f.width), y = (int)(i / (uint32_t)f.width);
+    F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+    F3 clip = F3{uv.x * 2.0f - 1.0f, to_clip_y(f, uv.y * 2.0f - 1.0f), 1.0f};
+    F3 far_point = clip_to_world(f.inv_view_proj, clip);
+    F3 eye = F3{f.camera_position[0], f.camera_position[1], f.camera_position[2]};
+    F3 dir = normalize(far_point - eye);
+    float dist;
+    int tri = trace_closest(v, eye, dir, 1e30f, dist, ring + threadIdx.x, 256);
+    float d = 1.0f;
+    if (tri >= 0) {
+        F3 p = eye + dir * dist;
+        F4 c = mul(f.view_proj, F4{p.x, p.y, p.z, 1.0f});
+        float z = c.z / c.w;
+        if (f.homogeneous_depth) z = z * 0.5f + 0.5f;
+        d = z;
+        if (!(z >= 0.0f && z < kSkyDepth)) tri = -1;
+    }
+    if (tri < 0) {
+        depth[i] = 1.0f;
+        albedo[i] = 0u;
+        normal[i] = 0u;
+        shading_normal[i] = 0u;
+        store_h4(emission, i, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        return;
+    }
+    depth[i] = d;
+    F3 v0 = vertex(v, tri, 0);
+    F3 nn = normalize(cross(vertex(v, tri, 1) - v0, vertex(v, tri, 2) - v0));
+    if (dot(nn, dir) > 0.0f) nn = -nn;
+    uint32_t material = (uint32_t)to_int(sv(v, v.tri_offset + (uint32_t)tri * 4u).w);
+    float roughness = 1.0f, metallic = 0.0f;
+    if (material < material_count) {
+        float4 m = materials[material];
+        roughness = m.x;
+        metallic = m.y;
+    }
+    F3 a = tri_albedo(v, tri);
+    albedo[i] = unorm_byte(a.x) | (unorm_byte(a.y) << 8) | (unorm_byte(a.z) << 16) | (unorm_byte(roughness) << 24);
+    uint32_t nw = unorm_byte(nn.x * 0.5f + 0.5f) | (unorm_byte(nn.y * 0.5f + 0.5f) << 8) |
+                  (unorm_byte(nn.z * 0.5f + 0.5f) << 16);
+    normal[i] = nw | (unorm_byte(metallic) << 24);
+    shading_normal[i] = nw;
+    F3 e = tri_emission(v, tri);
+    store_h4(emission, i, F4{e.x, e.y, e.z, 0.0f});
+}
+
+} // namespace
+
+int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* hit, float* dist, int any_hit,
+               hipStream_t stream)
+{
+    if (count == 0) return RGI_OK;
+    uint32_t blocks = (count + kTraceBlock - 1u) / kTraceBlock;
+    hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kTraceBlock), 0, stream, s, reinterpret_cast<const float4*>(rays),
+                       count, hit, dist, any_hit);
+    return check_launch("trace");
+}
+
+int gbuffer_primary(const rgi_frame& f, const rgi_scene& s, const float* materials, uint32_t material_count,
+                    float* depth, uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal, uint16_t* emission,
+                    hipStream_t stream)
+{
+    uint32_t n = (uint32_t)f.width * (uint32_t)f.height;
+    F2 texel = F2{1.0f / (float)f.width, 1.0f / (float)f.height};
+    hipLaunchKernelGGL(k_gbuffer, dim3((n + 255u) / 256u), dim3(256), 0, stream, f, s,
+                       reinterpret_cast<const float4*>(materials), material_count, texel, depth, albedo, normal,
+                       shading_normal, emission);
+    return check_launch("gbuffer primary");
+}
+
+} // namespace rgi

@@ -1,0 +1,183 @@
+"""Shared set-up of the tests: seeded scenes and cameras, and one frame of the
+hot path run either by the oracle (host arrays) or by the HIP library (device
+tensors), with the pass order and ping-pong of Renderer::render
+(src/renderer.cpp:626-833): DI temporal -> DI spatial -> grid clear -> grid
+insert -> GI temporal -> GI spatial.
+"""
+import ctypes
+
+import numpy as np
+
+import restir_amd as ra
+
+import orc
+
+GI_PLANES_F32 = ("sample", "receiver", "normal")
+
+
+def cornell(extra_lights=4, seed=1):
+    return ra.HostScene(ra.RGI_SCENE_CORNELL, seed=seed, extra_lights=extra_lights, light_seed=seed)
+
+
+def frame_for(hs, settings, width, height, frame_index, history_valid, di_history_valid=False, yaw_step=0.0,
+              jitter_seed=None):
+    eye, vp, inv = ra.orbit_camera(hs, frame_index, width, height, yaw_step=yaw_step)
+    if frame_index > 0:
+        peye, pvp, _ = ra.orbit_camera(hs, frame_index - 1, width, height, yaw_step=yaw_step)
+    else:
+        peye, pvp = eye, vp
+    return ra.make_frame(settings, hs, frame_index, eye, vp, inv, peye, pvp, width, height, history_valid,
+                         di_history_valid)
+
+
+def empty_history(width, height, table_size):
+    n = width * height
+    return {
+        "gi_sample": np.zeros((n, 4), np.float32),
+        "gi_receiver": np.zeros((n, 4), np.float32),
+        "gi_normal": np.zeros((n, 4), np.float32),
+        "color": np.zeros((n, 4), np.uint16),
+        "moments": np.zeros((n, 4), np.uint16),
+        "grid": np.zeros(34 * table_size, np.uint32),
+        "di_sample": np.zeros((n, 4), np.float32),
+        "di_receiver": np.zeros((n, 4), np.float32),
+        "di_normal": np.zeros((n, 4), np.uint16),
+    }
+
+
+# ------------------------------------------------------------------ oracle
+
+def oracle_gbuffer(hs, frame):
+    L = orc.lib()
+    n = frame.width * frame.height
+    g = {
+        "depth": np.zeros(n, np.float32),
+        "albedo": np.zeros(n, np.uint32),
+        "normal": np.zeros(n, np.uint32),
+        "shading": np.zeros(n, np.uint32),
+        "emission": np.zeros((n, 4), np.uint16),
+    }
+    scene = orc.host_scene(hs)
+    L.orc_gbuffer_primary(ctypes.byref(frame), ctypes.byref(scene), orc.ptr(hs.materials), hs.material_count,
+                          orc.ptr(g["depth"]), orc.ptr(g["albedo"]), orc.ptr(g["normal"]), orc.ptr(g["shading"]),
+                          orc.ptr(g["emission"]))
+    return g
+
+
+def host_gbuffer(g, width, height):
+    return ra.GBuffer(width, height, g["depth"].ctypes.data, g["albedo"].ctypes.data, g["normal"].ctypes.data,
+                      g["shading"].ctypes.data, g["emission"].ctypes.data)
+
+
+def oracle_frame(hs, frame, g, hist, insert=True):
+    """One frame on the oracle; returns this frame's outputs (host arrays)."""
+    L = orc.lib()
+    w, h, T = frame.width, frame.height, frame.grid_table_size
+    n = w * h
+    scene = orc.host_scene(hs)
+    gb = host_gbuffer(g, w, h)
+    out = {}
+    if frame.di_light_count > 0:
+        prev = ra.DiReservoirs(hist["di_sample"].ctypes.data, hist["di_receiver"].ctypes.data,
+                               hist["di_normal"].ctypes.data)
+        out["di_sample"] = np.zeros((n, 4), np.float32)
+        out["di_receiver"] = np.zeros((n, 4), np.float32)
+        out["di_normal"] = np.zeros((n, 4), np.uint16)
+        dres = ra.DiReservoirs(out["di_sample"].ctypes.data, out["di_receiver"].ctypes.data, out["di_normal"].ctypes.data)
+        L.orc_di_temporal(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(prev), ctypes.byref(dres))
+        out["diffuse"] = np.zeros((n, 4), np.uint16)
+        out["specular"] = np.zeros((n, 4), np.uint16)
+        L.orc_di_spatial(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(dres),
+                         orc.ptr(out["diffuse"]), orc.ptr(out["specular"]))
+        surface_light = out["diffuse"]
+    else:
+        surface_light = g["emission"]
+    grid = np.zeros(34 * T, np.uint32)
+    L.orc_grid_clear(orc.ptr(grid), T)
+    if insert:
+        L.orc_grid_insert(ctypes.byref(frame), ctypes.byref(gb), orc.ptr(grid))
+    out["grid"] = grid
+    gin = ra.GiInputs(hist["gi_sample"].ctypes.data, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
+                      hist["color"].ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, None,
+                      hist["grid"].ctypes.data)
+    for k in GI_PLANES_F32:
+        out["gi_" + k] = np.zeros((n, 4), np.float32)
+    gres = ra.GiReservoirs(out["gi_sample"].ctypes.data, out["gi_receiver"].ctypes.data, out["gi_normal"].ctypes.data)
+    L.orc_gi_temporal(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(gin), ctypes.byref(gres))
+    sin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data, hist["color"].ctypes.data,
+                      hist["moments"].ctypes.data, surface_light.ctypes.data, None, grid.ctypes.data)
+    out["color"] = np.zeros((n, 4), np.uint16)
+    out["moments"] = np.zeros((n, 4), np.uint16)
+    L.orc_gi_spatial(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(sin), ctypes.byref(gres),
+                     orc.ptr(out["color"]), orc.ptr(out["moments"]))
+    out["surface_light"] = surface_light
+    return out
+
+
+def next_history(hist, out):
+    """The ping-pong of renderer.cpp:683, 1119-1124 (denoiser off)."""
+    h = dict(hist)
+    for k in ("gi_sample", "gi_receiver", "gi_normal", "color", "moments", "grid"):
+        h[k] = out[k]
+    if "di_sample" in out:
+        for k in ("di_sample", "di_receiver", "di_normal"):
+            h[k] = out[k]
+    return h
+
+
+def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0):
+    """Runs `frames` frames on the oracle; returns (frames, gbuffers, histories, outputs)."""
+    hist = empty_history(width, height, settings.grid_table_size)
+    seq = []
+    for i in range(frames):
+        f = frame_for(hs, settings, width, height, i, history_valid=i > 0, di_history_valid=i > 0, yaw_step=yaw_step)
+        g = oracle_gbuffer(hs, f)
+        out = oracle_frame(hs, f, g, hist)
+        seq.append((f, g, hist, out))
+        hist = next_history(hist, out)
+    return seq
+
+
+# ---------------------------------------------------------------- device
+
+class Device:
+    """The HIP library over torch device tensors (plumbing only)."""
+
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.L = ra.lib()
+        ctx = ctypes.c_void_p()
+        ra.check(self.L.rgi_create(ctypes.byref(ctx)))
+        self.ctx = ctx
+
+    def close(self):
+        if self.ctx:
+            self.L.rgi_destroy(self.ctx)
+            self.ctx = None
+
+    @property
+    def stream(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    def put(self, a):
+        t = self.torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1)).cuda()
+        return t
+
+    def empty(self, nbytes):
+        return self.torch.zeros(nbytes, dtype=self.torch.uint8, device="cuda")
+
+    def get(self, t, like):
+        self.torch.cuda.synchronize()
+        return t.cpu().numpy().view(like.dtype).reshape(like.shape)
+
+    def scene(self, hs):
+        self.bvh_t = self.put(hs.bvh)
+        self.lights_t = self.put(hs.lights)
+        return hs.rgi_scene(self.bvh_t.data_ptr(), self.lights_t.data_ptr())
+
+    def gbuffer(self, g, width, height):
+        d = {k: self.put(v) for k, v in g.items()}
+        gb = ra.GBuffer(width, height, d["depth"].data_ptr(), d["albedo"].data_ptr(), d["normal"].data_ptr(),
+                        d["shading"].data_ptr(), d["emission"].data_ptr())
+        return gb, d

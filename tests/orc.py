@@ -1,0 +1,87 @@
+"""ctypes binding of the host oracle (oracle/liboracle.so) — the checker.
+
+Test infrastructure only: the oracle restates the reference's shader logic on
+the CPU; tests compare the HIP path against it on the same inputs.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from restir_amd import DiReservoirs, Frame, GBuffer, GiInputs, GiReservoirs, Scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_LIB):
+        build()
+    L = ctypes.CDLL(ORACLE_LIB)
+    P, U32, F = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float
+    PF = ctypes.POINTER(Frame)
+    PS = ctypes.POINTER(Scene)
+    PG = ctypes.POINTER(GBuffer)
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_set_threads.restype = None
+    L.orc_get_threads.restype = ctypes.c_int
+    L.orc_pcg.restype = U32
+    L.orc_pcg.argtypes = [U32]
+    L.orc_checksum_hash.restype = U32
+    L.orc_checksum_hash.argtypes = [U32]
+    L.orc_next_random.restype = F
+    L.orc_next_random.argtypes = [ctypes.POINTER(U32)]
+    L.orc_grid_cell_size.restype = F
+    L.orc_grid_cell_size.argtypes = [PF, P, P]
+    L.orc_grid_normal_bucket.restype = U32
+    L.orc_grid_normal_bucket.argtypes = [P]
+    L.orc_grid_key.restype = None
+    L.orc_grid_key.argtypes = [PF, P, P, F, ctypes.POINTER(U32), ctypes.POINTER(U32)]
+    L.orc_cosine_hemisphere.restype = None
+    L.orc_cosine_hemisphere.argtypes = [P, F, F, P]
+    L.orc_sincos.restype = None
+    L.orc_sincos.argtypes = [F, ctypes.POINTER(F), ctypes.POINTER(F)]
+    L.orc_half_bits.restype = ctypes.c_uint16
+    L.orc_half_bits.argtypes = [F]
+    L.orc_grid_keys.restype = None
+    L.orc_grid_keys.argtypes = [PF, PG, P, P, P]
+    L.orc_grid_clear.restype = None
+    L.orc_grid_clear.argtypes = [P, U32]
+    L.orc_grid_insert.restype = None
+    L.orc_grid_insert.argtypes = [PF, PG, P]
+    L.orc_grid_pick_entry.restype = U32
+    L.orc_grid_pick_entry.argtypes = [PF, P, P, P, P, F, F, F]
+    L.orc_trace.restype = None
+    L.orc_trace.argtypes = [PS, P, U32, P, P, ctypes.c_int]
+    L.orc_gi_temporal.restype = None
+    L.orc_gi_temporal.argtypes = [PF, PS, PG, ctypes.POINTER(GiInputs), ctypes.POINTER(GiReservoirs)]
+    L.orc_gi_spatial.restype = None
+    L.orc_gi_spatial.argtypes = [PF, PS, PG, ctypes.POINTER(GiInputs), ctypes.POINTER(GiReservoirs), P, P]
+    L.orc_di_temporal.restype = None
+    L.orc_di_temporal.argtypes = [PF, PS, PG, ctypes.POINTER(DiReservoirs), ctypes.POINTER(DiReservoirs)]
+    L.orc_di_spatial.restype = None
+    L.orc_di_spatial.argtypes = [PF, PS, PG, ctypes.POINTER(DiReservoirs), P, P]
+    L.orc_gbuffer_primary.restype = None
+    L.orc_gbuffer_primary.argtypes = [PF, PS, P, U32, P, P, P, P, P]
+    _lib = L
+    return L
+
+
+def ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def host_scene(hs):
+    """rgi_scene over the HostScene's host arrays."""
+    return hs.rgi_scene(hs.bvh.ctypes.data, hs.lights.ctypes.data)
