@@ -1,0 +1,238 @@
+"""Benchmark of the ReSTIR GI/DI hot path on one MI355X.
+
+Workload (the 1080p Cornell configuration): the procedural open-front
+Cornell box (36 triangles, emissive ceiling quad of radiance 17) seen at
+1920 x 1080 along a 16-frame camera orbit (yaw step 2 pi / 64), ReSTIR DI over
+the 2 emissive triangles plus 4 seeded point lights, a 2^20-cell hash grid,
+default settings (2 candidates, 2 temporal taps, 4 spatial taps, 32 march
+steps, hybrid rays, clamps and boiling filter on). A step is one GI frame:
+DI temporal, DI spatial, grid clear, grid insert, GI temporal, GI spatial,
+on G-buffers already resident in HBM. The separate traversal benchmark traces
+2^24 cosine rays from uniform surface points (closest hit, tmax 4 x scene
+scale) and reports Mrays/s.
+
+Prints one JSON line. --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import restir_amd as ra  # noqa: E402
+from restir_amd import driver  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per pixel of each pass (compulsory traffic with the
+# reference's formats; see DESIGN.md for the derivation).
+BYTES_PER_PIXEL = {
+    "grid_insert": 20.0,
+    "gi_temporal": 220.0,
+    "gi_spatial": 508.0,
+    "di_temporal": 96.0,
+    "di_spatial": 240.0,
+}
+
+
+def surface_rays(hs, count, seed):
+    """Cosine rays from points uniform over the scene's area, both sides."""
+    rng = np.random.default_rng(seed)
+    p = hs.positions.reshape(-1, 3, 3).astype(np.float64)
+    e1, e2 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]
+    cr = np.cross(e1, e2)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    rays = np.zeros((count, 8), np.float32)
+    chunk = 1 << 21
+    for start in range(0, count, chunk):
+        m = min(chunk, count - start)
+        tri = rng.choice(len(area), size=m, p=area / area.sum())
+        s = np.sqrt(rng.random(m))
+        r1 = rng.random(m)
+        pts = p[tri, 0] + e1[tri] * (1 - s)[:, None] + e2[tri] * (r1 * s)[:, None]
+        n = cr[tri] / np.linalg.norm(cr[tri], axis=1)[:, None]
+        n *= np.where(rng.random(m) < 0.5, -1.0, 1.0)[:, None]
+        a = np.where(np.abs(n[:, 1:2]) < 0.99, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+        t = np.cross(n, a)
+        t /= np.linalg.norm(t, axis=1)[:, None]
+        b = np.cross(n, t)
+        q0, q1 = rng.random(m), rng.random(m)
+        rr, phi = np.sqrt(q0), 2 * np.pi * q1
+        d = t * (rr * np.cos(phi))[:, None] + b * (rr * np.sin(phi))[:, None] + n * np.sqrt(1 - q0)[:, None]
+        rays[start:start + m, 0:3] = pts + n * (1e-4 * hs.radius)
+        rays[start:start + m, 3] = 4.0 * hs.scene_scale
+        rays[start:start + m, 4:7] = d
+    return rays
+
+
+def cpu_baseline(hs, settings, width, height, table):
+    """The oracle on the host cores: one steady-state frame (third of a sequence)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import common
+    import orc
+    threads = min(os.cpu_count() or 1, 16)
+    orc.lib().orc_set_threads(threads)
+    s = ra.Settings(**settings.__dict__)
+    s.grid_table_size = table
+    hist = common.empty_history(width, height, table)
+    elapsed = None
+    for i in range(3):
+        f = common.frame_for(hs, s, width, height, i, history_valid=i > 0, di_history_valid=i > 0,
+                             yaw_step=2 * math.pi / 64)
+        g = common.oracle_gbuffer(hs, f)
+        t0 = time.perf_counter()
+        out = common.oracle_frame(hs, f, g, hist)
+        elapsed = time.perf_counter() - t0
+        hist = common.next_history(hist, out)
+    return {"value": elapsed * 1e3, "unit": "ms per GI frame", "cores": threads, "kind": "port",
+            "sample": "oracle, one steady-state %dx%d frame (frame 3 of the orbit), all passes" % (width, height)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--table", type=int, default=1 << 20)
+    ap.add_argument("--frames", type=int, default=16)
+    ap.add_argument("--rays", type=int, default=1 << 24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid-mode", choices=["atomic", "deterministic"], default="atomic")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    torch.cuda.init()
+    W, H, T = args.width, args.height, args.table
+    hs = ra.HostScene(ra.RGI_SCENE_CORNELL, seed=1, extra_lights=4, light_seed=1)
+    settings = ra.Settings(grid_table_size=T)
+    gscene = driver.GpuScene(hs)
+    mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
+    drv = driver.Driver(W, H, T, mode)
+
+    # The orbit's cameras and G-buffers, produced once, resident in HBM.
+    yaw_step = 2 * math.pi / 64
+    cams = [ra.orbit_camera(hs, i, W, H, yaw_step=yaw_step) for i in range(args.frames)]
+    gbufs = []
+    for i in range(args.frames):
+        eye, vp, inv = cams[i]
+        f = ra.make_frame(settings, hs, i, eye, vp, inv, eye, vp, W, H, False)
+        gbufs.append(driver.GBufferSet(drv.ctx, gscene, f))
+    torch.cuda.synchronize()
+
+    def frame_for_step(step):
+        i = step % args.frames
+        p = (step - 1) % args.frames
+        eye, vp, inv = cams[i]
+        peye, pvp, _ = cams[p]
+        return ra.make_frame(settings, hs, step, eye, vp, inv, peye, pvp, W, H, step > 0, step > 0), i
+
+    frames = [frame_for_step(k) for k in range(args.warmup + args.steps)]
+
+    for k in range(args.warmup):
+        f, i = frames[k]
+        drv.render(f, gscene, gbufs[i])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        f, i = frames[k]
+        drv.render(f, gscene, gbufs[i])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_frame = elapsed * 1e3 / args.steps
+
+    # Per-pass durations with HIP events on the launch stream.
+    events = {}
+
+    def hook(name, start):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        events.setdefault(name, []).append(e)
+
+    timed = min(args.steps, 32)
+    for k in range(timed):
+        f, i = frames[(args.warmup + k) % len(frames)]
+        f = ra.Frame.from_buffer_copy(f)
+        drv.render(f, gscene, gbufs[i], hook)
+    torch.cuda.synchronize()
+    pass_ms = {}
+    for name, evs in events.items():
+        pass_ms[name] = sum(evs[j].elapsed_time(evs[j + 1]) for j in range(0, len(evs), 2)) / (len(evs) // 2)
+
+    # Traversal microbenchmark.
+    rays = surface_rays(hs, args.rays, seed=7)
+    rays_t = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()
+    hit = torch.empty(args.rays * 4, dtype=torch.uint8, device="cuda")
+    dist = torch.empty(args.rays * 4, dtype=torch.uint8, device="cuda")
+    L = ra.lib()
+    s = driver.stream()
+
+    def trace():
+        ra.check(L.rgi_trace(drv.ctx, ctypes.byref(gscene.scene), rays_t.data_ptr(), args.rays, hit.data_ptr(),
+                             dist.data_ptr(), 0, s))
+
+    trace()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record()
+    for _ in range(reps):
+        trace()
+    e1.record()
+    torch.cuda.synchronize()
+    trace_ms = e0.elapsed_time(e1) / reps
+    mrays = args.rays / (trace_ms * 1e-3) / 1e6
+    hit_rate = float((hit.view(torch.int32) >= 0).float().mean().item())
+
+    dominant = max((k for k in pass_ms if k in BYTES_PER_PIXEL), key=lambda k: pass_ms[k])
+    traffic = BYTES_PER_PIXEL[dominant] * W * H
+    achieved = traffic / (pass_ms[dominant] * 1e-3) / 1e9
+    roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "algorithmic_bytes": traffic, "avg_ms": round(pass_ms[dominant], 5)}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(hs, settings, W, H, T)
+
+    drv.close()
+    line = {
+        "metric": "GI frame ms @1080p (grid + resample + trace)",
+        "value": round(ms_per_frame, 5),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_frame, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural seeded Cornell box, G-buffers from primary rays)",
+        "config": {"workload": "cornell_1080p_orbit%d" % args.frames, "width": W, "height": H, "grid_table_size": T,
+                   "frames": args.frames, "di_lights": hs.light_count, "grid_mode": args.grid_mode,
+                   "parallelism": "single GPU"},
+        "frames_per_s": round(1e3 / ms_per_frame, 2),
+        "pass_ms": {k: round(v, 5) for k, v in sorted(pass_ms.items())},
+        "mrays_per_s": round(mrays, 2),
+        "trace": {"rays": args.rays, "ms": round(trace_ms, 4), "hit_rate": round(hit_rate, 4), "mode": "closest"},
+        "mse_vs_reference": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

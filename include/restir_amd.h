@@ -185,7 +185,8 @@ int rgi_grid_clear(uint32_t* grid, uint32_t table_size, void* stream);
  * schedules them. RGI_GRID_DETERMINISTIC gives the result of one legal order
  * of them: probe round by probe round, pixels in ascending index within a
  * round; a cell's count is every pixel resolved to it, and its entries are
- * its 32 lowest pixel indices, ascending. */
+ * its 32 lowest pixel indices, ascending. Both modes expect `grid` cleared by
+ * rgi_grid_clear first, as the reference always clears it before inserting. */
 int rgi_grid_insert(rgi_context* context, const rgi_frame* frame, const rgi_gbuffer* gbuffer,
                     uint32_t* grid, int mode, void* stream);
 

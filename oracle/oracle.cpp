@@ -1295,6 +1295,15 @@ void orc_cosine_hemisphere(const float* n, float r0, float r1, float* out)
     out[2] = d.z;
 }
 void orc_sincos(float x, float* s, float* c) { rgi_sincos(x, s, c); }
+void orc_brdf(const float* n, const float* v, const float* l, const float* a, float roughness, float metallic,
+              float* diffuse, float* specular)
+{
+    V3 d, sp;
+    evaluate_brdf(V3{n[0], n[1], n[2]}, V3{v[0], v[1], v[2]}, V3{l[0], l[1], l[2]}, V3{a[0], a[1], a[2]}, roughness,
+                  metallic, d, sp);
+    diffuse[0] = d.x; diffuse[1] = d.y; diffuse[2] = d.z;
+    specular[0] = sp.x; specular[1] = sp.y; specular[2] = sp.z;
+}
 uint16_t orc_half_bits(float value) { return rgi_half_bits(value); }
 
 void orc_grid_keys(const rgi_frame* frame, const rgi_gbuffer* g, uint32_t* out_slot, uint32_t* out_checksum,

@@ -39,6 +39,9 @@ void orc_grid_key(const rgi_frame* frame, const float* position3, const float* n
                   uint32_t* slot, uint32_t* checksum);  /* hashgrid.sh:116-127 */
 void orc_cosine_hemisphere(const float* normal3, float r0, float r1, float* out3); /* restir.sh:201-212 */
 void orc_sincos(float x, float* s, float* c);
+/* evaluateBrdf (brdf.sh:38-62): diffuse and specular (each times n.l). */
+void orc_brdf(const float* n3, const float* v3, const float* l3, const float* albedo3, float roughness,
+              float metallic, float* diffuse3, float* specular3);
 uint16_t orc_half_bits(float value);
 
 /* Per-pixel keys of cs_grid_insert (cs_grid_insert.sc:40-73): slot, checksum

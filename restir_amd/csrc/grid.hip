@@ -102,7 +102,7 @@ struct DetScratch {
     uint32_t* occupied;  // [min(T, n)] slots claimed, in claim order
     uint32_t* offset;    // [min(T, n)] CSR offsets per cell
     uint32_t* fill;      // [min(T, n)] CSR fill counters per cell
-    uint32_t* counters;  // [0] pending count, [1] occupied count
+    uint32_t* counters;  // [0] pending count, [1] occupied count, [2] resolved count
 };
 
 constexpr uint32_t kTagShift = 29;
@@ -158,6 +158,7 @@ RGI_DEV bool det_resolve(uint32_t i, uint32_t round, uint32_t table, uint32_t* g
     if (done) {
         d.resolved[i] = s;
         atomicAdd(&grid[table + s], 1u);
+        atomicAdd(&d.counters[2], 1u);
     }
     return done;
 }
@@ -238,7 +239,7 @@ __global__ void __launch_bounds__(256) k_det_scatter(uint32_t n, DetScratch d)
     if (s == kGridInvalid) return;
     uint32_t k = d.claim[s];
     uint32_t pos = d.offset[k] + atomicAdd(&d.fill[k], 1u);
-    d.list[pos] = i;
+    if (pos < d.counters[2]) d.list[pos] = i;
 }
 
 RGI_DEV uint32_t bitonic_sort64(uint32_t v, uint32_t lane)
@@ -261,7 +262,11 @@ __global__ void __launch_bounds__(256) k_det_select(uint32_t table, uint32_t* gr
     uint32_t cells = d.counters[1];
     for (uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6); k < cells; k += gridDim.x * 4u) {
         uint32_t s = d.occupied[k];
+        // Counts the grid held before this insert are not in the lists: read
+        // no further than what this insert listed.
         uint32_t n = grid[table + s];
+        uint32_t listed = d.counters[2] - d.offset[k];
+        n = n < listed ? n : listed;
         const uint32_t* src = d.list + d.offset[k];
         uint32_t v = lane < n ? src[lane] : 0xffffffffu;
         v = bitonic_sort64(v, lane);

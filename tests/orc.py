@@ -52,6 +52,8 @@ def lib():
     L.orc_cosine_hemisphere.argtypes = [P, F, F, P]
     L.orc_sincos.restype = None
     L.orc_sincos.argtypes = [F, ctypes.POINTER(F), ctypes.POINTER(F)]
+    L.orc_brdf.restype = None
+    L.orc_brdf.argtypes = [P, P, P, P, F, F, P, P]
     L.orc_half_bits.restype = ctypes.c_uint16
     L.orc_half_bits.argtypes = [F]
     L.orc_grid_keys.restype = None

@@ -1,0 +1,418 @@
+"""GPU parity: every hot-path entry point of the C ABI against the oracle on
+identical inputs. Integer outputs (grid keys, slots, checksums, counts,
+entries, hit triangle ids) must be bit-exact, and so must every float plane:
+the kernels evaluate each expression in the oracle's order with FP
+contraction off. Should a float plane ever differ, the test reports the
+fraction of texels outside the 1e-4 relative bound as well.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import restir_amd as ra
+
+import common
+import orc
+
+pytestmark = pytest.mark.gpu
+
+W, H = 320, 240
+
+
+@pytest.fixture(scope="module")
+def dev():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    d = common.Device()
+    yield d
+    d.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    """Exact equality of two planes (+0 == -0, NaN == NaN)."""
+    if a.dtype.kind == "f":
+        return (a == b) | (np.isnan(a) & np.isnan(b))
+    return a == b
+
+
+def assert_same(name, got, want, rel=None):
+    eq = same(got, want)
+    bad = int((~eq).sum())
+    if bad:
+        idx = np.argwhere(~eq)[:5]
+        detail = "; ".join("%s got %r want %r" % (tuple(i), got[tuple(i)], want[tuple(i)]) for i in idx)
+        if rel is not None and got.dtype.kind == "f":
+            err = np.abs(got.astype(np.float64) - want) / np.maximum(np.abs(want.astype(np.float64)), 1e-30)
+            say("%s: %d mismatches, %.6f%% beyond %g relative" % (name, bad, 100.0 * (err > rel).mean(), rel))
+        raise AssertionError("%s: %d of %d values differ: %s" % (name, bad, eq.size, detail))
+
+
+def halfs(a):
+    return a.view(np.float16).astype(np.float32)
+
+
+@pytest.fixture(scope="module")
+def cornell_seq():
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings()
+    seq = common.oracle_sequence(hs, s, W, H, 3, yaw_step=2 * np.pi / 64)
+    return hs, s, seq
+
+
+# ------------------------------------------------------------------ BVH
+
+def gpu_gbuffer(dev, hs, frame):
+    scene = dev.scene(hs)
+    n = frame.width * frame.height
+    mats = dev.put(hs.materials)
+    t = {k: dev.empty(n * s) for k, s in (("depth", 4), ("albedo", 4), ("normal", 4), ("shading", 4), ("emission", 8))}
+    ra.check(dev.L.rgi_gbuffer_primary(dev.ctx, ctypes.byref(frame), ctypes.byref(scene), mats.data_ptr(),
+                                       hs.material_count, t["depth"].data_ptr(), t["albedo"].data_ptr(),
+                                       t["normal"].data_ptr(), t["shading"].data_ptr(), t["emission"].data_ptr(),
+                                       dev.stream))
+    out = {
+        "depth": dev.get(t["depth"], np.zeros(n, np.float32)),
+        "albedo": dev.get(t["albedo"], np.zeros(n, np.uint32)),
+        "normal": dev.get(t["normal"], np.zeros(n, np.uint32)),
+        "shading": dev.get(t["shading"], np.zeros(n, np.uint32)),
+        "emission": dev.get(t["emission"], np.zeros((n, 4), np.uint16)),
+    }
+    return out
+
+
+@pytest.mark.parametrize("kind,size", [(ra.RGI_SCENE_CORNELL, 0), (ra.RGI_SCENE_BOXFIELD, 400)])
+def test_gbuffer_primary(dev, kind, size):
+    say("case gbuffer_primary kind=%d size=%d %dx%d" % (kind, size, W, H))
+    hs = ra.HostScene(kind, seed=3, size=size)
+    f = common.frame_for(hs, ra.Settings(), W, H, 0, False)
+    want = common.oracle_gbuffer(hs, f)
+    got = gpu_gbuffer(dev, hs, f)
+    for k in want:
+        assert_same("gbuffer." + k, got[k], want[k])
+
+
+def surface_rays(hs, count, seed, tmax_scale=4.0):
+    """Cosine rays from uniform surface points (the traversal benchmark's rays)."""
+    rng = np.random.default_rng(seed)
+    p = hs.positions.reshape(-1, 3, 3).astype(np.float64)
+    e1, e2 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]
+    cr = np.cross(e1, e2)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    tri = rng.choice(len(area), size=count, p=area / area.sum())
+    r0, r1 = rng.random(count), rng.random(count)
+    s = np.sqrt(r0)
+    u, v = 1 - s, r1 * s
+    pts = p[tri, 0] + e1[tri] * u[:, None] + e2[tri] * v[:, None]
+    n = cr[tri] / np.linalg.norm(cr[tri], axis=1)[:, None]
+    n *= np.where(rng.random(count) < 0.5, -1.0, 1.0)[:, None]
+    a = np.where(np.abs(n[:, 1:2]) < 0.99, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(n, a)
+    t /= np.linalg.norm(t, axis=1)[:, None]
+    b = np.cross(n, t)
+    q0, q1 = rng.random(count), rng.random(count)
+    rr, phi = np.sqrt(q0), 2 * np.pi * q1
+    d = t * (rr * np.cos(phi))[:, None] + b * (rr * np.sin(phi))[:, None] + n * np.sqrt(1 - q0)[:, None]
+    rays = np.zeros((count, 8), np.float32)
+    rays[:, 0:3] = pts + n * (1e-4 * hs.radius)
+    rays[:, 3] = tmax_scale * hs.scene_scale
+    rays[:, 4:7] = d
+    return rays
+
+
+def trace_both(dev, hs, rays, any_hit):
+    n = len(rays)
+    want_t = np.zeros(n, np.int32)
+    want_d = np.zeros(n, np.float32)
+    scene = orc.host_scene(hs)
+    orc.lib().orc_trace(ctypes.byref(scene), orc.ptr(rays), n, orc.ptr(want_t), orc.ptr(want_d), int(any_hit))
+    dscene = dev.scene(hs)
+    r = dev.put(rays)
+    ht, hd = dev.empty(4 * n), dev.empty(4 * n)
+    ra.check(dev.L.rgi_trace(dev.ctx, ctypes.byref(dscene), r.data_ptr(), n, ht.data_ptr(), hd.data_ptr(),
+                             int(any_hit), dev.stream))
+    return dev.get(ht, want_t), dev.get(hd, want_d), want_t, want_d
+
+
+@pytest.mark.parametrize("kind,size,any_hit", [
+    (ra.RGI_SCENE_CORNELL, 0, False), (ra.RGI_SCENE_CORNELL, 0, True),
+    (ra.RGI_SCENE_BOXFIELD, 3000, False), (ra.RGI_SCENE_BOXFIELD, 3000, True),
+    (ra.RGI_SCENE_ATRIUM, 60000, False)])
+def test_trace(dev, kind, size, any_hit):
+    say("case trace kind=%d size=%d any_hit=%d" % (kind, size, any_hit))
+    hs = ra.HostScene(kind, seed=5, size=size)
+    rays = surface_rays(hs, 1 << 16, seed=kind * 7 + size, tmax_scale=8.0)
+    gt, gd, wt, wd = trace_both(dev, hs, rays, any_hit)
+    assert_same("trace.triangle", gt, wt)
+    assert_same("trace.distance", gd, wd)
+    assert (wt >= 0).mean() > 0.3
+
+
+# ----------------------------------------------------------------- grid
+
+def gpu_grid(dev, frame, g, mode):
+    T = frame.grid_table_size
+    gb, keep = dev.gbuffer(g, frame.width, frame.height)
+    grid = dev.empty(34 * T * 4)
+    ra.check(dev.L.rgi_grid_clear(grid.data_ptr(), T, dev.stream))
+    ra.check(dev.L.rgi_grid_insert(dev.ctx, ctypes.byref(frame), ctypes.byref(gb), grid.data_ptr(), mode, dev.stream))
+    return dev.get(grid, np.zeros(34 * T, np.uint32))
+
+
+@pytest.mark.parametrize("table", [65536, 1024, 64])
+def test_grid_insert_deterministic(dev, cornell_seq, table):
+    hs, s, seq = cornell_seq
+    f, g, hist, out = seq[1]
+    f = ra.Frame.from_buffer_copy(f)
+    f.grid_table_size = table
+    say("case grid_insert_deterministic table=%d" % table)
+    want = np.zeros(34 * table, np.uint32)
+    orc.lib().orc_grid_insert(ctypes.byref(f), ctypes.byref(common.host_gbuffer(g, W, H)), orc.ptr(want))
+    got = gpu_grid(dev, f, g, ra.RGI_GRID_DETERMINISTIC)
+    assert_same("grid", got, want)
+    # Twice in a row gives the same buffer.
+    again = gpu_grid(dev, f, g, ra.RGI_GRID_DETERMINISTIC)
+    assert_same("grid.repeat", again, want)
+
+
+@pytest.mark.parametrize("table", [65536, 1024])
+def test_grid_insert_atomic(dev, cornell_seq, table):
+    """Atomic mode: schedule-dependent layout, schedule-independent cells."""
+    hs, s, seq = cornell_seq
+    f, g, hist, out = seq[1]
+    f = ra.Frame.from_buffer_copy(f)
+    f.grid_table_size = table
+    say("case grid_insert_atomic table=%d" % table)
+    want = np.zeros(34 * table, np.uint32)
+    orc.lib().orc_grid_insert(ctypes.byref(f), ctypes.byref(common.host_gbuffer(g, W, H)), orc.ptr(want))
+    got = gpu_grid(dev, f, g, ra.RGI_GRID_ATOMIC)
+    T = table
+    # Keys of every pixel, from the oracle.
+    n = W * H
+    slot0 = np.zeros(n, np.uint32)
+    cs = np.zeros(n, np.uint32)
+    orc.lib().orc_grid_keys(ctypes.byref(f), ctypes.byref(common.host_gbuffer(g, W, H)), orc.ptr(slot0), orc.ptr(cs), None)
+    if T == 65536:
+        # No probe collisions at this load: same checksums and counts per slot.
+        assert_same("grid.checksums", got[:T], want[:T])
+        assert_same("grid.counts", got[T:2 * T], want[T:2 * T])
+    # Every listed entry is a pixel of that cell, lists are unique, and their
+    # length is min(count, 32).
+    cnt = got[T:2 * T]
+    assert cnt.sum() <= (cs != 0).sum()
+    for s_ in np.nonzero(got[:T])[0][:4096]:
+        k = min(int(cnt[s_]), 32)
+        e = got[2 * T + s_ * 32: 2 * T + s_ * 32 + k]
+        assert len(set(e.tolist())) == k
+        assert np.all(cs[e] == got[s_])
+    # Multiset of (checksum, count) matches the deterministic result when nothing is dropped.
+    if T == 65536:
+        assert sorted(zip(got[:T][got[:T] != 0], cnt[got[:T] != 0])) == sorted(zip(want[:T][want[:T] != 0], want[T:2 * T][want[:T] != 0]))
+
+
+# ------------------------------------------------------------- GI / DI
+
+def put_hist(dev, hist):
+    return {k: dev.put(v) for k, v in hist.items()}
+
+
+def gpu_gi_temporal(dev, hs, f, g, hist, surface_light):
+    n = f.gi_width * f.gi_height
+    scene = dev.scene(hs)
+    gb, keep = dev.gbuffer(g, f.gi_width, f.gi_height)
+    h = put_hist(dev, hist)
+    sl = dev.put(surface_light)
+    shadow = dev.put(hist["shadow"]) if "shadow" in hist else None
+    o = {k: dev.empty(n * 16) for k in ("sample", "receiver", "normal")}
+    gin = ra.GiInputs(h["gi_sample"].data_ptr(), h["gi_receiver"].data_ptr(), h["gi_normal"].data_ptr(),
+                      h["color"].data_ptr(), h["moments"].data_ptr(), sl.data_ptr(),
+                      shadow.data_ptr() if shadow is not None else None, h["grid"].data_ptr())
+    gres = ra.GiReservoirs(o["sample"].data_ptr(), o["receiver"].data_ptr(), o["normal"].data_ptr())
+    ra.check(dev.L.rgi_gi_temporal(dev.ctx, ctypes.byref(f), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(gin),
+                                   ctypes.byref(gres), dev.stream))
+    like = np.zeros((n, 4), np.float32)
+    return {"gi_" + k: dev.get(v, like) for k, v in o.items()}
+
+
+def oracle_gi_temporal(hs, f, g, hist, surface_light):
+    n = f.gi_width * f.gi_height
+    out = {"gi_" + k: np.zeros((n, 4), np.float32) for k in ("sample", "receiver", "normal")}
+    shadow = hist.get("shadow")
+    gin = ra.GiInputs(hist["gi_sample"].ctypes.data, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
+                      hist["color"].ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data,
+                      shadow.ctypes.data if shadow is not None else None, hist["grid"].ctypes.data)
+    gres = ra.GiReservoirs(out["gi_sample"].ctypes.data, out["gi_receiver"].ctypes.data, out["gi_normal"].ctypes.data)
+    scene = orc.host_scene(hs)
+    orc.lib().orc_gi_temporal(ctypes.byref(f), ctypes.byref(scene), ctypes.byref(common.host_gbuffer(g, W, H)),
+                              ctypes.byref(gin), ctypes.byref(gres))
+    return out
+
+
+def gpu_gi_spatial(dev, hs, f, g, hist, res, grid, surface_light):
+    n = f.gi_width * f.gi_height
+    scene = dev.scene(hs)
+    gb, keep = dev.gbuffer(g, f.gi_width, f.gi_height)
+    h = put_hist(dev, hist)
+    r = {k: dev.put(v) for k, v in res.items()}
+    gr = dev.put(grid)
+    sl = dev.put(surface_light)
+    shadow = dev.put(hist["shadow"]) if "shadow" in hist else None
+    oc, om = dev.empty(n * 8), dev.empty(n * 8)
+    gin = ra.GiInputs(None, h["gi_receiver"].data_ptr(), h["gi_normal"].data_ptr(), h["color"].data_ptr(),
+                      h["moments"].data_ptr(), sl.data_ptr(), shadow.data_ptr() if shadow is not None else None,
+                      gr.data_ptr())
+    gres = ra.GiReservoirs(r["gi_sample"].data_ptr(), r["gi_receiver"].data_ptr(), r["gi_normal"].data_ptr())
+    ra.check(dev.L.rgi_gi_spatial(dev.ctx, ctypes.byref(f), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(gin),
+                                  ctypes.byref(gres), oc.data_ptr(), om.data_ptr(), dev.stream))
+    like = np.zeros((n, 4), np.uint16)
+    return {"color": dev.get(oc, like), "moments": dev.get(om, like)}
+
+
+def oracle_gi_spatial(hs, f, g, hist, res, grid, surface_light):
+    n = f.gi_width * f.gi_height
+    oc, om = np.zeros((n, 4), np.uint16), np.zeros((n, 4), np.uint16)
+    shadow = hist.get("shadow")
+    gin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data, hist["color"].ctypes.data,
+                      hist["moments"].ctypes.data, surface_light.ctypes.data,
+                      shadow.ctypes.data if shadow is not None else None, grid.ctypes.data)
+    gres = ra.GiReservoirs(res["gi_sample"].ctypes.data, res["gi_receiver"].ctypes.data, res["gi_normal"].ctypes.data)
+    scene = orc.host_scene(hs)
+    orc.lib().orc_gi_spatial(ctypes.byref(f), ctypes.byref(scene), ctypes.byref(common.host_gbuffer(g, W, H)),
+                             ctypes.byref(gin), ctypes.byref(gres), orc.ptr(oc), orc.ptr(om))
+    return {"color": oc, "moments": om}
+
+
+GI_CASES = {
+    "default": {},
+    "bvh_rays": {"ray_mode": ra.RGI_RAY_BVH},
+    "screen_rays": {"ray_mode": ra.RGI_RAY_SCREEN},
+    "screen_reuse": {"world_space_reuse": False},
+    "sky_gi_only": {"sky": True, "restir_di": False},
+    "no_clamps": {"clamp_jacobian": False, "clamp_outliers": False, "boiling_filter": False},
+    "shadowed_light": {"light_color": (1.0, 1.0, 1.0), "shadows": True},
+    "small_grid": {"grid_table_size": 1024},
+}
+
+
+def case_frame(hs, frame, **kw):
+    """`frame` with per-case overrides of the settings-derived fields."""
+    f = ra.Frame.from_buffer_copy(frame)
+    s = ra.Settings(**kw)
+    ref = ra.make_frame(s, hs, f.frame_index, list(f.camera_position)[:3], list(f.view_proj), list(f.inv_view_proj),
+                        list(f.prev_camera_position)[:3], list(f.prev_view_proj), f.width, f.height, f.history_valid,
+                        f.di_history_valid)
+    for name in ("ray_mode", "world_space_reuse", "sky_zenith", "lighting_surface", "lighting_emission",
+                 "jacobian_clamp", "outlier_clamp", "boiling_filter", "light_color", "light_position",
+                 "shadows_enabled", "grid_table_size", "di_light_count"):
+        setattr(f, name, getattr(ref, name))
+    if f.shadows_enabled:
+        f.shadow_width, f.shadow_height = 96, 64
+        rng = np.random.default_rng(11)
+        # A synthetic atlas: every face's matrix maps the scene into its tile.
+        for face in range(6):
+            m = np.eye(4, dtype=np.float32) * 0.4
+            m[3, 3] = 1.0
+            f.shadow_view_proj[face * 16:(face + 1) * 16] = [float(v) for v in m.T.reshape(-1)]
+        f.shadow_info[:] = [2.5 * hs.radius, 1.0, 1.0 / 32.0, 0.0]
+    return f
+
+
+def shadow_atlas(f):
+    rng = np.random.default_rng(17)
+    return (rng.random(f.shadow_width * f.shadow_height) * 0.8).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", sorted(GI_CASES))
+def test_gi_temporal(dev, cornell_seq, case):
+    hs, s, seq = cornell_seq
+    f0, g, hist, out = seq[2]
+    f = case_frame(hs, f0, **GI_CASES[case])
+    say("case gi_temporal %s" % case)
+    hist = dict(hist)
+    if f.grid_table_size != s.grid_table_size:
+        grid = np.zeros(34 * f.grid_table_size, np.uint32)
+        fp = ra.Frame.from_buffer_copy(seq[1][0])
+        fp.grid_table_size = f.grid_table_size
+        orc.lib().orc_grid_insert(ctypes.byref(fp), ctypes.byref(common.host_gbuffer(seq[1][1], W, H)), orc.ptr(grid))
+        hist["grid"] = grid
+    if f.shadows_enabled:
+        hist["shadow"] = shadow_atlas(f)
+    sl = out["surface_light"] if f.lighting_surface > 0 and f.di_light_count > 0 else g["emission"]
+    want = oracle_gi_temporal(hs, f, g, hist, sl)
+    got = gpu_gi_temporal(dev, hs, f, g, hist, sl)
+    working = (want["gi_receiver"][:, 3] > 0).mean()
+    say("  W>0 fraction %.3f" % working)
+    for k in want:
+        assert_same(k, got[k], want[k], rel=1e-4)
+
+
+@pytest.mark.parametrize("case", sorted(GI_CASES))
+def test_gi_spatial(dev, cornell_seq, case):
+    hs, s, seq = cornell_seq
+    f0, g, hist, out = seq[2]
+    f = case_frame(hs, f0, **GI_CASES[case])
+    say("case gi_spatial %s" % case)
+    hist = dict(hist)
+    grid = out["grid"]
+    if f.grid_table_size != s.grid_table_size:
+        grid = np.zeros(34 * f.grid_table_size, np.uint32)
+        orc.lib().orc_grid_insert(ctypes.byref(f), ctypes.byref(common.host_gbuffer(g, W, H)), orc.ptr(grid))
+    if f.shadows_enabled:
+        hist["shadow"] = shadow_atlas(f)
+    res = {k: out[k] for k in ("gi_sample", "gi_receiver", "gi_normal")}
+    sl = out["surface_light"]
+    want = oracle_gi_spatial(hs, f, g, hist, res, grid, sl)
+    got = gpu_gi_spatial(dev, hs, f, g, hist, res, grid, sl)
+    say("  color mean %s" % halfs(want["color"])[:, :3].mean(0))
+    for k in want:
+        assert_same(k, got[k], want[k])
+
+
+def di_both(dev, hs, f, g, prev):
+    n = f.width * f.height
+    scene = orc.host_scene(hs)
+    gbh = common.host_gbuffer(g, f.width, f.height)
+    w = {"di_sample": np.zeros((n, 4), np.float32), "di_receiver": np.zeros((n, 4), np.float32),
+         "di_normal": np.zeros((n, 4), np.uint16)}
+    pr = ra.DiReservoirs(prev["di_sample"].ctypes.data, prev["di_receiver"].ctypes.data, prev["di_normal"].ctypes.data)
+    wr = ra.DiReservoirs(w["di_sample"].ctypes.data, w["di_receiver"].ctypes.data, w["di_normal"].ctypes.data)
+    orc.lib().orc_di_temporal(ctypes.byref(f), ctypes.byref(scene), ctypes.byref(gbh), ctypes.byref(pr), ctypes.byref(wr))
+    wd, ws = np.zeros((n, 4), np.uint16), np.zeros((n, 4), np.uint16)
+    orc.lib().orc_di_spatial(ctypes.byref(f), ctypes.byref(scene), ctypes.byref(gbh), ctypes.byref(wr), orc.ptr(wd), orc.ptr(ws))
+    w["diffuse"], w["specular"] = wd, ws
+
+    dscene = dev.scene(hs)
+    gb, keep = dev.gbuffer(g, f.width, f.height)
+    p = {k: dev.put(prev[k]) for k in ("di_sample", "di_receiver", "di_normal")}
+    o = {"di_sample": dev.empty(n * 16), "di_receiver": dev.empty(n * 16), "di_normal": dev.empty(n * 8)}
+    dpr = ra.DiReservoirs(p["di_sample"].data_ptr(), p["di_receiver"].data_ptr(), p["di_normal"].data_ptr())
+    dor = ra.DiReservoirs(o["di_sample"].data_ptr(), o["di_receiver"].data_ptr(), o["di_normal"].data_ptr())
+    ra.check(dev.L.rgi_di_temporal(dev.ctx, ctypes.byref(f), ctypes.byref(dscene), ctypes.byref(gb), ctypes.byref(dpr),
+                                   ctypes.byref(dor), dev.stream))
+    od, osp = dev.empty(n * 8), dev.empty(n * 8)
+    # Spatial from the oracle's temporal output, so each pass is checked on its own inputs.
+    wi = {k: dev.put(w[k]) for k in ("di_sample", "di_receiver", "di_normal")}
+    dwr = ra.DiReservoirs(wi["di_sample"].data_ptr(), wi["di_receiver"].data_ptr(), wi["di_normal"].data_ptr())
+    ra.check(dev.L.rgi_di_spatial(dev.ctx, ctypes.byref(f), ctypes.byref(dscene), ctypes.byref(gb), ctypes.byref(dwr),
+                                  od.data_ptr(), osp.data_ptr(), dev.stream))
+    got = {k: dev.get(v, w[k]) for k, v in o.items()}
+    got["diffuse"] = dev.get(od, wd)
+    got["specular"] = dev.get(osp, ws)
+    return got, w
+
+
+@pytest.mark.parametrize("frame_no,extra", [(0, 4), (2, 4), (2, 0)])
+def test_di(dev, frame_no, extra):
+    hs = common.cornell(extra_lights=extra)
+    s = ra.Settings()
+    seq = common.oracle_sequence(hs, s, W, H, frame_no + 1, yaw_step=2 * np.pi / 64)
+    f, g, hist, out = seq[frame_no]
+    say("case di frame=%d extra=%d lights=%d" % (frame_no, extra, hs.light_count))
+    got, want = di_both(dev, hs, f, g, hist)
+    for k in want:
+        assert_same(k, got[k], want[k], rel=1e-4)

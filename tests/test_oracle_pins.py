@@ -1,0 +1,544 @@
+"""Pins of the oracle: each check ties an oracle function to something it did
+not derive from itself — a published known answer, an independent library,
+a closed form, an invariant, or brute force on a small input — chosen so that
+a plausible mistake (a dropped term, a wrong sign, index or constant) fails.
+
+The reference ships no tests, golden vectors or fixtures for this path, so
+these are the pins; see DESIGN.md for what each one covers.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+
+import restir_amd as ra
+
+import common
+import orc
+
+L = orc.lib()
+
+
+def f3(*v):
+    return np.array(v, np.float32)
+
+
+# ---------------------------------------------------------------- hashes
+
+def test_checksum_hash_is_xxh32_avalanche():
+    """gridChecksumHash (hashgrid.sh:69-78) is XXH32's final avalanche:
+    XXH32(empty input, seed) = avalanche(seed + PRIME32_5), checked against
+    the xxhash library for many seeds."""
+    xxhash = pytest.importorskip("xxhash")
+    prime5 = 374761393
+    rng = np.random.default_rng(1)
+    for seed in [0, 1, 2, 0xFFFFFFFF] + rng.integers(0, 2**32, 2000).tolist():
+        assert L.orc_checksum_hash((seed + prime5) & 0xFFFFFFFF) == xxhash.xxh32_intdigest(b"", seed=int(seed))
+
+
+def test_pcg_is_a_permutation_with_avalanche():
+    """gridHash/pcg (hashgrid.sh:61-67): an LCG step followed by the RXS-M-XS
+    output permutation, both bijections on uint32. A wrong shift or constant
+    breaks injectivity or the avalanche below."""
+    rng = np.random.default_rng(2)
+    xs = np.unique(rng.integers(0, 2**32, 1 << 18, dtype=np.uint64)).astype(np.uint32)
+    ys = np.array([L.orc_pcg(int(x)) for x in xs], np.uint64)
+    assert len(np.unique(ys)) == len(xs)
+    # Consecutive inputs: a bijection has no collisions on a dense range either.
+    dense = np.array([L.orc_pcg(i) for i in range(1 << 16)], np.uint64)
+    assert len(np.unique(dense)) == 1 << 16
+    # Strict avalanche: flipping any input bit flips about half the output bits.
+    sample = xs[:512]
+    for bit in range(32):
+        flips = [bin(L.orc_pcg(int(x)) ^ L.orc_pcg(int(x) ^ (1 << bit))).count("1") for x in sample]
+        assert 12.0 < np.mean(flips) < 20.0, bit
+
+
+def test_next_random_is_exact_24_bit():
+    """nextRandom (restir.sh:131-135): (pcg(state) >> 8) / 2^24, exactly."""
+    st = ctypes.c_uint32(12345)
+    for _ in range(1000):
+        before = st.value
+        r = L.orc_next_random(ctypes.byref(st))
+        assert st.value == L.orc_pcg(before)
+        assert r * 16777216.0 == float(st.value >> 8)
+        assert 0.0 <= r < 1.0
+
+
+# ----------------------------------------------------------- math, fp16
+
+def ulp32(x):
+    x = np.float32(abs(x))
+    return float(np.spacing(x)) if x != 0 else float(np.finfo(np.float32).tiny)
+
+
+def test_sincos_against_libm():
+    """rgi_sincos (Cody-Waite + Taylor) within 2 ulp of the true values."""
+    xs = np.concatenate([np.linspace(-64, 64, 40001, dtype=np.float32),
+                         (np.float32(6.2831855) * np.linspace(0, 1, 20001, dtype=np.float32)).astype(np.float32),
+                         np.array([0.0, 1e-30, -1e-7, math.pi / 4, math.pi / 2, math.pi], np.float32)])
+    s, c = ctypes.c_float(), ctypes.c_float()
+    worst = 0.0
+    for x in xs:
+        L.orc_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+        es, ec = math.sin(float(x)), math.cos(float(x))
+        # Error relative to the ulp of the larger of the value and 2^-24 (absolute floor near zeros).
+        worst = max(worst, abs(s.value - es) / max(ulp32(es), 2**-26), abs(c.value - ec) / max(ulp32(ec), 2**-26))
+    assert worst <= 2.0, worst
+
+
+def test_half_bits_match_numpy_rne():
+    """RGBA16F stores: IEEE binary16 round to nearest even (numpy's conversion)."""
+    rng = np.random.default_rng(3)
+    vals = np.concatenate([
+        rng.standard_normal(20000).astype(np.float32) * 100,
+        (rng.random(20000) * 6e-5).astype(np.float32),             # binary16 subnormals
+        np.array([65504, 65519.99, 65520, 1e9, -1e9, 0.0, -0.0, 2**-25, 2**-24, 3 * 2**-26, np.inf, -np.inf],
+                 np.float32),
+        # exact ties between neighbouring halves
+        (np.arange(1, 2049, dtype=np.float32) + 0.5) * np.float32(2**-10),
+    ])
+    with np.errstate(over="ignore"):
+        want = vals.astype(np.float16).view(np.uint16)
+    got = np.array([L.orc_half_bits(float(v)) for v in vals], np.uint16)
+    assert np.array_equal(got, want)
+    nan = L.orc_half_bits(float("nan"))
+    assert (nan & 0x7C00) == 0x7C00 and (nan & 0x3FF) != 0
+
+
+# ----------------------------------------------------------- hash grid
+
+def frame_with(**kw):
+    f = ra.Frame()
+    f.grid_min_cell = kw.get("min_cell", 0.0198)
+    f.grid_pixel_size = kw.get("pixel_size", 0.0037)
+    f.grid_cell_pixels = 6.0
+    f.grid_table_size = kw.get("table", 65536)
+    return f
+
+
+def test_grid_cell_size_closed_form():
+    """gridCellSize (hashgrid.sh:82-87) = x * 2^floor(log2(max(d*y*z/x, 1)))."""
+    f = frame_with()
+    rng = np.random.default_rng(4)
+    cam = f3(0, 0, 0)
+    for d in np.concatenate([rng.random(3000) * 50, [0.0, 1e-3, 0.5, 1.0, 1.1]]):
+        p = f3(d, 0, 0)
+        got = L.orc_grid_cell_size(ctypes.byref(f), orc.ptr(p), orc.ptr(cam))
+        target = float(np.float32(np.float32(np.float32(d) * np.float32(f.grid_pixel_size)) * np.float32(6.0)))
+        m = max(float(np.float32(target / np.float32(f.grid_min_cell))), 1.0)
+        lvl = math.floor(math.log2(m))
+        if abs(m / 2**round(math.log2(m)) - 1) < 1e-6:
+            continue  # exactly at a power of two, where log2 rounding decides
+        assert got == np.float32(f.grid_min_cell) * np.float32(2.0**lvl)
+
+
+def test_grid_normal_bucket_table():
+    """gridNormalBucket (hashgrid.sh:89-103): dominant axis and sign; ties
+    go to x, then y."""
+    cases = {(1, 0, 0): 0, (-1, 0, 0): 1, (0, 1, 0): 2, (0, -1, 0): 3, (0, 0, 1): 4, (0, 0, -1): 5,
+             (0.5, 0.5, 0.5): 0, (-0.5, 0.5, 0.5): 1, (0, 0.5, 0.5): 2, (0, -0.5, 0.5): 3, (0.1, 0.2, -0.9): 5}
+    for n, want in cases.items():
+        v = f3(*n)
+        assert L.orc_grid_normal_bucket(orc.ptr(v)) == want, n
+
+
+def test_grid_key_cell_and_range():
+    """gridKey (hashgrid.sh:116-127): one key per (cell, size, bucket): points
+    of a cell share it, the next cell differs; slot < T; checksum never 0."""
+    f = frame_with(table=1024)
+    rng = np.random.default_rng(5)
+    slot, cs = ctypes.c_uint32(), ctypes.c_uint32()
+    n = f3(0, 1, 0)
+    seen = {}
+    for _ in range(2000):
+        size = np.float32(0.0198 * 2 ** rng.integers(0, 6))
+        cell = rng.integers(-50, 50, 3)
+        inside = (cell + 0.05 + 0.9 * rng.random(3)) * float(size)
+        inside = inside.astype(np.float32)
+        L.orc_grid_key(ctypes.byref(f), orc.ptr(inside), orc.ptr(n), float(size),
+                       ctypes.byref(slot), ctypes.byref(cs))
+        assert slot.value < 1024 and cs.value != 0
+        key = (tuple(cell), float(size))
+        if key in seen:
+            assert seen[key] == (slot.value, cs.value)
+        seen[key] = (slot.value, cs.value)
+    # Distinct cells give distinct checksums (32-bit, few collisions expected).
+    sums = [v[1] for v in seen.values()]
+    assert len(set(sums)) >= len(sums) - 1
+
+
+def brute_insert(slot0, checksum, table):
+    """Serial simulation of the deterministic linearization: probe rounds,
+    ascending pixels within a round, then adds in pixel order."""
+    grid = np.zeros(34 * table, np.uint32)
+    resolved = {}
+    pending = [i for i in range(len(checksum)) if checksum[i] != 0]
+    for probe in range(8):
+        nxt = []
+        for i in pending:
+            s = (int(slot0[i]) + probe) & (table - 1)
+            if grid[s] == 0:
+                grid[s] = checksum[i]
+                resolved[i] = s
+            elif grid[s] == checksum[i]:
+                resolved[i] = s
+            else:
+                nxt.append(i)
+        pending = nxt
+    for i in sorted(resolved):
+        s = resolved[i]
+        k = grid[table + s]
+        grid[table + s] += 1
+        if k < 32:
+            grid[2 * table + s * 32 + k] = i
+    return grid, resolved
+
+
+@pytest.fixture(scope="module")
+def small_seq():
+    hs = common.cornell(extra_lights=4)
+    seq = common.oracle_sequence(hs, ra.Settings(), 64, 48, 2, yaw_step=0.2)
+    return hs, seq
+
+
+@pytest.mark.parametrize("table", [65536, 256, 64])
+def test_grid_insert_matches_serial_linearization(small_seq, table):
+    hs, seq = small_seq
+    f, g, hist, out = seq[1]
+    f = ra.Frame.from_buffer_copy(f)
+    f.grid_table_size = table
+    gb = common.host_gbuffer(g, 64, 48)
+    n = 64 * 48
+    slot0, cs = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    L.orc_grid_keys(ctypes.byref(f), ctypes.byref(gb), orc.ptr(slot0), orc.ptr(cs), None)
+    assert np.all(slot0 < table)
+    want, resolved = brute_insert(slot0, cs, table)
+    got = np.zeros(34 * table, np.uint32)
+    L.orc_grid_insert(ctypes.byref(f), ctypes.byref(gb), orc.ptr(got))
+    assert np.array_equal(got, want)
+    # Every resolved pixel is found again by a probe walk from its own key.
+    for i, s in list(resolved.items())[:500]:
+        found = None
+        for probe in range(8):
+            t = (int(slot0[i]) + probe) & (table - 1)
+            if got[t] == cs[i]:
+                found = t
+                break
+            if got[t] == 0:
+                break
+        assert found == s
+    if table == 64:
+        assert len(resolved) < int((cs != 0).sum())  # the tiny table drops pixels
+
+
+def test_grid_pick_entry_reads_listed_pixels(small_seq):
+    """gridPickEntry (hashgrid.sh:155-177): a pick is a listed pixel of the
+    jittered key's cell, chosen by r2 over min(count, 32)."""
+    hs, seq = small_seq
+    f, g, hist, out = seq[1]
+    grid = out["grid"]
+    T = f.grid_table_size
+    cam = np.array(list(f.camera_position)[:3], np.float32)
+    rng = np.random.default_rng(6)
+    hits = 0
+    for i in rng.integers(0, 64 * 48, 400):
+        if g["depth"][i] >= 0.99999:
+            continue
+        p = out["gi_receiver"][i, :3].copy()
+        nrm = out["gi_normal"][i, :3].copy()
+        r = rng.random(3).astype(np.float32)
+        e = L.orc_grid_pick_entry(ctypes.byref(f), orc.ptr(grid), orc.ptr(cam), orc.ptr(p), orc.ptr(nrm),
+                                  float(r[0]), float(r[1]), float(r[2]))
+        if e == 0xFFFFFFFF:
+            continue
+        hits += 1
+        slots = np.nonzero(grid[:T])[0]
+        listed = {int(v) for s in slots for v in grid[2 * T + s * 32: 2 * T + s * 32 + min(int(grid[T + s]), 32)]}
+        assert e in listed
+    assert hits > 50
+
+
+# ------------------------------------------------------------------ BVH
+
+@pytest.mark.parametrize("kind,size", [(ra.RGI_SCENE_CORNELL, 0), (ra.RGI_SCENE_BOXFIELD, 300),
+                                       (ra.RGI_SCENE_ATRIUM, 20000)])
+def test_bvh_layout_invariants(kind, size):
+    """The bvh.h:9-22 layout: DFS order, left child next, right child and skip
+    link consistent, child bounds inside the parent, every triangle in exactly
+    one leaf, records carrying the mesh's triangles."""
+    hs = ra.HostScene(kind, seed=2, size=size)
+    nodes = hs.bvh[: hs.node_count * 8].reshape(-1, 8)
+    seen = np.zeros(hs.triangle_count, np.int32)
+
+    def walk(i, lo, hi):
+        n = nodes[i]
+        assert np.all(n[0:3] >= lo - 1e-6) and np.all(n[4:7] <= hi + 1e-6)
+        if n[7] >= 0:
+            a, c = int(n[3]), int(n[7])
+            assert c >= 1 and (c <= 16 or hs.bvh_depth >= 40)
+            seen[a:a + c] += 1
+            return i + 1
+        right = walk(i + 1, n[0:3], n[4:7])
+        assert right == int(n[3])
+        end = walk(right, n[0:3], n[4:7])
+        assert end == int(-n[7]) - 1
+        return end
+
+    assert walk(0, nodes[0, 0:3], nodes[0, 4:7]) == hs.node_count
+    assert np.all(seen == 1)
+    rec = hs.bvh[hs.node_count * 8:].reshape(-1, 16)
+    mine = {tuple(np.round(r, 6)) for r in rec[:, [0, 1, 2, 4, 5, 6, 8, 9, 10]]}
+    theirs = {tuple(np.round(p, 6)) for p in hs.positions}
+    assert mine == theirs
+
+
+def brute_closest(hs, o, d, tmax):
+    p = hs.bvh[hs.node_count * 8:].reshape(-1, 16).astype(np.float64)
+    v0, v1, v2 = p[:, 0:3], p[:, 4:7], p[:, 8:11]
+    e1, e2 = v1 - v0, v2 - v0
+    pv = np.cross(d, e2)
+    det = (e1 * pv).sum(1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / det
+        s = o - v0
+        u = (s * pv).sum(1) * inv
+        q = np.cross(s, e1)
+        v = (q * d).sum(1) * inv
+        t = (e2 * q).sum(1) * inv
+    ok = (np.abs(det) > 1e-12) & (u >= 0) & (v >= 0) & (u + v <= 1) & (t > 0) & (t < tmax)
+    return np.where(ok, t, np.inf)
+
+
+@pytest.mark.parametrize("kind,size", [(ra.RGI_SCENE_CORNELL, 0), (ra.RGI_SCENE_BOXFIELD, 60)])
+def test_trace_against_brute_force(kind, size):
+    """traceBvh (bvh.sh:140-319): closest hit = the nearest triangle over all
+    triangles (ties: any of the equally near ones); any hit = whether one
+    exists."""
+    hs = ra.HostScene(kind, seed=4, size=size)
+    rng = np.random.default_rng(7)
+    n = 3000
+    o = (hs.center + (rng.random((n, 3)) - 0.5) * 1.6 * hs.radius).astype(np.float32)
+    d = rng.standard_normal((n, 3))
+    d = (d / np.linalg.norm(d, axis=1)[:, None]).astype(np.float32)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 4:7], rays[:, 3] = o, d, 3.0 * hs.radius
+    scene = orc.host_scene(hs)
+    for any_hit in (0, 1):
+        tri, dist = np.zeros(n, np.int32), np.zeros(n, np.float32)
+        L.orc_trace(ctypes.byref(scene), orc.ptr(rays), n, orc.ptr(tri), orc.ptr(dist), any_hit)
+        agree = 0
+        for i in range(n):
+            t = brute_closest(hs, o[i].astype(np.float64), d[i].astype(np.float64), 3.0 * hs.radius)
+            tmin = t.min()
+            if not np.isfinite(tmin):
+                agree += tri[i] < 0
+                continue
+            if any_hit:
+                agree += tri[i] >= 0
+                continue
+            if tri[i] >= 0 and abs(dist[i] - tmin) <= 1e-4 * max(tmin, 1.0) and t[tri[i]] <= tmin * (1 + 1e-5) + 1e-6:
+                agree += 1
+        # Grazing rays at triangle edges may differ between float and double.
+        assert agree >= n - 3, (any_hit, n - agree)
+
+
+# ---------------------------------------------------------------- lights
+
+def test_light_list_cdf_and_power():
+    """buildLightList (lights.cpp:86-165): pmf proportional to power
+    (pi * luminance * area for triangles, 4 pi * luminance for points), CDF
+    monotone, last entry exactly 1, triangle rows pointing at emissive BVH
+    records."""
+    hs = common.cornell(extra_lights=4)
+    assert hs.light_count == 6
+    lights = hs.lights.reshape(-1, 4, 4).astype(np.float64)
+    lum = lambda c: 0.2126 * c[0] + 0.7152 * c[1] + 0.0722 * c[2]
+    power = []
+    for l in lights:
+        if l[0, 3] > 0.5:
+            power.append(4 * math.pi * lum(l[3, :3]))
+        else:
+            area = 0.5 * np.linalg.norm(np.cross(l[1, :3] - l[0, :3], l[2, :3] - l[0, :3]))
+            power.append(math.pi * lum(l[3, :3]) * area)
+            rec = hs.bvh[hs.node_count * 8 + int(l[1, 3]) * 16:][:16]
+            assert rec[7] == l[3, 0] and rec[11] == l[3, 1] and rec[15] == l[3, 2]
+    power = np.array(power)
+    pmf = lights[:, 3, 3]
+    assert np.allclose(pmf, power / power.sum(), rtol=1e-5)
+    cdf = lights[:, 2, 3]
+    assert np.all(np.diff(cdf) >= 0) and cdf[-1] == 1.0
+    assert np.allclose(cdf, np.cumsum(pmf), atol=1e-6)
+
+
+# ------------------------------------------------------------------ BRDF
+
+def brdf(n, v, l, albedo, rough, metal):
+    d, s = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    args = [f3(*n), f3(*v), f3(*l), f3(*albedo)]  # kept alive across the call
+    L.orc_brdf(orc.ptr(args[0]), orc.ptr(args[1]), orc.ptr(args[2]), orc.ptr(args[3]), rough, metal, orc.ptr(d),
+               orc.ptr(s))
+    return d, s
+
+
+def test_brdf_closed_forms():
+    """evaluateBrdf (brdf.sh:38-62) at normal incidence: F = f0, D = 1/(pi a^2),
+    G = (1/(1-k+k))^2 = 1, specular = D F / 4, diffuse = (1-F)(1-m) albedo / pi."""
+    for rough in (0.3, 0.6, 1.0):
+        a = rough * rough
+        d, s = brdf((0, 0, 1), (0, 0, 1), (0, 0, 1), (0.5, 0.25, 0.75), rough, 0.0)
+        D = 1.0 / (math.pi * a * a)
+        assert np.allclose(s, D * 0.04 / 4.0, rtol=1e-5)
+        assert np.allclose(d, (1 - 0.04) * np.array([0.5, 0.25, 0.75]) / math.pi, rtol=1e-5)
+    # Metals: f0 = albedo, no diffuse.
+    d, s = brdf((0, 0, 1), (0, 0, 1), (0, 0, 1), (0.9, 0.6, 0.3), 0.5, 1.0)
+    assert np.all(d == 0) and np.allclose(s, np.array([0.9, 0.6, 0.3]) / (math.pi * 0.0625) / 4, rtol=1e-5)
+    # Light below the surface: nothing.
+    d, s = brdf((0, 0, 1), (0, 0, 1), (0, 0.6, -0.8), (0.5, 0.5, 0.5), 0.5, 0.0)
+    assert np.all(d == 0) and np.all(s == 0)
+
+
+def test_ggx_normalised():
+    """The GGX distribution integrates to 1 over projected half vectors:
+    the specular with F0 = 1 and no masking is checked through D alone by
+    integrating D(h) cos(h) over the hemisphere numerically (inverted from
+    the specular term at v = n, where G = Smith(n.l) Smith(1))."""
+    rough = 0.5
+    a2 = (rough * rough) ** 2
+    th = np.linspace(0, math.pi / 2, 20001)
+    D = a2 / (math.pi * ((np.cos(th) ** 2) * (a2 - 1) + 1) ** 2)
+    integral = np.trapezoid(D * np.cos(th) * np.sin(th) * 2 * math.pi, th)
+    assert abs(integral - 1.0) < 1e-4
+    # And the oracle's D matches this D where masking is 1 (l = v = n, h = n).
+    d, s = brdf((0, 0, 1), (0, 0, 1), (0, 0, 1), (1, 1, 1), rough, 1.0)
+    assert abs(s[0] - D[0] / 4.0) < 1e-5 * D[0]
+
+
+# ------------------------------------------------------------ sampling
+
+def test_cosine_hemisphere_distribution():
+    """cosineHemisphere (restir.sh:201-212): unit directions in the normal's
+    hemisphere with density cos/pi, i.e. cos^2 uniform on [0, 1]."""
+    rng = np.random.default_rng(8)
+    for n in (f3(0, 1, 0), f3(0.6, 0.0, 0.8), f3(0.0, -0.995, 0.0998749)):
+        n = (n / np.linalg.norm(n)).astype(np.float32)
+        out = np.zeros(3, np.float32)
+        c2 = []
+        for r0, r1 in rng.random((20000, 2)).astype(np.float32):
+            L.orc_cosine_hemisphere(orc.ptr(n), float(r0), float(r1), orc.ptr(out))
+            assert abs(np.linalg.norm(out) - 1) < 1e-5
+            c = float(np.dot(out, n))
+            assert c >= -1e-6
+            c2.append(c * c)
+        c2 = np.sort(np.array(c2))
+        ks = np.max(np.abs(c2 - np.arange(1, len(c2) + 1) / len(c2)))
+        assert ks < 0.015
+        assert abs(np.mean(np.sqrt(c2)) - 2.0 / 3.0) < 0.01
+
+
+# ------------------------------------------------------------ passes
+
+def test_di_single_candidate_weight_is_inverse_pdf():
+    """With one candidate, no history and no taps, RIS keeps W = 1 / pdf of
+    the drawn sample (pmf of its light times 1/area), or 0 when shadowed or
+    unlit (cs_di_temporal.sc:52-70, 143-151)."""
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings(di_candidates=1)
+    f = common.frame_for(hs, s, 64, 48, 0, False)
+    g = common.oracle_gbuffer(hs, f)
+    hist = common.empty_history(64, 48, s.grid_table_size)
+    out = common.oracle_frame(hs, f, g, hist)
+    lights = hs.lights.reshape(-1, 4, 4).astype(np.float64)
+    checked = 0
+    for i in range(64 * 48):
+        light = int(out["di_sample"][i, 3])
+        W = out["di_receiver"][i, 3]
+        if light < 0 or W == 0:
+            continue
+        l = lights[light]
+        pdf = l[3, 3] if l[0, 3] > 0.5 else l[3, 3] / (0.5 * np.linalg.norm(np.cross(l[1, :3] - l[0, :3], l[2, :3] - l[0, :3])))
+        assert abs(W * pdf - 1.0) < 1e-5
+        checked += 1
+    assert checked > 100
+
+
+def test_temporal_without_taps_is_plain_ris():
+    """Pairwise MIS with no neighbours hands the canonical reservoir the whole
+    weight (restir.sh:822-823): results are independent of the history."""
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings(temporal_taps=0)
+    seq = common.oracle_sequence(hs, s, 48, 32, 2, yaw_step=0.1)
+    f, g, hist, out = seq[1]
+    empty = common.empty_history(48, 32, s.grid_table_size)
+    f0 = ra.Frame.from_buffer_copy(f)
+    f0.history_valid = 0
+    f0.multiple_bounces = 0
+    f0.di_history_valid = 0
+    f1 = ra.Frame.from_buffer_copy(f0)
+    o0 = common.oracle_frame(hs, f0, g, empty)
+    o1 = common.oracle_frame(hs, f1, g, hist)
+    for k in ("gi_sample", "gi_receiver", "gi_normal"):
+        assert np.array_equal(o0[k], o1[k])
+
+
+def test_oracle_is_thread_count_independent():
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings()
+    outs = []
+    for threads in (1, 4):
+        L.orc_set_threads(threads)
+        seq = common.oracle_sequence(hs, s, 48, 40, 2, yaw_step=0.1)
+        outs.append(seq[1][3])
+    L.orc_set_threads(0)
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
+# ------------------------------------------------------------ C ABI
+
+def declared_functions():
+    import re
+    names = []
+    for h in ("restir_amd.h", "restir_amd_scene.h"):
+        text = open(os.path.join(orc.ROOT, "include", h)).read()
+        names += re.findall(r"^\w[\w\s\*]*?\b(rgi_\w+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 19
+    lib = ctypes.CDLL(ra.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_abi_struct_sizes_match():
+    lib = ra.lib()
+    for i, cls in enumerate(ra.ABI_STRUCTS):
+        assert lib.rgi_abi_sizeof(i) == ctypes.sizeof(cls)
+
+
+def test_invalid_arguments_are_rejected_without_a_gpu():
+    lib = ra.lib()
+    assert lib.rgi_grid_clear(None, 1024, None) == -1
+    assert b"NULL" in lib.rgi_last_error()
+    f = ra.Frame()
+    f.grid_table_size = 1000
+    g = ra.GBuffer()
+    assert lib.rgi_grid_insert(None, ctypes.byref(f), ctypes.byref(g), None, 0, None) == -1
+    assert b"power of two" in lib.rgi_last_error()
+    f.grid_table_size = 1024
+    f.ray_mode = 7
+    assert lib.rgi_grid_insert(None, ctypes.byref(f), ctypes.byref(g), None, 0, None) == -1
+
+
+def test_cornell_scene_framing():
+    hs = common.cornell()
+    assert hs.triangle_count == 36
+    assert abs(hs.radius - math.sqrt(3)) < 1e-5
+    assert abs(hs.scene_scale - math.sqrt(3) / 1.75) < 1e-6
+    em = hs.emission.sum(1) > 0
+    assert em.sum() == 2 and np.all(hs.emission[em] == 17.0)
