@@ -94,6 +94,37 @@ def cpu_baseline(hs, settings, width, height, table):
             "sample": "oracle, one steady-state %dx%d frame (frame 3 of the orbit), all passes" % (width, height)}
 
 
+def measure_mse(L, rend, gscene, gbuf, view, W, H, ref_frames, frames, ctx, s):
+    """MSE of the plain path against a `ref_frames`-frame reference at one
+    static camera (frame 0 of the orbit): the reference renders with every
+    biased shortcut off (renderer.cpp:437-452) and accumulates in FP64
+    (fs_reference.sc); the plain path then renders `frames` frames from an
+    empty history and its last image is measured (cs_error.sc)."""
+    import torch
+    n = W * H
+    acc = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    out = ra.RendererOutputs()
+    ra.check(L.rgi_renderer_set_reference_mode(rend, 1))
+    for _ in range(ref_frames):
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+        ra.check(L.rgi_reference_accumulate(ctypes.byref(gbuf.gb), out.color, out.di_diffuse, out.di_specular,
+                                            acc.data_ptr(), s))
+    ra.check(L.rgi_renderer_set_reference_mode(rend, 0))
+    L.rgi_renderer_invalidate_history(rend)
+    for _ in range(frames):
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+    ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+    res = torch.zeros(5, dtype=torch.float64, device="cuda")
+    ra.check(L.rgi_reference_error(ctx, ctypes.byref(gbuf.gb), out.color, out.di_diffuse, out.di_specular,
+                                   acc.data_ptr(), ref_frames, res.data_ptr(), s))
+    r = res.cpu().numpy()
+    return {"relative_mse": float(r[0]), "mse_rgb": float(r[1]), "mean_luminance": float(r[2]),
+            "reference_mean_luminance": float(r[3]), "pixels": int(r[4]), "reference_frames": ref_frames,
+            "frames": frames, "method": "plain ReSTIR (grid reuse, clamps on)",
+            "image": "GI x diffuse albedo + DI diffuse + DI specular, static camera"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +137,9 @@ def main():
     ap.add_argument("--rays", type=int, default=1 << 24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--grid-mode", choices=["atomic", "deterministic"], default="atomic")
+    ap.add_argument("--reference-frames", type=int, default=4096)
+    ap.add_argument("--mse-frames", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true", help="run one 256 MiB device copy first")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -114,22 +148,51 @@ def main():
 
     import torch
     torch.cuda.init()
+    if args.calibrate_copy:
+        # A plain device copy of known size, for scaling HBM counters.
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     W, H, T = args.width, args.height, args.table
     hs = ra.HostScene(ra.RGI_SCENE_CORNELL, seed=1, extra_lights=4, light_seed=1)
-    settings = ra.Settings(grid_table_size=T)
-    gscene = driver.GpuScene(hs)
     mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
+    settings = ra.Settings(grid_table_size=T, grid_mode=mode)
+    gscene = driver.GpuScene(hs)
     drv = driver.Driver(W, H, T, mode)
+    L = ra.lib()
+    s = driver.stream()
 
     # The orbit's cameras and G-buffers, produced once, resident in HBM.
     yaw_step = 2 * math.pi / 64
     cams = [ra.orbit_camera(hs, i, W, H, yaw_step=yaw_step) for i in range(args.frames)]
+    views = [ra.make_view(*c) for c in cams]
     gbufs = []
     for i in range(args.frames):
         eye, vp, inv = cams[i]
         f = ra.make_frame(settings, hs, i, eye, vp, inv, eye, vp, W, H, False)
         gbufs.append(driver.GBufferSet(drv.ctx, gscene, f))
     torch.cuda.synchronize()
+
+    # The timed path: the C++ frame driver, one call per frame.
+    rend = ctypes.c_void_p()
+    ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                   ctypes.byref(gscene.scene), hs.radius))
+
+    def render(k):
+        i = k % args.frames
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[i]), ctypes.byref(gbufs[i].gb), s))
+
+    for k in range(args.warmup):
+        render(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        render(k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_frame = elapsed * 1e3 / args.steps
 
     def frame_for_step(step):
         i = step % args.frames
@@ -138,19 +201,11 @@ def main():
         peye, pvp, _ = cams[p]
         return ra.make_frame(settings, hs, step, eye, vp, inv, peye, pvp, W, H, step > 0, step > 0), i
 
-    frames = [frame_for_step(k) for k in range(args.warmup + args.steps)]
-
+    frames = [frame_for_step(k) for k in range(args.warmup + min(args.steps, 32))]
     for k in range(args.warmup):
         f, i = frames[k]
         drv.render(f, gscene, gbufs[i])
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
-        f, i = frames[k]
-        drv.render(f, gscene, gbufs[i])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_frame = elapsed * 1e3 / args.steps
 
     # Per-pass durations with HIP events on the launch stream.
     events = {}
@@ -162,8 +217,7 @@ def main():
 
     timed = min(args.steps, 32)
     for k in range(timed):
-        f, i = frames[(args.warmup + k) % len(frames)]
-        f = ra.Frame.from_buffer_copy(f)
+        f, i = frames[args.warmup + k]
         drv.render(f, gscene, gbufs[i], hook)
     torch.cuda.synchronize()
     pass_ms = {}
@@ -175,9 +229,6 @@ def main():
     rays_t = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()
     hit = torch.empty(args.rays * 4, dtype=torch.uint8, device="cuda")
     dist = torch.empty(args.rays * 4, dtype=torch.uint8, device="cuda")
-    L = ra.lib()
-    s = driver.stream()
-
     def trace():
         ra.check(L.rgi_trace(drv.ctx, ctypes.byref(gscene.scene), rays_t.data_ptr(), args.rays, hit.data_ptr(),
                              dist.data_ptr(), 0, s))
@@ -202,10 +253,15 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "algorithmic_bytes": traffic, "avg_ms": round(pass_ms[dominant], 5)}
 
+    mse = None
+    if args.reference_frames > 0:
+        mse = measure_mse(L, rend, gscene, gbufs[0], views[0], W, H, args.reference_frames, args.mse_frames, drv.ctx, s)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(hs, settings, W, H, T)
 
+    L.rgi_renderer_destroy(rend)
     drv.close()
     line = {
         "metric": "GI frame ms @1080p (grid + resample + trace)",
@@ -227,7 +283,7 @@ def main():
         "pass_ms": {k: round(v, 5) for k, v in sorted(pass_ms.items())},
         "mrays_per_s": round(mrays, 2),
         "trace": {"rays": args.rays, "ms": round(trace_ms, 4), "hit_rate": round(hit_rate, 4), "mode": "closest"},
-        "mse_vs_reference": None,
+        "mse_vs_reference": mse,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

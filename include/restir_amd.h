@@ -239,6 +239,29 @@ int rgi_gbuffer_primary(rgi_context* context, const rgi_frame* frame, const rgi_
                         uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal,
                         uint16_t* emission, void* stream);
 
+/* The image the error metric compares, per pixel (linear RGB): the GI
+ * estimate re-modulated by the diffuse albedo, color.rgb * albedo.rgb *
+ * (1 - metallic), plus the DI diffuse and specular radiance when those planes
+ * are given (the light the reference composites, fs_composite.sc). Pixels
+ * whose depth is the sky are not part of the image. */
+
+/* fs_reference.sc's running mean, in FP64: accum[3 * i + c] += image_c(i) for
+ * every pixel of a width x height frame. Divide by the frame count to read the
+ * reference. NaN contributions are skipped, as fs_reference.sc:21-23 skips
+ * them. */
+int rgi_reference_accumulate(const rgi_gbuffer* gbuffer, const uint16_t* color, const uint16_t* di_diffuse,
+                             const uint16_t* di_specular, double* accum, void* stream);
+
+/* cs_error.sc's measurement against accum / frames, over geometry pixels, in
+ * FP64 with a fixed reduction order: out[0] relative MSE in luminance,
+ * mean((L - L_ref)^2 / (L_ref^2 + 0.01)) (cs_error.sc:59-63); out[1] MSE in
+ * linear RGB (mean over pixels and channels); out[2] mean image luminance;
+ * out[3] mean reference luminance; out[4] pixels compared. `out` is a device
+ * buffer of 5 doubles. */
+int rgi_reference_error(rgi_context* context, const rgi_gbuffer* gbuffer, const uint16_t* color,
+                        const uint16_t* di_diffuse, const uint16_t* di_specular, const double* accum,
+                        uint32_t frames, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

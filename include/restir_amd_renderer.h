@@ -1,0 +1,116 @@
+/*
+ * restir_amd frame driver — C ABI.
+ *
+ * The counterpart of the reference's Renderer::render for the hot path
+ * (src/renderer.cpp:435-531, 614-833, 1119-1142): it owns the ping-pong
+ * buffers (GI reservoirs, accumulated colour and moments, hash grids, DI
+ * reservoirs), packs one rgi_frame per frame from the settings and the
+ * camera, and runs DI temporal -> DI spatial -> grid clear -> grid insert ->
+ * GI temporal -> GI spatial on one stream. It reproduces the reference's
+ * parity swaps, the previous camera for temporal grid lookups and this
+ * camera for insert and spatial reuse, the history and DI validity flags,
+ * and the effective-settings override while a reference accumulates.
+ */
+#ifndef RESTIR_AMD_RENDERER_H
+#define RESTIR_AMD_RENDERER_H
+
+#include <stdint.h>
+
+#include "restir_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The hot path's part of the reference's Settings (src/settings.h:183-227);
+ * rgi_settings_default fills the reference defaults. */
+typedef struct rgi_settings {
+    int32_t initial_candidates;   /* 2 */
+    int32_t temporal_taps;        /* 2 */
+    int32_t spatial_taps;         /* 4 */
+    int32_t trace_steps;          /* 32 */
+    int32_t di_candidates;        /* 8 */
+    int32_t di_spatial_taps;      /* 3 */
+    int32_t ray_mode;             /* RGI_RAY_HYBRID */
+    int32_t world_space_reuse;    /* 1 */
+    int32_t multiple_bounces;     /* 1 */
+    int32_t clamp_jacobian;       /* 1 */
+    int32_t clamp_outliers;       /* 1 */
+    int32_t boiling_filter;       /* 1 */
+    int32_t restir_di;            /* 1: DirectLighting::ReStirDi, 0: GiOnly */
+    int32_t emissive_lighting;    /* 1 */
+    int32_t sky;                  /* 0 */
+    int32_t shadows;              /* 0 (no shadow atlas producer in this build) */
+    int32_t grid_mode;            /* RGI_GRID_ATOMIC */
+    uint32_t grid_table_size;     /* 65536 */
+    float sky_zenith[3];
+    float sky_horizon[3];
+    float sky_ground[3];
+    float sky_intensity;          /* 1 */
+    float light_position[3];      /* main point light */
+    float light_color[3];         /* 0: off */
+    int32_t origin_bottom_left;   /* 0 */
+    int32_t homogeneous_depth;    /* 0 */
+    int32_t reserved[4];
+} rgi_settings;
+
+/* One frame's camera: eye and (jittered) view-projection and its inverse. */
+typedef struct rgi_view {
+    float eye[4];
+    float view_proj[16];
+    float inv_view_proj[16];
+} rgi_view;
+
+/* Device buffers of the most recent frame, for the caller to read. */
+typedef struct rgi_renderer_outputs {
+    const float* gi_sample;       /* RGBA32F reservoirs */
+    const float* gi_receiver;
+    const float* gi_normal;
+    const uint16_t* color;        /* RGBA16F accumulated demodulated GI, history length */
+    const uint16_t* moments;      /* RGBA16F luminance moments */
+    const uint32_t* grid;         /* this frame's hash grid */
+    const uint16_t* di_diffuse;   /* RGBA16F (NULL when DI is off) */
+    const uint16_t* di_specular;
+} rgi_renderer_outputs;
+
+typedef struct rgi_renderer rgi_renderer;
+
+void rgi_settings_default(rgi_settings* out);
+uint32_t rgi_settings_sizeof(void);
+
+/* Allocates every per-frame buffer for a width x height frame (GI at the same
+ * size). `scene_radius` is describeScene's radius (scene_scale = radius /
+ * 1.75, renderer.cpp:52, 501). */
+int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const rgi_settings* settings,
+                        const rgi_scene* scene, float scene_radius);
+void rgi_renderer_destroy(rgi_renderer* renderer);
+
+/* Renders one frame of the hot path on `stream` from a G-buffer of the
+ * renderer's size (device planes, including shading_normal and emission). */
+int rgi_renderer_render(rgi_renderer* renderer, const rgi_view* view, const rgi_gbuffer* gbuffer, void* stream);
+
+/* setReferenceAccumulation (renderer.cpp:1127-1142): while on, reuse,
+ * clamps and DI reuse are off, and history restarts. */
+int rgi_renderer_set_reference_mode(rgi_renderer* renderer, int on);
+
+/* Drops the history (scene, size or settings change: renderer.cpp:416-418, 1448-1451). */
+void rgi_renderer_invalidate_history(rgi_renderer* renderer);
+
+int rgi_renderer_outputs_get(const rgi_renderer* renderer, rgi_renderer_outputs* out);
+
+/* Packs the uniforms of one frame as Renderer::render does
+ * (renderer.cpp:497-531, 614-621, 632-640, 693-731). `scene` supplies the
+ * light count; `effective` settings are used as given (apply the reference
+ * override before calling). */
+int rgi_frame_pack(const rgi_settings* effective, const rgi_scene* scene, float scene_radius, uint32_t frame_number,
+                   const rgi_view* view, const rgi_view* previous_view, int32_t width, int32_t height,
+                   int history_valid, int di_history_valid, rgi_frame* out);
+
+/* The rgi_frame the last render used (for tests and tools). */
+int rgi_renderer_last_frame(const rgi_renderer* renderer, rgi_frame* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RESTIR_AMD_RENDERER_H */

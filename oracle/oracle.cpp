@@ -1773,6 +1773,86 @@ void orc_di_spatial(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g
     });
 }
 
+// The displayed image of a pixel (GI re-modulated by the diffuse albedo, plus
+// DI), false over the sky (fs_composite.sc's light, cs_error.sc's mask).
+static bool reference_image(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                            size_t i, V3& L)
+{
+    if (g->depth[i] >= kSkyDepth) return false;
+    V4 c = half4(color + i * 4);
+    V4 a = unorm8(g->albedo[i]);
+    float metallic = unorm8(g->normal[i]).w;
+    L = V3{c.x, c.y, c.z} * (V3{a.x, a.y, a.z} * (1.0f - metallic));
+    if (dd) L = L + xyz(half4(dd + i * 4));
+    if (ds) L = L + xyz(half4(ds + i * 4));
+    return true;
+}
+
+// fs_reference.sc:18-25: the running mean, here as an FP64 running sum.
+void orc_reference_accumulate(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                              double* accum)
+{
+    size_t n = (size_t)g->width * g->height;
+    for (size_t i = 0; i < n; ++i) {
+        V3 L;
+        if (!reference_image(g, color, dd, ds, i, L)) continue;
+        if (L.x != L.x || L.y != L.y || L.z != L.z) continue;
+        accum[3 * i + 0] += (double)L.x;
+        accum[3 * i + 1] += (double)L.y;
+        accum[3 * i + 2] += (double)L.z;
+    }
+}
+
+static double lum64(double r, double g, double b) { return (0.2126 * r + 0.7152 * g) + 0.0722 * b; }
+
+// cs_error.sc:35-92 in FP64: 256-pixel blocks reduced by a halving tree, block
+// partials summed with stride 256 per lane, then the same tree.
+void orc_reference_error(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                         const double* accum, uint32_t frames, double* out)
+{
+    const int B = 256, K = 5;
+    size_t n = (size_t)g->width * g->height;
+    size_t blocks = (n + B - 1) / B;
+    double inv = 1.0 / (double)frames;
+    std::vector<double> partial(blocks * K);
+    auto tree = [&](std::vector<double>& s) {
+        for (int stride = B / 2; stride > 0; stride >>= 1)
+            for (int t = 0; t < stride; ++t)
+                for (int k = 0; k < K; ++k) s[k * B + t] = s[k * B + t] + s[k * B + t + stride];
+    };
+    for (size_t b = 0; b < blocks; ++b) {
+        std::vector<double> s(K * B, 0.0);
+        for (int t = 0; t < B; ++t) {
+            size_t i = b * B + t;
+            V3 L;
+            if (i >= n || !reference_image(g, color, dd, ds, i, L)) continue;
+            double rr = accum[3 * i] * inv, rg = accum[3 * i + 1] * inv, rb = accum[3 * i + 2] * inv;
+            double lr = L.x, lg = L.y, lb = L.z;
+            double li = lum64(lr, lg, lb), lref = lum64(rr, rg, rb);
+            double d = li - lref;
+            s[0 * B + t] = (d * d) / (lref * lref + 0.01);
+            s[1 * B + t] = ((lr - rr) * (lr - rr) + (lg - rg) * (lg - rg)) + (lb - rb) * (lb - rb);
+            s[2 * B + t] = li;
+            s[3 * B + t] = lref;
+            s[4 * B + t] = 1.0;
+        }
+        tree(s);
+        for (int k = 0; k < K; ++k) partial[b * K + k] = s[k * B];
+    }
+    std::vector<double> s(K * B, 0.0);
+    for (int t = 0; t < B; ++t)
+        for (size_t b = (size_t)t; b < blocks; b += B)
+            for (int k = 0; k < K; ++k) s[k * B + t] = s[k * B + t] + partial[b * K + k];
+    tree(s);
+    double count = s[4 * B];
+    double nn = count > 1.0 ? count : 1.0;
+    out[0] = s[0] / nn;
+    out[1] = s[1 * B] / (3.0 * nn);
+    out[2] = s[2 * B] / nn;
+    out[3] = s[3 * B] / nn;
+    out[4] = count;
+}
+
 // The G-buffer producer: one closest-hit primary ray per pixel centre.
 void orc_gbuffer_primary(const rgi_frame* f, const rgi_scene* s, const float* materials, uint32_t material_count,
                          float* depth, uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal, uint16_t* emission)

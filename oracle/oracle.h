@@ -70,6 +70,13 @@ void orc_di_temporal(const rgi_frame* frame, const rgi_scene* scene, const rgi_g
 void orc_di_spatial(const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
                     const rgi_di_reservoirs* reservoirs, uint16_t* out_diffuse, uint16_t* out_specular);
 
+/* fs_reference.sc / cs_error.sc as rgi_reference_accumulate and
+ * rgi_reference_error compute them, with the same FP64 reduction order. */
+void orc_reference_accumulate(const rgi_gbuffer* gbuffer, const uint16_t* color, const uint16_t* di_diffuse,
+                              const uint16_t* di_specular, double* accum);
+void orc_reference_error(const rgi_gbuffer* gbuffer, const uint16_t* color, const uint16_t* di_diffuse,
+                         const uint16_t* di_specular, const double* accum, uint32_t frames, double* out5);
+
 /* The G-buffer producer of rgi_gbuffer_primary, on the CPU. */
 void orc_gbuffer_primary(const rgi_frame* frame, const rgi_scene* scene, const float* materials,
                          uint32_t material_count, float* depth, uint32_t* albedo, uint32_t* normal,

@@ -70,6 +70,29 @@ class DiReservoirs(ctypes.Structure):
     _fields_ = [("sample", P), ("receiver", P), ("normal", P)]
 
 
+class CSettings(ctypes.Structure):
+    """rgi_settings (include/restir_amd_renderer.h)."""
+    _fields_ = [
+        ("initial_candidates", I32), ("temporal_taps", I32), ("spatial_taps", I32), ("trace_steps", I32),
+        ("di_candidates", I32), ("di_spatial_taps", I32), ("ray_mode", I32), ("world_space_reuse", I32),
+        ("multiple_bounces", I32), ("clamp_jacobian", I32), ("clamp_outliers", I32), ("boiling_filter", I32),
+        ("restir_di", I32), ("emissive_lighting", I32), ("sky", I32), ("shadows", I32), ("grid_mode", I32),
+        ("grid_table_size", U32), ("sky_zenith", F * 3), ("sky_horizon", F * 3), ("sky_ground", F * 3),
+        ("sky_intensity", F), ("light_position", F * 3), ("light_color", F * 3), ("origin_bottom_left", I32),
+        ("homogeneous_depth", I32), ("reserved", I32 * 4),
+    ]
+
+
+class View(ctypes.Structure):
+    """rgi_view: eye, view-projection and its inverse."""
+    _fields_ = [("eye", F * 4), ("view_proj", F * 16), ("inv_view_proj", F * 16)]
+
+
+class RendererOutputs(ctypes.Structure):
+    _fields_ = [("gi_sample", P), ("gi_receiver", P), ("gi_normal", P), ("color", P), ("moments", P), ("grid", P),
+                ("di_diffuse", P), ("di_specular", P)]
+
+
 class Mesh(ctypes.Structure):
     _fields_ = [("triangle_count", U32), ("positions", P), ("albedo", P), ("emission", P), ("material", P),
                 ("material_count", U32), ("materials", P)]
@@ -123,6 +146,24 @@ def lib():
     L.rgi_lights_build.argtypes = [P, U32, U32, I32, F, U32, P, P, F, P, U32]
     L.rgi_camera_matrices.restype = None
     L.rgi_camera_matrices.argtypes = [P, P, F, F, F, F, ctypes.c_int, P, P, P, P]
+    L.rgi_settings_default.restype = None
+    L.rgi_settings_default.argtypes = [ctypes.POINTER(CSettings)]
+    L.rgi_settings_sizeof.restype = U32
+    L.rgi_frame_pack.argtypes = [ctypes.POINTER(CSettings), ctypes.POINTER(Scene), F, U32, ctypes.POINTER(View),
+                                 ctypes.POINTER(View), I32, I32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Frame)]
+    L.rgi_renderer_create.argtypes = [ctypes.POINTER(P), I32, I32, ctypes.POINTER(CSettings), ctypes.POINTER(Scene), F]
+    L.rgi_renderer_destroy.restype = None
+    L.rgi_renderer_destroy.argtypes = [P]
+    L.rgi_renderer_render.argtypes = [P, ctypes.POINTER(View), ctypes.POINTER(GBuffer), P]
+    L.rgi_renderer_set_reference_mode.argtypes = [P, ctypes.c_int]
+    L.rgi_renderer_invalidate_history.restype = None
+    L.rgi_renderer_invalidate_history.argtypes = [P]
+    L.rgi_renderer_outputs_get.argtypes = [P, ctypes.POINTER(RendererOutputs)]
+    L.rgi_renderer_last_frame.argtypes = [P, ctypes.POINTER(Frame)]
+    L.rgi_reference_accumulate.argtypes = [ctypes.POINTER(GBuffer), P, P, P, P, P]
+    L.rgi_reference_error.argtypes = [P, ctypes.POINTER(GBuffer), P, P, P, P, U32, P, P]
+    if L.rgi_settings_sizeof() != ctypes.sizeof(CSettings):
+        raise RestirError("ABI mismatch: rgi_settings")
     for i, cls in enumerate(ABI_STRUCTS):
         size = L.rgi_abi_sizeof(i)
         if size != ctypes.sizeof(cls):
@@ -179,15 +220,19 @@ class HostScene:
         self.bvh_depth = int(depth.value)
         self.bvh = buf[: (2 * self.node_count + 4 * count) * 4].copy()
         # buildLightList's scale of an intensity of 1: the squared scene scale (renderer.cpp:497-503).
-        self.scene_scale = self.radius / 1.75
+        self.scene_scale = float(np.float32(self.radius) / np.float32(1.75))
         nl = L.rgi_lights_build(_ptr(self.bvh), self.node_count, count, extra_lights, extra_intensity, light_seed,
-                                _ptr(self.bounds_min), _ptr(self.bounds_max), self.scene_scale ** 2, None, 0)
+                                _ptr(self.bounds_min), _ptr(self.bounds_max), self._light_scale(), None, 0)
         self.lights = np.zeros((max(nl, 1), 16), np.float32)
         L.rgi_lights_build(_ptr(self.bvh), self.node_count, count, extra_lights, extra_intensity, light_seed,
-                           _ptr(self.bounds_min), _ptr(self.bounds_max), self.scene_scale ** 2,
+                           _ptr(self.bounds_min), _ptr(self.bounds_max), self._light_scale(),
                            _ptr(self.lights), max(nl, 1))
         self.light_count = int(nl)
         self.lights = self.lights[: max(nl, 1)]
+
+    def _light_scale(self):
+        s = np.float32(self.scene_scale)
+        return float(s * s)
 
     def rgi_scene(self, bvh_ptr, lights_ptr):
         return Scene(bvh_ptr, self.node_count, self.triangle_offset, self.triangle_count, self.light_count, lights_ptr)
@@ -245,6 +290,7 @@ class Settings:
         self.light_color = (0.0, 0.0, 0.0)  # main point light off
         self.light_position = (0.0, 1.7, 0.0)
         self.shadows = False
+        self.grid_mode = RGI_GRID_ATOMIC
         self.grid_table_size = 65536
         for k, v in kw.items():
             if not hasattr(self, k):
@@ -263,51 +309,46 @@ class Settings:
         return s
 
 
-def make_frame(settings, scene, frame_index, eye, vp, inv_vp, prev_eye, prev_vp, width, height, history_valid,
-               di_history_valid=False, gi_divisor=1, origin_bottom_left=False, homogeneous_depth=False):
-    """Packs rgi_frame as Renderer::render packs its uniforms."""
+def c_settings(settings):
+    """rgi_settings from a Settings."""
+    c = CSettings()
+    lib().rgi_settings_default(ctypes.byref(c))
     s = settings
+    for name in ("initial_candidates", "temporal_taps", "spatial_taps", "trace_steps", "di_candidates",
+                 "di_spatial_taps", "ray_mode"):
+        setattr(c, name, int(getattr(s, name)))
+    for name in ("world_space_reuse", "multiple_bounces", "clamp_jacobian", "clamp_outliers", "boiling_filter",
+                 "restir_di", "emissive_lighting", "sky", "shadows"):
+        setattr(c, name, int(bool(getattr(s, name))))
+    c.grid_mode = int(s.grid_mode)
+    c.grid_table_size = int(s.grid_table_size)
+    c.sky_zenith[:] = list(s.sky_zenith)
+    c.sky_horizon[:] = list(s.sky_horizon)
+    c.sky_ground[:] = list(s.sky_ground)
+    c.sky_intensity = s.sky_intensity
+    c.light_position[:] = list(s.light_position)
+    c.light_color[:] = list(s.light_color)
+    return c
+
+
+def make_view(eye, vp, inv_vp):
+    v = View()
+    v.eye[:] = [float(eye[0]), float(eye[1]), float(eye[2]), 1.0]
+    v.view_proj[:] = [float(x) for x in vp]
+    v.inv_view_proj[:] = [float(x) for x in inv_vp]
+    return v
+
+
+def make_frame(settings, scene, frame_index, eye, vp, inv_vp, prev_eye, prev_vp, width, height, history_valid,
+               di_history_valid=False):
+    """Packs rgi_frame as Renderer::render packs its uniforms (rgi_frame_pack)."""
     f = Frame()
-    f.inv_view_proj[:] = [float(v) for v in inv_vp]
-    f.view_proj[:] = [float(v) for v in vp]
-    f.prev_view_proj[:] = [float(v) for v in prev_vp]
-    f.camera_position[:] = [float(eye[0]), float(eye[1]), float(eye[2]), 1.0]
-    f.prev_camera_position[:] = [float(prev_eye[0]), float(prev_eye[1]), float(prev_eye[2]), 1.0]
-    f.light_position[:] = [float(v) for v in s.light_position] + [1.0]
-    scale = scene.scene_scale
-    f.light_color[:] = [c * scale * scale for c in s.light_color] + [1.0]
-    f.frame_index = frame_index & 0xFFFFFF
-    f.history_valid = int(history_valid)
-    f.origin_bottom_left = int(origin_bottom_left)
-    f.multiple_bounces = int(s.multiple_bounces)
-    f.homogeneous_depth = int(homogeneous_depth)
-    gw, gh = max(1, width // gi_divisor), max(1, height // gi_divisor)
-    f.gi_width, f.gi_height, f.width, f.height = gw, gh, width, height
-    f.scene_scale = scale
-    f.boiling_filter = 4.0 if s.boiling_filter else 0.0
-    f.jacobian_clamp = 10.0 if s.clamp_jacobian else 0.0
-    f.outlier_clamp = 8.0 if s.clamp_outliers else 0.0
-    f.lighting_surface = 1.0 if s.restir_di else (1.0 if s.emissive_lighting else 0.0)
-    f.lighting_emission = 0.0 if s.restir_di else (1.0 if s.emissive_lighting else 0.0)
-    f.initial_candidates = s.initial_candidates
-    f.temporal_taps = s.temporal_taps
-    f.spatial_taps = s.spatial_taps
-    f.trace_steps = s.trace_steps
-    f.grid_min_cell = 0.02 * scale
-    f.grid_pixel_size = 2.0 * math.tan(0.5 * math.radians(48.0)) / gh
-    f.grid_cell_pixels = 6.0
-    f.world_space_reuse = int(s.world_space_reuse)
-    f.grid_table_size = s.grid_table_size
-    f.ray_mode = s.ray_mode
-    w = 1.0 if s.sky else 0.0
-    f.sky_zenith[:] = [c * s.sky_intensity for c in s.sky_zenith] + [w]
-    f.sky_horizon[:] = [c * s.sky_intensity for c in s.sky_horizon] + [0.0]
-    f.sky_ground[:] = [c * s.sky_intensity for c in s.sky_ground] + [0.0]
-    f.shadows_enabled = int(s.shadows)
-    f.di_light_count = scene.light_count if s.restir_di else 0
-    f.di_candidates = s.di_candidates
-    f.di_spatial_taps = s.di_spatial_taps
-    f.di_history_valid = int(di_history_valid)
+    view = make_view(eye, vp, inv_vp)
+    prev = make_view(prev_eye, prev_vp, inv_vp)
+    sc = Scene(None, scene.node_count, scene.triangle_offset, scene.triangle_count, scene.light_count, None)
+    check(lib().rgi_frame_pack(ctypes.byref(c_settings(settings)), ctypes.byref(sc), scene.radius, frame_index,
+                               ctypes.byref(view), ctypes.byref(prev), width, height, int(history_valid),
+                               int(di_history_valid), ctypes.byref(f)))
     return f
 
 

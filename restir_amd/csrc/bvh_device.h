@@ -147,12 +147,19 @@ RGI_DEV int walk_short_stack(const SceneView& v, F3 o, F3 d, float& dist, int* r
     return hit;
 }
 
-// traceBvh's any-hit form: -1 when nothing is hit within max_distance.
-RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance)
+// traceBvh's any-hit form: -1 when nothing is hit within max_distance;
+// `dist` ends at the first hit found (or max_distance).
+RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance, float& dist)
 {
-    float dist = max_distance;
+    dist = max_distance;
     if (!bvh_ready(v)) return -1;
     return walk_stackless(v, o, d, true, dist);
+}
+
+RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance)
+{
+    float dist;
+    return trace_any(v, o, d, max_distance, dist);
 }
 
 RGI_DEV int trace_closest(const SceneView& v, F3 o, F3 d, float max_distance, float& dist, int* ring, int stride)

@@ -120,13 +120,11 @@ RGI_DEV float half_to_float(uint16_t bits)
     return (float)h;
 }
 
-RGI_DEV uint16_t float_to_half(float v)
-{
-    _Float16 h = (_Float16)v;
-    uint16_t bits;
-    __builtin_memcpy(&bits, &h, 2);
-    return bits;
-}
+// float -> binary16 with integer operations (rgi_half_bits), not the
+// hardware conversion: the backend fuses a multiply feeding a hardware f16
+// conversion into v_fma_mix, which rounds once where the arithmetic asks for
+// an f32 rounding first.
+RGI_DEV uint16_t float_to_half(float v) { return rgi_half_bits(v); }
 
 RGI_DEV F4 load_h4(const uint16_t* plane, uint32_t index)
 {

@@ -1,0 +1,330 @@
+// The frame driver (include/restir_amd_renderer.h): Renderer::render's hot
+// path (src/renderer.cpp:435-531, 614-833, 1119-1142) in C++ over the
+// per-pass kernels, with the reference's ping-pong and validity flags.
+#include <math.h>
+#include <string.h>
+
+#include <new>
+
+#include "../../include/restir_amd_renderer.h"
+#include "internal.h"
+
+struct rgi_renderer {
+    int32_t width = 0, height = 0;
+    rgi_settings settings{};
+    rgi_scene scene{};
+    float radius = 1.0f;
+    rgi_context* ctx = nullptr;
+    // Per parity: GI reservoirs, colour, moments, grid, DI reservoirs.
+    float* gi[2][3] = {};
+    uint16_t* color[2] = {};
+    uint16_t* moments[2] = {};
+    uint32_t* grid[2] = {};
+    float* di_sample[2] = {};
+    float* di_receiver[2] = {};
+    uint16_t* di_normal[2] = {};
+    uint16_t* diffuse = nullptr;
+    uint16_t* specular = nullptr;
+    int read = 0;              // m_historyRead / m_diRead: the parity last frame wrote
+    bool history_valid = false;
+    bool di_valid = false;
+    bool reference = false;
+    uint32_t frame_number = 0;
+    rgi_view previous{};
+    rgi_frame last{};
+    bool have_frame = false;
+};
+
+namespace {
+
+using namespace rgi;
+
+template <typename T>
+bool alloc(T*& p, size_t count)
+{
+    void* q = nullptr;
+    if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) return false;
+    if (hipMemset(q, 0, count * sizeof(T)) != hipSuccess) return false;
+    p = static_cast<T*>(q);
+    return true;
+}
+
+template <typename T>
+void release(T*& p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+void free_all(rgi_renderer* r)
+{
+    for (int k = 0; k < 2; ++k) {
+        for (int j = 0; j < 3; ++j) release(r->gi[k][j]);
+        release(r->color[k]);
+        release(r->moments[k]);
+        release(r->grid[k]);
+        release(r->di_sample[k]);
+        release(r->di_receiver[k]);
+        release(r->di_normal[k]);
+    }
+    release(r->diffuse);
+    release(r->specular);
+}
+
+// Effective settings while the reference accumulates (renderer.cpp:437-452).
+rgi_settings effective(const rgi_renderer* r)
+{
+    rgi_settings s = r->settings;
+    if (r->reference) {
+        s.temporal_taps = 0;
+        s.spatial_taps = 0;
+        s.clamp_jacobian = 0;
+        s.clamp_outliers = 0;
+        s.boiling_filter = 0;
+        s.di_spatial_taps = 0;
+    }
+    return s;
+}
+
+} // namespace
+
+extern "C" {
+
+void rgi_settings_default(rgi_settings* s)
+{
+    memset(s, 0, sizeof(*s));
+    s->initial_candidates = 2;
+    s->temporal_taps = 2;
+    s->spatial_taps = 4;
+    s->trace_steps = 32;
+    s->di_candidates = 8;
+    s->di_spatial_taps = 3;
+    s->ray_mode = RGI_RAY_HYBRID;
+    s->world_space_reuse = 1;
+    s->multiple_bounces = 1;
+    s->clamp_jacobian = 1;
+    s->clamp_outliers = 1;
+    s->boiling_filter = 1;
+    s->restir_di = 1;
+    s->emissive_lighting = 1;
+    s->sky = 0;
+    s->shadows = 0;
+    s->grid_mode = RGI_GRID_ATOMIC;
+    s->grid_table_size = 65536u;
+    const float zenith[3] = {0.30f, 0.50f, 0.95f}, horizon[3] = {0.85f, 0.88f, 0.92f}, ground[3] = {0.32f, 0.29f, 0.26f};
+    memcpy(s->sky_zenith, zenith, sizeof(zenith));
+    memcpy(s->sky_horizon, horizon, sizeof(horizon));
+    memcpy(s->sky_ground, ground, sizeof(ground));
+    s->sky_intensity = 1.0f;
+    s->light_position[1] = 1.7f;
+}
+
+uint32_t rgi_settings_sizeof(void) { return (uint32_t)sizeof(rgi_settings); }
+
+int rgi_frame_pack(const rgi_settings* s, const rgi_scene* scene, float radius, uint32_t frame_number,
+                   const rgi_view* view, const rgi_view* prev, int32_t width, int32_t height, int history_valid,
+                   int di_history_valid, rgi_frame* out)
+{
+    if (!s || !view || !prev || !out) return fail(RGI_E_INVALID, "frame_pack: NULL argument");
+    if (width <= 0 || height <= 0) return fail(RGI_E_INVALID, "frame_pack: size %d x %d", width, height);
+    rgi_frame f;
+    memset(&f, 0, sizeof(f));
+    memcpy(f.inv_view_proj, view->inv_view_proj, sizeof(f.inv_view_proj));
+    memcpy(f.view_proj, view->view_proj, sizeof(f.view_proj));
+    memcpy(f.prev_view_proj, prev->view_proj, sizeof(f.prev_view_proj));
+    for (int i = 0; i < 3; ++i) {
+        f.camera_position[i] = view->eye[i];
+        f.prev_camera_position[i] = prev->eye[i];
+        f.light_position[i] = s->light_position[i];
+    }
+    f.camera_position[3] = f.prev_camera_position[3] = f.light_position[3] = 1.0f;
+    // sceneScale and the light's scale with the square of it (renderer.cpp:497-516).
+    const float scale = radius / 1.75f;
+    for (int i = 0; i < 3; ++i) f.light_color[i] = s->light_color[i] * (scale * scale);
+    f.light_color[3] = 1.0f;
+    f.frame_index = frame_number & 0xffffffu;
+    f.history_valid = history_valid ? 1 : 0;
+    f.origin_bottom_left = s->origin_bottom_left;
+    f.multiple_bounces = s->multiple_bounces ? 1 : 0;
+    f.homogeneous_depth = s->homogeneous_depth;
+    f.gi_width = width;
+    f.gi_height = height;
+    f.width = width;
+    f.height = height;
+    f.scene_scale = scale;
+    f.boiling_filter = s->boiling_filter ? 4.0f : 0.0f;  // kBoilingFilter, settings.h:31
+    f.jacobian_clamp = s->clamp_jacobian ? 10.0f : 0.0f; // kJacobianClamp, settings.h:29
+    f.outlier_clamp = s->clamp_outliers ? 8.0f : 0.0f;   // kOutlierClamp, settings.h:30
+    // u_lightingInfo (renderer.cpp:694-702).
+    f.lighting_surface = s->restir_di ? 1.0f : (s->emissive_lighting ? 1.0f : 0.0f);
+    f.lighting_emission = s->restir_di ? 0.0f : (s->emissive_lighting ? 1.0f : 0.0f);
+    f.initial_candidates = s->initial_candidates;
+    f.temporal_taps = s->temporal_taps;
+    f.spatial_taps = s->spatial_taps;
+    f.trace_steps = s->trace_steps;
+    // u_gridInfo (renderer.cpp:714-720): kGridMinCellSize 0.02, kGridCellPixels 6, fovY 48.
+    const float pi = 3.14159265358979323846f;
+    f.grid_min_cell = 0.02f * scale;
+    f.grid_pixel_size = 2.0f * tanf(0.5f * (48.0f * (pi / 180.0f))) / (float)height;
+    f.grid_cell_pixels = 6.0f;
+    f.world_space_reuse = s->world_space_reuse ? 1 : 0;
+    f.grid_table_size = s->grid_table_size;
+    f.ray_mode = s->ray_mode;
+    for (int i = 0; i < 3; ++i) {
+        f.sky_zenith[i] = s->sky_zenith[i] * s->sky_intensity;
+        f.sky_horizon[i] = s->sky_horizon[i] * s->sky_intensity;
+        f.sky_ground[i] = s->sky_ground[i] * s->sky_intensity;
+    }
+    f.sky_zenith[3] = s->sky ? 1.0f : 0.0f;
+    f.shadows_enabled = s->shadows ? 1 : 0;
+    f.di_light_count = s->restir_di && scene ? (int32_t)scene->light_count : 0;
+    f.di_candidates = s->di_candidates;
+    f.di_spatial_taps = s->di_spatial_taps;
+    f.di_history_valid = di_history_valid ? 1 : 0;
+    *out = f;
+    return RGI_OK;
+}
+
+int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const rgi_settings* settings,
+                        const rgi_scene* scene, float radius)
+{
+    if (!out || !settings || !scene) return fail(RGI_E_INVALID, "renderer_create: NULL argument");
+    if (width <= 0 || height <= 0 || (int64_t)width * height >= (1ll << 29))
+        return fail(RGI_E_INVALID, "renderer_create: size %d x %d out of range", width, height);
+    uint32_t T = settings->grid_table_size;
+    if (T < 64u || (T & (T - 1u))) return fail(RGI_E_INVALID, "grid_table_size must be a power of two >= 64");
+    if (settings->shadows) return fail(RGI_E_INVALID, "the frame driver has no shadow atlas: set shadows = 0");
+    rgi_renderer* r = new (std::nothrow) rgi_renderer();
+    if (!r) return fail(RGI_E_NOMEM, "renderer allocation failed");
+    r->width = width;
+    r->height = height;
+    r->settings = *settings;
+    r->scene = *scene;
+    r->radius = radius;
+    int e = rgi_create(&r->ctx);
+    if (e != RGI_OK) {
+        delete r;
+        return e;
+    }
+    size_t n = (size_t)width * height;
+    bool ok = true;
+    for (int k = 0; k < 2 && ok; ++k) {
+        for (int j = 0; j < 3; ++j) ok = ok && alloc(r->gi[k][j], 4 * n);
+        ok = ok && alloc(r->color[k], 4 * n) && alloc(r->moments[k], 4 * n) && alloc(r->grid[k], (size_t)34 * T);
+        ok = ok && alloc(r->di_sample[k], 4 * n) && alloc(r->di_receiver[k], 4 * n) && alloc(r->di_normal[k], 4 * n);
+    }
+    ok = ok && alloc(r->diffuse, 4 * n) && alloc(r->specular, 4 * n);
+    if (!ok) {
+        free_all(r);
+        rgi_destroy(r->ctx);
+        delete r;
+        return fail(RGI_E_NOMEM, "renderer buffers for %d x %d (grid %u) could not be allocated", width, height, T);
+    }
+    *out = r;
+    return RGI_OK;
+}
+
+void rgi_renderer_destroy(rgi_renderer* r)
+{
+    if (!r) return;
+    hipDeviceSynchronize();
+    free_all(r);
+    rgi_destroy(r->ctx);
+    delete r;
+}
+
+int rgi_renderer_set_reference_mode(rgi_renderer* r, int on)
+{
+    if (!r) return fail(RGI_E_INVALID, "renderer is NULL");
+    if ((bool)on != r->reference) {
+        r->reference = on != 0;
+        r->history_valid = false; // renderer.cpp:1140-1141
+        r->di_valid = false;
+    }
+    return RGI_OK;
+}
+
+void rgi_renderer_invalidate_history(rgi_renderer* r)
+{
+    if (!r) return;
+    r->history_valid = false;
+    r->di_valid = false;
+}
+
+int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
+{
+    if (!r || !view || !g) return fail(RGI_E_INVALID, "renderer_render: NULL argument");
+    if (g->width != r->width || g->height != r->height)
+        return fail(RGI_E_INVALID, "G-buffer %d x %d does not match the renderer's %d x %d", g->width, g->height,
+                    r->width, r->height);
+    if (!g->depth || !g->normal || !g->albedo || !g->shading_normal || !g->emission)
+        return fail(RGI_E_INVALID, "renderer_render: every G-buffer plane is required");
+    hipStream_t stream = as_stream(stream_ptr);
+    rgi_settings s = effective(r);
+    if (r->reference) r->di_valid = false; // no DI temporal reuse either (renderer.cpp:451)
+    const rgi_view& prev = r->have_frame ? r->previous : *view;
+    rgi_frame f;
+    bool di_active = s.restir_di && r->scene.light_count > 0 && r->scene.node_count > 0;
+    int e = rgi_frame_pack(&s, &r->scene, r->radius, r->frame_number, view, &prev, r->width, r->height,
+                           r->history_valid, r->di_valid, &f);
+    if (e != RGI_OK) return e;
+    if (!di_active) f.di_light_count = 0;
+    const int read = r->read, write = 1 - r->read;
+
+    const uint16_t* surface_light = g->emission;
+    if (di_active) {
+        rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
+        rgi_di_reservoirs next_di{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
+        if ((e = di_temporal(f, r->scene, *g, prev_di, next_di, stream)) != RGI_OK) return e;
+        if ((e = di_spatial(f, r->scene, *g, next_di, r->diffuse, r->specular, stream)) != RGI_OK) return e;
+        surface_light = r->diffuse;
+        r->di_valid = true;
+    } else {
+        r->di_valid = false;
+    }
+    if ((e = grid_clear(r->grid[write], f.grid_table_size, stream)) != RGI_OK) return e;
+    if (s.grid_mode == RGI_GRID_DETERMINISTIC) e = grid_insert_deterministic(r->ctx, f, *g, r->grid[write], stream);
+    else e = grid_insert_atomic(f, *g, r->grid[write], stream);
+    if (e != RGI_OK) return e;
+    rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
+                      surface_light, nullptr, r->grid[read]};
+    rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
+    if ((e = gi_temporal(f, r->scene, *g, tin, res, stream)) != RGI_OK) return e;
+    rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
+                      surface_light, nullptr, r->grid[write]};
+    if ((e = gi_spatial(f, r->scene, *g, sin, res, r->color[write], r->moments[write], stream)) != RGI_OK) return e;
+
+    // End of frame (renderer.cpp:1119-1124).
+    r->previous = *view;
+    r->have_frame = true;
+    r->read = write;
+    r->history_valid = true;
+    ++r->frame_number;
+    r->last = f;
+    return RGI_OK;
+}
+
+int rgi_renderer_outputs_get(const rgi_renderer* r, rgi_renderer_outputs* out)
+{
+    if (!r || !out) return fail(RGI_E_INVALID, "renderer_outputs_get: NULL argument");
+    int k = r->read;
+    out->gi_sample = r->gi[k][0];
+    out->gi_receiver = r->gi[k][1];
+    out->gi_normal = r->gi[k][2];
+    out->color = r->color[k];
+    out->moments = r->moments[k];
+    out->grid = r->grid[k];
+    out->di_diffuse = r->diffuse;
+    out->di_specular = r->specular;
+    return RGI_OK;
+}
+
+int rgi_renderer_last_frame(const rgi_renderer* r, rgi_frame* out)
+{
+    if (!r || !out) return fail(RGI_E_INVALID, "renderer_last_frame: NULL argument");
+    if (!r->have_frame) return fail(RGI_E_INVALID, "no frame rendered yet");
+    *out = r->last;
+    return RGI_OK;
+}
+
+} // extern "C"

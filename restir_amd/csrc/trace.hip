@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4
     float t = a.w;
     int h;
     if (any_hit) {
-        h = trace_any(v, o, d, a.w);
+        h = trace_any(v, o, d, a.w, t);
     } else {
         h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock);
     }
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_persistent(rgi_scene s, c
             F3 o = F3{a.x, a.y, a.z}, d = F3{b.x, b.y, b.z};
             float t = a.w;
             int h;
-            if (any_hit) h = trace_any(v, o, d, a.w);
+            if (any_hit) h = trace_any(v, o, d, a.w, t);
             else h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock);
             hit[i] = h;
             dist[i] = t;

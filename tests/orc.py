@@ -74,6 +74,10 @@ def lib():
     L.orc_di_temporal.argtypes = [PF, PS, PG, ctypes.POINTER(DiReservoirs), ctypes.POINTER(DiReservoirs)]
     L.orc_di_spatial.restype = None
     L.orc_di_spatial.argtypes = [PF, PS, PG, ctypes.POINTER(DiReservoirs), P, P]
+    L.orc_reference_accumulate.restype = None
+    L.orc_reference_accumulate.argtypes = [PG, P, P, P, P]
+    L.orc_reference_error.restype = None
+    L.orc_reference_error.argtypes = [PG, P, P, P, P, U32, P]
     L.orc_gbuffer_primary.restype = None
     L.orc_gbuffer_primary.argtypes = [PF, PS, P, U32, P, P, P, P, P]
     _lib = L

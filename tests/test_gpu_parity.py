@@ -197,10 +197,6 @@ def test_grid_insert_atomic(dev, cornell_seq, table):
     slot0 = np.zeros(n, np.uint32)
     cs = np.zeros(n, np.uint32)
     orc.lib().orc_grid_keys(ctypes.byref(f), ctypes.byref(common.host_gbuffer(g, W, H)), orc.ptr(slot0), orc.ptr(cs), None)
-    if T == 65536:
-        # No probe collisions at this load: same checksums and counts per slot.
-        assert_same("grid.checksums", got[:T], want[:T])
-        assert_same("grid.counts", got[T:2 * T], want[T:2 * T])
     # Every listed entry is a pixel of that cell, lists are unique, and their
     # length is min(count, 32).
     cnt = got[T:2 * T]
@@ -210,7 +206,8 @@ def test_grid_insert_atomic(dev, cornell_seq, table):
         e = got[2 * T + s_ * 32: 2 * T + s_ * 32 + k]
         assert len(set(e.tolist())) == k
         assert np.all(cs[e] == got[s_])
-    # Multiset of (checksum, count) matches the deterministic result when nothing is dropped.
+    # Which cell owns a contested slot depends on the schedule, but at this
+    # load no pixel is dropped: the multiset of (checksum, count) is fixed.
     if T == 65536:
         assert sorted(zip(got[:T][got[:T] != 0], cnt[got[:T] != 0])) == sorted(zip(want[:T][want[:T] != 0], want[T:2 * T][want[:T] != 0]))
 

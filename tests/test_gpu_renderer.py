@@ -1,0 +1,119 @@
+"""GPU: the C++ frame driver (rgi_renderer_*) against a frame sequence run
+by the oracle with the same cameras, and the FP64 reference accumulation and
+error metric against the oracle. Bit-exact."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import restir_amd as ra
+
+import common
+import orc
+
+pytestmark = pytest.mark.gpu
+
+W, H = 160, 120
+
+
+@pytest.fixture(scope="module")
+def torch():
+    t = pytest.importorskip("torch")
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def upload(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8).reshape(-1)).cuda()
+
+
+def download(torch, ptr, like):
+    """Copy `like.nbytes` from a device pointer the library owns."""
+    hip = ra.lib()  # hipMemcpy resolves through the library's HIP runtime
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    torch.cuda.synchronize()
+    out = np.empty_like(like)
+    assert hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr), out.nbytes, 2) == 0
+    return out
+
+
+@pytest.mark.parametrize("frames,kw", [(4, {}), (3, {"restir_di": False, "sky": True}), (3, {"ray_mode": ra.RGI_RAY_BVH})])
+def test_renderer_matches_oracle_sequence(torch, frames, kw):
+    say("case renderer frames=%d %s" % (frames, kw))
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC, **kw)
+    yaw = 2 * np.pi / 64
+    seq = common.oracle_sequence(hs, s, W, H, frames, yaw_step=yaw)
+    L = ra.lib()
+    bvh, lights = upload(torch, hs.bvh), upload(torch, hs.lights)
+    scene = hs.rgi_scene(bvh.data_ptr(), lights.data_ptr())
+    r = ctypes.c_void_p()
+    ra.check(L.rgi_renderer_create(ctypes.byref(r), W, H, ctypes.byref(ra.c_settings(s)), ctypes.byref(scene),
+                                   hs.radius))
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    try:
+        for i, (f, g, hist, want) in enumerate(seq):
+            eye, vp, inv = ra.orbit_camera(hs, i, W, H, yaw_step=yaw)
+            view = ra.make_view(eye, vp, inv)
+            planes = {k: upload(torch, v) for k, v in g.items()}
+            gb = ra.GBuffer(W, H, planes["depth"].data_ptr(), planes["albedo"].data_ptr(), planes["normal"].data_ptr(),
+                            planes["shading"].data_ptr(), planes["emission"].data_ptr())
+            ra.check(L.rgi_renderer_render(r, ctypes.byref(view), ctypes.byref(gb), stream))
+            used = ra.Frame()
+            ra.check(L.rgi_renderer_last_frame(r, ctypes.byref(used)))
+            assert bytes(used) == bytes(f), "frame %d: packed uniforms differ" % i
+            out = ra.RendererOutputs()
+            ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(out)))
+            checks = [("gi_sample", out.gi_sample), ("gi_receiver", out.gi_receiver), ("gi_normal", out.gi_normal),
+                      ("color", out.color), ("moments", out.moments), ("grid", out.grid)]
+            if f.di_light_count > 0:
+                checks += [("diffuse", out.di_diffuse), ("specular", out.di_specular)]
+            for name, ptr in checks:
+                got = download(torch, ptr, want[name])
+                assert np.array_equal(got, want[name]), "frame %d: %s differs" % (i, name)
+    finally:
+        L.rgi_renderer_destroy(r)
+
+
+def test_reference_accumulate_and_error(torch):
+    say("case reference accumulate/error")
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings()
+    seq = common.oracle_sequence(hs, s, W, H, 3, yaw_step=0.0)
+    L = ra.lib()
+    n = W * H
+    want_acc = np.zeros(3 * n, np.float64)
+    acc = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    keep = []
+    for f, g, hist, out in seq:
+        gh = common.host_gbuffer(g, W, H)
+        orc.lib().orc_reference_accumulate(ctypes.byref(gh), orc.ptr(out["color"]), orc.ptr(out["diffuse"]),
+                                           orc.ptr(out["specular"]), orc.ptr(want_acc))
+        planes = {k: upload(torch, v) for k, v in g.items()}
+        c, d, sp = upload(torch, out["color"]), upload(torch, out["diffuse"]), upload(torch, out["specular"])
+        keep += [planes, c, d, sp]
+        gb = ra.GBuffer(W, H, planes["depth"].data_ptr(), planes["albedo"].data_ptr(), planes["normal"].data_ptr(),
+                        planes["shading"].data_ptr(), planes["emission"].data_ptr())
+        ra.check(L.rgi_reference_accumulate(ctypes.byref(gb), c.data_ptr(), d.data_ptr(), sp.data_ptr(),
+                                            acc.data_ptr(), stream))
+    got_acc = acc.cpu().numpy()
+    assert np.array_equal(got_acc, want_acc)
+    ctx = ctypes.c_void_p()
+    ra.check(L.rgi_create(ctypes.byref(ctx)))
+    res = torch.zeros(5, dtype=torch.float64, device="cuda")
+    ra.check(L.rgi_reference_error(ctx, ctypes.byref(gb), c.data_ptr(), d.data_ptr(), sp.data_ptr(), acc.data_ptr(), 3,
+                                   res.data_ptr(), stream))
+    got = res.cpu().numpy()
+    L.rgi_destroy(ctx)
+    want = np.zeros(5, np.float64)
+    f, g, hist, out = seq[-1]
+    gh = common.host_gbuffer(g, W, H)
+    orc.lib().orc_reference_error(ctypes.byref(gh), orc.ptr(out["color"]), orc.ptr(out["diffuse"]),
+                                  orc.ptr(out["specular"]), orc.ptr(want_acc), 3, orc.ptr(want))
+    assert np.array_equal(got, want), (got, want)

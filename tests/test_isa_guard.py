@@ -1,0 +1,36 @@
+"""The gfx950 code of every kernel file, compiled to AMD assembly here, holds
+none of the instructions that would round differently from the arithmetic
+as written: mixed-precision FMAs (a multiply fused into an fp16 conversion
+or an add) and packed FP32 math."""
+import concurrent.futures
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import pytest
+
+from restir_amd import build as rb
+
+FORBIDDEN = re.compile(r"\bv_(fma_mix\w*|mad_mix\w*|pk_fma_f32|pk_mul_f32|pk_add_f32)\b")
+
+
+def device_asm(src, out):
+    cmd = [rb.HIPCC] + rb.HIP_FLAGS + ["-S", "--offload-device-only", os.path.join(rb.CSRC, src), "-o", out]
+    subprocess.run(cmd, check=True, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    return open(out).read()
+
+
+@pytest.mark.skipif(shutil.which(rb.HIPCC) is None and not os.path.exists(rb.HIPCC), reason="no hipcc")
+def test_no_fused_or_packed_fp32_in_kernels():
+    tmp = tempfile.mkdtemp()
+    try:
+        with concurrent.futures.ThreadPoolExecutor(4) as pool:
+            outs = list(pool.map(lambda s: (s, device_asm(s, os.path.join(tmp, s + ".s"))), rb.HIP_SOURCES))
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    for src, text in outs:
+        code = [l for l in text.splitlines() if not l.lstrip().startswith((";", "."))]
+        bad = [l.strip() for l in code if FORBIDDEN.search(l)]
+        assert not bad, "%s: %s" % (src, bad[:5])

@@ -542,3 +542,62 @@ def test_cornell_scene_framing():
     assert abs(hs.scene_scale - math.sqrt(3) / 1.75) < 1e-6
     em = hs.emission.sum(1) > 0
     assert em.sum() == 2 and np.all(hs.emission[em] == 17.0)
+
+
+# ------------------------------------------------------ reference / error
+
+def test_reference_error_matches_numpy():
+    """cs_error.sc:59-63: relative MSE mean((L - L_ref)^2 / (L_ref^2 + 0.01))
+    in luminance over geometry pixels; the FP64 tree reduction agrees with a
+    plain numpy sum to rounding."""
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings()
+    seq = common.oracle_sequence(hs, s, 64, 48, 3, yaw_step=0.0)
+    n = 64 * 48
+    accum = np.zeros(3 * n, np.float64)
+    for f, g, hist, out in seq:
+        gb = common.host_gbuffer(g, 64, 48)
+        L.orc_reference_accumulate(ctypes.byref(gb), orc.ptr(out["color"]), orc.ptr(out["diffuse"]),
+                                   orc.ptr(out["specular"]), orc.ptr(accum))
+    f, g, hist, out = seq[-1]
+    gb = common.host_gbuffer(g, 64, 48)
+    res = np.zeros(5, np.float64)
+    L.orc_reference_error(ctypes.byref(gb), orc.ptr(out["color"]), orc.ptr(out["diffuse"]), orc.ptr(out["specular"]),
+                          orc.ptr(accum), 3, orc.ptr(res))
+    geo = g["depth"] < np.float32(0.99999)
+    col = out["color"].view(np.float16).astype(np.float32)[:, :3]
+    alb = np.stack([(g["albedo"] >> (8 * c)) & 255 for c in range(3)], 1).astype(np.float32) / np.float32(255)
+    metal = ((g["normal"] >> 24) & 255).astype(np.float32) / np.float32(255)
+    img = col * (alb * (np.float32(1) - metal)[:, None])
+    img = img + out["diffuse"].view(np.float16).astype(np.float32)[:, :3]
+    img = img + out["specular"].view(np.float16).astype(np.float32)[:, :3]
+    ref = accum.reshape(-1, 3) / 3.0
+    w = np.array([0.2126, 0.7152, 0.0722])
+    li, lr = img.astype(np.float64) @ w, ref @ w
+    rel = ((li - lr) ** 2 / (lr ** 2 + 0.01))[geo].mean()
+    mse = ((img.astype(np.float64) - ref) ** 2)[geo].mean()
+    assert res[4] == geo.sum()
+    assert abs(res[0] - rel) <= 1e-12 * max(rel, 1e-30) + 1e-18
+    assert abs(res[1] - mse) <= 1e-12 * max(mse, 1e-30) + 1e-18
+    assert abs(res[2] - li[geo].mean()) < 1e-12 and abs(res[3] - lr[geo].mean()) < 1e-12
+
+
+def test_frame_pack_follows_renderer_uniforms():
+    """rgi_frame_pack packs u_gridInfo, u_sceneInfo and u_lightingInfo as
+    renderer.cpp:497-531, 693-731 do."""
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings()
+    f = common.frame_for(hs, s, 320, 240, 5, True, True)
+    scale = np.float32(np.float32(hs.radius) / np.float32(1.75))
+    assert f.scene_scale == scale
+    assert f.grid_min_cell == np.float32(0.02) * scale
+    assert abs(f.grid_pixel_size - 2 * math.tan(math.radians(24)) / 240) < 1e-8
+    assert f.grid_cell_pixels == 6.0 and f.world_space_reuse == 1
+    assert (f.boiling_filter, f.jacobian_clamp, f.outlier_clamp) == (4.0, 10.0, 8.0)
+    assert (f.lighting_surface, f.lighting_emission) == (1.0, 0.0)
+    assert f.di_light_count == hs.light_count and f.frame_index == 5
+    g = common.frame_for(hs, ra.Settings(restir_di=False), 320, 240, 5, True)
+    assert (g.lighting_surface, g.lighting_emission, g.di_light_count) == (1.0, 1.0, 0)
+    r = common.frame_for(hs, s.reference_mode(), 320, 240, 5, True)
+    assert (r.temporal_taps, r.spatial_taps, r.di_spatial_taps) == (0, 0, 0)
+    assert (r.boiling_filter, r.jacobian_clamp, r.outlier_clamp) == (0.0, 0.0, 0.0)
