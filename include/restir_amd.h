@@ -173,6 +173,16 @@ typedef struct rgi_context rgi_context;
 int rgi_create(rgi_context** out_context);
 void rgi_destroy(rgi_context* context);
 
+/* Launch variants of the kernels, chosen per context (0 = default). They
+ * change how work maps onto the GPU, never a result: every variant is
+ * bit-exact with every other. rgi_autotune-style callers time them on their
+ * workload and keep the fastest. */
+#define RGI_OPT_GI_TEMPORAL_WAVES 1 /* occupancy target of GI temporal: 0 (compiler's choice) or 4 waves per SIMD */
+#define RGI_OPT_GI_SPATIAL_WAVES 2  /* the same for GI spatial */
+#define RGI_OPT_TRACE_PERSISTENT 3  /* rgi_trace: 0 one ray per lane, 1 persistent 64-ray batches per wave */
+int rgi_context_set_option(rgi_context* context, int option, int value);
+int rgi_context_get_option(const rgi_context* context, int option);
+
 /* cs_grid_clear (shaders/cs_grid_clear.sc:12-21, dispatched at
  * renderer.cpp:735-738): zeroes the checksums and counts of `grid`, a buffer
  * of (2 + 32) * table_size uint32 laid out [checksums | counts | entries]. */

@@ -98,6 +98,20 @@ void rgi_renderer_invalidate_history(rgi_renderer* renderer);
 
 int rgi_renderer_outputs_get(const rgi_renderer* renderer, rgi_renderer_outputs* out);
 
+/* rgi_context_set_option on the renderer's context (launch variants). */
+int rgi_renderer_set_option(rgi_renderer* renderer, int option, int value);
+
+/* The autotuner: renders `frames` frames per launch variant (cycling through
+ * `views` and their G-buffers), times each variant with HIP events on
+ * `stream`, and keeps the fastest. Variants are bit-exact with each other,
+ * so tuning never changes an image. `out_ms` (may be NULL) receives the mean
+ * frame time of each of the `RGI_TUNE_TRIALS` trials, in trial order:
+ * trial t uses GI temporal waves (t & 1 ? 4 : 0) and GI spatial waves
+ * (t & 2 ? 4 : 0). Returns the index of the chosen trial, or a negative code. */
+#define RGI_TUNE_TRIALS 4
+int rgi_renderer_autotune(rgi_renderer* renderer, const rgi_view* views, const rgi_gbuffer* gbuffers, uint32_t count,
+                          uint32_t frames, float* out_ms, void* stream);
+
 /* Packs the uniforms of one frame as Renderer::render does
  * (renderer.cpp:497-531, 614-621, 632-640, 693-731). `scene` supplies the
  * light count; `effective` settings are used as given (apply the reference

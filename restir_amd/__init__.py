@@ -21,6 +21,7 @@ RGI_OK = 0
 RGI_RAY_SCREEN, RGI_RAY_HYBRID, RGI_RAY_BVH = 0, 1, 2
 RGI_GRID_ATOMIC, RGI_GRID_DETERMINISTIC = 0, 1
 RGI_SCENE_CORNELL, RGI_SCENE_ATRIUM, RGI_SCENE_BOXFIELD = 0, 1, 2
+RGI_OPT_GI_TEMPORAL_WAVES, RGI_OPT_GI_SPATIAL_WAVES, RGI_OPT_TRACE_PERSISTENT = 1, 2, 3
 
 F, I32, U32, P = ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p
 
@@ -160,6 +161,10 @@ def lib():
     L.rgi_renderer_invalidate_history.argtypes = [P]
     L.rgi_renderer_outputs_get.argtypes = [P, ctypes.POINTER(RendererOutputs)]
     L.rgi_renderer_last_frame.argtypes = [P, ctypes.POINTER(Frame)]
+    L.rgi_context_set_option.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.rgi_context_get_option.argtypes = [P, ctypes.c_int]
+    L.rgi_renderer_set_option.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.rgi_renderer_autotune.argtypes = [P, P, P, U32, U32, P, P]
     L.rgi_reference_accumulate.argtypes = [ctypes.POINTER(GBuffer), P, P, P, P, P]
     L.rgi_reference_error.argtypes = [P, ctypes.POINTER(GBuffer), P, P, P, P, U32, P, P]
     if L.rgi_settings_sizeof() != ctypes.sizeof(CSettings):
@@ -248,11 +253,19 @@ def camera_matrices(eye, target, fovy=48.0, aspect=1.0, near=0.05, far=10.0, hom
     return view, proj, vp, inv
 
 
+def swing_yaw(frame, period=16, yaw=0.64, amplitude=0.32):
+    """Yaw of the benchmark's camera path: from the reference's starting yaw
+    (main.cpp:61) towards 0 and back over `period` frames, so the Cornell
+    box's open front stays in view (at most 7 degrees per frame)."""
+    return yaw - amplitude * (1.0 - math.cos(2.0 * math.pi * (frame % period) / period))
+
+
 def orbit_camera(scene, frame=0, width=320, height=240, yaw=0.64, pitch=0.255, distance=3.17, yaw_step=0.0,
-                 homogeneous_depth=False):
+                 homogeneous_depth=False, yaw_of=None):
     """The reference's starting framing (main.cpp:60-62, camera.cpp:112-122): `distance`
-    scene radii from the centre at (yaw, pitch), fovy 48; yaw advances by yaw_step per frame."""
-    y = yaw + yaw_step * frame
+    scene radii from the centre at (yaw, pitch), fovy 48; yaw advances by yaw_step per
+    frame, or follows yaw_of(frame) when given."""
+    y = yaw_of(frame) if yaw_of is not None else yaw + yaw_step * frame
     r = distance * scene.radius
     c = scene.center
     eye = np.array([c[0] + r * math.cos(pitch) * math.sin(y), c[1] + r * math.sin(pitch),

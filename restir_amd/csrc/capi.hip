@@ -125,11 +125,41 @@ int rgi_create(rgi_context** out)
 void rgi_destroy(rgi_context* ctx)
 {
     if (!ctx) return;
-    if (ctx->scratch) {
-        hipDeviceSynchronize();
-        hipFree(ctx->scratch);
-    }
+    if (ctx->scratch || ctx->counter) hipDeviceSynchronize();
+    if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->counter) hipFree(ctx->counter);
     delete ctx;
+}
+
+int rgi_context_set_option(rgi_context* ctx, int option, int value)
+{
+    if (!ctx) return fail(RGI_E_INVALID, "context is NULL");
+    switch (option) {
+    case RGI_OPT_GI_TEMPORAL_WAVES:
+    case RGI_OPT_GI_SPATIAL_WAVES:
+        if (value != 0 && value != 4) return fail(RGI_E_INVALID, "waves option takes 0 or 4, not %d", value);
+        (option == RGI_OPT_GI_TEMPORAL_WAVES ? ctx->tuning.gi_temporal_waves : ctx->tuning.gi_spatial_waves) = value;
+        return RGI_OK;
+    case RGI_OPT_TRACE_PERSISTENT:
+        if (value != 0 && value != 1) return fail(RGI_E_INVALID, "persistent option takes 0 or 1, not %d", value);
+        if (value && !ctx->counter && hipMalloc((void**)&ctx->counter, 256) != hipSuccess)
+            return fail(RGI_E_NOMEM, "work counter allocation failed");
+        ctx->tuning.trace_persistent = value;
+        return RGI_OK;
+    default:
+        return fail(RGI_E_INVALID, "unknown option %d", option);
+    }
+}
+
+int rgi_context_get_option(const rgi_context* ctx, int option)
+{
+    if (!ctx) return fail(RGI_E_INVALID, "context is NULL");
+    switch (option) {
+    case RGI_OPT_GI_TEMPORAL_WAVES: return ctx->tuning.gi_temporal_waves;
+    case RGI_OPT_GI_SPATIAL_WAVES: return ctx->tuning.gi_spatial_waves;
+    case RGI_OPT_TRACE_PERSISTENT: return ctx->tuning.trace_persistent;
+    default: return fail(RGI_E_INVALID, "unknown option %d", option);
+    }
 }
 
 int rgi_grid_clear(uint32_t* grid, uint32_t table_size, void* stream)
@@ -158,7 +188,6 @@ int rgi_grid_insert(rgi_context* ctx, const rgi_frame* frame, const rgi_gbuffer*
 int rgi_gi_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
                     const rgi_gi_inputs* in, const rgi_gi_reservoirs* out, void* stream)
 {
-    (void)ctx;
     int e;
     if ((e = check_frame(frame)) != RGI_OK) return e;
     if ((e = check_gbuffer(g, true, false)) != RGI_OK) return e;
@@ -171,14 +200,13 @@ int rgi_gi_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* s
     if (!out->sample || !out->receiver || !out->normal) return fail(RGI_E_INVALID, "gi temporal: output planes are required");
     if (frame->gi_width != g->width || frame->gi_height != g->height)
         return fail(RGI_E_INVALID, "frame GI size %d x %d != gbuffer %d x %d", frame->gi_width, frame->gi_height, g->width, g->height);
-    return gi_temporal(*frame, *scene, *g, *in, *out, as_stream(stream));
+    return gi_temporal(*frame, *scene, *g, *in, *out, ctx ? ctx->tuning : Tuning{}, as_stream(stream));
 }
 
 int rgi_gi_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
                    const rgi_gi_inputs* in, const rgi_gi_reservoirs* res, uint16_t* out_color, uint16_t* out_moments,
                    void* stream)
 {
-    (void)ctx;
     int e;
     if ((e = check_frame(frame)) != RGI_OK) return e;
     if ((e = check_gbuffer(g, true, false)) != RGI_OK) return e;
@@ -191,7 +219,8 @@ int rgi_gi_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* sc
     if (!res->sample || !res->receiver || !res->normal) return fail(RGI_E_INVALID, "gi spatial: reservoir planes are required");
     if (frame->gi_width != g->width || frame->gi_height != g->height)
         return fail(RGI_E_INVALID, "frame GI size %d x %d != gbuffer %d x %d", frame->gi_width, frame->gi_height, g->width, g->height);
-    return gi_spatial(*frame, *scene, *g, *in, *res, out_color, out_moments, as_stream(stream));
+    return gi_spatial(*frame, *scene, *g, *in, *res, out_color, out_moments, ctx ? ctx->tuning : Tuning{},
+                      as_stream(stream));
 }
 
 int rgi_di_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
@@ -227,12 +256,12 @@ int rgi_di_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* sc
 int rgi_trace(rgi_context* ctx, const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle,
               float* hit_distance, int any_hit, void* stream)
 {
-    (void)ctx;
     int e;
     if ((e = check_scene(scene, false)) != RGI_OK) return e;
     if (count > 0 && (!rays || !hit_triangle || !hit_distance)) return fail(RGI_E_INVALID, "trace: buffers are required");
     if (rays && !aligned(rays, 16)) return fail(RGI_E_INVALID, "rays must be 16-byte aligned");
-    return trace_rays(*scene, rays, count, hit_triangle, hit_distance, any_hit, as_stream(stream));
+    return trace_rays(*scene, rays, count, hit_triangle, hit_distance, any_hit, ctx ? ctx->tuning : Tuning{},
+                      ctx ? ctx->counter : nullptr, as_stream(stream));
 }
 
 int rgi_gbuffer_primary(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const float* materials,

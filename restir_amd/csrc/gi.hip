@@ -38,14 +38,22 @@ RGI_DEV void group_pixel(int& x, int& y)
     y = (int)(blockIdx.y * 16u + (group >> 2) * 8u + (t >> 2));
 }
 
-__global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
-                                                     rgi_gi_reservoirs out, F2 texel)
+struct TemporalLds {
+    int ring[kStackRing * 256];
+    float4 q_origin[256];
+    float4 q_dir[256];
+    int q_tri[256];
+    float q_dist[256];
+};
+
+RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+                              const rgi_gi_reservoirs& out, F2 texel, TemporalLds& lds)
 {
-    __shared__ int ring[kStackRing * 256];
-    __shared__ float4 q_origin[256];
-    __shared__ float4 q_dir[256];
-    __shared__ int q_tri[256];
-    __shared__ float q_dist[256];
+    int* ring = lds.ring;
+    float4* q_origin = lds.q_origin;
+    float4* q_dir = lds.q_dir;
+    int* q_tri = lds.q_tri;
+    float* q_dist = lds.q_dist;
 
     GiPass p;
     p.f = &f;
@@ -212,9 +220,8 @@ __global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, r
     }
 }
 
-__global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
-                                                    rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
-                                                    F2 texel)
+RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+                             const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel)
 {
     GiPass p;
     p.f = &f;
@@ -328,23 +335,60 @@ __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rg
     store_h4(out_moments, o, F4{m0, m1, history_length, 0.0f});
 }
 
+// The kernels: each pass with the compiler's occupancy, and capped at
+// 128 VGPRs for 4 waves per SIMD.
+__global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
+                                                     rgi_gi_reservoirs out, F2 texel)
+{
+    __shared__ TemporalLds lds;
+    gi_temporal_body(f, s, g, in, out, texel, lds);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs out, F2 texel)
+{
+    __shared__ TemporalLds lds;
+    gi_temporal_body(f, s, g, in, out, texel, lds);
+}
+
+__global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
+                                                    rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
+                                                    F2 texel)
+{
+    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
+                uint16_t* out_moments, F2 texel)
+{
+    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel);
+}
+
 } // namespace
 
 int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                const rgi_gi_reservoirs& out, hipStream_t stream)
+                const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream)
 {
     dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    hipLaunchKernelGGL(k_gi_temporal, blocks, dim3(256), 0, stream, f, s, g, in, out, texel);
+    if (tuning.gi_temporal_waves == 4)
+        hipLaunchKernelGGL(k_gi_temporal_w4, blocks, dim3(256), 0, stream, f, s, g, in, out, texel);
+    else
+        hipLaunchKernelGGL(k_gi_temporal, blocks, dim3(256), 0, stream, f, s, g, in, out, texel);
     return check_launch("gi temporal");
 }
 
 int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-               const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, hipStream_t stream)
+               const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
+               hipStream_t stream)
 {
     dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    hipLaunchKernelGGL(k_gi_spatial, blocks, dim3(256), 0, stream, f, s, g, in, res, color, moments, texel);
+    if (tuning.gi_spatial_waves == 4)
+        hipLaunchKernelGGL(k_gi_spatial_w4, blocks, dim3(256), 0, stream, f, s, g, in, res, color, moments, texel);
+    else
+        hipLaunchKernelGGL(k_gi_spatial, blocks, dim3(256), 0, stream, f, s, g, in, res, color, moments, texel);
     return check_launch("gi spatial");
 }
 

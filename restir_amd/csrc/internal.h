@@ -8,8 +8,19 @@
 
 #include "../../include/restir_amd.h"
 
+namespace rgi {
+// Launch variants (RGI_OPT_*), all bit-exact with each other.
+struct Tuning {
+    int gi_temporal_waves = 0;
+    int gi_spatial_waves = 0;
+    int trace_persistent = 0;
+};
+} // namespace rgi
+
 struct rgi_context {
     int device = 0;
+    rgi::Tuning tuning;
+    uint32_t* counter = nullptr;   // persistent-launch work counter
     void* scratch = nullptr;       // grid insert (deterministic) and other per-call scratch
     size_t scratch_bytes = 0;
 };
@@ -30,14 +41,15 @@ int grid_insert_atomic(const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
 int grid_insert_deterministic(rgi_context* ctx, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
                               hipStream_t stream);
 int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* hit, float* dist, int any_hit,
-               hipStream_t stream);
+               const Tuning& tuning, uint32_t* counter, hipStream_t stream);
 int gbuffer_primary(const rgi_frame& f, const rgi_scene& s, const float* materials, uint32_t material_count,
                     float* depth, uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal, uint16_t* emission,
                     hipStream_t stream);
 int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                const rgi_gi_reservoirs& out, hipStream_t stream);
+                const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream);
 int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-               const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, hipStream_t stream);
+               const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
+               hipStream_t stream);
 int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
                 const rgi_di_reservoirs& out, hipStream_t stream);
 int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,

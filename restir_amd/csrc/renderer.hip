@@ -289,10 +289,12 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
                       surface_light, nullptr, r->grid[read]};
     rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
-    if ((e = gi_temporal(f, r->scene, *g, tin, res, stream)) != RGI_OK) return e;
+    if ((e = gi_temporal(f, r->scene, *g, tin, res, r->ctx->tuning, stream)) != RGI_OK) return e;
     rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
                       surface_light, nullptr, r->grid[write]};
-    if ((e = gi_spatial(f, r->scene, *g, sin, res, r->color[write], r->moments[write], stream)) != RGI_OK) return e;
+    if ((e = gi_spatial(f, r->scene, *g, sin, res, r->color[write], r->moments[write], r->ctx->tuning, stream)) !=
+        RGI_OK)
+        return e;
 
     // End of frame (renderer.cpp:1119-1124).
     r->previous = *view;
@@ -302,6 +304,52 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     ++r->frame_number;
     r->last = f;
     return RGI_OK;
+}
+
+int rgi_renderer_set_option(rgi_renderer* r, int option, int value)
+{
+    if (!r) return fail(RGI_E_INVALID, "renderer is NULL");
+    return rgi_context_set_option(r->ctx, option, value);
+}
+
+int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuffer* gbuffers, uint32_t count,
+                          uint32_t frames, float* out_ms, void* stream_ptr)
+{
+    if (!r || !views || !gbuffers || count == 0 || frames == 0)
+        return fail(RGI_E_INVALID, "autotune: views, gbuffers, count and frames are required");
+    hipStream_t stream = as_stream(stream_ptr);
+    hipEvent_t start, stop;
+    if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess)
+        return fail(RGI_E_DEVICE, "autotune: event creation failed");
+    float best = 3.4e38f;
+    int chosen = 0, e = RGI_OK;
+    uint32_t k = 0;
+    for (int t = 0; t < RGI_TUNE_TRIALS && e == RGI_OK; ++t) {
+        r->ctx->tuning.gi_temporal_waves = (t & 1) ? 4 : 0;
+        r->ctx->tuning.gi_spatial_waves = (t & 2) ? 4 : 0;
+        // One untimed frame so each variant's code object is loaded and warm.
+        e = rgi_renderer_render(r, &views[k % count], &gbuffers[k % count], stream_ptr);
+        ++k;
+        hipEventRecord(start, stream);
+        for (uint32_t i = 0; i < frames && e == RGI_OK; ++i, ++k)
+            e = rgi_renderer_render(r, &views[k % count], &gbuffers[k % count], stream_ptr);
+        hipEventRecord(stop, stream);
+        hipEventSynchronize(stop);
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, start, stop);
+        ms /= (float)frames;
+        if (out_ms) out_ms[t] = ms;
+        if (ms < best) {
+            best = ms;
+            chosen = t;
+        }
+    }
+    hipEventDestroy(start);
+    hipEventDestroy(stop);
+    if (e != RGI_OK) return e;
+    r->ctx->tuning.gi_temporal_waves = (chosen & 1) ? 4 : 0;
+    r->ctx->tuning.gi_spatial_waves = (chosen & 2) ? 4 : 0;
+    return chosen;
 }
 
 int rgi_renderer_outputs_get(const rgi_renderer* r, rgi_renderer_outputs* out)

@@ -186,7 +186,7 @@ void make_atrium(Builder& b, uint32_t seed, uint32_t budget)
     double target = budget > 4096 ? (double)budget : 4096.0;
     int rings = 8;
     int n = 8;
-    for (int iter = 0; iter < 64; ++iter) {
+    for (int iter = 0; iter < 100000; ++iter) {
         double cols = (double)column_count * (24.0 * rings * 2.0 + 24.0 * 3.0 * 2.0);
         double slabs = (double)(storeys + 1) * 4.0 * n * n * 2.0 * 0.5 + (double)n * n * 2.0; // gallery rings + courtyard
         double walls = 4.0 * n * n * 2.0;

@@ -34,7 +34,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4
     dist[i] = t;
 }
 
-// Persistent form: waves pull 64-ray batches until the counter passes count.
+// Persistent form: waves pull 64-ray batches until the counter passes count;
+// every wave reaches the exit test each iteration, so the grid always drains.
 __global__ void __launch_bounds__(kTraceBlock) k_trace_persistent(rgi_scene s, const float4* rays, uint32_t count,
                                                                   int32_t* hit, float* dist, int any_hit,
                                                                   uint32_t* next_batch)
@@ -124,13 +125,29 @@ __global__ void __launch_bounds__(256) k_gbuffer(rgi_frame f, rgi_scene s, const
 
 } // namespace
 
+__global__ void k_zero_counter(uint32_t* counter) { *counter = 0u; }
+
 int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* hit, float* dist, int any_hit,
-               hipStream_t stream)
+               const Tuning& tuning, uint32_t* counter, hipStream_t stream)
 {
     if (count == 0) return RGI_OK;
-    uint32_t blocks = (count + kTraceBlock - 1u) / kTraceBlock;
-    hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kTraceBlock), 0, stream, s, reinterpret_cast<const float4*>(rays),
-                       count, hit, dist, any_hit);
+    const float4* r = reinterpret_cast<const float4*>(rays);
+    if (tuning.trace_persistent && counter) {
+        // Enough waves to fill every SIMD of the 256 CUs several times over;
+        // each exits once the batch counter passes `count`.
+        int device = 0, cus = 256;
+        hipGetDevice(&device);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        uint32_t blocks = (uint32_t)cus * 8u;
+        uint32_t needed = (count + kTraceBlock - 1u) / kTraceBlock;
+        blocks = blocks < needed ? blocks : needed;
+        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
+        hipLaunchKernelGGL(k_trace_persistent, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist,
+                           any_hit, counter);
+    } else {
+        uint32_t blocks = (count + kTraceBlock - 1u) / kTraceBlock;
+        hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist, any_hit);
+    }
     return check_launch("trace");
 }
 

@@ -129,8 +129,10 @@ def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0):
     """Runs `frames` frames on the oracle; returns (frames, gbuffers, histories, outputs)."""
     hist = empty_history(width, height, settings.grid_table_size)
     seq = []
+    di_on = settings.restir_di and hs.light_count > 0
     for i in range(frames):
-        f = frame_for(hs, settings, width, height, i, history_valid=i > 0, di_history_valid=i > 0, yaw_step=yaw_step)
+        f = frame_for(hs, settings, width, height, i, history_valid=i > 0, di_history_valid=i > 0 and di_on,
+                      yaw_step=yaw_step)
         g = oracle_gbuffer(hs, f)
         out = oracle_frame(hs, f, g, hist)
         seq.append((f, g, hist, out))
@@ -181,3 +183,15 @@ class Device:
         gb = ra.GBuffer(width, height, d["depth"].data_ptr(), d["albedo"].data_ptr(), d["normal"].data_ptr(),
                         d["shading"].data_ptr(), d["emission"].data_ptr())
         return gb, d
+
+
+def grid_view(grid, table):
+    """What readers of a grid can see (hashgrid.sh:155-177): checksums, counts
+    and each slot's first min(count, 32) entries; entries past the count are
+    never read, and a cleared grid keeps whatever they held."""
+    T = table
+    cs, cnt = grid[:T].copy(), grid[T:2 * T].copy()
+    entries = grid[2 * T:].reshape(T, 32).copy()
+    listed = np.minimum(cnt, 32)
+    entries[np.arange(32)[None, :] >= listed[:, None]] = 0
+    return cs, cnt, entries

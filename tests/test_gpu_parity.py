@@ -139,15 +139,20 @@ def trace_both(dev, hs, rays, any_hit):
     return dev.get(ht, want_t), dev.get(hd, want_d), want_t, want_d
 
 
-@pytest.mark.parametrize("kind,size,any_hit", [
-    (ra.RGI_SCENE_CORNELL, 0, False), (ra.RGI_SCENE_CORNELL, 0, True),
-    (ra.RGI_SCENE_BOXFIELD, 3000, False), (ra.RGI_SCENE_BOXFIELD, 3000, True),
-    (ra.RGI_SCENE_ATRIUM, 60000, False)])
-def test_trace(dev, kind, size, any_hit):
-    say("case trace kind=%d size=%d any_hit=%d" % (kind, size, any_hit))
+@pytest.mark.parametrize("kind,size,any_hit,persistent", [
+    (ra.RGI_SCENE_CORNELL, 0, False, 0), (ra.RGI_SCENE_CORNELL, 0, True, 0),
+    (ra.RGI_SCENE_BOXFIELD, 3000, False, 0), (ra.RGI_SCENE_BOXFIELD, 3000, True, 0),
+    (ra.RGI_SCENE_ATRIUM, 60000, False, 0), (ra.RGI_SCENE_ATRIUM, 60000, True, 0),
+    (ra.RGI_SCENE_BOXFIELD, 3000, False, 1), (ra.RGI_SCENE_CORNELL, 0, True, 1)])
+def test_trace(dev, kind, size, any_hit, persistent):
+    say("case trace kind=%d size=%d any_hit=%d persistent=%d" % (kind, size, any_hit, persistent))
     hs = ra.HostScene(kind, seed=5, size=size)
-    rays = surface_rays(hs, 1 << 16, seed=kind * 7 + size, tmax_scale=8.0)
-    gt, gd, wt, wd = trace_both(dev, hs, rays, any_hit)
+    rays = surface_rays(hs, (1 << 16) + 37, seed=kind * 7 + size, tmax_scale=8.0)
+    ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_TRACE_PERSISTENT, persistent))
+    try:
+        gt, gd, wt, wd = trace_both(dev, hs, rays, any_hit)
+    finally:
+        ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_TRACE_PERSISTENT, 0))
     assert_same("trace.triangle", gt, wt)
     assert_same("trace.distance", gd, wd)
     assert (wt >= 0).mean() > 0.3
@@ -286,6 +291,7 @@ def oracle_gi_spatial(hs, f, g, hist, res, grid, surface_light):
 
 GI_CASES = {
     "default": {},
+    "default_waves4": {"waves": 4},
     "bvh_rays": {"ray_mode": ra.RGI_RAY_BVH},
     "screen_rays": {"ray_mode": ra.RGI_RAY_SCREEN},
     "screen_reuse": {"world_space_reuse": False},
@@ -298,6 +304,7 @@ GI_CASES = {
 
 def case_frame(hs, frame, **kw):
     """`frame` with per-case overrides of the settings-derived fields."""
+    kw.pop("waves", None)
     f = ra.Frame.from_buffer_copy(frame)
     s = ra.Settings(**kw)
     ref = ra.make_frame(s, hs, f.frame_index, list(f.camera_position)[:3], list(f.view_proj), list(f.inv_view_proj),
@@ -341,7 +348,12 @@ def test_gi_temporal(dev, cornell_seq, case):
         hist["shadow"] = shadow_atlas(f)
     sl = out["surface_light"] if f.lighting_surface > 0 and f.di_light_count > 0 else g["emission"]
     want = oracle_gi_temporal(hs, f, g, hist, sl)
-    got = gpu_gi_temporal(dev, hs, f, g, hist, sl)
+    waves = GI_CASES[case].get("waves", 0)
+    ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_TEMPORAL_WAVES, waves))
+    try:
+        got = gpu_gi_temporal(dev, hs, f, g, hist, sl)
+    finally:
+        ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_TEMPORAL_WAVES, 0))
     working = (want["gi_receiver"][:, 3] > 0).mean()
     say("  W>0 fraction %.3f" % working)
     for k in want:
@@ -364,7 +376,12 @@ def test_gi_spatial(dev, cornell_seq, case):
     res = {k: out[k] for k in ("gi_sample", "gi_receiver", "gi_normal")}
     sl = out["surface_light"]
     want = oracle_gi_spatial(hs, f, g, hist, res, grid, sl)
-    got = gpu_gi_spatial(dev, hs, f, g, hist, res, grid, sl)
+    waves = GI_CASES[case].get("waves", 0)
+    ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_SPATIAL_WAVES, waves))
+    try:
+        got = gpu_gi_spatial(dev, hs, f, g, hist, res, grid, sl)
+    finally:
+        ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_SPATIAL_WAVES, 0))
     say("  color mean %s" % halfs(want["color"])[:, :3].mean(0))
     for k in want:
         assert_same(k, got[k], want[k])
