@@ -190,13 +190,17 @@ def main():
     for k in range(args.warmup):
         render(k)
     # Pick the fastest launch variants on this workload (bit-exact with each other).
-    tune_ms = (ctypes.c_float * 4)()
+    tune_ms = (ctypes.c_float * ra.RGI_TUNE_TRIALS)()
     view_arr = (ra.View * args.frames)(*views)
     gb_arr = (ra.GBuffer * args.frames)(*[g.gb for g in gbufs])
     chosen = L.rgi_renderer_autotune(rend, view_arr, gb_arr, args.frames, args.tune_frames, tune_ms, s)
     if chosen < 0:
         ra.check(chosen)
-    tuning = {"gi_temporal_waves": 4 if chosen & 1 else 0, "gi_spatial_waves": 4 if chosen & 2 else 0,
+    options = ra.tuning_of(chosen)
+    tuning = {"gi_temporal_waves": options[ra.RGI_OPT_GI_TEMPORAL_WAVES],
+              "gi_spatial_waves": options[ra.RGI_OPT_GI_SPATIAL_WAVES],
+              "gi_temporal_lockstep": options[ra.RGI_OPT_GI_TEMPORAL_LOCKSTEP],
+              "tile_order": options[ra.RGI_OPT_TILE_ORDER],
               "trial_ms": [round(float(v), 5) for v in tune_ms]}
     for k in range(args.warmup):
         render(k)
@@ -216,8 +220,8 @@ def main():
         return ra.make_frame(settings, hs, step, eye, vp, inv, peye, pvp, W, H, step > 0, step > 0), i
 
     frames = [frame_for_step(k) for k in range(args.warmup + min(args.steps, 32))]
-    for opt, key in ((ra.RGI_OPT_GI_TEMPORAL_WAVES, "gi_temporal_waves"), (ra.RGI_OPT_GI_SPATIAL_WAVES, "gi_spatial_waves")):
-        ra.check(L.rgi_context_set_option(drv.ctx, opt, tuning[key]))
+    for opt, value in options.items():
+        ra.check(L.rgi_context_set_option(drv.ctx, opt, value))
     for k in range(args.warmup):
         f, i = frames[k]
         drv.render(f, gscene, gbufs[i])

@@ -50,7 +50,10 @@ inline float fmin_(float a, float b) { return b < a ? b : a; }
 inline float fmax_(float a, float b) { return a < b ? b : a; }
 inline V3 vmin(V3 a, V3 b) { return V3{fmin_(a.x, b.x), fmin_(a.y, b.y), fmin_(a.z, b.z)}; }
 inline V3 vmax(V3 a, V3 b) { return V3{fmax_(a.x, b.x), fmax_(a.y, b.y), fmax_(a.z, b.z)}; }
-inline float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+// Products of sums are written with explicit fused multiply-adds in a fixed
+// order (x, then y, then z, then w); all other arithmetic is plain IEEE
+// single precision, uncontracted.
+inline float dot(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 inline V3 cross(V3 a, V3 b)
 {
     return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
@@ -77,11 +80,18 @@ const float kSkyDepth = 0.99999f;
 V4 mul(const float* m, V4 v)
 {
     V4 r;
-    r.x = ((m[0] * v.x + m[4] * v.y) + m[8] * v.z) + m[12] * v.w;
-    r.y = ((m[1] * v.x + m[5] * v.y) + m[9] * v.z) + m[13] * v.w;
-    r.z = ((m[2] * v.x + m[6] * v.y) + m[10] * v.z) + m[14] * v.w;
-    r.w = ((m[3] * v.x + m[7] * v.y) + m[11] * v.z) + m[15] * v.w;
+    r.x = fmaf(m[12], v.w, fmaf(m[8], v.z, fmaf(m[4], v.y, m[0] * v.x)));
+    r.y = fmaf(m[13], v.w, fmaf(m[9], v.z, fmaf(m[5], v.y, m[1] * v.x)));
+    r.z = fmaf(m[14], v.w, fmaf(m[10], v.z, fmaf(m[6], v.y, m[2] * v.x)));
+    r.w = fmaf(m[15], v.w, fmaf(m[11], v.z, fmaf(m[7], v.y, m[3] * v.x)));
     return r;
+}
+
+// mul(mat4, vec4(d, 0)): the image of a direction.
+V4 mul_dir(const float* m, V3 d)
+{
+    return V4{fmaf(m[8], d.z, fmaf(m[4], d.y, m[0] * d.x)), fmaf(m[9], d.z, fmaf(m[5], d.y, m[1] * d.x)),
+              fmaf(m[10], d.z, fmaf(m[6], d.y, m[2] * d.x)), fmaf(m[11], d.z, fmaf(m[7], d.y, m[3] * d.x))};
 }
 
 // ---------------------------------------------------------------- textures
@@ -176,10 +186,12 @@ float scene_scale(const Pass& p) { return p.f->scene_scale; }
 float to_clip_depth(const rgi_frame* f, float d) { return f->homogeneous_depth ? d * 2.0f - 1.0f : d; }
 float to_clip_y(const rgi_frame* f, float y) { return f->origin_bottom_left ? y : -y; }
 
+// The perspective divide is one reciprocal and a product per component.
 V3 clip_to_world(const float* inv, V3 clip)
 {
     V4 w = mul(inv, V4{clip.x, clip.y, clip.z, 1.0f});
-    return V3{w.x / w.w, w.y / w.w, w.z / w.w};
+    float r = 1.0f / w.w;
+    return V3{w.x * r, w.y * r, w.z * r};
 }
 
 // reconstructWorldPosition (restir.sh:150-155, di.sh:71-76, cs_grid_insert.sc:58-60).
@@ -194,7 +206,8 @@ V3 reconstruct_world_position(const rgi_frame* f, V2 uv, float depth)
 V2 clip_to_uv(const rgi_frame* f, V4 clip)
 {
     float origin_sign = f->origin_bottom_left ? 1.0f : -1.0f;
-    return V2{(clip.x / clip.w) * 0.5f + 0.5f, ((clip.y * origin_sign) / clip.w) * 0.5f + 0.5f};
+    float r = 1.0f / clip.w;
+    return V2{(clip.x * r) * 0.5f + 0.5f, ((clip.y * origin_sign) * r) * 0.5f + 0.5f};
 }
 
 // isOnScreen (restir.sh:163-166).
@@ -645,12 +658,33 @@ bool history_irradiance(const Pass& p, V2 uv, V3 position, V3 n, V3& irradiance)
     return true;
 }
 
-// depthBehindSurface (restir.sh:327-358).
-float depth_behind_surface(const Pass& p, V3 ray_position, V2& uv, V3& surface_position, bool& on_screen)
+// A marched ray and its clip-space image: the point at distance t is
+// origin + direction * t, whose clip position is clip_origin + t * clip_direction
+// (projectToScreen of that point, restir.sh:187-199, with the projection
+// distributed over the ray).
+struct ClipRay {
+    V3 origin, direction;
+    V4 clip_origin, clip_direction;
+};
+
+ClipRay clip_ray(const rgi_frame* f, V3 origin, V3 direction)
 {
-    float ray_depth;
+    return ClipRay{origin, direction, mul(f->view_proj, V4{origin.x, origin.y, origin.z, 1.0f}),
+                   mul_dir(f->view_proj, direction)};
+}
+
+// depthBehindSurface (restir.sh:327-358) at distance t along the ray.
+float depth_behind_surface(const Pass& p, const ClipRay& ray, float t, V2& uv, V3& surface_position, bool& on_screen)
+{
     surface_position = v3s(0.0f);
-    on_screen = project_to_screen(p.f, ray_position, uv, ray_depth);
+    V4 clip = V4{fmaf(ray.clip_direction.x, t, ray.clip_origin.x), fmaf(ray.clip_direction.y, t, ray.clip_origin.y),
+                 0.0f, fmaf(ray.clip_direction.w, t, ray.clip_origin.w)};
+    float ray_depth = clip.w;
+    uv = V2{0.0f, 0.0f};
+    on_screen = false;
+    if (clip.w <= 0.0001f) return -1.0f;
+    uv = clip_to_uv(p.f, clip);
+    on_screen = is_on_screen(uv);
     if (!on_screen) return -1.0f;
     float depth = fetch_depth(p, uv);
     if (depth >= kSkyDepth) return -1.0f;
@@ -658,6 +692,7 @@ float depth_behind_surface(const Pass& p, V3 ray_position, V2& uv, V3& surface_p
     float behind = ray_depth - mul(p.f->view_proj, V4{surface_position.x, surface_position.y, surface_position.z, 1.0f}).w;
     if (behind > 0.0f) {
         V3 sn = decode_normal(p, uv);
+        V3 ray_position = ray.origin + ray.direction * t;
         if (dot(ray_position - surface_position, sn) > -(0.01f * scene_scale(p))) return -1.0f;
     }
     return behind;
@@ -672,6 +707,7 @@ bool trace_screen_space_ray(const Pass& p, V3 origin, V3 direction, float max_di
     bool previous_in_front = true;
     int steps = p.f->trace_steps;
     float step_scale = 32.0f / (float)steps;
+    ClipRay ray = clip_ray(p.f, origin, direction);
     for (int i = 1; i <= steps; ++i) {
         if (previous_distance >= max_distance) break;
         float sp = (float)i * step_scale;
@@ -679,7 +715,7 @@ bool trace_screen_space_ray(const Pass& p, V3 origin, V3 direction, float max_di
         V2 uv;
         V3 surface;
         bool on_screen;
-        float behind = depth_behind_surface(p, origin + direction * ray_distance, uv, surface, on_screen);
+        float behind = depth_behind_surface(p, ray, ray_distance, uv, surface, on_screen);
         if (!on_screen) return false;
         bool in_front = behind <= 0.0f;
         if (!in_front && previous_in_front) {
@@ -690,7 +726,7 @@ bool trace_screen_space_ray(const Pass& p, V3 origin, V3 direction, float max_di
                 V2 muv;
                 V3 msurf;
                 bool mon;
-                float mb = depth_behind_surface(p, origin + direction * middle, muv, msurf, mon);
+                float mb = depth_behind_surface(p, ray, middle, muv, msurf, mon);
                 if (mon && mb > 0.0f) {
                     far_d = middle;
                     behind = mb;

@@ -146,6 +146,11 @@ int rgi_context_set_option(rgi_context* ctx, int option, int value)
             return fail(RGI_E_NOMEM, "work counter allocation failed");
         ctx->tuning.trace_persistent = value;
         return RGI_OK;
+    case RGI_OPT_GI_TEMPORAL_LOCKSTEP:
+    case RGI_OPT_TILE_ORDER:
+        if (value != 0 && value != 1) return fail(RGI_E_INVALID, "option %d takes 0 or 1, not %d", option, value);
+        (option == RGI_OPT_TILE_ORDER ? ctx->tuning.tile_order : ctx->tuning.gi_temporal_lockstep) = value;
+        return RGI_OK;
     default:
         return fail(RGI_E_INVALID, "unknown option %d", option);
     }
@@ -158,6 +163,8 @@ int rgi_context_get_option(const rgi_context* ctx, int option)
     case RGI_OPT_GI_TEMPORAL_WAVES: return ctx->tuning.gi_temporal_waves;
     case RGI_OPT_GI_SPATIAL_WAVES: return ctx->tuning.gi_spatial_waves;
     case RGI_OPT_TRACE_PERSISTENT: return ctx->tuning.trace_persistent;
+    case RGI_OPT_GI_TEMPORAL_LOCKSTEP: return ctx->tuning.gi_temporal_lockstep;
+    case RGI_OPT_TILE_ORDER: return ctx->tuning.tile_order;
     default: return fail(RGI_E_INVALID, "unknown option %d", option);
     }
 }
@@ -235,7 +242,7 @@ int rgi_di_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* s
         return fail(RGI_E_INVALID, "di_light_count %d exceeds scene light_count %u", frame->di_light_count, scene->light_count);
     if (!prev || !out || !prev->sample || !prev->receiver || !prev->normal || !out->sample || !out->receiver || !out->normal)
         return fail(RGI_E_INVALID, "di temporal: reservoir planes are required");
-    return di_temporal(*frame, *scene, *g, *prev, *out, as_stream(stream));
+    return di_temporal(*frame, *scene, *g, *prev, *out, ctx ? ctx->tuning : Tuning{}, as_stream(stream));
 }
 
 int rgi_di_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
@@ -250,7 +257,8 @@ int rgi_di_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* sc
         return fail(RGI_E_INVALID, "di_light_count %d exceeds scene light_count %u", frame->di_light_count, scene->light_count);
     if (!res || !res->sample || !res->receiver || !res->normal || !out_diffuse || !out_specular)
         return fail(RGI_E_INVALID, "di spatial: reservoir and output planes are required");
-    return di_spatial(*frame, *scene, *g, *res, out_diffuse, out_specular, as_stream(stream));
+    return di_spatial(*frame, *scene, *g, *res, out_diffuse, out_specular, ctx ? ctx->tuning : Tuning{},
+                      as_stream(stream));
 }
 
 int rgi_trace(rgi_context* ctx, const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle,

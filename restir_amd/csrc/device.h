@@ -25,6 +25,23 @@ struct F2 { float x, y; };
 struct F3 { float x, y, z; };
 struct F4 { float x, y, z, w; };
 
+// Workgroup -> 16 x 16 screen tile. Order 0 is row-major in workgroup id.
+// Order 1 hands each XCD a contiguous band of rows: the dispatcher places
+// workgroup b on XCD b % 8, so numbering that XCD's workgroups b / 8 = 0, 1,
+// ... through a band keeps the neighbourhood reads of concurrently running
+// tiles in one XCD's L2 instead of spreading every region over all eight.
+// Both orders visit every tile exactly once.
+RGI_DEV void block_tile(uint32_t order, uint32_t& tx, uint32_t& ty)
+{
+    uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+    if (order) {
+        uint32_t n = gridDim.x * gridDim.y, q = n >> 3, r = n & 7u, xcd = b & 7u, i = b >> 3;
+        b = xcd < r ? xcd * (q + 1u) + i : r * (q + 1u) + (xcd - r) * q + i;
+    }
+    tx = b % gridDim.x;
+    ty = b / gridDim.x;
+}
+
 RGI_DEV F3 f3(float x, float y, float z) { return F3{x, y, z}; }
 RGI_DEV F3 splat3(float s) { return F3{s, s, s}; }
 RGI_DEV F3 xyz(F4 v) { return F3{v.x, v.y, v.z}; }
@@ -39,7 +56,9 @@ RGI_DEV float minf_(float a, float b) { return b < a ? b : a; }
 RGI_DEV float maxf_(float a, float b) { return a < b ? b : a; }
 RGI_DEV F3 min3(F3 a, F3 b) { return F3{minf_(a.x, b.x), minf_(a.y, b.y), minf_(a.z, b.z)}; }
 RGI_DEV F3 max3(F3 a, F3 b) { return F3{maxf_(a.x, b.x), maxf_(a.y, b.y), maxf_(a.z, b.z)}; }
-RGI_DEV float dot(F3 a, F3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+// Products of sums use explicit fused multiply-adds in a fixed order (x, y,
+// z, then w); everything else is plain IEEE single precision, uncontracted.
+RGI_DEV float dot(F3 a, F3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 RGI_DEV F3 cross(F3 a, F3 b) { return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 RGI_DEV float length(F3 v) { return sqrtf(dot(v, v)); }
 RGI_DEV float rsqrt_(float x) { return 1.0f / sqrtf(x); }
@@ -70,15 +89,22 @@ constexpr uint32_t kGridInvalid = 0xffffffffu;
 RGI_DEV F4 mul(const float* m, F4 v)
 {
     F4 r;
-    r.x = ((m[0] * v.x + m[4] * v.y) + m[8] * v.z) + m[12] * v.w;
-    r.y = ((m[1] * v.x + m[5] * v.y) + m[9] * v.z) + m[13] * v.w;
-    r.z = ((m[2] * v.x + m[6] * v.y) + m[10] * v.z) + m[14] * v.w;
-    r.w = ((m[3] * v.x + m[7] * v.y) + m[11] * v.z) + m[15] * v.w;
+    r.x = fmaf(m[12], v.w, fmaf(m[8], v.z, fmaf(m[4], v.y, m[0] * v.x)));
+    r.y = fmaf(m[13], v.w, fmaf(m[9], v.z, fmaf(m[5], v.y, m[1] * v.x)));
+    r.z = fmaf(m[14], v.w, fmaf(m[10], v.z, fmaf(m[6], v.y, m[2] * v.x)));
+    r.w = fmaf(m[15], v.w, fmaf(m[11], v.z, fmaf(m[7], v.y, m[3] * v.x)));
     return r;
 }
 
-// Only the w row of mat4 * (p, 1): the view depth.
-RGI_DEV float mul_w(const float* m, F3 p) { return ((m[3] * p.x + m[7] * p.y) + m[11] * p.z) + m[15] * 1.0f; }
+// mat4 * (d, 0): the image of a direction.
+RGI_DEV F4 mul_dir(const float* m, F3 d)
+{
+    return F4{fmaf(m[8], d.z, fmaf(m[4], d.y, m[0] * d.x)), fmaf(m[9], d.z, fmaf(m[5], d.y, m[1] * d.x)),
+              fmaf(m[10], d.z, fmaf(m[6], d.y, m[2] * d.x)), fmaf(m[11], d.z, fmaf(m[7], d.y, m[3] * d.x))};
+}
+
+// Only the w row of mat4 * (p, 1): the view depth (fma(m15, 1, x) = x + m15).
+RGI_DEV float mul_w(const float* m, F3 p) { return fmaf(m[11], p.z, fmaf(m[7], p.y, m[3] * p.x)) + m[15]; }
 
 // ------------------------------------------------------------- plane fetch
 
@@ -173,7 +199,8 @@ RGI_DEV float to_clip_y(const rgi_frame& f, float y) { return f.origin_bottom_le
 RGI_DEV F3 clip_to_world(const float* inv, F3 clip)
 {
     F4 w = mul(inv, F4{clip.x, clip.y, clip.z, 1.0f});
-    return F3{w.x / w.w, w.y / w.w, w.z / w.w};
+    float r = 1.0f / w.w;
+    return F3{w.x * r, w.y * r, w.z * r};
 }
 
 RGI_DEV F3 reconstruct_position(const rgi_frame& f, F2 uv, float depth)
@@ -186,7 +213,8 @@ RGI_DEV F3 reconstruct_position(const rgi_frame& f, F2 uv, float depth)
 RGI_DEV F2 clip_to_uv(const rgi_frame& f, F4 clip)
 {
     float sign = f.origin_bottom_left ? 1.0f : -1.0f;
-    return F2{(clip.x / clip.w) * 0.5f + 0.5f, ((clip.y * sign) / clip.w) * 0.5f + 0.5f};
+    float r = 1.0f / clip.w;
+    return F2{(clip.x * r) * 0.5f + 0.5f, ((clip.y * sign) * r) * 0.5f + 0.5f};
 }
 
 RGI_DEV bool on_screen(F2 uv) { return uv.x > 0.001f && uv.x < 0.999f && uv.y > 0.001f && uv.y < 0.999f; }

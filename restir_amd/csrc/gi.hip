@@ -31,23 +31,162 @@ RGI_DEV uint32_t rank_in(uint64_t mask)
 }
 
 // Pixel of this thread: 16 x 16 per block, two 4 x 8 groups per wave.
-RGI_DEV void group_pixel(int& x, int& y)
+RGI_DEV void group_pixel(uint32_t order, int& x, int& y)
 {
-    uint32_t group = threadIdx.x >> 5, t = threadIdx.x & 31u;
-    x = (int)(blockIdx.x * 16u + (group & 3u) * 4u + (t & 3u));
-    y = (int)(blockIdx.y * 16u + (group >> 2) * 8u + (t >> 2));
+    uint32_t group = threadIdx.x >> 5, t = threadIdx.x & 31u, tx, ty;
+    block_tile(order, tx, ty);
+    x = (int)(tx * 16u + (group & 3u) * 4u + (t & 3u));
+    y = (int)(ty * 16u + (group >> 2) * 8u + (t >> 2));
 }
 
+// Per-block LDS of the temporal pass: the closest-hit stack rings and a ray
+// queue of kQueue rays per lane.
+template <int kQueue>
 struct TemporalLds {
     int ring[kStackRing * 256];
-    float4 q_origin[256];
-    float4 q_dir[256];
-    int q_tri[256];
-    float q_dist[256];
+    float4 q_origin[256 * kQueue];
+    float4 q_dir[256 * kQueue];
+    int q_tri[256 * kQueue];
+    float q_dist[256 * kQueue];
 };
 
+// Source density of a candidate hit, converted from the ray origin to the
+// receiver through the area measure at the hit (cs_restir_temporal.sc:163-184),
+// and its stream into the canonical reservoir.
+RGI_DEV void stream_candidate(const GiPass& p, F3 position, F3 n, F3 albedo, F3 ray_origin, F3 dir, bool hit,
+                              F3 hit_position, float hit_id, float rsel, Res& canonical)
+{
+    F3 contribution = splat3(0.0f), hit_normal = splat3(0.0f);
+    bool evaluated = false;
+    if (hit) evaluated = shade_sample(p, position, n, albedo, hit_position, hit_id, contribution, hit_normal);
+    if (evaluated) {
+        float target = luminance(contribution);
+        float source_pdf = dot(n, dir) / kPi;
+        if (!is_sky(hit_id)) {
+            F3 from_receiver = hit_position - position;
+            F3 from_origin = hit_position - ray_origin;
+            float lim = ((0.0001f * p.sc) * p.sc) * 0.01f;
+            float rs = maxf_(dot(from_receiver, from_receiver), lim);
+            float os = maxf_(dot(from_origin, from_origin), lim);
+            float hco = maxf_(-dot(hit_normal, from_origin), 0.0f) * rsqrt_(os);
+            float hcr = -dot(hit_normal, from_receiver) * rsqrt_(rs);
+            source_pdf = hcr > 0.0001f ? (source_pdf * (hco / os)) * (rs / hcr) : 0.0f;
+        }
+        if (target > 0.0f && source_pdf > 0.0001f)
+            stream_in(canonical, contribution, hit_position, hit_normal, hit_id, target, target / source_pdf, rsel);
+    }
+}
+
+// The candidate loop with the depth-buffer marches of kChunk candidates
+// advancing in lockstep (independent fetch chains in flight per lane) and
+// all their misses traced from one per-wave queue. Randoms are drawn and
+// reservoir updates applied in candidate order, so the result is the same
+// as the one-candidate-at-a-time loop.
+constexpr int kChunk = 2;
+
+RGI_DEV void candidates_lockstep(const GiPass& p, const rgi_frame& f, F3 position, F3 n, F3 albedo, F3 ray_origin,
+                                 bool working, uint32_t& rng, Res& canonical, TemporalLds<kChunk>& lds)
+{
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    float4* q_origin = lds.q_origin;
+    float4* q_dir = lds.q_dir;
+    int* q_tri = lds.q_tri;
+    float* q_dist = lds.q_dist;
+    const int steps = f.trace_steps;
+    const float step_scale = 32.0f / (float)steps;
+    for (int base = 0; base < f.initial_candidates; base += kChunk) {
+        int count = f.initial_candidates - base < kChunk ? f.initial_candidates - base : kChunk;
+        F3 dir[kChunk];
+        float rsel[kChunk];
+        MarchRay m[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            if (j < count) {
+                float r0 = next_random(rng);
+                float r1 = next_random(rng);
+                rsel[j] = next_random(rng);
+                dir[j] = cosine_lobe(n, r0, r1);
+                canonical.count += 1.0f;
+                march_begin(p, m[j], ray_origin, dir[j], 3.6f * p.sc, working && f.ray_mode != RGI_RAY_BVH);
+            } else {
+                dir[j] = splat3(0.0f);
+                rsel[j] = 0.0f;
+                march_begin(p, m[j], ray_origin, dir[j], 0.0f, false);
+            }
+        }
+        for (int i = 1; i <= steps; ++i) {
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+                march_step(p, m[j], i, step_scale);
+                any = any || m[j].active;
+            }
+            if (!any) break;
+        }
+        // One queue for the chunk's misses: slot = rays of earlier
+        // candidates, then rank within this candidate.
+        bool need[kChunk];
+        uint32_t slot[kChunk];
+        float hit_id[kChunk];
+        uint32_t queued = 0;
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            need[j] = j < count && working && !m[j].hit && f.ray_mode != RGI_RAY_SCREEN;
+            hit_id[j] = 0.0f;
+            uint64_t mask = __ballot(need[j]);
+            slot[j] = queued + rank_in(mask);
+            queued += (uint32_t)__popcll(mask);
+        }
+        if (queued != 0u) {
+            uint32_t qbase = wave * (64u * kChunk);
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+                if (need[j]) {
+                    q_origin[qbase + slot[j]] = make_float4(ray_origin.x, ray_origin.y, ray_origin.z, 0.0f);
+                    q_dir[qbase + slot[j]] = make_float4(dir[j].x, dir[j].y, dir[j].z, 0.0f);
+                }
+            }
+            wave_sync();
+            for (uint32_t q = lane; q < queued; q += 64u) {
+                float4 qo = q_origin[qbase + q], qd = q_dir[qbase + q];
+                float dist;
+                int tri = trace_closest(p.v, F3{qo.x, qo.y, qo.z}, F3{qd.x, qd.y, qd.z}, 4.0f * p.sc, dist,
+                                        lds.ring + threadIdx.x, 256);
+                q_tri[qbase + q] = tri;
+                q_dist[qbase + q] = dist;
+            }
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < kChunk; ++j) {
+                if (need[j]) {
+                    int tri = q_tri[qbase + slot[j]];
+                    float dist = q_dist[qbase + slot[j]];
+                    if (tri >= 0) {
+                        m[j].hit = true;
+                        m[j].hit_position = ray_origin + dir[j] * dist;
+                        hit_id[j] = sample_id_of(p.v, tri, dir[j]);
+                    } else if (sky_on(f)) {
+                        m[j].hit = true;
+                        m[j].hit_position = dir[j];
+                        hit_id[j] = kSkySampleId;
+                    }
+                }
+            }
+            wave_sync(); // the queue is free again before the next chunk
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+            if (j < count)
+                stream_candidate(p, position, n, albedo, ray_origin, dir[j], m[j].hit, m[j].hit_position, hit_id[j],
+                                 rsel[j], canonical);
+        }
+    }
+}
+
+template <bool kLockstep>
 RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                              const rgi_gi_reservoirs& out, F2 texel, TemporalLds& lds)
+                              const rgi_gi_reservoirs& out, F2 texel, uint32_t order,
+                              TemporalLds<kLockstep ? kChunk : 1>& lds)
 {
     int* ring = lds.ring;
     float4* q_origin = lds.q_origin;
@@ -66,7 +205,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     p.sc = f.scene_scale;
 
     int x, y;
-    group_pixel(x, y);
+    group_pixel(order, x, y);
     uint32_t lane = threadIdx.x & 63u, qbase = threadIdx.x & ~63u;
     bool inside = x < g.width && y < g.height;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -80,6 +219,9 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     uint32_t rng = pcg((uint32_t)to_int(gx) + pcg((uint32_t)to_int(gy) + pcg(f.frame_index)));
 
     Res canonical = empty_res();
+    if constexpr (kLockstep) {
+        candidates_lockstep(p, f, position, n, albedo, ray_origin, working, rng, canonical, lds);
+    } else {
     for (int k = 0; k < f.initial_candidates; ++k) {
         float r0 = next_random(rng);
         float r1 = next_random(rng);
@@ -124,27 +266,8 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
             }
             wave_sync();
         }
-        F3 contribution = splat3(0.0f), hit_normal = splat3(0.0f);
-        bool evaluated = false;
-        if (hit) evaluated = shade_sample(p, position, n, albedo, hit_position, hit_id, contribution, hit_normal);
-        if (evaluated) {
-            float target = luminance(contribution);
-            // Source density converted from the ray origin to the receiver
-            // through the area measure at the hit (cs_restir_temporal.sc:163-184).
-            float source_pdf = dot(n, dir) / kPi;
-            if (!is_sky(hit_id)) {
-                F3 from_receiver = hit_position - position;
-                F3 from_origin = hit_position - ray_origin;
-                float lim = ((0.0001f * p.sc) * p.sc) * 0.01f;
-                float rs = maxf_(dot(from_receiver, from_receiver), lim);
-                float os = maxf_(dot(from_origin, from_origin), lim);
-                float hco = maxf_(-dot(hit_normal, from_origin), 0.0f) * rsqrt_(os);
-                float hcr = -dot(hit_normal, from_receiver) * rsqrt_(rs);
-                source_pdf = hcr > 0.0001f ? (source_pdf * (hco / os)) * (rs / hcr) : 0.0f;
-            }
-            if (target > 0.0f && source_pdf > 0.0001f)
-                stream_in(canonical, contribution, hit_position, hit_normal, hit_id, target, target / source_pdf, rsel);
-        }
+        stream_candidate(p, position, n, albedo, ray_origin, dir, hit, hit_position, hit_id, rsel, canonical);
+    }
     }
     float canonical_weight = canonical.target > 0.000001f ? canonical.wsum / (canonical.count * canonical.target) : 0.0f;
 
@@ -221,7 +344,8 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
 }
 
 RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                             const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel)
+                             const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel,
+                             uint32_t order)
 {
     GiPass p;
     p.f = &f;
@@ -234,7 +358,7 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     p.sc = f.scene_scale;
 
     int x, y;
-    group_pixel(x, y);
+    group_pixel(order, x, y);
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -336,33 +460,32 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
 }
 
 // The kernels: each pass with the compiler's occupancy, and capped at
-// 128 VGPRs for 4 waves per SIMD.
-__global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
-                                                     rgi_gi_reservoirs out, F2 texel)
-{
-    __shared__ TemporalLds lds;
-    gi_temporal_body(f, s, g, in, out, texel, lds);
-}
-
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs out, F2 texel)
-{
-    __shared__ TemporalLds lds;
-    gi_temporal_body(f, s, g, in, out, texel, lds);
-}
+// 128 VGPRs for 4 waves per SIMD; the temporal pass also in its lockstep form.
+#define RGI_TEMPORAL_KERNEL(name, lockstep, attrs)                                                              \
+    __global__ void __launch_bounds__(256) attrs name(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, \
+                                                      rgi_gi_reservoirs out, F2 texel, uint32_t order)          \
+    {                                                                                                          \
+        __shared__ TemporalLds<lockstep ? kChunk : 1> lds;                                                     \
+        gi_temporal_body<lockstep>(f, s, g, in, out, texel, order, lds);                                       \
+    }
+#define RGI_WAVES4 __attribute__((amdgpu_waves_per_eu(4, 4)))
+RGI_TEMPORAL_KERNEL(k_gi_temporal, false, )
+RGI_TEMPORAL_KERNEL(k_gi_temporal_w4, false, RGI_WAVES4)
+RGI_TEMPORAL_KERNEL(k_gi_temporal_lockstep, true, )
+RGI_TEMPORAL_KERNEL(k_gi_temporal_lockstep_w4, true, RGI_WAVES4)
 
 __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                     rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
-                                                    F2 texel)
+                                                    F2 texel, uint32_t order)
 {
-    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel);
+    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, order);
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
-                uint16_t* out_moments, F2 texel)
+                uint16_t* out_moments, F2 texel, uint32_t order)
 {
-    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel);
+    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, order);
 }
 
 } // namespace
@@ -372,10 +495,11 @@ int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
 {
     dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    if (tuning.gi_temporal_waves == 4)
-        hipLaunchKernelGGL(k_gi_temporal_w4, blocks, dim3(256), 0, stream, f, s, g, in, out, texel);
-    else
-        hipLaunchKernelGGL(k_gi_temporal, blocks, dim3(256), 0, stream, f, s, g, in, out, texel);
+    uint32_t order = (uint32_t)tuning.tile_order;
+    bool w4 = tuning.gi_temporal_waves == 4;
+    auto kernel = tuning.gi_temporal_lockstep ? (w4 ? k_gi_temporal_lockstep_w4 : k_gi_temporal_lockstep)
+                                              : (w4 ? k_gi_temporal_w4 : k_gi_temporal);
+    hipLaunchKernelGGL(kernel, blocks, dim3(256), 0, stream, f, s, g, in, out, texel, order);
     return check_launch("gi temporal");
 }
 
@@ -385,10 +509,9 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
 {
     dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    if (tuning.gi_spatial_waves == 4)
-        hipLaunchKernelGGL(k_gi_spatial_w4, blocks, dim3(256), 0, stream, f, s, g, in, res, color, moments, texel);
-    else
-        hipLaunchKernelGGL(k_gi_spatial, blocks, dim3(256), 0, stream, f, s, g, in, res, color, moments, texel);
+    auto kernel = tuning.gi_spatial_waves == 4 ? k_gi_spatial_w4 : k_gi_spatial;
+    hipLaunchKernelGGL(kernel, blocks, dim3(256), 0, stream, f, s, g, in, res, color, moments, texel,
+                       (uint32_t)tuning.tile_order);
     return check_launch("gi spatial");
 }
 

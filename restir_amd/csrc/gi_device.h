@@ -148,68 +148,123 @@ RGI_DEV F3 previous_indirect(const GiPass& p, F3 position, F3 n, F3 albedo)
     return xyz(fetch_h4(p.in.history_color, p, uv)) * albedo;
 }
 
-RGI_DEV float depth_behind(const GiPass& p, F3 ray_position, F3& surface, bool& visible)
+// One ray of traceScreenSpaceRay's march, advanced a step at a time so that
+// several rays can march in lockstep (independent fetch chains per lane).
+// The point at distance t is origin + dir * t; its clip position is
+// clip_origin + t * clip_dir (x, y, w kept), the projection distributed over
+// the ray.
+struct MarchRay {
+    F3 origin;
+    F3 dir;
+    float co_x, co_y, co_w, cd_x, cd_y, cd_w;
+    float max_distance;
+    float previous;
+    bool previous_front;
+    bool active;
+    bool hit;
+    F3 hit_position;
+};
+
+// depthBehindSurface (restir.sh:327-358) at distance t along the ray.
+RGI_DEV float depth_behind(const GiPass& p, const MarchRay& m, float t, F3& surface, bool& visible)
 {
-    float ray_depth;
-    F2 uv;
     surface = splat3(0.0f);
-    visible = project_to_screen(*p.f, ray_position, uv, ray_depth);
+    visible = false;
+    float cw = fmaf(m.cd_w, t, m.co_w);
+    if (cw <= 0.0001f) return -1.0f;
+    F2 uv = clip_to_uv(*p.f, F4{fmaf(m.cd_x, t, m.co_x), fmaf(m.cd_y, t, m.co_y), 0.0f, cw});
+    visible = on_screen(uv);
     if (!visible) return -1.0f;
-    uint32_t t = tix(p, uv);
-    float depth = p.g.depth[t];
+    uint32_t ti = tix(p, uv);
+    float depth = p.g.depth[ti];
     if (depth >= kSkyDepth) return -1.0f;
     surface = reconstruct_position(*p.f, uv, depth);
-    float behind = ray_depth - mul_w(p.f->view_proj, surface);
+    float behind = cw - mul_w(p.f->view_proj, surface);
     if (behind > 0.0f) {
-        F4 nr = unorm8(p.g.normal[t]);
+        F4 nr = unorm8(p.g.normal[ti]);
         F3 sn = normalize(F3{nr.x * 2.0f - 1.0f, nr.y * 2.0f - 1.0f, nr.z * 2.0f - 1.0f});
+        F3 ray_position = m.origin + m.dir * t;
         if (dot(ray_position - surface, sn) > -(0.01f * p.sc)) return -1.0f;
     }
     return behind;
 }
 
-// The depth-buffer march of traceScreenSpaceRay: quadratic steps, then five
-// bisection steps on the first crossing.
-RGI_DEV bool march(const GiPass& p, F3 origin, F3 dir, float max_distance, F3& hit_position)
+RGI_DEV void march_begin(const GiPass& p, MarchRay& m, F3 origin, F3 dir, float max_distance, bool active)
 {
-    hit_position = splat3(0.0f);
-    float previous = 0.0f;
-    bool previous_front = true;
-    int steps = p.f->trace_steps;
-    float step_scale = 32.0f / (float)steps;
-    for (int i = 1; i <= steps; ++i) {
-        if (previous >= max_distance) break;
-        float sp = (float)i * step_scale;
-        float distance = minf_((0.02f + (sp * sp) * 0.0035f) * p.sc, max_distance);
-        F3 surface;
-        bool visible;
-        float behind = depth_behind(p, origin + dir * distance, surface, visible);
-        if (!visible) return false;
-        bool front = behind <= 0.0f;
-        if (!front && previous_front) {
-            float near_d = previous, far_d = distance;
-            for (int r = 0; r < 5; ++r) {
-                float middle = 0.5f * (near_d + far_d);
-                F3 ms;
-                bool mv;
-                float mb = depth_behind(p, origin + dir * middle, ms, mv);
-                if (mv && mb > 0.0f) {
-                    far_d = middle;
-                    behind = mb;
-                    surface = ms;
-                } else {
-                    near_d = middle;
-                }
-            }
-            if (behind < 0.03f * p.sc + 0.03f * far_d) {
-                hit_position = surface;
-                return true;
+    m.origin = origin;
+    m.dir = dir;
+    F4 co = mul(p.f->view_proj, F4{origin.x, origin.y, origin.z, 1.0f});
+    F4 cd = mul_dir(p.f->view_proj, dir);
+    m.co_x = co.x;
+    m.co_y = co.y;
+    m.co_w = co.w;
+    m.cd_x = cd.x;
+    m.cd_y = cd.y;
+    m.cd_w = cd.w;
+    m.max_distance = max_distance;
+    m.previous = 0.0f;
+    m.previous_front = true;
+    m.active = active;
+    m.hit = false;
+    m.hit_position = splat3(0.0f);
+}
+
+// Step i (1-based) of the march: quadratic steps, five bisection steps on
+// the first crossing; the ray stops on a hit, on leaving the frustum, or
+// past its max distance.
+RGI_DEV void march_step(const GiPass& p, MarchRay& m, int i, float step_scale)
+{
+    if (!m.active) return;
+    if (m.previous >= m.max_distance) {
+        m.active = false;
+        return;
+    }
+    float sp = (float)i * step_scale;
+    float distance = minf_((0.02f + (sp * sp) * 0.0035f) * p.sc, m.max_distance);
+    F3 surface;
+    bool visible;
+    float behind = depth_behind(p, m, distance, surface, visible);
+    if (!visible) {
+        m.active = false;
+        return;
+    }
+    bool front = behind <= 0.0f;
+    if (!front && m.previous_front) {
+        float near_d = m.previous, far_d = distance;
+        for (int r = 0; r < 5; ++r) {
+            float middle = 0.5f * (near_d + far_d);
+            F3 ms;
+            bool mv;
+            float mb = depth_behind(p, m, middle, ms, mv);
+            if (mv && mb > 0.0f) {
+                far_d = middle;
+                behind = mb;
+                surface = ms;
+            } else {
+                near_d = middle;
             }
         }
-        previous_front = front;
-        previous = distance;
+        if (behind < 0.03f * p.sc + 0.03f * far_d) {
+            m.hit = true;
+            m.hit_position = surface;
+            m.active = false;
+            return;
+        }
     }
-    return false;
+    m.previous_front = front;
+    m.previous = distance;
+}
+
+// The depth-buffer march of traceScreenSpaceRay for one ray.
+RGI_DEV bool march(const GiPass& p, F3 origin, F3 dir, float max_distance, F3& hit_position)
+{
+    MarchRay m;
+    march_begin(p, m, origin, dir, max_distance, true);
+    int steps = p.f->trace_steps;
+    float step_scale = 32.0f / (float)steps;
+    for (int i = 1; i <= steps && m.active; ++i) march_step(p, m, i, step_scale);
+    hit_position = m.hit_position;
+    return m.hit;
 }
 
 // evaluateSample: the integrand of a sample point at a receiver.

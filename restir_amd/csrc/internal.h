@@ -14,6 +14,8 @@ struct Tuning {
     int gi_temporal_waves = 0;
     int gi_spatial_waves = 0;
     int trace_persistent = 0;
+    int gi_temporal_lockstep = 0;
+    int tile_order = 0;
 };
 } // namespace rgi
 
@@ -51,8 +53,8 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
                const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
                hipStream_t stream);
 int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
-                const rgi_di_reservoirs& out, hipStream_t stream);
+                const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream);
 int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
-               uint16_t* diffuse, uint16_t* specular, hipStream_t stream);
+               uint16_t* diffuse, uint16_t* specular, const Tuning& tuning, hipStream_t stream);
 
 } // namespace rgi

@@ -275,8 +275,9 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     if (di_active) {
         rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
         rgi_di_reservoirs next_di{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
-        if ((e = di_temporal(f, r->scene, *g, prev_di, next_di, stream)) != RGI_OK) return e;
-        if ((e = di_spatial(f, r->scene, *g, next_di, r->diffuse, r->specular, stream)) != RGI_OK) return e;
+        if ((e = di_temporal(f, r->scene, *g, prev_di, next_di, r->ctx->tuning, stream)) != RGI_OK) return e;
+        if ((e = di_spatial(f, r->scene, *g, next_di, r->diffuse, r->specular, r->ctx->tuning, stream)) != RGI_OK)
+            return e;
         surface_light = r->diffuse;
         r->di_valid = true;
     } else {
@@ -321,12 +322,17 @@ int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuf
     hipEvent_t start, stop;
     if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess)
         return fail(RGI_E_DEVICE, "autotune: event creation failed");
-    float best = 3.4e38f;
+    auto apply = [&](int config) {
+        rgi::Tuning& t = r->ctx->tuning;
+        t.gi_temporal_waves = (config & RGI_TUNE_TEMPORAL_WAVES4) ? 4 : 0;
+        t.gi_spatial_waves = (config & RGI_TUNE_SPATIAL_WAVES4) ? 4 : 0;
+        t.gi_temporal_lockstep = (config & RGI_TUNE_TEMPORAL_LOCKSTEP) ? 1 : 0;
+        t.tile_order = (config & RGI_TUNE_TILE_ORDER) ? 1 : 0;
+    };
     int chosen = 0, e = RGI_OK;
     uint32_t k = 0;
-    for (int t = 0; t < RGI_TUNE_TRIALS && e == RGI_OK; ++t) {
-        r->ctx->tuning.gi_temporal_waves = (t & 1) ? 4 : 0;
-        r->ctx->tuning.gi_spatial_waves = (t & 2) ? 4 : 0;
+    auto time_config = [&](int config, float& ms) {
+        apply(config);
         // One untimed frame so each variant's code object is loaded and warm.
         e = rgi_renderer_render(r, &views[k % count], &gbuffers[k % count], stream_ptr);
         ++k;
@@ -335,20 +341,37 @@ int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuf
             e = rgi_renderer_render(r, &views[k % count], &gbuffers[k % count], stream_ptr);
         hipEventRecord(stop, stream);
         hipEventSynchronize(stop);
-        float ms = 0.0f;
+        ms = 0.0f;
         hipEventElapsedTime(&ms, start, stop);
         ms /= (float)frames;
-        if (out_ms) out_ms[t] = ms;
-        if (ms < best) {
-            best = ms;
-            chosen = t;
+    };
+    // Knob t: the bits it owns and the values it tries.
+    const int knob_mask[3] = {RGI_TUNE_TILE_ORDER, RGI_TUNE_TEMPORAL_WAVES4 | RGI_TUNE_TEMPORAL_LOCKSTEP,
+                              RGI_TUNE_SPATIAL_WAVES4};
+    const int knob_values[3][4] = {{0, RGI_TUNE_TILE_ORDER, -1, -1},
+                                   {0, RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_TEMPORAL_LOCKSTEP,
+                                    RGI_TUNE_TEMPORAL_WAVES4 | RGI_TUNE_TEMPORAL_LOCKSTEP},
+                                   {0, RGI_TUNE_SPATIAL_WAVES4, -1, -1}};
+    int trial = 0;
+    for (int knob = 0; knob < 3 && e == RGI_OK; ++knob) {
+        float best = 3.4e38f;
+        int best_config = chosen;
+        for (int v = 0; v < 4 && knob_values[knob][v] >= 0 && e == RGI_OK; ++v, ++trial) {
+            int config = (chosen & ~knob_mask[knob]) | knob_values[knob][v];
+            float ms;
+            time_config(config, ms);
+            if (out_ms) out_ms[trial] = ms;
+            if (ms < best) {
+                best = ms;
+                best_config = config;
+            }
         }
+        chosen = best_config;
     }
     hipEventDestroy(start);
     hipEventDestroy(stop);
     if (e != RGI_OK) return e;
-    r->ctx->tuning.gi_temporal_waves = (chosen & 1) ? 4 : 0;
-    r->ctx->tuning.gi_spatial_waves = (chosen & 2) ? 4 : 0;
+    apply(chosen);
     return chosen;
 }
 

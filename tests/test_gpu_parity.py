@@ -5,6 +5,7 @@ the kernels evaluate each expression in the oracle's order with FP
 contraction off. Should a float plane ever differ, the test reports the
 fraction of texels outside the 1e-4 relative bound as well.
 """
+import contextlib
 import ctypes
 
 import numpy as np
@@ -289,9 +290,22 @@ def oracle_gi_spatial(hs, f, g, hist, res, grid, surface_light):
     return {"color": oc, "moments": om}
 
 
+LOCKSTEP = {ra.RGI_OPT_GI_TEMPORAL_LOCKSTEP: 1}
+XCD_TILES = {ra.RGI_OPT_TILE_ORDER: 1}
+
+# Per case: settings overrides, and "options" = launch variants of the
+# context (temporal waves also select spatial waves in the spatial test).
 GI_CASES = {
     "default": {},
-    "default_waves4": {"waves": 4},
+    "default_waves4": {"options": {ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
+    "default_lockstep": {"options": LOCKSTEP},
+    "default_lockstep_waves4_xcd": {"options": {**LOCKSTEP, **XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4,
+                                                ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
+    "default_xcd_tiles": {"options": XCD_TILES},
+    "bvh_rays_lockstep": {"ray_mode": ra.RGI_RAY_BVH, "options": LOCKSTEP},
+    "screen_rays_lockstep": {"ray_mode": ra.RGI_RAY_SCREEN, "options": LOCKSTEP},
+    "three_candidates_lockstep": {"initial_candidates": 3, "options": LOCKSTEP},
+    "sky_gi_only_lockstep": {"sky": True, "restir_di": False, "options": LOCKSTEP},
     "bvh_rays": {"ray_mode": ra.RGI_RAY_BVH},
     "screen_rays": {"ray_mode": ra.RGI_RAY_SCREEN},
     "screen_reuse": {"world_space_reuse": False},
@@ -304,7 +318,7 @@ GI_CASES = {
 
 def case_frame(hs, frame, **kw):
     """`frame` with per-case overrides of the settings-derived fields."""
-    kw.pop("waves", None)
+    kw.pop("options", None)
     f = ra.Frame.from_buffer_copy(frame)
     s = ra.Settings(**kw)
     ref = ra.make_frame(s, hs, f.frame_index, list(f.camera_position)[:3], list(f.view_proj), list(f.inv_view_proj),
@@ -312,7 +326,7 @@ def case_frame(hs, frame, **kw):
                         f.di_history_valid)
     for name in ("ray_mode", "world_space_reuse", "sky_zenith", "lighting_surface", "lighting_emission",
                  "jacobian_clamp", "outlier_clamp", "boiling_filter", "light_color", "light_position",
-                 "shadows_enabled", "grid_table_size", "di_light_count"):
+                 "shadows_enabled", "grid_table_size", "di_light_count", "initial_candidates"):
         setattr(f, name, getattr(ref, name))
     if f.shadows_enabled:
         f.shadow_width, f.shadow_height = 96, 64
@@ -329,6 +343,18 @@ def case_frame(hs, frame, **kw):
 def shadow_atlas(f):
     rng = np.random.default_rng(17)
     return (rng.random(f.shadow_width * f.shadow_height) * 0.8).astype(np.float32)
+
+
+@contextlib.contextmanager
+def options(dev, values):
+    """Context launch variants set for the body of the block, then reset."""
+    try:
+        for opt, value in values.items():
+            ra.check(dev.L.rgi_context_set_option(dev.ctx, opt, value))
+        yield
+    finally:
+        for opt in values:
+            ra.check(dev.L.rgi_context_set_option(dev.ctx, opt, 0))
 
 
 @pytest.mark.parametrize("case", sorted(GI_CASES))
@@ -348,12 +374,8 @@ def test_gi_temporal(dev, cornell_seq, case):
         hist["shadow"] = shadow_atlas(f)
     sl = out["surface_light"] if f.lighting_surface > 0 and f.di_light_count > 0 else g["emission"]
     want = oracle_gi_temporal(hs, f, g, hist, sl)
-    waves = GI_CASES[case].get("waves", 0)
-    ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_TEMPORAL_WAVES, waves))
-    try:
+    with options(dev, GI_CASES[case].get("options", {})):
         got = gpu_gi_temporal(dev, hs, f, g, hist, sl)
-    finally:
-        ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_TEMPORAL_WAVES, 0))
     working = (want["gi_receiver"][:, 3] > 0).mean()
     say("  W>0 fraction %.3f" % working)
     for k in want:
@@ -376,12 +398,8 @@ def test_gi_spatial(dev, cornell_seq, case):
     res = {k: out[k] for k in ("gi_sample", "gi_receiver", "gi_normal")}
     sl = out["surface_light"]
     want = oracle_gi_spatial(hs, f, g, hist, res, grid, sl)
-    waves = GI_CASES[case].get("waves", 0)
-    ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_SPATIAL_WAVES, waves))
-    try:
+    with options(dev, GI_CASES[case].get("options", {})):
         got = gpu_gi_spatial(dev, hs, f, g, hist, res, grid, sl)
-    finally:
-        ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_GI_SPATIAL_WAVES, 0))
     say("  color mean %s" % halfs(want["color"])[:, :3].mean(0))
     for k in want:
         assert_same(k, got[k], want[k])
@@ -420,13 +438,14 @@ def di_both(dev, hs, f, g, prev):
     return got, w
 
 
-@pytest.mark.parametrize("frame_no,extra", [(0, 4), (2, 4), (2, 0)])
-def test_di(dev, frame_no, extra):
+@pytest.mark.parametrize("frame_no,extra,order", [(0, 4, 0), (2, 4, 0), (2, 0, 0), (2, 4, 1)])
+def test_di(dev, frame_no, extra, order):
     hs = common.cornell(extra_lights=extra)
     s = ra.Settings()
     seq = common.oracle_sequence(hs, s, W, H, frame_no + 1, yaw_step=2 * np.pi / 64)
     f, g, hist, out = seq[frame_no]
-    say("case di frame=%d extra=%d lights=%d" % (frame_no, extra, hs.light_count))
-    got, want = di_both(dev, hs, f, g, hist)
+    say("case di frame=%d extra=%d lights=%d tile_order=%d" % (frame_no, extra, hs.light_count, order))
+    with options(dev, {ra.RGI_OPT_TILE_ORDER: order}):
+        got, want = di_both(dev, hs, f, g, hist)
     for k in want:
         assert_same(k, got[k], want[k], rel=1e-4)

@@ -42,9 +42,14 @@ def download(torch, ptr, like):
     return out
 
 
-@pytest.mark.parametrize("frames,kw", [(4, {}), (3, {"restir_di": False, "sky": True}), (3, {"ray_mode": ra.RGI_RAY_BVH})])
-def test_renderer_matches_oracle_sequence(torch, frames, kw):
-    say("case renderer frames=%d %s" % (frames, kw))
+ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES4 | ra.RGI_TUNE_TEMPORAL_LOCKSTEP
+                             | ra.RGI_TUNE_TILE_ORDER)
+
+
+@pytest.mark.parametrize("frames,kw,variants", [(4, {}, {}), (3, {"restir_di": False, "sky": True}, {}),
+                                                (3, {"ray_mode": ra.RGI_RAY_BVH}, {}), (4, {}, ALL_VARIANTS)])
+def test_renderer_matches_oracle_sequence(torch, frames, kw, variants):
+    say("case renderer frames=%d %s variants=%s" % (frames, kw, variants))
     hs = common.cornell(extra_lights=4)
     s = ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC, **kw)
     yaw = 2 * np.pi / 64
@@ -57,6 +62,8 @@ def test_renderer_matches_oracle_sequence(torch, frames, kw):
                                    hs.radius))
     stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     try:
+        for opt, value in variants.items():
+            ra.check(L.rgi_renderer_set_option(r, opt, value))
         for i, (f, g, hist, want) in enumerate(seq):
             eye, vp, inv = ra.orbit_camera(hs, i, W, H, yaw_step=yaw)
             view = ra.make_view(eye, vp, inv)
