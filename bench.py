@@ -199,7 +199,6 @@ def main():
     options = ra.tuning_of(chosen)
     tuning = {"gi_temporal_waves": options[ra.RGI_OPT_GI_TEMPORAL_WAVES],
               "gi_spatial_waves": options[ra.RGI_OPT_GI_SPATIAL_WAVES],
-              "gi_temporal_lockstep": options[ra.RGI_OPT_GI_TEMPORAL_LOCKSTEP],
               "tile_order": options[ra.RGI_OPT_TILE_ORDER],
               "trial_ms": [round(float(v), 5) for v in tune_ms]}
     for k in range(args.warmup):

@@ -180,8 +180,7 @@ void rgi_destroy(rgi_context* context);
 #define RGI_OPT_GI_TEMPORAL_WAVES 1 /* occupancy target of GI temporal: 0 (compiler's choice) or 4 waves per SIMD */
 #define RGI_OPT_GI_SPATIAL_WAVES 2  /* the same for GI spatial */
 #define RGI_OPT_TRACE_PERSISTENT 3  /* rgi_trace: 0 one ray per lane, 1 persistent 64-ray batches per wave */
-#define RGI_OPT_GI_TEMPORAL_LOCKSTEP 4 /* GI temporal: 1 marches two candidates in lockstep, one shared miss queue */
-#define RGI_OPT_TILE_ORDER 5 /* screen passes: 0 row-major tiles, 1 a contiguous band of tiles per XCD */
+#define RGI_OPT_TILE_ORDER 4 /* screen passes: 0 row-major tiles, 1 a contiguous band of tiles per XCD */
 int rgi_context_set_option(rgi_context* context, int option, int value);
 int rgi_context_get_option(const rgi_context* context, int option);
 

@@ -22,9 +22,9 @@ RGI_RAY_SCREEN, RGI_RAY_HYBRID, RGI_RAY_BVH = 0, 1, 2
 RGI_GRID_ATOMIC, RGI_GRID_DETERMINISTIC = 0, 1
 RGI_SCENE_CORNELL, RGI_SCENE_ATRIUM, RGI_SCENE_BOXFIELD = 0, 1, 2
 RGI_OPT_GI_TEMPORAL_WAVES, RGI_OPT_GI_SPATIAL_WAVES, RGI_OPT_TRACE_PERSISTENT = 1, 2, 3
-RGI_OPT_GI_TEMPORAL_LOCKSTEP, RGI_OPT_TILE_ORDER = 4, 5
-RGI_TUNE_TRIALS = 8
-RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4, RGI_TUNE_TEMPORAL_LOCKSTEP, RGI_TUNE_TILE_ORDER = 1, 2, 4, 8
+RGI_OPT_TILE_ORDER = 4
+RGI_TUNE_TRIALS = 6
+RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4, RGI_TUNE_TILE_ORDER = 1, 2, 4
 
 
 def tuning_of(config):
@@ -32,7 +32,6 @@ def tuning_of(config):
     return {
         RGI_OPT_GI_TEMPORAL_WAVES: 4 if config & RGI_TUNE_TEMPORAL_WAVES4 else 0,
         RGI_OPT_GI_SPATIAL_WAVES: 4 if config & RGI_TUNE_SPATIAL_WAVES4 else 0,
-        RGI_OPT_GI_TEMPORAL_LOCKSTEP: 1 if config & RGI_TUNE_TEMPORAL_LOCKSTEP else 0,
         RGI_OPT_TILE_ORDER: 1 if config & RGI_TUNE_TILE_ORDER else 0,
     }
 

@@ -146,10 +146,9 @@ int rgi_context_set_option(rgi_context* ctx, int option, int value)
             return fail(RGI_E_NOMEM, "work counter allocation failed");
         ctx->tuning.trace_persistent = value;
         return RGI_OK;
-    case RGI_OPT_GI_TEMPORAL_LOCKSTEP:
     case RGI_OPT_TILE_ORDER:
-        if (value != 0 && value != 1) return fail(RGI_E_INVALID, "option %d takes 0 or 1, not %d", option, value);
-        (option == RGI_OPT_TILE_ORDER ? ctx->tuning.tile_order : ctx->tuning.gi_temporal_lockstep) = value;
+        if (value != 0 && value != 1) return fail(RGI_E_INVALID, "tile order takes 0 or 1, not %d", value);
+        ctx->tuning.tile_order = value;
         return RGI_OK;
     default:
         return fail(RGI_E_INVALID, "unknown option %d", option);
@@ -163,7 +162,6 @@ int rgi_context_get_option(const rgi_context* ctx, int option)
     case RGI_OPT_GI_TEMPORAL_WAVES: return ctx->tuning.gi_temporal_waves;
     case RGI_OPT_GI_SPATIAL_WAVES: return ctx->tuning.gi_spatial_waves;
     case RGI_OPT_TRACE_PERSISTENT: return ctx->tuning.trace_persistent;
-    case RGI_OPT_GI_TEMPORAL_LOCKSTEP: return ctx->tuning.gi_temporal_lockstep;
     case RGI_OPT_TILE_ORDER: return ctx->tuning.tile_order;
     default: return fail(RGI_E_INVALID, "unknown option %d", option);
     }

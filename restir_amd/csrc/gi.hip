@@ -39,15 +39,14 @@ RGI_DEV void group_pixel(uint32_t order, int& x, int& y)
     y = (int)(ty * 16u + (group >> 2) * 8u + (t >> 2));
 }
 
-// Per-block LDS of the temporal pass: the closest-hit stack rings and a ray
-// queue of kQueue rays per lane.
-template <int kQueue>
+// Per-block LDS of the temporal pass: the closest-hit stack rings and the
+// per-wave ray queue.
 struct TemporalLds {
     int ring[kStackRing * 256];
-    float4 q_origin[256 * kQueue];
-    float4 q_dir[256 * kQueue];
-    int q_tri[256 * kQueue];
-    float q_dist[256 * kQueue];
+    float4 q_origin[256];
+    float4 q_dir[256];
+    int q_tri[256];
+    float q_dist[256];
 };
 
 // Source density of a candidate hit, converted from the ray origin to the
@@ -77,116 +76,8 @@ RGI_DEV void stream_candidate(const GiPass& p, F3 position, F3 n, F3 albedo, F3 
     }
 }
 
-// The candidate loop with the depth-buffer marches of kChunk candidates
-// advancing in lockstep (independent fetch chains in flight per lane) and
-// all their misses traced from one per-wave queue. Randoms are drawn and
-// reservoir updates applied in candidate order, so the result is the same
-// as the one-candidate-at-a-time loop.
-constexpr int kChunk = 2;
-
-RGI_DEV void candidates_lockstep(const GiPass& p, const rgi_frame& f, F3 position, F3 n, F3 albedo, F3 ray_origin,
-                                 bool working, uint32_t& rng, Res& canonical, TemporalLds<kChunk>& lds)
-{
-    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    float4* q_origin = lds.q_origin;
-    float4* q_dir = lds.q_dir;
-    int* q_tri = lds.q_tri;
-    float* q_dist = lds.q_dist;
-    const int steps = f.trace_steps;
-    const float step_scale = 32.0f / (float)steps;
-    for (int base = 0; base < f.initial_candidates; base += kChunk) {
-        int count = f.initial_candidates - base < kChunk ? f.initial_candidates - base : kChunk;
-        F3 dir[kChunk];
-        float rsel[kChunk];
-        MarchRay m[kChunk];
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-            if (j < count) {
-                float r0 = next_random(rng);
-                float r1 = next_random(rng);
-                rsel[j] = next_random(rng);
-                dir[j] = cosine_lobe(n, r0, r1);
-                canonical.count += 1.0f;
-                march_begin(p, m[j], ray_origin, dir[j], 3.6f * p.sc, working && f.ray_mode != RGI_RAY_BVH);
-            } else {
-                dir[j] = splat3(0.0f);
-                rsel[j] = 0.0f;
-                march_begin(p, m[j], ray_origin, dir[j], 0.0f, false);
-            }
-        }
-        for (int i = 1; i <= steps; ++i) {
-            bool any = false;
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) {
-                march_step(p, m[j], i, step_scale);
-                any = any || m[j].active;
-            }
-            if (!any) break;
-        }
-        // One queue for the chunk's misses: slot = rays of earlier
-        // candidates, then rank within this candidate.
-        bool need[kChunk];
-        uint32_t slot[kChunk];
-        float hit_id[kChunk];
-        uint32_t queued = 0;
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-            need[j] = j < count && working && !m[j].hit && f.ray_mode != RGI_RAY_SCREEN;
-            hit_id[j] = 0.0f;
-            uint64_t mask = __ballot(need[j]);
-            slot[j] = queued + rank_in(mask);
-            queued += (uint32_t)__popcll(mask);
-        }
-        if (queued != 0u) {
-            uint32_t qbase = wave * (64u * kChunk);
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) {
-                if (need[j]) {
-                    q_origin[qbase + slot[j]] = make_float4(ray_origin.x, ray_origin.y, ray_origin.z, 0.0f);
-                    q_dir[qbase + slot[j]] = make_float4(dir[j].x, dir[j].y, dir[j].z, 0.0f);
-                }
-            }
-            wave_sync();
-            for (uint32_t q = lane; q < queued; q += 64u) {
-                float4 qo = q_origin[qbase + q], qd = q_dir[qbase + q];
-                float dist;
-                int tri = trace_closest(p.v, F3{qo.x, qo.y, qo.z}, F3{qd.x, qd.y, qd.z}, 4.0f * p.sc, dist,
-                                        lds.ring + threadIdx.x, 256);
-                q_tri[qbase + q] = tri;
-                q_dist[qbase + q] = dist;
-            }
-            wave_sync();
-#pragma unroll
-            for (int j = 0; j < kChunk; ++j) {
-                if (need[j]) {
-                    int tri = q_tri[qbase + slot[j]];
-                    float dist = q_dist[qbase + slot[j]];
-                    if (tri >= 0) {
-                        m[j].hit = true;
-                        m[j].hit_position = ray_origin + dir[j] * dist;
-                        hit_id[j] = sample_id_of(p.v, tri, dir[j]);
-                    } else if (sky_on(f)) {
-                        m[j].hit = true;
-                        m[j].hit_position = dir[j];
-                        hit_id[j] = kSkySampleId;
-                    }
-                }
-            }
-            wave_sync(); // the queue is free again before the next chunk
-        }
-#pragma unroll
-        for (int j = 0; j < kChunk; ++j) {
-            if (j < count)
-                stream_candidate(p, position, n, albedo, ray_origin, dir[j], m[j].hit, m[j].hit_position, hit_id[j],
-                                 rsel[j], canonical);
-        }
-    }
-}
-
-template <bool kLockstep>
 RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                              const rgi_gi_reservoirs& out, F2 texel, uint32_t order,
-                              TemporalLds<kLockstep ? kChunk : 1>& lds)
+                              const rgi_gi_reservoirs& out, F2 texel, uint32_t order, TemporalLds& lds)
 {
     int* ring = lds.ring;
     float4* q_origin = lds.q_origin;
@@ -219,9 +110,6 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     uint32_t rng = pcg((uint32_t)to_int(gx) + pcg((uint32_t)to_int(gy) + pcg(f.frame_index)));
 
     Res canonical = empty_res();
-    if constexpr (kLockstep) {
-        candidates_lockstep(p, f, position, n, albedo, ray_origin, working, rng, canonical, lds);
-    } else {
     for (int k = 0; k < f.initial_candidates; ++k) {
         float r0 = next_random(rng);
         float r1 = next_random(rng);
@@ -267,7 +155,6 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
             wave_sync();
         }
         stream_candidate(p, position, n, albedo, ray_origin, dir, hit, hit_position, hit_id, rsel, canonical);
-    }
     }
     float canonical_weight = canonical.target > 0.000001f ? canonical.wsum / (canonical.count * canonical.target) : 0.0f;
 
@@ -460,19 +347,23 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
 }
 
 // The kernels: each pass with the compiler's occupancy, and capped at
-// 128 VGPRs for 4 waves per SIMD; the temporal pass also in its lockstep form.
-#define RGI_TEMPORAL_KERNEL(name, lockstep, attrs)                                                              \
-    __global__ void __launch_bounds__(256) attrs name(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, \
-                                                      rgi_gi_reservoirs out, F2 texel, uint32_t order)          \
-    {                                                                                                          \
-        __shared__ TemporalLds<lockstep ? kChunk : 1> lds;                                                     \
-        gi_temporal_body<lockstep>(f, s, g, in, out, texel, order, lds);                                       \
-    }
+// 128 VGPRs for 4 waves per SIMD.
 #define RGI_WAVES4 __attribute__((amdgpu_waves_per_eu(4, 4)))
-RGI_TEMPORAL_KERNEL(k_gi_temporal, false, )
-RGI_TEMPORAL_KERNEL(k_gi_temporal_w4, false, RGI_WAVES4)
-RGI_TEMPORAL_KERNEL(k_gi_temporal_lockstep, true, )
-RGI_TEMPORAL_KERNEL(k_gi_temporal_lockstep_w4, true, RGI_WAVES4)
+
+__global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
+                                                     rgi_gi_reservoirs out, F2 texel, uint32_t order)
+{
+    __shared__ TemporalLds lds;
+    gi_temporal_body(f, s, g, in, out, texel, order, lds);
+}
+
+__global__ void __launch_bounds__(256) RGI_WAVES4
+k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs out, F2 texel,
+                 uint32_t order)
+{
+    __shared__ TemporalLds lds;
+    gi_temporal_body(f, s, g, in, out, texel, order, lds);
+}
 
 __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                     rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
@@ -495,11 +386,8 @@ int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
 {
     dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    uint32_t order = (uint32_t)tuning.tile_order;
-    bool w4 = tuning.gi_temporal_waves == 4;
-    auto kernel = tuning.gi_temporal_lockstep ? (w4 ? k_gi_temporal_lockstep_w4 : k_gi_temporal_lockstep)
-                                              : (w4 ? k_gi_temporal_w4 : k_gi_temporal);
-    hipLaunchKernelGGL(kernel, blocks, dim3(256), 0, stream, f, s, g, in, out, texel, order);
+    auto kernel = tuning.gi_temporal_waves == 4 ? k_gi_temporal_w4 : k_gi_temporal;
+    hipLaunchKernelGGL(kernel, blocks, dim3(256), 0, stream, f, s, g, in, out, texel, (uint32_t)tuning.tile_order);
     return check_launch("gi temporal");
 }
 

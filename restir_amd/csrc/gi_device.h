@@ -148,25 +148,24 @@ RGI_DEV F3 previous_indirect(const GiPass& p, F3 position, F3 n, F3 albedo)
     return xyz(fetch_h4(p.in.history_color, p, uv)) * albedo;
 }
 
-// One ray of traceScreenSpaceRay's march, advanced a step at a time so that
-// several rays can march in lockstep (independent fetch chains per lane).
-// The point at distance t is origin + dir * t; its clip position is
-// clip_origin + t * clip_dir (x, y, w kept), the projection distributed over
-// the ray.
-struct MarchRay {
+// A marched ray of traceScreenSpaceRay: the point at distance t is
+// origin + dir * t; its clip position is clip_origin + t * clip_dir (x, y
+// and w kept), the projection distributed over the ray.
+struct ClipRay {
     F3 origin;
     F3 dir;
     float co_x, co_y, co_w, cd_x, cd_y, cd_w;
-    float max_distance;
-    float previous;
-    bool previous_front;
-    bool active;
-    bool hit;
-    F3 hit_position;
 };
 
+RGI_DEV ClipRay clip_ray(const GiPass& p, F3 origin, F3 dir)
+{
+    F4 co = mul(p.f->view_proj, F4{origin.x, origin.y, origin.z, 1.0f});
+    F4 cd = mul_dir(p.f->view_proj, dir);
+    return ClipRay{origin, dir, co.x, co.y, co.w, cd.x, cd.y, cd.w};
+}
+
 // depthBehindSurface (restir.sh:327-358) at distance t along the ray.
-RGI_DEV float depth_behind(const GiPass& p, const MarchRay& m, float t, F3& surface, bool& visible)
+RGI_DEV float depth_behind(const GiPass& p, const ClipRay& m, float t, F3& surface, bool& visible)
 {
     surface = splat3(0.0f);
     visible = false;
@@ -189,82 +188,84 @@ RGI_DEV float depth_behind(const GiPass& p, const MarchRay& m, float t, F3& surf
     return behind;
 }
 
-RGI_DEV void march_begin(const GiPass& p, MarchRay& m, F3 origin, F3 dir, float max_distance, bool active)
-{
-    m.origin = origin;
-    m.dir = dir;
-    F4 co = mul(p.f->view_proj, F4{origin.x, origin.y, origin.z, 1.0f});
-    F4 cd = mul_dir(p.f->view_proj, dir);
-    m.co_x = co.x;
-    m.co_y = co.y;
-    m.co_w = co.w;
-    m.cd_x = cd.x;
-    m.cd_y = cd.y;
-    m.cd_w = cd.w;
-    m.max_distance = max_distance;
-    m.previous = 0.0f;
-    m.previous_front = true;
-    m.active = active;
-    m.hit = false;
-    m.hit_position = splat3(0.0f);
-}
-
-// Step i (1-based) of the march: quadratic steps, five bisection steps on
-// the first crossing; the ray stops on a hit, on leaving the frustum, or
-// past its max distance.
-RGI_DEV void march_step(const GiPass& p, MarchRay& m, int i, float step_scale)
-{
-    if (!m.active) return;
-    if (m.previous >= m.max_distance) {
-        m.active = false;
-        return;
-    }
-    float sp = (float)i * step_scale;
-    float distance = minf_((0.02f + (sp * sp) * 0.0035f) * p.sc, m.max_distance);
-    F3 surface;
-    bool visible;
-    float behind = depth_behind(p, m, distance, surface, visible);
-    if (!visible) {
-        m.active = false;
-        return;
-    }
-    bool front = behind <= 0.0f;
-    if (!front && m.previous_front) {
-        float near_d = m.previous, far_d = distance;
-        for (int r = 0; r < 5; ++r) {
-            float middle = 0.5f * (near_d + far_d);
-            F3 ms;
-            bool mv;
-            float mb = depth_behind(p, m, middle, ms, mv);
-            if (mv && mb > 0.0f) {
-                far_d = middle;
-                behind = mb;
-                surface = ms;
-            } else {
-                near_d = middle;
-            }
-        }
-        if (behind < 0.03f * p.sc + 0.03f * far_d) {
-            m.hit = true;
-            m.hit_position = surface;
-            m.active = false;
-            return;
-        }
-    }
-    m.previous_front = front;
-    m.previous = distance;
-}
-
-// The depth-buffer march of traceScreenSpaceRay for one ray.
+// traceScreenSpaceRay's march (restir.sh:364-436): quadratic steps; on the
+// first step behind the surface after one in front, five bisection steps,
+// and a hit if the bisected depth is within the thickness; the ray stops on
+// a hit, on leaving the frustum, or past its max distance.
+//
+// Written as one depth lookup per loop iteration: a lane is either stepping
+// (round < 0) or in bisection round `round`. Each lane evaluates the same
+// lookups in the same order as the nested step/bisection loops, so the
+// result is the same; a wave then iterates as often as its busiest lane,
+// instead of running every lane's bisections one after another.
 RGI_DEV bool march(const GiPass& p, F3 origin, F3 dir, float max_distance, F3& hit_position)
 {
-    MarchRay m;
-    march_begin(p, m, origin, dir, max_distance, true);
-    int steps = p.f->trace_steps;
-    float step_scale = 32.0f / (float)steps;
-    for (int i = 1; i <= steps && m.active; ++i) march_step(p, m, i, step_scale);
-    hit_position = m.hit_position;
-    return m.hit;
+    ClipRay m = clip_ray(p, origin, dir);
+    const int steps = p.f->trace_steps;
+    const float step_scale = 32.0f / (float)steps;
+    const float sc = p.sc;
+    int i = 1, round = -1;
+    float previous = 0.0f, near_d = 0.0f, far_d = 0.0f, step_d = 0.0f, behind_b = 0.0f;
+    bool previous_front = true, hit = false;
+    F3 surface_b = splat3(0.0f);
+    bool active = i <= steps && !(previous >= max_distance);
+    while (active) {
+        float t;
+        if (round < 0) {
+            float sp = (float)i * step_scale;
+            t = minf_((0.02f + (sp * sp) * 0.0035f) * sc, max_distance);
+        } else {
+            t = 0.5f * (near_d + far_d);
+        }
+        F3 surface;
+        bool visible;
+        float behind = depth_behind(p, m, t, surface, visible);
+        bool next_step = false;
+        if (round < 0) {
+            if (!visible) {
+                active = false;
+            } else {
+                bool front = behind <= 0.0f;
+                if (!front && previous_front) {
+                    near_d = previous;
+                    far_d = t;
+                    step_d = t;
+                    behind_b = behind;
+                    surface_b = surface;
+                    round = 0;
+                } else {
+                    previous_front = front;
+                    previous = t;
+                    next_step = true;
+                }
+            }
+        } else {
+            if (visible && behind > 0.0f) {
+                far_d = t;
+                behind_b = behind;
+                surface_b = surface;
+            } else {
+                near_d = t;
+            }
+            if (++round == 5) {
+                if (behind_b < 0.03f * sc + 0.03f * far_d) {
+                    hit = true;
+                    active = false;
+                } else {
+                    round = -1;
+                    previous_front = false;
+                    previous = step_d;
+                    next_step = true;
+                }
+            }
+        }
+        if (next_step) {
+            ++i;
+            active = i <= steps && !(previous >= max_distance);
+        }
+    }
+    hit_position = hit ? surface_b : splat3(0.0f);
+    return hit;
 }
 
 // evaluateSample: the integrand of a sample point at a receiver.

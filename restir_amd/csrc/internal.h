@@ -14,7 +14,6 @@ struct Tuning {
     int gi_temporal_waves = 0;
     int gi_spatial_waves = 0;
     int trace_persistent = 0;
-    int gi_temporal_lockstep = 0;
     int tile_order = 0;
 };
 } // namespace rgi

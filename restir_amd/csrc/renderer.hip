@@ -326,7 +326,6 @@ int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuf
         rgi::Tuning& t = r->ctx->tuning;
         t.gi_temporal_waves = (config & RGI_TUNE_TEMPORAL_WAVES4) ? 4 : 0;
         t.gi_spatial_waves = (config & RGI_TUNE_SPATIAL_WAVES4) ? 4 : 0;
-        t.gi_temporal_lockstep = (config & RGI_TUNE_TEMPORAL_LOCKSTEP) ? 1 : 0;
         t.tile_order = (config & RGI_TUNE_TILE_ORDER) ? 1 : 0;
     };
     int chosen = 0, e = RGI_OK;
@@ -346,17 +345,13 @@ int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuf
         ms /= (float)frames;
     };
     // Knob t: the bits it owns and the values it tries.
-    const int knob_mask[3] = {RGI_TUNE_TILE_ORDER, RGI_TUNE_TEMPORAL_WAVES4 | RGI_TUNE_TEMPORAL_LOCKSTEP,
-                              RGI_TUNE_SPATIAL_WAVES4};
-    const int knob_values[3][4] = {{0, RGI_TUNE_TILE_ORDER, -1, -1},
-                                   {0, RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_TEMPORAL_LOCKSTEP,
-                                    RGI_TUNE_TEMPORAL_WAVES4 | RGI_TUNE_TEMPORAL_LOCKSTEP},
-                                   {0, RGI_TUNE_SPATIAL_WAVES4, -1, -1}};
+    const int knob_mask[3] = {RGI_TUNE_TILE_ORDER, RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4};
+    const int knob_values[3][2] = {{0, RGI_TUNE_TILE_ORDER}, {0, RGI_TUNE_TEMPORAL_WAVES4}, {0, RGI_TUNE_SPATIAL_WAVES4}};
     int trial = 0;
     for (int knob = 0; knob < 3 && e == RGI_OK; ++knob) {
         float best = 3.4e38f;
         int best_config = chosen;
-        for (int v = 0; v < 4 && knob_values[knob][v] >= 0 && e == RGI_OK; ++v, ++trial) {
+        for (int v = 0; v < 2 && e == RGI_OK; ++v, ++trial) {
             int config = (chosen & ~knob_mask[knob]) | knob_values[knob][v];
             float ms;
             time_config(config, ms);

@@ -290,7 +290,6 @@ def oracle_gi_spatial(hs, f, g, hist, res, grid, surface_light):
     return {"color": oc, "moments": om}
 
 
-LOCKSTEP = {ra.RGI_OPT_GI_TEMPORAL_LOCKSTEP: 1}
 XCD_TILES = {ra.RGI_OPT_TILE_ORDER: 1}
 
 # Per case: settings overrides, and "options" = launch variants of the
@@ -298,14 +297,10 @@ XCD_TILES = {ra.RGI_OPT_TILE_ORDER: 1}
 GI_CASES = {
     "default": {},
     "default_waves4": {"options": {ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
-    "default_lockstep": {"options": LOCKSTEP},
-    "default_lockstep_waves4_xcd": {"options": {**LOCKSTEP, **XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4,
-                                                ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
+    "default_waves4_xcd": {"options": {**XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
     "default_xcd_tiles": {"options": XCD_TILES},
-    "bvh_rays_lockstep": {"ray_mode": ra.RGI_RAY_BVH, "options": LOCKSTEP},
-    "screen_rays_lockstep": {"ray_mode": ra.RGI_RAY_SCREEN, "options": LOCKSTEP},
-    "three_candidates_lockstep": {"initial_candidates": 3, "options": LOCKSTEP},
-    "sky_gi_only_lockstep": {"sky": True, "restir_di": False, "options": LOCKSTEP},
+    "three_candidates": {"initial_candidates": 3},
+    "short_march": {"trace_steps": 5},
     "bvh_rays": {"ray_mode": ra.RGI_RAY_BVH},
     "screen_rays": {"ray_mode": ra.RGI_RAY_SCREEN},
     "screen_reuse": {"world_space_reuse": False},
@@ -326,7 +321,7 @@ def case_frame(hs, frame, **kw):
                         f.di_history_valid)
     for name in ("ray_mode", "world_space_reuse", "sky_zenith", "lighting_surface", "lighting_emission",
                  "jacobian_clamp", "outlier_clamp", "boiling_filter", "light_color", "light_position",
-                 "shadows_enabled", "grid_table_size", "di_light_count", "initial_candidates"):
+                 "shadows_enabled", "grid_table_size", "di_light_count", "initial_candidates", "trace_steps"):
         setattr(f, name, getattr(ref, name))
     if f.shadows_enabled:
         f.shadow_width, f.shadow_height = 96, 64

@@ -42,8 +42,7 @@ def download(torch, ptr, like):
     return out
 
 
-ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES4 | ra.RGI_TUNE_TEMPORAL_LOCKSTEP
-                             | ra.RGI_TUNE_TILE_ORDER)
+ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES4 | ra.RGI_TUNE_TILE_ORDER)
 
 
 @pytest.mark.parametrize("frames,kw,variants", [(4, {}, {}), (3, {"restir_di": False, "sky": True}, {}),
