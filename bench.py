@@ -43,6 +43,25 @@ BYTES_PER_PIXEL = {
 }
 
 
+def measured_traffic(pass_name, options, width, height):
+    """HBM bytes per launch of the kernel variant the autotuner chose, from
+    the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload
+    (calibrated against a copy of known size), or None when none matches."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_cornell1080p.json")
+    if (width, height) != (1920, 1080) or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        summary = json.load(fh)
+    waves = {"gi_temporal": options.get(ra.RGI_OPT_GI_TEMPORAL_WAVES, 0),
+             "gi_spatial": options.get(ra.RGI_OPT_GI_SPATIAL_WAVES, 0)}.get(pass_name, 0)
+    kernel = {"grid_insert": "k_grid_insert_atomic"}.get(pass_name, "k_" + pass_name) + ("_w4" if waves == 4 else "")
+    k = summary["kernels"].get(kernel)
+    if k is None:
+        return None
+    return {"bytes": round(k["fetch_bytes"] + k["write_bytes"]), "fetch_bytes": round(k["fetch_bytes"]),
+            "write_bytes": round(k["write_bytes"]), "kernel": kernel, "source": "profiles/hbm_traffic_cornell1080p.txt"}
+
+
 def surface_rays(hs, count, seed):
     """Cosine rays from points uniform over the scene's area, both sides."""
     rng = np.random.default_rng(seed)
@@ -278,8 +297,12 @@ def main():
     traffic = BYTES_PER_PIXEL[dominant] * W * H
     achieved = traffic / (pass_ms[dominant] * 1e-3) / 1e9
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "algorithmic_bytes": traffic, "avg_ms": round(pass_ms[dominant], 5)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": None, "algorithmic_bytes": traffic, "avg_ms": round(pass_ms[dominant], 5)}
+    measured = measured_traffic(dominant, options, W, H)
+    if measured is not None:
+        roofline["traffic"] = measured.pop("bytes")
+        roofline["traffic_detail"] = measured
 
     mse = None
     if args.reference_frames > 0:
