@@ -147,6 +147,94 @@ def measure_mse(L, rend, gscene, gbuf, view, W, H, ref_frames, frames, ctx, s):
             "image": "GI x diffuse albedo + DI diffuse + DI specular, static camera"}
 
 
+def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames, distance=3.17):
+    """One workload through the C++ frame driver: the swing camera path's
+    G-buffers produced once and resident in HBM, the autotuner, `steps` timed
+    frames (wall clock between device synchronizations), then per-pass HIP
+    event timing of up to 32 frames through the per-pass entry points."""
+    import torch
+    T = settings.grid_table_size
+    gscene = driver.GpuScene(hs)
+    drv = driver.Driver(W, H, T, settings.grid_mode)
+    L = ra.lib()
+    s = driver.stream()
+    cams = [ra.orbit_camera(hs, i, W, H, yaw_of=lambda k: ra.swing_yaw(k, frames_in_path), distance=distance)
+            for i in range(frames_in_path)]
+    views = [ra.make_view(*c) for c in cams]
+    gbufs = []
+    for i in range(frames_in_path):
+        eye, vp, inv = cams[i]
+        f = ra.make_frame(settings, hs, i, eye, vp, inv, eye, vp, W, H, False)
+        gbufs.append(driver.GBufferSet(drv.ctx, gscene, f))
+    torch.cuda.synchronize()
+
+    rend = ctypes.c_void_p()
+    ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                   ctypes.byref(gscene.scene), hs.radius))
+
+    def render(k):
+        i = k % frames_in_path
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[i]), ctypes.byref(gbufs[i].gb), s))
+
+    for k in range(warmup):
+        render(k)
+    # Pick the fastest launch variants on this workload (bit-exact with each other).
+    tune_ms = (ctypes.c_float * ra.RGI_TUNE_TRIALS)()
+    view_arr = (ra.View * frames_in_path)(*views)
+    gb_arr = (ra.GBuffer * frames_in_path)(*[g.gb for g in gbufs])
+    chosen = L.rgi_renderer_autotune(rend, view_arr, gb_arr, frames_in_path, tune_frames, tune_ms, s)
+    if chosen < 0:
+        ra.check(chosen)
+    options = ra.tuning_of(chosen)
+    tuning = {"gi_temporal_waves": options[ra.RGI_OPT_GI_TEMPORAL_WAVES],
+              "gi_spatial_waves": options[ra.RGI_OPT_GI_SPATIAL_WAVES],
+              "tile_order": options[ra.RGI_OPT_TILE_ORDER],
+              "trial_ms": [round(float(v), 5) for v in tune_ms]}
+    for k in range(warmup):
+        render(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        render(k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_frame = elapsed * 1e3 / steps
+
+    def frame_for_step(step):
+        i = step % frames_in_path
+        p = (step - 1) % frames_in_path
+        eye, vp, inv = cams[i]
+        peye, pvp, _ = cams[p]
+        return ra.make_frame(settings, hs, step, eye, vp, inv, peye, pvp, W, H, step > 0, step > 0), i
+
+    frames = [frame_for_step(k) for k in range(warmup + min(steps, 32))]
+    for opt, value in options.items():
+        ra.check(L.rgi_context_set_option(drv.ctx, opt, value))
+    for k in range(warmup):
+        f, i = frames[k]
+        drv.render(f, gscene, gbufs[i])
+    torch.cuda.synchronize()
+
+    # Per-pass durations with HIP events on the launch stream.
+    events = {}
+
+    def hook(name, start):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        events.setdefault(name, []).append(e)
+
+    timed = min(steps, 32)
+    for k in range(timed):
+        f, i = frames[warmup + k]
+        drv.render(f, gscene, gbufs[i], hook)
+    torch.cuda.synchronize()
+    pass_ms = {}
+    for name, evs in events.items():
+        pass_ms[name] = sum(evs[j].elapsed_time(evs[j + 1]) for j in range(0, len(evs), 2)) / (len(evs) // 2)
+    return {"ms_per_frame": ms_per_frame, "pass_ms": pass_ms, "tuning": tuning, "options": options, "gscene": gscene,
+            "drv": drv, "rend": rend, "gbufs": gbufs, "views": views}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +251,7 @@ def main():
     ap.add_argument("--mse-frames", type=int, default=64)
     ap.add_argument("--tune-frames", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true", help="run one 256 MiB device copy first")
+    ap.add_argument("--atrium-steps", type=int, default=32, help="timed frames of the atrium workload (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -182,85 +271,11 @@ def main():
     hs = ra.HostScene(ra.RGI_SCENE_CORNELL, seed=1, extra_lights=4, light_seed=1)
     mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
     settings = ra.Settings(grid_table_size=T, grid_mode=mode)
-    gscene = driver.GpuScene(hs)
-    drv = driver.Driver(W, H, T, mode)
+    run = run_workload(hs, settings, W, H, args.frames, args.steps, args.warmup, args.tune_frames)
+    ms_per_frame, pass_ms, tuning, options = run["ms_per_frame"], run["pass_ms"], run["tuning"], run["options"]
+    gscene, drv, rend, gbufs, views = run["gscene"], run["drv"], run["rend"], run["gbufs"], run["views"]
     L = ra.lib()
     s = driver.stream()
-
-    # The orbit's cameras and G-buffers, produced once, resident in HBM.
-    cams = [ra.orbit_camera(hs, i, W, H, yaw_of=lambda k: ra.swing_yaw(k, args.frames)) for i in range(args.frames)]
-    views = [ra.make_view(*c) for c in cams]
-    gbufs = []
-    for i in range(args.frames):
-        eye, vp, inv = cams[i]
-        f = ra.make_frame(settings, hs, i, eye, vp, inv, eye, vp, W, H, False)
-        gbufs.append(driver.GBufferSet(drv.ctx, gscene, f))
-    torch.cuda.synchronize()
-
-    # The timed path: the C++ frame driver, one call per frame.
-    rend = ctypes.c_void_p()
-    ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
-                                   ctypes.byref(gscene.scene), hs.radius))
-
-    def render(k):
-        i = k % args.frames
-        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[i]), ctypes.byref(gbufs[i].gb), s))
-
-    for k in range(args.warmup):
-        render(k)
-    # Pick the fastest launch variants on this workload (bit-exact with each other).
-    tune_ms = (ctypes.c_float * ra.RGI_TUNE_TRIALS)()
-    view_arr = (ra.View * args.frames)(*views)
-    gb_arr = (ra.GBuffer * args.frames)(*[g.gb for g in gbufs])
-    chosen = L.rgi_renderer_autotune(rend, view_arr, gb_arr, args.frames, args.tune_frames, tune_ms, s)
-    if chosen < 0:
-        ra.check(chosen)
-    options = ra.tuning_of(chosen)
-    tuning = {"gi_temporal_waves": options[ra.RGI_OPT_GI_TEMPORAL_WAVES],
-              "gi_spatial_waves": options[ra.RGI_OPT_GI_SPATIAL_WAVES],
-              "tile_order": options[ra.RGI_OPT_TILE_ORDER],
-              "trial_ms": [round(float(v), 5) for v in tune_ms]}
-    for k in range(args.warmup):
-        render(k)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
-        render(k)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_frame = elapsed * 1e3 / args.steps
-
-    def frame_for_step(step):
-        i = step % args.frames
-        p = (step - 1) % args.frames
-        eye, vp, inv = cams[i]
-        peye, pvp, _ = cams[p]
-        return ra.make_frame(settings, hs, step, eye, vp, inv, peye, pvp, W, H, step > 0, step > 0), i
-
-    frames = [frame_for_step(k) for k in range(args.warmup + min(args.steps, 32))]
-    for opt, value in options.items():
-        ra.check(L.rgi_context_set_option(drv.ctx, opt, value))
-    for k in range(args.warmup):
-        f, i = frames[k]
-        drv.render(f, gscene, gbufs[i])
-    torch.cuda.synchronize()
-
-    # Per-pass durations with HIP events on the launch stream.
-    events = {}
-
-    def hook(name, start):
-        e = torch.cuda.Event(enable_timing=True)
-        e.record()
-        events.setdefault(name, []).append(e)
-
-    timed = min(args.steps, 32)
-    for k in range(timed):
-        f, i = frames[args.warmup + k]
-        drv.render(f, gscene, gbufs[i], hook)
-    torch.cuda.synchronize()
-    pass_ms = {}
-    for name, evs in events.items():
-        pass_ms[name] = sum(evs[j].elapsed_time(evs[j + 1]) for j in range(0, len(evs), 2)) / (len(evs) // 2)
 
     # Traversal microbenchmark.
     rays = surface_rays(hs, args.rays, seed=7)
@@ -314,6 +329,23 @@ def main():
 
     L.rgi_renderer_destroy(rend)
     drv.close()
+
+    atrium = None
+    if args.atrium_steps > 0:
+        # The atrium workload: ~256k triangles, 2^22-cell grid, DI over 64
+        # point lights and the emissive triangles; the same swing path, with
+        # the camera inside the courtyard (0.7 scene radii from the centre).
+        ahs = ra.HostScene(ra.RGI_SCENE_ATRIUM, seed=1, size=262144, extra_lights=64, light_seed=1)
+        aset = ra.Settings(grid_table_size=1 << 22, grid_mode=mode)
+        arun = run_workload(ahs, aset, W, H, args.frames, args.atrium_steps, 8, 4, distance=0.7)
+        atrium = {"ms_per_frame": round(arun["ms_per_frame"], 5),
+                  "pass_ms": {k: round(v, 5) for k, v in sorted(arun["pass_ms"].items())},
+                  "triangles": ahs.triangle_count, "bvh_nodes": ahs.node_count, "di_lights": ahs.light_count,
+                  "grid_table_size": 1 << 22, "steps": args.atrium_steps, "camera_distance_radii": 0.7,
+                  "tuning": arun["tuning"]}
+        L.rgi_renderer_destroy(arun["rend"])
+        arun["drv"].close()
+
     line = {
         "metric": "GI frame ms @1080p (grid + resample + trace)",
         "value": round(ms_per_frame, 5),
@@ -338,6 +370,7 @@ def main():
         "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "atrium_1080p": atrium,
     }
     print(json.dumps(line), flush=True)
 

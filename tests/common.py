@@ -20,10 +20,10 @@ def cornell(extra_lights=4, seed=1):
 
 
 def frame_for(hs, settings, width, height, frame_index, history_valid, di_history_valid=False, yaw_step=0.0,
-              jitter_seed=None):
-    eye, vp, inv = ra.orbit_camera(hs, frame_index, width, height, yaw_step=yaw_step)
+              jitter_seed=None, distance=3.17):
+    eye, vp, inv = ra.orbit_camera(hs, frame_index, width, height, yaw_step=yaw_step, distance=distance)
     if frame_index > 0:
-        peye, pvp, _ = ra.orbit_camera(hs, frame_index - 1, width, height, yaw_step=yaw_step)
+        peye, pvp, _ = ra.orbit_camera(hs, frame_index - 1, width, height, yaw_step=yaw_step, distance=distance)
     else:
         peye, pvp = eye, vp
     return ra.make_frame(settings, hs, frame_index, eye, vp, inv, peye, pvp, width, height, history_valid,
@@ -125,14 +125,14 @@ def next_history(hist, out):
     return h
 
 
-def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0):
+def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0, distance=3.17):
     """Runs `frames` frames on the oracle; returns (frames, gbuffers, histories, outputs)."""
     hist = empty_history(width, height, settings.grid_table_size)
     seq = []
     di_on = settings.restir_di and hs.light_count > 0
     for i in range(frames):
         f = frame_for(hs, settings, width, height, i, history_valid=i > 0, di_history_valid=i > 0 and di_on,
-                      yaw_step=yaw_step)
+                      yaw_step=yaw_step, distance=distance)
         g = oracle_gbuffer(hs, f)
         out = oracle_frame(hs, f, g, hist)
         seq.append((f, g, hist, out))

@@ -45,14 +45,25 @@ def download(torch, ptr, like):
 ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES4 | ra.RGI_TUNE_TILE_ORDER)
 
 
-@pytest.mark.parametrize("frames,kw,variants", [(4, {}, {}), (3, {"restir_di": False, "sky": True}, {}),
-                                                (3, {"ray_mode": ra.RGI_RAY_BVH}, {}), (4, {}, ALL_VARIANTS)])
-def test_renderer_matches_oracle_sequence(torch, frames, kw, variants):
-    say("case renderer frames=%d %s variants=%s" % (frames, kw, variants))
-    hs = common.cornell(extra_lights=4)
+ATRIUM_DISTANCE = 0.7  # camera inside the courtyard: about 70% of pixels hit geometry
+
+
+def atrium():
+    """The ~256k-triangle atrium with 64 point lights besides its emitters."""
+    return ra.HostScene(ra.RGI_SCENE_ATRIUM, seed=1, size=262144, extra_lights=64, light_seed=1)
+
+
+@pytest.mark.parametrize("frames,kw,variants,scene", [
+    (4, {}, {}, "cornell"), (3, {"restir_di": False, "sky": True}, {}, "cornell"),
+    (3, {"ray_mode": ra.RGI_RAY_BVH}, {}, "cornell"), (4, {}, ALL_VARIANTS, "cornell"),
+    (3, {}, ALL_VARIANTS, "atrium")])
+def test_renderer_matches_oracle_sequence(torch, frames, kw, variants, scene):
+    say("case renderer scene=%s frames=%d %s variants=%s" % (scene, frames, kw, variants))
+    hs = atrium() if scene == "atrium" else common.cornell(extra_lights=4)
+    distance = ATRIUM_DISTANCE if scene == "atrium" else 3.17
     s = ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC, **kw)
     yaw = 2 * np.pi / 64
-    seq = common.oracle_sequence(hs, s, W, H, frames, yaw_step=yaw)
+    seq = common.oracle_sequence(hs, s, W, H, frames, yaw_step=yaw, distance=distance)
     L = ra.lib()
     bvh, lights = upload(torch, hs.bvh), upload(torch, hs.lights)
     scene = hs.rgi_scene(bvh.data_ptr(), lights.data_ptr())
@@ -64,7 +75,7 @@ def test_renderer_matches_oracle_sequence(torch, frames, kw, variants):
         for opt, value in variants.items():
             ra.check(L.rgi_renderer_set_option(r, opt, value))
         for i, (f, g, hist, want) in enumerate(seq):
-            eye, vp, inv = ra.orbit_camera(hs, i, W, H, yaw_step=yaw)
+            eye, vp, inv = ra.orbit_camera(hs, i, W, H, yaw_step=yaw, distance=distance)
             view = ra.make_view(eye, vp, inv)
             planes = {k: upload(torch, v) for k, v in g.items()}
             gb = ra.GBuffer(W, H, planes["depth"].data_ptr(), planes["albedo"].data_ptr(), planes["normal"].data_ptr(),
