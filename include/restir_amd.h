@@ -163,7 +163,7 @@ typedef struct rgi_di_reservoirs {
 const char* rgi_last_error(void);
 int rgi_abi_version(void);
 /* sizeof of the ABI structs (0: rgi_frame, 1: rgi_scene, 2: rgi_gbuffer,
- * 3: rgi_gi_reservoirs, 4: rgi_gi_inputs, 5: rgi_di_reservoirs), so that
+ * 3: rgi_gi_reservoirs, 4: rgi_gi_inputs, 5: rgi_di_reservoirs, 6: rgi_shadow), so that
  * bindings can check their mirrors. */
 uint32_t rgi_abi_sizeof(int which);
 
@@ -249,6 +249,43 @@ int rgi_gbuffer_primary(rgi_context* context, const rgi_frame* frame, const rgi_
                         const float* materials, uint32_t material_count, float* depth,
                         uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal,
                         uint16_t* emission, void* stream);
+
+/* The main point light's omnidirectional shadow map (renderer.cpp:527-564,
+ * fs_shadow.sc, shadow.sh): six cube faces (+X, -X, +Y, -Y, +Z, -Z) in a
+ * 3 x 2 atlas of face_size texels per face side, face f at column f % 3,
+ * row f / 3 (rows counted from the top; from the bottom when
+ * origin_bottom_left). Each face is a right-handed look-at from the light
+ * (up +Z for the Y faces, +Y otherwise) with a square perspective of
+ * tan(half fov) = F / (F - 8), near 0.01 radius and far 2.5 radius. */
+typedef struct rgi_shadow {
+    float view_proj[96];      /* u_shadowViewProj[6], column-major */
+    float inv_view_proj[96];  /* their inverses (computed in double, rounded once) */
+    float info[4];            /* u_shadowInfo: far, tan(half fov), 1 / face size, origin bottom-left */
+    float light_position[4];
+    float near_plane;
+    int32_t face_size;
+    int32_t homogeneous_depth;
+    int32_t reserved;
+} rgi_shadow;
+
+/* Fills `out` for a light at `light_position` in a scene of `scene_radius`;
+ * face_size is one of 256, 512, 1024, 2048 (kShadowFaceSizes, settings.h:26). */
+int rgi_shadow_setup(const float* light_position, float scene_radius, int32_t face_size, int homogeneous_depth,
+                     int origin_bottom_left, rgi_shadow* out);
+
+/* Copies the shadow uniforms into a frame (u_shadowViewProj, u_shadowInfo,
+ * atlas size) and sets shadows_enabled, as renderer.cpp:782-783 binds them. */
+int rgi_shadow_apply(const rgi_shadow* shadow, int enabled, rgi_frame* frame);
+
+/* The shadow pass (renderer.cpp:566-596 with fs_shadow.sc): for every atlas
+ * texel, the distance from the light to the nearest surface along the
+ * texel centre's ray, divided by the far plane, every triangle two-sided,
+ * surfaces before the near plane or past the far plane clipped; texels that
+ * see nothing keep the clear value 1. One closest-hit BVH ray per texel takes
+ * the place of rasterizing the scene six times. `atlas` holds
+ * 3 face_size x 2 face_size floats, row-major. */
+int rgi_shadow_atlas(rgi_context* context, const rgi_shadow* shadow, const rgi_scene* scene, float* atlas,
+                     void* stream);
 
 /* The image the error metric compares, per pixel (linear RGB): the GI
  * estimate re-modulated by the diffuse albedo, color.rgb * albedo.rgb *

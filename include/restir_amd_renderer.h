@@ -40,7 +40,7 @@ typedef struct rgi_settings {
     int32_t restir_di;            /* 1: DirectLighting::ReStirDi, 0: GiOnly */
     int32_t emissive_lighting;    /* 1 */
     int32_t sky;                  /* 0 */
-    int32_t shadows;              /* 0 (no shadow atlas producer in this build) */
+    int32_t shadows;              /* 0: the main light's shadow atlas (rgi_shadow_atlas) and its lookups */
     int32_t grid_mode;            /* RGI_GRID_ATOMIC */
     uint32_t grid_table_size;     /* 65536 */
     float sky_zenith[3];
@@ -51,7 +51,8 @@ typedef struct rgi_settings {
     float light_color[3];         /* 0: off */
     int32_t origin_bottom_left;   /* 0 */
     int32_t homogeneous_depth;    /* 0 */
-    int32_t reserved[4];
+    int32_t shadow_face_size;     /* 1: index into 256, 512, 1024, 2048 (kShadowFaceSizes) */
+    int32_t reserved[3];
 } rgi_settings;
 
 /* One frame's camera: eye and (jittered) view-projection and its inverse. */

@@ -1890,6 +1890,36 @@ void orc_reference_error(const rgi_gbuffer* g, const uint16_t* color, const uint
 }
 
 // The G-buffer producer: one closest-hit primary ray per pixel centre.
+// The shadow pass (renderer.cpp:566-596, fs_shadow.sc): the distance from
+// the light to the nearest surface through each atlas texel centre, over the
+// far plane, both faces of every triangle, clipped to [near, far]; 1 (the
+// clear value) where nothing is hit. Texel (X, Y) is on face column + 3 row,
+// column = X / F, row = Y / F from the top (from the bottom when the origin
+// is bottom-left): shadowAtlasUv (shadow.sh:52-69) inverted at texel centres.
+void orc_shadow_atlas(const rgi_shadow* sh, const rgi_scene* s, float* atlas)
+{
+    int F = sh->face_size, W = 3 * F, H = 2 * F;
+    bool obl = sh->info[3] > 0.5f;
+    V3 light = V3{sh->light_position[0], sh->light_position[1], sh->light_position[2]};
+    parallel_for((int64_t)W * H, [&](int64_t i) {
+        int X = (int)(i % W), Y = (int)(i / W);
+        int column = X / F, texture_row = Y / F;
+        int face = column + 3 * (obl ? 1 - texture_row : texture_row);
+        float lx = ((float)(X - column * F) + 0.5f) / (float)F;
+        float ly = ((float)(Y - texture_row * F) + 0.5f) / (float)F;
+        V3 ndc = V3{lx * 2.0f - 1.0f, obl ? ly * 2.0f - 1.0f : 1.0f - ly * 2.0f, 1.0f};
+        V3 far_point = clip_to_world(sh->inv_view_proj + face * 16, ndc);
+        V3 to_far = far_point - light;
+        float far_distance = length(to_far);
+        V3 dir = to_far * (1.0f / far_distance);
+        float near_distance = far_distance * (sh->near_plane / sh->info[0]);
+        V3 origin = light + dir * near_distance;
+        float dist;
+        int tri = trace_bvh(s, origin, dir, far_distance - near_distance, false, dist);
+        atlas[i] = tri >= 0 ? length((origin + dir * dist) - light) / sh->info[0] : 1.0f;
+    });
+}
+
 void orc_gbuffer_primary(const rgi_frame* f, const rgi_scene* s, const float* materials, uint32_t material_count,
                          float* depth, uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal, uint16_t* emission)
 {

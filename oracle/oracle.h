@@ -77,6 +77,10 @@ void orc_reference_accumulate(const rgi_gbuffer* gbuffer, const uint16_t* color,
 void orc_reference_error(const rgi_gbuffer* gbuffer, const uint16_t* color, const uint16_t* di_diffuse,
                          const uint16_t* di_specular, const double* accum, uint32_t frames, double* out5);
 
+/* The shadow pass of rgi_shadow_atlas (renderer.cpp:566-596, fs_shadow.sc),
+ * on the CPU: one closest-hit ray per atlas texel. */
+void orc_shadow_atlas(const rgi_shadow* shadow, const rgi_scene* scene, float* atlas);
+
 /* The G-buffer producer of rgi_gbuffer_primary, on the CPU. */
 void orc_gbuffer_primary(const rgi_frame* frame, const rgi_scene* scene, const float* materials,
                          uint32_t material_count, float* depth, uint32_t* albedo, uint32_t* normal,

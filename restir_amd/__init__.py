@@ -92,7 +92,7 @@ class CSettings(ctypes.Structure):
         ("restir_di", I32), ("emissive_lighting", I32), ("sky", I32), ("shadows", I32), ("grid_mode", I32),
         ("grid_table_size", U32), ("sky_zenith", F * 3), ("sky_horizon", F * 3), ("sky_ground", F * 3),
         ("sky_intensity", F), ("light_position", F * 3), ("light_color", F * 3), ("origin_bottom_left", I32),
-        ("homogeneous_depth", I32), ("reserved", I32 * 4),
+        ("homogeneous_depth", I32), ("shadow_face_size", I32), ("reserved", I32 * 3),
     ]
 
 
@@ -106,12 +106,18 @@ class RendererOutputs(ctypes.Structure):
                 ("di_diffuse", P), ("di_specular", P)]
 
 
+class Shadow(ctypes.Structure):
+    """rgi_shadow: the point light's six face matrices, their inverses, u_shadowInfo."""
+    _fields_ = [("view_proj", F * 96), ("inv_view_proj", F * 96), ("info", F * 4), ("light_position", F * 4),
+                ("near_plane", F), ("face_size", I32), ("homogeneous_depth", I32), ("reserved", I32)]
+
+
 class Mesh(ctypes.Structure):
     _fields_ = [("triangle_count", U32), ("positions", P), ("albedo", P), ("emission", P), ("material", P),
                 ("material_count", U32), ("materials", P)]
 
 
-ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs]
+ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs, Shadow]
 
 
 class RestirError(RuntimeError):
@@ -147,6 +153,9 @@ def lib():
                                  ctypes.POINTER(DiReservoirs), P, P, P]
     L.rgi_trace.argtypes = [P, ctypes.POINTER(Scene), P, U32, P, P, ctypes.c_int, P]
     L.rgi_gbuffer_primary.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), P, U32, P, P, P, P, P, P]
+    L.rgi_shadow_setup.argtypes = [P, F, I32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Shadow)]
+    L.rgi_shadow_apply.argtypes = [ctypes.POINTER(Shadow), ctypes.c_int, ctypes.POINTER(Frame)]
+    L.rgi_shadow_atlas.argtypes = [P, ctypes.POINTER(Shadow), ctypes.POINTER(Scene), P, P]
     L.rgi_scene_generate.restype = ctypes.c_int64
     L.rgi_scene_generate.argtypes = [ctypes.c_int, U32, U32, ctypes.POINTER(Mesh), U32]
     L.rgi_mesh_bounds.restype = None
@@ -187,6 +196,15 @@ def lib():
             raise RestirError("ABI mismatch: %s is %d bytes in C, %d in Python" % (cls.__name__, size, ctypes.sizeof(cls)))
     _lib = L
     return L
+
+
+def shadow_setup(light_position, scene_radius, face_size=512, homogeneous_depth=False, origin_bottom_left=False):
+    """rgi_shadow for a light (renderer.cpp:527-564)."""
+    sh = Shadow()
+    p = np.asarray(light_position, np.float32)[:3].copy()
+    check(lib().rgi_shadow_setup(_ptr(p), float(scene_radius), int(face_size), int(homogeneous_depth),
+                                 int(origin_bottom_left), ctypes.byref(sh)))
+    return sh
 
 
 def check(status):
@@ -315,6 +333,7 @@ class Settings:
         self.light_color = (0.0, 0.0, 0.0)  # main point light off
         self.light_position = (0.0, 1.7, 0.0)
         self.shadows = False
+        self.shadow_face_size = 1      # index into 256, 512, 1024, 2048
         self.grid_mode = RGI_GRID_ATOMIC
         self.grid_table_size = 65536
         for k, v in kw.items():
@@ -353,6 +372,7 @@ def c_settings(settings):
     c.sky_intensity = s.sky_intensity
     c.light_position[:] = list(s.light_position)
     c.light_color[:] = list(s.light_color)
+    c.shadow_face_size = int(s.shadow_face_size)
     return c
 
 

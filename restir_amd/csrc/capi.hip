@@ -105,6 +105,7 @@ uint32_t rgi_abi_sizeof(int which)
     case 3: return (uint32_t)sizeof(rgi_gi_reservoirs);
     case 4: return (uint32_t)sizeof(rgi_gi_inputs);
     case 5: return (uint32_t)sizeof(rgi_di_reservoirs);
+    case 6: return (uint32_t)sizeof(rgi_shadow);
     default: return 0;
     }
 }
@@ -268,6 +269,29 @@ int rgi_trace(rgi_context* ctx, const rgi_scene* scene, const float* rays, uint3
     if (rays && !aligned(rays, 16)) return fail(RGI_E_INVALID, "rays must be 16-byte aligned");
     return trace_rays(*scene, rays, count, hit_triangle, hit_distance, any_hit, ctx ? ctx->tuning : Tuning{},
                       ctx ? ctx->counter : nullptr, as_stream(stream));
+}
+
+int rgi_shadow_apply(const rgi_shadow* shadow, int enabled, rgi_frame* frame)
+{
+    if (!shadow || !frame) return fail(RGI_E_INVALID, "shadow_apply: NULL argument");
+    memcpy(frame->shadow_view_proj, shadow->view_proj, sizeof(frame->shadow_view_proj));
+    memcpy(frame->shadow_info, shadow->info, sizeof(frame->shadow_info));
+    frame->shadow_width = 3 * shadow->face_size;
+    frame->shadow_height = 2 * shadow->face_size;
+    frame->shadows_enabled = enabled ? 1 : 0;
+    return RGI_OK;
+}
+
+int rgi_shadow_atlas(rgi_context* ctx, const rgi_shadow* shadow, const rgi_scene* scene, float* atlas, void* stream)
+{
+    (void)ctx;
+    int e;
+    if (!shadow || !atlas) return fail(RGI_E_INVALID, "shadow_atlas: shadow and atlas are required");
+    if ((e = check_scene(scene, false)) != RGI_OK) return e;
+    int32_t F = shadow->face_size;
+    if (F != 256 && F != 512 && F != 1024 && F != 2048)
+        return fail(RGI_E_INVALID, "shadow face size must be 256, 512, 1024 or 2048, not %d", F);
+    return shadow_atlas(*shadow, *scene, atlas, as_stream(stream));
 }
 
 int rgi_gbuffer_primary(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const float* materials,

@@ -46,6 +46,7 @@ int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* h
 int gbuffer_primary(const rgi_frame& f, const rgi_scene& s, const float* materials, uint32_t material_count,
                     float* depth, uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal, uint16_t* emission,
                     hipStream_t stream);
+int shadow_atlas(const rgi_shadow& sh, const rgi_scene& s, float* atlas, hipStream_t stream);
 int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                 const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream);
 int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,

@@ -25,6 +25,11 @@ struct rgi_renderer {
     uint16_t* di_normal[2] = {};
     uint16_t* diffuse = nullptr;
     uint16_t* specular = nullptr;
+    // The main light's shadow atlas: rendered on the first frame with
+    // shadows on; scene and light are fixed for the renderer's lifetime, so
+    // every later frame's shadow pass would write the same texels.
+    float* atlas = nullptr;
+    bool atlas_valid = false;
     int read = 0;              // m_historyRead / m_diRead: the parity last frame wrote
     bool history_valid = false;
     bool di_valid = false;
@@ -69,6 +74,14 @@ void free_all(rgi_renderer* r)
     }
     release(r->diffuse);
     release(r->specular);
+    release(r->atlas);
+}
+
+// kShadowFaceSizes (settings.h:26); 0 for an index out of range.
+int32_t shadow_face_size(const rgi_settings* s)
+{
+    static const int32_t sizes[4] = {256, 512, 1024, 2048};
+    return s->shadow_face_size >= 0 && s->shadow_face_size < 4 ? sizes[s->shadow_face_size] : 0;
 }
 
 // Effective settings while the reference accumulates (renderer.cpp:437-452).
@@ -117,6 +130,7 @@ void rgi_settings_default(rgi_settings* s)
     memcpy(s->sky_ground, ground, sizeof(ground));
     s->sky_intensity = 1.0f;
     s->light_position[1] = 1.7f;
+    s->shadow_face_size = 1;
 }
 
 uint32_t rgi_settings_sizeof(void) { return (uint32_t)sizeof(rgi_settings); }
@@ -177,6 +191,14 @@ int rgi_frame_pack(const rgi_settings* s, const rgi_scene* scene, float radius, 
     }
     f.sky_zenith[3] = s->sky ? 1.0f : 0.0f;
     f.shadows_enabled = s->shadows ? 1 : 0;
+    if (s->shadows) {
+        // The shadow uniforms of this light (renderer.cpp:527-564, 782-783).
+        rgi_shadow sh;
+        int e = rgi_shadow_setup(s->light_position, radius, shadow_face_size(s), s->homogeneous_depth,
+                                 s->origin_bottom_left, &sh);
+        if (e != RGI_OK) return fail(e, "frame_pack: shadow setup failed (face size index %d)", s->shadow_face_size);
+        rgi_shadow_apply(&sh, 1, &f);
+    }
     f.di_light_count = s->restir_di && scene ? (int32_t)scene->light_count : 0;
     f.di_candidates = s->di_candidates;
     f.di_spatial_taps = s->di_spatial_taps;
@@ -193,7 +215,8 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         return fail(RGI_E_INVALID, "renderer_create: size %d x %d out of range", width, height);
     uint32_t T = settings->grid_table_size;
     if (T < 64u || (T & (T - 1u))) return fail(RGI_E_INVALID, "grid_table_size must be a power of two >= 64");
-    if (settings->shadows) return fail(RGI_E_INVALID, "the frame driver has no shadow atlas: set shadows = 0");
+    if (settings->shadows && shadow_face_size(settings) == 0)
+        return fail(RGI_E_INVALID, "shadow_face_size index %d is not in 0..3", settings->shadow_face_size);
     rgi_renderer* r = new (std::nothrow) rgi_renderer();
     if (!r) return fail(RGI_E_NOMEM, "renderer allocation failed");
     r->width = width;
@@ -214,6 +237,10 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         ok = ok && alloc(r->di_sample[k], 4 * n) && alloc(r->di_receiver[k], 4 * n) && alloc(r->di_normal[k], 4 * n);
     }
     ok = ok && alloc(r->diffuse, 4 * n) && alloc(r->specular, 4 * n);
+    if (settings->shadows) {
+        size_t F = (size_t)shadow_face_size(settings);
+        ok = ok && alloc(r->atlas, 6 * F * F);
+    }
     if (!ok) {
         free_all(r);
         rgi_destroy(r->ctx);
@@ -271,6 +298,19 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     if (!di_active) f.di_light_count = 0;
     const int read = r->read, write = 1 - r->read;
 
+    const float* shadow_map = nullptr;
+    if (s.shadows) {
+        if (!r->atlas_valid) {
+            // The shadow pass (renderer.cpp:527-596).
+            rgi_shadow sh;
+            if ((e = rgi_shadow_setup(s.light_position, r->radius, shadow_face_size(&s), s.homogeneous_depth,
+                                      s.origin_bottom_left, &sh)) != RGI_OK)
+                return fail(e, "shadow setup failed");
+            if ((e = shadow_atlas(sh, r->scene, r->atlas, stream)) != RGI_OK) return e;
+            r->atlas_valid = true;
+        }
+        shadow_map = r->atlas;
+    }
     const uint16_t* surface_light = g->emission;
     if (di_active) {
         rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
@@ -288,11 +328,11 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     else e = grid_insert_atomic(f, *g, r->grid[write], stream);
     if (e != RGI_OK) return e;
     rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
-                      surface_light, nullptr, r->grid[read]};
+                      surface_light, shadow_map, r->grid[read]};
     rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
     if ((e = gi_temporal(f, r->scene, *g, tin, res, r->ctx->tuning, stream)) != RGI_OK) return e;
     rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
-                      surface_light, nullptr, r->grid[write]};
+                      surface_light, shadow_map, r->grid[write]};
     if ((e = gi_spatial(f, r->scene, *g, sin, res, r->color[write], r->moments[write], r->ctx->tuning, stream)) !=
         RGI_OK)
         return e;

@@ -1,6 +1,7 @@
 // Scene data layer of the hot path (host): seeded procedural scenes, the
 // binned-SAH BVH in the reference's GPU layout (src/bvh.h:9-22), the light
 // list (src/lights.h:7-14) and camera matrices. See restir_amd_scene.h.
+#include "../../include/restir_amd.h"
 #include "../../include/restir_amd_scene.h"
 
 #include <math.h>
@@ -455,6 +456,48 @@ bool mat_inverse(const double* m, double* out)
     return true;
 }
 
+// A right-handed look-at (bx::mtxLookAt semantics: the camera looks down -z,
+// right = normalize(up x (eye - target))) and a perspective with cot(half
+// fov) `t`, depth in [0, 1] or [-1, 1], in double; vp = proj * view.
+void look_at_projection(const float* eye3, const float* target3, const double* up, double t, double aspect,
+                        double n, double fa, int homogeneous_depth, double* view, double* proj, double* vp,
+                        double* inv)
+{
+    double e[3] = {eye3[0], eye3[1], eye3[2]};
+    double f[3] = {target3[0] - e[0], target3[1] - e[1], target3[2] - e[2]};
+    double fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (double& v : f) v /= fl;
+    double s[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    double sl = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    if (sl < 1e-12) {
+        // Looking along `up`: take +z as the reference instead.
+        double alt[3] = {0.0, 0.0, 1.0};
+        s[0] = f[1] * alt[2] - f[2] * alt[1];
+        s[1] = f[2] * alt[0] - f[0] * alt[2];
+        s[2] = f[0] * alt[1] - f[1] * alt[0];
+        sl = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    }
+    for (double& v : s) v /= sl;
+    double u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+    const double vw[16] = {s[0], u[0], -f[0], 0.0, s[1], u[1], -f[1], 0.0, s[2], u[2], -f[2], 0.0,
+                           -(s[0] * e[0] + s[1] * e[1] + s[2] * e[2]), -(u[0] * e[0] + u[1] * e[1] + u[2] * e[2]),
+                           (f[0] * e[0] + f[1] * e[1] + f[2] * e[2]), 1.0};
+    memcpy(view, vw, sizeof(vw));
+    for (int i = 0; i < 16; ++i) proj[i] = 0.0;
+    proj[0] = t / aspect;
+    proj[5] = t;
+    proj[11] = -1.0;
+    if (homogeneous_depth) {
+        proj[10] = (fa + n) / (n - fa);
+        proj[14] = 2.0 * fa * n / (n - fa);
+    } else {
+        proj[10] = fa / (n - fa);
+        proj[14] = fa * n / (n - fa);
+    }
+    mat_mul(proj, view, vp);
+    mat_inverse(vp, inv);
+}
+
 } // namespace
 
 extern "C" {
@@ -591,48 +634,52 @@ void rgi_camera_matrices(const float* eye3, const float* target3, float fovy_deg
                          float far_plane, int homogeneous_depth, float* view_out, float* proj_out, float* vp_out,
                          float* inv_out)
 {
-    double e[3] = {eye3[0], eye3[1], eye3[2]};
-    double f[3] = {target3[0] - e[0], target3[1] - e[1], target3[2] - e[2]};
-    double fl = sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
-    for (double& v : f) v /= fl;
-    double up[3] = {0.0, 1.0, 0.0};
-    double s[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
-    double sl = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-    if (sl < 1e-12) {
-        // Looking straight up or down: take +z as the reference instead.
-        double alt[3] = {0.0, 0.0, 1.0};
-        s[0] = f[1] * alt[2] - f[2] * alt[1];
-        s[1] = f[2] * alt[0] - f[0] * alt[2];
-        s[2] = f[0] * alt[1] - f[1] * alt[0];
-        sl = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-    }
-    for (double& v : s) v /= sl;
-    double u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
-    double view[16] = {s[0], u[0], -f[0], 0.0, s[1], u[1], -f[1], 0.0, s[2], u[2], -f[2], 0.0,
-                       -(s[0] * e[0] + s[1] * e[1] + s[2] * e[2]), -(u[0] * e[0] + u[1] * e[1] + u[2] * e[2]),
-                       (f[0] * e[0] + f[1] * e[1] + f[2] * e[2]), 1.0};
-    double t = 1.0 / tan(0.5 * (double)fovy_degrees * 3.14159265358979323846 / 180.0);
-    double n = near_plane, fa = far_plane;
-    double proj[16] = {0};
-    proj[0] = t / aspect;
-    proj[5] = t;
-    proj[11] = -1.0;
-    if (homogeneous_depth) {
-        proj[10] = (fa + n) / (n - fa);
-        proj[14] = 2.0 * fa * n / (n - fa);
-    } else {
-        proj[10] = fa / (n - fa);
-        proj[14] = fa * n / (n - fa);
-    }
-    double vp[16], inv[16];
-    mat_mul(proj, view, vp);
-    mat_inverse(vp, inv);
+    const double up[3] = {0.0, 1.0, 0.0};
+    double view[16], proj[16], vp[16], inv[16];
+    look_at_projection(eye3, target3, up, 1.0 / tan(0.5 * (double)fovy_degrees * 3.14159265358979323846 / 180.0),
+                       aspect, near_plane, far_plane, homogeneous_depth, view, proj, vp, inv);
     for (int i = 0; i < 16; ++i) {
         if (view_out) view_out[i] = (float)view[i];
         if (proj_out) proj_out[i] = (float)proj[i];
         if (vp_out) vp_out[i] = (float)vp[i];
         if (inv_out) inv_out[i] = (float)inv[i];
     }
+}
+
+int rgi_shadow_setup(const float* light_position, float scene_radius, int32_t face_size, int homogeneous_depth,
+                     int origin_bottom_left, rgi_shadow* out)
+{
+    if (!light_position || !out) return RGI_E_INVALID;
+    if (face_size != 256 && face_size != 512 && face_size != 1024 && face_size != 2048) return RGI_E_INVALID;
+    if (!(scene_radius > 0.0f)) return RGI_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    // renderer.cpp:529-564: kShadowFaceBorder = 4 texels on each side.
+    const float tan_half = (float)face_size / (float)(face_size - 2 * 4);
+    const float far_plane = 2.5f * scene_radius;
+    const float near_plane = 0.01f * scene_radius;
+    static const double directions[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    for (int face = 0; face < 6; ++face) {
+        const double up[3] = {0.0, face == 2 || face == 3 ? 0.0 : 1.0, face == 2 || face == 3 ? 1.0 : 0.0};
+        float target[3];
+        for (int i = 0; i < 3; ++i) target[i] = (float)((double)light_position[i] + directions[face][i]);
+        double view[16], proj[16], vp[16], inv[16];
+        look_at_projection(light_position, target, up, 1.0 / (double)tan_half, 1.0, near_plane, far_plane,
+                           homogeneous_depth, view, proj, vp, inv);
+        for (int i = 0; i < 16; ++i) {
+            out->view_proj[face * 16 + i] = (float)vp[i];
+            out->inv_view_proj[face * 16 + i] = (float)inv[i];
+        }
+    }
+    out->info[0] = far_plane;
+    out->info[1] = tan_half;
+    out->info[2] = 1.0f / (float)face_size;
+    out->info[3] = origin_bottom_left ? 1.0f : 0.0f;
+    for (int i = 0; i < 3; ++i) out->light_position[i] = light_position[i];
+    out->light_position[3] = 1.0f;
+    out->near_plane = near_plane;
+    out->face_size = face_size;
+    out->homogeneous_depth = homogeneous_depth ? 1 : 0;
+    return RGI_OK;
 }
 
 } // extern "C"

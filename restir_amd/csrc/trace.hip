@@ -123,9 +123,49 @@ __global__ void __launch_bounds__(256) k_gbuffer(rgi_frame f, rgi_scene s, const
     store_h4(emission, i, F4{e.x, e.y, e.z, 0.0f});
 }
 
+// The shadow pass: texel (X, Y) of the 3F x 2F atlas belongs to face
+// column + 3 row, with column = X / F and row = Y / F (counted from the top,
+// from the bottom when the origin is bottom-left), which inverts
+// shadowAtlasUv (shadow.sh:52-69) at the texel centre. Its ray runs from the
+// light through the texel's point on the far plane (NDC z = 1), between the
+// near and far planes.
+__global__ void __launch_bounds__(256) k_shadow_atlas(rgi_shadow sh, rgi_scene s, float* atlas)
+{
+    __shared__ int ring[kStackRing * 256];
+    SceneView v = scene_view(s);
+    int F = sh.face_size;
+    int X = (int)(blockIdx.x * 16u + (threadIdx.x & 15u)), Y = (int)(blockIdx.y * 16u + (threadIdx.x >> 4));
+    if (X >= 3 * F || Y >= 2 * F) return;
+    bool obl = sh.info[3] > 0.5f;
+    int column = X / F, texture_row = Y / F;
+    int face = column + 3 * (obl ? 1 - texture_row : texture_row);
+    float lx = ((float)(X - column * F) + 0.5f) / (float)F;
+    float ly = ((float)(Y - texture_row * F) + 0.5f) / (float)F;
+    F3 ndc = F3{lx * 2.0f - 1.0f, obl ? ly * 2.0f - 1.0f : 1.0f - ly * 2.0f, 1.0f};
+    F3 far_point = clip_to_world(sh.inv_view_proj + face * 16, ndc);
+    F3 light = F3{sh.light_position[0], sh.light_position[1], sh.light_position[2]};
+    F3 to_far = far_point - light;
+    float far_distance = length(to_far);
+    F3 dir = to_far * (1.0f / far_distance);
+    float near_distance = far_distance * (sh.near_plane / sh.info[0]);
+    F3 origin = light + dir * near_distance;
+    float dist;
+    int tri = trace_closest(v, origin, dir, far_distance - near_distance, dist, ring + threadIdx.x, 256);
+    float value = 1.0f;
+    if (tri >= 0) value = length((origin + dir * dist) - light) / sh.info[0];
+    atlas[(uint32_t)Y * (uint32_t)(3 * F) + (uint32_t)X] = value;
+}
+
 } // namespace
 
 __global__ void k_zero_counter(uint32_t* counter) { *counter = 0u; }
+
+int shadow_atlas(const rgi_shadow& sh, const rgi_scene& s, float* atlas, hipStream_t stream)
+{
+    uint32_t F = (uint32_t)sh.face_size;
+    hipLaunchKernelGGL(k_shadow_atlas, dim3(3u * F / 16u, 2u * F / 16u), dim3(256), 0, stream, sh, s, atlas);
+    return check_launch("shadow atlas");
+}
 
 int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* hit, float* dist, int any_hit,
                const Tuning& tuning, uint32_t* counter, hipStream_t stream)

@@ -69,6 +69,19 @@ def host_gbuffer(g, width, height):
                       g["shading"].ctypes.data, g["emission"].ctypes.data)
 
 
+def frame_shadow(hs, frame):
+    """The rgi_shadow a frame's shadow uniforms came from (rgi_frame_pack)."""
+    return ra.shadow_setup(list(frame.light_position)[:3], hs.radius, frame.shadow_width // 3,
+                           frame.homogeneous_depth, frame.origin_bottom_left)
+
+
+def oracle_shadow_atlas(hs, frame):
+    sh = frame_shadow(hs, frame)
+    atlas = np.zeros(6 * sh.face_size * sh.face_size, np.float32)
+    orc.lib().orc_shadow_atlas(ctypes.byref(sh), ctypes.byref(orc.host_scene(hs)), orc.ptr(atlas))
+    return atlas
+
+
 def oracle_frame(hs, frame, g, hist, insert=True):
     """One frame on the oracle; returns this frame's outputs (host arrays)."""
     L = orc.lib()
@@ -77,6 +90,8 @@ def oracle_frame(hs, frame, g, hist, insert=True):
     scene = orc.host_scene(hs)
     gb = host_gbuffer(g, w, h)
     out = {}
+    atlas = oracle_shadow_atlas(hs, frame) if frame.shadows_enabled else None
+    shadow_ptr = atlas.ctypes.data if atlas is not None else None
     if frame.di_light_count > 0:
         prev = ra.DiReservoirs(hist["di_sample"].ctypes.data, hist["di_receiver"].ctypes.data,
                                hist["di_normal"].ctypes.data)
@@ -98,19 +113,21 @@ def oracle_frame(hs, frame, g, hist, insert=True):
         L.orc_grid_insert(ctypes.byref(frame), ctypes.byref(gb), orc.ptr(grid))
     out["grid"] = grid
     gin = ra.GiInputs(hist["gi_sample"].ctypes.data, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
-                      hist["color"].ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, None,
+                      hist["color"].ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, shadow_ptr,
                       hist["grid"].ctypes.data)
     for k in GI_PLANES_F32:
         out["gi_" + k] = np.zeros((n, 4), np.float32)
     gres = ra.GiReservoirs(out["gi_sample"].ctypes.data, out["gi_receiver"].ctypes.data, out["gi_normal"].ctypes.data)
     L.orc_gi_temporal(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(gin), ctypes.byref(gres))
     sin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data, hist["color"].ctypes.data,
-                      hist["moments"].ctypes.data, surface_light.ctypes.data, None, grid.ctypes.data)
+                      hist["moments"].ctypes.data, surface_light.ctypes.data, shadow_ptr, grid.ctypes.data)
     out["color"] = np.zeros((n, 4), np.uint16)
     out["moments"] = np.zeros((n, 4), np.uint16)
     L.orc_gi_spatial(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(sin), ctypes.byref(gres),
                      orc.ptr(out["color"]), orc.ptr(out["moments"]))
     out["surface_light"] = surface_light
+    if atlas is not None:
+        out["shadow_atlas"] = atlas
     return out
 
 

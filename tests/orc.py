@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from restir_amd import DiReservoirs, Frame, GBuffer, GiInputs, GiReservoirs, Scene
+from restir_amd import DiReservoirs, Frame, GBuffer, GiInputs, GiReservoirs, Scene, Shadow
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -78,6 +78,8 @@ def lib():
     L.orc_reference_accumulate.argtypes = [PG, P, P, P, P]
     L.orc_reference_error.restype = None
     L.orc_reference_error.argtypes = [PG, P, P, P, P, U32, P]
+    L.orc_shadow_atlas.restype = None
+    L.orc_shadow_atlas.argtypes = [ctypes.POINTER(Shadow), PS, P]
     L.orc_gbuffer_primary.restype = None
     L.orc_gbuffer_primary.argtypes = [PF, PS, P, U32, P, P, P, P, P]
     _lib = L

@@ -170,6 +170,27 @@ def gpu_grid(dev, frame, g, mode):
     return dev.get(grid, np.zeros(34 * T, np.uint32))
 
 
+@pytest.mark.parametrize("scene,face,obl,hd", [("cornell", 256, False, False), ("atrium", 512, True, True)])
+def test_shadow_atlas(dev, scene, face, obl, hd):
+    """rgi_shadow_atlas against the oracle's shadow pass, bit-exact."""
+    if scene == "atrium":
+        hs = ra.HostScene(ra.RGI_SCENE_ATRIUM, seed=1, size=262144)
+        light = hs.center + np.array([2.0, 3.0, -1.5], np.float32)
+    else:
+        hs = common.cornell(extra_lights=0)
+        light = np.array([0.1, 1.2, -0.2], np.float32)
+    say("case shadow atlas %s F=%d obl=%d hd=%d" % (scene, face, obl, hd))
+    sh = ra.shadow_setup(light, hs.radius, face, hd, obl)
+    want = np.zeros(6 * face * face, np.float32)
+    orc.lib().orc_shadow_atlas(ctypes.byref(sh), ctypes.byref(orc.host_scene(hs)), orc.ptr(want))
+    dscene = dev.scene(hs)
+    out = dev.empty(want.nbytes)
+    ra.check(dev.L.rgi_shadow_atlas(dev.ctx, ctypes.byref(sh), ctypes.byref(dscene), out.data_ptr(), dev.stream))
+    got = dev.get(out, want)
+    say("  lit texels %.3f" % (want < 1.0).mean())
+    assert_same("atlas", got, want)
+
+
 @pytest.mark.parametrize("table", [65536, 1024, 64])
 def test_grid_insert_deterministic(dev, cornell_seq, table):
     hs, s, seq = cornell_seq

@@ -56,6 +56,8 @@ def atrium():
 @pytest.mark.parametrize("frames,kw,variants,scene", [
     (4, {}, {}, "cornell"), (3, {"restir_di": False, "sky": True}, {}, "cornell"),
     (3, {"ray_mode": ra.RGI_RAY_BVH}, {}, "cornell"), (4, {}, ALL_VARIANTS, "cornell"),
+    (3, {"shadows": True, "light_color": (1.0, 0.9, 0.8), "light_position": (0.1, 1.2, -0.2),
+         "shadow_face_size": 0}, {}, "cornell"),
     (3, {}, ALL_VARIANTS, "atrium")])
 def test_renderer_matches_oracle_sequence(torch, frames, kw, variants, scene):
     say("case renderer scene=%s frames=%d %s variants=%s" % (scene, frames, kw, variants))

@@ -345,6 +345,101 @@ def test_trace_against_brute_force(kind, size):
         assert agree >= n - 3, (any_hit, n - agree)
 
 
+# ---------------------------------------------------------- shadow atlas
+
+def shadow_face(d):
+    """shadowFace (shadow.sh:31-45), vectorised: ties go to x, then y."""
+    a = np.abs(d)
+    x = (a[:, 0] >= a[:, 1]) & (a[:, 0] >= a[:, 2])
+    y = ~x & (a[:, 1] >= a[:, 2])
+    return np.where(x, np.where(d[:, 0] >= 0, 0, 1), np.where(y, np.where(d[:, 1] >= 0, 2, 3),
+                                                             np.where(d[:, 2] >= 0, 4, 5)))
+
+
+def shadow_atlas_uv(sh, p, face):
+    """shadowAtlasUv (shadow.sh:52-69) in double, from the float matrices."""
+    m = np.array(sh.view_proj, np.float64).reshape(6, 4, 4).transpose(0, 2, 1)[face]  # row-major per point
+    clip = np.einsum("nij,nj->ni", m, np.concatenate([p, np.ones((len(p), 1))], 1))
+    ndc = clip[:, :2] / clip[:, 3:4]
+    obl = sh.info[3] > 0.5
+    local = np.stack([ndc[:, 0] * 0.5 + 0.5, ndc[:, 1] * 0.5 + 0.5 if obl else 0.5 - ndc[:, 1] * 0.5], 1)
+    row = (face >= 3).astype(np.float64)
+    column = face - 3.0 * row
+    if obl:
+        row = 1.0 - row
+    return (np.stack([column, row], 1) + local) * np.array([1.0 / 3.0, 0.5])
+
+
+def texel_rays(sh, X, Y):
+    """The ray of atlas texel (X, Y): the documented inverse of shadowAtlasUv
+    at the texel centre, unprojected to the far plane, in double."""
+    F = sh.face_size
+    obl = sh.info[3] > 0.5
+    column, trow = X // F, Y // F
+    face = column + 3 * (1 - trow if obl else trow)
+    lx = (X - column * F + 0.5) / F
+    ly = (Y - trow * F + 0.5) / F
+    ndc = np.stack([lx * 2 - 1, ly * 2 - 1 if obl else 1 - ly * 2, np.ones_like(lx), np.ones_like(lx)], 1)
+    inv = np.array(sh.inv_view_proj, np.float64).reshape(6, 4, 4).transpose(0, 2, 1)[face]
+    w = np.einsum("nij,nj->ni", inv, ndc)
+    far_point = w[:, :3] / w[:, 3:4]
+    return face, far_point - np.array(sh.light_position[:3], np.float64)
+
+
+@pytest.mark.parametrize("obl,hd", [(False, False), (True, True)])
+def test_shadow_atlas_inverts_lookup_and_matches_brute_force(obl, hd):
+    """The shadow pass (renderer.cpp:527-596, fs_shadow.sc) against the
+    lookup that reads it (shadowAtlasUv, shadow.sh:52-69): every texel's ray,
+    looked up again, lands on that texel wherever shadowFace picks the texel's
+    own face (the 4-texel borders belong to neighbouring faces). The stored
+    value is the nearest triangle over all triangles along that ray, clipped
+    to [near, far], over far (1 where nothing is hit)."""
+    hs = common.cornell(extra_lights=0)
+    light = np.array([0.1, 1.2, -0.2], np.float32)
+    sh = ra.shadow_setup(light, hs.radius, 256, hd, obl)
+    F = 256
+    atlas = np.zeros(6 * F * F, np.float32)
+    L.orc_shadow_atlas(ctypes.byref(sh), ctypes.byref(orc.host_scene(hs)), orc.ptr(atlas))
+    Y, X = np.divmod(np.arange(6 * F * F), 3 * F)
+    face, d = texel_rays(sh, X, Y)
+    own = shadow_face(d) == face
+    assert own.mean() > 0.9
+    uv = shadow_atlas_uv(sh, light.astype(np.float64) + d[own], face[own])
+    tx, ty = np.floor(uv[:, 0] * 3 * F).astype(int), np.floor(uv[:, 1] * 2 * F).astype(int)
+    assert np.array_equal(tx, X[own]) and np.array_equal(ty, Y[own])
+
+    far = float(sh.info[0])
+    rng = np.random.default_rng(3)
+    pick = rng.choice(len(X), 3000, replace=False)
+    agree = 0
+    for i in pick:
+        dn = d[i] / np.linalg.norm(d[i])
+        far_distance = np.linalg.norm(d[i])
+        near_distance = far_distance * float(sh.near_plane) / far
+        t = brute_closest(hs, light.astype(np.float64) + dn * near_distance, dn, far_distance - near_distance)
+        want = 1.0 if not np.isfinite(t.min()) else (t.min() + near_distance) / far
+        agree += abs(float(atlas[i]) - want) <= 1e-4 * max(want, 1e-3)
+    assert agree >= len(pick) - 3, len(pick) - agree
+    assert (atlas < 1.0).mean() > 0.9  # the light is inside the box: nearly every texel sees a wall
+
+
+def test_frame_pack_carries_shadow_uniforms():
+    """With shadows on, rgi_frame_pack binds the light's six face matrices and
+    u_shadowInfo (renderer.cpp:527-537, 782-783): far 2.5 R, tan(half fov)
+    F / (F - 8), 1 / F."""
+    hs = common.cornell(extra_lights=0)
+    s = ra.Settings(shadows=True, light_position=(0.1, 1.2, -0.2), shadow_face_size=2)
+    eye, vp, inv = ra.orbit_camera(hs, 0, 64, 48)
+    f = ra.make_frame(s, hs, 0, eye, vp, inv, eye, vp, 64, 48, False)
+    sh = ra.shadow_setup((0.1, 1.2, -0.2), hs.radius, 1024)
+    assert f.shadows_enabled == 1 and f.shadow_width == 3072 and f.shadow_height == 2048
+    assert list(f.shadow_view_proj) == list(sh.view_proj) and list(f.shadow_info) == list(sh.info)
+    assert sh.info[0] == np.float32(2.5 * np.float32(hs.radius))
+    assert abs(sh.info[1] - 1024.0 / 1016.0) < 1e-6 and sh.info[2] == 1.0 / 1024.0
+    off = ra.make_frame(ra.Settings(), hs, 0, eye, vp, inv, eye, vp, 64, 48, False)
+    assert off.shadows_enabled == 0 and off.shadow_width == 0
+
+
 # ---------------------------------------------------------------- lights
 
 def test_light_list_cdf_and_power():
