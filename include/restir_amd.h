@@ -287,6 +287,21 @@ int rgi_shadow_apply(const rgi_shadow* shadow, int enabled, rgi_frame* frame);
 int rgi_shadow_atlas(rgi_context* context, const rgi_shadow* shadow, const rgi_scene* scene, float* atlas,
                      void* stream);
 
+/* The denoiser (renderer.cpp:857-907), over the GI planes of frame->width x
+ * frame->height texels. Step 1, fs_svgf_variance.sc: from this frame's
+ * accumulated colour and moments (rgi_gi_spatial's outputs) and the GI
+ * receiver and normal planes (rgi_gi_reservoirs.receiver, .normal), writes
+ * rgb = colour, a = luminance variance. */
+int rgi_svgf_variance(rgi_context* context, const rgi_frame* frame, const uint16_t* color, const uint16_t* moments,
+                      const float* receiver, const float* normal, uint16_t* out, void* stream);
+
+/* Step 2, fs_svgf_atrous.sc: one edge-avoiding a-trous iteration with taps
+ * `step` texels apart (1, 2, 4, 8, 16 for iterations 0-4), input and output
+ * as rgi_svgf_variance writes them; `out` must not alias `input`. The
+ * reference keeps iteration 0's output as next frame's GI history colour. */
+int rgi_svgf_atrous(rgi_context* context, const rgi_frame* frame, const uint16_t* input, const float* receiver,
+                    const float* normal, int32_t step, uint16_t* out, void* stream);
+
 /* The image the error metric compares, per pixel (linear RGB): the GI
  * estimate re-modulated by the diffuse albedo, color.rgb * albedo.rgb *
  * (1 - metallic), plus the DI diffuse and specular radiance when those planes

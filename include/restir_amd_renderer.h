@@ -52,7 +52,9 @@ typedef struct rgi_settings {
     int32_t origin_bottom_left;   /* 0 */
     int32_t homogeneous_depth;    /* 0 */
     int32_t shadow_face_size;     /* 1: index into 256, 512, 1024, 2048 (kShadowFaceSizes) */
-    int32_t reserved[3];
+    int32_t denoiser;             /* 1: variance + a-trous passes; GI history colour = filtered history */
+    int32_t atrous_iterations;    /* 5 (kMaxAtrousIterations), 0..5 */
+    int32_t reserved[1];
 } rgi_settings;
 
 /* One frame's camera: eye and (jittered) view-projection and its inverse. */
@@ -72,6 +74,8 @@ typedef struct rgi_renderer_outputs {
     const uint32_t* grid;         /* this frame's hash grid */
     const uint16_t* di_diffuse;   /* RGBA16F (NULL when DI is off) */
     const uint16_t* di_specular;
+    const uint16_t* filtered_history; /* RGBA16F first a-trous iteration (next frame's GI history colour) */
+    const uint16_t* denoised;     /* RGBA16F the composite's GI input: the last a-trous output, or `color` */
 } rgi_renderer_outputs;
 
 typedef struct rgi_renderer rgi_renderer;

@@ -1279,6 +1279,35 @@ bool same_surface(const Pass& p, V3 position, V3 n, V3 other_position, V3 other_
            fabsf(dot(off, n)) <= 0.03f * scene_scale(p) && dot(normalize(other_normal), n) >= 0.9f;
 }
 
+// ------------------------------------------------------------- denoiser
+
+// geometryWeight (svgf.sh:27-37): exp(-plane distance / (0.02 scale)) times
+// the normal cosine to the 64th power; 0 for a neighbour without a normal.
+float svgf_geometry_weight(float scale, V3 p, V3 n, V3 np, V3 nn)
+{
+    if (dot(nn, nn) < 0.5f) return 0.0f;
+    float plane = fabsf(dot(n, np - p)) / (0.02f * scale);
+    float normal_weight = rgi_pow64(fmax_(dot(n, normalize(nn)), 0.0f));
+    return rgi_expf(-plane) * normal_weight;
+}
+
+// isInside (svgf.sh:39-42).
+bool svgf_inside(V2 uv) { return uv.x >= 0.0f && uv.x <= 1.0f && uv.y >= 0.0f && uv.y <= 1.0f; }
+
+// Plane fetches at a uv, point sampled with clamp.
+struct Planes {
+    int w, h;
+    V4 f4(const float* p, V2 uv) const { return float4(p + texel(uv, w, h) * 4); }
+    V4 h4(const uint16_t* p, V2 uv) const { return half4(p + texel(uv, w, h) * 4); }
+};
+
+// splineWeight (fs_svgf_atrous.sc:29-33): the B3 spline 3/8, 1/4, 1/16.
+float spline_weight(int offset)
+{
+    int d = offset < 0 ? -offset : offset;
+    return d == 0 ? 0.375f : (d == 1 ? 0.25f : 0.0625f);
+}
+
 int thread_count = 0;
 
 template <typename F>
@@ -1331,6 +1360,107 @@ void orc_cosine_hemisphere(const float* n, float r0, float r1, float* out)
     out[2] = d.z;
 }
 void orc_sincos(float x, float* s, float* c) { rgi_sincos(x, s, c); }
+float orc_expf(float x) { return rgi_expf(x); }
+float orc_pow64(float x) { return rgi_pow64(x); }
+
+// fs_svgf_variance.sc:24-73: per-pixel luminance variance, from the pixel's
+// own moments after 4 frames of history, else from its 7 x 7 same-surface
+// neighbourhood, boosted by 4 / history length.
+void orc_svgf_variance(const rgi_frame* f, const uint16_t* color, const uint16_t* moments, const float* receiver,
+                       const float* normal, uint16_t* out)
+{
+    int w = f->width, h = f->height;
+    Planes t{w, h};
+    V2 ts = V2{1.0f / (float)w, 1.0f / (float)h};
+    float scale = f->scene_scale;
+    parallel_for((int64_t)w * h, [&](int64_t i) {
+        int x = (int)(i % w), y = (int)(i / w);
+        V2 uv0 = V2{((float)x + 0.5f) * ts.x, ((float)y + 0.5f) * ts.y};
+        V3 n = xyz(t.f4(normal, uv0));
+        if (dot(n, n) < 0.5f) {
+            store_half4(out + i * 4, V4{0.0f, 0.0f, 0.0f, 0.0f});
+            return;
+        }
+        n = normalize(n);
+        V3 c = xyz(t.h4(color, uv0));
+        V3 m = xyz(t.h4(moments, uv0));
+        float history = m.z;
+        if (history >= 4.0f) {
+            store_half4(out + i * 4, V4{c.x, c.y, c.z, fmax_(m.y - m.x * m.x, 0.0f)});
+            return;
+        }
+        V3 p = xyz(t.f4(receiver, uv0));
+        float sx = 0.0f, sy = 0.0f, wsum = 0.0f;
+        for (int dy = -3; dy <= 3; ++dy) {
+            for (int dx = -3; dx <= 3; ++dx) {
+                V2 uv = V2{uv0.x + (float)dx * ts.x, uv0.y + (float)dy * ts.y};
+                if (!svgf_inside(uv)) continue;
+                float wt = svgf_geometry_weight(scale, p, n, xyz(t.f4(receiver, uv)), xyz(t.f4(normal, uv)));
+                V4 mm = t.h4(moments, uv);
+                sx = sx + mm.x * wt;
+                sy = sy + mm.y * wt;
+                wsum = wsum + wt;
+            }
+        }
+        float d = fmax_(wsum, 0.0001f);
+        float mx = sx / d, my = sy / d;
+        float variance = fmax_(my - mx * mx, 0.0f) * (4.0f / fmax_(history, 1.0f));
+        store_half4(out + i * 4, V4{c.x, c.y, c.z, variance});
+    });
+}
+
+// fs_svgf_atrous.sc:35-101: one a-trous iteration with taps `step` texels
+// apart; rgb filtered with the weights, variance with their squares.
+void orc_svgf_atrous(const rgi_frame* f, const uint16_t* input, const float* receiver, const float* normal,
+                     int step, uint16_t* out)
+{
+    int w = f->width, h = f->height;
+    Planes t{w, h};
+    V2 ts = V2{1.0f / (float)w, 1.0f / (float)h};
+    float scale = f->scene_scale;
+    V2 spacing = V2{(float)step * ts.x, (float)step * ts.y};
+    parallel_for((int64_t)w * h, [&](int64_t i) {
+        int x = (int)(i % w), y = (int)(i / w);
+        V2 uv0 = V2{((float)x + 0.5f) * ts.x, ((float)y + 0.5f) * ts.y};
+        V3 n = xyz(t.f4(normal, uv0));
+        if (dot(n, n) < 0.5f) {
+            store_half4(out + i * 4, V4{0.0f, 0.0f, 0.0f, 0.0f});
+            return;
+        }
+        n = normalize(n);
+        V3 p = xyz(t.f4(receiver, uv0));
+        V4 center = t.h4(input, uv0);
+        float center_lum = luminance(xyz(center));
+        float variance = 0.0f;
+        for (int by = -1; by <= 1; ++by) {
+            for (int bx = -1; bx <= 1; ++bx) {
+                float wt = (bx == 0 ? 0.5f : 0.25f) * (by == 0 ? 0.5f : 0.25f);
+                V2 uv = V2{uv0.x + (float)bx * ts.x, uv0.y + (float)by * ts.y};
+                variance = variance + t.h4(input, uv).w * wt;
+            }
+        }
+        float lum_scale = 1.0f / (4.0f * sqrtf(fmax_(variance, 0.0f)) + 0.0001f);
+        V3 csum = v3s(0.0f);
+        float vsum = 0.0f, wsum = 0.0f;
+        for (int dy = -2; dy <= 2; ++dy) {
+            for (int dx = -2; dx <= 2; ++dx) {
+                V2 uv = V2{uv0.x + (float)dx * spacing.x, uv0.y + (float)dy * spacing.y};
+                if (!svgf_inside(uv)) continue;
+                V4 tap = t.h4(input, uv);
+                float wt = spline_weight(dx) * spline_weight(dy);
+                if (dx != 0 || dy != 0) {
+                    wt = wt * svgf_geometry_weight(scale, p, n, xyz(t.f4(receiver, uv)), xyz(t.f4(normal, uv)));
+                    wt = wt * rgi_expf(-fabsf(luminance(xyz(tap)) - center_lum) * lum_scale);
+                }
+                csum = csum + xyz(tap) * wt;
+                vsum = vsum + (tap.w * wt) * wt;
+                wsum = wsum + wt;
+            }
+        }
+        V3 c = csum / wsum;
+        store_half4(out + i * 4, V4{c.x, c.y, c.z, vsum / (wsum * wsum)});
+    });
+}
 void orc_brdf(const float* n, const float* v, const float* l, const float* a, float roughness, float metallic,
               float* diffuse, float* specular)
 {

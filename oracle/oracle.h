@@ -39,6 +39,15 @@ void orc_grid_key(const rgi_frame* frame, const float* position3, const float* n
                   uint32_t* slot, uint32_t* checksum);  /* hashgrid.sh:116-127 */
 void orc_cosine_hemisphere(const float* normal3, float r0, float r1, float* out3); /* restir.sh:201-212 */
 void orc_sincos(float x, float* s, float* c);
+float orc_expf(float x);
+float orc_pow64(float x);
+
+/* The SVGF denoiser passes (fs_svgf_variance.sc, fs_svgf_atrous.sc) over
+ * frame->width x frame->height half4 planes. */
+void orc_svgf_variance(const rgi_frame* frame, const uint16_t* color, const uint16_t* moments,
+                       const float* receiver, const float* normal, uint16_t* out);
+void orc_svgf_atrous(const rgi_frame* frame, const uint16_t* input, const float* receiver, const float* normal,
+                     int step, uint16_t* out);
 /* evaluateBrdf (brdf.sh:38-62): diffuse and specular (each times n.l). */
 void orc_brdf(const float* n3, const float* v3, const float* l3, const float* albedo3, float roughness,
               float metallic, float* diffuse3, float* specular3);

@@ -92,7 +92,8 @@ class CSettings(ctypes.Structure):
         ("restir_di", I32), ("emissive_lighting", I32), ("sky", I32), ("shadows", I32), ("grid_mode", I32),
         ("grid_table_size", U32), ("sky_zenith", F * 3), ("sky_horizon", F * 3), ("sky_ground", F * 3),
         ("sky_intensity", F), ("light_position", F * 3), ("light_color", F * 3), ("origin_bottom_left", I32),
-        ("homogeneous_depth", I32), ("shadow_face_size", I32), ("reserved", I32 * 3),
+        ("homogeneous_depth", I32), ("shadow_face_size", I32), ("denoiser", I32), ("atrous_iterations", I32),
+        ("reserved", I32 * 1),
     ]
 
 
@@ -103,7 +104,7 @@ class View(ctypes.Structure):
 
 class RendererOutputs(ctypes.Structure):
     _fields_ = [("gi_sample", P), ("gi_receiver", P), ("gi_normal", P), ("color", P), ("moments", P), ("grid", P),
-                ("di_diffuse", P), ("di_specular", P)]
+                ("di_diffuse", P), ("di_specular", P), ("filtered_history", P), ("denoised", P)]
 
 
 class Shadow(ctypes.Structure):
@@ -153,6 +154,8 @@ def lib():
                                  ctypes.POINTER(DiReservoirs), P, P, P]
     L.rgi_trace.argtypes = [P, ctypes.POINTER(Scene), P, U32, P, P, ctypes.c_int, P]
     L.rgi_gbuffer_primary.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), P, U32, P, P, P, P, P, P]
+    L.rgi_svgf_variance.argtypes = [P, ctypes.POINTER(Frame), P, P, P, P, P, P]
+    L.rgi_svgf_atrous.argtypes = [P, ctypes.POINTER(Frame), P, P, P, I32, P, P]
     L.rgi_shadow_setup.argtypes = [P, F, I32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Shadow)]
     L.rgi_shadow_apply.argtypes = [ctypes.POINTER(Shadow), ctypes.c_int, ctypes.POINTER(Frame)]
     L.rgi_shadow_atlas.argtypes = [P, ctypes.POINTER(Shadow), ctypes.POINTER(Scene), P, P]
@@ -334,6 +337,8 @@ class Settings:
         self.light_position = (0.0, 1.7, 0.0)
         self.shadows = False
         self.shadow_face_size = 1      # index into 256, 512, 1024, 2048
+        self.denoiser = True           # SVGF variance + a-trous (settings.h:190)
+        self.atrous_iterations = 5     # kMaxAtrousIterations
         self.grid_mode = RGI_GRID_ATOMIC
         self.grid_table_size = 65536
         for k, v in kw.items():
@@ -373,6 +378,8 @@ def c_settings(settings):
     c.light_position[:] = list(s.light_position)
     c.light_color[:] = list(s.light_color)
     c.shadow_face_size = int(s.shadow_face_size)
+    c.denoiser = int(bool(s.denoiser))
+    c.atrous_iterations = int(s.atrous_iterations)
     return c
 
 

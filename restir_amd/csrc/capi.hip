@@ -294,6 +294,32 @@ int rgi_shadow_atlas(rgi_context* ctx, const rgi_shadow* shadow, const rgi_scene
     return shadow_atlas(*shadow, *scene, atlas, as_stream(stream));
 }
 
+int rgi_svgf_variance(rgi_context* ctx, const rgi_frame* frame, const uint16_t* color, const uint16_t* moments,
+                      const float* receiver, const float* normal, uint16_t* out, void* stream)
+{
+    (void)ctx;
+    if (!frame) return fail(RGI_E_INVALID, "frame is NULL");
+    if (frame->width <= 0 || frame->height <= 0) return fail(RGI_E_INVALID, "frame width/height must be positive");
+    if (!color || !moments || !receiver || !normal || !out) return fail(RGI_E_INVALID, "svgf_variance: NULL plane");
+    if (!aligned(color, 8) || !aligned(moments, 8) || !aligned(out, 8) || !aligned(receiver, 16) || !aligned(normal, 16))
+        return fail(RGI_E_INVALID, "svgf_variance: misaligned plane");
+    return svgf_variance(*frame, color, moments, receiver, normal, out, as_stream(stream));
+}
+
+int rgi_svgf_atrous(rgi_context* ctx, const rgi_frame* frame, const uint16_t* input, const float* receiver,
+                    const float* normal, int32_t step, uint16_t* out, void* stream)
+{
+    (void)ctx;
+    if (!frame) return fail(RGI_E_INVALID, "frame is NULL");
+    if (frame->width <= 0 || frame->height <= 0) return fail(RGI_E_INVALID, "frame width/height must be positive");
+    if (!input || !receiver || !normal || !out) return fail(RGI_E_INVALID, "svgf_atrous: NULL plane");
+    if (input == out) return fail(RGI_E_INVALID, "svgf_atrous: out must not alias input");
+    if (step < 1 || step > 1 << 20) return fail(RGI_E_INVALID, "svgf_atrous: step %d out of range", step);
+    if (!aligned(input, 8) || !aligned(out, 8) || !aligned(receiver, 16) || !aligned(normal, 16))
+        return fail(RGI_E_INVALID, "svgf_atrous: misaligned plane");
+    return svgf_atrous(*frame, input, receiver, normal, step, out, as_stream(stream));
+}
+
 int rgi_gbuffer_primary(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const float* materials,
                         uint32_t material_count, float* depth, uint32_t* albedo, uint32_t* normal,
                         uint32_t* shading_normal, uint16_t* emission, void* stream)

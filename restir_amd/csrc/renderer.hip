@@ -30,6 +30,11 @@ struct rgi_renderer {
     // every later frame's shadow pass would write the same texels.
     float* atlas = nullptr;
     bool atlas_valid = false;
+    // Denoiser targets (renderer.cpp:1424-1430): the filtered history per
+    // parity and two ping-pong filter targets, RGBA16F.
+    uint16_t* filtered[2] = {};
+    uint16_t* filter[2] = {};
+    const uint16_t* denoised = nullptr;
     int read = 0;              // m_historyRead / m_diRead: the parity last frame wrote
     bool history_valid = false;
     bool di_valid = false;
@@ -75,6 +80,10 @@ void free_all(rgi_renderer* r)
     release(r->diffuse);
     release(r->specular);
     release(r->atlas);
+    for (int k = 0; k < 2; ++k) {
+        release(r->filtered[k]);
+        release(r->filter[k]);
+    }
 }
 
 // kShadowFaceSizes (settings.h:26); 0 for an index out of range.
@@ -95,6 +104,7 @@ rgi_settings effective(const rgi_renderer* r)
         s.clamp_outliers = 0;
         s.boiling_filter = 0;
         s.di_spatial_taps = 0;
+        s.denoiser = 0;
     }
     return s;
 }
@@ -131,6 +141,8 @@ void rgi_settings_default(rgi_settings* s)
     s->sky_intensity = 1.0f;
     s->light_position[1] = 1.7f;
     s->shadow_face_size = 1;
+    s->denoiser = 1;
+    s->atrous_iterations = 5;
 }
 
 uint32_t rgi_settings_sizeof(void) { return (uint32_t)sizeof(rgi_settings); }
@@ -215,6 +227,8 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         return fail(RGI_E_INVALID, "renderer_create: size %d x %d out of range", width, height);
     uint32_t T = settings->grid_table_size;
     if (T < 64u || (T & (T - 1u))) return fail(RGI_E_INVALID, "grid_table_size must be a power of two >= 64");
+    if (settings->atrous_iterations < 0 || settings->atrous_iterations > 5)
+        return fail(RGI_E_INVALID, "atrous_iterations %d is not in 0..5", settings->atrous_iterations);
     if (settings->shadows && shadow_face_size(settings) == 0)
         return fail(RGI_E_INVALID, "shadow_face_size index %d is not in 0..3", settings->shadow_face_size);
     rgi_renderer* r = new (std::nothrow) rgi_renderer();
@@ -237,6 +251,7 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         ok = ok && alloc(r->di_sample[k], 4 * n) && alloc(r->di_receiver[k], 4 * n) && alloc(r->di_normal[k], 4 * n);
     }
     ok = ok && alloc(r->diffuse, 4 * n) && alloc(r->specular, 4 * n);
+    for (int k = 0; k < 2 && ok; ++k) ok = alloc(r->filtered[k], 4 * n) && alloc(r->filter[k], 4 * n);
     if (settings->shadows) {
         size_t F = (size_t)shadow_face_size(settings);
         ok = ok && alloc(r->atlas, 6 * F * F);
@@ -327,15 +342,36 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     if (s.grid_mode == RGI_GRID_DETERMINISTIC) e = grid_insert_deterministic(r->ctx, f, *g, r->grid[write], stream);
     else e = grid_insert_atomic(f, *g, r->grid[write], stream);
     if (e != RGI_OK) return e;
-    rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
+    // With the denoiser, the GI history colour is its first filtered
+    // iteration (renderer.cpp:774-776).
+    const uint16_t* history_color = s.denoiser ? r->filtered[read] : r->color[read];
+    rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                       surface_light, shadow_map, r->grid[read]};
     rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
     if ((e = gi_temporal(f, r->scene, *g, tin, res, r->ctx->tuning, stream)) != RGI_OK) return e;
-    rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], r->color[read], r->moments[read],
+    rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                       surface_light, shadow_map, r->grid[write]};
     if ((e = gi_spatial(f, r->scene, *g, sin, res, r->color[write], r->moments[write], r->ctx->tuning, stream)) !=
         RGI_OK)
         return e;
+
+    // Denoiser (renderer.cpp:857-907): variance into filter target 0;
+    // iteration 0 writes the filtered history, iteration 1 target 1, then
+    // 1 -> 0, 0 -> 1, 1 -> 0.
+    r->denoised = r->color[write];
+    if (s.denoiser) {
+        const float* receiver = r->gi[write][1];
+        const float* normal = r->gi[write][2];
+        if ((e = svgf_variance(f, r->color[write], r->moments[write], receiver, normal, r->filter[0], stream)) !=
+            RGI_OK)
+            return e;
+        for (int it = 0; it < s.atrous_iterations; ++it) {
+            const uint16_t* input = it == 0 ? r->filter[0] : it == 1 ? r->filtered[write] : r->filter[(it - 1) & 1];
+            uint16_t* output = it == 0 ? r->filtered[write] : r->filter[it & 1];
+            if ((e = svgf_atrous(f, input, receiver, normal, 1 << it, output, stream)) != RGI_OK) return e;
+            r->denoised = output;
+        }
+    }
 
     // End of frame (renderer.cpp:1119-1124).
     r->previous = *view;
@@ -422,6 +458,8 @@ int rgi_renderer_outputs_get(const rgi_renderer* r, rgi_renderer_outputs* out)
     out->grid = r->grid[k];
     out->di_diffuse = r->diffuse;
     out->di_specular = r->specular;
+    out->filtered_history = r->filtered[k];
+    out->denoised = r->denoised ? r->denoised : r->color[k];
     return RGI_OK;
 }
 

@@ -11,6 +11,13 @@
 //                [-2^10, 2^10].
 //   rgi_half_bits  float -> binary16, round to nearest even, with subnormals,
 //                overflow to infinity and NaN kept quiet: IEEE 754 conversion.
+//   rgi_expf     exp for the denoiser's edge-stopping weights (svgf.sh,
+//                fs_svgf_atrous.sc): Cody-Waite reduction by ln 2 (13-bit
+//                high part, exact times |k| <= 150), a degree-7 Taylor
+//                polynomial of e^r on |r| <= ln(2)/2, and scaling by 2^k
+//                through the exponent bits, with one rounding into the
+//                subnormal range. Within 2 ulp of the true value.
+//   rgi_pow64    pow(x, 64) (kNormalPower, svgf.sh:14) as six squarings.
 //
 // Every function is plain float arithmetic in a fixed order, so it rounds
 // the same wherever it is compiled with FP contraction off.
@@ -144,4 +151,49 @@ RGI_HD float rgi_half_float(uint16_t bits)
         return rgi_bits_float(sign | 0x7f800000u | (mantissa << 13));
     }
     return rgi_bits_float(sign | ((exponent + 112u) << 23) | (mantissa << 13));
+}
+
+// ln 2 = RGI_LN2_HI + RGI_LN2_LO, RGI_LN2_HI with 13 significant bits.
+#define RGI_LN2_HI 0x1.62ep-1f
+#define RGI_LN2_LO 0x1.0bfbe8p-15f
+#define RGI_LOG2E 0x1.715476p+0f
+
+RGI_HD float rgi_expf(float x)
+{
+    if (x != x) return x;
+    if (x > 0x1.62e42ep+6f) return __builtin_inff();
+    if (x < -104.0f) return 0.0f;
+    float kf = __builtin_floorf(x * RGI_LOG2E + 0.5f);
+    float r = (x - kf * RGI_LN2_HI) - kf * RGI_LN2_LO;
+    // 1/n! rounded to float, n = 2 .. 7.
+    float q = 0x1.a01a02p-13f;             // 1/7!
+    q = q * r + 0x1.6c16c2p-10f;           // 1/6!
+    q = q * r + 0x1.111112p-7f;            // 1/5!
+    q = q * r + 0x1.555556p-5f;            // 1/4!
+    q = q * r + 0x1.555556p-3f;            // 1/3!
+    q = q * r + 0.5f;                      // 1/2!
+    float p = 1.0f + (r + (r * r) * q);
+    int k = (int)kf;
+    if (k > 127) {
+        p = p * 2.0f;
+        k -= 1;
+    }
+    if (k < -149) {
+        p = p * 0.5f;
+        k += 1;
+    }
+    uint32_t bits = k >= -126 ? (uint32_t)(k + 127) << 23 : 1u << (uint32_t)(k + 149);
+    float scale;
+    memcpy(&scale, &bits, 4);
+    return p * scale;
+}
+
+RGI_HD float rgi_pow64(float x)
+{
+    float x2 = x * x;
+    float x4 = x2 * x2;
+    float x8 = x4 * x4;
+    float x16 = x8 * x8;
+    float x32 = x16 * x16;
+    return x32 * x32;
 }

@@ -38,6 +38,7 @@ def empty_history(width, height, table_size):
         "gi_normal": np.zeros((n, 4), np.float32),
         "color": np.zeros((n, 4), np.uint16),
         "moments": np.zeros((n, 4), np.uint16),
+        "filtered": np.zeros((n, 4), np.uint16),
         "grid": np.zeros(34 * table_size, np.uint32),
         "di_sample": np.zeros((n, 4), np.float32),
         "di_receiver": np.zeros((n, 4), np.float32),
@@ -82,7 +83,28 @@ def oracle_shadow_atlas(hs, frame):
     return atlas
 
 
-def oracle_frame(hs, frame, g, hist, insert=True):
+def oracle_denoise(frame, out, atrous_iterations):
+    """The denoiser of Renderer::render (renderer.cpp:857-907) on the oracle:
+    returns (filtered history, denoised)."""
+    L = orc.lib()
+    n = frame.width * frame.height
+    targets = [np.zeros((n, 4), np.uint16), np.zeros((n, 4), np.uint16)]
+    filtered = None
+    rec, nor = out["gi_receiver"], out["gi_normal"]
+    L.orc_svgf_variance(ctypes.byref(frame), orc.ptr(out["color"]), orc.ptr(out["moments"]), orc.ptr(rec),
+                        orc.ptr(nor), orc.ptr(targets[0]))
+    denoised = out["color"]
+    for it in range(atrous_iterations):
+        src = targets[0] if it == 0 else filtered if it == 1 else targets[(it - 1) & 1]
+        dst = np.zeros((n, 4), np.uint16) if it == 0 else targets[it & 1]
+        L.orc_svgf_atrous(ctypes.byref(frame), orc.ptr(src), orc.ptr(rec), orc.ptr(nor), 1 << it, orc.ptr(dst))
+        if it == 0:
+            filtered = dst
+        denoised = dst.copy()
+    return filtered, denoised
+
+
+def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterations=5):
     """One frame on the oracle; returns this frame's outputs (host arrays)."""
     L = orc.lib()
     w, h, T = frame.width, frame.height, frame.grid_table_size
@@ -112,14 +134,16 @@ def oracle_frame(hs, frame, g, hist, insert=True):
     if insert:
         L.orc_grid_insert(ctypes.byref(frame), ctypes.byref(gb), orc.ptr(grid))
     out["grid"] = grid
+    # With the denoiser, the GI history colour is the filtered history (renderer.cpp:774-776).
+    history_color = hist["filtered"] if denoiser else hist["color"]
     gin = ra.GiInputs(hist["gi_sample"].ctypes.data, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
-                      hist["color"].ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, shadow_ptr,
+                      history_color.ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, shadow_ptr,
                       hist["grid"].ctypes.data)
     for k in GI_PLANES_F32:
         out["gi_" + k] = np.zeros((n, 4), np.float32)
     gres = ra.GiReservoirs(out["gi_sample"].ctypes.data, out["gi_receiver"].ctypes.data, out["gi_normal"].ctypes.data)
     L.orc_gi_temporal(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(gin), ctypes.byref(gres))
-    sin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data, hist["color"].ctypes.data,
+    sin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data, history_color.ctypes.data,
                       hist["moments"].ctypes.data, surface_light.ctypes.data, shadow_ptr, grid.ctypes.data)
     out["color"] = np.zeros((n, 4), np.uint16)
     out["moments"] = np.zeros((n, 4), np.uint16)
@@ -128,6 +152,12 @@ def oracle_frame(hs, frame, g, hist, insert=True):
     out["surface_light"] = surface_light
     if atlas is not None:
         out["shadow_atlas"] = atlas
+    out["denoised"] = out["color"]
+    if denoiser:
+        filtered, denoised = oracle_denoise(frame, out, atrous_iterations)
+        if filtered is not None:
+            out["filtered"] = filtered
+        out["denoised"] = denoised
     return out
 
 
@@ -136,6 +166,8 @@ def next_history(hist, out):
     h = dict(hist)
     for k in ("gi_sample", "gi_receiver", "gi_normal", "color", "moments", "grid"):
         h[k] = out[k]
+    if "filtered" in out:
+        h["filtered"] = out["filtered"]
     if "di_sample" in out:
         for k in ("di_sample", "di_receiver", "di_normal"):
             h[k] = out[k]
@@ -151,7 +183,7 @@ def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0, distance=
         f = frame_for(hs, settings, width, height, i, history_valid=i > 0, di_history_valid=i > 0 and di_on,
                       yaw_step=yaw_step, distance=distance)
         g = oracle_gbuffer(hs, f)
-        out = oracle_frame(hs, f, g, hist)
+        out = oracle_frame(hs, f, g, hist, denoiser=settings.denoiser, atrous_iterations=settings.atrous_iterations)
         seq.append((f, g, hist, out))
         hist = next_history(hist, out)
     return seq

@@ -78,6 +78,14 @@ def lib():
     L.orc_reference_accumulate.argtypes = [PG, P, P, P, P]
     L.orc_reference_error.restype = None
     L.orc_reference_error.argtypes = [PG, P, P, P, P, U32, P]
+    L.orc_expf.restype = F
+    L.orc_expf.argtypes = [F]
+    L.orc_pow64.restype = F
+    L.orc_pow64.argtypes = [F]
+    L.orc_svgf_variance.restype = None
+    L.orc_svgf_variance.argtypes = [PF, P, P, P, P, P]
+    L.orc_svgf_atrous.restype = None
+    L.orc_svgf_atrous.argtypes = [PF, P, P, P, ctypes.c_int, P]
     L.orc_shadow_atlas.restype = None
     L.orc_shadow_atlas.argtypes = [ctypes.POINTER(Shadow), PS, P]
     L.orc_gbuffer_primary.restype = None

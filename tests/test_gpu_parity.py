@@ -191,6 +191,34 @@ def test_shadow_atlas(dev, scene, face, obl, hd):
     assert_same("atlas", got, want)
 
 
+@pytest.mark.parametrize("step", [1, 2, 16])
+def test_svgf(dev, cornell_seq, step):
+    """rgi_svgf_variance and rgi_svgf_atrous against the oracle, bit-exact,
+    on a frame's accumulated colour, moments and reservoir planes (frame 1
+    has history shorter than 4 frames everywhere: the spatial estimate)."""
+    hs, s, seq = cornell_seq
+    for frame_no in (1, 2):
+        f, g, hist, out = seq[frame_no]
+        say("case svgf frame=%d step=%d" % (frame_no, step))
+        n = f.width * f.height
+        want_v = np.zeros((n, 4), np.uint16)
+        L = orc.lib()
+        L.orc_svgf_variance(ctypes.byref(f), orc.ptr(out["color"]), orc.ptr(out["moments"]),
+                            orc.ptr(out["gi_receiver"]), orc.ptr(out["gi_normal"]), orc.ptr(want_v))
+        want_a = np.zeros((n, 4), np.uint16)
+        L.orc_svgf_atrous(ctypes.byref(f), orc.ptr(want_v), orc.ptr(out["gi_receiver"]), orc.ptr(out["gi_normal"]),
+                          step, orc.ptr(want_a))
+        d = {k: dev.put(out[k]) for k in ("color", "moments", "gi_receiver", "gi_normal")}
+        var_t, atr_t = dev.empty(n * 8), dev.empty(n * 8)
+        ra.check(dev.L.rgi_svgf_variance(dev.ctx, ctypes.byref(f), d["color"].data_ptr(), d["moments"].data_ptr(),
+                                         d["gi_receiver"].data_ptr(), d["gi_normal"].data_ptr(), var_t.data_ptr(),
+                                         dev.stream))
+        ra.check(dev.L.rgi_svgf_atrous(dev.ctx, ctypes.byref(f), var_t.data_ptr(), d["gi_receiver"].data_ptr(),
+                                       d["gi_normal"].data_ptr(), step, atr_t.data_ptr(), dev.stream))
+        assert_same("variance", dev.get(var_t, want_v), want_v)
+        assert_same("atrous", dev.get(atr_t, want_a), want_a)
+
+
 @pytest.mark.parametrize("table", [65536, 1024, 64])
 def test_grid_insert_deterministic(dev, cornell_seq, table):
     hs, s, seq = cornell_seq

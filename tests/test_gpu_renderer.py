@@ -92,6 +92,8 @@ def test_renderer_matches_oracle_sequence(torch, frames, kw, variants, scene):
                       ("color", out.color), ("moments", out.moments), ("grid", out.grid)]
             if f.di_light_count > 0:
                 checks += [("diffuse", out.di_diffuse), ("specular", out.di_specular)]
+            if s.denoiser:
+                checks += [("filtered", out.filtered_history), ("denoised", out.denoised)]
             for name, ptr in checks:
                 got = download(torch, ptr, want[name])
                 if name == "grid":

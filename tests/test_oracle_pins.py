@@ -345,6 +345,143 @@ def test_trace_against_brute_force(kind, size):
         assert agree >= n - 3, (any_hit, n - agree)
 
 
+# -------------------------------------------------------------- denoiser
+
+def test_expf_against_libm():
+    """rgi_expf (the denoiser's exp) is within 2 ulp of the true exp over the
+    whole float range, including subnormal results, overflow and underflow."""
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.uniform(-104.0, 88.72, 60000), rng.uniform(-20.0, 0.0, 20000),
+                         np.array([0.0, -0.0, 1e-30, -1e-30, 1.0, -1.0, 88.7228, -87.3, -103.9, -104.5, 89.0,
+                                   float("inf"), float("-inf")])]).astype(np.float32)
+    worst = 0
+    for x in xs:
+        got = np.float32(L.orc_expf(float(x)))
+        want = np.float32(math.exp(float(x))) if float(x) < 89.0 else np.float32(np.inf)
+        if not np.isfinite(want) or want == 0.0:
+            assert got == want or (want == 0.0 and got <= np.float32(2.0 ** -149)), (x, got, want)
+            continue
+        ulp = abs(int(got.view(np.int32)) - int(want.view(np.int32)))
+        worst = max(worst, ulp)
+    assert worst <= 2, worst
+    assert math.isnan(L.orc_expf(float("nan")))
+
+
+def test_pow64_is_the_64th_power():
+    """kNormalPower (svgf.sh:14): six squarings; their roundings compound to
+    at most 63 half-ulps (2^-24 relative each) of x^64."""
+    x = np.random.default_rng(6).random(5000).astype(np.float32)
+    got = np.array([L.orc_pow64(float(v)) for v in x], np.float64)
+    want = x.astype(np.float64) ** 64
+    ok = want > 1e-30
+    assert np.all(np.abs(got[ok] - want[ok]) <= 64 * 2.0 ** -24 * want[ok])
+
+
+def svgf_frame(w, h, scale=1.0):
+    f = ra.Frame()
+    f.width, f.height, f.gi_width, f.gi_height, f.scene_scale = w, h, w, h, scale
+    return f
+
+
+def half4(rgb, a):
+    out = np.zeros(rgb.shape[:-1] + (4,), np.float16)
+    out[..., :3], out[..., 3] = rgb, a
+    return out.view(np.uint16)
+
+
+def wall(w, h, normal=(0.0, 0.0, 1.0)):
+    """A flat wall z = 0 seen face on: receiver positions on a 0.01 grid."""
+    rec = np.zeros((h, w, 4), np.float32)
+    yy, xx = np.mgrid[0:h, 0:w]
+    rec[..., 0], rec[..., 1] = xx * 0.01, yy * 0.01
+    nor = np.zeros((h, w, 4), np.float32)
+    nor[..., :3] = normal
+    return rec, nor
+
+
+def test_svgf_variance_closed_forms():
+    """fs_svgf_variance.sc: with 4+ frames of history the pixel's own
+    moments give the variance; with less, the mean moments of its 7 x 7
+    in-image window (all on one plane, so every geometry weight is 1),
+    times 4 / history; sky texels are zero."""
+    w, h = 24, 16
+    f = svgf_frame(w, h)
+    rec, nor = wall(w, h)
+    nor[0, 0, :3] = 0.0  # one sky texel
+    rng = np.random.default_rng(8)
+    m1 = rng.uniform(0.1, 1.0, (h, w)).astype(np.float16)
+    m2 = (m1.astype(np.float32) ** 2 + rng.uniform(0.0, 0.5, (h, w))).astype(np.float16)
+    history = np.where(rng.random((h, w)) < 0.5, 2.0, 6.0).astype(np.float16)
+    moments = np.zeros((h, w, 4), np.float16)
+    moments[..., 0], moments[..., 1], moments[..., 2] = m1, m2, history
+    color = half4(rng.random((h, w, 3)).astype(np.float32), history.astype(np.float32))
+    out = np.zeros((h, w, 4), np.uint16)
+    L.orc_svgf_variance(ctypes.byref(f), orc.ptr(color), orc.ptr(moments.view(np.uint16)), orc.ptr(rec),
+                        orc.ptr(nor), orc.ptr(out))
+    got = out.view(np.float16).astype(np.float64)
+    assert np.all(got[0, 0] == 0.0)
+    a, b = m1.astype(np.float64), m2.astype(np.float64)
+    sky = np.zeros((h, w), bool)
+    sky[0, 0] = True
+    for y in range(h):
+        for x in range(w):
+            if sky[y, x]:
+                continue
+            assert np.array_equal(got[y, x, :3], color.view(np.float16)[y, x, :3].astype(np.float64))
+            if history[y, x] >= 4:
+                want = max(b[y, x] - a[y, x] ** 2, 0.0)
+            else:
+                ys, xs = slice(max(y - 3, 0), y + 4), slice(max(x - 3, 0), x + 4)
+                keep = ~sky[ys, xs]
+                ma, mb = a[ys, xs][keep].mean(), b[ys, xs][keep].mean()
+                want = max(mb - ma * ma, 0.0) * 4.0 / max(float(history[y, x]), 1.0)
+            assert abs(got[y, x, 3] - want) <= 2e-3 * max(want, 1e-2), (y, x, got[y, x, 3], want)
+
+
+def run_atrous(f, inp, rec, nor, step):
+    out = np.zeros(inp.shape, np.uint16)
+    L.orc_svgf_atrous(ctypes.byref(f), orc.ptr(inp), orc.ptr(rec), orc.ptr(nor), step, orc.ptr(out))
+    return out.view(np.float16).astype(np.float64)
+
+
+def test_svgf_atrous_kernel_edges_and_constants():
+    """fs_svgf_atrous.sc: (1) with a huge variance the luminance weight is 1
+    and an impulse spreads as the separable B3 spline (3/8, 1/4, 1/16) at
+    taps `step` texels apart; (2) a constant image stays constant; (3) no
+    weight crosses to a perpendicular surface (pow(cos, 64) = 0)."""
+    w, h = 40, 40
+    f = svgf_frame(w, h)
+    rec, nor = wall(w, h)
+    for step in (1, 4):
+        rgb = np.zeros((h, w, 3), np.float32)
+        rgb[20, 20] = 1.0
+        got = run_atrous(f, half4(rgb, np.full((h, w), 60000.0, np.float32)), rec, nor, step)
+        spline = {0: 0.375, 1: 0.25, 2: 0.0625}
+        for dy in range(-2, 3):
+            for dx in range(-2, 3):
+                want = spline[abs(dx)] * spline[abs(dy)]
+                v = got[20 + dy * step, 20 + dx * step, 0]
+                assert abs(v - want) <= 2e-3 * want + 1e-4, (step, dx, dy, v, want)
+        mask = np.zeros((h, w), bool)
+        for dy in range(-2, 3):
+            for dx in range(-2, 3):
+                mask[20 + dy * step, 20 + dx * step] = True
+        assert np.all(got[..., 0][~mask] == 0.0)
+    const = half4(np.full((h, w, 3), 0.3, np.float32), np.full((h, w), 0.01, np.float32))
+    got = run_atrous(f, const, rec, nor, 2)
+    assert np.all(np.abs(got[..., :3] - float(np.float16(0.3))) <= 1e-3)
+    assert np.all(got[..., 3] <= 0.01 + 1e-5)
+    # Left half faces +z, right half faces +x, in different colours.
+    rec2, nor2 = wall(w, h)
+    nor2[:, w // 2:, :3] = (1.0, 0.0, 0.0)
+    rgb = np.zeros((h, w, 3), np.float32)
+    rgb[:, : w // 2] = 0.2
+    rgb[:, w // 2:] = 0.8
+    got = run_atrous(f, half4(rgb, np.full((h, w), 60000.0, np.float32)), rec2, nor2, 1)
+    assert np.all(np.abs(got[:, : w // 2, 0] - float(np.float16(0.2))) <= 1e-3)
+    assert np.all(np.abs(got[:, w // 2:, 0] - float(np.float16(0.8))) <= 1e-3)
+
+
 # ---------------------------------------------------------- shadow atlas
 
 def shadow_face(d):
