@@ -163,7 +163,8 @@ typedef struct rgi_di_reservoirs {
 const char* rgi_last_error(void);
 int rgi_abi_version(void);
 /* sizeof of the ABI structs (0: rgi_frame, 1: rgi_scene, 2: rgi_gbuffer,
- * 3: rgi_gi_reservoirs, 4: rgi_gi_inputs, 5: rgi_di_reservoirs, 6: rgi_shadow), so that
+ * 3: rgi_gi_reservoirs, 4: rgi_gi_inputs, 5: rgi_di_reservoirs, 6: rgi_shadow,
+ * 7: rgi_composite_inputs), so that
  * bindings can check their mirrors. */
 uint32_t rgi_abi_sizeof(int which);
 
@@ -301,6 +302,30 @@ int rgi_svgf_variance(rgi_context* context, const rgi_frame* frame, const uint16
  * reference keeps iteration 0's output as next frame's GI history colour. */
 int rgi_svgf_atrous(rgi_context* context, const rgi_frame* frame, const uint16_t* input, const float* receiver,
                     const float* normal, int32_t step, uint16_t* out, void* stream);
+
+/* The composite pass's inputs besides the G-buffer (fs_composite.sc:10-29).
+ * GI planes are frame->width x frame->height (the GI resolution); the DI
+ * planes, the G-buffer and the output are gbuffer->width x height. */
+typedef struct rgi_composite_inputs {
+    const uint16_t* gi;          /* RGBA16F demodulated GI: the denoiser's output, or the accumulated colour */
+    const uint16_t* di_diffuse;  /* RGBA16F ReSTIR DI light (NULL: none) */
+    const uint16_t* di_specular; /* RGBA16F (NULL: none) */
+    const uint16_t* specular;    /* RGBA16F demodulated indirect specular (NULL: none) */
+    const float* shadow_map;     /* R32F shadow atlas (NULL: every light sample visible) */
+    float options[4];            /* u_compositeOptions: direct, indirect, albedo modulation, emission (final image: 1, 1, 1, 1) */
+    float specular_weight;       /* u_compositeSpecular.x: 1 with indirect specular */
+    float reserved[3];
+} rgi_composite_inputs;
+
+/* fs_composite.sc:82-143: the scene-referred HDR image, RGBA16F. Per pixel:
+ * the main light through the BRDF with the 4 x 4 soft shadow-atlas compare,
+ * plus DI diffuse and specular; the GI irradiance upsampled over the four
+ * nearest same-surface GI texels (fs_composite.sc:48-80) times albedo,
+ * (1 - metallic) and ambient occlusion (emission.a); indirect specular times
+ * the specular albedo; emission. Sky pixels: the sky radiance with alpha 1,
+ * or 0 when the sky is off. */
+int rgi_composite(rgi_context* context, const rgi_frame* frame, const rgi_gbuffer* gbuffer,
+                  const rgi_composite_inputs* inputs, uint16_t* out_hdr, void* stream);
 
 /* The image the error metric compares, per pixel (linear RGB): the GI
  * estimate re-modulated by the diffuse albedo, color.rgb * albedo.rgb *

@@ -533,23 +533,54 @@ V2 shadow_atlas_uv(const rgi_frame* f, V3 position, int face)
     return V2{(column + local.x) * (1.0f / 3.0f), (row + local.y) * 0.5f};
 }
 
-// lightVisibilityHard via shadowLookup with a filter radius of 0.5 (shadow.sh:72-108).
-float light_visibility_hard(const Pass& p, V3 position, V3 n)
+// shadowLookup (shadow.sh:72-97): the atlas uv of a surface point and the
+// biased, normalised distance compared there; false when shadows are off.
+bool shadow_lookup(const rgi_frame* f, const float* atlas, V3 position, V3 n, float filter_radius, V2& uv,
+                   float& reference)
 {
-    const rgi_frame* f = p.f;
-    if (!f->shadows_enabled || p.in == nullptr || p.in->shadow_map == nullptr) return 1.0f;
+    uv = V2{0.0f, 0.0f};
+    reference = 0.0f;
+    if (!f->shadows_enabled || atlas == nullptr) return false;
     V3 light = V3{f->light_position[0], f->light_position[1], f->light_position[2]};
     float light_distance = length(light - position);
     float texel_size = ((2.0f * light_distance) * f->shadow_info[1]) * f->shadow_info[2];
     V3 pos = position + n * (texel_size * 1.5f);
     V3 from_light = pos - light;
-    V2 uv = shadow_atlas_uv(f, pos, shadow_face(from_light));
+    uv = shadow_atlas_uv(f, pos, shadow_face(from_light));
     float cosine = clampf(dot(n, -normalize(from_light)), 0.0f, 1.0f);
     float slope = fmin_(sqrtf(1.0f - cosine * cosine) / fmax_(cosine, 0.0001f), 8.0f);
-    float bias = texel_size * (1.0f + 0.5f * slope);
-    float reference = (length(from_light) - bias) / f->shadow_info[0];
+    float bias = texel_size * (1.0f + filter_radius * slope);
+    reference = (length(from_light) - bias) / f->shadow_info[0];
+    return true;
+}
+
+// lightVisibilityHard (shadow.sh:99-108).
+float light_visibility_hard(const Pass& p, V3 position, V3 n)
+{
+    const rgi_frame* f = p.f;
+    V2 uv;
+    float reference;
+    if (p.in == nullptr || !shadow_lookup(f, p.in->shadow_map, position, n, 0.5f, uv, reference)) return 1.0f;
     float stored = p.in->shadow_map[texel(uv, f->shadow_width, f->shadow_height)];
     return reference <= stored ? 1.0f : 0.0f;
+}
+
+// lightVisibilitySoft (shadow.sh:113-134): 4 x 4 compares one atlas texel apart.
+float light_visibility_soft(const rgi_frame* f, const float* atlas, V3 position, V3 n)
+{
+    V2 uv;
+    float reference;
+    if (!shadow_lookup(f, atlas, position, n, 2.0f, uv, reference)) return 1.0f;
+    V2 atlas_texel = V2{f->shadow_info[2] / 3.0f, f->shadow_info[2] * 0.5f};
+    float lit = 0.0f;
+    for (int y = 0; y < 4; ++y) {
+        for (int x = 0; x < 4; ++x) {
+            V2 offset = V2{((float)x - 1.5f) * atlas_texel.x, ((float)y - 1.5f) * atlas_texel.y};
+            V2 tap = V2{uv.x + offset.x, uv.y + offset.y};
+            lit += reference <= atlas[texel(tap, f->shadow_width, f->shadow_height)] ? 1.0f : 0.0f;
+        }
+    }
+    return lit / 16.0f;
 }
 
 // --------------------------------------------------------------------- sky
@@ -1230,6 +1261,19 @@ void evaluate_brdf(V3 n, V3 v, V3 l, V3 albedo, float roughness_in, float metall
     diffuse = (((v3s(1.0f) - fr) * (1.0f - metallic)) * albedo / kPi) * nl;
 }
 
+// specularAlbedo (brdf.sh:68-76): Karis' fit of the split-sum specular albedo.
+V3 specular_albedo(V3 f0, float roughness, float nv)
+{
+    float rx = -1.0f * roughness + 1.0f;
+    float ry = -0.0275f * roughness + 0.0425f;
+    float rz = -0.572f * roughness + 1.04f;
+    float rw = 0.022f * roughness + -0.04f;
+    float a004 = fmin_(rx * rx, rgi_exp2f(-9.28f * nv)) * rx + ry;
+    float sx = -1.04f * a004 + rz;
+    float sy = 1.04f * a004 + rw;
+    return f0 * sx + v3s(sy);
+}
+
 // evaluateLightSample (di.sh:180-212).
 void evaluate_light_sample(const Pass& p, const Surface& s, int light, V3 point, V3& diffuse, V3& specular)
 {
@@ -1362,6 +1406,101 @@ void orc_cosine_hemisphere(const float* n, float r0, float r1, float* out)
 void orc_sincos(float x, float* s, float* c) { rgi_sincos(x, s, c); }
 float orc_expf(float x) { return rgi_expf(x); }
 float orc_pow64(float x) { return rgi_pow64(x); }
+float orc_exp2f(float x) { return rgi_exp2f(x); }
+void orc_specular_albedo(const float* f0, float roughness, float nv, float* out)
+{
+    V3 r = specular_albedo(V3{f0[0], f0[1], f0[2]}, roughness, nv);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+// fs_composite.sc:82-143 over gbuffer->width x height pixels; GI planes at
+// frame->width x height.
+void orc_composite(const rgi_frame* f, const rgi_gbuffer* g, const rgi_composite_inputs* in, uint16_t* out)
+{
+    int w = g->width, h = g->height;
+    V2 ts = V2{1.0f / (float)w, 1.0f / (float)h};
+    V2 gi_size = V2{(float)f->width, (float)f->height};
+    V2 gi_texel = V2{1.0f / (float)f->width, 1.0f / (float)f->height};
+    V3 camera = V3{f->camera_position[0], f->camera_position[1], f->camera_position[2]};
+    auto h4 = [](const uint16_t* plane, size_t index) {
+        return plane ? half4(plane + index * 4) : V4{0.0f, 0.0f, 0.0f, 0.0f};
+    };
+    auto gnormal = [](uint32_t word) {
+        V4 n = unorm8(word);
+        return normalize(V3{n.x * 2.0f - 1.0f, n.y * 2.0f - 1.0f, n.z * 2.0f - 1.0f});
+    };
+    parallel_for((int64_t)w * h, [&](int64_t i) {
+        int x = (int)(i % w), y = (int)(i / w);
+        V2 uv = V2{((float)x + 0.5f) * ts.x, ((float)y + 0.5f) * ts.y};
+        size_t t = texel(uv, w, h);
+        float depth = g->depth[t];
+        if (depth >= kSkyDepth) {
+            V3 far_position = reconstruct_world_position(f, uv, 1.0f);
+            V3 direction = normalize(far_position - camera);
+            V3 sky = sky_radiance(f, direction);
+            store_half4(out + i * 4, sky_enabled(f) ? V4{sky.x, sky.y, sky.z, 1.0f} : V4{0.0f, 0.0f, 0.0f, 0.0f});
+            return;
+        }
+        V4 albedo_roughness = unorm8(g->albedo[t]);
+        V4 normal_metallic = unorm8(g->normal[t]);
+        V3 albedo = xyz(albedo_roughness);
+        float roughness = fmax_(albedo_roughness.w, 0.05f);
+        float metallic = normal_metallic.w;
+        V3 surface_normal = normalize(V3{normal_metallic.x * 2.0f - 1.0f, normal_metallic.y * 2.0f - 1.0f,
+                                         normal_metallic.z * 2.0f - 1.0f});
+        V3 n = gnormal(g->shading_normal[t]);
+        V4 eo = half4(g->emission + t * 4);
+        V3 position = reconstruct_world_position(f, uv, depth);
+        V3 view_direction = normalize(camera - position);
+        V3 to_light = V3{f->light_position[0], f->light_position[1], f->light_position[2]} - position;
+        float distance_squared = fmax_(dot(to_light, to_light), (0.01f * f->scene_scale) * f->scene_scale);
+        V3 light_direction = to_light * inversesqrt(distance_squared);
+        V3 diffuse, specular;
+        evaluate_brdf(n, view_direction, light_direction, albedo, roughness, metallic, diffuse, specular);
+        float visibility = 0.0f;
+        if (dot(n, light_direction) > 0.0f)
+            visibility = light_visibility_soft(f, in->shadow_map, position, surface_normal);
+        V3 light_color = V3{f->light_color[0], f->light_color[1], f->light_color[2]};
+        V3 direct = (((diffuse + specular) * (light_color / distance_squared)) * visibility + xyz(h4(in->di_diffuse, t))) +
+                    xyz(h4(in->di_specular, t));
+        // upsampleIndirect (fs_composite.sc:48-80).
+        float tcx = uv.x * gi_size.x - 0.5f, tcy = uv.y * gi_size.y - 0.5f;
+        float bx = floorf(tcx), by = floorf(tcy);
+        float blx = tcx - bx, bly = tcy - by;
+        V3 sum = v3s(0.0f), specular_sum = v3s(0.0f);
+        float weight_sum = 0.0f;
+        for (int k = 0; k < 4; ++k) {
+            float ox = (float)(k & 1), oy = (float)(k >> 1);
+            V2 tuv = V2{((bx + ox) + 0.5f) * gi_texel.x, ((by + oy) + 0.5f) * gi_texel.y};
+            size_t gt = texel(tuv, w, h);
+            float tap_depth = g->depth[gt];
+            if (tap_depth >= kSkyDepth) continue;
+            V3 tap_position = reconstruct_world_position(f, tuv, tap_depth);
+            V3 tap_normal = gnormal(g->normal[gt]);
+            float wx = mixf(1.0f - blx, blx, ox), wy = mixf(1.0f - bly, bly, oy);
+            float plane_step = 0.03f * f->scene_scale >= fabsf(dot(tap_position - position, surface_normal)) ? 1.0f : 0.0f;
+            float normal_step = dot(tap_normal, surface_normal) >= 0.9f ? 1.0f : 0.0f;
+            float weight = ((wx * wy + 0.01f) * plane_step) * normal_step;
+            size_t it = texel(tuv, f->width, f->height);
+            sum = sum + xyz(h4(in->gi, it)) * weight;
+            specular_sum = specular_sum + xyz(h4(in->specular, it)) * weight;
+            weight_sum += weight;
+        }
+        size_t own = texel(uv, f->width, f->height);
+        V3 irradiance = weight_sum > 0.0f ? sum / weight_sum : xyz(h4(in->gi, own));
+        V3 specular_radiance = weight_sum > 0.0f ? specular_sum / weight_sum : xyz(h4(in->specular, own));
+        V3 modulate = in->options[2] > 0.5f ? (albedo * (1.0f - metallic)) * eo.w : v3s(1.0f);
+        V3 indirect = irradiance * modulate;
+        V3 f0 = mix3(v3s(0.04f), albedo, metallic);
+        indirect = indirect + ((specular_radiance * specular_albedo(f0, roughness, fmax_(dot(n, view_direction), 0.001f))) *
+                               eo.w) *
+                                  in->specular_weight;
+        V3 color = (direct * in->options[0] + indirect * in->options[1]) + xyz(eo) * in->options[3];
+        store_half4(out + i * 4, V4{color.x, color.y, color.z, 1.0f});
+    });
+}
 
 // fs_svgf_variance.sc:24-73: per-pixel luminance variance, from the pixel's
 // own moments after 4 frames of history, else from its 7 x 7 same-surface
@@ -2102,7 +2241,8 @@ void orc_gbuffer_primary(const rgi_frame* f, const rgi_scene* s, const float* ma
         normal[i] = nw | (byte(metallic) << 24);
         shading_normal[i] = nw;
         V3 e = bvh_triangle_emission(s, tri);
-        store_half4(emission + (size_t)i * 4, V4{e.x, e.y, e.z, 0.0f});
+        // a = ambient occlusion: 1 without an occlusion map (fs_gbuffer.sc:81-93).
+        store_half4(emission + (size_t)i * 4, V4{e.x, e.y, e.z, 1.0f});
     });
 }
 

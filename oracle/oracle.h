@@ -41,6 +41,12 @@ void orc_cosine_hemisphere(const float* normal3, float r0, float r1, float* out3
 void orc_sincos(float x, float* s, float* c);
 float orc_expf(float x);
 float orc_pow64(float x);
+float orc_exp2f(float x);
+void orc_specular_albedo(const float* f0, float roughness, float nv, float* out);
+
+/* The composite pass of rgi_composite (fs_composite.sc), on the CPU. */
+void orc_composite(const rgi_frame* frame, const rgi_gbuffer* gbuffer, const rgi_composite_inputs* inputs,
+                   uint16_t* out_hdr);
 
 /* The SVGF denoiser passes (fs_svgf_variance.sc, fs_svgf_atrous.sc) over
  * frame->width x frame->height half4 planes. */

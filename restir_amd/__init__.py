@@ -113,12 +113,18 @@ class Shadow(ctypes.Structure):
                 ("near_plane", F), ("face_size", I32), ("homogeneous_depth", I32), ("reserved", I32)]
 
 
+class CompositeInputs(ctypes.Structure):
+    """rgi_composite_inputs: the composite pass's planes and u_compositeOptions."""
+    _fields_ = [("gi", P), ("di_diffuse", P), ("di_specular", P), ("specular", P), ("shadow_map", P),
+                ("options", F * 4), ("specular_weight", F), ("reserved", F * 3)]
+
+
 class Mesh(ctypes.Structure):
     _fields_ = [("triangle_count", U32), ("positions", P), ("albedo", P), ("emission", P), ("material", P),
                 ("material_count", U32), ("materials", P)]
 
 
-ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs, Shadow]
+ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs, Shadow, CompositeInputs]
 
 
 class RestirError(RuntimeError):
@@ -154,6 +160,7 @@ def lib():
                                  ctypes.POINTER(DiReservoirs), P, P, P]
     L.rgi_trace.argtypes = [P, ctypes.POINTER(Scene), P, U32, P, P, ctypes.c_int, P]
     L.rgi_gbuffer_primary.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), P, U32, P, P, P, P, P, P]
+    L.rgi_composite.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), ctypes.POINTER(CompositeInputs), P, P]
     L.rgi_svgf_variance.argtypes = [P, ctypes.POINTER(Frame), P, P, P, P, P, P]
     L.rgi_svgf_atrous.argtypes = [P, ctypes.POINTER(Frame), P, P, P, I32, P, P]
     L.rgi_shadow_setup.argtypes = [P, F, I32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Shadow)]

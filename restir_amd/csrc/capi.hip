@@ -106,6 +106,7 @@ uint32_t rgi_abi_sizeof(int which)
     case 4: return (uint32_t)sizeof(rgi_gi_inputs);
     case 5: return (uint32_t)sizeof(rgi_di_reservoirs);
     case 6: return (uint32_t)sizeof(rgi_shadow);
+    case 7: return (uint32_t)sizeof(rgi_composite_inputs);
     default: return 0;
     }
 }
@@ -292,6 +293,21 @@ int rgi_shadow_atlas(rgi_context* ctx, const rgi_shadow* shadow, const rgi_scene
     if (F != 256 && F != 512 && F != 1024 && F != 2048)
         return fail(RGI_E_INVALID, "shadow face size must be 256, 512, 1024 or 2048, not %d", F);
     return shadow_atlas(*shadow, *scene, atlas, as_stream(stream));
+}
+
+int rgi_composite(rgi_context* ctx, const rgi_frame* frame, const rgi_gbuffer* g, const rgi_composite_inputs* in,
+                  uint16_t* out, void* stream)
+{
+    (void)ctx;
+    int e;
+    if (!frame || !in || !out) return fail(RGI_E_INVALID, "composite: frame, inputs and output are required");
+    if (frame->width <= 0 || frame->height <= 0) return fail(RGI_E_INVALID, "frame width/height must be positive");
+    if ((e = check_gbuffer(g, true, true)) != RGI_OK) return e;
+    if (!g->emission) return fail(RGI_E_INVALID, "composite: the emission plane is required");
+    if (!in->gi || !aligned(in->gi, 8)) return fail(RGI_E_INVALID, "composite: the GI plane is required (8-byte aligned)");
+    if (frame->shadows_enabled && in->shadow_map && (frame->shadow_width <= 0 || frame->shadow_height <= 0))
+        return fail(RGI_E_INVALID, "composite: shadows on without an atlas size");
+    return composite(*frame, *g, *in, out, as_stream(stream));
 }
 
 int rgi_svgf_variance(rgi_context* ctx, const rgi_frame* frame, const uint16_t* color, const uint16_t* moments,

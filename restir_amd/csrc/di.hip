@@ -4,6 +4,7 @@
 // pairwise-MIS spatial reuse and shading. Shadow rays are any-hit walks of
 // the BVH. One pixel per lane, 16 x 16 pixels per 256-thread block.
 #include "bvh_device.h"
+#include "shading.h"
 #include "internal.h"
 
 namespace rgi {
@@ -83,38 +84,6 @@ RGI_DEV F3 light_point(const DiPass& p, int light, float r0, float r1, float& de
     float v = r1 * root;
     density = 1.0f / maxf_(light_area(p, light), 1e-12f);
     return (v0 + (xyz(light_row(p, light, 1)) - v0) * u) + (xyz(light_row(p, light, 2)) - v0) * v;
-}
-
-RGI_DEV float ggx_d(F3 n, F3 h, float roughness)
-{
-    float alpha = roughness * roughness;
-    float a2 = alpha * alpha;
-    float nh = maxf_(dot(n, h), 0.0f);
-    float den = (nh * nh) * (a2 - 1.0f) + 1.0f;
-    return a2 / maxf_((kPi * den) * den, 0.0001f);
-}
-
-RGI_DEV float smith_g1(float nd, float roughness)
-{
-    float k = ((roughness + 1.0f) * (roughness + 1.0f)) / 8.0f;
-    return nd / maxf_(nd * (1.0f - k) + k, 0.0001f);
-}
-
-RGI_DEV void brdf(F3 n, F3 v, F3 l, F3 albedo, float roughness_in, float metallic, F3& diffuse, F3& specular)
-{
-    float nl = maxf_(dot(n, l), 0.0f);
-    float nv = maxf_(dot(n, v), 0.0f);
-    F3 h = normalize(v + l);
-    F3 f0 = mix3(splat3(0.04f), albedo, metallic);
-    float c = clampf_(dot(h, v), 0.0f, 1.0f);
-    float x = 1.0f - c;
-    float w = ((x * x) * (x * x)) * x;
-    F3 fr = f0 + (splat3(1.0f) - f0) * w;
-    float roughness = maxf_(roughness_in, 0.05f);
-    float d = ggx_d(n, h, roughness);
-    float g = smith_g1(nl, roughness) * smith_g1(nv, roughness);
-    specular = ((fr * (d * g)) / maxf_((4.0f * nl) * nv, 0.001f)) * nl;
-    diffuse = (((splat3(1.0f) - fr) * (1.0f - metallic)) * albedo / kPi) * nl;
 }
 
 RGI_DEV void light_sample(const DiPass& p, const Surf& s, int light, F3 point, F3& diffuse, F3& specular)

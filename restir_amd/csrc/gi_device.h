@@ -5,6 +5,7 @@
 #pragma once
 
 #include "bvh_device.h"
+#include "shading.h"
 
 namespace rgi {
 
@@ -39,50 +40,14 @@ RGI_DEV F3 diffuse_albedo(const GiPass& p, F2 uv)
 
 // ------------------------------------------------------------ shadow, sky
 
-RGI_DEV int shadow_face(F3 d)
-{
-    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
-    if (ax >= ay && ax >= az) return d.x >= 0.0f ? 0 : 1;
-    if (ay >= az) return d.y >= 0.0f ? 2 : 3;
-    return d.z >= 0.0f ? 4 : 5;
-}
-
+// lightVisibilityHard (shadow.sh:99-108): one hard compare.
 RGI_DEV float light_visibility_hard(const GiPass& p, F3 position, F3 n)
 {
-    const rgi_frame& f = *p.f;
-    if (!f.shadows_enabled || p.in.shadow_map == nullptr) return 1.0f;
-    F3 light = F3{f.light_position[0], f.light_position[1], f.light_position[2]};
-    float light_distance = length(light - position);
-    float texel_size = ((2.0f * light_distance) * f.shadow_info[1]) * f.shadow_info[2];
-    F3 pos = position + n * (texel_size * 1.5f);
-    F3 from_light = pos - light;
-    int face = shadow_face(from_light);
-    F4 clip = mul(f.shadow_view_proj + face * 16, F4{pos.x, pos.y, pos.z, 1.0f});
-    float ndx = clip.x / clip.w, ndy = clip.y / clip.w;
-    bool obl = f.shadow_info[3] > 0.5f;
-    float lx = ndx * 0.5f + 0.5f;
-    float ly = obl ? ndy * 0.5f + 0.5f : 0.5f - ndy * 0.5f;
-    float row = face >= 3 ? 1.0f : 0.0f;
-    float column = (float)face - 3.0f * row;
-    if (obl) row = 1.0f - row;
-    F2 uv = F2{(column + lx) * (1.0f / 3.0f), (row + ly) * 0.5f};
-    float cosine = clampf_(dot(n, -normalize(from_light)), 0.0f, 1.0f);
-    float slope = minf_(sqrtf(1.0f - cosine * cosine) / maxf_(cosine, 0.0001f), 8.0f);
-    float bias = texel_size * (1.0f + 0.5f * slope);
-    float reference = (length(from_light) - bias) / f.shadow_info[0];
-    float stored = p.in.shadow_map[texel_index(uv, f.shadow_width, f.shadow_height)];
+    F2 uv;
+    float reference;
+    if (!shadow_lookup(*p.f, p.in.shadow_map, position, n, 0.5f, uv, reference)) return 1.0f;
+    float stored = p.in.shadow_map[texel_index(uv, p.f->shadow_width, p.f->shadow_height)];
     return reference <= stored ? 1.0f : 0.0f;
-}
-
-RGI_DEV bool sky_on(const rgi_frame& f) { return f.sky_zenith[3] > 0.5f; }
-
-RGI_DEV F3 sky_radiance(const rgi_frame& f, F3 d)
-{
-    if (!sky_on(f)) return splat3(0.0f);
-    F3 horizon = F3{f.sky_horizon[0], f.sky_horizon[1], f.sky_horizon[2]};
-    float h = clampf_(d.y, -1.0f, 1.0f);
-    if (h >= 0.0f) return mix3(horizon, F3{f.sky_zenith[0], f.sky_zenith[1], f.sky_zenith[2]}, sqrtf(h));
-    return mix3(horizon, F3{f.sky_ground[0], f.sky_ground[1], f.sky_ground[2]}, sqrtf(-h));
 }
 
 // ------------------------------------------------------------ shading

@@ -17,6 +17,9 @@
 //                polynomial of e^r on |r| <= ln(2)/2, and scaling by 2^k
 //                through the exponent bits, with one rounding into the
 //                subnormal range. Within 2 ulp of the true value.
+//   rgi_exp2f    exp2 for specularAlbedo (brdf.sh:73): k = round(x), then
+//                e^((x - k) ln 2) by the polynomial of rgi_expf (x - k is
+//                exact) and the same scaling. Within 2 ulp.
 //   rgi_pow64    pow(x, 64) (kNormalPower, svgf.sh:14) as six squarings.
 //
 // Every function is plain float arithmetic in a fixed order, so it rounds
@@ -157,14 +160,11 @@ RGI_HD float rgi_half_float(uint16_t bits)
 #define RGI_LN2_HI 0x1.62ep-1f
 #define RGI_LN2_LO 0x1.0bfbe8p-15f
 #define RGI_LOG2E 0x1.715476p+0f
+#define RGI_LN2 0x1.62e430p-1f
 
-RGI_HD float rgi_expf(float x)
+// e^r times 2^k, for |r| <= ln(2)/2 + a little and k in [-150, 128].
+RGI_HD float rgi_exp_scaled(float r, float kf)
 {
-    if (x != x) return x;
-    if (x > 0x1.62e42ep+6f) return __builtin_inff();
-    if (x < -104.0f) return 0.0f;
-    float kf = __builtin_floorf(x * RGI_LOG2E + 0.5f);
-    float r = (x - kf * RGI_LN2_HI) - kf * RGI_LN2_LO;
     // 1/n! rounded to float, n = 2 .. 7.
     float q = 0x1.a01a02p-13f;             // 1/7!
     q = q * r + 0x1.6c16c2p-10f;           // 1/6!
@@ -186,6 +186,25 @@ RGI_HD float rgi_expf(float x)
     float scale;
     memcpy(&scale, &bits, 4);
     return p * scale;
+}
+
+RGI_HD float rgi_expf(float x)
+{
+    if (x != x) return x;
+    if (x > 0x1.62e42ep+6f) return __builtin_inff();
+    if (x < -104.0f) return 0.0f;
+    float kf = __builtin_floorf(x * RGI_LOG2E + 0.5f);
+    float r = (x - kf * RGI_LN2_HI) - kf * RGI_LN2_LO;
+    return rgi_exp_scaled(r, kf);
+}
+
+RGI_HD float rgi_exp2f(float x)
+{
+    if (x != x) return x;
+    if (x >= 128.0f) return __builtin_inff();
+    if (x < -150.0f) return 0.0f;
+    float kf = __builtin_floorf(x + 0.5f);
+    return rgi_exp_scaled((x - kf) * RGI_LN2, kf);
 }
 
 RGI_HD float rgi_pow64(float x)

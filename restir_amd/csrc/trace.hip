@@ -120,7 +120,9 @@ __global__ void __launch_bounds__(256) k_gbuffer(rgi_frame f, rgi_scene s, const
     normal[i] = nw | (unorm_byte(metallic) << 24);
     shading_normal[i] = nw;
     F3 e = tri_emission(v, tri);
-    store_h4(emission, i, F4{e.x, e.y, e.z, 0.0f});
+    // Emissive target: rgb = emitted radiance, a = ambient occlusion, 1
+    // without an occlusion map (fs_gbuffer.sc:81-93).
+    store_h4(emission, i, F4{e.x, e.y, e.z, 1.0f});
 }
 
 // The shadow pass: texel (X, Y) of the 3F x 2F atlas belongs to face

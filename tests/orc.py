@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from restir_amd import DiReservoirs, Frame, GBuffer, GiInputs, GiReservoirs, Scene, Shadow
+from restir_amd import CompositeInputs, DiReservoirs, Frame, GBuffer, GiInputs, GiReservoirs, Scene, Shadow
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -78,6 +78,12 @@ def lib():
     L.orc_reference_accumulate.argtypes = [PG, P, P, P, P]
     L.orc_reference_error.restype = None
     L.orc_reference_error.argtypes = [PG, P, P, P, P, U32, P]
+    L.orc_exp2f.restype = F
+    L.orc_exp2f.argtypes = [F]
+    L.orc_specular_albedo.restype = None
+    L.orc_specular_albedo.argtypes = [P, F, F, P]
+    L.orc_composite.restype = None
+    L.orc_composite.argtypes = [PF, PG, ctypes.POINTER(CompositeInputs), P]
     L.orc_expf.restype = F
     L.orc_expf.argtypes = [F]
     L.orc_pow64.restype = F

@@ -219,6 +219,34 @@ def test_svgf(dev, cornell_seq, step):
         assert_same("atrous", dev.get(atr_t, want_a), want_a)
 
 
+def test_composite(dev):
+    """rgi_composite against the oracle, bit-exact: main light with the 4 x 4
+    soft shadow compare of a real atlas, DI planes, denoised GI and the sky."""
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings(shadows=True, light_color=(1.0, 0.9, 0.8), light_position=(0.1, 1.2, -0.2), shadow_face_size=0,
+                    sky=True)
+    seq = common.oracle_sequence(hs, s, W, H, 2, yaw_step=2 * np.pi / 64)
+    f, g, hist, out = seq[-1]
+    say("case composite")
+    gi, dd, ds, atlas = out["denoised"], out["diffuse"], out["specular"], out["shadow_atlas"]
+    ci = ra.CompositeInputs(gi.ctypes.data, dd.ctypes.data, ds.ctypes.data, None, atlas.ctypes.data)
+    ci.options[:] = [1.0, 1.0, 1.0, 1.0]
+    want = np.zeros((W * H, 4), np.uint16)
+    orc.lib().orc_composite(ctypes.byref(f), ctypes.byref(common.host_gbuffer(g, W, H)), ctypes.byref(ci),
+                            orc.ptr(want))
+    gb, keep = dev.gbuffer(g, W, H)
+    d = {k: dev.put(v) for k, v in (("gi", gi), ("dd", dd), ("ds", ds), ("atlas", atlas))}
+    dci = ra.CompositeInputs(d["gi"].data_ptr(), d["dd"].data_ptr(), d["ds"].data_ptr(), None, d["atlas"].data_ptr())
+    dci.options[:] = [1.0, 1.0, 1.0, 1.0]
+    o = dev.empty(W * H * 8)
+    ra.check(dev.L.rgi_composite(dev.ctx, ctypes.byref(f), ctypes.byref(gb), ctypes.byref(dci), o.data_ptr(),
+                                 dev.stream))
+    got = dev.get(o, want)
+    lum = want.view(np.float16)[:, :3].astype(np.float64).mean(0)
+    say("  mean rgb %s" % lum)
+    assert_same("hdr", got, want)
+
+
 @pytest.mark.parametrize("table", [65536, 1024, 64])
 def test_grid_insert_deterministic(dev, cornell_seq, table):
     hs, s, seq = cornell_seq

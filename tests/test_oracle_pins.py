@@ -482,6 +482,99 @@ def test_svgf_atrous_kernel_edges_and_constants():
     assert np.all(np.abs(got[:, w // 2:, 0] - float(np.float16(0.8))) <= 1e-3)
 
 
+def test_exp2f_against_libm():
+    """rgi_exp2f (specularAlbedo's exp2, brdf.sh:73) within 2 ulp of 2^x."""
+    rng = np.random.default_rng(9)
+    xs = np.concatenate([rng.uniform(-150.0, 128.0, 40000), rng.uniform(-10.0, 0.0, 20000),
+                         np.array([0.0, -1.0, 1.0, -9.28, 127.9, -126.5, -149.2, -150.5, 128.0])]).astype(np.float32)
+    worst = 0
+    for x in xs:
+        got = np.float32(L.orc_exp2f(float(x)))
+        want = np.float32(2.0 ** float(x)) if float(x) < 128.0 else np.float32(np.inf)
+        if not np.isfinite(want) or want == 0.0:
+            assert got == want or (want == 0.0 and got <= np.float32(2.0 ** -149)), (x, got, want)
+            continue
+        worst = max(worst, abs(int(got.view(np.int32)) - int(want.view(np.int32))))
+    assert worst <= 2, worst
+
+
+def test_specular_albedo_is_karis_fit():
+    """specularAlbedo (brdf.sh:68-76) is Karis' analytic approximation of the
+    split-sum environment BRDF ("Physically Based Shading on Mobile", 2014):
+    r = c0 roughness + c1, a004 = min(r.x^2, 2^(-9.28 n.v)) r.x + r.y,
+    (A, B) = (-1.04, 1.04) a004 + r.zw, albedo = f0 A + B."""
+    c0 = np.array([-1.0, -0.0275, -0.572, 0.022])
+    c1 = np.array([1.0, 0.0425, 1.04, -0.04])
+    rng = np.random.default_rng(10)
+    for _ in range(500):
+        f0 = rng.random(3).astype(np.float32)
+        rough, nv = float(rng.random()), float(rng.random())
+        r = c0 * rough + c1
+        a004 = min(r[0] * r[0], 2.0 ** (-9.28 * nv)) * r[0] + r[1]
+        a, b = -1.04 * a004 + r[2], 1.04 * a004 + r[3]
+        want = f0.astype(np.float64) * a + b
+        out = np.zeros(3, np.float32)
+        L.orc_specular_albedo(orc.ptr(f0), rough, nv, orc.ptr(out))
+        assert np.allclose(out, want, rtol=1e-5, atol=1e-6), (f0, rough, nv, out, want)
+
+
+def composite_inputs(gi, dd=None, ds=None, spec=None, shadow=None, options=(1.0, 1.0, 1.0, 1.0), spec_w=0.0):
+    ci = ra.CompositeInputs()
+    ci.gi = gi.ctypes.data
+    ci.di_diffuse = dd.ctypes.data if dd is not None else None
+    ci.di_specular = ds.ctypes.data if ds is not None else None
+    ci.specular = spec.ctypes.data if spec is not None else None
+    ci.shadow_map = shadow.ctypes.data if shadow is not None else None
+    ci.options[:] = list(options)
+    ci.specular_weight = spec_w
+    return ci
+
+
+def test_composite_closed_forms():
+    """fs_composite.sc with the main light off and no DI: a constant GI
+    irradiance c comes back as c x albedo x (1 - metallic) x occlusion plus
+    the emission on every surface pixel (the same-surface upsample weights
+    renormalise to 1); with options.z = 0 the albedo is not multiplied in;
+    sky pixels are the sky gradient (sky.sh:20-31) when the sky is on."""
+    hs = common.cornell(extra_lights=0)
+    w, h = 64, 48
+    s = ra.Settings(sky=True)
+    eye, vp, inv = ra.orbit_camera(hs, 0, w, h)
+    f = ra.make_frame(s, hs, 0, eye, vp, inv, eye, vp, w, h, False)
+    g = common.oracle_gbuffer(hs, f)
+    gb = common.host_gbuffer(g, w, h)
+    c = np.array([0.25, 0.5, 1.0], np.float32)
+    gi = np.zeros((w * h, 4), np.float16)
+    gi[:, :3] = c
+    gi = gi.view(np.uint16)
+    for modulate in (1.0, 0.0):
+        out = np.zeros((w * h, 4), np.uint16)
+        ci = composite_inputs(gi, options=(1.0, 1.0, modulate, 1.0))
+        L.orc_composite(ctypes.byref(f), ctypes.byref(gb), ctypes.byref(ci), orc.ptr(out))
+        hdr = out.view(np.float16).astype(np.float64)
+        geo = g["depth"] < 0.99999
+        albedo = np.stack([(g["albedo"] >> (8 * k)) & 255 for k in range(3)], 1) / 255.0
+        metallic = (g["normal"] >> 24) / 255.0
+        emission = g["emission"].view(np.float16).astype(np.float64)
+        scale = albedo * (1.0 - metallic)[:, None] * emission[:, 3:4] if modulate else 1.0
+        want = (c.astype(np.float64) * scale + emission[:, :3])[geo] if modulate else (c + emission[:, :3])[geo]
+        assert np.allclose(hdr[geo, :3], want, rtol=2e-3, atol=1e-4)
+        assert np.all(hdr[geo, 3] == 1.0)
+        # Sky: the gradient between horizon, zenith and ground along the view ray.
+        sky = ~geo
+        ys, xs = np.divmod(np.arange(w * h)[sky], w)
+        ndc = np.stack([(xs + 0.5) / w * 2 - 1, -((ys + 0.5) / h * 2 - 1), np.ones(len(xs)), np.ones(len(xs))], 1)
+        wp = ndc @ np.array(f.inv_view_proj, np.float64).reshape(4, 4)
+        d = wp[:, :3] / wp[:, 3:4] - np.array(eye, np.float64)
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        hgt = np.clip(d[:, 1], -1, 1)
+        zen, hor, gnd = (np.array(f.sky_zenith[:3]), np.array(f.sky_horizon[:3]), np.array(f.sky_ground[:3]))
+        t = np.sqrt(np.abs(hgt))[:, None]
+        want_sky = np.where(hgt[:, None] >= 0, hor * (1 - t) + zen * t, hor * (1 - t) + gnd * t)
+        assert np.allclose(hdr[sky, :3], want_sky, rtol=2e-3, atol=1e-3)
+        assert np.all(hdr[sky, 3] == 1.0)
+
+
 # ---------------------------------------------------------- shadow atlas
 
 def shadow_face(d):
