@@ -116,35 +116,49 @@ def cpu_baseline(hs, settings, width, height, table):
             "sample": "oracle, one steady-state %dx%d frame (frame 3 of the orbit), all passes" % (width, height)}
 
 
-def measure_mse(L, rend, gscene, gbuf, view, W, H, ref_frames, frames, ctx, s):
-    """MSE of the plain path against a `ref_frames`-frame reference at one
-    static camera (frame 0 of the orbit): the reference renders with every
-    biased shortcut off (renderer.cpp:437-452) and accumulates in FP64
-    (fs_reference.sc); the plain path then renders `frames` frames from an
-    empty history and its last image is measured (cs_error.sc)."""
+def measure_mse(L, hs, gscene, gbuf, view, W, H, T, mode, ref_frames, frames, ctx, s):
+    """MSE against a `ref_frames`-frame reference at one static camera
+    (frame 0 of the path), on the composite HDR image that the reference
+    accumulates and measures (fs_reference.sc, cs_error.sc; no TAA here):
+    the reference renders with every biased shortcut and the denoiser off
+    (renderer.cpp:437-452) and accumulates in FP64; the path under test then
+    renders `frames` frames from an empty history with the reference's
+    default settings (denoiser on), and its last composite is measured.
+    The same without the denoiser is reported beside it."""
     import torch
     n = W * H
     acc = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
     out = ra.RendererOutputs()
-    ra.check(L.rgi_renderer_set_reference_mode(rend, 1))
-    for _ in range(ref_frames):
-        ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
-        ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
-        ra.check(L.rgi_reference_accumulate(ctypes.byref(gbuf.gb), out.color, out.di_diffuse, out.di_specular,
-                                            acc.data_ptr(), s))
-    ra.check(L.rgi_renderer_set_reference_mode(rend, 0))
-    L.rgi_renderer_invalidate_history(rend)
-    for _ in range(frames):
-        ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
-    ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
     res = torch.zeros(5, dtype=torch.float64, device="cuda")
-    ra.check(L.rgi_reference_error(ctx, ctypes.byref(gbuf.gb), out.color, out.di_diffuse, out.di_specular,
-                                   acc.data_ptr(), ref_frames, res.data_ptr(), s))
-    r = res.cpu().numpy()
-    return {"relative_mse": float(r[0]), "mse_rgb": float(r[1]), "mean_luminance": float(r[2]),
-            "reference_mean_luminance": float(r[3]), "pixels": int(r[4]), "reference_frames": ref_frames,
-            "frames": frames, "method": "plain ReSTIR (grid reuse, clamps on)",
-            "image": "GI x diffuse albedo + DI diffuse + DI specular, static camera"}
+    results = {}
+    for name, denoise in (("display", True), ("no_denoiser", False)):
+        settings = ra.Settings(grid_table_size=T, grid_mode=mode, denoiser=denoise)
+        rend = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                       ctypes.byref(gscene.scene), hs.radius))
+        if name == "display":
+            ra.check(L.rgi_renderer_set_reference_mode(rend, 1))
+            for _ in range(ref_frames):
+                ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+                ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+                ra.check(L.rgi_reference_accumulate_composite(ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), s))
+            ra.check(L.rgi_renderer_set_reference_mode(rend, 0))
+            L.rgi_renderer_invalidate_history(rend)
+        for _ in range(frames):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+        ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), ref_frames,
+                                                 res.data_ptr(), s))
+        r = res.cpu().numpy()
+        results[name] = {"relative_mse": float(r[0]), "mse_rgb": float(r[1]), "mean_luminance": float(r[2]),
+                         "reference_mean_luminance": float(r[3]), "pixels": int(r[4])}
+        L.rgi_renderer_destroy(rend)
+    line = dict(results["display"])
+    line.update({"reference_frames": ref_frames, "frames": frames,
+                 "method": "ReSTIR GI + DI, SVGF denoiser (reference defaults), composite; no TAA",
+                 "image": "composite HDR (fs_composite.sc), static camera, geometry pixels",
+                 "no_denoiser": results["no_denoiser"]})
+    return line
 
 
 def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames, distance=3.17):
@@ -272,7 +286,7 @@ def main():
     mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
     # The metric's frame: grid + resample + trace. The denoiser that follows
     # it in the reference is timed on its own below.
-    settings = ra.Settings(grid_table_size=T, grid_mode=mode, denoiser=False)
+    settings = ra.Settings(grid_table_size=T, grid_mode=mode, denoiser=False, composite=False)
     run = run_workload(hs, settings, W, H, args.frames, args.steps, args.warmup, args.tune_frames)
     ms_per_frame, pass_ms, tuning, options = run["ms_per_frame"], run["pass_ms"], run["tuning"], run["options"]
     gscene, drv, rend, gbufs, views = run["gscene"], run["drv"], run["rend"], run["gbufs"], run["views"]
@@ -302,6 +316,24 @@ def main():
     e1.record()
     torch.cuda.synchronize()
     denoiser_ms = e0.elapsed_time(e1) / 10
+
+    # The composite of that frame (main light off, as the workload has it).
+    hdr_t = torch.empty(W * H * 8, dtype=torch.uint8, device="cuda")
+    ci = ra.CompositeInputs(den[1].data_ptr(), outs.di_diffuse, outs.di_specular, None, None)
+    ci.options[:] = [1.0, 1.0, 1.0, 1.0]
+
+    def comp():
+        ra.check(L.rgi_composite(drv.ctx, ctypes.byref(last), ctypes.byref(gbufs[0].gb), ctypes.byref(ci),
+                                 hdr_t.data_ptr(), s))
+
+    comp()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        comp()
+    e1.record()
+    torch.cuda.synchronize()
+    composite_ms = e0.elapsed_time(e1) / 10
 
     # Traversal microbenchmark.
     rays = surface_rays(hs, args.rays, seed=7)
@@ -347,7 +379,8 @@ def main():
 
     mse = None
     if args.reference_frames > 0:
-        mse = measure_mse(L, rend, gscene, gbufs[0], views[0], W, H, args.reference_frames, args.mse_frames, drv.ctx, s)
+        mse = measure_mse(L, hs, gscene, gbufs[0], views[0], W, H, T, mode, args.reference_frames, args.mse_frames,
+                          drv.ctx, s)
 
     cpu = None
     if not args.no_cpu:
@@ -362,7 +395,7 @@ def main():
         # point lights and the emissive triangles; the same swing path, with
         # the camera inside the courtyard (0.7 scene radii from the centre).
         ahs = ra.HostScene(ra.RGI_SCENE_ATRIUM, seed=1, size=262144, extra_lights=64, light_seed=1)
-        aset = ra.Settings(grid_table_size=1 << 22, grid_mode=mode, denoiser=False)
+        aset = ra.Settings(grid_table_size=1 << 22, grid_mode=mode, denoiser=False, composite=False)
         arun = run_workload(ahs, aset, W, H, args.frames, args.atrium_steps, 8, 4, distance=0.7)
         atrium = {"ms_per_frame": round(arun["ms_per_frame"], 5),
                   "pass_ms": {k: round(v, 5) for k, v in sorted(arun["pass_ms"].items())},
@@ -391,6 +424,7 @@ def main():
         "frames_per_s": round(1e3 / ms_per_frame, 2),
         "pass_ms": {k: round(v, 5) for k, v in sorted(pass_ms.items())},
         "denoiser_ms": round(denoiser_ms, 5),
+        "composite_ms": round(composite_ms, 5),
         "mrays_per_s": round(mrays, 2),
         "trace": {"rays": args.rays, "ms": round(trace_ms, 4), "hit_rate": round(hit_rate, 4), "mode": "closest"},
         "mse_vs_reference": mse,

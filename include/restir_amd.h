@@ -350,6 +350,14 @@ int rgi_reference_error(rgi_context* context, const rgi_gbuffer* gbuffer, const 
                         const uint16_t* di_diffuse, const uint16_t* di_specular, const double* accum,
                         uint32_t frames, double* out, void* stream);
 
+/* The same two on the composite's HDR image (rgi_composite), the image the
+ * reference accumulates and measures (fs_reference.sc samples the composite,
+ * cs_error.sc the display before tone mapping; this build has no TAA). The
+ * G-buffer supplies only the depth that selects geometry pixels. */
+int rgi_reference_accumulate_composite(const rgi_gbuffer* gbuffer, const uint16_t* hdr, double* accum, void* stream);
+int rgi_reference_error_composite(rgi_context* context, const rgi_gbuffer* gbuffer, const uint16_t* hdr,
+                                  const double* accum, uint32_t frames, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -54,7 +54,7 @@ typedef struct rgi_settings {
     int32_t shadow_face_size;     /* 1: index into 256, 512, 1024, 2048 (kShadowFaceSizes) */
     int32_t denoiser;             /* 1: variance + a-trous passes; GI history colour = filtered history */
     int32_t atrous_iterations;    /* 5 (kMaxAtrousIterations), 0..5 */
-    int32_t reserved[1];
+    int32_t composite;            /* 1: the composite pass into an RGBA16F HDR image every frame */
 } rgi_settings;
 
 /* One frame's camera: eye and (jittered) view-projection and its inverse. */
@@ -76,6 +76,7 @@ typedef struct rgi_renderer_outputs {
     const uint16_t* di_specular;
     const uint16_t* filtered_history; /* RGBA16F first a-trous iteration (next frame's GI history colour) */
     const uint16_t* denoised;     /* RGBA16F the composite's GI input: the last a-trous output, or `color` */
+    const uint16_t* hdr;          /* RGBA16F composite (NULL when the composite is off) */
 } rgi_renderer_outputs;
 
 typedef struct rgi_renderer rgi_renderer;

@@ -2080,11 +2080,17 @@ void orc_di_spatial(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g
 
 // The displayed image of a pixel (GI re-modulated by the diffuse albedo, plus
 // DI), false over the sky (fs_composite.sc's light, cs_error.sc's mask).
+// The compared image at pixel i: the composite itself (`composited`), or the
+// GI re-modulated by the albedo plus the DI planes; geometry pixels only.
 static bool reference_image(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
-                            size_t i, V3& L)
+                            bool composited, size_t i, V3& L)
 {
     if (g->depth[i] >= kSkyDepth) return false;
     V4 c = half4(color + i * 4);
+    if (composited) {
+        L = xyz(c);
+        return true;
+    }
     V4 a = unorm8(g->albedo[i]);
     float metallic = unorm8(g->normal[i]).w;
     L = V3{c.x, c.y, c.z} * (V3{a.x, a.y, a.z} * (1.0f - metallic));
@@ -2094,13 +2100,13 @@ static bool reference_image(const rgi_gbuffer* g, const uint16_t* color, const u
 }
 
 // fs_reference.sc:18-25: the running mean, here as an FP64 running sum.
-void orc_reference_accumulate(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
-                              double* accum)
+static void reference_accumulate(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                                 bool composited, double* accum)
 {
     size_t n = (size_t)g->width * g->height;
     for (size_t i = 0; i < n; ++i) {
         V3 L;
-        if (!reference_image(g, color, dd, ds, i, L)) continue;
+        if (!reference_image(g, color, dd, ds, composited, i, L)) continue;
         if (L.x != L.x || L.y != L.y || L.z != L.z) continue;
         accum[3 * i + 0] += (double)L.x;
         accum[3 * i + 1] += (double)L.y;
@@ -2112,8 +2118,8 @@ static double lum64(double r, double g, double b) { return (0.2126 * r + 0.7152 
 
 // cs_error.sc:35-92 in FP64: 256-pixel blocks reduced by a halving tree, block
 // partials summed with stride 256 per lane, then the same tree.
-void orc_reference_error(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
-                         const double* accum, uint32_t frames, double* out)
+static void reference_error(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                            bool composited, const double* accum, uint32_t frames, double* out)
 {
     const int B = 256, K = 5;
     size_t n = (size_t)g->width * g->height;
@@ -2130,7 +2136,7 @@ void orc_reference_error(const rgi_gbuffer* g, const uint16_t* color, const uint
         for (int t = 0; t < B; ++t) {
             size_t i = b * B + t;
             V3 L;
-            if (i >= n || !reference_image(g, color, dd, ds, i, L)) continue;
+            if (i >= n || !reference_image(g, color, dd, ds, composited, i, L)) continue;
             double rr = accum[3 * i] * inv, rg = accum[3 * i + 1] * inv, rb = accum[3 * i + 2] * inv;
             double lr = L.x, lg = L.y, lb = L.z;
             double li = lum64(lr, lg, lb), lref = lum64(rr, rg, rb);
@@ -2156,6 +2162,29 @@ void orc_reference_error(const rgi_gbuffer* g, const uint16_t* color, const uint
     out[2] = s[2 * B] / nn;
     out[3] = s[3 * B] / nn;
     out[4] = count;
+}
+
+void orc_reference_accumulate(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                              double* accum)
+{
+    reference_accumulate(g, color, dd, ds, false, accum);
+}
+
+void orc_reference_error(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                         const double* accum, uint32_t frames, double* out)
+{
+    reference_error(g, color, dd, ds, false, accum, frames, out);
+}
+
+void orc_reference_accumulate_composite(const rgi_gbuffer* g, const uint16_t* hdr, double* accum)
+{
+    reference_accumulate(g, hdr, nullptr, nullptr, true, accum);
+}
+
+void orc_reference_error_composite(const rgi_gbuffer* g, const uint16_t* hdr, const double* accum, uint32_t frames,
+                                   double* out)
+{
+    reference_error(g, hdr, nullptr, nullptr, true, accum, frames, out);
 }
 
 // The G-buffer producer: one closest-hit primary ray per pixel centre.

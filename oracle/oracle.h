@@ -96,6 +96,11 @@ void orc_reference_error(const rgi_gbuffer* gbuffer, const uint16_t* color, cons
  * on the CPU: one closest-hit ray per atlas texel. */
 void orc_shadow_atlas(const rgi_shadow* shadow, const rgi_scene* scene, float* atlas);
 
+/* The same two over the composite's HDR image. */
+void orc_reference_accumulate_composite(const rgi_gbuffer* gbuffer, const uint16_t* hdr, double* accum);
+void orc_reference_error_composite(const rgi_gbuffer* gbuffer, const uint16_t* hdr, const double* accum,
+                                   uint32_t frames, double* out5);
+
 /* The G-buffer producer of rgi_gbuffer_primary, on the CPU. */
 void orc_gbuffer_primary(const rgi_frame* frame, const rgi_scene* scene, const float* materials,
                          uint32_t material_count, float* depth, uint32_t* albedo, uint32_t* normal,

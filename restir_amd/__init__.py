@@ -93,7 +93,7 @@ class CSettings(ctypes.Structure):
         ("grid_table_size", U32), ("sky_zenith", F * 3), ("sky_horizon", F * 3), ("sky_ground", F * 3),
         ("sky_intensity", F), ("light_position", F * 3), ("light_color", F * 3), ("origin_bottom_left", I32),
         ("homogeneous_depth", I32), ("shadow_face_size", I32), ("denoiser", I32), ("atrous_iterations", I32),
-        ("reserved", I32 * 1),
+        ("composite", I32),
     ]
 
 
@@ -104,7 +104,7 @@ class View(ctypes.Structure):
 
 class RendererOutputs(ctypes.Structure):
     _fields_ = [("gi_sample", P), ("gi_receiver", P), ("gi_normal", P), ("color", P), ("moments", P), ("grid", P),
-                ("di_diffuse", P), ("di_specular", P), ("filtered_history", P), ("denoised", P)]
+                ("di_diffuse", P), ("di_specular", P), ("filtered_history", P), ("denoised", P), ("hdr", P)]
 
 
 class Shadow(ctypes.Structure):
@@ -198,6 +198,8 @@ def lib():
     L.rgi_renderer_autotune.argtypes = [P, P, P, U32, U32, P, P]
     L.rgi_reference_accumulate.argtypes = [ctypes.POINTER(GBuffer), P, P, P, P, P]
     L.rgi_reference_error.argtypes = [P, ctypes.POINTER(GBuffer), P, P, P, P, U32, P, P]
+    L.rgi_reference_accumulate_composite.argtypes = [ctypes.POINTER(GBuffer), P, P, P]
+    L.rgi_reference_error_composite.argtypes = [P, ctypes.POINTER(GBuffer), P, P, U32, P, P]
     if L.rgi_settings_sizeof() != ctypes.sizeof(CSettings):
         raise RestirError("ABI mismatch: rgi_settings")
     for i, cls in enumerate(ABI_STRUCTS):
@@ -346,6 +348,7 @@ class Settings:
         self.shadow_face_size = 1      # index into 256, 512, 1024, 2048
         self.denoiser = True           # SVGF variance + a-trous (settings.h:190)
         self.atrous_iterations = 5     # kMaxAtrousIterations
+        self.composite = True          # fs_composite.sc into an HDR image
         self.grid_mode = RGI_GRID_ATOMIC
         self.grid_table_size = 65536
         for k, v in kw.items():
@@ -387,6 +390,7 @@ def c_settings(settings):
     c.shadow_face_size = int(s.shadow_face_size)
     c.denoiser = int(bool(s.denoiser))
     c.atrous_iterations = int(s.atrous_iterations)
+    c.composite = int(bool(s.composite))
     return c
 
 

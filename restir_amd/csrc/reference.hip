@@ -14,11 +14,17 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTerms = 5; // relative, squared rgb, image luminance, reference luminance, count
 
-RGI_DEV bool image_at(const rgi_gbuffer& g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds, uint32_t i,
-                      F3& L)
+// The compared image at pixel i: the composite itself (`composited`), or the
+// GI re-modulated by the albedo plus the DI planes; geometry pixels only.
+RGI_DEV bool image_at(const rgi_gbuffer& g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+                      bool composited, uint32_t i, F3& L)
 {
     if (g.depth[i] >= kSkyDepth) return false;
     F4 c = load_h4(color, i);
+    if (composited) {
+        L = xyz(c);
+        return true;
+    }
     F4 a = unorm8(g.albedo[i]);
     float metallic = unorm8(g.normal[i]).w;
     L = F3{c.x, c.y, c.z} * (F3{a.x, a.y, a.z} * (1.0f - metallic));
@@ -28,12 +34,12 @@ RGI_DEV bool image_at(const rgi_gbuffer& g, const uint16_t* color, const uint16_
 }
 
 __global__ void __launch_bounds__(kBlock) k_ref_accumulate(rgi_gbuffer g, const uint16_t* color, const uint16_t* dd,
-                                                           const uint16_t* ds, double* accum)
+                                                           const uint16_t* ds, bool composited, double* accum)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)g.width * (uint32_t)g.height) return;
     F3 L;
-    if (!image_at(g, color, dd, ds, i, L)) return;
+    if (!image_at(g, color, dd, ds, composited, i, L)) return;
     if (L.x != L.x || L.y != L.y || L.z != L.z) return;
     accum[3u * i + 0u] += (double)L.x;
     accum[3u * i + 1u] += (double)L.y;
@@ -54,14 +60,14 @@ RGI_DEV void tree(double (*s)[kBlock], double* v)
 }
 
 __global__ void __launch_bounds__(kBlock) k_ref_error(rgi_gbuffer g, const uint16_t* color, const uint16_t* dd,
-                                                      const uint16_t* ds, const double* accum, double inv_frames,
-                                                      double* partial)
+                                                      const uint16_t* ds, bool composited, const double* accum,
+                                                      double inv_frames, double* partial)
 {
     __shared__ double s[kTerms][kBlock];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[kTerms] = {0.0, 0.0, 0.0, 0.0, 0.0};
     F3 L;
-    if (i < (uint32_t)g.width * (uint32_t)g.height && image_at(g, color, dd, ds, i, L)) {
+    if (i < (uint32_t)g.width * (uint32_t)g.height && image_at(g, color, dd, ds, composited, i, L)) {
         double rr = accum[3u * i] * inv_frames, rg = accum[3u * i + 1u] * inv_frames, rb = accum[3u * i + 2u] * inv_frames;
         double lr = (double)L.x, lg = (double)L.y, lb = (double)L.z;
         double li = lum(lr, lg, lb), lref = lum(rr, rg, rb);
@@ -100,22 +106,24 @@ __global__ void __launch_bounds__(kBlock) k_ref_final(const double* partial, uin
 
 using namespace rgi;
 
-extern "C" int rgi_reference_accumulate(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd,
-                                        const uint16_t* ds, double* accum, void* stream)
+namespace {
+
+int accumulate(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds, bool composited,
+               double* accum, void* stream)
 {
-    if (!g || !g->depth || !g->albedo || !g->normal || !color || !accum)
-        return fail(RGI_E_INVALID, "reference_accumulate: depth, albedo, normal, color and accum are required");
+    if (!g || !g->depth || (!composited && (!g->albedo || !g->normal)) || !color || !accum)
+        return fail(RGI_E_INVALID, "reference_accumulate: G-buffer planes, image and accum are required");
     if (g->width <= 0 || g->height <= 0) return fail(RGI_E_INVALID, "reference_accumulate: empty frame");
     uint32_t n = (uint32_t)g->width * (uint32_t)g->height;
     hipLaunchKernelGGL(k_ref_accumulate, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, as_stream(stream), *g, color,
-                       dd, ds, accum);
+                       dd, ds, composited, accum);
     return check_launch("reference accumulate");
 }
 
-extern "C" int rgi_reference_error(rgi_context* ctx, const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd,
-                                   const uint16_t* ds, const double* accum, uint32_t frames, double* out, void* stream)
+int error(rgi_context* ctx, const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd, const uint16_t* ds,
+          bool composited, const double* accum, uint32_t frames, double* out, void* stream)
 {
-    if (!ctx || !g || !g->depth || !g->albedo || !g->normal || !color || !accum || !out)
+    if (!ctx || !g || !g->depth || (!composited && (!g->albedo || !g->normal)) || !color || !accum || !out)
         return fail(RGI_E_INVALID, "reference_error: context, planes, accum and out are required");
     if (frames == 0) return fail(RGI_E_INVALID, "reference_error: no reference frames");
     if (g->width <= 0 || g->height <= 0) return fail(RGI_E_INVALID, "reference_error: empty frame");
@@ -124,8 +132,34 @@ extern "C" int rgi_reference_error(rgi_context* ctx, const rgi_gbuffer* g, const
     double* partial = static_cast<double*>(scratch(ctx, (size_t)blocks * kTerms * sizeof(double)));
     if (!partial) return RGI_E_NOMEM;
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_ref_error, dim3(blocks), dim3(kBlock), 0, s, *g, color, dd, ds, accum, 1.0 / (double)frames,
-                       partial);
+    hipLaunchKernelGGL(k_ref_error, dim3(blocks), dim3(kBlock), 0, s, *g, color, dd, ds, composited, accum,
+                       1.0 / (double)frames, partial);
     hipLaunchKernelGGL(k_ref_final, dim3(1), dim3(kBlock), 0, s, partial, blocks, out);
     return check_launch("reference error");
+}
+
+} // namespace
+
+extern "C" int rgi_reference_accumulate(const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd,
+                                        const uint16_t* ds, double* accum, void* stream)
+{
+    return accumulate(g, color, dd, ds, false, accum, stream);
+}
+
+extern "C" int rgi_reference_error(rgi_context* ctx, const rgi_gbuffer* g, const uint16_t* color, const uint16_t* dd,
+                                   const uint16_t* ds, const double* accum, uint32_t frames, double* out, void* stream)
+{
+    return error(ctx, g, color, dd, ds, false, accum, frames, out, stream);
+}
+
+extern "C" int rgi_reference_accumulate_composite(const rgi_gbuffer* g, const uint16_t* hdr, double* accum,
+                                                  void* stream)
+{
+    return accumulate(g, hdr, nullptr, nullptr, true, accum, stream);
+}
+
+extern "C" int rgi_reference_error_composite(rgi_context* ctx, const rgi_gbuffer* g, const uint16_t* hdr,
+                                             const double* accum, uint32_t frames, double* out, void* stream)
+{
+    return error(ctx, g, hdr, nullptr, nullptr, true, accum, frames, out, stream);
 }

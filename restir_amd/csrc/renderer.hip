@@ -35,6 +35,8 @@ struct rgi_renderer {
     uint16_t* filtered[2] = {};
     uint16_t* filter[2] = {};
     const uint16_t* denoised = nullptr;
+    uint16_t* hdr = nullptr; // the composite (m_hdrTexture, RGBA16F)
+    uint16_t* black = nullptr; // m_blackTexture: DI light when ReSTIR DI is selected but has no lights
     int read = 0;              // m_historyRead / m_diRead: the parity last frame wrote
     bool history_valid = false;
     bool di_valid = false;
@@ -80,6 +82,8 @@ void free_all(rgi_renderer* r)
     release(r->diffuse);
     release(r->specular);
     release(r->atlas);
+    release(r->hdr);
+    release(r->black);
     for (int k = 0; k < 2; ++k) {
         release(r->filtered[k]);
         release(r->filter[k]);
@@ -143,6 +147,7 @@ void rgi_settings_default(rgi_settings* s)
     s->shadow_face_size = 1;
     s->denoiser = 1;
     s->atrous_iterations = 5;
+    s->composite = 1;
 }
 
 uint32_t rgi_settings_sizeof(void) { return (uint32_t)sizeof(rgi_settings); }
@@ -252,6 +257,8 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
     }
     ok = ok && alloc(r->diffuse, 4 * n) && alloc(r->specular, 4 * n);
     for (int k = 0; k < 2 && ok; ++k) ok = alloc(r->filtered[k], 4 * n) && alloc(r->filter[k], 4 * n);
+    if (settings->composite) ok = ok && alloc(r->hdr, 4 * n);
+    if (settings->restir_di) ok = ok && alloc(r->black, 4 * n);
     if (settings->shadows) {
         size_t F = (size_t)shadow_face_size(settings);
         ok = ok && alloc(r->atlas, 6 * F * F);
@@ -326,7 +333,9 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         }
         shadow_map = r->atlas;
     }
-    const uint16_t* surface_light = g->emission;
+    // The GI passes' surface light (renderer.cpp:690-695): the DI diffuse
+    // light with ReSTIR DI (black when it has no lights), else the emission.
+    const uint16_t* surface_light = s.restir_di ? r->black : g->emission;
     if (di_active) {
         rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
         rgi_di_reservoirs next_di{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
@@ -371,6 +380,20 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
             if ((e = svgf_atrous(f, input, receiver, normal, 1 << it, output, stream)) != RGI_OK) return e;
             r->denoised = output;
         }
+    }
+
+    // Composite (renderer.cpp:967-990): the final image's weights, no
+    // indirect specular in this build.
+    if (s.composite && r->hdr) {
+        rgi_composite_inputs ci{};
+        ci.gi = r->denoised;
+        ci.di_diffuse = di_active ? r->diffuse : nullptr;
+        ci.di_specular = di_active ? r->specular : nullptr;
+        ci.specular = nullptr;
+        ci.shadow_map = shadow_map;
+        for (int k = 0; k < 4; ++k) ci.options[k] = 1.0f;
+        ci.specular_weight = 0.0f;
+        if ((e = composite(f, *g, ci, r->hdr, stream)) != RGI_OK) return e;
     }
 
     // End of frame (renderer.cpp:1119-1124).
@@ -460,6 +483,7 @@ int rgi_renderer_outputs_get(const rgi_renderer* r, rgi_renderer_outputs* out)
     out->di_specular = r->specular;
     out->filtered_history = r->filtered[k];
     out->denoised = r->denoised ? r->denoised : r->color[k];
+    out->hdr = r->hdr;
     return RGI_OK;
 }
 

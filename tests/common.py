@@ -104,7 +104,21 @@ def oracle_denoise(frame, out, atrous_iterations):
     return filtered, denoised
 
 
-def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterations=5):
+def oracle_composite(frame, g, out, atlas, di_on):
+    """The composite of Renderer::render's final image (renderer.cpp:967-990)."""
+    n = g["depth"].shape[0]
+    hdr = np.zeros((n, 4), np.uint16)
+    ci = ra.CompositeInputs(out["denoised"].ctypes.data, out["diffuse"].ctypes.data if di_on else None,
+                            out["specular"].ctypes.data if di_on else None, None,
+                            atlas.ctypes.data if atlas is not None else None)
+    ci.options[:] = [1.0, 1.0, 1.0, 1.0]
+    ci.specular_weight = 0.0
+    orc.lib().orc_composite(ctypes.byref(frame), ctypes.byref(host_gbuffer(g, frame.width, frame.height)),
+                            ctypes.byref(ci), orc.ptr(hdr))
+    return hdr
+
+
+def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterations=5, composite=False):
     """One frame on the oracle; returns this frame's outputs (host arrays)."""
     L = orc.lib()
     w, h, T = frame.width, frame.height, frame.grid_table_size
@@ -127,6 +141,9 @@ def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterati
         L.orc_di_spatial(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(dres),
                          orc.ptr(out["diffuse"]), orc.ptr(out["specular"]))
         surface_light = out["diffuse"]
+    elif frame.lighting_surface == 1.0 and frame.lighting_emission == 0.0:
+        # ReSTIR DI selected without lights: a black surface light (renderer.cpp:690-695).
+        surface_light = np.zeros_like(g["emission"])
     else:
         surface_light = g["emission"]
     grid = np.zeros(34 * T, np.uint32)
@@ -158,6 +175,8 @@ def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterati
         if filtered is not None:
             out["filtered"] = filtered
         out["denoised"] = denoised
+    if composite:
+        out["hdr"] = oracle_composite(frame, g, out, atlas, frame.di_light_count > 0)
     return out
 
 
@@ -183,7 +202,8 @@ def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0, distance=
         f = frame_for(hs, settings, width, height, i, history_valid=i > 0, di_history_valid=i > 0 and di_on,
                       yaw_step=yaw_step, distance=distance)
         g = oracle_gbuffer(hs, f)
-        out = oracle_frame(hs, f, g, hist, denoiser=settings.denoiser, atrous_iterations=settings.atrous_iterations)
+        out = oracle_frame(hs, f, g, hist, denoiser=settings.denoiser, atrous_iterations=settings.atrous_iterations,
+                           composite=settings.composite)
         seq.append((f, g, hist, out))
         hist = next_history(hist, out)
     return seq

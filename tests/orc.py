@@ -84,6 +84,10 @@ def lib():
     L.orc_specular_albedo.argtypes = [P, F, F, P]
     L.orc_composite.restype = None
     L.orc_composite.argtypes = [PF, PG, ctypes.POINTER(CompositeInputs), P]
+    L.orc_reference_accumulate_composite.restype = None
+    L.orc_reference_accumulate_composite.argtypes = [PG, P, P]
+    L.orc_reference_error_composite.restype = None
+    L.orc_reference_error_composite.argtypes = [PG, P, P, U32, P]
     L.orc_expf.restype = F
     L.orc_expf.argtypes = [F]
     L.orc_pow64.restype = F

@@ -94,6 +94,8 @@ def test_renderer_matches_oracle_sequence(torch, frames, kw, variants, scene):
                 checks += [("diffuse", out.di_diffuse), ("specular", out.di_specular)]
             if s.denoiser:
                 checks += [("filtered", out.filtered_history), ("denoised", out.denoised)]
+            if s.composite:
+                checks += [("hdr", out.hdr)]
             for name, ptr in checks:
                 got = download(torch, ptr, want[name])
                 if name == "grid":
@@ -142,4 +144,27 @@ def test_reference_accumulate_and_error(torch):
     gh = common.host_gbuffer(g, W, H)
     orc.lib().orc_reference_error(ctypes.byref(gh), orc.ptr(out["color"]), orc.ptr(out["diffuse"]),
                                   orc.ptr(out["specular"]), orc.ptr(want_acc), 3, orc.ptr(want))
+    assert np.array_equal(got, want), (got, want)
+
+    # The same over the composite (the image the reference measures).
+    want_acc = np.zeros(3 * n, np.float64)
+    acc.zero_()
+    for f, g, hist, out in seq:
+        gh = common.host_gbuffer(g, W, H)
+        orc.lib().orc_reference_accumulate_composite(ctypes.byref(gh), orc.ptr(out["hdr"]), orc.ptr(want_acc))
+        planes = {k: upload(torch, v) for k, v in g.items()}
+        hdr = upload(torch, out["hdr"])
+        keep += [planes, hdr]
+        gb = ra.GBuffer(W, H, planes["depth"].data_ptr(), planes["albedo"].data_ptr(), planes["normal"].data_ptr(),
+                        planes["shading"].data_ptr(), planes["emission"].data_ptr())
+        ra.check(L.rgi_reference_accumulate_composite(ctypes.byref(gb), hdr.data_ptr(), acc.data_ptr(), stream))
+    assert np.array_equal(acc.cpu().numpy(), want_acc)
+    ctx = ctypes.c_void_p()
+    ra.check(L.rgi_create(ctypes.byref(ctx)))
+    ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gb), hdr.data_ptr(), acc.data_ptr(), 3, res.data_ptr(),
+                                             stream))
+    got = res.cpu().numpy()
+    L.rgi_destroy(ctx)
+    orc.lib().orc_reference_error_composite(ctypes.byref(gh), orc.ptr(out["hdr"]), orc.ptr(want_acc), 3,
+                                            orc.ptr(want))
     assert np.array_equal(got, want), (got, want)

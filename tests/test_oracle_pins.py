@@ -906,6 +906,21 @@ def test_reference_error_matches_numpy():
     assert abs(res[1] - mse) <= 1e-12 * max(mse, 1e-30) + 1e-18
     assert abs(res[2] - li[geo].mean()) < 1e-12 and abs(res[3] - lr[geo].mean()) < 1e-12
 
+    # The composite variant: the HDR image itself, the same statistics.
+    accum = np.zeros(3 * n, np.float64)
+    for f, g, hist, out in seq:
+        L.orc_reference_accumulate_composite(ctypes.byref(common.host_gbuffer(g, 64, 48)), orc.ptr(out["hdr"]),
+                                             orc.ptr(accum))
+    f, g, hist, out = seq[-1]
+    L.orc_reference_error_composite(ctypes.byref(common.host_gbuffer(g, 64, 48)), orc.ptr(out["hdr"]),
+                                    orc.ptr(accum), 3, orc.ptr(res))
+    img = out["hdr"].view(np.float16).astype(np.float64)[:, :3]
+    ref = accum.reshape(-1, 3) / 3.0
+    li, lr = img @ w, ref @ w
+    rel = ((li - lr) ** 2 / (lr ** 2 + 0.01))[geo].mean()
+    assert res[4] == geo.sum() and abs(res[0] - rel) <= 1e-12 * max(rel, 1e-30) + 1e-18
+    assert abs(res[1] - ((img - ref) ** 2)[geo].mean()) <= 1e-12
+
 
 def test_frame_pack_follows_renderer_uniforms():
     """rgi_frame_pack packs u_gridInfo, u_sceneInfo and u_lightingInfo as
