@@ -155,7 +155,7 @@ def measure_mse(L, hs, gscene, gbuf, view, W, H, T, mode, ref_frames, frames, ct
         L.rgi_renderer_destroy(rend)
     line = dict(results["display"])
     line.update({"reference_frames": ref_frames, "frames": frames,
-                 "method": "ReSTIR GI + DI, SVGF denoiser (reference defaults), composite; no TAA",
+                 "method": "ReSTIR GI + DI, indirect specular, SVGF denoiser (reference defaults), composite; no TAA",
                  "image": "composite HDR (fs_composite.sc), static camera, geometry pixels",
                  "no_denoiser": results["no_denoiser"]})
     return line
@@ -286,7 +286,7 @@ def main():
     mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
     # The metric's frame: grid + resample + trace. The denoiser that follows
     # it in the reference is timed on its own below.
-    settings = ra.Settings(grid_table_size=T, grid_mode=mode, denoiser=False, composite=False)
+    settings = ra.Settings(grid_table_size=T, grid_mode=mode, denoiser=False, composite=False, specular_gi=False)
     run = run_workload(hs, settings, W, H, args.frames, args.steps, args.warmup, args.tune_frames)
     ms_per_frame, pass_ms, tuning, options = run["ms_per_frame"], run["pass_ms"], run["tuning"], run["options"]
     gscene, drv, rend, gbufs, views = run["gscene"], run["drv"], run["rend"], run["gbufs"], run["views"]
@@ -317,10 +317,29 @@ def main():
     torch.cuda.synchronize()
     denoiser_ms = e0.elapsed_time(e1) / 10
 
+    # Indirect specular of that frame: one GGX ray per pixel (traced like the GI rays).
+    spec_t = torch.empty(W * H * 8, dtype=torch.uint8, device="cuda")
+    sgin = ra.GiInputs(None, outs.gi_receiver, outs.gi_normal, outs.color, outs.moments,
+                       outs.di_diffuse if outs.di_diffuse else gbufs[0].gb.emission, None, outs.grid)
+
+    def spec():
+        ra.check(L.rgi_specular_gi(drv.ctx, ctypes.byref(last), ctypes.byref(gscene.scene), ctypes.byref(gbufs[0].gb),
+                                   ctypes.byref(sgin), None, None, spec_t.data_ptr(), s))
+
+    spec()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        spec()
+    e1.record()
+    torch.cuda.synchronize()
+    specular_ms = e0.elapsed_time(e1) / 10
+
     # The composite of that frame (main light off, as the workload has it).
     hdr_t = torch.empty(W * H * 8, dtype=torch.uint8, device="cuda")
-    ci = ra.CompositeInputs(den[1].data_ptr(), outs.di_diffuse, outs.di_specular, None, None)
+    ci = ra.CompositeInputs(den[1].data_ptr(), outs.di_diffuse, outs.di_specular, spec_t.data_ptr(), None)
     ci.options[:] = [1.0, 1.0, 1.0, 1.0]
+    ci.specular_weight = 1.0
 
     def comp():
         ra.check(L.rgi_composite(drv.ctx, ctypes.byref(last), ctypes.byref(gbufs[0].gb), ctypes.byref(ci),
@@ -395,7 +414,8 @@ def main():
         # point lights and the emissive triangles; the same swing path, with
         # the camera inside the courtyard (0.7 scene radii from the centre).
         ahs = ra.HostScene(ra.RGI_SCENE_ATRIUM, seed=1, size=262144, extra_lights=64, light_seed=1)
-        aset = ra.Settings(grid_table_size=1 << 22, grid_mode=mode, denoiser=False, composite=False)
+        aset = ra.Settings(grid_table_size=1 << 22, grid_mode=mode, denoiser=False, composite=False,
+                           specular_gi=False)
         arun = run_workload(ahs, aset, W, H, args.frames, args.atrium_steps, 8, 4, distance=0.7)
         atrium = {"ms_per_frame": round(arun["ms_per_frame"], 5),
                   "pass_ms": {k: round(v, 5) for k, v in sorted(arun["pass_ms"].items())},
@@ -425,6 +445,7 @@ def main():
         "pass_ms": {k: round(v, 5) for k, v in sorted(pass_ms.items())},
         "denoiser_ms": round(denoiser_ms, 5),
         "composite_ms": round(composite_ms, 5),
+        "specular_gi_ms": round(specular_ms, 5),
         "mrays_per_s": round(mrays, 2),
         "trace": {"rays": args.rays, "ms": round(trace_ms, 4), "hit_rate": round(hit_rate, 4), "mode": "closest"},
         "mse_vs_reference": mse,

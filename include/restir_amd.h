@@ -303,6 +303,19 @@ int rgi_svgf_variance(rgi_context* context, const rgi_frame* frame, const uint16
 int rgi_svgf_atrous(rgi_context* context, const rgi_frame* frame, const uint16_t* input, const float* receiver,
                     const float* normal, int32_t step, uint16_t* out, void* stream);
 
+/* Indirect specular (fs_specular.sc, dispatched at renderer.cpp:836-855):
+ * one GGX visible-normal ray per GI pixel, traced like the GI rays (depth
+ * buffer, then BVH, then sky) and shaded with last frame's composite where
+ * the hit is a visible surface that was on screen. `in` supplies last frame's
+ * GI receiver, normal and colour planes (the same-surface test and bounce
+ * light), the surface light and the shadow atlas (history_sample and grid
+ * are not read). `previous_hdr`: last frame's composite (NULL: none);
+ * `history`: last frame's output of this pass (NULL: none). Writes RGBA16F
+ * rgb = demodulated specular radiance, a = history length. */
+int rgi_specular_gi(rgi_context* context, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
+                    const rgi_gi_inputs* in, const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out,
+                    void* stream);
+
 /* The composite pass's inputs besides the G-buffer (fs_composite.sc:10-29).
  * GI planes are frame->width x frame->height (the GI resolution); the DI
  * planes, the G-buffer and the output are gbuffer->width x height. */

@@ -55,6 +55,8 @@ typedef struct rgi_settings {
     int32_t denoiser;             /* 1: variance + a-trous passes; GI history colour = filtered history */
     int32_t atrous_iterations;    /* 5 (kMaxAtrousIterations), 0..5 */
     int32_t composite;            /* 1: the composite pass into an RGBA16F HDR image every frame */
+    int32_t specular_gi;          /* 1: indirect specular (fs_specular.sc), composited with weight 1 */
+    int32_t reserved[3];
 } rgi_settings;
 
 /* One frame's camera: eye and (jittered) view-projection and its inverse. */
@@ -77,6 +79,7 @@ typedef struct rgi_renderer_outputs {
     const uint16_t* filtered_history; /* RGBA16F first a-trous iteration (next frame's GI history colour) */
     const uint16_t* denoised;     /* RGBA16F the composite's GI input: the last a-trous output, or `color` */
     const uint16_t* hdr;          /* RGBA16F composite (NULL when the composite is off) */
+    const uint16_t* specular_gi;  /* RGBA16F demodulated indirect specular, history length (NULL when off) */
 } rgi_renderer_outputs;
 
 typedef struct rgi_renderer rgi_renderer;

@@ -1044,6 +1044,83 @@ Pass make_pass(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g, con
     return p;
 }
 
+// ----------------------------------------------------------- specular GI
+
+// sampleVisibleNormal (fs_specular.sc:51-68): Heitz 2018's sampling of the
+// GGX normals visible from `view` (tangent frame, z up).
+V3 sample_visible_normal(V3 view, float alpha, float r0, float r1)
+{
+    V3 stretched = normalize(V3{alpha * view.x, alpha * view.y, view.z});
+    float length_squared = stretched.x * stretched.x + stretched.y * stretched.y;
+    V3 t1 = length_squared > 0.0f ? V3{-stretched.y, stretched.x, 0.0f} * inversesqrt(length_squared)
+                                  : V3{1.0f, 0.0f, 0.0f};
+    V3 t2 = cross(stretched, t1);
+    float radius = sqrtf(r0);
+    float angle = (2.0f * kPi) * r1;
+    float sa, ca;
+    rgi_sincos(angle, &sa, &ca);
+    float p1 = radius * ca;
+    float p2 = radius * sa;
+    float s = 0.5f * (1.0f + stretched.z);
+    p2 = (1.0f - s) * sqrtf(fmax_(1.0f - p1 * p1, 0.0f)) + s * p2;
+    V3 normal = (t1 * p1 + t2 * p2) + stretched * sqrtf(fmax_((1.0f - p1 * p1) - p2 * p2, 0.0f));
+    return normalize(V3{alpha * normal.x, alpha * normal.y, fmax_(normal.z, 0.0f)});
+}
+
+// smithG1 (fs_specular.sc:71-75).
+float smith_g1_ggx(float c, float alpha)
+{
+    float a2 = alpha * alpha;
+    return (2.0f * c) / (c + sqrtf(a2 + ((1.0f - a2) * c) * c));
+}
+
+V3 reflect3(V3 i, V3 n) { return i - n * (2.0f * dot(n, i)); }
+
+// The reflected direction of fs_specular.sc:160-167 at a surface.
+V3 specular_direction(V3 n, V3 view, float alpha, float r0, float r1, V3& halfway)
+{
+    V3 tangent, bitangent;
+    tangent_frame(n, tangent, bitangent);
+    V3 local_view = V3{dot(view, tangent), dot(view, bitangent), fmax_(dot(n, view), 0.001f)};
+    V3 lh = sample_visible_normal(local_view, alpha, r0, r1);
+    halfway = (tangent * lh.x + bitangent * lh.y) + n * lh.z;
+    return reflect3(-view, halfway);
+}
+
+// hitRadiance (fs_specular.sc:78-124).
+V3 specular_hit_radiance(const Pass& p, const uint16_t* previous_hdr, V3 hit, float id, V3 dir)
+{
+    const rgi_frame* f = p.f;
+    if (is_sky_sample(id)) return sky_radiance(f, dir);
+    V2 uv;
+    float view_depth;
+    bool visible_surface = false;
+    if (project_to_screen(f, hit, uv, view_depth)) {
+        float depth = fetch_depth(p, uv);
+        visible_surface = depth < kSkyDepth && length(reconstruct_world_position(f, uv, depth) - hit) <= 0.1f * scene_scale(p);
+    }
+    if (visible_surface) {
+        V4 clip = mul(f->prev_view_proj, V4{hit.x, hit.y, hit.z, 1.0f});
+        if (f->history_valid && clip.w > 0.0001f && previous_hdr != nullptr) {
+            V2 previous_uv = clip_to_uv(f, clip);
+            V4 previous = half4(previous_hdr + texel(previous_uv, p.g->width, p.g->height) * 4);
+            if (is_on_screen(previous_uv) && previous.w > 0.5f) return xyz(previous);
+        }
+        V3 n = decode_normal(p, uv);
+        V3 albedo = diffuse_albedo(p, uv);
+        return (world_direct(p, albedo, hit, n) + previous_indirect(p, hit, n, albedo)) +
+               xyz(fetch_h4(p.in->surface_light, p, uv)) * f->lighting_surface;
+    }
+    int tri = to_int(fabsf(id) + 0.5f) - 1;
+    if (tri < 0 || !bvh_available(p.s)) return v3s(0.0f);
+    V3 v0 = bvh_vertex(p.s, tri, 0);
+    V3 n = normalize(cross(bvh_vertex(p.s, tri, 1) - v0, bvh_vertex(p.s, tri, 2) - v0));
+    if (dot(n, -dir) < 0.0f) n = -n;
+    V3 albedo = bvh_triangle_albedo(p.s, tri);
+    return (world_direct(p, albedo, hit, n) + previous_indirect(p, hit, n, albedo)) +
+           bvh_triangle_emission(p.s, tri) * f->lighting_emission;
+}
+
 // One thread of cs_restir_temporal up to the group sum (cs_restir_temporal.sc:57-269).
 struct TemporalThread {
     bool inside, working;
@@ -1505,6 +1582,101 @@ void orc_composite(const rgi_frame* f, const rgi_gbuffer* g, const rgi_composite
 // fs_svgf_variance.sc:24-73: per-pixel luminance variance, from the pixel's
 // own moments after 4 frames of history, else from its 7 x 7 same-surface
 // neighbourhood, boosted by 4 / history length.
+// The reflected direction and its weight G1(l) for roughness `roughness`
+// (alpha = roughness^2) from two randoms: fs_specular.sc:139-170.
+void orc_specular_sample(const float* n3, const float* v3, float roughness, float r0, float r1, float* out_dir,
+                         float* out_g1)
+{
+    V3 n = V3{n3[0], n3[1], n3[2]}, view = V3{v3[0], v3[1], v3[2]};
+    float alpha = roughness * roughness;
+    V3 halfway;
+    V3 d = specular_direction(n, view, alpha, r0, r1, halfway);
+    out_dir[0] = d.x;
+    out_dir[1] = d.y;
+    out_dir[2] = d.z;
+    float nl = dot(n, d);
+    *out_g1 = nl > 0.0f ? smith_g1_ggx(nl, alpha) : 0.0f;
+}
+
+// fs_specular.sc:127-226: one ray per pixel, sequentially per pixel.
+void orc_specular_gi(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g, const rgi_gi_inputs* in,
+                     const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out)
+{
+    Pass p = make_pass(f, s, g, in, false);
+    float sc = f->scene_scale;
+    int w = g->width, h = g->height;
+    V3 camera = V3{f->camera_position[0], f->camera_position[1], f->camera_position[2]};
+    parallel_for((int64_t)w * h, [&](int64_t i) {
+        int x = (int)(i % w), y = (int)(i / w);
+        V2 uv = V2{((float)x + 0.5f) * p.texel_size.x, ((float)y + 0.5f) * p.texel_size.y};
+        float depth = fetch_depth(p, uv);
+        if (depth >= kSkyDepth) {
+            store_half4(out + i * 4, V4{0.0f, 0.0f, 0.0f, 0.0f});
+            return;
+        }
+        V3 position = reconstruct_world_position(f, uv, depth);
+        V3 n = decode_normal(p, uv);
+        size_t t = texel(uv, w, h);
+        V4 albedo_roughness = unorm8(g->albedo[t]);
+        float metallic = unorm8(g->normal[t]).w;
+        float roughness = fmax_(albedo_roughness.w, 0.02f);
+        float alpha = roughness * roughness;
+        V3 view = normalize(camera - position);
+        float normal_view = fmax_(dot(n, view), 0.001f);
+        V3 f0 = mix3(v3s(0.04f), xyz(albedo_roughness), metallic);
+        float gx = floorf(uv.x * p.size.x), gy = floorf(uv.y * p.size.y);
+        uint32_t rng = pcg((uint32_t)to_int(gx) + pcg((uint32_t)to_int(gy) + pcg(f->frame_index ^ 0x2545f491u)));
+        float r0 = next_random(rng);
+        float r1 = next_random(rng);
+        V3 halfway;
+        V3 direction = specular_direction(n, view, alpha, r0, r1, halfway);
+        float normal_light = dot(n, direction);
+        V3 estimate = v3s(0.0f);
+        if (normal_light > 0.0f) {
+            V3 origin = position + n * (0.02f * sc);
+            V3 hit_position = v3s(0.0f);
+            float hit_id = 0.0f;
+            bool hit = false;
+            if (f->ray_mode != RGI_RAY_BVH) hit = trace_screen_space_ray(p, origin, direction, 3.6f * sc, hit_position);
+            if (!hit && f->ray_mode != RGI_RAY_SCREEN) {
+                float dist = 0.0f;
+                int tri = trace_bvh(s, origin, direction, 4.0f * sc, false, dist);
+                if (tri >= 0) {
+                    hit = true;
+                    hit_position = origin + direction * dist;
+                    hit_id = encode_sample_id(s, tri, direction);
+                } else if (sky_enabled(f)) {
+                    hit = true;
+                    hit_id = kSkySampleId;
+                }
+            }
+            if (hit) {
+                V3 fresnel = fresnel_schlick(halfway, view, f0);
+                estimate = (specular_hit_radiance(p, previous_hdr, hit_position, hit_id, direction) * fresnel) *
+                           smith_g1_ggx(normal_light, alpha);
+            }
+        }
+        V3 demodulated = estimate / vmax(specular_albedo(f0, roughness, normal_view), v3s(0.001f));
+        if (f->outlier_clamp > 0.0f) demodulated = vmin(demodulated, v3s(f->outlier_clamp));
+        V3 previous = v3s(0.0f);
+        float history_length = 0.0f;
+        V4 clip_previous = mul(f->prev_view_proj, V4{position.x, position.y, position.z, 1.0f});
+        if (f->history_valid && clip_previous.w > 0.0001f && history != nullptr) {
+            V2 cuv = clip_to_uv(f, clip_previous);
+            V2 puv = V2{(floorf(cuv.x * p.size.x) + 0.5f) * p.texel_size.x, (floorf(cuv.y * p.size.y) + 0.5f) * p.texel_size.y};
+            if (is_on_screen(puv) && history_matches(p, puv, position, n)) {
+                V4 hv = fetch_h4(history, p, puv);
+                previous = xyz(hv);
+                history_length = hv.w;
+            }
+        }
+        float max_history = mixf(4.0f, 32.0f, clampf(roughness * 1.5f, 0.0f, 1.0f));
+        history_length = fmin_(history_length + 1.0f, max_history);
+        V3 color = mix3(previous, demodulated, 1.0f / history_length);
+        store_half4(out + i * 4, V4{color.x, color.y, color.z, history_length});
+    });
+}
+
 void orc_svgf_variance(const rgi_frame* f, const uint16_t* color, const uint16_t* moments, const float* receiver,
                        const float* normal, uint16_t* out)
 {

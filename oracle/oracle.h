@@ -44,6 +44,13 @@ float orc_pow64(float x);
 float orc_exp2f(float x);
 void orc_specular_albedo(const float* f0, float roughness, float nv, float* out);
 
+/* The indirect specular pass of rgi_specular_gi (fs_specular.sc), and its
+ * sampled direction with weight G1(l) for pins. */
+void orc_specular_gi(const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
+                     const rgi_gi_inputs* in, const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out);
+void orc_specular_sample(const float* n3, const float* v3, float roughness, float r0, float r1, float* out_dir,
+                         float* out_g1);
+
 /* The composite pass of rgi_composite (fs_composite.sc), on the CPU. */
 void orc_composite(const rgi_frame* frame, const rgi_gbuffer* gbuffer, const rgi_composite_inputs* inputs,
                    uint16_t* out_hdr);

@@ -93,7 +93,7 @@ class CSettings(ctypes.Structure):
         ("grid_table_size", U32), ("sky_zenith", F * 3), ("sky_horizon", F * 3), ("sky_ground", F * 3),
         ("sky_intensity", F), ("light_position", F * 3), ("light_color", F * 3), ("origin_bottom_left", I32),
         ("homogeneous_depth", I32), ("shadow_face_size", I32), ("denoiser", I32), ("atrous_iterations", I32),
-        ("composite", I32),
+        ("composite", I32), ("specular_gi", I32), ("reserved", I32 * 3),
     ]
 
 
@@ -104,7 +104,8 @@ class View(ctypes.Structure):
 
 class RendererOutputs(ctypes.Structure):
     _fields_ = [("gi_sample", P), ("gi_receiver", P), ("gi_normal", P), ("color", P), ("moments", P), ("grid", P),
-                ("di_diffuse", P), ("di_specular", P), ("filtered_history", P), ("denoised", P), ("hdr", P)]
+                ("di_diffuse", P), ("di_specular", P), ("filtered_history", P), ("denoised", P), ("hdr", P),
+                ("specular_gi", P)]
 
 
 class Shadow(ctypes.Structure):
@@ -160,6 +161,8 @@ def lib():
                                  ctypes.POINTER(DiReservoirs), P, P, P]
     L.rgi_trace.argtypes = [P, ctypes.POINTER(Scene), P, U32, P, P, ctypes.c_int, P]
     L.rgi_gbuffer_primary.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), P, U32, P, P, P, P, P, P]
+    L.rgi_specular_gi.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), ctypes.POINTER(GBuffer),
+                                  ctypes.POINTER(GiInputs), P, P, P, P]
     L.rgi_composite.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), ctypes.POINTER(CompositeInputs), P, P]
     L.rgi_svgf_variance.argtypes = [P, ctypes.POINTER(Frame), P, P, P, P, P, P]
     L.rgi_svgf_atrous.argtypes = [P, ctypes.POINTER(Frame), P, P, P, I32, P, P]
@@ -349,6 +352,7 @@ class Settings:
         self.denoiser = True           # SVGF variance + a-trous (settings.h:190)
         self.atrous_iterations = 5     # kMaxAtrousIterations
         self.composite = True          # fs_composite.sc into an HDR image
+        self.specular_gi = True        # fs_specular.sc (settings.h:196)
         self.grid_mode = RGI_GRID_ATOMIC
         self.grid_table_size = 65536
         for k, v in kw.items():
@@ -391,6 +395,7 @@ def c_settings(settings):
     c.denoiser = int(bool(s.denoiser))
     c.atrous_iterations = int(s.atrous_iterations)
     c.composite = int(bool(s.composite))
+    c.specular_gi = int(bool(s.specular_gi))
     return c
 
 

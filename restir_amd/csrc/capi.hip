@@ -295,6 +295,23 @@ int rgi_shadow_atlas(rgi_context* ctx, const rgi_shadow* shadow, const rgi_scene
     return shadow_atlas(*shadow, *scene, atlas, as_stream(stream));
 }
 
+int rgi_specular_gi(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,
+                    const rgi_gi_inputs* in, const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out,
+                    void* stream)
+{
+    int e;
+    if ((e = check_frame(frame)) != RGI_OK) return e;
+    if ((e = check_scene(scene, false)) != RGI_OK) return e;
+    if ((e = check_gbuffer(g, true, false)) != RGI_OK) return e;
+    if (!in || !out) return fail(RGI_E_INVALID, "specular_gi: inputs and output are required");
+    if (frame->history_valid && (!in->history_receiver || !in->history_normal || !in->history_color))
+        return fail(RGI_E_INVALID, "specular_gi: history planes are required when history_valid");
+    if (!in->surface_light) return fail(RGI_E_INVALID, "specular_gi: surface_light is required");
+    if (history == out) return fail(RGI_E_INVALID, "specular_gi: out must not alias history");
+    return specular_gi(*frame, *scene, *g, *in, previous_hdr, history, out, ctx ? ctx->tuning : Tuning{},
+                       as_stream(stream));
+}
+
 int rgi_composite(rgi_context* ctx, const rgi_frame* frame, const rgi_gbuffer* g, const rgi_composite_inputs* in,
                   uint16_t* out, void* stream)
 {

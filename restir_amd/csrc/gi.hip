@@ -18,27 +18,6 @@
 namespace rgi {
 namespace {
 
-RGI_DEV void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-RGI_DEV uint32_t rank_in(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-// Pixel of this thread: 16 x 16 per block, two 4 x 8 groups per wave.
-RGI_DEV void group_pixel(uint32_t order, int& x, int& y)
-{
-    uint32_t group = threadIdx.x >> 5, t = threadIdx.x & 31u, tx, ty;
-    block_tile(order, tx, ty);
-    x = (int)(tx * 16u + (group & 3u) * 4u + (t & 3u));
-    y = (int)(ty * 16u + (group >> 2) * 8u + (t >> 2));
-}
-
 // Per-block LDS of the temporal pass: the closest-hit stack rings and the
 // per-wave ray queue.
 struct TemporalLds {

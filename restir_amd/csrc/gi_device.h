@@ -113,6 +113,30 @@ RGI_DEV F3 previous_indirect(const GiPass& p, F3 position, F3 n, F3 albedo)
     return xyz(fetch_h4(p.in.history_color, p, uv)) * albedo;
 }
 
+// ------------------------------------------------------------ waves
+
+// Orders this wave's LDS writes before its reads (the wave runs in lockstep).
+RGI_DEV void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+RGI_DEV uint32_t rank_in(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Pixel of this thread: 16 x 16 per block, two 4 x 8 groups per wave.
+RGI_DEV void group_pixel(uint32_t order, int& x, int& y)
+{
+    uint32_t group = threadIdx.x >> 5, t = threadIdx.x & 31u, tx, ty;
+    block_tile(order, tx, ty);
+    x = (int)(tx * 16u + (group & 3u) * 4u + (t & 3u));
+    y = (int)(ty * 16u + (group >> 2) * 8u + (t >> 2));
+}
+
 // A marched ray of traceScreenSpaceRay: the point at distance t is
 // origin + dir * t; its clip position is clip_origin + t * clip_dir (x, y
 // and w kept), the projection distributed over the ray.

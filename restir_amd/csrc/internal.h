@@ -47,6 +47,9 @@ int gbuffer_primary(const rgi_frame& f, const rgi_scene& s, const float* materia
                     float* depth, uint32_t* albedo, uint32_t* normal, uint32_t* shading_normal, uint16_t* emission,
                     hipStream_t stream);
 int shadow_atlas(const rgi_shadow& sh, const rgi_scene& s, float* atlas, hipStream_t stream);
+int specular_gi(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+                const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out, const Tuning& tuning,
+                hipStream_t stream);
 int composite(const rgi_frame& f, const rgi_gbuffer& g, const rgi_composite_inputs& in, uint16_t* out,
               hipStream_t stream);
 int svgf_variance(const rgi_frame& f, const uint16_t* color, const uint16_t* moments, const float* receiver,

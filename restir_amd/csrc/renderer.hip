@@ -37,6 +37,8 @@ struct rgi_renderer {
     const uint16_t* denoised = nullptr;
     uint16_t* hdr = nullptr; // the composite (m_hdrTexture, RGBA16F)
     uint16_t* black = nullptr; // m_blackTexture: DI light when ReSTIR DI is selected but has no lights
+    uint16_t* specular_gi[2] = {}; // m_specularTextures, RGBA16F, own parity m_specularRead
+    int specular_read = 0;
     int read = 0;              // m_historyRead / m_diRead: the parity last frame wrote
     bool history_valid = false;
     bool di_valid = false;
@@ -84,6 +86,8 @@ void free_all(rgi_renderer* r)
     release(r->atlas);
     release(r->hdr);
     release(r->black);
+    release(r->specular_gi[0]);
+    release(r->specular_gi[1]);
     for (int k = 0; k < 2; ++k) {
         release(r->filtered[k]);
         release(r->filter[k]);
@@ -148,6 +152,7 @@ void rgi_settings_default(rgi_settings* s)
     s->denoiser = 1;
     s->atrous_iterations = 5;
     s->composite = 1;
+    s->specular_gi = 1;
 }
 
 uint32_t rgi_settings_sizeof(void) { return (uint32_t)sizeof(rgi_settings); }
@@ -259,6 +264,7 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
     for (int k = 0; k < 2 && ok; ++k) ok = alloc(r->filtered[k], 4 * n) && alloc(r->filter[k], 4 * n);
     if (settings->composite) ok = ok && alloc(r->hdr, 4 * n);
     if (settings->restir_di) ok = ok && alloc(r->black, 4 * n);
+    if (settings->specular_gi) ok = ok && alloc(r->specular_gi[0], 4 * n) && alloc(r->specular_gi[1], 4 * n);
     if (settings->shadows) {
         size_t F = (size_t)shadow_face_size(settings);
         ok = ok && alloc(r->atlas, 6 * F * F);
@@ -364,6 +370,20 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         RGI_OK)
         return e;
 
+    // Indirect specular (renderer.cpp:836-855): last frame's composite is
+    // still in the HDR target.
+    const uint16_t* specular = nullptr;
+    if (s.specular_gi && r->specular_gi[0]) {
+        const int sw = 1 - r->specular_read;
+        rgi_gi_inputs spin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
+                           surface_light, shadow_map, r->grid[write]};
+        if ((e = specular_gi(f, r->scene, *g, spin, r->hdr, r->specular_gi[r->specular_read], r->specular_gi[sw],
+                             r->ctx->tuning, stream)) != RGI_OK)
+            return e;
+        r->specular_read = sw;
+        specular = r->specular_gi[sw];
+    }
+
     // Denoiser (renderer.cpp:857-907): variance into filter target 0;
     // iteration 0 writes the filtered history, iteration 1 target 1, then
     // 1 -> 0, 0 -> 1, 1 -> 0.
@@ -382,17 +402,16 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         }
     }
 
-    // Composite (renderer.cpp:967-990): the final image's weights, no
-    // indirect specular in this build.
+    // Composite (renderer.cpp:967-990): the final image's weights.
     if (s.composite && r->hdr) {
         rgi_composite_inputs ci{};
         ci.gi = r->denoised;
         ci.di_diffuse = di_active ? r->diffuse : nullptr;
         ci.di_specular = di_active ? r->specular : nullptr;
-        ci.specular = nullptr;
+        ci.specular = specular;
         ci.shadow_map = shadow_map;
         for (int k = 0; k < 4; ++k) ci.options[k] = 1.0f;
-        ci.specular_weight = 0.0f;
+        ci.specular_weight = specular ? 1.0f : 0.0f;
         if ((e = composite(f, *g, ci, r->hdr, stream)) != RGI_OK) return e;
     }
 
@@ -484,6 +503,7 @@ int rgi_renderer_outputs_get(const rgi_renderer* r, rgi_renderer_outputs* out)
     out->filtered_history = r->filtered[k];
     out->denoised = r->denoised ? r->denoised : r->color[k];
     out->hdr = r->hdr;
+    out->specular_gi = r->specular_gi[0] ? r->specular_gi[r->specular_read] : nullptr;
     return RGI_OK;
 }
 

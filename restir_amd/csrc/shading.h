@@ -27,7 +27,16 @@ RGI_DEV float smith_g1(float nd, float roughness)
     return nd / maxf_(nd * (1.0f - k) + k, 0.0001f);
 }
 
-// evaluateBrdf (brdf.sh:29-62, with fresnelSchlick): Lambert + GGX with Schlick Fresnel, both
+// fresnelSchlick (brdf.sh:29-39).
+RGI_DEV F3 fresnel_schlick(F3 h, F3 v, F3 f0)
+{
+    float c = clampf_(dot(h, v), 0.0f, 1.0f);
+    float x = 1.0f - c;
+    float w = ((x * x) * (x * x)) * x;
+    return f0 + (splat3(1.0f) - f0) * w;
+}
+
+// evaluateBrdf (brdf.sh:41-66): Lambert + GGX with Schlick Fresnel, both
 // times n.l.
 RGI_DEV void brdf(F3 n, F3 v, F3 l, F3 albedo, float roughness_in, float metallic, F3& diffuse, F3& specular)
 {
@@ -35,10 +44,7 @@ RGI_DEV void brdf(F3 n, F3 v, F3 l, F3 albedo, float roughness_in, float metalli
     float nv = maxf_(dot(n, v), 0.0f);
     F3 h = normalize(v + l);
     F3 f0 = mix3(splat3(0.04f), albedo, metallic);
-    float c = clampf_(dot(h, v), 0.0f, 1.0f);
-    float x = 1.0f - c;
-    float w = ((x * x) * (x * x)) * x;
-    F3 fr = f0 + (splat3(1.0f) - f0) * w;
+    F3 fr = fresnel_schlick(h, v, f0);
     float roughness = maxf_(roughness_in, 0.05f);
     float d = ggx_d(n, h, roughness);
     float g = smith_g1(nl, roughness) * smith_g1(nv, roughness);

@@ -39,6 +39,8 @@ def empty_history(width, height, table_size):
         "color": np.zeros((n, 4), np.uint16),
         "moments": np.zeros((n, 4), np.uint16),
         "filtered": np.zeros((n, 4), np.uint16),
+        "specular_gi": np.zeros((n, 4), np.uint16),
+        "hdr": np.zeros((n, 4), np.uint16),
         "grid": np.zeros(34 * table_size, np.uint32),
         "di_sample": np.zeros((n, 4), np.float32),
         "di_receiver": np.zeros((n, 4), np.float32),
@@ -108,17 +110,20 @@ def oracle_composite(frame, g, out, atlas, di_on):
     """The composite of Renderer::render's final image (renderer.cpp:967-990)."""
     n = g["depth"].shape[0]
     hdr = np.zeros((n, 4), np.uint16)
+    spec = out.get("specular_gi")
     ci = ra.CompositeInputs(out["denoised"].ctypes.data, out["diffuse"].ctypes.data if di_on else None,
-                            out["specular"].ctypes.data if di_on else None, None,
+                            out["specular"].ctypes.data if di_on else None,
+                            spec.ctypes.data if spec is not None else None,
                             atlas.ctypes.data if atlas is not None else None)
     ci.options[:] = [1.0, 1.0, 1.0, 1.0]
-    ci.specular_weight = 0.0
+    ci.specular_weight = 1.0 if spec is not None else 0.0
     orc.lib().orc_composite(ctypes.byref(frame), ctypes.byref(host_gbuffer(g, frame.width, frame.height)),
                             ctypes.byref(ci), orc.ptr(hdr))
     return hdr
 
 
-def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterations=5, composite=False):
+def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterations=5, composite=False,
+                 specular=False):
     """One frame on the oracle; returns this frame's outputs (host arrays)."""
     L = orc.lib()
     w, h, T = frame.width, frame.height, frame.grid_table_size
@@ -169,6 +174,16 @@ def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterati
     out["surface_light"] = surface_light
     if atlas is not None:
         out["shadow_atlas"] = atlas
+    if specular:
+        # Indirect specular (renderer.cpp:836-855), with last frame's composite.
+        spin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
+                           history_color.ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data,
+                           shadow_ptr, grid.ctypes.data)
+        out["specular_gi"] = np.zeros((n, 4), np.uint16)
+        prev_hdr = hist["hdr"] if composite else None
+        L.orc_specular_gi(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(spin),
+                          prev_hdr.ctypes.data if prev_hdr is not None else None, hist["specular_gi"].ctypes.data,
+                          orc.ptr(out["specular_gi"]))
     out["denoised"] = out["color"]
     if denoiser:
         filtered, denoised = oracle_denoise(frame, out, atrous_iterations)
@@ -185,8 +200,9 @@ def next_history(hist, out):
     h = dict(hist)
     for k in ("gi_sample", "gi_receiver", "gi_normal", "color", "moments", "grid"):
         h[k] = out[k]
-    if "filtered" in out:
-        h["filtered"] = out["filtered"]
+    for k in ("filtered", "specular_gi", "hdr"):
+        if k in out:
+            h[k] = out[k]
     if "di_sample" in out:
         for k in ("di_sample", "di_receiver", "di_normal"):
             h[k] = out[k]
@@ -203,7 +219,7 @@ def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0, distance=
                       yaw_step=yaw_step, distance=distance)
         g = oracle_gbuffer(hs, f)
         out = oracle_frame(hs, f, g, hist, denoiser=settings.denoiser, atrous_iterations=settings.atrous_iterations,
-                           composite=settings.composite)
+                           composite=settings.composite, specular=settings.specular_gi)
         seq.append((f, g, hist, out))
         hist = next_history(hist, out)
     return seq

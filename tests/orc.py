@@ -88,6 +88,10 @@ def lib():
     L.orc_reference_accumulate_composite.argtypes = [PG, P, P]
     L.orc_reference_error_composite.restype = None
     L.orc_reference_error_composite.argtypes = [PG, P, P, U32, P]
+    L.orc_specular_gi.restype = None
+    L.orc_specular_gi.argtypes = [PF, PS, PG, ctypes.POINTER(GiInputs), P, P, P]
+    L.orc_specular_sample.restype = None
+    L.orc_specular_sample.argtypes = [P, P, F, F, F, P, P]
     L.orc_expf.restype = F
     L.orc_expf.argtypes = [F]
     L.orc_pow64.restype = F

@@ -219,6 +219,39 @@ def test_svgf(dev, cornell_seq, step):
         assert_same("atrous", dev.get(atr_t, want_a), want_a)
 
 
+@pytest.mark.parametrize("case", ["default", "bvh_rays_sky"])
+def test_specular_gi(dev, cornell_seq, case):
+    """rgi_specular_gi against the oracle, bit-exact, with last frame's
+    composite and specular history."""
+    hs, s, seq = cornell_seq
+    f0, g, hist, out = seq[2]
+    kw = {} if case == "default" else {"ray_mode": ra.RGI_RAY_BVH, "sky": True}
+    f = case_frame(hs, f0, **kw)
+    say("case specular_gi %s" % case)
+    n = W * H
+    sl = out["surface_light"]
+    hc = hist["filtered"] if s.denoiser else hist["color"]
+    gin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data, hc.ctypes.data,
+                      hist["moments"].ctypes.data, sl.ctypes.data, None, out["grid"].ctypes.data)
+    want = np.zeros((n, 4), np.uint16)
+    orc.lib().orc_specular_gi(ctypes.byref(f), ctypes.byref(orc.host_scene(hs)),
+                              ctypes.byref(common.host_gbuffer(g, W, H)), ctypes.byref(gin),
+                              hist["hdr"].ctypes.data, hist["specular_gi"].ctypes.data, orc.ptr(want))
+    dscene = dev.scene(hs)
+    gb, keep = dev.gbuffer(g, W, H)
+    d = {k: dev.put(v) for k, v in (("rec", hist["gi_receiver"]), ("nor", hist["gi_normal"]), ("col", hc),
+                                     ("mom", hist["moments"]), ("sl", sl), ("grid", out["grid"]), ("hdr", hist["hdr"]),
+                                     ("spec", hist["specular_gi"]))}
+    dgin = ra.GiInputs(None, d["rec"].data_ptr(), d["nor"].data_ptr(), d["col"].data_ptr(), d["mom"].data_ptr(),
+                       d["sl"].data_ptr(), None, d["grid"].data_ptr())
+    o = dev.empty(n * 8)
+    ra.check(dev.L.rgi_specular_gi(dev.ctx, ctypes.byref(f), ctypes.byref(dscene), ctypes.byref(gb), ctypes.byref(dgin),
+                                   d["hdr"].data_ptr(), d["spec"].data_ptr(), o.data_ptr(), dev.stream))
+    got = dev.get(o, want)
+    say("  nonzero %.3f" % (want.view(np.float16)[:, :3] != 0).any(1).mean())
+    assert_same("specular", got, want)
+
+
 def test_composite(dev):
     """rgi_composite against the oracle, bit-exact: main light with the 4 x 4
     soft shadow compare of a real atlas, DI planes, denoised GI and the sky."""

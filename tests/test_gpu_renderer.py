@@ -96,6 +96,8 @@ def test_renderer_matches_oracle_sequence(torch, frames, kw, variants, scene):
                 checks += [("filtered", out.filtered_history), ("denoised", out.denoised)]
             if s.composite:
                 checks += [("hdr", out.hdr)]
+            if s.specular_gi:
+                checks += [("specular_gi", out.specular_gi)]
             for name, ptr in checks:
                 got = download(torch, ptr, want[name])
                 if name == "grid":

@@ -575,6 +575,66 @@ def test_composite_closed_forms():
         assert np.all(hdr[sky, 3] == 1.0)
 
 
+# ----------------------------------------------------------- specular GI
+
+def specular_samples(n, v, roughness, count, seed):
+    rng = np.random.default_rng(seed)
+    r = rng.random((count, 2)).astype(np.float32)
+    n3, v3 = np.asarray(n, np.float32), np.asarray(v, np.float32)
+    dirs = np.zeros((count, 3), np.float32)
+    g1 = np.zeros(count, np.float32)
+    out_d, out_g = np.zeros(3, np.float32), np.zeros(1, np.float32)
+    for i in range(count):
+        L.orc_specular_sample(orc.ptr(n3), orc.ptr(v3), roughness, float(r[i, 0]), float(r[i, 1]), orc.ptr(out_d),
+                              orc.ptr(out_g))
+        dirs[i], g1[i] = out_d, out_g[0]
+    return dirs, g1
+
+
+def smith_g1(c, a):
+    return 2 * c / (c + np.sqrt(a * a + (1 - a * a) * c * c))
+
+
+def test_specular_vndf_sampling():
+    """fs_specular.sc:139-170: the visible-normal sample (Heitz 2018) and its
+    weight G1(l). (1) Near a mirror the reflected direction is the mirror
+    direction. (2) E[G1(l) [n.l > 0]] over the sampler equals the integral of
+    the VNDF D_v(h) = G1(v) max(v.h, 0) D(h) / (n.v) times G1(l(h)) over the
+    hemisphere of h, computed by quadrature in double."""
+    n = np.array([0.0, 0.0, 1.0])
+    for theta in (0.3, 1.1):
+        v = np.array([np.sin(theta), 0.0, np.cos(theta)])
+        d, _ = specular_samples(n, v, 0.001, 16, 1)
+        mirror = np.array([-v[0], -v[1], v[2]])
+        assert np.all(np.abs(d - mirror) < 1e-4)
+    # Tilted normal: the tangent frame of a non-axis normal.
+    nt = np.array([0.3, 0.8, 0.52])
+    nt /= np.linalg.norm(nt)
+    vt = np.array([0.0, 1.0, 0.0])
+    d, _ = specular_samples(nt, vt, 0.001, 8, 2)
+    mirror = 2 * np.dot(nt, vt) * nt - vt
+    assert np.all(np.abs(d - mirror) < 1e-4)
+
+    for roughness, theta in ((0.5, 0.6), (0.8, 1.2)):
+        a = roughness * roughness
+        v = np.array([np.sin(theta), 0.0, np.cos(theta)])
+        _, g1 = specular_samples(n, v, roughness, 60000, 3)
+        th, ph = np.meshgrid((np.arange(800) + 0.5) * (np.pi / 2) / 800, (np.arange(800) + 0.5) * 2 * np.pi / 800,
+                             indexing="ij")
+        h = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+        nh = h[..., 2]
+        D = a * a / (np.pi * (nh * nh * (a * a - 1) + 1) ** 2)
+        vh = h @ v
+        l = 2 * vh[..., None] * h - v
+        nl = l[..., 2]
+        nv = v[2]
+        integrand = smith_g1(nv, a) * np.maximum(vh, 0) * D / nv * np.where(nl > 0, smith_g1(np.maximum(nl, 1e-9), a), 0)
+        want = (integrand * np.sin(th)).sum() * (np.pi / 2 / 800) * (2 * np.pi / 800)
+        got = float(g1.mean())
+        err = 4 * g1.std() / np.sqrt(len(g1)) + 2e-3
+        assert abs(got - want) <= err, (roughness, theta, got, want)
+
+
 # ---------------------------------------------------------- shadow atlas
 
 def shadow_face(d):
