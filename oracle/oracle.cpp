@@ -719,9 +719,14 @@ float depth_behind_surface(const Pass& p, const ClipRay& ray, float t, V2& uv, V
     if (!on_screen) return -1.0f;
     float depth = fetch_depth(p, uv);
     if (depth >= kSkyDepth) return -1.0f;
-    surface_position = reconstruct_world_position(p.f, uv, depth);
-    float behind = ray_depth - mul(p.f->view_proj, V4{surface_position.x, surface_position.y, surface_position.z, 1.0f}).w;
+    // The surface's view depth, (view_proj * reconstructWorldPosition(uv,
+    // depth)).w, is 1 / (c z + d) for any uv, with (0, 0, c, d) the w row of
+    // the inverse view-projection of a perspective camera. The position is
+    // needed only when the ray is behind the surface.
+    const float* inv = p.f->inv_view_proj;
+    float behind = ray_depth - 1.0f / fmaf(inv[11], to_clip_depth(p.f, depth), inv[15]);
     if (behind > 0.0f) {
+        surface_position = reconstruct_world_position(p.f, uv, depth);
         V3 sn = decode_normal(p, uv);
         V3 ray_position = ray.origin + ray.direction * t;
         if (dot(ray_position - surface_position, sn) > -(0.01f * scene_scale(p))) return -1.0f;

@@ -166,9 +166,16 @@ RGI_DEV float depth_behind(const GiPass& p, const ClipRay& m, float t, F3& surfa
     uint32_t ti = tix(p, uv);
     float depth = p.g.depth[ti];
     if (depth >= kSkyDepth) return -1.0f;
-    surface = reconstruct_position(*p.f, uv, depth);
-    float behind = cw - mul_w(p.f->view_proj, surface);
+    // The view depth of the surface behind the texel. For a perspective
+    // camera the w row of the inverse view-projection is (0, 0, c, d), so
+    // the point reconstructed at any uv with this depth has view depth
+    // 1 / (c z + d): reconstructWorldPosition then the view-projection's w,
+    // without the matrix products. The position itself is needed only when
+    // the ray is behind the surface.
+    const float* inv = p.f->inv_view_proj;
+    float behind = cw - 1.0f / fmaf(inv[11], to_clip_depth(*p.f, depth), inv[15]);
     if (behind > 0.0f) {
+        surface = reconstruct_position(*p.f, uv, depth);
         F4 nr = unorm8(p.g.normal[ti]);
         F3 sn = normalize(F3{nr.x * 2.0f - 1.0f, nr.y * 2.0f - 1.0f, nr.z * 2.0f - 1.0f});
         F3 ray_position = m.origin + m.dir * t;

@@ -345,6 +345,32 @@ def test_trace_against_brute_force(kind, size):
         assert agree >= n - 3, (any_hit, n - agree)
 
 
+def test_surface_view_depth_is_independent_of_uv():
+    """depthBehindSurface (restir.sh:327-358) compares the ray's view depth
+    with that of reconstructWorldPosition(uv, depth) re-projected. The march
+    restates the latter as 1 / (c z + d), (0, 0, c, d) being the w row of the
+    inverse view-projection: for a perspective camera the re-projected w does
+    not depend on uv. Checked in double for random uv and depth on the
+    bench's cameras, both depth conventions, to 1e-5: the float matrices are
+    each other's inverses only to float precision."""
+    hs = common.cornell(extra_lights=0)
+    rng = np.random.default_rng(12)
+    for hd in (False, True):
+        for frame in range(4):
+            eye = ra.orbit_camera(hs, frame, 1920, 1080, yaw_of=ra.swing_yaw, homogeneous_depth=hd)
+            vp = np.array(eye[1], np.float64).reshape(4, 4).T
+            inv = np.array(eye[2], np.float64).reshape(4, 4).T
+            uv = rng.random((2000, 2))
+            z = rng.uniform(0.2, 0.9999, 2000)
+            zc = z * 2 - 1 if hd else z
+            clip = np.stack([uv[:, 0] * 2 - 1, -(uv[:, 1] * 2 - 1), zc, np.ones(2000)], 1)
+            wp = clip @ inv.T
+            world = wp[:, :3] / wp[:, 3:4]
+            w_ref = np.concatenate([world, np.ones((2000, 1))], 1) @ vp[3]
+            w_fast = 1.0 / (inv[3, 2] * zc + inv[3, 3])
+            assert np.allclose(w_fast, w_ref, rtol=1e-5), (hd, frame, np.abs(w_fast / w_ref - 1).max())
+
+
 # -------------------------------------------------------------- denoiser
 
 def test_expf_against_libm():
