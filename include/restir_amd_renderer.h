@@ -110,6 +110,14 @@ int rgi_renderer_outputs_get(const rgi_renderer* renderer, rgi_renderer_outputs*
 /* rgi_context_set_option on the renderer's context (launch variants). */
 int rgi_renderer_set_option(rgi_renderer* renderer, int option, int value);
 
+/* rgi_renderer_render as one HIP graph launch: the frame's launches are
+ * captured from `stream` (not the null stream), the renderer's executable
+ * graph is updated in place with them (instantiated when the set of passes
+ * changed), then launched on `stream`. The first call renders eagerly, so
+ * that scratch buffers and the shadow atlas are in their final state before
+ * any capture. Results are those of rgi_renderer_render. */
+int rgi_renderer_render_graph(rgi_renderer* renderer, const rgi_view* view, const rgi_gbuffer* gbuffer, void* stream);
+
 /* The autotuner: renders `frames` frames per launch variant (cycling through
  * `views` and their G-buffers), times each variant with HIP events on
  * `stream`, and keeps the fastest. Variants are bit-exact with each other,

@@ -39,6 +39,9 @@ struct rgi_renderer {
     uint16_t* black = nullptr; // m_blackTexture: DI light when ReSTIR DI is selected but has no lights
     uint16_t* specular_gi[2] = {}; // m_specularTextures, RGBA16F, own parity m_specularRead
     int specular_read = 0;
+    // The graph form of a frame (rgi_renderer_render_graph).
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_warm = false;
     int read = 0;              // m_historyRead / m_diRead: the parity last frame wrote
     bool history_valid = false;
     bool di_valid = false;
@@ -283,6 +286,7 @@ void rgi_renderer_destroy(rgi_renderer* r)
 {
     if (!r) return;
     hipDeviceSynchronize();
+    if (r->graph_exec) hipGraphExecDestroy(r->graph_exec);
     free_all(r);
     rgi_destroy(r->ctx);
     delete r;
@@ -422,6 +426,48 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     r->history_valid = true;
     ++r->frame_number;
     r->last = f;
+    return RGI_OK;
+}
+
+int rgi_renderer_render_graph(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
+{
+    if (!r) return fail(RGI_E_INVALID, "renderer is NULL");
+    if (!stream_ptr) return fail(RGI_E_INVALID, "render_graph: capture needs a stream other than the null stream");
+    // The first frame runs eagerly: scratch buffers and the shadow atlas
+    // reach their final state outside capture.
+    if (!r->graph_warm) {
+        int e = rgi_renderer_render(r, view, g, stream_ptr);
+        if (e == RGI_OK) r->graph_warm = true;
+        return e;
+    }
+    hipStream_t stream = as_stream(stream_ptr);
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(RGI_E_DEVICE, "render_graph: stream capture could not begin");
+    int e = rgi_renderer_render(r, view, g, stream_ptr);
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(stream, &graph);
+    if (e != RGI_OK || ce != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return e != RGI_OK ? e : fail(RGI_E_DEVICE, "render_graph: capture failed: %s", hipGetErrorString(ce));
+    }
+    // Same launches as last frame with new arguments: update in place;
+    // a changed topology (another pass set) instantiates afresh.
+    if (r->graph_exec) {
+        hipGraphNode_t error_node = nullptr;
+        hipGraphExecUpdateResult result;
+        if (hipGraphExecUpdate(r->graph_exec, graph, &error_node, &result) != hipSuccess) {
+            hipGetLastError();
+            hipGraphExecDestroy(r->graph_exec);
+            r->graph_exec = nullptr;
+        }
+    }
+    if (!r->graph_exec && hipGraphInstantiate(&r->graph_exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        r->graph_exec = nullptr;
+        hipGraphDestroy(graph);
+        return fail(RGI_E_DEVICE, "render_graph: instantiation failed");
+    }
+    hipGraphDestroy(graph);
+    if (hipGraphLaunch(r->graph_exec, stream) != hipSuccess) return fail(RGI_E_DEVICE, "render_graph: launch failed");
     return RGI_OK;
 }
 

@@ -170,3 +170,59 @@ def test_reference_accumulate_and_error(torch):
     orc.lib().orc_reference_error_composite(ctypes.byref(gh), orc.ptr(out["hdr"]), orc.ptr(want_acc), 3,
                                             orc.ptr(want))
     assert np.array_equal(got, want), (got, want)
+
+
+def test_render_graph_matches_eager(torch):
+    """rgi_renderer_render_graph (the frame as one updated-and-replayed HIP
+    graph) against rgi_renderer_render: every output plane bit-identical on
+    every frame of a camera sequence, with eager work and synchronisations
+    between the replays (the first graph frame renders eagerly, the rest are
+    captured)."""
+    say("case renderer graph vs eager")
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC, shadows=True, light_color=(1.0, 0.9, 0.8),
+                    light_position=(0.1, 1.2, -0.2), shadow_face_size=0)
+    L = ra.lib()
+    bvh, lights = upload(torch, hs.bvh), upload(torch, hs.lights)
+    scene = hs.rgi_scene(bvh.data_ptr(), lights.data_ptr())
+    rs = []
+    for _ in range(2):
+        r = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(r), W, H, ctypes.byref(ra.c_settings(s)), ctypes.byref(scene),
+                                       hs.radius))
+        rs.append(r)
+    side = torch.cuda.Stream()
+    eager_stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    graph_stream = ctypes.c_void_p(side.cuda_stream)
+    yaw = 2 * np.pi / 64
+    keep = []
+    try:
+        for i in range(5):
+            eye, vp, inv = ra.orbit_camera(hs, i, W, H, yaw_step=yaw)
+            f = common.frame_for(hs, s, W, H, i, i > 0, yaw_step=yaw)
+            g = common.oracle_gbuffer(hs, f)
+            planes = {k: upload(torch, v) for k, v in g.items()}
+            keep.append(planes)
+            gb = ra.GBuffer(W, H, planes["depth"].data_ptr(), planes["albedo"].data_ptr(), planes["normal"].data_ptr(),
+                            planes["shading"].data_ptr(), planes["emission"].data_ptr())
+            view = ra.make_view(eye, vp, inv)
+            torch.cuda.synchronize()
+            ra.check(L.rgi_renderer_render(rs[0], ctypes.byref(view), ctypes.byref(gb), eager_stream))
+            ra.check(L.rgi_renderer_render_graph(rs[1], ctypes.byref(view), ctypes.byref(gb), graph_stream))
+            torch.cuda.synchronize()
+            scratch = torch.ones(1 << 20, device="cuda") * 2.0  # eager work between replays
+            assert float(scratch.sum().item()) == float(1 << 21)
+            outs = [ra.RendererOutputs(), ra.RendererOutputs()]
+            for r, o in zip(rs, outs):
+                ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(o)))
+            n = W * H
+            for name, like in (("gi_sample", np.zeros((n, 4), np.float32)), ("gi_receiver", np.zeros((n, 4), np.float32)),
+                               ("color", np.zeros((n, 4), np.uint16)), ("moments", np.zeros((n, 4), np.uint16)),
+                               ("di_diffuse", np.zeros((n, 4), np.uint16)), ("denoised", np.zeros((n, 4), np.uint16)),
+                               ("hdr", np.zeros((n, 4), np.uint16)), ("specular_gi", np.zeros((n, 4), np.uint16))):
+                a = download(torch, getattr(outs[0], name), like)
+                b = download(torch, getattr(outs[1], name), like)
+                assert np.array_equal(a, b), "frame %d: %s differs between graph and eager" % (i, name)
+    finally:
+        for r in rs:
+            L.rgi_renderer_destroy(r)
