@@ -249,6 +249,80 @@ def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames,
             "drv": drv, "rend": rend, "gbufs": gbufs, "views": views}
 
 
+def run_uneven_batch(mode, steps, warmup):
+    """Scenes of uneven size and resolution as one batch frame: every
+    renderer rendered eagerly one after another on one stream, then all of
+    them as one graph (rgi_batch_render), each on a stream of its own."""
+    import torch
+    L = ra.lib()
+    cases = [("cornell", ra.RGI_SCENE_CORNELL, 0, 640, 360, 3.17), ("boxfield_722", ra.RGI_SCENE_BOXFIELD, 60, 1280, 720, 3.17),
+             ("atrium_256k", ra.RGI_SCENE_ATRIUM, 262144, 1920, 1080, 0.7),
+             ("boxfield_12k", ra.RGI_SCENE_BOXFIELD, 1000, 960, 540, 3.17),
+             ("cornell", ra.RGI_SCENE_CORNELL, 0, 2560, 1440, 3.17),
+             ("boxfield_120k", ra.RGI_SCENE_BOXFIELD, 10000, 800, 450, 3.17)]
+    ctx = ctypes.c_void_p()
+    ra.check(L.rgi_create(ctypes.byref(ctx)))
+    es = driver.stream()
+    rends, frames, keep = [], [], []
+    for name, kind, size, w, h, dist in cases:
+        hs = ra.HostScene(kind, seed=1, size=size, extra_lights=4, light_seed=1)
+        table = 1 << 20 if w * h <= (1 << 21) else 1 << 21
+        st = ra.Settings(grid_table_size=table, grid_mode=mode, denoiser=False, composite=False, specular_gi=False)
+        gscene = driver.GpuScene(hs)
+        views, gbs = [], []
+        for i in range(4):
+            eye, vp, inv = ra.orbit_camera(hs, i, w, h, yaw_of=ra.swing_yaw, distance=dist)
+            f = ra.make_frame(st, hs, i, eye, vp, inv, eye, vp, w, h, False)
+            gbs.append(driver.GBufferSet(ctx, gscene, f))
+            views.append(ra.make_view(eye, vp, inv))
+        r = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(r), w, h, ctypes.byref(ra.c_settings(st)), ctypes.byref(gscene.scene),
+                                       hs.radius))
+        rends.append(r)
+        frames.append((views, gbs))
+        keep.append(gscene)
+    torch.cuda.synchronize()
+
+    def eager(k):
+        for r, (views, gbs) in zip(rends, frames):
+            ra.check(L.rgi_renderer_render(r, ctypes.byref(views[k % 4]), ctypes.byref(gbs[k % 4].gb), es))
+
+    for k in range(warmup):
+        eager(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        eager(k)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+
+    batch = ctypes.c_void_p()
+    arr = (ctypes.c_void_p * len(rends))(*[r.value for r in rends])
+    ra.check(L.rgi_batch_create(ctypes.byref(batch), arr, len(rends)))
+    side = torch.cuda.Stream()
+    bs = ctypes.c_void_p(side.cuda_stream)
+
+    def batched(k):
+        va = (ra.View * len(rends))(*[v[k % 4] for v, _ in frames])
+        ga = (ra.GBuffer * len(rends))(*[g[k % 4].gb for _, g in frames])
+        ra.check(L.rgi_batch_render(batch, va, ga, bs))
+
+    for k in range(warmup):
+        batched(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        batched(k)
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / steps
+    L.rgi_batch_destroy(batch)
+    for r in rends:
+        L.rgi_renderer_destroy(r)
+    L.rgi_destroy(ctx)
+    return {"scenes": ["%s %dx%d" % (c[0], c[3], c[4]) for c in cases], "eager_ms_per_batch_frame": round(eager_ms, 5),
+            "graph_batch_ms_per_batch_frame": round(batch_ms, 5), "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +340,7 @@ def main():
     ap.add_argument("--tune-frames", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true", help="run one 256 MiB device copy first")
     ap.add_argument("--atrium-steps", type=int, default=32, help="timed frames of the atrium workload (0: skip)")
+    ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the uneven batch workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -440,6 +515,8 @@ def main():
         L.rgi_renderer_destroy(arun["rend"])
         arun["drv"].close()
 
+    uneven = run_uneven_batch(mode, 16, 8) if args.batch else None
+
     line = {
         "metric": "GI frame ms @1080p (grid + resample + trace)",
         "value": round(ms_per_frame, 5),
@@ -469,6 +546,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "atrium_1080p": atrium,
+        "uneven_batch": uneven,
     }
     print(json.dumps(line), flush=True)
 

@@ -118,6 +118,19 @@ int rgi_renderer_set_option(rgi_renderer* renderer, int option, int value);
  * any capture. Results are those of rgi_renderer_render. */
 int rgi_renderer_render_graph(rgi_renderer* renderer, const rgi_view* view, const rgi_gbuffer* gbuffer, void* stream);
 
+/* A batch of renderers (any sizes and scenes) rendered together: one HIP
+ * graph per call holds every renderer's frame, each on a stream of its own
+ * forked from and joined to `stream` (not the null stream), so small views
+ * overlap on the GPU. views[i] and gbuffers[i] go to renderers[i]. The first
+ * call renders eagerly, one renderer after another; later calls capture,
+ * update the batch's executable graph in place and launch it. Results are
+ * those of rgi_renderer_render on each renderer. The renderers stay owned by
+ * the caller and must outlive the batch. */
+typedef struct rgi_batch rgi_batch;
+int rgi_batch_create(rgi_batch** out, rgi_renderer* const* renderers, uint32_t count);
+int rgi_batch_render(rgi_batch* batch, const rgi_view* views, const rgi_gbuffer* gbuffers, void* stream);
+void rgi_batch_destroy(rgi_batch* batch);
+
 /* The autotuner: renders `frames` frames per launch variant (cycling through
  * `views` and their G-buffers), times each variant with HIP events on
  * `stream`, and keeps the fastest. Variants are bit-exact with each other,

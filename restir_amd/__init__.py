@@ -191,6 +191,10 @@ def lib():
     L.rgi_renderer_destroy.argtypes = [P]
     L.rgi_renderer_render.argtypes = [P, ctypes.POINTER(View), ctypes.POINTER(GBuffer), P]
     L.rgi_renderer_render_graph.argtypes = [P, ctypes.POINTER(View), ctypes.POINTER(GBuffer), P]
+    L.rgi_batch_create.argtypes = [ctypes.POINTER(P), P, U32]
+    L.rgi_batch_render.argtypes = [P, P, P, P]
+    L.rgi_batch_destroy.argtypes = [P]
+    L.rgi_batch_destroy.restype = None
     L.rgi_renderer_set_reference_mode.argtypes = [P, ctypes.c_int]
     L.rgi_renderer_invalidate_history.restype = None
     L.rgi_renderer_invalidate_history.argtypes = [P]

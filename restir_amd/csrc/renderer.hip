@@ -9,6 +9,8 @@
 #include "../../include/restir_amd_renderer.h"
 #include "internal.h"
 
+#include <vector>
+
 struct rgi_renderer {
     int32_t width = 0, height = 0;
     rgi_settings settings{};
@@ -558,6 +560,103 @@ int rgi_renderer_last_frame(const rgi_renderer* r, rgi_frame* out)
     if (!r || !out) return fail(RGI_E_INVALID, "renderer_last_frame: NULL argument");
     if (!r->have_frame) return fail(RGI_E_INVALID, "no frame rendered yet");
     *out = r->last;
+    return RGI_OK;
+}
+
+// ------------------------------------------------------------------ batch
+
+// Renderers of any sizes and scenes whose frames go out together as one
+// graph: each renderer's frame is captured on a stream of its own, forked
+// from and joined back to the caller's stream, so small views overlap on the
+// GPU instead of running one after another.
+struct rgi_batch {
+    std::vector<rgi_renderer*> renderers;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> joined;
+    hipEvent_t fork = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool warm = false;
+};
+
+int rgi_batch_create(rgi_batch** out, rgi_renderer* const* renderers, uint32_t count)
+{
+    if (!out || !renderers || count == 0) return fail(RGI_E_INVALID, "batch_create: renderers are required");
+    rgi_batch* b = new (std::nothrow) rgi_batch();
+    if (!b) return fail(RGI_E_NOMEM, "batch allocation failed");
+    b->renderers.assign(renderers, renderers + count);
+    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (uint32_t i = 0; i < count && ok; ++i) {
+        if (!renderers[i]) ok = false;
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+        if (st) b->streams.push_back(st);
+        ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+        if (ev) b->joined.push_back(ev);
+    }
+    if (!ok) {
+        rgi_batch_destroy(b);
+        return fail(RGI_E_DEVICE, "batch_create: streams, events or renderers missing");
+    }
+    *out = b;
+    return RGI_OK;
+}
+
+void rgi_batch_destroy(rgi_batch* b)
+{
+    if (!b) return;
+    hipDeviceSynchronize();
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    for (hipStream_t st : b->streams) hipStreamDestroy(st);
+    for (hipEvent_t ev : b->joined) hipEventDestroy(ev);
+    if (b->fork) hipEventDestroy(b->fork);
+    delete b;
+}
+
+int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffers, void* stream_ptr)
+{
+    if (!b || !views || !gbuffers) return fail(RGI_E_INVALID, "batch_render: NULL argument");
+    if (!stream_ptr) return fail(RGI_E_INVALID, "batch_render: capture needs a stream other than the null stream");
+    hipStream_t stream = as_stream(stream_ptr);
+    size_t n = b->renderers.size();
+    int e = RGI_OK;
+    if (!b->warm) {
+        // First frame eagerly, on the caller's stream (allocations happen here).
+        for (size_t i = 0; i < n && e == RGI_OK; ++i) e = rgi_renderer_render(b->renderers[i], &views[i], &gbuffers[i], stream);
+        if (e == RGI_OK) b->warm = true;
+        return e;
+    }
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return fail(RGI_E_DEVICE, "batch_render: stream capture could not begin");
+    hipEventRecord(b->fork, stream);
+    for (size_t i = 0; i < n && e == RGI_OK; ++i) {
+        hipStreamWaitEvent(b->streams[i], b->fork, 0);
+        e = rgi_renderer_render(b->renderers[i], &views[i], &gbuffers[i], b->streams[i]);
+        hipEventRecord(b->joined[i], b->streams[i]);
+        hipStreamWaitEvent(stream, b->joined[i], 0);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ce = hipStreamEndCapture(stream, &graph);
+    if (e != RGI_OK || ce != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return e != RGI_OK ? e : fail(RGI_E_DEVICE, "batch_render: capture failed: %s", hipGetErrorString(ce));
+    }
+    if (b->exec) {
+        hipGraphNode_t error_node = nullptr;
+        hipGraphExecUpdateResult result;
+        if (hipGraphExecUpdate(b->exec, graph, &error_node, &result) != hipSuccess) {
+            hipGetLastError();
+            hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+        }
+    }
+    if (!b->exec && hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        b->exec = nullptr;
+        hipGraphDestroy(graph);
+        return fail(RGI_E_DEVICE, "batch_render: instantiation failed");
+    }
+    hipGraphDestroy(graph);
+    if (hipGraphLaunch(b->exec, stream) != hipSuccess) return fail(RGI_E_DEVICE, "batch_render: launch failed");
     return RGI_OK;
 }
 

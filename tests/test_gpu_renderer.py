@@ -226,3 +226,67 @@ def test_render_graph_matches_eager(torch):
     finally:
         for r in rs:
             L.rgi_renderer_destroy(r)
+
+
+def test_batch_matches_single_renderers(torch):
+    """rgi_batch_render (renderers of different scenes and sizes in one HIP
+    graph, each on its own stream) against each renderer rendered alone:
+    bit-identical outputs on every frame."""
+    say("case batch vs single renderers")
+    L = ra.lib()
+    cases = [(common.cornell(extra_lights=4), 96, 64),
+             (ra.HostScene(ra.RGI_SCENE_BOXFIELD, seed=3, size=60, extra_lights=4), 80, 48),
+             (atrium(), 64, 64)]
+    s = ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC)
+    keep, batch_r, single_r, scenes = [], [], [], []
+    side = torch.cuda.Stream()
+    bs = ctypes.c_void_p(side.cuda_stream)
+    es = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    batch = ctypes.c_void_p()
+    try:
+        for hs, w, h in cases:
+            bvh, lights = upload(torch, hs.bvh), upload(torch, hs.lights)
+            keep += [bvh, lights]
+            scene = hs.rgi_scene(bvh.data_ptr(), lights.data_ptr())
+            scenes.append(scene)
+            for lst in (batch_r, single_r):
+                r = ctypes.c_void_p()
+                ra.check(L.rgi_renderer_create(ctypes.byref(r), w, h, ctypes.byref(ra.c_settings(s)),
+                                               ctypes.byref(scene), hs.radius))
+                lst.append(r)
+        arr = (ctypes.c_void_p * len(batch_r))(*[r.value for r in batch_r])
+        ra.check(L.rgi_batch_create(ctypes.byref(batch), arr, len(batch_r)))
+        yaw = 2 * np.pi / 64
+        for i in range(4):
+            views, gbs = [], []
+            for hs, w, h in cases:
+                eye, vp, inv = ra.orbit_camera(hs, i, w, h, yaw_step=yaw)
+                f = common.frame_for(hs, s, w, h, i, i > 0, yaw_step=yaw)
+                g = common.oracle_gbuffer(hs, f)
+                planes = {k: upload(torch, v) for k, v in g.items()}
+                keep.append(planes)
+                gbs.append(ra.GBuffer(w, h, planes["depth"].data_ptr(), planes["albedo"].data_ptr(),
+                                      planes["normal"].data_ptr(), planes["shading"].data_ptr(),
+                                      planes["emission"].data_ptr()))
+                views.append(ra.make_view(eye, vp, inv))
+            torch.cuda.synchronize()
+            for r, v, gb in zip(single_r, views, gbs):
+                ra.check(L.rgi_renderer_render(r, ctypes.byref(v), ctypes.byref(gb), es))
+            va = (ra.View * len(views))(*views)
+            ga = (ra.GBuffer * len(gbs))(*gbs)
+            ra.check(L.rgi_batch_render(batch, va, ga, bs))
+            torch.cuda.synchronize()
+            for k, ((hs, w, h), rb, rsg) in enumerate(zip(cases, batch_r, single_r)):
+                ob, osg = ra.RendererOutputs(), ra.RendererOutputs()
+                ra.check(L.rgi_renderer_outputs_get(rb, ctypes.byref(ob)))
+                ra.check(L.rgi_renderer_outputs_get(rsg, ctypes.byref(osg)))
+                for name in ("gi_sample", "color", "hdr", "specular_gi", "denoised"):
+                    like = np.zeros((w * h, 4), np.float32 if name == "gi_sample" else np.uint16)
+                    a = download(torch, getattr(ob, name), like)
+                    b = download(torch, getattr(osg, name), like)
+                    assert np.array_equal(a, b), "frame %d scene %d: %s differs" % (i, k, name)
+    finally:
+        if batch:
+            L.rgi_batch_destroy(batch)
+        for r in batch_r + single_r:
+            L.rgi_renderer_destroy(r)
