@@ -1,0 +1,125 @@
+/*
+ * restir_amd neural radiance cache — C ABI.
+ *
+ * The reference has no counterpart: this is the learned extension of the GI
+ * path. A small MLP over a multiresolution hash encoding of position (Mueller
+ * et al. 2022, "Instant Neural Graphics Primitives") predicts the radiance
+ * that a diffuse surface point reflects, trained online on the surface points
+ * that ReSTIR GI and DI shade each frame (Mueller et al. 2021, "Real-time
+ * Neural Radiance Caching for Path Tracing": relative L2 loss, Adam). The
+ * frame driver queries it at every visible surface and hands the result to
+ * the GI passes in place of the reference's surface light plus last frame's
+ * indirect light (restir.sh:265-298, evaluateSample 445-530), so it stands in
+ * for the rest of the path and becomes part of the GI target function.
+ *
+ * Model (fixed shape):
+ *   encoding  16 levels x 2 features, 2^log2_table entries per level, level l
+ *             with res_l = floor(base * b^l) cells per axis, b chosen so that
+ *             res_15 = max_resolution. A level whose (res+1)^3 corners fit the
+ *             table is indexed densely, x + y (res+1) + z (res+1)^2; finer
+ *             levels hash the corner, (x * 1 ^ y * 2654435761 ^ z * 805459861)
+ *             mod 2^log2_table (Teschner et al. 2003). Trilinear weights.
+ *   input     64 features: the 32 encoded values, normal * 0.5 + 0.5 (3),
+ *             albedo (3), a constant 1, zeros.
+ *   MLP       5 hidden layers of 64 with ReLU, a linear output of 3, no
+ *             biases (the constant input feeds the first layer's).
+ *   loss      relative L2 per sample, sum_c (y_c - t_c)^2 / (sg(mean_c y)^2
+ *             + 0.01), times the sample's weight, averaged over the batch.
+ *   optimiser Adam with bias correction on FP32 master weights; a table entry
+ *             whose gradient is exactly zero keeps its value and moments.
+ *
+ * Precision: training runs the MLP in BF16 on the matrix cores (products of
+ * BF16 operands, FP32 accumulation); activations and back-propagated
+ * gradients are rounded to BF16 between layers, table values to FP16. The
+ * table gradient is accumulated with FP32 atomics, so its last bits depend on
+ * the order the GPU runs them. Inference runs in BF16, FP16 or FP8 (OCP E4M3,
+ * per-layer scales for weights and activations).
+ *
+ * Buffers are caller-owned device pointers; calls are stream ordered.
+ * Records are 12 floats per point: position xyz, 0, unit normal xyz, 0,
+ * albedo rgb, 0. Targets are 4 floats: radiance rgb and a weight (0 drops the
+ * sample, 1 keeps it). Outputs are 4 floats: radiance rgb and 0.
+ */
+#ifndef RESTIR_AMD_NRC_H
+#define RESTIR_AMD_NRC_H
+
+#include <stdint.h>
+
+#include "restir_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RGI_NRC_LEVELS 16
+#define RGI_NRC_FEATURES 2
+#define RGI_NRC_WIDTH 64
+#define RGI_NRC_HIDDEN 5
+#define RGI_NRC_OUTPUTS 3
+/* Logical MLP parameters: W1..W5 (64 x 64, [output][input]) then W6 (3 x 64). */
+#define RGI_NRC_MLP_PARAMS (5 * 64 * 64 + 3 * 64)
+#define RGI_NRC_RECORD_FLOATS 12
+#define RGI_NRC_TARGET_FLOATS 4
+
+/* Inference precision. */
+#define RGI_NRC_BF16 0
+#define RGI_NRC_FP16 1
+#define RGI_NRC_FP8 2
+
+typedef struct rgi_nrc rgi_nrc;
+
+typedef struct rgi_nrc_config {
+    uint32_t log2_table;      /* 19: entries per level = 2^log2_table (10..22) */
+    uint32_t base_resolution; /* 16: cells per axis of level 0 */
+    uint32_t max_resolution;  /* 512: cells per axis of level 15 */
+    uint32_t seed;            /* 1: initialisation of the weights and the table */
+    float learning_rate;      /* 0.005 */
+    float beta1;              /* 0.9 */
+    float beta2;              /* 0.99 */
+    float epsilon;            /* 1e-8 */
+    float aabb_min[3];        /* positions map linearly onto [0, 1]^3, clamped */
+    float aabb_max[3];
+    int32_t reserved[4];
+} rgi_nrc_config;
+
+void rgi_nrc_config_default(rgi_nrc_config* config);
+
+/* Cells per axis of `level` for `config` (the host computes them in double). */
+int rgi_nrc_level_resolution(const rgi_nrc_config* config, uint32_t level, uint32_t* resolution);
+
+/* Allocates the parameters (FP32 master table and weights, their FP16, BF16
+ * and FP8 copies), the Adam moments and the gradient buffers on the current
+ * device and initialises them from config->seed: weights uniform in
+ * +-sqrt(6 / (fan_in + fan_out)), table values uniform in +-1e-4. */
+int rgi_nrc_create(rgi_nrc** out, const rgi_nrc_config* config);
+void rgi_nrc_destroy(rgi_nrc* nrc);
+
+/* One Adam step on `count` records and targets (any count >= 1). `loss`, if
+ * not NULL, is a device float that receives the batch loss. */
+int rgi_nrc_train(rgi_nrc* nrc, const float* records, const float* targets, uint32_t count, float* loss,
+                  void* stream);
+
+/* Radiance predictions for `count` records into out[4 * count]. */
+int rgi_nrc_infer(rgi_nrc* nrc, const float* records, uint32_t count, int precision, float* out, void* stream);
+
+/* The loss and its gradients without a step: mlp_grad[RGI_NRC_MLP_PARAMS]
+ * (logical layout) and, if not NULL, table_grad[16 * 2^log2_table * 2]
+ * ([level][entry][feature]). Both are device buffers that the call
+ * overwrites. `loss` as for rgi_nrc_train. */
+int rgi_nrc_gradients(rgi_nrc* nrc, const float* records, const float* targets, uint32_t count, float* mlp_grad,
+                      float* table_grad, float* loss, void* stream);
+
+/* The FP32 master parameters: mlp[RGI_NRC_MLP_PARAMS] and table[16 *
+ * 2^log2_table * 2] (device buffers; either may be NULL). Setting them
+ * clears the Adam moments and the step count. */
+int rgi_nrc_get_params(const rgi_nrc* nrc, float* mlp, float* table, void* stream);
+int rgi_nrc_set_params(rgi_nrc* nrc, const float* mlp, const float* table, void* stream);
+
+/* Adam steps taken since creation or the last rgi_nrc_set_params. */
+int rgi_nrc_steps(const rgi_nrc* nrc, uint32_t* steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RESTIR_AMD_NRC_H */

@@ -113,6 +113,22 @@ void orc_gbuffer_primary(const rgi_frame* frame, const rgi_scene* scene, const f
                          uint32_t material_count, float* depth, uint32_t* albedo, uint32_t* normal,
                          uint32_t* shading_normal, uint16_t* emission);
 
+/* The neural radiance cache of include/restir_amd_nrc.h (oracle/nrc.cpp).
+ * res: the 16 level resolutions; lo/hi: the bounds; mlp: the logical weights
+ * (RGI_NRC_MLP_PARAMS floats); table: [16][2^log2_table][2]; records and
+ * targets as in the header. emulate = 1 rounds where the GPU rounds, 0
+ * nowhere. */
+void orc_nrc_forward(const uint32_t* res, uint32_t log2_table, const float* lo, const float* hi, const float* mlp,
+                     const float* table, const float* records, uint32_t count, int emulate, double* out);
+void orc_nrc_gradients(const uint32_t* res, uint32_t log2_table, const float* lo, const float* hi, const float* mlp,
+                       const float* table, const float* records, const float* targets, uint32_t count, int emulate,
+                       double* loss, double* mlp_grad, double* table_grad, double* denominators, int frozen);
+void orc_nrc_adam(float* p, float* m, float* v, const float* g, uint32_t count, float lr, float beta1, float beta2,
+                  float eps, float c1, float c2, int skip_zero);
+void orc_nrc_corner(const uint32_t* res, uint32_t log2_table, int level, const float* p01, int k, uint32_t* idx,
+                    float* w);
+float orc_bf16_round(float x);
+
 #ifdef __cplusplus
 }
 #endif

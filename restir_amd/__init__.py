@@ -125,6 +125,18 @@ class Mesh(ctypes.Structure):
                 ("material_count", U32), ("materials", P)]
 
 
+class NrcConfig(ctypes.Structure):
+    """rgi_nrc_config (include/restir_amd_nrc.h)."""
+    _fields_ = [("log2_table", U32), ("base_resolution", U32), ("max_resolution", U32), ("seed", U32),
+                ("learning_rate", F), ("beta1", F), ("beta2", F), ("epsilon", F),
+                ("aabb_min", F * 3), ("aabb_max", F * 3), ("reserved", I32 * 4)]
+
+
+RGI_NRC_BF16, RGI_NRC_FP16, RGI_NRC_FP8 = 0, 1, 2
+RGI_NRC_MLP_PARAMS = 5 * 64 * 64 + 3 * 64
+RGI_NRC_LEVELS = 16
+
+
 ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs, Shadow, CompositeInputs]
 
 
@@ -208,6 +220,18 @@ def lib():
     L.rgi_reference_error.argtypes = [P, ctypes.POINTER(GBuffer), P, P, P, P, U32, P, P]
     L.rgi_reference_accumulate_composite.argtypes = [ctypes.POINTER(GBuffer), P, P, P]
     L.rgi_reference_error_composite.argtypes = [P, ctypes.POINTER(GBuffer), P, P, U32, P, P]
+    L.rgi_nrc_config_default.restype = None
+    L.rgi_nrc_config_default.argtypes = [ctypes.POINTER(NrcConfig)]
+    L.rgi_nrc_level_resolution.argtypes = [ctypes.POINTER(NrcConfig), U32, ctypes.POINTER(U32)]
+    L.rgi_nrc_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(NrcConfig)]
+    L.rgi_nrc_destroy.restype = None
+    L.rgi_nrc_destroy.argtypes = [P]
+    L.rgi_nrc_train.argtypes = [P, P, P, U32, P, P]
+    L.rgi_nrc_infer.argtypes = [P, P, U32, ctypes.c_int, P, P]
+    L.rgi_nrc_gradients.argtypes = [P, P, P, U32, P, P, P, P]
+    L.rgi_nrc_get_params.argtypes = [P, P, P, P]
+    L.rgi_nrc_set_params.argtypes = [P, P, P, P]
+    L.rgi_nrc_steps.argtypes = [P, ctypes.POINTER(U32)]
     if L.rgi_settings_sizeof() != ctypes.sizeof(CSettings):
         raise RestirError("ABI mismatch: rgi_settings")
     for i, cls in enumerate(ABI_STRUCTS):

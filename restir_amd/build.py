@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librestir_amd.so")
 
-HIP_SOURCES = ["capi.hip", "grid.hip", "trace.hip", "gi.hip", "di.hip", "renderer.hip", "reference.hip", "svgf.hip", "composite.hip", "specular.hip"]
+HIP_SOURCES = ["capi.hip", "grid.hip", "trace.hip", "gi.hip", "di.hip", "renderer.hip", "reference.hip", "svgf.hip", "composite.hip", "specular.hip", "nrc.hip"]
 CPP_SOURCES = ["scene.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

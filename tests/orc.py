@@ -104,8 +104,53 @@ def lib():
     L.orc_shadow_atlas.argtypes = [ctypes.POINTER(Shadow), PS, P]
     L.orc_gbuffer_primary.restype = None
     L.orc_gbuffer_primary.argtypes = [PF, PS, P, U32, P, P, P, P, P]
+    L.orc_nrc_forward.restype = None
+    L.orc_nrc_forward.argtypes = [P, U32, P, P, P, P, P, U32, ctypes.c_int, P]
+    L.orc_nrc_gradients.restype = None
+    L.orc_nrc_gradients.argtypes = [P, U32, P, P, P, P, P, P, U32, ctypes.c_int, P, P, P, P, ctypes.c_int]
+    L.orc_nrc_adam.restype = None
+    L.orc_nrc_adam.argtypes = [P, P, P, P, U32, F, F, F, F, F, F, ctypes.c_int]
+    L.orc_nrc_corner.restype = None
+    L.orc_nrc_corner.argtypes = [P, U32, ctypes.c_int, P, ctypes.c_int, ctypes.POINTER(U32), ctypes.POINTER(F)]
+    L.orc_bf16_round.restype = F
+    L.orc_bf16_round.argtypes = [F]
     _lib = L
     return L
+
+
+class NrcModel:
+    """Host copy of a cache's shape and parameters for the oracle's NRC
+    functions (oracle/nrc.cpp)."""
+
+    def __init__(self, res, log2_table, lo, hi, mlp, table):
+        self.res = np.ascontiguousarray(res, np.uint32)
+        self.log2_table = int(log2_table)
+        self.lo = np.ascontiguousarray(lo, np.float32)
+        self.hi = np.ascontiguousarray(hi, np.float32)
+        self.mlp = np.ascontiguousarray(mlp, np.float32)
+        self.table = np.ascontiguousarray(table, np.float32)
+
+    def forward(self, records, emulate=1):
+        records = np.ascontiguousarray(records, np.float32)
+        n = records.shape[0]
+        out = np.zeros((n, 3), np.float64)
+        lib().orc_nrc_forward(ptr(self.res), self.log2_table, ptr(self.lo), ptr(self.hi), ptr(self.mlp),
+                              ptr(self.table), ptr(records), n, emulate, ptr(out))
+        return out
+
+    def gradients(self, records, targets, emulate=1, denominators=None, frozen=False):
+        """Loss and gradients; `denominators` (float64, one per record)
+        receives the loss denominators, or supplies them when `frozen`."""
+        records = np.ascontiguousarray(records, np.float32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        n = records.shape[0]
+        loss = np.zeros(1, np.float64)
+        mg = np.zeros(self.mlp.size, np.float64)
+        tg = np.zeros(self.table.size, np.float64)
+        lib().orc_nrc_gradients(ptr(self.res), self.log2_table, ptr(self.lo), ptr(self.hi), ptr(self.mlp),
+                                ptr(self.table), ptr(records), ptr(targets), n, emulate, ptr(loss), ptr(mg), ptr(tg),
+                                ptr(denominators), int(frozen))
+        return float(loss[0]), mg, tg
 
 
 def ptr(a):

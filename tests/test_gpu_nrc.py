@@ -1,0 +1,169 @@
+"""The neural radiance cache's HIP kernels against the oracle (oracle/nrc.cpp).
+
+Inference in BF16 is compared with the oracle's emulated mode (the same
+rounding points; the kernels differ only in the order of FP32 accumulation
+inside the matrix cores), FP16 and FP8 with its exact mode. The gradients of
+one batch are compared layer by layer and over the table; the Adam step is
+checked bit for bit on the MLP weights against the oracle's FP32 Adam applied
+to the GPU's own gradients. Training is shown to reduce the loss and the
+error on held-out points.
+"""
+import numpy as np
+import pytest
+
+import orc
+import restir_amd as ra
+from restir_amd import nrc as rnrc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def setup_cache(seed, log2_table=14, table_scale=0.5):
+    rng = np.random.default_rng(seed)
+    nc = rnrc.NeuralCache(log2_table=log2_table, seed=seed, aabb_min=(-1, -1, -1), aabb_max=(1, 1, 1))
+    mlp = np.empty(ra.RGI_NRC_MLP_PARAMS, np.float32)
+    mlp[:20480] = rng.uniform(-1, 1, 20480) * np.sqrt(6.0 / 128.0)
+    mlp[20480:] = rng.uniform(-1, 1, 192) * np.sqrt(6.0 / 67.0)
+    table = rng.uniform(-table_scale, table_scale, nc.table_floats).astype(np.float32)
+    nc.set_params(torch.from_numpy(mlp).cuda(), torch.from_numpy(table).cuda())
+    res = rnrc.level_resolutions(nc.cfg)
+    model = orc.NrcModel(res, log2_table, [-1, -1, -1], [1, 1, 1], mlp, table)
+    return nc, model, rng
+
+
+def records(rng, n):
+    rec = np.zeros((n, 12), np.float32)
+    rec[:, 0:3] = rng.uniform(-1, 1, (n, 3))
+    nrm = rng.normal(size=(n, 3))
+    rec[:, 4:7] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    rec[:, 8:11] = rng.uniform(0, 1, (n, 3))
+    return rec
+
+
+def targets(rng, n):
+    t = np.zeros((n, 4), np.float32)
+    t[:, 0:3] = rng.uniform(0, 2, (n, 3))
+    t[:, 3] = (rng.uniform(size=n) < 0.9).astype(np.float32)
+    return t
+
+
+def rel(a, b, floor):
+    return np.abs(a - b) / (np.abs(b) + floor)
+
+
+@pytest.mark.parametrize("count", [4096, 1000, 17])
+def test_nrc_infer_bf16_matches_oracle(count):
+    nc, model, rng = setup_cache(11)
+    rec = records(rng, count)
+    print("nrc infer bf16 count", count, flush=True)
+    out = nc.infer(torch.from_numpy(rec).cuda(), ra.RGI_NRC_BF16).cpu().numpy()
+    want = model.forward(rec, emulate=1)
+    assert np.all(out[:, 3] == 0.0)
+    r = rel(out[:, :3].astype(np.float64), want, 0.05)
+    assert np.median(r) < 2e-3, np.median(r)
+    assert np.quantile(r, 0.99) < 3e-2, np.quantile(r, 0.99)
+
+
+def test_nrc_infer_fp16_fp8_close_to_exact():
+    nc, model, rng = setup_cache(12)
+    rec = records(rng, 8192)
+    exact = model.forward(rec, emulate=0)
+    scale = np.sqrt(np.mean(exact ** 2))
+    # FP8 activation scales come from a training batch: take one step first,
+    # then compare against the updated parameters.
+    tg = targets(rng, 8192)
+    nc.train(torch.from_numpy(rec).cuda(), torch.from_numpy(tg).cuda())
+    mlp, table = nc.params()
+    model.mlp[:] = mlp.cpu().numpy()
+    model.table[:] = table.cpu().numpy()
+    exact = model.forward(rec, emulate=0)
+    err = {}
+    for name, prec in (("bf16", ra.RGI_NRC_BF16), ("fp16", ra.RGI_NRC_FP16), ("fp8", ra.RGI_NRC_FP8)):
+        print("nrc infer", name, flush=True)
+        out = nc.infer(torch.from_numpy(rec).cuda(), prec).cpu().numpy()[:, :3].astype(np.float64)
+        err[name] = np.sqrt(np.mean((out - exact) ** 2)) / scale
+    print("relative RMS error against the exact function:", err)
+    assert err["fp16"] < 2e-3
+    assert err["bf16"] < 2e-2
+    assert err["fp8"] < 0.15
+    assert err["fp16"] < err["bf16"] < err["fp8"]
+
+
+def test_nrc_gradients_match_oracle():
+    nc, model, rng = setup_cache(13)
+    n = 2048
+    rec, tg = records(rng, n), targets(rng, n)
+    print("nrc gradients", n, flush=True)
+    loss, mg, tgrad = nc.gradients(torch.from_numpy(rec).cuda(), torch.from_numpy(tg).cuda())
+    loss, mg, tgrad = float(loss.cpu()[0]), mg.cpu().numpy().astype(np.float64), tgrad.cpu().numpy().astype(np.float64)
+    wl, wm, wt = model.gradients(rec, tg, emulate=1)
+    assert abs(loss - wl) < 1e-3 * wl, (loss, wl)
+    for k in range(6):
+        sl = slice(k * 4096, k * 4096 + 4096) if k < 5 else slice(20480, 20672)
+        e = np.linalg.norm(mg[sl] - wm[sl]) / np.linalg.norm(wm[sl])
+        assert e < 2e-2, (k, e)
+    assert np.linalg.norm(tgrad - wt) < 5e-2 * np.linalg.norm(wt)
+    # The same entries are touched.
+    assert np.array_equal(tgrad != 0.0, wt != 0.0)
+
+
+def test_nrc_adam_step_matches_oracle():
+    nc, model, rng = setup_cache(14)
+    n = 3000
+    rec, tg = torch.from_numpy(records(rng, n)).cuda(), torch.from_numpy(targets(rng, n)).cuda()
+    _, mg, tgrad = nc.gradients(rec, tg)
+    mlp0, table0 = nc.params()
+    print("nrc train step", flush=True)
+    nc.train(rec, tg)
+    assert nc.steps() == 1
+    mlp1, table1 = nc.params()
+    cfg = nc.cfg
+    p = mlp0.cpu().numpy().copy()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    g = mg.cpu().numpy()
+    c1, c2 = np.float32(1 - cfg.beta1), np.float32(1 - cfg.beta2)
+    orc.lib().orc_nrc_adam(orc.ptr(p), orc.ptr(m), orc.ptr(v), orc.ptr(g), p.size, cfg.learning_rate, cfg.beta1,
+                           cfg.beta2, cfg.epsilon, float(np.float32(1 - np.float64(cfg.beta1))),
+                           float(np.float32(1 - np.float64(cfg.beta2))), 0)
+    assert np.array_equal(p.view(np.uint32), mlp1.cpu().numpy().view(np.uint32))
+    # Table: untouched entries keep their value bit for bit; touched ones move
+    # by the learning rate against the gradient's sign.
+    t0, t1, gt = table0.cpu().numpy(), table1.cpu().numpy(), tgrad.cpu().numpy()
+    untouched = gt == 0.0
+    assert np.array_equal(t0[untouched].view(np.uint32), t1[untouched].view(np.uint32))
+    big = np.abs(gt) > 1e-5
+    step = (t1[big].astype(np.float64) - t0[big]) / cfg.learning_rate
+    assert np.allclose(step, -np.sign(gt[big]), atol=2e-3)
+
+
+def smooth_radiance(rec):
+    p, n = rec[:, 0:3], rec[:, 4:7]
+    r = 0.6 + 0.4 * np.sin(2.0 * p[:, 0]) * np.cos(1.5 * p[:, 1]) + 0.2 * n[:, 1]
+    g = 0.5 + 0.3 * np.cos(3.0 * p[:, 2]) + 0.2 * p[:, 0] * p[:, 1]
+    b = 0.4 + 0.3 * rec[:, 8] + 0.2 * np.sin(4.0 * p[:, 1] + p[:, 2])
+    return np.stack([r, g, b], axis=1).astype(np.float32)
+
+
+def test_nrc_training_learns_a_smooth_field():
+    rng = np.random.default_rng(15)
+    nc = rnrc.NeuralCache(log2_table=16, seed=3, learning_rate=0.01, aabb_min=(-1, -1, -1), aabb_max=(1, 1, 1))
+    held = records(rng, 8192)
+    held_t = smooth_radiance(held)
+
+    def held_error():
+        out = nc.infer(torch.from_numpy(held).cuda()).cpu().numpy()[:, :3]
+        return float(np.mean((out - held_t) ** 2) / np.mean(held_t ** 2))
+
+    before = held_error()
+    losses = []
+    for step in range(300):
+        rec = records(rng, 16384)
+        t = np.concatenate([smooth_radiance(rec), np.ones((16384, 1), np.float32)], axis=1)
+        losses.append(float(nc.train(torch.from_numpy(rec).cuda(), torch.from_numpy(t).cuda()).cpu()[0]))
+    after = held_error()
+    print("held-out relative MSE before %.4f after %.5f; loss %.4f -> %.5f" % (before, after, losses[0], losses[-1]))
+    assert np.mean(losses[-20:]) < 0.05 * losses[0]
+    assert after < 0.02 * before

@@ -912,7 +912,9 @@ def test_oracle_is_thread_count_independent():
 def declared_functions():
     import re
     names = []
-    for h in ("restir_amd.h", "restir_amd_scene.h"):
+    for h in sorted(os.listdir(os.path.join(orc.ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
         text = open(os.path.join(orc.ROOT, "include", h)).read()
         names += re.findall(r"^\w[\w\s\*]*?\b(rgi_\w+)\s*\(", text, re.M)
     return sorted(set(names))
@@ -920,7 +922,7 @@ def declared_functions():
 
 def test_library_exports_every_declared_symbol():
     names = declared_functions()
-    assert len(names) >= 19
+    assert len(names) >= 40 and "rgi_nrc_train" in names and "rgi_batch_render" in names
     lib = ctypes.CDLL(ra.LIB_PATH)
     for n in names:
         assert hasattr(lib, n), n
@@ -930,6 +932,19 @@ def test_abi_struct_sizes_match():
     lib = ra.lib()
     for i, cls in enumerate(ra.ABI_STRUCTS):
         assert lib.rgi_abi_sizeof(i) == ctypes.sizeof(cls)
+
+
+def test_nrc_config_is_checked_without_a_gpu():
+    from restir_amd import nrc as rnrc
+    lib = ra.lib()
+    out = ctypes.c_void_p()
+    for bad in ({"log2_table": 9}, {"log2_table": 23}, {"base_resolution": 1}, {"max_resolution": 8},
+                {"learning_rate": 0.0}, {"beta2": 1.0}, {"aabb_min": (0, 0, 0), "aabb_max": (1, 0, 1)}):
+        cfg = rnrc.config(**bad)
+        assert lib.rgi_nrc_create(ctypes.byref(out), ctypes.byref(cfg)) == -1, bad
+        assert out.value is None
+    assert lib.rgi_nrc_train(None, None, None, 0, None, None) == -1
+    assert lib.rgi_nrc_infer(None, None, 0, 0, None, None) == -1
 
 
 def test_invalid_arguments_are_rejected_without_a_gpu():
