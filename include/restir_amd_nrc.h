@@ -118,6 +118,47 @@ int rgi_nrc_set_params(rgi_nrc* nrc, const float* mlp, const float* table, void*
 /* Adam steps taken since creation or the last rgi_nrc_set_params. */
 int rgi_nrc_steps(const rgi_nrc* nrc, uint32_t* steps);
 
+/* ---------------------------------------------------------------------
+ * The cache in the frame driver (include/restir_amd_renderer.h).
+ *
+ * With a cache set, each frame (outside reference accumulation):
+ *  1. after the DI passes and the grid, once the cache has trained `warmup`
+ *     frames, queries it at every G-buffer pixel (one query per pixel) and
+ *     gives the GI temporal, GI spatial and specular passes its radiance as
+ *     their surface light, with multiple bounces off: a sample point seen on
+ *     screen then reflects the point light plus the cache's radiance (plus
+ *     its emission when the G-buffer emission is the surface light), where
+ *     the reference reads the DI light plus last frame's indirect light. The
+ *     GI target function and the shaded samples both use it;
+ *  2. after the denoiser, gathers `batch` training records from pixels
+ *     stratified over the image, each with target DI diffuse + albedo * the
+ *     next frame's GI history colour (the ReSTIR estimate of the light the
+ *     surface reflects, which already contains the cache's own prediction at
+ *     the sample points: self-training), and takes one Adam step.
+ * Until `warmup` frames have trained, and whenever the reference
+ * accumulates, the frame is the plain ReSTIR path. */
+typedef struct rgi_renderer rgi_renderer;
+
+typedef struct rgi_nrc_settings {
+    rgi_nrc_config cache;     /* aabb: the scene's bounds */
+    uint32_t batch;           /* 65536: training records per frame */
+    uint32_t warmup;          /* 4: training frames before the GI passes read the cache */
+    int32_t precision;        /* RGI_NRC_BF16: precision of the per-pixel queries */
+    int32_t reserved;
+} rgi_nrc_settings;
+
+void rgi_nrc_settings_default(rgi_nrc_settings* settings);
+
+/* Gives the renderer a new cache (NULL: none; the previous one is freed). */
+int rgi_renderer_set_nrc(rgi_renderer* renderer, const rgi_nrc_settings* settings);
+
+/* The renderer's cache (NULL when none), its per-pixel radiance plane
+ * (RGBA16F, valid after a frame that queried it), the last training loss (a
+ * device float) and the number of frames it has trained. Any output may be
+ * NULL. */
+int rgi_renderer_nrc_state(const rgi_renderer* renderer, rgi_nrc** cache, const uint16_t** radiance,
+                           const float** loss, uint32_t* trained_frames);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,7 @@ Model model_of(const uint32_t* res, uint32_t log2_table, const float* lo, const 
 }
 
 float unit(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
+float bounded(float x, float lo, float hi) { return x > lo ? (x < hi ? x : hi) : lo; }
 
 // Entry and trilinear weight of corner k (x + 2y + 4z) of level l, in FP32
 // exactly as the kernel computes them.
@@ -147,8 +148,12 @@ void forward(const Model& m, const float* mlp, const float* table, const float* 
             s.x[2 * l + 1] = a1;
         }
     }
-    float ex[8] = {rec[4] * 0.5f + 0.5f, rec[5] * 0.5f + 0.5f, rec[6] * 0.5f + 0.5f, rec[8], rec[9], rec[10], 1.0f,
-                   0.0f};
+    float nn[3], al[3];
+    for (int c = 0; c < 3; ++c) {
+        nn[c] = bounded(rec[4 + c], -1.0f, 1.0f);
+        al[c] = bounded(rec[8 + c], 0.0f, 1.0f);
+    }
+    float ex[8] = {nn[0] * 0.5f + 0.5f, nn[1] * 0.5f + 0.5f, nn[2] * 0.5f + 0.5f, al[0], al[1], al[2], 1.0f, 0.0f};
     for (int q = 0; q < 32; ++q) s.x[32 + q] = q < 8 ? (emulate ? bf16_round(ex[q]) : ex[q]) : 0.0;
     for (int i = 0; i < 64; ++i) s.a[0][i] = s.x[i];
     for (int k = 0; k < 5; ++k)

@@ -132,6 +132,11 @@ class NrcConfig(ctypes.Structure):
                 ("aabb_min", F * 3), ("aabb_max", F * 3), ("reserved", I32 * 4)]
 
 
+class NrcSettings(ctypes.Structure):
+    """rgi_nrc_settings (include/restir_amd_nrc.h): the cache in the frame driver."""
+    _fields_ = [("cache", NrcConfig), ("batch", U32), ("warmup", U32), ("precision", I32), ("reserved", I32)]
+
+
 RGI_NRC_BF16, RGI_NRC_FP16, RGI_NRC_FP8 = 0, 1, 2
 RGI_NRC_MLP_PARAMS = 5 * 64 * 64 + 3 * 64
 RGI_NRC_LEVELS = 16
@@ -232,6 +237,11 @@ def lib():
     L.rgi_nrc_get_params.argtypes = [P, P, P, P]
     L.rgi_nrc_set_params.argtypes = [P, P, P, P]
     L.rgi_nrc_steps.argtypes = [P, ctypes.POINTER(U32)]
+    L.rgi_nrc_settings_default.restype = None
+    L.rgi_nrc_settings_default.argtypes = [ctypes.POINTER(NrcSettings)]
+    L.rgi_renderer_set_nrc.argtypes = [P, ctypes.POINTER(NrcSettings)]
+    L.rgi_renderer_nrc_state.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
+                                         ctypes.POINTER(U32)]
     if L.rgi_settings_sizeof() != ctypes.sizeof(CSettings):
         raise RestirError("ABI mismatch: rgi_settings")
     for i, cls in enumerate(ABI_STRUCTS):

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "../../include/restir_amd.h"
+#include "../../include/restir_amd_nrc.h"
 
 namespace rgi {
 // Launch variants (RGI_OPT_*), all bit-exact with each other.
@@ -65,5 +66,13 @@ int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
                 const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream);
 int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
                uint16_t* diffuse, uint16_t* specular, const Tuning& tuning, hipStream_t stream);
+
+// The neural radiance cache's frame-driver passes (nrc.hip): the cache's
+// radiance at every G-buffer pixel, and one frame's training records.
+int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* emission, int precision,
+                      uint16_t* out, hipStream_t stream);
+int nrc_gather(const rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
+               const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
+               hipStream_t stream);
 
 } // namespace rgi

@@ -48,6 +48,7 @@ struct rgi_nrc {
     uint8_t* w_fwd_8 = nullptr;
     float* scales = nullptr;      // [0..5] weight scales, [8..13] activation scales, [16..21] 1/(sa*sw)
     uint32_t* act_max = nullptr;  // max |a_k| of the last training batch, float bits, k = 0..5
+    uint32_t* levels = nullptr;   // res[16], stride[16] (res + 1 for a densely indexed level, else 0)
     float* loss = nullptr;        // internal loss accumulator
     // Training scratch, grown on demand.
     __bf16* acts = nullptr;       // [6][tiles][64][256]
@@ -63,18 +64,21 @@ typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
-constexpr int kTile = 256;       // samples per training workgroup (4 waves x 64)
-constexpr int kChunkTiles = 4;   // tiles per weight-gradient workgroup
-constexpr int kFwdBlocks = 42;   // fragments (64 lanes x 8) of the forward copy
-constexpr int kBwdBlocks = 40;   // fragments of the transposed copy
+constexpr int kTrainB = 2;              // 16-sample column blocks per wave in training
+static_assert(kTrainB >= 1 && 16 * kTrainB <= 64, "the ReLU masks hold 16 bits per column block in 64 bits");
+constexpr int kTile = 64 * kTrainB;     // samples per training workgroup (4 waves)
+constexpr int kChunk = 1024;            // samples per weight-gradient workgroup
+constexpr int kChunkTiles = kChunk / kTile;
+constexpr int kInferB = 1;              // column blocks per wave in inference
+constexpr int kFwdBlocks = 42;          // fragments (64 lanes x 8) of the forward copy
+constexpr int kBwdBlocks = 40;          // fragments of the transposed copy
 constexpr int kFrag = 512;
 constexpr uint32_t kPrime1 = 2654435761u;
 constexpr uint32_t kPrime2 = 805459861u;
 
 struct Grid {
-    uint32_t res[16];
-    uint32_t stride[16]; // res + 1 for a densely indexed level, 0 for a hashed one
-    uint32_t mask;       // entries per level - 1
+    const uint32_t* levels; // device: res[16], then stride[16] (res + 1 when dense, else 0)
+    uint32_t mask;          // entries per level - 1
     uint32_t entries;
     float lo[3];
     float inv_extent[3];
@@ -93,37 +97,73 @@ RGI_DEV float half_hi(uint32_t v) { return half_to_float((uint16_t)(v >> 16)); }
 // Clamp to [0, 1]; NaN goes to 0, so every corner index stays in its level.
 RGI_DEV float unit(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
-// Corner k (x + 2 y + 4 z) of level l around p: entry index and trilinear
-// weight.
-RGI_DEV void corner(const Grid& gp, int l, float px, float py, float pz, int k, uint32_t& idx, float& w)
+// The four levels a lane encodes: lane group g owns levels 4g .. 4g+3, which
+// are input features 8g .. 8g+7 (slot q of the first K fragment is feature
+// 8g + q).
+struct LaneLevels {
+    float rf[4];
+    int rmax[4];
+    uint32_t stride[4];
+    uint32_t offset[4]; // first entry of the level
+};
+
+RGI_DEV LaneLevels lane_levels(const Grid& gp, int g)
 {
-    uint32_t r = gp.res[l];
-    float rf = (float)r;
-    float fx = px * rf, fy = py * rf, fz = pz * rf;
-    int cx = min((int)floorf(fx), (int)r - 1), cy = min((int)floorf(fy), (int)r - 1),
-        cz = min((int)floorf(fz), (int)r - 1);
-    float wx = fx - (float)cx, wy = fy - (float)cy, wz = fz - (float)cz;
-    uint32_t x = (uint32_t)cx + (uint32_t)(k & 1), y = (uint32_t)cy + (uint32_t)((k >> 1) & 1),
-             z = (uint32_t)cz + (uint32_t)(k >> 2);
-    w = (((k & 1) ? wx : 1.0f - wx) * ((k & 2) ? wy : 1.0f - wy)) * ((k & 4) ? wz : 1.0f - wz);
-    uint32_t s = gp.stride[l];
-    idx = s ? x + y * s + z * (s * s) : ((x ^ (y * kPrime1) ^ (z * kPrime2)) & gp.mask);
+    LaneLevels L;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        uint32_t r = gp.levels[4 * g + m];
+        L.rf[m] = (float)r;
+        L.rmax[m] = (int)r - 1;
+        L.stride[m] = gp.levels[16 + 4 * g + m];
+        L.offset[m] = (uint32_t)(4 * g + m) * gp.entries;
+    }
+    return L;
 }
 
-// Features of levels 4g .. 4g+3 (eight values, level-major).
-RGI_DEV void encode4(const Grid& gp, const uint32_t* table_h, float px, float py, float pz, int g, float* e)
+// The cell of level m around p and the fractional position in it.
+RGI_DEV void level_cell(const LaneLevels& L, int m, const float* p, uint32_t* c, float* w)
 {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float f = p[a] * L.rf[m];
+        int fl = min((int)floorf(f), L.rmax[m]);
+        c[a] = (uint32_t)fl;
+        w[a] = f - (float)fl;
+    }
+}
+
+// Entry (within its level) and trilinear weight of corner k = x + 2y + 4z.
+RGI_DEV void corner_of(uint32_t stride, uint32_t mask, const uint32_t* c, const float* w, int k, uint32_t& idx,
+                       float& weight)
+{
+    uint32_t x = c[0] + (uint32_t)(k & 1), y = c[1] + (uint32_t)((k >> 1) & 1), z = c[2] + (uint32_t)(k >> 2);
+    weight = (((k & 1) ? w[0] : 1.0f - w[0]) * ((k & 2) ? w[1] : 1.0f - w[1])) * ((k & 4) ? w[2] : 1.0f - w[2]);
+    idx = stride ? x + y * stride + z * (stride * stride) : ((x ^ (y * kPrime1) ^ (z * kPrime2)) & mask);
+}
+
+// Features of this lane's four levels (eight values, level-major): FP32 sums
+// of FP16 table values in corner order.
+RGI_DEV void encode_lane(const LaneLevels& L, uint32_t mask, const uint32_t* table_h, const float* p, float* e)
+{
+#pragma unroll
     for (int m = 0; m < 4; ++m) {
-        int l = 4 * g + m;
-        const uint32_t* level = table_h + (size_t)l * gp.entries;
-        float a0 = 0.0f, a1 = 0.0f;
+        uint32_t c[3];
+        float w[3];
+        level_cell(L, m, p, c, w);
+        uint32_t v[8];
+        float wt[8];
+#pragma unroll
         for (int k = 0; k < 8; ++k) {
             uint32_t idx;
-            float w;
-            corner(gp, l, px, py, pz, k, idx, w);
-            uint32_t v = level[idx];
-            a0 = a0 + w * half_lo(v);
-            a1 = a1 + w * half_hi(v);
+            corner_of(L.stride[m], mask, c, w, k, idx, wt[k]);
+            v[k] = table_h[L.offset[m] + idx];
+        }
+        float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            a0 = a0 + wt[k] * half_lo(v[k]);
+            a1 = a1 + wt[k] * half_hi(v[k]);
         }
         e[2 * m] = a0;
         e[2 * m + 1] = a1;
@@ -137,33 +177,61 @@ struct Point {
     float a[3];
 };
 
+// x clamped to [lo, hi], NaN to lo.
+RGI_DEV float bounded(float x, float lo, float hi) { return x > lo ? (x < hi ? x : hi) : lo; }
+
+// Every input is finite: a NaN anywhere would reach the weight gradients
+// through products with zero back-propagated gradients.
 RGI_DEV Point point_of(const Grid& gp, const float* rec)
 {
     Point pt;
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         pt.p[c] = unit((rec[c] - gp.lo[c]) * gp.inv_extent[c]);
-        pt.n[c] = rec[4 + c];
-        pt.a[c] = rec[8 + c];
+        pt.n[c] = bounded(rec[4 + c], -1.0f, 1.0f);
+        pt.a[c] = bounded(rec[8 + c], 0.0f, 1.0f);
     }
     return pt;
 }
 
-// Inputs 32 .. 39 held by lane group 0: normal * 0.5 + 0.5, albedo, 1, 0.
-RGI_DEV void extras(const Point& pt, float* e)
+RGI_DEV Point load_point(const Grid& gp, const float* records, uint32_t s, uint32_t count)
 {
-    e[0] = pt.n[0] * 0.5f + 0.5f;
-    e[1] = pt.n[1] * 0.5f + 0.5f;
-    e[2] = pt.n[2] * 0.5f + 0.5f;
-    e[3] = pt.a[0];
-    e[4] = pt.a[1];
-    e[5] = pt.a[2];
-    e[6] = 1.0f;
-    e[7] = 0.0f;
+    float rec[12];
+    if (s < count) {
+        const float4* r = reinterpret_cast<const float4*>(records) + (size_t)s * 3;
+        float4 a = r[0], b = r[1], c = r[2];
+        rec[0] = a.x; rec[1] = a.y; rec[2] = a.z; rec[3] = a.w;
+        rec[4] = b.x; rec[5] = b.y; rec[6] = b.z; rec[7] = b.w;
+        rec[8] = c.x; rec[9] = c.y; rec[10] = c.z; rec[11] = c.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 12; ++i) rec[i] = 0.0f;
+    }
+    return point_of(gp, rec);
+}
+
+// Inputs 32 .. 39, held by lane group 0: normal * 0.5 + 0.5, albedo, 1, 0.
+RGI_DEV void extras(const Point& pt, int g, float* e)
+{
+    if (g == 0) {
+        e[0] = pt.n[0] * 0.5f + 0.5f;
+        e[1] = pt.n[1] * 0.5f + 0.5f;
+        e[2] = pt.n[2] * 0.5f + 0.5f;
+        e[3] = pt.a[0];
+        e[4] = pt.a[1];
+        e[5] = pt.a[2];
+        e[6] = 1.0f;
+        e[7] = 0.0f;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) e[q] = 0.0f;
+    }
 }
 
 RGI_DEV bf16x8 pack_bf16(const float* e)
 {
     bf16x8 v;
+#pragma unroll
     for (int q = 0; q < 8; ++q) v[q] = (__bf16)e[q];
     return v;
 }
@@ -191,10 +259,10 @@ RGI_DEV void note_max(uint32_t* slot, float v)
     if ((threadIdx.x & 63u) == 0u) atomicMax(slot, __float_as_uint(v));
 }
 
-// [feature][sample] store of one fragment's four rows for the sample of this
-// lane.
+// [feature][sample] store of one fragment's four rows for this lane's sample.
 RGI_DEV void store_rows(__bf16* plane, int t, int g, int sample, const float* v)
 {
+#pragma unroll
     for (int r = 0; r < 4; ++r) plane[(size_t)(16 * t + 4 * g + r) * kTile + sample] = (__bf16)v[r];
 }
 
@@ -210,80 +278,91 @@ struct TrainArgs {
     const __bf16* w_bwd;
     const uint32_t* table_h;
     float* table_grad;
-    __bf16* acts;   // [6][tiles][64][256]
-    __bf16* deltas; // [6][tiles][64][256]
+    __bf16* acts;   // [6][tiles][64][kTile]
+    __bf16* deltas; // [6][tiles][64][kTile]
     uint32_t tiles;
     float* loss;
     uint32_t* act_max;
 };
 
-// Forward, loss and backward of 64 samples per wave: stores the activations
-// and back-propagated gradients for the weight-gradient GEMMs and scatters
-// the input gradient into the table gradient.
+// Forward, loss and backward of 16 * kTrainB samples per wave: stores the
+// activations and back-propagated gradients for the weight-gradient GEMMs
+// and scatters the input gradient into the table gradient.
 __global__ void __launch_bounds__(256) k_nrc_train(TrainArgs a)
 {
+    constexpr int B = kTrainB;
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u), g = lane >> 4, s16 = lane & 15;
     const uint32_t tile = blockIdx.x;
     const size_t layer_stride = (size_t)a.tiles * 64 * kTile;
-    bf16x8 X[2][4];
-    Point pts[4];
+    const size_t tile_base = (size_t)tile * 64 * kTile;
+    const LaneLevels L = lane_levels(a.gp, g);
+    bf16x8 X[2][B];
+    Point pts[B];
     float amax = 0.0f;
-    for (int b = 0; b < 4; ++b) {
-        int local = wave * 64 + b * 16 + s16;
-        uint32_t s = tile * kTile + (uint32_t)local;
-        float rec[12] = {0};
-        if (s < a.count)
-            for (int c = 0; c < 12; ++c) rec[c] = a.records[(size_t)s * 12 + c];
-        pts[b] = point_of(a.gp, rec);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int local = wave * 16 * B + b * 16 + s16;
+        pts[b] = load_point(a.gp, a.records, tile * kTile + (uint32_t)local, a.count);
         float e[8], x[8];
-        encode4(a.gp, a.table_h, pts[b].p[0], pts[b].p[1], pts[b].p[2], g, e);
-        if (g == 0) extras(pts[b], x);
-        else
-            for (int q = 0; q < 8; ++q) x[q] = 0.0f;
+        encode_lane(L, a.gp.mask, a.table_h, pts[b].p, e);
+        extras(pts[b], g, x);
         X[0][b] = pack_bf16(e);
         X[1][b] = pack_bf16(x);
-        __bf16* a0 = a.acts + (size_t)tile * 64 * kTile;
+        __bf16* a0 = a.acts + tile_base;
+#pragma unroll
         for (int q = 0; q < 8; ++q) {
             a0[(size_t)(8 * g + q) * kTile + local] = X[0][b][q];
             a0[(size_t)(32 + 8 * g + q) * kTile + local] = X[1][b][q];
-            amax = fmaxf(amax, fabsf((float)X[0][b][q]));
-            amax = fmaxf(amax, fabsf((float)X[1][b][q]));
+            amax = fmaxf(amax, fmaxf(fabsf((float)X[0][b][q]), fabsf((float)X[1][b][q])));
         }
     }
     note_max(a.act_max + 0, amax);
 
     // Hidden layers 1..5.
     uint64_t masks[5];
+#pragma unroll
     for (int k = 0; k < 5; ++k) {
-        f32x4 acc[4][4];
-        for (int b = 0; b < 4; ++b)
+        f32x4 acc[B][4];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
             for (int t = 0; t < 4; ++t) acc[b][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
         for (int j = 0; j < 2; ++j)
+#pragma unroll
             for (int t = 0; t < 4; ++t) {
                 bf16x8 w = frag(a.w_fwd, k * 8 + t * 2 + j, lane);
-                for (int b = 0; b < 4; ++b) acc[b][t] = mfma_bf16(w, X[j][b], acc[b][t]);
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b][t] = mfma_bf16(w, X[j][b], acc[b][t]);
             }
         uint64_t mask = 0;
-        __bf16* plane = a.acts + (size_t)(k + 1) * layer_stride + (size_t)tile * 64 * kTile;
+        __bf16* plane = a.acts + (size_t)(k + 1) * layer_stride + tile_base;
         amax = 0.0f;
-        for (int b = 0; b < 4; ++b) {
-            int local = wave * 64 + b * 16 + s16;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int local = wave * 16 * B + b * 16 + s16;
             float v[4][4];
+#pragma unroll
             for (int t = 0; t < 4; ++t)
+#pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     float z = acc[b][t][r];
                     bool on = z > 0.0f;
                     mask |= (uint64_t)on << (b * 16 + t * 4 + r);
                     v[t][r] = on ? z : 0.0f;
                 }
+#pragma unroll
             for (int t = 0; t < 4; ++t) store_rows(plane, t, g, local, v[t]);
+#pragma unroll
             for (int j = 0; j < 2; ++j) {
                 float e[8];
+#pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     e[r] = v[2 * j][r];
                     e[4 + r] = v[2 * j + 1][r];
                 }
                 X[j][b] = pack_bf16(e);
+#pragma unroll
                 for (int q = 0; q < 8; ++q) amax = fmaxf(amax, (float)X[j][b][q]);
             }
         }
@@ -292,68 +371,87 @@ __global__ void __launch_bounds__(256) k_nrc_train(TrainArgs a)
     }
 
     // Output layer, loss and its gradient (lane group 0 holds rows 0..3).
-    f32x4 y[4];
-    for (int b = 0; b < 4; ++b) y[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 y[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) y[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int j = 0; j < 2; ++j) {
         bf16x8 w = frag(a.w_fwd, 40 + j, lane);
-        for (int b = 0; b < 4; ++b) y[b] = mfma_bf16(w, X[j][b], y[b]);
+#pragma unroll
+        for (int b = 0; b < B; ++b) y[b] = mfma_bf16(w, X[j][b], y[b]);
     }
     float loss = 0.0f;
-    bf16x8 D[2][4];
-    __bf16* d6 = a.deltas + (size_t)5 * layer_stride + (size_t)tile * 64 * kTile;
-    for (int b = 0; b < 4; ++b) {
-        int local = wave * 64 + b * 16 + s16;
-        uint32_t s = tile * kTile + (uint32_t)local;
+    bf16x8 D[2][B];
+    __bf16* d6 = a.deltas + (size_t)5 * layer_stride + tile_base;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int local = wave * 16 * B + b * 16 + s16;
+        const uint32_t s = tile * kTile + (uint32_t)local;
         float d[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (g == 0 && s < a.count) {
-            const float* t = a.targets + (size_t)s * 4;
-            float m = t[3];
-            float mean = ((y[b][0] + y[b][1]) + y[b][2]) / 3.0f;
-            float denom = mean * mean + 0.01f;
-            float e0 = y[b][0] - t[0], e1 = y[b][1] - t[1], e2 = y[b][2] - t[2];
-            loss = loss + (m * ((e0 * e0 + e1 * e1) + e2 * e2)) / denom;
-            float scale = ((2.0f * m) * a.inv_count) / denom;
-            d[0] = e0 * scale;
-            d[1] = e1 * scale;
-            d[2] = e2 * scale;
+            float4 t = reinterpret_cast<const float4*>(a.targets)[s];
+            if (t.w > 0.0f) {
+                float mean = ((y[b][0] + y[b][1]) + y[b][2]) / 3.0f;
+                float denom = mean * mean + 0.01f;
+                float e0 = y[b][0] - t.x, e1 = y[b][1] - t.y, e2 = y[b][2] - t.z;
+                loss = loss + (t.w * ((e0 * e0 + e1 * e1) + e2 * e2)) / denom;
+                float scale = ((2.0f * t.w) * a.inv_count) / denom;
+                d[0] = e0 * scale;
+                d[1] = e1 * scale;
+                d[2] = e2 * scale;
+            }
         }
         store_rows(d6, 0, g, local, d);
         float e[8] = {d[0], d[1], d[2], d[3], 0.0f, 0.0f, 0.0f, 0.0f};
         D[0][b] = pack_bf16(e);
+        D[1][b] = pack_bf16(e); // unused for the output layer (K = 16 rows)
     }
     loss = wave_sum(loss);
     if (lane == 0 && loss != 0.0f) atomicAdd(a.loss, loss * a.inv_count);
 
-    // Back through W6 (K = the 16 output rows, slots q < 4 only), then W5..W2.
+    // Back through W6 (K = the 16 output rows, slots q < 4 only), then W5..W2:
+    // after step k, D holds the gradient of hidden layer k's pre-activation.
+#pragma unroll
     for (int k = 5; k >= 1; --k) {
-        f32x4 acc[4][4];
-        for (int b = 0; b < 4; ++b)
+        f32x4 acc[B][4];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
             for (int t = 0; t < 4; ++t) acc[b][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         if (k == 5) {
+#pragma unroll
             for (int t = 0; t < 4; ++t) {
                 bf16x8 w = frag(a.w_bwd, t, lane);
-                for (int b = 0; b < 4; ++b) acc[b][t] = mfma_bf16(w, D[0][b], acc[b][t]);
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b][t] = mfma_bf16(w, D[0][b], acc[b][t]);
             }
         } else {
-            // k + 1 is the layer whose gradient D holds; block base of W_{k+1}^T.
-            int base = 4 + (4 - k) * 8;
+            const int base = 4 + (4 - k) * 8; // fragments of W_{k+1}^T
+#pragma unroll
             for (int j = 0; j < 2; ++j)
+#pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     bf16x8 w = frag(a.w_bwd, base + t * 2 + j, lane);
-                    for (int b = 0; b < 4; ++b) acc[b][t] = mfma_bf16(w, D[j][b], acc[b][t]);
+#pragma unroll
+                    for (int b = 0; b < B; ++b) acc[b][t] = mfma_bf16(w, D[j][b], acc[b][t]);
                 }
         }
-        // Gradient of layer k's pre-activation: times ReLU'(z_k).
-        __bf16* plane = a.deltas + (size_t)(k - 1) * layer_stride + (size_t)tile * 64 * kTile;
-        for (int b = 0; b < 4; ++b) {
-            int local = wave * 64 + b * 16 + s16;
+        __bf16* plane = a.deltas + (size_t)(k - 1) * layer_stride + tile_base;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            const int local = wave * 16 * B + b * 16 + s16;
             float v[4][4];
+#pragma unroll
             for (int t = 0; t < 4; ++t)
+#pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    v[t][r] = ((masks[k - 1] >> (b * 16 + t * 4 + r)) & 1u) ? acc[b][t][r] : 0.0f;
+                    v[t][r] = ((masks[k - 1] >> (b * 16 + t * 4 + r)) & 1ull) ? acc[b][t][r] : 0.0f;
+#pragma unroll
             for (int t = 0; t < 4; ++t) store_rows(plane, t, g, local, v[t]);
+#pragma unroll
             for (int j = 0; j < 2; ++j) {
                 float e[8];
+#pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     e[r] = v[2 * j][r];
                     e[4 + r] = v[2 * j + 1][r];
@@ -363,32 +461,41 @@ __global__ void __launch_bounds__(256) k_nrc_train(TrainArgs a)
         }
     }
 
-    // Input gradient of the 32 encoded features: W1^T (rows 0..31) times D.
-    for (int b = 0; b < 4; ++b) {
+    // Input gradient of the encoded features, W1^T times D. The transposed
+    // copy orders its rows so that output row 16t + 4g + r is input feature
+    // 8g + 4t + r: this lane receives the gradients of its own four levels
+    // (level 4g + 2t + r/2, feature r%2).
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
         f32x4 dx[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
         for (int j = 0; j < 2; ++j)
+#pragma unroll
             for (int t = 0; t < 2; ++t) dx[t] = mfma_bf16(frag(a.w_bwd, 36 + t * 2 + j, lane), D[j][b], dx[t]);
-        // Rows 16t + 4g + r: level 8t + 2g + r/2, feature r%2.
-        for (int t = 0; t < 2; ++t)
-            for (int h = 0; h < 2; ++h) {
-                float d0 = dx[t][2 * h], d1 = dx[t][2 * h + 1];
-                if (d0 == 0.0f && d1 == 0.0f) continue;
-                int l = 8 * t + 2 * g + h;
-                float* level = a.table_grad + (size_t)l * a.gp.entries * 2;
-                for (int k = 0; k < 8; ++k) {
-                    uint32_t idx;
-                    float w;
-                    corner(a.gp, l, pts[b].p[0], pts[b].p[1], pts[b].p[2], k, idx, w);
-                    if (d0 != 0.0f) unsafeAtomicAdd(level + 2 * idx, w * d0);
-                    if (d1 != 0.0f) unsafeAtomicAdd(level + 2 * idx + 1, w * d1);
-                }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            float d0 = dx[m >> 1][2 * (m & 1)], d1 = dx[m >> 1][2 * (m & 1) + 1];
+            if (d0 == 0.0f && d1 == 0.0f) continue;
+            uint32_t c[3];
+            float w[3];
+            level_cell(L, m, pts[b].p, c, w);
+            float* level = a.table_grad + (size_t)L.offset[m] * 2;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                uint32_t idx;
+                float wt;
+                corner_of(L.stride[m], a.gp.mask, c, w, k, idx, wt);
+                if (d0 != 0.0f) unsafeAtomicAdd(level + 2 * idx, wt * d0);
+                if (d1 != 0.0f) unsafeAtomicAdd(level + 2 * idx + 1, wt * d1);
             }
+        }
     }
 }
 
 // Weight gradients: dW_k[o][i] = sum over samples of delta_k[o][s] a_{k-1}[i][s]
-// for one chunk of kChunkTiles tiles; one workgroup per (chunk, layer), wave w
-// owns output rows 16w..16w+15 (layer 6: column block w of row block 0).
+// over one chunk of kChunk samples; one workgroup per (chunk, layer). For
+// W1..W5, wave w owns output rows 16w..16w+15; for W6, column block w of
+// rows 0..15.
 __global__ void __launch_bounds__(256) k_nrc_wgrad(const __bf16* acts, const __bf16* deltas, uint32_t tiles,
                                                    float* partial)
 {
@@ -398,33 +505,55 @@ __global__ void __launch_bounds__(256) k_nrc_wgrad(const __bf16* acts, const __b
     const size_t layer_stride = (size_t)tiles * 64 * kTile;
     const __bf16* dl = deltas + (size_t)layer * layer_stride;
     const __bf16* al = acts + (size_t)layer * layer_stride;
-    const int row_tile = layer == 5 ? 0 : wave;
-    const int col_lo = layer == 5 ? wave : 0, col_hi = layer == 5 ? wave + 1 : 4;
-    f32x4 acc[4];
-    for (int c = 0; c < 4; ++c) acc[c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int tt = 0; tt < kChunkTiles; ++tt) {
-        uint32_t tile = chunk * kChunkTiles + (uint32_t)tt;
-        if (tile >= tiles) break;
-        const __bf16* dt = dl + (size_t)tile * 64 * kTile;
-        const __bf16* at = al + (size_t)tile * 64 * kTile;
-        for (int kb = 0; kb < kTile / 32; ++kb) {
-            int s0 = kb * 32 + 8 * g;
-            bf16x8 A = *reinterpret_cast<const bf16x8*>(dt + (size_t)(16 * row_tile + s16) * kTile + s0);
-            for (int c = col_lo; c < col_hi; ++c) {
-                bf16x8 B = *reinterpret_cast<const bf16x8*>(at + (size_t)(16 * c + s16) * kTile + s0);
-                acc[c] = mfma_bf16(A, B, acc[c]);
+    float* out = partial + (size_t)chunk * RGI_NRC_MLP_PARAMS + (size_t)layer * 4096;
+    if (layer < 5) {
+        f32x4 acc[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int tt = 0; tt < kChunkTiles; ++tt) {
+            uint32_t tile = chunk * kChunkTiles + (uint32_t)tt;
+            if (tile >= tiles) break;
+            const __bf16* dt = dl + (size_t)tile * 64 * kTile;
+            const __bf16* at = al + (size_t)tile * 64 * kTile;
+#pragma unroll
+            for (int kb = 0; kb < kTile / 32; ++kb) {
+                int s0 = kb * 32 + 8 * g;
+                bf16x8 A = *reinterpret_cast<const bf16x8*>(dt + (size_t)(16 * wave + s16) * kTile + s0);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    bf16x8 Bv = *reinterpret_cast<const bf16x8*>(at + (size_t)(16 * c + s16) * kTile + s0);
+                    acc[c] = mfma_bf16(A, Bv, acc[c]);
+                }
             }
         }
-    }
-    float* out = partial + (size_t)chunk * RGI_NRC_MLP_PARAMS + (size_t)layer * 4096;
-    for (int c = col_lo; c < col_hi; ++c)
-        for (int r = 0; r < 4; ++r) {
-            int o = 16 * row_tile + 4 * g + r, i = 16 * c + s16;
-            if (layer == 5 && o >= 3) continue;
-            out[o * 64 + i] = acc[c][r];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) out[(16 * wave + 4 * g + r) * 64 + 16 * c + s16] = acc[c][r];
+    } else {
+        f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int tt = 0; tt < kChunkTiles; ++tt) {
+            uint32_t tile = chunk * kChunkTiles + (uint32_t)tt;
+            if (tile >= tiles) break;
+            const __bf16* dt = dl + (size_t)tile * 64 * kTile;
+            const __bf16* at = al + (size_t)tile * 64 * kTile;
+#pragma unroll
+            for (int kb = 0; kb < kTile / 32; ++kb) {
+                int s0 = kb * 32 + 8 * g;
+                bf16x8 A = *reinterpret_cast<const bf16x8*>(dt + (size_t)s16 * kTile + s0);
+                bf16x8 Bv = *reinterpret_cast<const bf16x8*>(at + (size_t)(16 * wave + s16) * kTile + s0);
+                acc = mfma_bf16(A, Bv, acc);
+            }
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int o = 4 * g + r;
+            if (o < 3) out[o * 64 + 16 * wave + s16] = acc[r];
+        }
+    }
 }
 
+// Sum of the chunks' partial gradients, in chunk order.
 __global__ void k_nrc_reduce(const float* partial, uint32_t chunks, float* grad)
 {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -463,23 +592,26 @@ RGI_DEV float fwd_weight(const float* mlp, int block, int lane, int q)
     return s16 < 3 ? mlp[20480 + s16 * 64 + phi(j, g, q)] : 0.0f;
 }
 
+// The same for the transposed copy: W6^T (K = output row 4g + q, q < 4),
+// W5^T .. W2^T, then W1^T with row 16t + i holding input feature
+// 8(i / 4) + 4t + i % 4.
 RGI_DEV float bwd_weight(const float* mlp, int block, int lane, int q)
 {
     int g = lane >> 4, s16 = lane & 15;
-    if (block < 4) { // W6^T: row = layer-5 feature, K = output row 4g + q (q < 4)
+    if (block < 4) {
         int o = 4 * g + q;
         return (q < 4 && o < 3) ? mlp[20480 + o * 64 + 16 * block + s16] : 0.0f;
     }
-    if (block < 36) { // W_k^T, k = 5..2
+    if (block < 36) {
         int kk = (block - 4) / 8, t = ((block - 4) % 8) / 2, j = (block - 4) % 2;
         int k = 5 - kk; // 1-based layer
         return mlp[(k - 1) * 4096 + phi(j, g, q) * 64 + 16 * t + s16];
     }
-    int t = (block - 36) / 2, j = (block - 36) % 2; // W1^T rows 0..31
-    return mlp[phi(j, g, q) * 64 + 16 * t + s16];
+    int t = (block - 36) / 2, j = (block - 36) % 2;
+    int feature = 8 * (s16 >> 2) + 4 * t + (s16 & 3);
+    return mlp[phi(j, g, q) * 64 + feature];
 }
 
-// Layer of a forward / backward fragment (0-based W index).
 RGI_DEV int fwd_layer(int block) { return block < 40 ? block / 8 : 5; }
 RGI_DEV int bwd_layer(int block) { return block < 4 ? 5 : (block < 36 ? 4 - (block - 4) / 8 : 0); }
 
@@ -489,20 +621,20 @@ RGI_DEV uint8_t to_fp8(float x)
     return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
 }
 
-// One workgroup per weight matrix: Adam on its parameters, then its
-// fragments in every copy and its FP8 scales.
+// One workgroup per weight matrix: Adam on its parameters (when `step`), its
+// largest magnitude, and its FP8 weight and activation scales.
 __global__ void __launch_bounds__(256) k_nrc_adam_mlp(float* mlp, float* m, float* v, const float* grad, AdamArgs h,
-                                                      __bf16* w_fwd, __bf16* w_bwd, _Float16* w_fwd_h,
-                                                      uint8_t* w_fwd_8, float* scales, uint32_t* act_max, int step)
+                                                      float* scales, const uint32_t* act_max, int step)
 {
     __shared__ float red[4];
     const int layer = (int)blockIdx.x;
     const int lo = layer * 4096, hi = layer == 5 ? 20480 + 192 : lo + 4096;
     float wmax = 0.0f;
     for (int p = lo + (int)threadIdx.x; p < hi; p += 256) {
-        float mm = m[p], vv = v[p];
-        float w = step ? adam(h, grad[p], mm, vv, mlp[p]) : mlp[p];
+        float w = mlp[p];
         if (step) {
+            float mm = m[p], vv = v[p];
+            w = adam(h, grad[p], mm, vv, w);
             m[p] = mm;
             v[p] = vv;
             mlp[p] = w;
@@ -512,26 +644,34 @@ __global__ void __launch_bounds__(256) k_nrc_adam_mlp(float* mlp, float* m, floa
     wmax = wave_max(wmax);
     if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = wmax;
     __syncthreads();
-    wmax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    float sw = wmax > 0.0f ? 448.0f / wmax : 1.0f;
-    float amax = __uint_as_float(act_max[layer]);
-    float sa = amax > 0.0f ? 224.0f / amax : 1.0f;
     if (threadIdx.x == 0) {
+        wmax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        float sw = wmax > 0.0f ? 448.0f / wmax : 1.0f;
+        float amax = __uint_as_float(act_max[layer]);
+        float sa = amax > 0.0f ? 224.0f / amax : 1.0f;
         scales[layer] = sw;
         scales[8 + layer] = sa;
         scales[16 + layer] = 1.0f / (sa * sw);
     }
-    for (int e = (int)threadIdx.x; e < kFwdBlocks * kFrag; e += 256) {
+}
+
+// Every fragment slot of the forward (BF16, FP16, FP8) and transposed (BF16)
+// copies, one per thread.
+__global__ void k_nrc_copies(const float* mlp, const float* scales, __bf16* w_fwd, __bf16* w_bwd, _Float16* w_fwd_h,
+                             uint8_t* w_fwd_8)
+{
+    int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e < kFwdBlocks * kFrag) {
         int block = e / kFrag, lane = (e % kFrag) / 8, q = e % 8;
-        if (fwd_layer(block) != layer) continue;
         float w = fwd_weight(mlp, block, lane, q);
         w_fwd[e] = (__bf16)w;
         w_fwd_h[e] = (_Float16)w;
-        w_fwd_8[e] = to_fp8(w * sw);
+        w_fwd_8[e] = to_fp8(w * scales[fwd_layer(block)]);
+        return;
     }
-    for (int e = (int)threadIdx.x; e < kBwdBlocks * kFrag; e += 256) {
+    e -= kFwdBlocks * kFrag;
+    if (e < kBwdBlocks * kFrag) {
         int block = e / kFrag, lane = (e % kFrag) / 8, q = e % 8;
-        if (bwd_layer(block) != layer) continue;
         w_bwd[e] = (__bf16)bwd_weight(mlp, block, lane, q);
     }
 }
@@ -561,6 +701,7 @@ __global__ void k_nrc_adam_table(float* table, float* m, float* v, float* grad, 
     float4 vq = reinterpret_cast<float4*>(v)[i];
     float gs[4] = {gq.x, gq.y, gq.z, gq.w}, ps[4] = {pq.x, pq.y, pq.z, pq.w}, ms[4] = {mq.x, mq.y, mq.z, mq.w},
           vs[4] = {vq.x, vq.y, vq.z, vq.w};
+#pragma unroll
     for (int c = 0; c < 4; ++c)
         if (gs[c] != 0.0f) ps[c] = adam(h, gs[c], ms[c], vs[c], ps[c]);
     reinterpret_cast<float4*>(table)[i] = make_float4(ps[0], ps[1], ps[2], ps[3]);
@@ -624,26 +765,6 @@ __global__ void k_nrc_half_table(const float* table, uint32_t* table_h, size_t e
 
 // --------------------------------------------------------------- inference
 
-// Sources of query points: a record array, or the G-buffer's visible surfaces.
-struct RecordSource {
-    const float* records;
-    uint32_t count;
-    float* out;
-    RGI_DEV bool fetch(const Grid& gp, uint32_t s, Point& pt) const
-    {
-        float rec[12] = {0};
-        bool ok = s < count;
-        if (ok)
-            for (int c = 0; c < 12; ++c) rec[c] = records[(size_t)s * 12 + c];
-        pt = point_of(gp, rec);
-        return ok;
-    }
-    RGI_DEV void put(uint32_t s, const float* y) const
-    {
-        if (s < count) reinterpret_cast<float4*>(out)[s] = make_float4(y[0], y[1], y[2], 0.0f);
-    }
-};
-
 template <int P>
 struct Operand;
 template <>
@@ -659,6 +780,7 @@ struct Operand<RGI_NRC_FP16> {
     static RGI_DEV T pack(const float* e, float)
     {
         f16x8 v;
+#pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = (_Float16)e[q];
         return v;
     }
@@ -674,6 +796,7 @@ struct Operand<RGI_NRC_FP8> {
     static RGI_DEV T pack(const float* e, float scale)
     {
         uint64_t v = 0;
+#pragma unroll
         for (int q = 0; q < 8; ++q) v |= (uint64_t)to_fp8(e[q] * scale) << (8 * q);
         return (long)v;
     }
@@ -684,43 +807,46 @@ struct Operand<RGI_NRC_FP8> {
     static RGI_DEV f32x4 mma(T a, T b, f32x4 c) { return mfma_fp8(a, b, c); }
 };
 
-// The forward pass of 64 points per wave: Point pts[4] of this lane's samples
-// in, y[4] (rows 0..3 in lane group 0) out.
-template <int P>
-RGI_DEV void forward(const rgi_nrc& n, const Grid& gp, const float* scales, const Point* pts, f32x4* y)
+// The forward pass of 16 * B points per wave: this lane's points in, y[b]
+// (rows 0..3 in lane group 0) out.
+template <int P, int B>
+RGI_DEV void forward(const rgi_nrc& n, const Grid& gp, const LaneLevels& L, const Point* pts, f32x4* y)
 {
     typedef Operand<P> Op;
     const int lane = (int)(threadIdx.x & 63u), g = lane >> 4;
-    typename Op::T X[2][4];
-    float sa = P == RGI_NRC_FP8 ? scales[8] : 1.0f;
-    for (int b = 0; b < 4; ++b) {
+    typename Op::T X[2][B];
+    const float sa = P == RGI_NRC_FP8 ? n.scales[8] : 1.0f;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
         float e[8], x[8];
-        encode4(gp, n.table_h, pts[b].p[0], pts[b].p[1], pts[b].p[2], g, e);
-        if (g == 0) extras(pts[b], x);
-        else
-            for (int q = 0; q < 8; ++q) x[q] = 0.0f;
-        if (P != RGI_NRC_FP8) { // the BF16 / FP16 input is the rounded value
-            X[0][b] = Op::pack(e, 1.0f);
-            X[1][b] = Op::pack(x, 1.0f);
-        } else {
-            X[0][b] = Op::pack(e, sa);
-            X[1][b] = Op::pack(x, sa);
-        }
+        encode_lane(L, gp.mask, n.table_h, pts[b].p, e);
+        extras(pts[b], g, x);
+        X[0][b] = Op::pack(e, sa);
+        X[1][b] = Op::pack(x, sa);
     }
+#pragma unroll
     for (int k = 0; k < 5; ++k) {
-        f32x4 acc[4][4];
-        for (int b = 0; b < 4; ++b)
+        f32x4 acc[B][4];
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
             for (int t = 0; t < 4; ++t) acc[b][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
         for (int j = 0; j < 2; ++j)
+#pragma unroll
             for (int t = 0; t < 4; ++t) {
                 typename Op::T w = Op::weights(n, k * 8 + t * 2 + j, lane);
-                for (int b = 0; b < 4; ++b) acc[b][t] = Op::mma(w, X[j][b], acc[b][t]);
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc[b][t] = Op::mma(w, X[j][b], acc[b][t]);
             }
-        float deq = P == RGI_NRC_FP8 ? scales[16 + k] : 1.0f;
-        float next = P == RGI_NRC_FP8 ? scales[8 + k + 1] : 1.0f;
-        for (int b = 0; b < 4; ++b)
+        const float deq = P == RGI_NRC_FP8 ? n.scales[16 + k] : 1.0f;
+        const float next = P == RGI_NRC_FP8 ? n.scales[8 + k + 1] : 1.0f;
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
             for (int j = 0; j < 2; ++j) {
                 float e[8];
+#pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     float z0 = acc[b][2 * j][r] * deq, z1 = acc[b][2 * j + 1][r] * deq;
                     e[r] = z0 > 0.0f ? z0 : 0.0f;
@@ -729,29 +855,134 @@ RGI_DEV void forward(const rgi_nrc& n, const Grid& gp, const float* scales, cons
                 X[j][b] = Op::pack(e, next);
             }
     }
-    for (int b = 0; b < 4; ++b) y[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int b = 0; b < B; ++b) y[b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
     for (int j = 0; j < 2; ++j) {
         typename Op::T w = Op::weights(n, 40 + j, lane);
-        for (int b = 0; b < 4; ++b) y[b] = Op::mma(w, X[j][b], y[b]);
+#pragma unroll
+        for (int b = 0; b < B; ++b) y[b] = Op::mma(w, X[j][b], y[b]);
     }
-    if (P == RGI_NRC_FP8)
-        for (int b = 0; b < 4; ++b) y[b] = y[b] * scales[16 + 5];
+    if (P == RGI_NRC_FP8) {
+        const float deq = n.scales[16 + 5];
+#pragma unroll
+        for (int b = 0; b < B; ++b) y[b] = y[b] * deq;
+    }
 }
 
 template <int P>
-__global__ void __launch_bounds__(256) k_nrc_infer(rgi_nrc n, Grid gp, RecordSource src)
+__global__ void __launch_bounds__(256) k_nrc_infer(rgi_nrc n, Grid gp, const float* records, uint32_t count, float* out)
 {
+    constexpr int B = kInferB;
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u), g = lane >> 4, s16 = lane & 15;
-    Point pts[4];
-    uint32_t base = blockIdx.x * 256u + (uint32_t)wave * 64u;
-    for (int b = 0; b < 4; ++b) src.fetch(gp, base + (uint32_t)(b * 16 + s16), pts[b]);
-    f32x4 y[4];
-    forward<P>(n, gp, n.scales, pts, y);
+    const LaneLevels L = lane_levels(gp, g);
+    const uint32_t base = blockIdx.x * (64u * B) + (uint32_t)wave * (16u * B);
+    Point pts[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) pts[b] = load_point(gp, records, base + (uint32_t)(b * 16 + s16), count);
+    f32x4 y[B];
+    forward<P, B>(n, gp, L, pts, y);
     if (g != 0) return;
-    for (int b = 0; b < 4; ++b) {
-        float v[3] = {y[b][0], y[b][1], y[b][2]};
-        src.put(base + (uint32_t)(b * 16 + s16), v);
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        uint32_t s = base + (uint32_t)(b * 16 + s16);
+        if (s < count) reinterpret_cast<float4*>(out)[s] = make_float4(y[b][0], y[b][1], y[b][2], 0.0f);
     }
+}
+
+// A visible surface's inputs from the G-buffer: the reconstructed position,
+// the decoded surface normal and the diffuse albedo, albedo * (1 - metallic),
+// as the GI passes shade it. False for the sky.
+RGI_DEV bool gbuffer_point(const Grid& gp, const rgi_frame& f, const rgi_gbuffer& g, F2 texel, int x, int y,
+                           Point& pt)
+{
+    uint32_t t = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    float depth = g.depth[t];
+    F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+    F3 pos = reconstruct_position(f, uv, depth);
+    F4 nm = unorm8(g.normal[t]);
+    F3 n = normalize(F3{nm.x * 2.0f - 1.0f, nm.y * 2.0f - 1.0f, nm.z * 2.0f - 1.0f});
+    F4 al = unorm8(g.albedo[t]);
+    bool surface = depth < kSkyDepth;
+    float rec[12] = {pos.x, pos.y, pos.z, 0.0f, n.x, n.y, n.z, 0.0f, al.x * (1.0f - nm.w), al.y * (1.0f - nm.w),
+                     al.z * (1.0f - nm.w), 0.0f};
+    if (!surface)
+        for (int i = 0; i < 12; ++i) rec[i] = i == 6 ? 1.0f : 0.0f;
+    pt = point_of(gp, rec);
+    return surface;
+}
+
+// Every pixel of the G-buffer: a wave takes an 8 x 2B pixel block
+// (neighbouring pixels share the encoding's cells), rows 2b and 2b + 1 of
+// the block in column block b; a workgroup 16 x 4B pixels. Writes the
+// cache's radiance, clamped to >= 0, plus the G-buffer emission when
+// `emission` is set, as RGBA16F (alpha 1); sky pixels get 0.
+template <int P>
+__global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, rgi_frame f, rgi_gbuffer gb,
+                                                           F2 texel, const uint16_t* emission, uint16_t* out)
+{
+    constexpr int B = kInferB;
+    const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u), g = lane >> 4, s16 = lane & 15;
+    const LaneLevels L = lane_levels(gp, g);
+    const int x = (int)(blockIdx.x * 16u) + (wave & 1) * 8 + (s16 & 7);
+    Point pts[B];
+    bool surface[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        int y = (int)(blockIdx.y * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
+        surface[b] = gbuffer_point(gp, f, gb, texel, min(x, gb.width - 1), min(y, gb.height - 1), pts[b]);
+    }
+    f32x4 yv[B];
+    forward<P, B>(n, gp, L, pts, yv);
+    if (g != 0 || x >= gb.width) return;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        int y = (int)(blockIdx.y * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
+        if (y >= gb.height) continue;
+        uint32_t o = (uint32_t)y * (uint32_t)gb.width + (uint32_t)x;
+        F4 r = F4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (surface[b]) {
+            r = F4{fmaxf(yv[b][0], 0.0f), fmaxf(yv[b][1], 0.0f), fmaxf(yv[b][2], 0.0f), 1.0f};
+            if (emission) {
+                F4 e = load_h4(emission, o);
+                r = F4{r.x + e.x, r.y + e.y, r.z + e.z, 1.0f};
+            }
+        }
+        store_h4(out, o, r);
+    }
+}
+
+// Training records of one frame: `batch` pixels stratified over the image
+// with a random offset per record, each with its surface inputs (as the
+// G-buffer query computes them) and the target radiance di_diffuse + albedo
+// * color (either term optional); weight 0 for sky pixels.
+__global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_t* di_diffuse, const uint16_t* color,
+                             uint32_t batch, uint32_t seed, float* records, float* targets)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= batch) return;
+    uint64_t npix = (uint64_t)g.width * (uint64_t)g.height;
+    uint32_t h = pcg(i + pcg(seed));
+    uint64_t offset = ((uint64_t)(h >> 8) * npix) >> 24;
+    uint32_t p = (uint32_t)(((uint64_t)i * npix + offset) / batch);
+    int x = (int)(p % (uint32_t)g.width), y = (int)(p / (uint32_t)g.width);
+    uint32_t t = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    float depth = g.depth[t];
+    F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+    F3 pos = reconstruct_position(f, uv, depth);
+    F4 nm = unorm8(g.normal[t]);
+    F3 n = normalize(F3{nm.x * 2.0f - 1.0f, nm.y * 2.0f - 1.0f, nm.z * 2.0f - 1.0f});
+    F4 al = unorm8(g.albedo[t]);
+    F3 albedo = F3{al.x, al.y, al.z} * (1.0f - nm.w);
+    bool surface = depth < kSkyDepth;
+    F3 target = splat3(0.0f);
+    if (di_diffuse) target = target + xyz(load_h4(di_diffuse, t));
+    if (color) target = target + xyz(load_h4(color, t)) * albedo;
+    float4* r = reinterpret_cast<float4*>(records) + (size_t)i * 3;
+    r[0] = surface ? make_float4(pos.x, pos.y, pos.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r[1] = surface ? make_float4(n.x, n.y, n.z, 0.0f) : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    r[2] = surface ? make_float4(albedo.x, albedo.y, albedo.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    reinterpret_cast<float4*>(targets)[i] = make_float4(target.x, target.y, target.z, surface ? 1.0f : 0.0f);
 }
 
 } // namespace nrc
@@ -761,15 +992,50 @@ __global__ void __launch_bounds__(256) k_nrc_infer(rgi_nrc n, Grid gp, RecordSou
 namespace {
 
 using namespace nrc;
+Grid grid_of(const rgi_nrc& n);
+} // namespace
+
+int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* emission, int precision,
+                      uint16_t* out, hipStream_t stream)
+{
+    nrc::Grid gp = grid_of(*n);
+    F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 4 * nrc::kInferB - 1) / (4u * nrc::kInferB));
+    switch (precision) {
+    case RGI_NRC_BF16:
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_BF16>, blocks, dim3(256), 0, stream, *n, gp, f, g, texel,
+                           emission, out);
+        break;
+    case RGI_NRC_FP16:
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_FP16>, blocks, dim3(256), 0, stream, *n, gp, f, g, texel,
+                           emission, out);
+        break;
+    case RGI_NRC_FP8:
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_FP8>, blocks, dim3(256), 0, stream, *n, gp, f, g, texel,
+                           emission, out);
+        break;
+    default: return fail(RGI_E_INVALID, "nrc: unknown precision %d", precision);
+    }
+    return check_launch("nrc infer (G-buffer)");
+}
+
+int nrc_gather(const rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
+               const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
+               hipStream_t stream)
+{
+    (void)n;
+    F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    hipLaunchKernelGGL(nrc::k_nrc_gather, dim3((batch + 255) / 256), dim3(256), 0, stream, f, g, texel, di_diffuse,
+                       color, batch, seed, records, targets);
+    return check_launch("nrc gather");
+}
+
+namespace {
 
 Grid grid_of(const rgi_nrc& n)
 {
     Grid gp;
-    for (int l = 0; l < 16; ++l) {
-        gp.res[l] = n.res[l];
-        uint64_t s = (uint64_t)n.res[l] + 1;
-        gp.stride[l] = s * s * s <= n.entries ? (uint32_t)s : 0u;
-    }
+    gp.levels = n.levels;
     gp.mask = n.entries - 1;
     gp.entries = n.entries;
     for (int c = 0; c < 3; ++c) {
@@ -859,8 +1125,12 @@ int refresh_copies(rgi_nrc* n, int step, hipStream_t stream)
 {
     AdamArgs h = adam_args(*n, step ? n->steps : 1);
     hipLaunchKernelGGL(k_nrc_adam_mlp, dim3(6), dim3(256), 0, stream, n->mlp, n->mlp_m, n->mlp_v, n->mlp_grad, h,
-                       n->w_fwd, n->w_bwd, n->w_fwd_h, n->w_fwd_8, n->scales, n->act_max, step);
-    return check_launch("nrc adam (mlp)");
+                       n->scales, n->act_max, step);
+    int e = check_launch("nrc adam (mlp)");
+    if (e) return e;
+    hipLaunchKernelGGL(k_nrc_copies, dim3(((kFwdBlocks + kBwdBlocks) * kFrag + 255) / 256), dim3(256), 0, stream,
+                       n->mlp, n->scales, n->w_fwd, n->w_bwd, n->w_fwd_h, n->w_fwd_8);
+    return check_launch("nrc weight copies");
 }
 
 } // namespace
@@ -930,7 +1200,7 @@ int rgi_nrc_create(rgi_nrc** out, const rgi_nrc_config* config)
         (e = alloc(&n->mlp_grad, RGI_NRC_MLP_PARAMS)) || (e = alloc(&n->w_fwd, (size_t)kFwdBlocks * kFrag)) ||
         (e = alloc(&n->w_bwd, (size_t)kBwdBlocks * kFrag)) || (e = alloc(&n->w_fwd_h, (size_t)kFwdBlocks * kFrag)) ||
         (e = alloc(&n->w_fwd_8, (size_t)kFwdBlocks * kFrag)) || (e = alloc(&n->scales, 32)) ||
-        (e = alloc(&n->act_max, 8)) || (e = alloc(&n->loss, 1))) {
+        (e = alloc(&n->act_max, 8)) || (e = alloc(&n->loss, 1)) || (e = alloc(&n->levels, 32))) {
         rgi_nrc_destroy(n);
         return e;
     }
@@ -941,6 +1211,16 @@ int rgi_nrc_create(rgi_nrc** out, const rgi_nrc_config* config)
     (void)hipMemset(n->mlp_v, 0, RGI_NRC_MLP_PARAMS * 4);
     (void)hipMemset(n->mlp_grad, 0, RGI_NRC_MLP_PARAMS * 4);
     (void)hipMemset(n->act_max, 0, 32);
+    uint32_t levels[32];
+    for (int l = 0; l < 16; ++l) {
+        uint64_t st = (uint64_t)n->res[l] + 1;
+        levels[l] = n->res[l];
+        levels[16 + l] = st * st * st <= n->entries ? (uint32_t)st : 0u;
+    }
+    if (hipMemcpy(n->levels, levels, sizeof(levels), hipMemcpyHostToDevice) != hipSuccess) {
+        rgi_nrc_destroy(n);
+        return fail(RGI_E_DEVICE, "nrc_create: level table upload failed");
+    }
     (void)hipMemset(n->scales, 0, 128);
     size_t entries_total = n->table_floats / 2;
     hipLaunchKernelGGL(k_nrc_init_table, dim3((unsigned)((entries_total + 255) / 256)), dim3(256), 0, 0, n->table,
@@ -963,7 +1243,7 @@ void rgi_nrc_destroy(rgi_nrc* n)
     if (!n) return;
     (void)hipDeviceSynchronize();
     void* ptrs[] = {n->table, n->table_m, n->table_v, n->table_grad, n->table_h, n->mlp, n->mlp_m, n->mlp_v,
-                    n->mlp_grad, n->w_fwd, n->w_bwd, n->w_fwd_h, n->w_fwd_8, n->scales, n->act_max, n->loss,
+                    n->mlp_grad, n->w_fwd, n->w_bwd, n->w_fwd_h, n->w_fwd_8, n->scales, n->act_max, n->loss, n->levels,
                     n->acts, n->deltas, n->partial};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1009,13 +1289,18 @@ int rgi_nrc_infer(rgi_nrc* n, const float* records, uint32_t count, int precisio
     if (!n || !records || !out) return fail(RGI_E_INVALID, "nrc_infer: bad argument");
     if (count == 0) return RGI_OK;
     hipStream_t s = as_stream(stream);
-    RecordSource src{records, count, out};
     Grid gp = grid_of(*n);
-    dim3 blocks((count + 255) / 256);
+    dim3 blocks((count + 64 * kInferB - 1) / (64 * kInferB));
     switch (precision) {
-    case RGI_NRC_BF16: hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_BF16>, blocks, dim3(256), 0, s, *n, gp, src); break;
-    case RGI_NRC_FP16: hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP16>, blocks, dim3(256), 0, s, *n, gp, src); break;
-    case RGI_NRC_FP8: hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP8>, blocks, dim3(256), 0, s, *n, gp, src); break;
+    case RGI_NRC_BF16:
+        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_BF16>, blocks, dim3(256), 0, s, *n, gp, records, count, out);
+        break;
+    case RGI_NRC_FP16:
+        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP16>, blocks, dim3(256), 0, s, *n, gp, records, count, out);
+        break;
+    case RGI_NRC_FP8:
+        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP8>, blocks, dim3(256), 0, s, *n, gp, records, count, out);
+        break;
     default: return fail(RGI_E_INVALID, "nrc_infer: unknown precision %d", precision);
     }
     return check_launch("nrc infer");

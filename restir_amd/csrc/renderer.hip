@@ -44,6 +44,14 @@ struct rgi_renderer {
     // The graph form of a frame (rgi_renderer_render_graph).
     hipGraphExec_t graph_exec = nullptr;
     bool graph_warm = false;
+    // The neural radiance cache (rgi_renderer_set_nrc).
+    rgi_nrc* nrc = nullptr;
+    rgi_nrc_settings nrc_settings{};
+    uint16_t* nrc_plane = nullptr;
+    float* nrc_records = nullptr;
+    float* nrc_targets = nullptr;
+    float* nrc_loss = nullptr;
+    uint32_t nrc_trained = 0;
     int read = 0;              // m_historyRead / m_diRead: the parity last frame wrote
     bool history_valid = false;
     bool di_valid = false;
@@ -75,6 +83,17 @@ void release(T*& p)
     p = nullptr;
 }
 
+void release_nrc(rgi_renderer* r)
+{
+    if (r->nrc) rgi_nrc_destroy(r->nrc);
+    r->nrc = nullptr;
+    release(r->nrc_plane);
+    release(r->nrc_records);
+    release(r->nrc_targets);
+    release(r->nrc_loss);
+    r->nrc_trained = 0;
+}
+
 void free_all(rgi_renderer* r)
 {
     for (int k = 0; k < 2; ++k) {
@@ -97,6 +116,7 @@ void free_all(rgi_renderer* r)
         release(r->filtered[k]);
         release(r->filter[k]);
     }
+    release_nrc(r);
 }
 
 // kShadowFaceSizes (settings.h:26); 0 for an index out of range.
@@ -366,13 +386,26 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     // With the denoiser, the GI history colour is its first filtered
     // iteration (renderer.cpp:774-776).
     const uint16_t* history_color = s.denoiser ? r->filtered[read] : r->color[read];
+    // The neural radiance cache stands in for the surface light and last
+    // frame's indirect light of on-screen sample points once it has trained.
+    const bool nrc_on = r->nrc && !r->reference;
+    rgi_frame fg = f;
+    if (nrc_on && r->nrc_trained >= r->nrc_settings.warmup) {
+        const uint16_t* emission = (!s.restir_di && s.emissive_lighting) ? g->emission : nullptr;
+        if ((e = nrc_infer_gbuffer(r->nrc, f, *g, emission, r->nrc_settings.precision, r->nrc_plane, stream)) !=
+            RGI_OK)
+            return e;
+        surface_light = r->nrc_plane;
+        fg.multiple_bounces = 0;
+        fg.lighting_surface = 1.0f;
+    }
     rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                       surface_light, shadow_map, r->grid[read]};
     rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
-    if ((e = gi_temporal(f, r->scene, *g, tin, res, r->ctx->tuning, stream)) != RGI_OK) return e;
+    if ((e = gi_temporal(fg, r->scene, *g, tin, res, r->ctx->tuning, stream)) != RGI_OK) return e;
     rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                       surface_light, shadow_map, r->grid[write]};
-    if ((e = gi_spatial(f, r->scene, *g, sin, res, r->color[write], r->moments[write], r->ctx->tuning, stream)) !=
+    if ((e = gi_spatial(fg, r->scene, *g, sin, res, r->color[write], r->moments[write], r->ctx->tuning, stream)) !=
         RGI_OK)
         return e;
 
@@ -383,7 +416,7 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         const int sw = 1 - r->specular_read;
         rgi_gi_inputs spin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                            surface_light, shadow_map, r->grid[write]};
-        if ((e = specular_gi(f, r->scene, *g, spin, r->hdr, r->specular_gi[r->specular_read], r->specular_gi[sw],
+        if ((e = specular_gi(fg, r->scene, *g, spin, r->hdr, r->specular_gi[r->specular_read], r->specular_gi[sw],
                              r->ctx->tuning, stream)) != RGI_OK)
             return e;
         r->specular_read = sw;
@@ -406,6 +439,20 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
             if ((e = svgf_atrous(f, input, receiver, normal, 1 << it, output, stream)) != RGI_OK) return e;
             r->denoised = output;
         }
+    }
+
+    // One training step of the cache on this frame's surfaces: target = DI
+    // diffuse + albedo * next frame's GI history colour.
+    if (nrc_on) {
+        const uint16_t* next_history = s.denoiser ? r->filtered[write] : r->color[write];
+        if ((e = nrc_gather(r->nrc, f, *g, di_active ? r->diffuse : nullptr, s.multiple_bounces ? next_history : nullptr,
+                            r->nrc_settings.batch, f.frame_index ^ 0x4e5243u, r->nrc_records, r->nrc_targets,
+                            stream)) != RGI_OK)
+            return e;
+        if ((e = rgi_nrc_train(r->nrc, r->nrc_records, r->nrc_targets, r->nrc_settings.batch, r->nrc_loss, stream)) !=
+            RGI_OK)
+            return e;
+        ++r->nrc_trained;
     }
 
     // Composite (renderer.cpp:967-990): the final image's weights.
@@ -552,6 +599,51 @@ int rgi_renderer_outputs_get(const rgi_renderer* r, rgi_renderer_outputs* out)
     out->denoised = r->denoised ? r->denoised : r->color[k];
     out->hdr = r->hdr;
     out->specular_gi = r->specular_gi[0] ? r->specular_gi[r->specular_read] : nullptr;
+    return RGI_OK;
+}
+
+void rgi_nrc_settings_default(rgi_nrc_settings* s)
+{
+    if (!s) return;
+    memset(s, 0, sizeof(*s));
+    rgi_nrc_config_default(&s->cache);
+    s->batch = 65536;
+    s->warmup = 4;
+    s->precision = RGI_NRC_BF16;
+}
+
+int rgi_renderer_set_nrc(rgi_renderer* r, const rgi_nrc_settings* settings)
+{
+    if (!r) return fail(RGI_E_INVALID, "renderer is NULL");
+    if (settings && (settings->batch == 0 || settings->batch > (1u << 24)))
+        return fail(RGI_E_INVALID, "set_nrc: batch %u is not in 1..2^24", settings ? settings->batch : 0u);
+    if (settings && (settings->precision < RGI_NRC_BF16 || settings->precision > RGI_NRC_FP8))
+        return fail(RGI_E_INVALID, "set_nrc: unknown precision %d", settings->precision);
+    (void)hipDeviceSynchronize();
+    release_nrc(r);
+    if (!settings) return RGI_OK;
+    int e = rgi_nrc_create(&r->nrc, &settings->cache);
+    if (e != RGI_OK) return e;
+    r->nrc_settings = *settings;
+    size_t n = (size_t)r->width * r->height;
+    if (!alloc(r->nrc_plane, 4 * n) || !alloc(r->nrc_records, (size_t)12 * settings->batch) ||
+        !alloc(r->nrc_targets, (size_t)4 * settings->batch) || !alloc(r->nrc_loss, 1)) {
+        release_nrc(r);
+        return fail(RGI_E_NOMEM, "set_nrc: buffers could not be allocated");
+    }
+    // A changed pass set: the next graph frame instantiates afresh.
+    r->graph_warm = false;
+    return RGI_OK;
+}
+
+int rgi_renderer_nrc_state(const rgi_renderer* r, rgi_nrc** cache, const uint16_t** radiance, const float** loss,
+                           uint32_t* trained)
+{
+    if (!r) return fail(RGI_E_INVALID, "renderer is NULL");
+    if (cache) *cache = r->nrc;
+    if (radiance) *radiance = r->nrc ? r->nrc_plane : nullptr;
+    if (loss) *loss = r->nrc ? r->nrc_loss : nullptr;
+    if (trained) *trained = r->nrc_trained;
     return RGI_OK;
 }
 

@@ -94,3 +94,23 @@ class NeuralCache:
         s = ra.U32()
         ra.check(self.L.rgi_nrc_steps(self.handle, ctypes.byref(s)))
         return s.value
+
+
+def renderer_settings(hs, batch=65536, warmup=4, precision=ra.RGI_NRC_BF16, margin=0.02, **kw):
+    """rgi_nrc_settings for a HostScene: the cache's bounds are the scene's
+    triangle bounds, widened by `margin` of their extent."""
+    st = ra.NrcSettings()
+    ra.lib().rgi_nrc_settings_default(ctypes.byref(st))
+    p = hs.positions.reshape(-1, 3)
+    lo, hi = p.min(axis=0), p.max(axis=0)
+    pad = (hi - lo) * margin + 1e-3
+    st.cache = config(aabb_min=lo - pad, aabb_max=hi + pad, **kw)
+    st.batch, st.warmup, st.precision = batch, warmup, precision
+    return st
+
+
+def renderer_state(rend):
+    """(cache handle, radiance plane pointer, loss pointer, trained frames) of a renderer."""
+    c, r, l, t = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p(), ra.U32()
+    ra.check(ra.lib().rgi_renderer_nrc_state(rend, ctypes.byref(c), ctypes.byref(r), ctypes.byref(l), ctypes.byref(t)))
+    return c.value, r.value, l.value, t.value

@@ -4,7 +4,11 @@ as written: mixed-precision FMAs with an fp16 result (a multiply-add fused
 into the fp16 conversion, one rounding where the source asks for an fp32
 rounding first), the non-fused mixed mad, and packed FP32 math.
 v_fma_mix_f32 stays allowed: it is an fp32 fma whose fp16 inputs are
-converted exactly, the same result as the explicit fmaf it comes from."""
+converted exactly, the same result as the explicit fmaf it comes from.
+
+Every kernel also ends in s_endpgm: undefined behaviour in a kernel (a shift
+by the operand's width or more, say) lets the compiler drop the whole body,
+and a body without an end runs into whatever code follows it."""
 import concurrent.futures
 import os
 import re
@@ -37,3 +41,8 @@ def test_no_fused_or_packed_fp32_in_kernels():
         code = [l for l in text.splitlines() if not l.lstrip().startswith((";", "."))]
         bad = [l.strip() for l in code if FORBIDDEN.search(l)]
         assert not bad, "%s: %s" % (src, bad[:5])
+        kernels = re.findall(r"^\s*\.amdhsa_kernel (\S+)", text, re.M)
+        for k in kernels:
+            start = text.index("\n" + k + ":")
+            end = text.index(".Lfunc_end", start)
+            assert "s_endpgm" in text[start:end], "%s: kernel %s has no s_endpgm" % (src, k)
