@@ -13,6 +13,12 @@ on G-buffers already resident in HBM. The separate traversal benchmark traces
 2^24 cosine rays from uniform surface points (closest hit, tmax 4 x scene
 scale) and reports Mrays/s.
 
+Beside the metric it reports the neural radiance cache: the metric's frame
+with the cache on, its kernels alone (2M per-pixel queries in BF16, FP16 and
+FP8; one training step of 65536 records) with the MLP's matrix-core
+throughput, and the GI and composite MSE of the plain path and the cache
+against the same 4096-frame reference.
+
 Prints one JSON line. --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -128,6 +134,7 @@ def measure_mse(L, hs, gscene, gbuf, view, W, H, T, mode, ref_frames, frames, ct
     import torch
     n = W * H
     acc = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    acc_gi = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
     out = ra.RendererOutputs()
     res = torch.zeros(5, dtype=torch.float64, device="cuda")
     results = {}
@@ -142,6 +149,7 @@ def measure_mse(L, hs, gscene, gbuf, view, W, H, T, mode, ref_frames, frames, ct
                 ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
                 ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
                 ra.check(L.rgi_reference_accumulate_composite(ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), s))
+                ra.check(L.rgi_reference_accumulate(ctypes.byref(gbuf.gb), out.color, None, None, acc_gi.data_ptr(), s))
             ra.check(L.rgi_renderer_set_reference_mode(rend, 0))
             L.rgi_renderer_invalidate_history(rend)
         for _ in range(frames):
@@ -158,7 +166,123 @@ def measure_mse(L, hs, gscene, gbuf, view, W, H, T, mode, ref_frames, frames, ct
                  "method": "ReSTIR GI + DI, indirect specular, SVGF denoiser (reference defaults), composite; no TAA",
                  "image": "composite HDR (fs_composite.sc), static camera, geometry pixels",
                  "no_denoiser": results["no_denoiser"]})
-    return line
+    return line, acc, acc_gi
+
+
+NRC_FLOP_PER_QUERY = 2 * (5 * 64 * 64 + 3 * 64)  # the MLP's multiply-adds per query (logical shapes)
+
+
+def nrc_quality(L, hs, gscene, gbuf, view, W, H, T, mode, acc, acc_gi, ref_frames, frames, ctx, s):
+    """The neural radiance cache against the plain path, each rendered
+    `frames` frames from an empty history at the static camera with the
+    reference's default settings, measured against the same `ref_frames`
+    reference accumulations: the GI estimate alone (the GI frame's output,
+    colour x albedo) and the composite."""
+    import torch
+    from restir_amd import nrc as rnrc
+    res = torch.zeros(5, dtype=torch.float64, device="cuda")
+    out = ra.RendererOutputs()
+    result = {"frames": frames, "reference_frames": ref_frames}
+    for name in ("plain", "cache"):
+        settings = ra.Settings(grid_table_size=T, grid_mode=mode)
+        rend = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                       ctypes.byref(gscene.scene), hs.radius))
+        if name == "cache":
+            ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs))))
+        for _ in range(frames):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+        line = {}
+        ra.check(L.rgi_reference_error(ctx, ctypes.byref(gbuf.gb), out.color, None, None, acc_gi.data_ptr(),
+                                       ref_frames, res.data_ptr(), s))
+        r = res.cpu().numpy()
+        line["gi"] = {"mse_rgb": float(r[1]), "relative_mse": float(r[0]), "mean_luminance": float(r[2]),
+                      "reference_mean_luminance": float(r[3])}
+        ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), ref_frames,
+                                                 res.data_ptr(), s))
+        r = res.cpu().numpy()
+        line["composite"] = {"mse_rgb": float(r[1]), "relative_mse": float(r[0])}
+        result[name] = line
+        L.rgi_renderer_destroy(rend)
+    result["gi_mse_plain_over_cache"] = round(result["plain"]["gi"]["mse_rgb"] / result["cache"]["gi"]["mse_rgb"], 4)
+    return result
+
+
+def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, steps, plain_ms, s):
+    """Cost of the cache: the metric's frame with the cache on (same path,
+    same settings), and its kernels alone with HIP events: 2M queries per
+    precision (the 1080p frame's per-pixel query count) and one training
+    step of 65536 records, with the MLP's matrix-core throughput."""
+    import torch
+    from restir_amd import nrc as rnrc
+    rend = ctypes.c_void_p()
+    ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                   ctypes.byref(gscene.scene), hs.radius))
+    ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs))))
+    for k in range(16):
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[k % frames_in_path]),
+                                       ctypes.byref(gbufs[k % frames_in_path].gb), s))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[k % frames_in_path]),
+                                       ctypes.byref(gbufs[k % frames_in_path].gb), s))
+    torch.cuda.synchronize()
+    frame_ms = (time.perf_counter() - t0) * 1e3 / steps
+    L.rgi_renderer_destroy(rend)
+
+    # The kernels alone, on the frame's own data: the per-pixel query of the
+    # first camera's G-buffer and one training step on records gathered from
+    # it (DI diffuse and GI colour planes of a rendered frame).
+    st = rnrc.renderer_settings(hs)
+    cache = rnrc.NeuralCache(st.cache)
+    plain = ctypes.c_void_p()
+    ra.check(L.rgi_renderer_create(ctypes.byref(plain), W, H, ctypes.byref(ra.c_settings(settings)),
+                                   ctypes.byref(gscene.scene), hs.radius))
+    for k in range(4):
+        ra.check(L.rgi_renderer_render(plain, ctypes.byref(views[0]), ctypes.byref(gbufs[0].gb), s))
+    outs = ra.RendererOutputs()
+    ra.check(L.rgi_renderer_outputs_get(plain, ctypes.byref(outs)))
+    frame = ra.Frame()
+    ra.check(L.rgi_renderer_last_frame(plain, ctypes.byref(frame)))
+    batch = st.batch
+    train_rec = torch.empty((batch, 12), dtype=torch.float32, device="cuda")
+    train_tgt = torch.empty((batch, 4), dtype=torch.float32, device="cuda")
+    radiance = torch.empty(W * H * 4, dtype=torch.float16, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def gather():
+        ra.check(L.rgi_nrc_gather_gbuffer(ctypes.byref(frame), ctypes.byref(gbufs[0].gb), outs.di_diffuse, outs.color,
+                                          batch, 7, train_rec.data_ptr(), train_tgt.data_ptr(), s))
+
+    gather_ms = timed(gather)
+    train_ms = timed(lambda: cache.train(train_rec, train_tgt))
+    queries = W * H
+    infer_ms = {}
+    for name, prec in (("bf16", ra.RGI_NRC_BF16), ("fp16", ra.RGI_NRC_FP16), ("fp8", ra.RGI_NRC_FP8)):
+        infer_ms[name] = timed(lambda: ra.check(L.rgi_nrc_query_gbuffer(cache.handle, ctypes.byref(frame),
+                                                                        ctypes.byref(gbufs[0].gb), None, prec,
+                                                                        radiance.data_ptr(), s)))
+    L.rgi_renderer_destroy(plain)
+    cache.close()
+    tflops = queries * NRC_FLOP_PER_QUERY / (infer_ms["bf16"] * 1e-3) / 1e12
+    return {"frame_ms_with_cache": round(frame_ms, 5), "frame_ms_plain": round(plain_ms, 5),
+            "gather_ms": round(gather_ms, 5), "train_ms": round(train_ms, 5), "train_batch": batch,
+            "query_ms": {k: round(v, 5) for k, v in infer_ms.items()}, "queries": queries,
+            "mfma": {"kernel": "k_nrc_infer_gbuffer (bf16)", "achieved": round(tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 6), "flop_per_query": NRC_FLOP_PER_QUERY,
+                     "bound": "hash-table gathers (L1 address processing), not the matrix cores"}}
 
 
 def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames, distance=3.17):
@@ -341,6 +465,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="run one 256 MiB device copy first")
     ap.add_argument("--atrium-steps", type=int, default=32, help="timed frames of the atrium workload (0: skip)")
     ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the uneven batch workload")
+    ap.add_argument("--no-nrc", dest="nrc", action="store_false", help="skip the neural radiance cache measurements")
+    ap.add_argument("--nrc-frames", type=int, default=256, help="frames rendered before the cache's MSE is measured")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -487,9 +613,16 @@ def main():
         roofline["traffic_detail"] = measured
 
     mse = None
+    nrc = None
+    if args.nrc:
+        nrc = nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, args.frames, 64, ms_per_frame, s)
     if args.reference_frames > 0:
-        mse = measure_mse(L, hs, gscene, gbufs[0], views[0], W, H, T, mode, args.reference_frames, args.mse_frames,
-                          drv.ctx, s)
+        mse, acc, acc_gi = measure_mse(L, hs, gscene, gbufs[0], views[0], W, H, T, mode, args.reference_frames,
+                                       args.mse_frames, drv.ctx, s)
+        if args.nrc:
+            nrc["quality"] = nrc_quality(L, hs, gscene, gbufs[0], views[0], W, H, T, mode, acc, acc_gi,
+                                         args.reference_frames, args.nrc_frames, drv.ctx, s)
+        del acc, acc_gi
 
     cpu = None
     if not args.no_cpu:
@@ -547,6 +680,7 @@ def main():
         "cpu_baseline": cpu,
         "atrium_1080p": atrium,
         "uneven_batch": uneven,
+        "neural_radiance_cache": nrc,
     }
     print(json.dumps(line), flush=True)
 

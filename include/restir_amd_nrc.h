@@ -118,6 +118,23 @@ int rgi_nrc_set_params(rgi_nrc* nrc, const float* mlp, const float* table, void*
 /* Adam steps taken since creation or the last rgi_nrc_set_params. */
 int rgi_nrc_steps(const rgi_nrc* nrc, uint32_t* steps);
 
+/* The cache at every pixel of a G-buffer: the radiance of each visible
+ * surface (position reconstructed from the depth with frame->inv_view_proj,
+ * surface normal, albedo * (1 - metallic)), clamped to >= 0, plus the
+ * G-buffer emission when `emission` is not NULL, into out_radiance as
+ * RGBA16F with alpha 1; sky pixels get 0. This is the frame driver's query. */
+int rgi_nrc_query_gbuffer(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuffer* gbuffer, const uint16_t* emission,
+                          int precision, uint16_t* out_radiance, void* stream);
+
+/* Training records of a G-buffer, as the frame driver gathers them: `batch`
+ * pixels stratified over the image (record i from pixel floor((i * N + u_i) /
+ * batch), N pixels, u_i in [0, N) from `seed`), each with its surface inputs
+ * and the target di_diffuse + albedo * color (either plane may be NULL);
+ * weight 0 for sky pixels. records[12 * batch], targets[4 * batch]. */
+int rgi_nrc_gather_gbuffer(const rgi_frame* frame, const rgi_gbuffer* gbuffer, const uint16_t* di_diffuse,
+                           const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
+                           void* stream);
+
 /* ---------------------------------------------------------------------
  * The cache in the frame driver (include/restir_amd_renderer.h).
  *

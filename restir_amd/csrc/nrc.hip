@@ -1338,6 +1338,24 @@ int rgi_nrc_set_params(rgi_nrc* n, const float* mlp, const float* table, void* s
     return refresh_copies(n, 0, s);
 }
 
+int rgi_nrc_query_gbuffer(rgi_nrc* n, const rgi_frame* f, const rgi_gbuffer* g, const uint16_t* emission,
+                          int precision, uint16_t* out, void* stream)
+{
+    if (!n || !f || !g || !out) return fail(RGI_E_INVALID, "nrc_query_gbuffer: NULL argument");
+    if (g->width <= 0 || g->height <= 0 || !g->depth || !g->normal || !g->albedo)
+        return fail(RGI_E_INVALID, "nrc_query_gbuffer: depth, normal and albedo planes are required");
+    return nrc_infer_gbuffer(n, *f, *g, emission, precision, out, as_stream(stream));
+}
+
+int rgi_nrc_gather_gbuffer(const rgi_frame* f, const rgi_gbuffer* g, const uint16_t* di_diffuse, const uint16_t* color,
+                           uint32_t batch, uint32_t seed, float* records, float* targets, void* stream)
+{
+    if (!f || !g || !records || !targets || batch == 0) return fail(RGI_E_INVALID, "nrc_gather_gbuffer: bad argument");
+    if (g->width <= 0 || g->height <= 0 || !g->depth || !g->normal || !g->albedo)
+        return fail(RGI_E_INVALID, "nrc_gather_gbuffer: depth, normal and albedo planes are required");
+    return nrc_gather(nullptr, *f, *g, di_diffuse, color, batch, seed, records, targets, as_stream(stream));
+}
+
 int rgi_nrc_steps(const rgi_nrc* n, uint32_t* steps)
 {
     if (!n || !steps) return fail(RGI_E_INVALID, "nrc_steps: bad argument");

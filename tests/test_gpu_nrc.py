@@ -293,3 +293,46 @@ def test_renderer_cache_beats_the_surface_light_it_replaces():
         L.rgi_renderer_destroy(plain)
         L.rgi_renderer_destroy(cached)
         L.rgi_destroy(ctx)
+
+
+def test_gbuffer_query_matches_records_query():
+    """The per-pixel query reads a pixel's inputs exactly as the training
+    gather records them: gathering every pixel (batch = pixel count puts
+    record i on pixel i) and querying those records gives the G-buffer
+    query's radiance, bit for bit after its clamp and FP16 store; the
+    records query is itself checked against the oracle above."""
+    from restir_amd import driver
+    W, H = 320, 240
+    n = W * H
+    st = ra.Settings(grid_table_size=1 << 16)
+    hs, gscene, ctx, gbuf, view = _setup_scene(W, H, st)
+    L = ra.lib()
+    s = driver.stream()
+    eye, vp, inv = ra.orbit_camera(hs, 0, W, H, yaw_of=ra.swing_yaw)
+    frame = ra.make_frame(st, hs, 0, eye, vp, inv, eye, vp, W, H, False)
+    cfg = rnrc.renderer_settings(hs).cache
+    nc = rnrc.NeuralCache(cfg)
+    rng = np.random.default_rng(21)
+    mlp = np.empty(ra.RGI_NRC_MLP_PARAMS, np.float32)
+    mlp[:20480] = rng.uniform(-1, 1, 20480) * np.sqrt(6.0 / 128.0)
+    mlp[20480:] = rng.uniform(-1, 1, 192) * np.sqrt(6.0 / 67.0)
+    table = rng.uniform(-0.5, 0.5, nc.table_floats).astype(np.float32)
+    nc.set_params(torch.from_numpy(mlp).cuda(), torch.from_numpy(table).cuda())
+    rec = torch.empty((n, 12), dtype=torch.float32, device="cuda")
+    tgt = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+    ra.check(L.rgi_nrc_gather_gbuffer(ctypes.byref(frame), ctypes.byref(gbuf.gb), None, None, n, 5, rec.data_ptr(),
+                                      tgt.data_ptr(), s))
+    for prec in (ra.RGI_NRC_BF16, ra.RGI_NRC_FP16, ra.RGI_NRC_FP8):
+        print("gbuffer query precision", prec, flush=True)
+        rad = torch.empty(n * 4, dtype=torch.float16, device="cuda")
+        ra.check(L.rgi_nrc_query_gbuffer(nc.handle, ctypes.byref(frame), ctypes.byref(gbuf.gb), None, prec,
+                                         rad.data_ptr(), s))
+        y = nc.infer(rec, prec)
+        surface = tgt[:, 3] > 0
+        want = torch.where(surface[:, None], torch.clamp(y[:, :3], min=0.0), torch.zeros_like(y[:, :3])).half()
+        got = rad.view(n, 4)
+        assert torch.equal(got[:, :3].view(torch.int16), want.view(torch.int16)), prec
+        assert torch.equal(got[:, 3], surface.half())
+    assert int(surface.sum()) > n // 10  # the box covers about a fifth of this view
+    nc.close()
+    L.rgi_destroy(ctx)
