@@ -373,50 +373,87 @@ def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames,
             "drv": drv, "rend": rend, "gbufs": gbufs, "views": views}
 
 
-def run_uneven_batch(mode, steps, warmup):
-    """Scenes of uneven size and resolution as one batch frame: every
-    renderer rendered eagerly one after another on one stream, then all of
-    them as one graph (rgi_batch_render), each on a stream of its own."""
+UNEVEN_SCENES = [
+    # name, kind, size, seed, width, height, camera distance (scene radii)
+    ("cornell_s1", ra.RGI_SCENE_CORNELL, 0, 1, 640, 360, 3.17),
+    ("cornell_s2", ra.RGI_SCENE_CORNELL, 0, 2, 960, 540, 3.17),
+    ("boxfield_1k", ra.RGI_SCENE_BOXFIELD, 84, 1, 800, 450, 3.17),
+    ("cornell_s3", ra.RGI_SCENE_CORNELL, 0, 3, 1280, 720, 3.17),
+    ("boxfield_10k", ra.RGI_SCENE_BOXFIELD, 840, 1, 1280, 720, 3.17),
+    ("atrium_128k", ra.RGI_SCENE_ATRIUM, 131072, 1, 640, 360, 0.7),
+    ("boxfield_25k", ra.RGI_SCENE_BOXFIELD, 2100, 1, 1920, 1080, 3.17),
+    ("atrium_64k", ra.RGI_SCENE_ATRIUM, 65536, 1, 1280, 720, 0.7),
+    ("boxfield_50k", ra.RGI_SCENE_BOXFIELD, 4200, 1, 1920, 1080, 3.17),
+    ("boxfield_100k", ra.RGI_SCENE_BOXFIELD, 8400, 1, 960, 540, 3.17),
+    ("atrium_256k", ra.RGI_SCENE_ATRIUM, 262144, 1, 1920, 1080, 0.7),
+    ("boxfield_500k", ra.RGI_SCENE_BOXFIELD, 42000, 1, 1600, 900, 3.17),
+    ("atrium_512k", ra.RGI_SCENE_ATRIUM, 524288, 1, 1600, 900, 0.7),
+    ("cornell_s4", ra.RGI_SCENE_CORNELL, 0, 4, 2560, 1440, 3.17),
+    ("boxfield_1m", ra.RGI_SCENE_BOXFIELD, 83334, 1, 2560, 1440, 3.17),
+    ("atrium_1m", ra.RGI_SCENE_ATRIUM, 1000000, 1, 2560, 1440, 0.7),
+]
+
+
+def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offsets=None):
+    """Renderers of `cases` as one batch frame: each renderer alone (its own
+    frame time), all of them eagerly one after another on one stream, then
+    all of them as one graph (rgi_batch_render), each on a stream of its
+    own. With `nrc`, every renderer has its own neural radiance cache."""
     import torch
+    from restir_amd import nrc as rnrc
     L = ra.lib()
-    cases = [("cornell", ra.RGI_SCENE_CORNELL, 0, 640, 360, 3.17), ("boxfield_722", ra.RGI_SCENE_BOXFIELD, 60, 1280, 720, 3.17),
-             ("atrium_256k", ra.RGI_SCENE_ATRIUM, 262144, 1920, 1080, 0.7),
-             ("boxfield_12k", ra.RGI_SCENE_BOXFIELD, 1000, 960, 540, 3.17),
-             ("cornell", ra.RGI_SCENE_CORNELL, 0, 2560, 1440, 3.17),
-             ("boxfield_120k", ra.RGI_SCENE_BOXFIELD, 10000, 800, 450, 3.17)]
     ctx = ctypes.c_void_p()
     ra.check(L.rgi_create(ctypes.byref(ctx)))
     es = driver.stream()
-    rends, frames, keep = [], [], []
-    for name, kind, size, w, h, dist in cases:
-        hs = ra.HostScene(kind, seed=1, size=size, extra_lights=4, light_seed=1)
+    scenes = {}
+    rends, frames, names, pixels, tris = [], [], [], [], []
+    for c, (name, kind, size, seed, w, h, dist) in enumerate(cases):
+        key = (kind, size, seed)
+        if key not in scenes:
+            hs = ra.HostScene(kind, seed=seed, size=size, extra_lights=4 if kind != ra.RGI_SCENE_ATRIUM else 64,
+                              light_seed=1)
+            scenes[key] = (hs, driver.GpuScene(hs))
+        hs, gscene = scenes[key]
         table = 1 << 20 if w * h <= (1 << 21) else 1 << 21
         st = ra.Settings(grid_table_size=table, grid_mode=mode, denoiser=False, composite=False, specular_gi=False)
-        gscene = driver.GpuScene(hs)
+        yaw0 = yaw_offsets[c] if yaw_offsets else 0.0
         views, gbs = [], []
-        for i in range(4):
-            eye, vp, inv = ra.orbit_camera(hs, i, w, h, yaw_of=ra.swing_yaw, distance=dist)
+        for i in range(views_per_case):
+            eye, vp, inv = ra.orbit_camera(hs, i, w, h, yaw_of=lambda k: ra.swing_yaw(k) + yaw0, distance=dist)
             f = ra.make_frame(st, hs, i, eye, vp, inv, eye, vp, w, h, False)
             gbs.append(driver.GBufferSet(ctx, gscene, f))
             views.append(ra.make_view(eye, vp, inv))
         r = ctypes.c_void_p()
-        ra.check(L.rgi_renderer_create(ctypes.byref(r), w, h, ctypes.byref(ra.c_settings(st)), ctypes.byref(gscene.scene),
-                                       hs.radius))
+        ra.check(L.rgi_renderer_create(ctypes.byref(r), w, h, ctypes.byref(ra.c_settings(st)),
+                                       ctypes.byref(gscene.scene), hs.radius))
+        if nrc:
+            ra.check(L.rgi_renderer_set_nrc(r, ctypes.byref(rnrc.renderer_settings(hs))))
         rends.append(r)
         frames.append((views, gbs))
-        keep.append(gscene)
+        names.append("%s %dx%d" % (name, w, h))
+        pixels.append(w * h)
+        tris.append(hs.triangle_count)
     torch.cuda.synchronize()
 
-    def eager(k):
-        for r, (views, gbs) in zip(rends, frames):
-            ra.check(L.rgi_renderer_render(r, ctypes.byref(views[k % 4]), ctypes.byref(gbs[k % 4].gb), es))
+    def one(r, views, gbs, k):
+        ra.check(L.rgi_renderer_render(r, ctypes.byref(views[k % views_per_case]),
+                                       ctypes.byref(gbs[k % views_per_case].gb), es))
 
-    for k in range(warmup):
-        eager(k)
+    per_scene = []
+    for r, (views, gbs) in zip(rends, frames):
+        for k in range(warmup):
+            one(r, views, gbs, k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            one(r, views, gbs, k)
+        torch.cuda.synchronize()
+        per_scene.append((time.perf_counter() - t0) * 1e3 / steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
-        eager(k)
+        for r, (views, gbs) in zip(rends, frames):
+            one(r, views, gbs, k)
     torch.cuda.synchronize()
     eager_ms = (time.perf_counter() - t0) * 1e3 / steps
 
@@ -427,8 +464,8 @@ def run_uneven_batch(mode, steps, warmup):
     bs = ctypes.c_void_p(side.cuda_stream)
 
     def batched(k):
-        va = (ra.View * len(rends))(*[v[k % 4] for v, _ in frames])
-        ga = (ra.GBuffer * len(rends))(*[g[k % 4].gb for _, g in frames])
+        va = (ra.View * len(rends))(*[v[k % views_per_case] for v, _ in frames])
+        ga = (ra.GBuffer * len(rends))(*[g[k % views_per_case].gb for _, g in frames])
         ra.check(L.rgi_batch_render(batch, va, ga, bs))
 
     for k in range(warmup):
@@ -443,8 +480,10 @@ def run_uneven_batch(mode, steps, warmup):
     for r in rends:
         L.rgi_renderer_destroy(r)
     L.rgi_destroy(ctx)
-    return {"scenes": ["%s %dx%d" % (c[0], c[3], c[4]) for c in cases], "eager_ms_per_batch_frame": round(eager_ms, 5),
-            "graph_batch_ms_per_batch_frame": round(batch_ms, 5), "steps": steps}
+    return {"scenes": names, "triangles": tris, "per_scene_ms": [round(v, 4) for v in per_scene],
+            "sum_of_scene_ms": round(sum(per_scene), 4), "eager_ms_per_batch_frame": round(eager_ms, 5),
+            "graph_batch_ms_per_batch_frame": round(batch_ms, 5), "pixels": sum(pixels),
+            "gi_pixels_per_s": round(sum(pixels) / (batch_ms * 1e-3), 1), "steps": steps, "nrc": nrc}
 
 
 def main():
@@ -648,7 +687,15 @@ def main():
         L.rgi_renderer_destroy(arun["rend"])
         arun["drv"].close()
 
-    uneven = run_uneven_batch(mode, 16, 8) if args.batch else None
+    uneven = views8 = None
+    if args.batch:
+        # 16 scenes of 1k to 1M triangles at 640x360 to 2560x1440, each with
+        # its own neural radiance cache, as one batch frame.
+        uneven = run_batch(UNEVEN_SCENES, mode, 8, 8, nrc=args.nrc)
+        # The atrium seen from 8 independent cameras (yaw offsets of 45
+        # degrees) at 1080p, as one batch frame.
+        atrium8 = [("atrium_256k_view%d" % v, ra.RGI_SCENE_ATRIUM, 262144, 1, W, H, 0.7) for v in range(8)]
+        views8 = run_batch(atrium8, mode, 8, 4, yaw_offsets=[v * math.pi / 4 for v in range(8)])
 
     line = {
         "metric": "GI frame ms @1080p (grid + resample + trace)",
@@ -680,6 +727,7 @@ def main():
         "cpu_baseline": cpu,
         "atrium_1080p": atrium,
         "uneven_batch": uneven,
+        "atrium_8_views_batch": views8,
         "neural_radiance_cache": nrc,
     }
     print(json.dumps(line), flush=True)
