@@ -98,8 +98,20 @@ def surface_rays(hs, count, seed):
     return rays
 
 
-def cpu_baseline(hs, settings, width, height, table):
-    """The oracle on the host cores: one steady-state frame (third of a sequence)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(hs, settings, width, height, table, reps=5):
+    """The oracle on the host cores: the third frame of the sequence (reuse
+    and history active), all passes, `reps` times at up to 16 threads
+    (median per pass and per frame), then once on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import common
     import orc
@@ -108,18 +120,35 @@ def cpu_baseline(hs, settings, width, height, table):
     s = ra.Settings(**settings.__dict__)
     s.grid_table_size = table
     hist = common.empty_history(width, height, table)
-    elapsed = None
+    frames = []
     for i in range(3):
         eye, vp, inv = ra.orbit_camera(hs, i, width, height, yaw_of=ra.swing_yaw)
         peye, pvp, _ = ra.orbit_camera(hs, max(i - 1, 0), width, height, yaw_of=ra.swing_yaw)
         f = ra.make_frame(s, hs, i, eye, vp, inv, peye, pvp, width, height, i > 0, i > 0)
         g = common.oracle_gbuffer(hs, f)
+        frames.append((f, g))
+        if i < 2:
+            hist = common.next_history(hist, common.oracle_frame(hs, f, g, hist))
+    f, g = frames[2]
+    totals, per_pass = [], {}
+    for _ in range(reps):
+        times = {}
         t0 = time.perf_counter()
-        out = common.oracle_frame(hs, f, g, hist)
-        elapsed = time.perf_counter() - t0
-        hist = common.next_history(hist, out)
-    return {"value": elapsed * 1e3, "unit": "ms per GI frame", "cores": threads, "kind": "port",
-            "sample": "oracle, one steady-state %dx%d frame (frame 3 of the orbit), all passes" % (width, height)}
+        common.oracle_frame(hs, f, g, hist, times=times)
+        totals.append(time.perf_counter() - t0)
+        for k, v in times.items():
+            per_pass.setdefault(k.replace("orc_", ""), []).append(v)
+    orc.lib().orc_set_threads(1)
+    t0 = time.perf_counter()
+    common.oracle_frame(hs, f, g, hist)
+    single = time.perf_counter() - t0
+    orc.lib().orc_set_threads(threads)
+    return {"value": float(np.median(totals)) * 1e3, "unit": "ms per GI frame", "cores": threads, "kind": "port",
+            "sample": "oracle, one steady-state %dx%d frame (frame 3 of the path), all passes, median of %d"
+                      % (width, height, reps),
+            "pass_ms_median": {k: round(float(np.median(v)) * 1e3, 2) for k, v in sorted(per_pass.items())},
+            "single_thread_ms": round(single * 1e3, 1), "cpu_model": cpu_model(),
+            "host_cpus": os.cpu_count()}
 
 
 def measure_mse(L, hs, gscene, gbuf, view, W, H, T, mode, ref_frames, frames, ctx, s):

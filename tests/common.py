@@ -5,6 +5,7 @@ tensors), with the pass order and ping-pong of Renderer::render
 insert -> GI temporal -> GI spatial.
 """
 import ctypes
+import time
 
 import numpy as np
 
@@ -122,10 +123,31 @@ def oracle_composite(frame, g, out, atlas, di_on):
     return hdr
 
 
+class _Timed:
+    """The oracle library with each call's wall time added to `times[name]`."""
+
+    def __init__(self, lib, times):
+        self.lib, self.times = lib, times
+
+    def __getattr__(self, name):
+        fn = getattr(self.lib, name)
+        if self.times is None:
+            return fn
+
+        def call(*a):
+            t0 = time.perf_counter()
+            r = fn(*a)
+            self.times[name] = self.times.get(name, 0.0) + time.perf_counter() - t0
+            return r
+        return call
+
+
 def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterations=5, composite=False,
-                 specular=False):
-    """One frame on the oracle; returns this frame's outputs (host arrays)."""
-    L = orc.lib()
+                 specular=False, times=None):
+    """One frame on the oracle; returns this frame's outputs (host arrays).
+    With `times` (a dict), each oracle pass's wall time in seconds is added
+    under its function name."""
+    L = _Timed(orc.lib(), times)
     w, h, T = frame.width, frame.height, frame.grid_table_size
     n = w * h
     scene = orc.host_scene(hs)
