@@ -238,6 +238,46 @@ def nrc_quality(L, hs, gscene, gbuf, view, W, H, T, mode, acc, acc_gi, ref_frame
     return result
 
 
+def cache_frames(hs, W, H, T, frames_in_path, steps):
+    """The metric's swing path and settings with the neural radiance cache on:
+    16 warm-up frames, then `steps` timed frames."""
+    import torch
+    from restir_amd import nrc as rnrc
+    L = ra.lib()
+    settings = ra.Settings(grid_table_size=T, denoiser=False, composite=False, specular_gi=False)
+    gscene = driver.GpuScene(hs)
+    ctx = ctypes.c_void_p()
+    ra.check(L.rgi_create(ctypes.byref(ctx)))
+    views, gbufs = [], []
+    for i in range(frames_in_path):
+        eye, vp, inv = ra.orbit_camera(hs, i, W, H, yaw_of=lambda k: ra.swing_yaw(k, frames_in_path))
+        f = ra.make_frame(settings, hs, i, eye, vp, inv, eye, vp, W, H, False)
+        gbufs.append(driver.GBufferSet(ctx, gscene, f))
+        views.append(ra.make_view(eye, vp, inv))
+    s = driver.stream()
+    rend = ctypes.c_void_p()
+    ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                   ctypes.byref(gscene.scene), hs.radius))
+    ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs))))
+
+    def render(k):
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[k % frames_in_path]),
+                                       ctypes.byref(gbufs[k % frames_in_path].gb), s))
+
+    for k in range(16):
+        render(k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        render(k)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    L.rgi_renderer_destroy(rend)
+    L.rgi_destroy(ctx)
+    return {"workload": "cornell_1080p_swing%d with the neural radiance cache" % frames_in_path,
+            "ms_per_frame_with_cache": round(ms, 5), "steps": steps}
+
+
 def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, steps, plain_ms, s):
     """Cost of the cache: the metric's frame with the cache on (same path,
     same settings), and its kernels alone with HIP events: 2M queries per
@@ -535,6 +575,9 @@ def main():
     ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the uneven batch workload")
     ap.add_argument("--no-nrc", dest="nrc", action="store_false", help="skip the neural radiance cache measurements")
     ap.add_argument("--nrc-frames", type=int, default=256, help="frames rendered before the cache's MSE is measured")
+    ap.add_argument("--cache-frames-only", action="store_true",
+                    help="render only the metric's path with the neural radiance cache on (16 warm-up + --steps "
+                         "frames) and print their frame time: the workload of the cache's kernel profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus %d refused)" % args.gpus,
@@ -552,6 +595,9 @@ def main():
         del src, dst
     W, H, T = args.width, args.height, args.table
     hs = ra.HostScene(ra.RGI_SCENE_CORNELL, seed=1, extra_lights=4, light_seed=1)
+    if args.cache_frames_only:
+        print(json.dumps(cache_frames(hs, W, H, T, args.frames, args.steps)), flush=True)
+        return
     mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
     # The metric's frame: grid + resample + trace. The denoiser that follows
     # it in the reference is timed on its own below.
