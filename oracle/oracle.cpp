@@ -2453,3 +2453,24 @@ void orc_gbuffer_primary(const rgi_frame* f, const rgi_scene* s, const float* ma
 }
 
 } // extern "C"
+
+// The kernels' division by a constant c (device.h div255, div3): q = x *
+// RN(1 / c) refined by one FMA step. Counts the non-negative floats with bit
+// patterns lo, lo + stride, ... below hi where it differs from IEEE x / c.
+extern "C" uint64_t orc_div_const_mismatches(float c, uint32_t lo, uint32_t hi, uint32_t stride)
+{
+    volatile float one = 1.0f;
+    const float r = one / c;
+    uint64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i += stride) {
+        uint32_t u = (uint32_t)i;
+        float x;
+        memcpy(&x, &u, 4);
+        float q = x * r;
+        float refined = fmaf(fmaf(-c, q, x), r, q);
+        float exact = x / c;
+        if (memcmp(&refined, &exact, 4) != 0) ++bad;
+    }
+    return bad;
+}

@@ -129,6 +129,11 @@ void orc_nrc_corner(const uint32_t* res, uint32_t log2_table, int level, const f
                     float* w);
 float orc_bf16_round(float x);
 
+/* Floats (bit patterns lo, lo + stride, ... < hi, non-negative) for which the
+ * kernels' constant division, fma(fma(-c, q, x), RN(1/c), q) with
+ * q = x * RN(1/c), differs from IEEE x / c. */
+uint64_t orc_div_const_mismatches(float c, uint32_t lo, uint32_t hi, uint32_t stride);
+
 #ifdef __cplusplus
 }
 #endif

@@ -114,7 +114,7 @@ RGI_DEV float light_visibility_soft(const rgi_frame& f, const float* atlas, F3 p
     F2 uv;
     float reference;
     if (!shadow_lookup(f, atlas, position, n, 2.0f, uv, reference)) return 1.0f;
-    F2 atlas_texel = F2{f.shadow_info[2] / 3.0f, f.shadow_info[2] * 0.5f};
+    F2 atlas_texel = F2{div3(f.shadow_info[2]), f.shadow_info[2] * 0.5f};
     float lit = 0.0f;
     for (int y = 0; y < 4; ++y) {
         for (int x = 0; x < 4; ++x) {

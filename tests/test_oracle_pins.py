@@ -1042,3 +1042,19 @@ def test_frame_pack_follows_renderer_uniforms():
     r = common.frame_for(hs, s.reference_mode(), 320, 240, 5, True)
     assert (r.temporal_taps, r.spatial_taps, r.di_spatial_taps) == (0, 0, 0)
     assert (r.boiling_filter, r.jacobian_clamp, r.outlier_clamp) == (0.0, 0.0, 0.0)
+
+
+def test_constant_division_matches_ieee():
+    """The kernels decode RGBA8 with x / 255 as q = x RN(1/255) plus one FMA
+    correction: the same float as IEEE division for every byte and over a
+    sweep of all non-negative floats; likewise x / 3. For pi the same
+    refinement fails on tiny inputs, which shows the check can fail."""
+    L = orc.lib()
+    L.orc_div_const_mismatches.restype = ctypes.c_uint64
+    L.orc_div_const_mismatches.argtypes = [ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+    for b in range(256):
+        bits = int(np.float32(b).view(np.uint32))
+        assert L.orc_div_const_mismatches(255.0, bits, bits + 1, 1) == 0
+    for c in (255.0, 3.0):
+        assert L.orc_div_const_mismatches(c, 0, 0x7F800000, 251) == 0, c
+    assert L.orc_div_const_mismatches(float(np.float32(np.pi)), 0, 0x0B800000, 7) > 0
