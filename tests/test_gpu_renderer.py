@@ -60,7 +60,18 @@ def atrium():
          "shadow_face_size": 0}, {}, "cornell"),
     (3, {}, ALL_VARIANTS, "atrium")])
 def test_renderer_matches_oracle_sequence(torch, frames, kw, variants, scene):
-    say("case renderer scene=%s frames=%d %s variants=%s" % (scene, frames, kw, variants))
+    run_renderer_case(torch, frames, kw, variants, scene, W, H)
+
+
+@pytest.mark.parametrize("width,height", [(97, 61), (7, 5)])
+def test_renderer_matches_oracle_at_ragged_sizes(torch, width, height):
+    """Sizes that are no multiple of the 16 x 16 blocks, the 4 x 8 groups or
+    the 8 x 8 insert tiles: every pass's edge handling, all launch variants."""
+    run_renderer_case(torch, 3, {}, ALL_VARIANTS, "cornell", width, height)
+
+
+def run_renderer_case(torch, frames, kw, variants, scene, W, H):
+    say("case renderer scene=%s frames=%d %s variants=%s size=%dx%d" % (scene, frames, kw, variants, W, H))
     hs = atrium() if scene == "atrium" else common.cornell(extra_lights=4)
     distance = ATRIUM_DISTANCE if scene == "atrium" else 3.17
     s = ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC, **kw)
