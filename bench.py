@@ -714,6 +714,24 @@ def main():
     tuning["trace_ms"] = {str(k): round(v, 4) for k, v in trace_variants.items()}
     mrays = args.rays / (trace_ms * 1e-3) / 1e6
     hit_rate = float((hit.view(torch.int32) >= 0).float().mean().item())
+    # The same rays with the watertight triangle test (RGI_OPT_TRIANGLE_TEST):
+    # its speed, and how often its hit differs from the reference's test.
+    mt_hit = hit.view(torch.int32).clone()
+    ra.check(L.rgi_context_set_option(drv.ctx, ra.RGI_OPT_TRIANGLE_TEST, 1))
+    trace()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(5):
+        trace()
+    e1.record()
+    torch.cuda.synchronize()
+    wt_ms = e0.elapsed_time(e1) / 5
+    wt_hit = hit.view(torch.int32)
+    watertight = {"ms": round(wt_ms, 4), "mrays_per_s": round(args.rays / (wt_ms * 1e-3) / 1e6, 2),
+                  "hit_rate": round(float((wt_hit >= 0).float().mean().item()), 5),
+                  "differs_from_moeller_trumbore": round(float((wt_hit != mt_hit).float().mean().item()), 7)}
+    ra.check(L.rgi_context_set_option(drv.ctx, ra.RGI_OPT_TRIANGLE_TEST, 0))
+    del mt_hit
 
     dominant = max((k for k in pass_ms if k in BYTES_PER_PIXEL), key=lambda k: pass_ms[k])
     traffic = BYTES_PER_PIXEL[dominant] * W * H
@@ -795,7 +813,8 @@ def main():
         "composite_ms": round(composite_ms, 5),
         "specular_gi_ms": round(specular_ms, 5),
         "mrays_per_s": round(mrays, 2),
-        "trace": {"rays": args.rays, "ms": round(trace_ms, 4), "hit_rate": round(hit_rate, 4), "mode": "closest"},
+        "trace": {"rays": args.rays, "ms": round(trace_ms, 4), "hit_rate": round(hit_rate, 4), "mode": "closest",
+                  "triangle_test": "moeller_trumbore (the reference's)", "watertight": watertight},
         "mse_vs_reference": mse,
         "tuning": tuning,
         "roofline": roofline,

@@ -182,6 +182,13 @@ void rgi_destroy(rgi_context* context);
 #define RGI_OPT_GI_SPATIAL_WAVES 2  /* the same for GI spatial */
 #define RGI_OPT_TRACE_PERSISTENT 3  /* rgi_trace: 0 one ray per lane, 1 persistent 64-ray batches per wave */
 #define RGI_OPT_TILE_ORDER 4 /* screen passes: 0 row-major tiles, 1 a contiguous band of tiles per XCD */
+/* Not a launch variant: the triangle test of rgi_trace. 0 (default) the
+ * reference's Moeller-Trumbore test (bvh.sh:44-71), 1 the watertight test of
+ * Woop, Benthin and Wald 2013 (FP32, with an FP64 recomputation of an edge
+ * function that comes out exactly zero), which never lets a ray through the
+ * shared edge or vertex of two triangles. The resampling passes and the
+ * G-buffer and shadow producers always use the reference's test. */
+#define RGI_OPT_TRIANGLE_TEST 5
 int rgi_context_set_option(rgi_context* context, int option, int value);
 int rgi_context_get_option(const rgi_context* context, int option);
 

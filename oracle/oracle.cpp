@@ -22,6 +22,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <utility>
 #include <vector>
 
 #ifdef _OPENMP
@@ -376,6 +377,27 @@ float intersect_box(V3 origin, V3 inv_dir, V3 mn, V3 mx, float max_distance)
     return enter <= leave ? enter : -1.0f;
 }
 
+// The slab test made conservative for the watertight mode (Pharr, Jakob and
+// Humphreys, PBR 3rd ed., 3.9.2): far distances scaled by 1 + 2 gamma(3);
+// the twin of the kernels' box_enter_conservative, not a reference function.
+float intersect_box_conservative(V3 origin, V3 inv_dir, V3 mn, V3 mx, float max_distance)
+{
+    const float grow = 1.0f + 2.0f * (3.0f * 0x1p-24f / (1.0f - 3.0f * 0x1p-24f));
+    V3 t0 = (mn - origin) * inv_dir;
+    V3 t1 = (mx - origin) * inv_dir;
+    V3 n = vmin(t0, t1);
+    V3 fa = vmax(t0, t1);
+    float enter = fmax_(fmax_(n.x, n.y), fmax_(n.z, 0.0f));
+    float leave = fmin_(fmin_(fa.x * grow, fa.y * grow), fmin_(fa.z * grow, max_distance));
+    return enter <= leave ? enter : -1.0f;
+}
+
+float box_test(bool watertight, V3 origin, V3 inv_dir, V3 mn, V3 mx, float max_distance)
+{
+    return watertight ? intersect_box_conservative(origin, inv_dir, mn, mx, max_distance)
+                      : intersect_box(origin, inv_dir, mn, mx, max_distance);
+}
+
 // intersectTriangle: Moeller-Trumbore, both sides (bvh.sh:44-71).
 float intersect_triangle(V3 o, V3 d, V3 v0, V3 v1, V3 v2)
 {
@@ -394,6 +416,40 @@ float intersect_triangle(V3 o, V3 d, V3 v0, V3 v1, V3 v2)
     return dot(e2, q) * inv;
 }
 
+// The watertight ray/triangle test of Woop, Benthin and Wald 2013 (JCGT
+// 2(1), 65-82), both sides, in the operation order of the kernels'
+// watertight_hit (restir_amd/csrc/bvh_device.h): not a reference function, the
+// twin of rgi_trace's RGI_OPT_TRIANGLE_TEST = 1.
+float axis_of(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+float intersect_triangle_watertight(V3 o, V3 d, V3 v0, V3 v1, V3 v2)
+{
+    float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
+    int kz = adx >= ady ? (adx >= adz ? 0 : 2) : (ady >= adz ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = axis_of(d, kz);
+    if (dz < 0.0f) std::swap(kx, ky);
+    float sx = axis_of(d, kx) / dz, sy = axis_of(d, ky) / dz, sz = 1.0f / dz;
+    V3 a = v0 - o, b = v1 - o, c = v2 - o;
+    float az = axis_of(a, kz), bz = axis_of(b, kz), cz = axis_of(c, kz);
+    float ax = axis_of(a, kx) - sx * az, ay = axis_of(a, ky) - sy * az;
+    float bx = axis_of(b, kx) - sx * bz, by = axis_of(b, ky) - sy * bz;
+    float cx = axis_of(c, kx) - sx * cz, cy = axis_of(c, ky) - sy * cz;
+    float u = cx * by - cy * bx;
+    float v = ax * cy - ay * cx;
+    float w = bx * ay - by * ax;
+    if (u == 0.0f || v == 0.0f || w == 0.0f) {
+        u = (float)((double)cx * (double)by - (double)cy * (double)bx);
+        v = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+        w = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    }
+    if ((u < 0.0f || v < 0.0f || w < 0.0f) && (u > 0.0f || v > 0.0f || w > 0.0f)) return -1.0f;
+    float det = (u + v) + w;
+    if (det == 0.0f) return -1.0f;
+    return ((u * (sz * az) + v * (sz * bz)) + w * (sz * cz)) / det;
+}
+
 // bvhInverseDirection (bvh.sh:113-121).
 float safe_component(float d) { return fabsf(d) > 1e-8f ? d : (d >= 0.0f ? 1e-8f : -1e-8f); }
 V3 inverse_direction(V3 d)
@@ -402,10 +458,14 @@ V3 inverse_direction(V3 d)
 }
 
 // bvhIntersectLeaf (bvh.sh:124-138).
-int intersect_leaf(const rgi_scene* s, V3 o, V3 d, int first, int count, float& distance, int hit)
+int intersect_leaf(const rgi_scene* s, V3 o, V3 d, int first, int count, float& distance, int hit,
+                   bool watertight = false)
 {
     for (int tri = first; tri < first + count; ++tri) {
-        float t = intersect_triangle(o, d, bvh_vertex(s, tri, 0), bvh_vertex(s, tri, 1), bvh_vertex(s, tri, 2));
+        float t = watertight ? intersect_triangle_watertight(o, d, bvh_vertex(s, tri, 0), bvh_vertex(s, tri, 1),
+                                                             bvh_vertex(s, tri, 2))
+                             : intersect_triangle(o, d, bvh_vertex(s, tri, 0), bvh_vertex(s, tri, 1),
+                                                  bvh_vertex(s, tri, 2));
         if (t > 0.0f && t < distance) {
             distance = t;
             hit = tri;
@@ -415,7 +475,7 @@ int intersect_leaf(const rgi_scene* s, V3 o, V3 d, int first, int count, float& 
 }
 
 // traceBvhStackless (bvh.sh:140-174).
-int trace_stackless(const rgi_scene* s, V3 o, V3 d, bool any_hit, float& distance)
+int trace_stackless(const rgi_scene* s, V3 o, V3 d, bool any_hit, float& distance, bool watertight = false)
 {
     V3 inv = inverse_direction(d);
     int node_count = (int)s->node_count;
@@ -426,12 +486,12 @@ int trace_stackless(const rgi_scene* s, V3 o, V3 d, bool any_hit, float& distanc
         V4 n0 = scene_vec4(s, (size_t)node * 2);
         V4 n1 = scene_vec4(s, (size_t)node * 2 + 1);
         bool leaf = n1.w >= 0.0f;
-        if (intersect_box(o, inv, xyz(n0), xyz(n1), distance) < 0.0f) {
+        if (box_test(watertight, o, inv, xyz(n0), xyz(n1), distance) < 0.0f) {
             node = leaf ? node + 1 : to_int(-n1.w) - 1;
             continue;
         }
         if (leaf) {
-            hit = intersect_leaf(s, o, d, to_int(n0.w), to_int(n1.w), distance, hit);
+            hit = intersect_leaf(s, o, d, to_int(n0.w), to_int(n1.w), distance, hit, watertight);
             if (any_hit && hit >= 0) return hit;
         }
         node = node + 1;
@@ -440,10 +500,10 @@ int trace_stackless(const rgi_scene* s, V3 o, V3 d, bool any_hit, float& distanc
 }
 
 // traceBvhSharedStack (bvh.sh:216-295), with this thread's 8-entry ring.
-int trace_short_stack(const rgi_scene* s, V3 o, V3 d, float& distance)
+int trace_short_stack(const rgi_scene* s, V3 o, V3 d, float& distance, bool watertight = false)
 {
     V3 inv = inverse_direction(d);
-    if (intersect_box(o, inv, xyz(scene_vec4(s, 0)), xyz(scene_vec4(s, 1)), distance) < 0.0f) return -1;
+    if (box_test(watertight, o, inv, xyz(scene_vec4(s, 0)), xyz(scene_vec4(s, 1)), distance) < 0.0f) return -1;
     int stack[kBvhShortStack];
     int top = 0, held = 0;
     bool dropped = false;
@@ -453,14 +513,14 @@ int trace_short_stack(const rgi_scene* s, V3 o, V3 d, float& distance)
         V4 n0 = scene_vec4(s, (size_t)node * 2);
         V4 n1 = scene_vec4(s, (size_t)node * 2 + 1);
         if (n1.w >= 0.0f) {
-            hit = intersect_leaf(s, o, d, to_int(n0.w), to_int(n1.w), distance, hit);
+            hit = intersect_leaf(s, o, d, to_int(n0.w), to_int(n1.w), distance, hit, watertight);
         } else {
             int left = node + 1;
             int right = to_int(n0.w);
-            float ld = intersect_box(o, inv, xyz(scene_vec4(s, (size_t)left * 2)),
-                                     xyz(scene_vec4(s, (size_t)left * 2 + 1)), distance);
-            float rd = intersect_box(o, inv, xyz(scene_vec4(s, (size_t)right * 2)),
-                                     xyz(scene_vec4(s, (size_t)right * 2 + 1)), distance);
+            float ld = box_test(watertight, o, inv, xyz(scene_vec4(s, (size_t)left * 2)),
+                                xyz(scene_vec4(s, (size_t)left * 2 + 1)), distance);
+            float rd = box_test(watertight, o, inv, xyz(scene_vec4(s, (size_t)right * 2)),
+                                xyz(scene_vec4(s, (size_t)right * 2 + 1)), distance);
             bool vl = ld >= 0.0f, vr = rd >= 0.0f;
             if (vl && vr) {
                 bool left_first = ld <= rd;
@@ -482,7 +542,7 @@ int trace_short_stack(const rgi_scene* s, V3 o, V3 d, float& distance)
         node = stack[top & (kBvhShortStack - 1)];
     }
     if (dropped) {
-        int again = trace_stackless(s, o, d, false, distance);
+        int again = trace_stackless(s, o, d, false, distance, watertight);
         if (again >= 0) hit = again;
     }
     return hit;
@@ -490,12 +550,13 @@ int trace_short_stack(const rgi_scene* s, V3 o, V3 d, float& distance)
 
 // traceBvh (bvh.sh:298-319): closest hit through the short stack (as the GI
 // temporal pass compiles it), any hit stackless.
-int trace_bvh(const rgi_scene* s, V3 o, V3 d, float max_distance, bool any_hit, float& distance)
+int trace_bvh(const rgi_scene* s, V3 o, V3 d, float max_distance, bool any_hit, float& distance,
+              bool watertight = false)
 {
     distance = max_distance;
     if (!bvh_available(s)) return -1;
-    if (!any_hit) return trace_short_stack(s, o, d, distance);
-    return trace_stackless(s, o, d, true, distance);
+    if (!any_hit) return trace_short_stack(s, o, d, distance, watertight);
+    return trace_stackless(s, o, d, true, distance, watertight);
 }
 
 V3 bvh_triangle_albedo(const rgi_scene* s, int tri)
@@ -1855,6 +1916,20 @@ uint32_t orc_grid_pick_entry(const rgi_frame* frame, const uint32_t* grid, const
     return grid_pick_entry(frame, grid, V3{camera3[0], camera3[1], camera3[2]},
                            V3{position3[0], position3[1], position3[2]}, V3{normal3[0], normal3[1], normal3[2]}, r0,
                            r1, r2);
+}
+
+// rgi_trace with RGI_OPT_TRIANGLE_TEST = 1: the same traversal with the
+// watertight triangle test.
+void orc_trace_watertight(const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle,
+                          float* hit_distance, int any_hit)
+{
+    parallel_for((int64_t)count, [&](int64_t i) {
+        const float* r = rays + (size_t)i * 8;
+        float dist;
+        int tri = trace_bvh(scene, V3{r[0], r[1], r[2]}, V3{r[4], r[5], r[6]}, r[3], any_hit != 0, dist, true);
+        hit_triangle[i] = tri;
+        hit_distance[i] = dist;
+    });
 }
 
 void orc_trace(const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle, float* hit_distance,

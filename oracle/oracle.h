@@ -81,6 +81,10 @@ uint32_t orc_grid_pick_entry(const rgi_frame* frame, const uint32_t* grid, const
  * rgi_trace. */
 void orc_trace(const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle,
                float* hit_distance, int any_hit);
+/* The same with the watertight triangle test (rgi_trace with
+ * RGI_OPT_TRIANGLE_TEST = 1). */
+void orc_trace_watertight(const rgi_scene* scene, const float* rays, uint32_t count, int32_t* hit_triangle,
+                          float* hit_distance, int any_hit);
 
 void orc_gi_temporal(const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* gbuffer,
                      const rgi_gi_inputs* inputs, const rgi_gi_reservoirs* out);

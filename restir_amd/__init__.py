@@ -23,6 +23,7 @@ RGI_GRID_ATOMIC, RGI_GRID_DETERMINISTIC = 0, 1
 RGI_SCENE_CORNELL, RGI_SCENE_ATRIUM, RGI_SCENE_BOXFIELD = 0, 1, 2
 RGI_OPT_GI_TEMPORAL_WAVES, RGI_OPT_GI_SPATIAL_WAVES, RGI_OPT_TRACE_PERSISTENT = 1, 2, 3
 RGI_OPT_TILE_ORDER = 4
+RGI_OPT_TRIANGLE_TEST = 5
 RGI_TUNE_TRIALS = 6
 RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4, RGI_TUNE_TILE_ORDER = 1, 2, 4
 

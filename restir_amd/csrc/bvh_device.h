@@ -42,6 +42,21 @@ RGI_DEV float box_enter(F3 o, F3 inv, F4 lo, F4 hi, float max_distance)
     return enter <= leave ? enter : -1.0f;
 }
 
+// The slab test made conservative for the watertight triangle test (Pharr,
+// Jakob and Humphreys, PBR 3rd ed., 3.9.2): each axis's far distance is
+// scaled by 1 + 2 gamma(3), gamma(n) = n u / (1 - n u), u = 2^-24, so that
+// rounding never rejects a box that the ray grazes.
+RGI_DEV float box_enter_conservative(F3 o, F3 inv, F4 lo, F4 hi, float max_distance)
+{
+    const float grow = 1.0f + 2.0f * (3.0f * 0x1p-24f / (1.0f - 3.0f * 0x1p-24f));
+    float t0x = (lo.x - o.x) * inv.x, t1x = (hi.x - o.x) * inv.x;
+    float t0y = (lo.y - o.y) * inv.y, t1y = (hi.y - o.y) * inv.y;
+    float t0z = (lo.z - o.z) * inv.z, t1z = (hi.z - o.z) * inv.z;
+    float enter = maxf_(maxf_(minf_(t0x, t1x), minf_(t0y, t1y)), maxf_(minf_(t0z, t1z), 0.0f));
+    float leave = minf_(minf_(maxf_(t0x, t1x) * grow, maxf_(t0y, t1y) * grow), minf_(maxf_(t0z, t1z) * grow, max_distance));
+    return enter <= leave ? enter : -1.0f;
+}
+
 // Moeller-Trumbore, both sides, |det| < 1e-12 rejected.
 RGI_DEV float tri_hit(F3 o, F3 d, F3 v0, F3 v1, F3 v2)
 {
@@ -60,14 +75,83 @@ RGI_DEV float tri_hit(F3 o, F3 d, F3 v0, F3 v1, F3 v2)
     return dot(e2, q) * inv;
 }
 
+// Watertight ray/triangle test (Woop, Benthin and Wald 2013, JCGT 2(1)),
+// both sides. The ray's dominant axis becomes z (x and y swapped when the
+// ray points down it, to keep the winding), the triangle is sheared into ray
+// space once per vertex, and the signs of the three edge functions decide
+// the hit. An edge function that comes out exactly zero in FP32 is
+// recomputed from the same sheared coordinates in FP64, so a ray through a
+// shared edge or vertex hits at least one of the triangles that meet there.
+struct WatertightRay {
+    F3 o;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+RGI_DEV float axis(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+RGI_DEV WatertightRay watertight_ray(F3 o, F3 d)
+{
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = ax >= ay ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = axis(d, kz);
+    if (dz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    return WatertightRay{o, kx, ky, kz, axis(d, kx) / dz, axis(d, ky) / dz, 1.0f / dz};
+}
+
+RGI_DEV float watertight_hit(const WatertightRay& r, F3 v0, F3 v1, F3 v2)
+{
+    F3 a = v0 - r.o, b = v1 - r.o, c = v2 - r.o;
+    float az = axis(a, r.kz), bz = axis(b, r.kz), cz = axis(c, r.kz);
+    float ax = axis(a, r.kx) - r.sx * az, ay = axis(a, r.ky) - r.sy * az;
+    float bx = axis(b, r.kx) - r.sx * bz, by = axis(b, r.ky) - r.sy * bz;
+    float cx = axis(c, r.kx) - r.sx * cz, cy = axis(c, r.ky) - r.sy * cz;
+    float u = cx * by - cy * bx;
+    float v = ax * cy - ay * cx;
+    float w = bx * ay - by * ax;
+    if (u == 0.0f || v == 0.0f || w == 0.0f) {
+        u = (float)((double)cx * (double)by - (double)cy * (double)bx);
+        v = (float)((double)ax * (double)cy - (double)ay * (double)cx);
+        w = (float)((double)bx * (double)ay - (double)by * (double)ax);
+    }
+    if ((u < 0.0f || v < 0.0f || w < 0.0f) && (u > 0.0f || v > 0.0f || w > 0.0f)) return -1.0f;
+    float det = (u + v) + w;
+    if (det == 0.0f) return -1.0f;
+    float t = ((u * (r.sz * az) + v * (r.sz * bz)) + w * (r.sz * cz)) / det;
+    return t;
+}
+
+// The two triangle tests as traversal functors.
+// The two triangle tests as traversal functors, each with its box test.
+struct MollerTrumbore {
+    F3 o, d;
+    RGI_DEV float hit(F3 v0, F3 v1, F3 v2) const { return tri_hit(o, d, v0, v1, v2); }
+    RGI_DEV float box(F3 inv, F4 lo, F4 hi, float max_distance) const { return box_enter(o, inv, lo, hi, max_distance); }
+};
+struct Watertight {
+    WatertightRay r;
+    RGI_DEV float hit(F3 v0, F3 v1, F3 v2) const { return watertight_hit(r, v0, v1, v2); }
+    RGI_DEV float box(F3 inv, F4 lo, F4 hi, float max_distance) const
+    {
+        return box_enter_conservative(r.o, inv, lo, hi, max_distance);
+    }
+};
+
 RGI_DEV float safe_dir(float d) { return fabsf(d) > 1e-8f ? d : (d >= 0.0f ? 1e-8f : -1e-8f); }
 RGI_DEV F3 inverse_dir(F3 d) { return F3{1.0f / safe_dir(d.x), 1.0f / safe_dir(d.y), 1.0f / safe_dir(d.z)}; }
 
-RGI_DEV int leaf_hits(const SceneView& v, F3 o, F3 d, int first, int count, float& dist, int hit)
+template <class Tri>
+RGI_DEV int leaf_hits(const SceneView& v, const Tri& test, int first, int count, float& dist, int hit)
 {
     for (int t = first; t < first + count; ++t) {
         uint32_t b = v.tri_offset + (uint32_t)t * 4u;
-        float h = tri_hit(o, d, xyz(sv(v, b)), xyz(sv(v, b + 1u)), xyz(sv(v, b + 2u)));
+        float h = test.hit(xyz(sv(v, b)), xyz(sv(v, b + 1u)), xyz(sv(v, b + 2u)));
         if (h > 0.0f && h < dist) {
             dist = h;
             hit = t;
@@ -76,7 +160,8 @@ RGI_DEV int leaf_hits(const SceneView& v, F3 o, F3 d, int first, int count, floa
     return hit;
 }
 
-RGI_DEV int walk_stackless(const SceneView& v, F3 o, F3 d, bool any_hit, float& dist)
+template <class Tri>
+RGI_DEV int walk_stackless(const SceneView& v, const Tri& test, F3 o, F3 d, bool any_hit, float& dist)
 {
     F3 inv = inverse_dir(d);
     int nodes = (int)v.node_count;
@@ -87,12 +172,12 @@ RGI_DEV int walk_stackless(const SceneView& v, F3 o, F3 d, bool any_hit, float& 
         F4 n0 = sv(v, (uint32_t)node * 2u);
         F4 n1 = sv(v, (uint32_t)node * 2u + 1u);
         bool leaf = n1.w >= 0.0f;
-        if (box_enter(o, inv, n0, n1, dist) < 0.0f) {
+        if (test.box(inv, n0, n1, dist) < 0.0f) {
             node = leaf ? node + 1 : to_int(-n1.w) - 1;
             continue;
         }
         if (leaf) {
-            hit = leaf_hits(v, o, d, to_int(n0.w), to_int(n1.w), dist, hit);
+            hit = leaf_hits(v, test, to_int(n0.w), to_int(n1.w), dist, hit);
             if (any_hit && hit >= 0) return hit;
         }
         node = node + 1;
@@ -102,10 +187,11 @@ RGI_DEV int walk_stackless(const SceneView& v, F3 o, F3 d, bool any_hit, float& 
 
 // `ring` points at this lane's first ring entry; entries are `stride` ints
 // apart (the ring is laid out slot-major across the lanes of a block).
-RGI_DEV int walk_short_stack(const SceneView& v, F3 o, F3 d, float& dist, int* ring, int stride)
+template <class Tri>
+RGI_DEV int walk_short_stack(const SceneView& v, const Tri& test, F3 o, F3 d, float& dist, int* ring, int stride)
 {
     F3 inv = inverse_dir(d);
-    if (box_enter(o, inv, sv(v, 0), sv(v, 1), dist) < 0.0f) return -1;
+    if (test.box(inv, sv(v, 0), sv(v, 1), dist) < 0.0f) return -1;
     int top = 0, held = 0;
     bool dropped = false;
     int hit = -1;
@@ -114,12 +200,12 @@ RGI_DEV int walk_short_stack(const SceneView& v, F3 o, F3 d, float& dist, int* r
         F4 n0 = sv(v, (uint32_t)node * 2u);
         F4 n1 = sv(v, (uint32_t)node * 2u + 1u);
         if (n1.w >= 0.0f) {
-            hit = leaf_hits(v, o, d, to_int(n0.w), to_int(n1.w), dist, hit);
+            hit = leaf_hits(v, test, to_int(n0.w), to_int(n1.w), dist, hit);
         } else {
             int left = node + 1;
             int right = to_int(n0.w);
-            float ld = box_enter(o, inv, sv(v, (uint32_t)left * 2u), sv(v, (uint32_t)left * 2u + 1u), dist);
-            float rd = box_enter(o, inv, sv(v, (uint32_t)right * 2u), sv(v, (uint32_t)right * 2u + 1u), dist);
+            float ld = test.box(inv, sv(v, (uint32_t)left * 2u), sv(v, (uint32_t)left * 2u + 1u), dist);
+            float rd = test.box(inv, sv(v, (uint32_t)right * 2u), sv(v, (uint32_t)right * 2u + 1u), dist);
             bool vl = ld >= 0.0f, vr = rd >= 0.0f;
             if (vl && vr) {
                 bool left_first = ld <= rd;
@@ -141,19 +227,21 @@ RGI_DEV int walk_short_stack(const SceneView& v, F3 o, F3 d, float& dist, int* r
         node = ring[(top & (kStackRing - 1)) * stride];
     }
     if (dropped) {
-        int again = walk_stackless(v, o, d, false, dist);
+        int again = walk_stackless(v, test, o, d, false, dist);
         if (again >= 0) hit = again;
     }
     return hit;
 }
 
 // traceBvh's any-hit form: -1 when nothing is hit within max_distance;
-// `dist` ends at the first hit found (or max_distance).
-RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance, float& dist)
+// `dist` ends at the first hit found (or max_distance). The reference's
+// Moeller-Trumbore test unless `watertight`.
+RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance, float& dist, bool watertight = false)
 {
     dist = max_distance;
     if (!bvh_ready(v)) return -1;
-    return walk_stackless(v, o, d, true, dist);
+    if (watertight) return walk_stackless(v, Watertight{watertight_ray(o, d)}, o, d, true, dist);
+    return walk_stackless(v, MollerTrumbore{o, d}, o, d, true, dist);
 }
 
 RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance)
@@ -162,11 +250,13 @@ RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance)
     return trace_any(v, o, d, max_distance, dist);
 }
 
-RGI_DEV int trace_closest(const SceneView& v, F3 o, F3 d, float max_distance, float& dist, int* ring, int stride)
+RGI_DEV int trace_closest(const SceneView& v, F3 o, F3 d, float max_distance, float& dist, int* ring, int stride,
+                          bool watertight = false)
 {
     dist = max_distance;
     if (!bvh_ready(v)) return -1;
-    return walk_short_stack(v, o, d, dist, ring, stride);
+    if (watertight) return walk_short_stack(v, Watertight{watertight_ray(o, d)}, o, d, dist, ring, stride);
+    return walk_short_stack(v, MollerTrumbore{o, d}, o, d, dist, ring, stride);
 }
 
 RGI_DEV F3 tri_albedo(const SceneView& v, int tri) { return xyz(sv(v, v.tri_offset + (uint32_t)tri * 4u + 3u)); }

@@ -15,7 +15,7 @@ namespace {
 constexpr int kTraceBlock = 256;
 
 __global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4* rays, uint32_t count, int32_t* hit,
-                                                       float* dist, int any_hit)
+                                                       float* dist, int any_hit, int watertight)
 {
     __shared__ int ring[kStackRing * kTraceBlock];
     SceneView v = scene_view(s);
@@ -26,9 +26,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4
     float t = a.w;
     int h;
     if (any_hit) {
-        h = trace_any(v, o, d, a.w, t);
+        h = trace_any(v, o, d, a.w, t, watertight != 0);
     } else {
-        h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock);
+        h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock, watertight != 0);
     }
     hit[i] = h;
     dist[i] = t;
@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4
 // every wave reaches the exit test each iteration, so the grid always drains.
 __global__ void __launch_bounds__(kTraceBlock) k_trace_persistent(rgi_scene s, const float4* rays, uint32_t count,
                                                                   int32_t* hit, float* dist, int any_hit,
-                                                                  uint32_t* next_batch)
+                                                                  int watertight, uint32_t* next_batch)
 {
     __shared__ int ring[kStackRing * kTraceBlock];
     SceneView v = scene_view(s);
@@ -54,8 +54,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_persistent(rgi_scene s, c
             F3 o = F3{a.x, a.y, a.z}, d = F3{b.x, b.y, b.z};
             float t = a.w;
             int h;
-            if (any_hit) h = trace_any(v, o, d, a.w, t);
-            else h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock);
+            if (any_hit) h = trace_any(v, o, d, a.w, t, watertight != 0);
+            else h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock, watertight != 0);
             hit[i] = h;
             dist[i] = t;
         }
@@ -185,10 +185,11 @@ int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* h
         blocks = blocks < needed ? blocks : needed;
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
         hipLaunchKernelGGL(k_trace_persistent, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist,
-                           any_hit, counter);
+                           any_hit, tuning.watertight, counter);
     } else {
         uint32_t blocks = (count + kTraceBlock - 1u) / kTraceBlock;
-        hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist, any_hit);
+        hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist, any_hit,
+                           tuning.watertight);
     }
     return check_launch("trace");
 }

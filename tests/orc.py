@@ -66,6 +66,8 @@ def lib():
     L.orc_grid_pick_entry.argtypes = [PF, P, P, P, P, F, F, F]
     L.orc_trace.restype = None
     L.orc_trace.argtypes = [PS, P, U32, P, P, ctypes.c_int]
+    L.orc_trace_watertight.restype = None
+    L.orc_trace_watertight.argtypes = [PS, P, U32, P, P, ctypes.c_int]
     L.orc_gi_temporal.restype = None
     L.orc_gi_temporal.argtypes = [PF, PS, PG, ctypes.POINTER(GiInputs), ctypes.POINTER(GiReservoirs)]
     L.orc_gi_spatial.restype = None

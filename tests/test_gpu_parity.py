@@ -126,12 +126,13 @@ def surface_rays(hs, count, seed, tmax_scale=4.0):
     return rays
 
 
-def trace_both(dev, hs, rays, any_hit):
+def trace_both(dev, hs, rays, any_hit, watertight=False):
     n = len(rays)
     want_t = np.zeros(n, np.int32)
     want_d = np.zeros(n, np.float32)
     scene = orc.host_scene(hs)
-    orc.lib().orc_trace(ctypes.byref(scene), orc.ptr(rays), n, orc.ptr(want_t), orc.ptr(want_d), int(any_hit))
+    trace = orc.lib().orc_trace_watertight if watertight else orc.lib().orc_trace
+    trace(ctypes.byref(scene), orc.ptr(rays), n, orc.ptr(want_t), orc.ptr(want_d), int(any_hit))
     dscene = dev.scene(hs)
     r = dev.put(rays)
     ht, hd = dev.empty(4 * n), dev.empty(4 * n)
@@ -156,6 +157,27 @@ def test_trace(dev, kind, size, any_hit, persistent):
         ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_TRACE_PERSISTENT, 0))
     assert_same("trace.triangle", gt, wt)
     assert_same("trace.distance", gd, wd)
+    assert (wt >= 0).mean() > 0.3
+
+
+@pytest.mark.parametrize("kind,size,any_hit,persistent", [
+    (ra.RGI_SCENE_CORNELL, 0, False, 0), (ra.RGI_SCENE_BOXFIELD, 3000, True, 0),
+    (ra.RGI_SCENE_ATRIUM, 60000, False, 1)])
+def test_trace_watertight(dev, kind, size, any_hit, persistent):
+    """rgi_trace with the watertight triangle test: hit ids and distances
+    bit-exact against the oracle's twin of the test."""
+    say("case trace watertight kind=%d size=%d any_hit=%d persistent=%d" % (kind, size, any_hit, persistent))
+    hs = ra.HostScene(kind, seed=5, size=size)
+    rays = surface_rays(hs, (1 << 16) + 37, seed=kind * 11 + size, tmax_scale=8.0)
+    ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_TRACE_PERSISTENT, persistent))
+    ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_TRIANGLE_TEST, 1))
+    try:
+        gt, gd, wt, wd = trace_both(dev, hs, rays, any_hit, watertight=True)
+    finally:
+        ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_TRACE_PERSISTENT, 0))
+        ra.check(dev.L.rgi_context_set_option(dev.ctx, ra.RGI_OPT_TRIANGLE_TEST, 0))
+    assert_same("trace_watertight.triangle", gt, wt)
+    assert_same("trace_watertight.distance", gd, wd)
     assert (wt >= 0).mean() > 0.3
 
 

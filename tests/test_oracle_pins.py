@@ -1058,3 +1058,103 @@ def test_constant_division_matches_ieee():
     for c in (255.0, 3.0):
         assert L.orc_div_const_mismatches(c, 0, 0x7F800000, 251) == 0, c
     assert L.orc_div_const_mismatches(float(np.float32(np.pi)), 0, 0x0B800000, 7) > 0
+
+
+def closed_mesh_scene(subdivisions=6, seed=3):
+    """A closed surface: the cube [-1, 1]^3, each face split into a grid of
+    quads (two triangles each), rotated by a random rotation so that no
+    coordinate is round. Returns the rgi_scene over host arrays, the arrays
+    (kept alive) and the vertex and edge-midpoint targets."""
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    tris, targets = [], set()
+    g = np.linspace(-1.0, 1.0, subdivisions + 1)
+    for axis in range(3):
+        for side in (-1.0, 1.0):
+            for i in range(subdivisions):
+                for j in range(subdivisions):
+                    corners = []
+                    for (a, b) in ((g[i], g[j]), (g[i + 1], g[j]), (g[i + 1], g[j + 1]), (g[i], g[j + 1])):
+                        p = [0.0, 0.0, 0.0]
+                        p[axis], p[(axis + 1) % 3], p[(axis + 2) % 3] = side, a, b
+                        corners.append(tuple(p))
+                    tris.append((corners[0], corners[1], corners[2]))
+                    tris.append((corners[0], corners[2], corners[3]))
+    pos = (np.array(tris, np.float64).reshape(-1, 3) @ q.T).astype(np.float32).reshape(-1, 9)
+    for t in pos.reshape(-1, 3, 3):
+        for k in range(3):
+            a, b = t[k], t[(k + 1) % 3]
+            targets.add(tuple(a))
+            targets.add(tuple(((a.astype(np.float64) + b) * 0.5).astype(np.float32)))
+    n = len(pos)
+    L = ra.lib()
+    albedo = np.full((n, 3), 0.5, np.float32)
+    emission = np.zeros((n, 3), np.float32)
+    material = np.zeros(n, np.uint32)
+    materials = np.zeros((16, 4), np.float32)
+    mesh = ra.Mesh(n, pos.ctypes.data, albedo.ctypes.data, emission.ctypes.data, material.ctypes.data, 0,
+                   materials.ctypes.data)
+    buf = np.zeros(L.rgi_bvh_capacity(n), np.float32)
+    depth = ctypes.c_uint32()
+    nodes = L.rgi_bvh_build(ctypes.byref(mesh), buf.ctypes.data, ctypes.byref(depth))
+    assert nodes > 0
+    lights = np.zeros(16, np.float32)
+    scene = ra.Scene(buf.ctypes.data, nodes, 2 * nodes, n, 0, lights.ctypes.data)
+    return scene, (buf, pos, albedo, emission, material, materials, lights), np.array(sorted(targets), np.float32)
+
+
+def test_watertight_triangle_test_never_leaks():
+    """Rays from inside a closed, rotated, subdivided cube aimed exactly at
+    every vertex and every edge midpoint (where two or more triangles meet):
+    the watertight test hits a triangle for every ray, in both traversal
+    forms. The reference's Moeller-Trumbore test is counted beside it (it
+    may let such rays through)."""
+    scene, keep, targets = closed_mesh_scene()
+    L = orc.lib()
+    origins = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 0.1], [-0.55, 0.45, 0.6]], np.float32)
+    rays = []
+    for o in origins:
+        d = targets.astype(np.float64) - o
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        r = np.zeros((len(d), 8), np.float32)
+        r[:, 0:3] = o
+        r[:, 3] = 10.0
+        r[:, 4:7] = d
+        rays.append(r)
+    rays = np.ascontiguousarray(np.concatenate(rays))
+    n = len(rays)
+    misses = {}
+    for name, fn in (("watertight", L.orc_trace_watertight), ("moeller_trumbore", L.orc_trace)):
+        for any_hit in (0, 1):
+            hit = np.zeros(n, np.int32)
+            dist = np.zeros(n, np.float32)
+            fn(ctypes.byref(scene), orc.ptr(rays), n, orc.ptr(hit), orc.ptr(dist), any_hit)
+            misses[(name, any_hit)] = int((hit < 0).sum())
+    print("rays through edges and vertices: %d; misses %s" % (n, misses))
+    assert misses[("watertight", 0)] == 0 and misses[("watertight", 1)] == 0
+
+
+def test_watertight_agrees_with_moeller_trumbore_inside_triangles():
+    """Away from edges both tests hit the same triangle at nearly the same
+    distance: rays at triangle centroids of the closed cube."""
+    scene, keep, _ = closed_mesh_scene(subdivisions=4, seed=8)
+    pos = keep[1].reshape(-1, 3, 3).astype(np.float64)
+    cent = pos.mean(axis=1)
+    o = np.array([0.1, 0.05, -0.2])
+    d = cent - o
+    dist_true = np.linalg.norm(d, axis=1)
+    d /= dist_true[:, None]
+    rays = np.zeros((len(d), 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 10.0
+    rays[:, 4:7] = d
+    L = orc.lib()
+    out = {}
+    for name, fn in (("wt", L.orc_trace_watertight), ("mt", L.orc_trace)):
+        hit = np.zeros(len(d), np.int32)
+        dist = np.zeros(len(d), np.float32)
+        fn(ctypes.byref(scene), orc.ptr(rays), len(d), orc.ptr(hit), orc.ptr(dist), 0)
+        out[name] = (hit, dist)
+    assert np.array_equal(out["wt"][0], out["mt"][0]) and np.all(out["wt"][0] >= 0)
+    assert np.allclose(out["wt"][1], dist_true, rtol=1e-5)
+    assert np.allclose(out["mt"][1], dist_true, rtol=1e-5)
