@@ -181,7 +181,9 @@ void rgi_destroy(rgi_context* context);
 #define RGI_OPT_GI_TEMPORAL_WAVES 1 /* occupancy target of GI temporal: 0 (compiler's choice) or 4 waves per SIMD */
 #define RGI_OPT_GI_SPATIAL_WAVES 2  /* the same for GI spatial */
 #define RGI_OPT_TRACE_PERSISTENT 3  /* rgi_trace: 0 one ray per lane, 1 persistent 64-ray batches per wave */
-#define RGI_OPT_TILE_ORDER 4 /* screen passes: 0 row-major tiles, 1 a contiguous band of tiles per XCD */
+/* Screen passes' 16 x 16 tiles: 0 row-major; 1 a contiguous band of tiles per
+ * XCD; 2 64 x 64-pixel blocks dealt to the 8 XCDs in turn. */
+#define RGI_OPT_TILE_ORDER 4
 /* Not a launch variant: the triangle test of rgi_trace. 0 (default) the
  * reference's Moeller-Trumbore test (bvh.sh:44-71), 1 the watertight test of
  * Woop, Benthin and Wald 2013 (FP32, with an FP64 recomputation of an edge

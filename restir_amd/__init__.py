@@ -33,7 +33,7 @@ def tuning_of(config):
     return {
         RGI_OPT_GI_TEMPORAL_WAVES: 4 if config & RGI_TUNE_TEMPORAL_WAVES4 else 0,
         RGI_OPT_GI_SPATIAL_WAVES: 4 if config & RGI_TUNE_SPATIAL_WAVES4 else 0,
-        RGI_OPT_TILE_ORDER: 1 if config & RGI_TUNE_TILE_ORDER else 0,
+        RGI_OPT_TILE_ORDER: 2 if config & RGI_TUNE_TILE_ORDER else 0,
     }
 
 F, I32, U32, P = ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p

@@ -149,7 +149,7 @@ int rgi_context_set_option(rgi_context* ctx, int option, int value)
         ctx->tuning.trace_persistent = value;
         return RGI_OK;
     case RGI_OPT_TILE_ORDER:
-        if (value != 0 && value != 1) return fail(RGI_E_INVALID, "tile order takes 0 or 1, not %d", value);
+        if (value < 0 || value > 2) return fail(RGI_E_INVALID, "tile order takes 0, 1 or 2, not %d", value);
         ctx->tuning.tile_order = value;
         return RGI_OK;
     case RGI_OPT_TRIANGLE_TEST:

@@ -25,21 +25,60 @@ struct F2 { float x, y; };
 struct F3 { float x, y, z; };
 struct F4 { float x, y, z, w; };
 
-// Workgroup -> 16 x 16 screen tile. Order 0 is row-major in workgroup id.
+// Workgroup -> 16 x 16 screen tile.
+// Order 0 is row-major in workgroup id.
 // Order 1 hands each XCD a contiguous band of rows: the dispatcher places
 // workgroup b on XCD b % 8, so numbering that XCD's workgroups b / 8 = 0, 1,
 // ... through a band keeps the neighbourhood reads of concurrently running
 // tiles in one XCD's L2 instead of spreading every region over all eight.
-// Both orders visit every tile exactly once.
-RGI_DEV void block_tile(uint32_t order, uint32_t& tx, uint32_t& ty)
+// Order 2 deals 4 x 4-tile blocks (64 x 64 pixels) to the XCDs in turn:
+// block s (row-major over blocks) goes to XCD s % 8, and that XCD's
+// workgroups walk its blocks tile by tile, so a block's tiles and most of
+// their neighbourhood share one L2 while every region of the screen still
+// spreads over all eight XCDs. Its grid is one-dimensional, 16 workgroups
+// per block slot; slots past the screen edge have no tile.
+// Every order visits every tile exactly once; false means this workgroup has
+// no tile (order 2 only), and the whole workgroup leaves.
+struct TileMap {
+    uint32_t order;
+    uint32_t tiles_x, tiles_y;
+    uint32_t blocks_x; // order 2: 4 x 4-tile blocks per row
+};
+
+RGI_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty)
 {
     uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
-    if (order) {
+    if (m.order == 2u) {
+        uint32_t k = b >> 3, s = (b & 7u) + 8u * (k >> 4), j = k & 15u;
+        tx = (s % m.blocks_x) * 4u + (j & 3u);
+        ty = (s / m.blocks_x) * 4u + (j >> 2);
+        return tx < m.tiles_x && ty < m.tiles_y;
+    }
+    if (m.order == 1u) {
         uint32_t n = gridDim.x * gridDim.y, q = n >> 3, r = n & 7u, xcd = b & 7u, i = b >> 3;
         b = xcd < r ? xcd * (q + 1u) + i : r * (q + 1u) + (xcd - r) * q + i;
     }
     tx = b % gridDim.x;
     ty = b / gridDim.x;
+    return true;
+}
+
+// Host side: the map and the grid of a width x height screen pass.
+inline TileMap tile_map(int order, uint32_t width, uint32_t height)
+{
+    TileMap m;
+    m.order = (uint32_t)order;
+    m.tiles_x = (width + 15u) / 16u;
+    m.tiles_y = (height + 15u) / 16u;
+    m.blocks_x = (m.tiles_x + 3u) / 4u;
+    return m;
+}
+
+inline dim3 tile_grid(const TileMap& m)
+{
+    if (m.order != 2u) return dim3(m.tiles_x, m.tiles_y);
+    uint32_t blocks = m.blocks_x * ((m.tiles_y + 3u) / 4u);
+    return dim3(128u * ((blocks + 7u) / 8u));
 }
 
 RGI_DEV F3 f3(float x, float y, float z) { return F3{x, y, z}; }

@@ -130,12 +130,13 @@ RGI_DEV bool same_surface(const DiPass& p, F3 position, F3 n, F3 other_position,
            fabsf(dot(off, n)) <= 0.03f * p.sc && dot(normalize(other_normal), n) >= 0.9f;
 }
 
-RGI_DEV void block_pixel(uint32_t order, int& x, int& y)
+RGI_DEV bool block_pixel(const TileMap& tiles, int& x, int& y)
 {
     uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, tx, ty;
-    block_tile(order, tx, ty);
+    if (!block_tile(tiles, tx, ty)) return false;
     x = (int)(tx * 16u + (wave & 1u) * 8u + (lane & 7u));
     y = (int)(ty * 16u + (wave >> 1) * 8u + (lane >> 3));
+    return true;
 }
 
 RGI_DEV DiPass make_pass(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, F2 texel)
@@ -151,11 +152,11 @@ RGI_DEV DiPass make_pass(const rgi_frame& f, const rgi_scene& s, const rgi_gbuff
 }
 
 __global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs prev,
-                                                     rgi_di_reservoirs out, F2 texel, uint32_t order)
+                                                     rgi_di_reservoirs out, F2 texel, TileMap tiles)
 {
     DiPass p = make_pass(f, s, g, texel);
     int x, y;
-    block_pixel(order, x, y);
+    if (!block_pixel(tiles, x, y)) return;
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -263,11 +264,11 @@ __global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, r
 
 __global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs res,
                                                     uint16_t* out_diffuse, uint16_t* out_specular, F2 texel,
-                                                    uint32_t order)
+                                                    TileMap tiles)
 {
     DiPass p = make_pass(f, s, g, texel);
     int x, y;
-    block_pixel(order, x, y);
+    if (!block_pixel(tiles, x, y)) return;
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -368,20 +369,20 @@ __global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rg
 int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
                 const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream)
 {
-    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
+    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    hipLaunchKernelGGL(k_di_temporal, blocks, dim3(256), 0, stream, f, s, g, prev, out, texel,
-                       (uint32_t)tuning.tile_order);
+    hipLaunchKernelGGL(k_di_temporal, tile_grid(tiles), dim3(256), 0, stream, f, s, g, prev, out,
+                       texel, tiles);
     return check_launch("di temporal");
 }
 
 int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
                uint16_t* diffuse, uint16_t* specular, const Tuning& tuning, hipStream_t stream)
 {
-    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
+    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    hipLaunchKernelGGL(k_di_spatial, blocks, dim3(256), 0, stream, f, s, g, res, diffuse, specular, texel,
-                       (uint32_t)tuning.tile_order);
+    hipLaunchKernelGGL(k_di_spatial, tile_grid(tiles), dim3(256), 0, stream, f, s, g, res,
+                       diffuse, specular, texel, tiles);
     return check_launch("di spatial");
 }
 

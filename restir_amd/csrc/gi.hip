@@ -56,7 +56,7 @@ RGI_DEV void stream_candidate(const GiPass& p, F3 position, F3 n, F3 albedo, F3 
 }
 
 RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                              const rgi_gi_reservoirs& out, F2 texel, uint32_t order, TemporalLds& lds)
+                              const rgi_gi_reservoirs& out, F2 texel, const TileMap& tiles, TemporalLds& lds)
 {
     int* ring = lds.ring;
     float4* q_origin = lds.q_origin;
@@ -75,7 +75,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     p.sc = f.scene_scale;
 
     int x, y;
-    group_pixel(order, x, y);
+    if (!group_pixel(tiles, x, y)) return;
     uint32_t lane = threadIdx.x & 63u, qbase = threadIdx.x & ~63u;
     bool inside = x < g.width && y < g.height;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -211,7 +211,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
 
 RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                              const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel,
-                             uint32_t order)
+                             const TileMap& tiles)
 {
     GiPass p;
     p.f = &f;
@@ -224,7 +224,7 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     p.sc = f.scene_scale;
 
     int x, y;
-    group_pixel(order, x, y);
+    if (!group_pixel(tiles, x, y)) return;
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -330,32 +330,32 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
 #define RGI_WAVES4 __attribute__((amdgpu_waves_per_eu(4, 4)))
 
 __global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
-                                                     rgi_gi_reservoirs out, F2 texel, uint32_t order)
+                                                     rgi_gi_reservoirs out, F2 texel, TileMap tiles)
 {
     __shared__ TemporalLds lds;
-    gi_temporal_body(f, s, g, in, out, texel, order, lds);
+    gi_temporal_body(f, s, g, in, out, texel, tiles, lds);
 }
 
 __global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs out, F2 texel,
-                 uint32_t order)
+                 TileMap tiles)
 {
     __shared__ TemporalLds lds;
-    gi_temporal_body(f, s, g, in, out, texel, order, lds);
+    gi_temporal_body(f, s, g, in, out, texel, tiles, lds);
 }
 
 __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                     rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
-                                                    F2 texel, uint32_t order)
+                                                    F2 texel, TileMap tiles)
 {
-    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, order);
+    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, tiles);
 }
 
 __global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
-                uint16_t* out_moments, F2 texel, uint32_t order)
+                uint16_t* out_moments, F2 texel, TileMap tiles)
 {
-    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, order);
+    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, tiles);
 }
 
 } // namespace
@@ -363,10 +363,10 @@ k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_g
 int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                 const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream)
 {
-    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
+    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     auto kernel = tuning.gi_temporal_waves == 4 ? k_gi_temporal_w4 : k_gi_temporal;
-    hipLaunchKernelGGL(kernel, blocks, dim3(256), 0, stream, f, s, g, in, out, texel, (uint32_t)tuning.tile_order);
+    hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, out, texel, tiles);
     return check_launch("gi temporal");
 }
 
@@ -374,11 +374,11 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
                const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
                hipStream_t stream)
 {
-    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
+    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     auto kernel = tuning.gi_spatial_waves == 4 ? k_gi_spatial_w4 : k_gi_spatial;
-    hipLaunchKernelGGL(kernel, blocks, dim3(256), 0, stream, f, s, g, in, res, color, moments, texel,
-                       (uint32_t)tuning.tile_order);
+    hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, res, color,
+                       moments, texel, tiles);
     return check_launch("gi spatial");
 }
 

@@ -128,13 +128,15 @@ RGI_DEV uint32_t rank_in(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Pixel of this thread: 16 x 16 per block, two 4 x 8 groups per wave.
-RGI_DEV void group_pixel(uint32_t order, int& x, int& y)
+// Pixel of this thread: 16 x 16 per block, two 4 x 8 groups per wave;
+// false when the workgroup has no tile.
+RGI_DEV bool group_pixel(const TileMap& tiles, int& x, int& y)
 {
     uint32_t group = threadIdx.x >> 5, t = threadIdx.x & 31u, tx, ty;
-    block_tile(order, tx, ty);
+    if (!block_tile(tiles, tx, ty)) return false;
     x = (int)(tx * 16u + (group & 3u) * 4u + (t & 3u));
     y = (int)(ty * 16u + (group >> 2) * 8u + (t >> 2));
+    return true;
 }
 
 // A marched ray of traceScreenSpaceRay: the point at distance t is

@@ -539,7 +539,7 @@ int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuf
         rgi::Tuning& t = r->ctx->tuning;
         t.gi_temporal_waves = (config & RGI_TUNE_TEMPORAL_WAVES4) ? 4 : 0;
         t.gi_spatial_waves = (config & RGI_TUNE_SPATIAL_WAVES4) ? 4 : 0;
-        t.tile_order = (config & RGI_TUNE_TILE_ORDER) ? 1 : 0;
+        t.tile_order = (config & RGI_TUNE_TILE_ORDER) ? 2 : 0;
     };
     int chosen = 0, e = RGI_OK;
     uint32_t k = 0;

@@ -91,7 +91,7 @@ RGI_DEV F3 hit_radiance(const GiPass& p, const uint16_t* previous_hdr, F3 hit, f
 // fs_specular.sc:127-226.
 __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                   const uint16_t* previous_hdr, const uint16_t* history,
-                                                  uint16_t* out, F2 texel, uint32_t order)
+                                                  uint16_t* out, F2 texel, TileMap tiles)
 {
     __shared__ SpecularLds lds;
     GiPass p;
@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_
     p.sc = f.scene_scale;
 
     int x, y;
-    group_pixel(order, x, y);
+    if (!group_pixel(tiles, x, y)) return;
     uint32_t lane = threadIdx.x & 63u, qbase = threadIdx.x & ~63u;
     bool inside = x < g.width && y < g.height;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -212,10 +212,10 @@ int specular_gi(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
                 const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out, const Tuning& tuning,
                 hipStream_t stream)
 {
-    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
+    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
-    hipLaunchKernelGGL(k_specular, blocks, dim3(256), 0, stream, f, s, g, in, previous_hdr, history, out, texel,
-                       (uint32_t)tuning.tile_order);
+    hipLaunchKernelGGL(k_specular, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, previous_hdr, history, out,
+                       texel, tiles);
     return check_launch("specular gi");
 }
 

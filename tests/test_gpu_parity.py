@@ -241,13 +241,13 @@ def test_svgf(dev, cornell_seq, step):
         assert_same("atrous", dev.get(atr_t, want_a), want_a)
 
 
-@pytest.mark.parametrize("case", ["default", "bvh_rays_sky"])
+@pytest.mark.parametrize("case", ["default", "bvh_rays_sky", "xcd_blocks"])
 def test_specular_gi(dev, cornell_seq, case):
     """rgi_specular_gi against the oracle, bit-exact, with last frame's
     composite and specular history."""
     hs, s, seq = cornell_seq
     f0, g, hist, out = seq[2]
-    kw = {} if case == "default" else {"ray_mode": ra.RGI_RAY_BVH, "sky": True}
+    kw = {"ray_mode": ra.RGI_RAY_BVH, "sky": True} if case == "bvh_rays_sky" else {}
     f = case_frame(hs, f0, **kw)
     say("case specular_gi %s" % case)
     n = W * H
@@ -267,8 +267,10 @@ def test_specular_gi(dev, cornell_seq, case):
     dgin = ra.GiInputs(None, d["rec"].data_ptr(), d["nor"].data_ptr(), d["col"].data_ptr(), d["mom"].data_ptr(),
                        d["sl"].data_ptr(), None, d["grid"].data_ptr())
     o = dev.empty(n * 8)
-    ra.check(dev.L.rgi_specular_gi(dev.ctx, ctypes.byref(f), ctypes.byref(dscene), ctypes.byref(gb), ctypes.byref(dgin),
-                                   d["hdr"].data_ptr(), d["spec"].data_ptr(), o.data_ptr(), dev.stream))
+    with options(dev, {ra.RGI_OPT_TILE_ORDER: 2} if case == "xcd_blocks" else {}):
+        ra.check(dev.L.rgi_specular_gi(dev.ctx, ctypes.byref(f), ctypes.byref(dscene), ctypes.byref(gb),
+                                       ctypes.byref(dgin), d["hdr"].data_ptr(), d["spec"].data_ptr(), o.data_ptr(),
+                                       dev.stream))
     got = dev.get(o, want)
     say("  nonzero %.3f" % (want.view(np.float16)[:, :3] != 0).any(1).mean())
     assert_same("specular", got, want)
@@ -423,6 +425,7 @@ def oracle_gi_spatial(hs, f, g, hist, res, grid, surface_light):
 
 
 XCD_TILES = {ra.RGI_OPT_TILE_ORDER: 1}
+XCD_BLOCKS = {ra.RGI_OPT_TILE_ORDER: 2}
 
 # Per case: settings overrides, and "options" = launch variants of the
 # context (temporal waves also select spatial waves in the spatial test).
@@ -431,6 +434,7 @@ GI_CASES = {
     "default_waves4": {"options": {ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
     "default_waves4_xcd": {"options": {**XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
     "default_xcd_tiles": {"options": XCD_TILES},
+    "default_xcd_blocks": {"options": XCD_BLOCKS},
     "three_candidates": {"initial_candidates": 3},
     "short_march": {"trace_steps": 5},
     "bvh_rays": {"ray_mode": ra.RGI_RAY_BVH},
@@ -565,7 +569,7 @@ def di_both(dev, hs, f, g, prev):
     return got, w
 
 
-@pytest.mark.parametrize("frame_no,extra,order", [(0, 4, 0), (2, 4, 0), (2, 0, 0), (2, 4, 1)])
+@pytest.mark.parametrize("frame_no,extra,order", [(0, 4, 0), (2, 4, 0), (2, 0, 0), (2, 4, 1), (2, 4, 2)])
 def test_di(dev, frame_no, extra, order):
     hs = common.cornell(extra_lights=extra)
     s = ra.Settings()
