@@ -201,28 +201,43 @@ def measure_mse(L, hs, gscene, gbuf, view, W, H, T, mode, ref_frames, frames, ct
 NRC_FLOP_PER_QUERY = 2 * (5 * 64 * 64 + 3 * 64)  # the MLP's multiply-adds per query (logical shapes)
 
 
+# The plain path given more samples, to at least the resolved cache's frame
+# time: twice the GI candidates, GI spatial taps and DI spatial taps, three
+# times the DI candidates.
+PLAIN_HEAVY = {"initial_candidates": 4, "di_candidates": 24, "di_spatial_taps": 6, "spatial_taps": 8}
+
+
 def nrc_quality(L, hs, gscene, gbuf, view, W, H, T, mode, acc, acc_gi, ref_frames, frames, ctx, s):
     """The neural radiance cache against the plain path, each rendered
     `frames` frames from an empty history at the static camera with the
     reference's default settings, measured against the same `ref_frames`
     reference accumulations: the GI estimate alone (the GI frame's output,
-    colour x albedo) and the composite."""
+    colour x albedo) and the composite. Variants: plain; the cache standing in
+    for path continuation; the same with the composite resolved from the
+    cache; the plain path with PLAIN_HEAVY's extra samples. Each variant's
+    full frame (denoiser and composite included) is timed over its last 32
+    frames."""
     import torch
     from restir_amd import nrc as rnrc
     res = torch.zeros(5, dtype=torch.float64, device="cuda")
     out = ra.RendererOutputs()
-    result = {"frames": frames, "reference_frames": ref_frames}
-    for name in ("plain", "cache"):
-        settings = ra.Settings(grid_table_size=T, grid_mode=mode)
+    result = {"frames": frames, "reference_frames": ref_frames, "plain_heavy_settings": PLAIN_HEAVY}
+    for name in ("plain", "cache", "resolved", "plain_heavy"):
+        settings = ra.Settings(grid_table_size=T, grid_mode=mode, **(PLAIN_HEAVY if name == "plain_heavy" else {}))
         rend = ctypes.c_void_p()
         ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
                                        ctypes.byref(gscene.scene), hs.radius))
-        if name == "cache":
-            ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs))))
-        for _ in range(frames):
+        if name in ("cache", "resolved"):
+            ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs, resolve=int(name == "resolved")))))
+        timed = min(32, frames)
+        for k in range(frames):
+            if k == frames - timed:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
             ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        torch.cuda.synchronize()
+        line = {"ms_per_frame": round((time.perf_counter() - t0) * 1e3 / timed, 4)}
         ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
-        line = {}
         ra.check(L.rgi_reference_error(ctx, ctypes.byref(gbuf.gb), out.color, None, None, acc_gi.data_ptr(),
                                        ref_frames, res.data_ptr(), s))
         r = res.cpu().numpy()
@@ -231,10 +246,64 @@ def nrc_quality(L, hs, gscene, gbuf, view, W, H, T, mode, acc, acc_gi, ref_frame
         ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), ref_frames,
                                                  res.data_ptr(), s))
         r = res.cpu().numpy()
-        line["composite"] = {"mse_rgb": float(r[1]), "relative_mse": float(r[0])}
+        line["composite"] = {"mse_rgb": float(r[1]), "relative_mse": float(r[0]), "mean_luminance": float(r[2]),
+                             "reference_mean_luminance": float(r[3])}
         result[name] = line
         L.rgi_renderer_destroy(rend)
+    comp = lambda k: result[k]["composite"]["mse_rgb"]
     result["gi_mse_plain_over_cache"] = round(result["plain"]["gi"]["mse_rgb"] / result["cache"]["gi"]["mse_rgb"], 4)
+    result["composite_mse_plain_over_resolved"] = round(comp("plain") / comp("resolved"), 3)
+    result["composite_mse_plain_heavy_over_resolved"] = round(comp("plain_heavy") / comp("resolved"), 3)
+    result["plain_heavy_frame_at_least_as_long"] = result["plain_heavy"]["ms_per_frame"] >= result["resolved"]["ms_per_frame"]
+    return result
+
+
+def nrc_motion_quality(L, hs, gscene, views, gbufs, T, mode, ctx, s, ref_frames=2048, loops=8, measured=(0, 4, 8, 12)):
+    """The resolved cache against the plain path with the camera moving: the
+    metric's swing path, `loops` times round, with the composite measured at
+    the `measured` views of the last loop against `ref_frames` reference
+    accumulations of each of those views (static camera, FP64)."""
+    import torch
+    from restir_amd import nrc as rnrc
+    W, H = gbufs[0].gb.width, gbufs[0].gb.height
+    n = W * H
+    settings = ra.Settings(grid_table_size=T, grid_mode=mode)
+    out = ra.RendererOutputs()
+    res = torch.zeros(5, dtype=torch.float64, device="cuda")
+    accs = {}
+    for v in measured:
+        acc = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+        rend = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                       ctypes.byref(gscene.scene), hs.radius))
+        ra.check(L.rgi_renderer_set_reference_mode(rend, 1))
+        for _ in range(ref_frames):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[v]), ctypes.byref(gbufs[v].gb), s))
+            ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+            ra.check(L.rgi_reference_accumulate_composite(ctypes.byref(gbufs[v].gb), out.hdr, acc.data_ptr(), s))
+        L.rgi_renderer_destroy(rend)
+        accs[v] = acc
+    path = len(views)
+    result = {"reference_frames": ref_frames, "path_views": path, "loops": loops, "measured_views": list(measured)}
+    for name in ("plain", "resolved"):
+        rend = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                       ctypes.byref(gscene.scene), hs.radius))
+        if name == "resolved":
+            ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs, resolve=1))))
+        mses = []
+        for k in range(loops * path):
+            v = k % path
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[v]), ctypes.byref(gbufs[v].gb), s))
+            if k >= (loops - 1) * path and v in accs:
+                ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+                ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbufs[v].gb), out.hdr, accs[v].data_ptr(),
+                                                         ref_frames, res.data_ptr(), s))
+                mses.append(float(res.cpu().numpy()[1]))
+        result[name] = {"composite_mse_rgb": [round(m, 9) for m in mses], "mean": sum(mses) / len(mses)}
+        L.rgi_renderer_destroy(rend)
+    result["composite_mse_plain_over_resolved"] = round(result["plain"]["mean"] / result["resolved"]["mean"], 3)
+    del accs
     return result
 
 
@@ -754,6 +823,7 @@ def main():
         if args.nrc:
             nrc["quality"] = nrc_quality(L, hs, gscene, gbufs[0], views[0], W, H, T, mode, acc, acc_gi,
                                          args.reference_frames, args.nrc_frames, drv.ctx, s)
+            nrc["quality_moving_camera"] = nrc_motion_quality(L, hs, gscene, views, gbufs, T, mode, drv.ctx, s)
         del acc, acc_gi
 
     cpu = None

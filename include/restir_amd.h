@@ -334,6 +334,12 @@ typedef struct rgi_composite_inputs {
     const uint16_t* di_specular; /* RGBA16F (NULL: none) */
     const uint16_t* specular;    /* RGBA16F demodulated indirect specular (NULL: none) */
     const float* shadow_map;     /* R32F shadow atlas (NULL: every light sample visible) */
+    /* RGBA16F, NULL for the reference's composite. When given, the pixel's
+     * whole diffuse light, taken as is in place of di_diffuse plus the
+     * re-modulated GI irradiance (the neural radiance cache's per-pixel
+     * prediction, include/restir_amd_nrc.h); the indirect specular still
+     * comes from the GI planes. No counterpart in the reference. */
+    const uint16_t* resolved_diffuse;
     float options[4];            /* u_compositeOptions: direct, indirect, albedo modulation, emission (final image: 1, 1, 1, 1) */
     float specular_weight;       /* u_compositeSpecular.x: 1 with indirect specular */
     float reserved[3];

@@ -152,6 +152,16 @@ int rgi_nrc_gather_gbuffer(const rgi_frame* frame, const rgi_gbuffer* gbuffer, c
  *     next frame's GI history colour (the ReSTIR estimate of the light the
  *     surface reflects, which already contains the cache's own prediction at
  *     the sample points: self-training), and takes one Adam step.
+ * With `resolve` set (and multiple bounces on, so that the target is the
+ * surface's whole diffuse light), the composite also takes the query of
+ * step 1 at each pixel as the pixel's diffuse light, in place of the DI
+ * diffuse light plus the re-modulated GI (rgi_composite_inputs::
+ * resolved_diffuse): the cache's average over every frame it has learnt
+ * from replaces one frame's DI and GI estimates, so their noise leaves the
+ * image, and the cache's error (spatial detail finer than its finest
+ * level, lag behind changed lighting) takes its place. Ambient occlusion
+ * (emission.a) does not scale the resolved light. The GI frame's own
+ * outputs (reservoirs, colour, history) are unchanged by it.
  * Until `warmup` frames have trained, and whenever the reference
  * accumulates, the frame is the plain ReSTIR path. */
 typedef struct rgi_renderer rgi_renderer;
@@ -161,7 +171,7 @@ typedef struct rgi_nrc_settings {
     uint32_t batch;           /* 65536: training records per frame */
     uint32_t warmup;          /* 4: training frames before the GI passes read the cache */
     int32_t precision;        /* RGI_NRC_BF16: precision of the per-pixel queries */
-    int32_t reserved;
+    int32_t resolve;          /* 0; 1: the composite takes the cache's prediction as each pixel's diffuse light */
 } rgi_nrc_settings;
 
 void rgi_nrc_settings_default(rgi_nrc_settings* settings);

@@ -1606,7 +1606,10 @@ void orc_composite(const rgi_frame* f, const rgi_gbuffer* g, const rgi_composite
         if (dot(n, light_direction) > 0.0f)
             visibility = light_visibility_soft(f, in->shadow_map, position, surface_normal);
         V3 light_color = V3{f->light_color[0], f->light_color[1], f->light_color[2]};
-        V3 direct = (((diffuse + specular) * (light_color / distance_squared)) * visibility + xyz(h4(in->di_diffuse, t))) +
+        // A resolved diffuse plane (the cache's prediction; no counterpart in
+        // the reference) stands for the DI diffuse light and the GI together.
+        V3 diffuse_light = in->resolved_diffuse ? xyz(h4(in->resolved_diffuse, t)) : xyz(h4(in->di_diffuse, t));
+        V3 direct = (((diffuse + specular) * (light_color / distance_squared)) * visibility + diffuse_light) +
                     xyz(h4(in->di_specular, t));
         // upsampleIndirect (fs_composite.sc:48-80).
         float tcx = uv.x * gi_size.x - 0.5f, tcy = uv.y * gi_size.y - 0.5f;
@@ -1635,7 +1638,7 @@ void orc_composite(const rgi_frame* f, const rgi_gbuffer* g, const rgi_composite
         V3 irradiance = weight_sum > 0.0f ? sum / weight_sum : xyz(h4(in->gi, own));
         V3 specular_radiance = weight_sum > 0.0f ? specular_sum / weight_sum : xyz(h4(in->specular, own));
         V3 modulate = in->options[2] > 0.5f ? (albedo * (1.0f - metallic)) * eo.w : v3s(1.0f);
-        V3 indirect = irradiance * modulate;
+        V3 indirect = in->resolved_diffuse ? v3s(0.0f) : irradiance * modulate;
         V3 f0 = mix3(v3s(0.04f), albedo, metallic);
         indirect = indirect + ((specular_radiance * specular_albedo(f0, roughness, fmax_(dot(n, view_direction), 0.001f))) *
                                eo.w) *

@@ -118,7 +118,7 @@ class Shadow(ctypes.Structure):
 class CompositeInputs(ctypes.Structure):
     """rgi_composite_inputs: the composite pass's planes and u_compositeOptions."""
     _fields_ = [("gi", P), ("di_diffuse", P), ("di_specular", P), ("specular", P), ("shadow_map", P),
-                ("options", F * 4), ("specular_weight", F), ("reserved", F * 3)]
+                ("resolved_diffuse", P), ("options", F * 4), ("specular_weight", F), ("reserved", F * 3)]
 
 
 class Mesh(ctypes.Structure):
@@ -135,7 +135,7 @@ class NrcConfig(ctypes.Structure):
 
 class NrcSettings(ctypes.Structure):
     """rgi_nrc_settings (include/restir_amd_nrc.h): the cache in the frame driver."""
-    _fields_ = [("cache", NrcConfig), ("batch", U32), ("warmup", U32), ("precision", I32), ("reserved", I32)]
+    _fields_ = [("cache", NrcConfig), ("batch", U32), ("warmup", U32), ("precision", I32), ("resolve", I32)]
 
 
 RGI_NRC_BF16, RGI_NRC_FP16, RGI_NRC_FP8 = 0, 1, 2

@@ -3,7 +3,8 @@
 // the ReSTIR DI light, the GI irradiance upsampled from the GI planes over
 // same-surface texels and re-modulated by the albedo, indirect specular and
 // emission; the sky where the depth is the far plane. One thread per pixel,
-// 16 x 16 pixels per block.
+// 16 x 16 pixels per block. A resolved diffuse plane, when given, replaces
+// the DI diffuse light and the re-modulated GI irradiance together.
 #include "internal.h"
 #include "shading.h"
 
@@ -92,14 +93,14 @@ __global__ void __launch_bounds__(256) k_composite(rgi_frame f, rgi_gbuffer g, r
     float visibility = 0.0f;
     if (dot(n, light_direction) > 0.0f) visibility = light_visibility_soft(f, in.shadow_map, position, surface_normal);
     F3 light_color = F3{f.light_color[0], f.light_color[1], f.light_color[2]};
-    F3 direct = (((diffuse + specular) * (light_color / distance_squared)) * visibility +
-                 xyz(h4_or_zero(in.di_diffuse, t))) +
+    F3 diffuse_light = in.resolved_diffuse ? xyz(load_h4(in.resolved_diffuse, t)) : xyz(h4_or_zero(in.di_diffuse, t));
+    F3 direct = (((diffuse + specular) * (light_color / distance_squared)) * visibility + diffuse_light) +
                 xyz(h4_or_zero(in.di_specular, t));
 
     F3 irradiance, specular_radiance;
     upsample_indirect(f, g, in, uv, position, surface_normal, irradiance, specular_radiance);
     F3 modulate = in.options[2] > 0.5f ? (albedo * (1.0f - metallic)) * emissive_occlusion.w : splat3(1.0f);
-    F3 indirect = irradiance * modulate;
+    F3 indirect = in.resolved_diffuse ? splat3(0.0f) : irradiance * modulate;
     F3 f0 = mix3(splat3(0.04f), albedo, metallic);
     indirect = indirect + ((specular_radiance * specular_albedo(f0, roughness, maxf_(dot(n, view_direction), 0.001f))) *
                            emissive_occlusion.w) *

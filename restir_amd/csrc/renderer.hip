@@ -48,6 +48,7 @@ struct rgi_renderer {
     rgi_nrc* nrc = nullptr;
     rgi_nrc_settings nrc_settings{};
     uint16_t* nrc_plane = nullptr;
+    uint16_t* nrc_resolved = nullptr; // the query without emission, when the plane above carries it
     float* nrc_records = nullptr;
     float* nrc_targets = nullptr;
     float* nrc_loss = nullptr;
@@ -88,6 +89,7 @@ void release_nrc(rgi_renderer* r)
     if (r->nrc) rgi_nrc_destroy(r->nrc);
     r->nrc = nullptr;
     release(r->nrc_plane);
+    release(r->nrc_resolved);
     release(r->nrc_records);
     release(r->nrc_targets);
     release(r->nrc_loss);
@@ -390,6 +392,7 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     // frame's indirect light of on-screen sample points once it has trained.
     const bool nrc_on = r->nrc && !r->reference;
     rgi_frame fg = f;
+    const uint16_t* resolved_diffuse = nullptr;
     if (nrc_on && r->nrc_trained >= r->nrc_settings.warmup) {
         const uint16_t* emission = (!s.restir_di && s.emissive_lighting) ? g->emission : nullptr;
         if ((e = nrc_infer_gbuffer(r->nrc, f, *g, emission, r->nrc_settings.precision, r->nrc_plane, stream)) !=
@@ -398,6 +401,17 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         surface_light = r->nrc_plane;
         fg.multiple_bounces = 0;
         fg.lighting_surface = 1.0f;
+        // The composite's diffuse light: the cache's prediction itself, which
+        // covers DI and GI only when the training target holds both.
+        if (r->nrc_settings.resolve && s.multiple_bounces) {
+            resolved_diffuse = r->nrc_plane;
+            if (emission) {
+                if ((e = nrc_infer_gbuffer(r->nrc, f, *g, nullptr, r->nrc_settings.precision, r->nrc_resolved,
+                                           stream)) != RGI_OK)
+                    return e;
+                resolved_diffuse = r->nrc_resolved;
+            }
+        }
     }
     rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                       surface_light, shadow_map, r->grid[read]};
@@ -463,6 +477,7 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         ci.di_specular = di_active ? r->specular : nullptr;
         ci.specular = specular;
         ci.shadow_map = shadow_map;
+        ci.resolved_diffuse = resolved_diffuse;
         for (int k = 0; k < 4; ++k) ci.options[k] = 1.0f;
         ci.specular_weight = specular ? 1.0f : 0.0f;
         if ((e = composite(f, *g, ci, r->hdr, stream)) != RGI_OK) return e;
@@ -619,6 +634,8 @@ int rgi_renderer_set_nrc(rgi_renderer* r, const rgi_nrc_settings* settings)
         return fail(RGI_E_INVALID, "set_nrc: batch %u is not in 1..2^24", settings ? settings->batch : 0u);
     if (settings && (settings->precision < RGI_NRC_BF16 || settings->precision > RGI_NRC_FP8))
         return fail(RGI_E_INVALID, "set_nrc: unknown precision %d", settings->precision);
+    if (settings && settings->resolve != 0 && settings->resolve != 1)
+        return fail(RGI_E_INVALID, "set_nrc: resolve takes 0 or 1, not %d", settings->resolve);
     (void)hipDeviceSynchronize();
     release_nrc(r);
     if (!settings) return RGI_OK;
@@ -626,7 +643,8 @@ int rgi_renderer_set_nrc(rgi_renderer* r, const rgi_nrc_settings* settings)
     if (e != RGI_OK) return e;
     r->nrc_settings = *settings;
     size_t n = (size_t)r->width * r->height;
-    if (!alloc(r->nrc_plane, 4 * n) || !alloc(r->nrc_records, (size_t)12 * settings->batch) ||
+    if (!alloc(r->nrc_plane, 4 * n) || (settings->resolve && !alloc(r->nrc_resolved, 4 * n)) ||
+        !alloc(r->nrc_records, (size_t)12 * settings->batch) ||
         !alloc(r->nrc_targets, (size_t)4 * settings->batch) || !alloc(r->nrc_loss, 1)) {
         release_nrc(r);
         return fail(RGI_E_NOMEM, "set_nrc: buffers could not be allocated");

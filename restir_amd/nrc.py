@@ -96,7 +96,7 @@ class NeuralCache:
         return s.value
 
 
-def renderer_settings(hs, batch=65536, warmup=4, precision=ra.RGI_NRC_BF16, margin=0.02, **kw):
+def renderer_settings(hs, batch=65536, warmup=4, precision=ra.RGI_NRC_BF16, resolve=0, margin=0.02, **kw):
     """rgi_nrc_settings for a HostScene: the cache's bounds are the scene's
     triangle bounds, widened by `margin` of their extent."""
     st = ra.NrcSettings()
@@ -105,7 +105,7 @@ def renderer_settings(hs, batch=65536, warmup=4, precision=ra.RGI_NRC_BF16, marg
     lo, hi = p.min(axis=0), p.max(axis=0)
     pad = (hi - lo) * margin + 1e-3
     st.cache = config(aabb_min=lo - pad, aabb_max=hi + pad, **kw)
-    st.batch, st.warmup, st.precision = batch, warmup, precision
+    st.batch, st.warmup, st.precision, st.resolve = batch, warmup, precision, resolve
     return st
 
 

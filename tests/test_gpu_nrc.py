@@ -295,6 +295,63 @@ def test_renderer_cache_beats_the_surface_light_it_replaces():
         L.rgi_destroy(ctx)
 
 
+def test_renderer_resolve_lowers_composite_error():
+    """What the resolved cache gains on the frame's image: against an FP64
+    accumulation of 1024 reference frames at a static camera, the composite
+    of a renderer whose cache resolves each pixel's diffuse light has a
+    several times lower MSE than the plain renderer's after 96 frames (one
+    frame's DI and GI noise against the cache's average over every frame it
+    has learnt from). Before the cache is warm the image is the plain
+    path's, bit for bit."""
+    from restir_amd import driver
+    W, H = 320, 240
+    n = W * H
+    ref_frames, frames = 1024, 96
+    st = ra.Settings(grid_table_size=1 << 16, grid_mode=ra.RGI_GRID_DETERMINISTIC)
+    hs, gscene, ctx, gbuf, view = _setup_scene(W, H, st)
+    L = ra.lib()
+    s = driver.stream()
+    acc = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    res = torch.zeros(5, dtype=torch.float64, device="cuda")
+    out_p, out_c = ra.RendererOutputs(), ra.RendererOutputs()
+    ref = _renderer(hs, gscene, st, W, H)
+    plain = _renderer(hs, gscene, st, W, H)
+    cached = _renderer(hs, gscene, st, W, H, rnrc.renderer_settings(hs, batch=16384, warmup=4, resolve=1))
+    try:
+        ra.check(L.rgi_renderer_set_reference_mode(ref, 1))
+        for _ in range(ref_frames):
+            ra.check(L.rgi_renderer_render(ref, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+            ra.check(L.rgi_renderer_outputs_get(ref, ctypes.byref(out_p)))
+            ra.check(L.rgi_reference_accumulate_composite(ctypes.byref(gbuf.gb), out_p.hdr, acc.data_ptr(), s))
+        for frame in range(frames):
+            for r in (plain, cached):
+                ra.check(L.rgi_renderer_render(r, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+            if frame < 4:
+                ra.check(L.rgi_renderer_outputs_get(plain, ctypes.byref(out_p)))
+                ra.check(L.rgi_renderer_outputs_get(cached, ctypes.byref(out_c)))
+                a, b = _half_plane(out_p.hdr, n), _half_plane(out_c.hdr, n)
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), frame
+        mse = {}
+        for name, r in (("plain", plain), ("resolved", cached)):
+            ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(out_p)))
+            hdr = _half_plane(out_p.hdr, n)
+            assert np.all(np.isfinite(hdr))
+            ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbuf.gb), out_p.hdr, acc.data_ptr(),
+                                                     ref_frames, res.data_ptr(), s))
+            r5 = res.cpu().numpy()
+            mse[name] = float(r5[1])
+            print("%s: composite MSE %.4e, mean luminance %.5f (reference %.5f)" % (name, r5[1], r5[2], r5[3]),
+                  flush=True)
+            if name == "resolved":
+                assert abs(r5[2] - r5[3]) < 0.1 * r5[3]
+        print("plain / resolved MSE: %.1f" % (mse["plain"] / mse["resolved"]), flush=True)
+        assert mse["resolved"] < 0.5 * mse["plain"]
+    finally:
+        for r in (ref, plain, cached):
+            L.rgi_renderer_destroy(r)
+        L.rgi_destroy(ctx)
+
+
 def test_gbuffer_query_matches_records_query():
     """The per-pixel query reads a pixel's inputs exactly as the training
     gather records them: gathering every pixel (batch = pixel count puts

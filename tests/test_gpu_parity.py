@@ -276,24 +276,30 @@ def test_specular_gi(dev, cornell_seq, case):
     assert_same("specular", got, want)
 
 
-def test_composite(dev):
+@pytest.mark.parametrize("resolved", [False, True])
+def test_composite(dev, resolved):
     """rgi_composite against the oracle, bit-exact: main light with the 4 x 4
-    soft shadow compare of a real atlas, DI planes, denoised GI and the sky."""
+    soft shadow compare of a real atlas, DI planes, denoised GI and the sky;
+    with `resolved`, a resolved diffuse plane in place of the DI diffuse
+    light and the GI irradiance."""
     hs = common.cornell(extra_lights=4)
     s = ra.Settings(shadows=True, light_color=(1.0, 0.9, 0.8), light_position=(0.1, 1.2, -0.2), shadow_face_size=0,
                     sky=True)
     seq = common.oracle_sequence(hs, s, W, H, 2, yaw_step=2 * np.pi / 64)
     f, g, hist, out = seq[-1]
-    say("case composite")
+    say("case composite resolved=%d" % resolved)
     gi, dd, ds, atlas = out["denoised"], out["diffuse"], out["specular"], out["shadow_atlas"]
+    rd = np.random.default_rng(11).uniform(0, 2, (W * H, 4)).astype(np.float16).view(np.uint16)
     ci = ra.CompositeInputs(gi.ctypes.data, dd.ctypes.data, ds.ctypes.data, None, atlas.ctypes.data)
+    ci.resolved_diffuse = rd.ctypes.data if resolved else None
     ci.options[:] = [1.0, 1.0, 1.0, 1.0]
     want = np.zeros((W * H, 4), np.uint16)
     orc.lib().orc_composite(ctypes.byref(f), ctypes.byref(common.host_gbuffer(g, W, H)), ctypes.byref(ci),
                             orc.ptr(want))
     gb, keep = dev.gbuffer(g, W, H)
-    d = {k: dev.put(v) for k, v in (("gi", gi), ("dd", dd), ("ds", ds), ("atlas", atlas))}
+    d = {k: dev.put(v) for k, v in (("gi", gi), ("dd", dd), ("ds", ds), ("atlas", atlas), ("rd", rd))}
     dci = ra.CompositeInputs(d["gi"].data_ptr(), d["dd"].data_ptr(), d["ds"].data_ptr(), None, d["atlas"].data_ptr())
+    dci.resolved_diffuse = d["rd"].data_ptr() if resolved else None
     dci.options[:] = [1.0, 1.0, 1.0, 1.0]
     o = dev.empty(W * H * 8)
     ra.check(dev.L.rgi_composite(dev.ctx, ctypes.byref(f), ctypes.byref(gb), ctypes.byref(dci), o.data_ptr(),
