@@ -41,6 +41,10 @@ struct rgi_renderer {
     uint16_t* black = nullptr; // m_blackTexture: DI light when ReSTIR DI is selected but has no lights
     uint16_t* specular_gi[2] = {}; // m_specularTextures, RGBA16F, own parity m_specularRead
     int specular_read = 0;
+    // The side stream that builds the frame's hash grid beside the DI passes
+    // and GI temporal (default priority), with its fork and join events.
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
     // The graph form of a frame (rgi_renderer_render_graph).
     hipGraphExec_t graph_exec = nullptr;
     bool graph_warm = false;
@@ -119,6 +123,11 @@ void free_all(rgi_renderer* r)
         release(r->filter[k]);
     }
     release_nrc(r);
+    if (r->side_fork) hipEventDestroy(r->side_fork);
+    if (r->side_join) hipEventDestroy(r->side_join);
+    if (r->side) hipStreamDestroy(r->side);
+    r->side_fork = r->side_join = nullptr;
+    r->side = nullptr;
 }
 
 // kShadowFaceSizes (settings.h:26); 0 for an index out of range.
@@ -296,6 +305,9 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         size_t F = (size_t)shadow_face_size(settings);
         ok = ok && alloc(r->atlas, 6 * F * F);
     }
+    ok = ok && hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&r->side_fork, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&r->side_join, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         free_all(r);
         rgi_destroy(r->ctx);
@@ -334,7 +346,8 @@ void rgi_renderer_invalidate_history(rgi_renderer* r)
     r->di_valid = false;
 }
 
-int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
+// One frame; `fork_grid` builds the hash grid on the renderer's side stream.
+static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr, bool fork_grid)
 {
     if (!r || !view || !g) return fail(RGI_E_INVALID, "renderer_render: NULL argument");
     if (g->width != r->width || g->height != r->height)
@@ -367,24 +380,28 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         }
         shadow_map = r->atlas;
     }
+    // This frame's hash grid (renderer.cpp:735-750). Only GI spatial and the
+    // specular pass read it (GI temporal reads last frame's), so with
+    // `fork_grid` it is built on the side stream beside the DI passes and GI
+    // temporal, and joined before GI spatial. The grid buffer it clears was
+    // last read by last frame's GI temporal, which precedes the fork on
+    // `stream`.
+    hipStream_t grid_stream = fork_grid ? r->side : stream;
+    if (fork_grid) {
+        hipEventRecord(r->side_fork, stream);
+        hipStreamWaitEvent(r->side, r->side_fork, 0);
+    }
+    e = grid_clear(r->grid[write], f.grid_table_size, grid_stream);
+    if (e == RGI_OK) {
+        if (s.grid_mode == RGI_GRID_DETERMINISTIC)
+            e = grid_insert_deterministic(r->ctx, f, *g, r->grid[write], grid_stream);
+        else
+            e = grid_insert_atomic(f, *g, r->grid[write], grid_stream);
+    }
+    if (fork_grid) hipEventRecord(r->side_join, r->side);
     // The GI passes' surface light (renderer.cpp:690-695): the DI diffuse
     // light with ReSTIR DI (black when it has no lights), else the emission.
     const uint16_t* surface_light = s.restir_di ? r->black : g->emission;
-    if (di_active) {
-        rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
-        rgi_di_reservoirs next_di{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
-        if ((e = di_temporal(f, r->scene, *g, prev_di, next_di, r->ctx->tuning, stream)) != RGI_OK) return e;
-        if ((e = di_spatial(f, r->scene, *g, next_di, r->diffuse, r->specular, r->ctx->tuning, stream)) != RGI_OK)
-            return e;
-        surface_light = r->diffuse;
-        r->di_valid = true;
-    } else {
-        r->di_valid = false;
-    }
-    if ((e = grid_clear(r->grid[write], f.grid_table_size, stream)) != RGI_OK) return e;
-    if (s.grid_mode == RGI_GRID_DETERMINISTIC) e = grid_insert_deterministic(r->ctx, f, *g, r->grid[write], stream);
-    else e = grid_insert_atomic(f, *g, r->grid[write], stream);
-    if (e != RGI_OK) return e;
     // With the denoiser, the GI history colour is its first filtered
     // iteration (renderer.cpp:774-776).
     const uint16_t* history_color = s.denoiser ? r->filtered[read] : r->color[read];
@@ -393,30 +410,50 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     const bool nrc_on = r->nrc && !r->reference;
     rgi_frame fg = f;
     const uint16_t* resolved_diffuse = nullptr;
-    if (nrc_on && r->nrc_trained >= r->nrc_settings.warmup) {
-        const uint16_t* emission = (!s.restir_di && s.emissive_lighting) ? g->emission : nullptr;
-        if ((e = nrc_infer_gbuffer(r->nrc, f, *g, emission, r->nrc_settings.precision, r->nrc_plane, stream)) !=
-            RGI_OK)
-            return e;
-        surface_light = r->nrc_plane;
-        fg.multiple_bounces = 0;
-        fg.lighting_surface = 1.0f;
-        // The composite's diffuse light: the cache's prediction itself, which
-        // covers DI and GI only when the training target holds both.
-        if (r->nrc_settings.resolve && s.multiple_bounces) {
-            resolved_diffuse = r->nrc_plane;
-            if (emission) {
-                if ((e = nrc_infer_gbuffer(r->nrc, f, *g, nullptr, r->nrc_settings.precision, r->nrc_resolved,
-                                           stream)) != RGI_OK)
-                    return e;
-                resolved_diffuse = r->nrc_resolved;
+    // The DI passes, the cache's query and GI temporal, beside the grid.
+    auto front = [&]() -> int {
+        int fe;
+        if (di_active) {
+            rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
+            rgi_di_reservoirs next_di{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
+            if ((fe = di_temporal(f, r->scene, *g, prev_di, next_di, r->ctx->tuning, stream)) != RGI_OK) return fe;
+            if ((fe = di_spatial(f, r->scene, *g, next_di, r->diffuse, r->specular, r->ctx->tuning, stream)) !=
+                RGI_OK)
+                return fe;
+            surface_light = r->diffuse;
+            r->di_valid = true;
+        } else {
+            r->di_valid = false;
+        }
+        if (nrc_on && r->nrc_trained >= r->nrc_settings.warmup) {
+            const uint16_t* emission = (!s.restir_di && s.emissive_lighting) ? g->emission : nullptr;
+            if ((fe = nrc_infer_gbuffer(r->nrc, f, *g, emission, r->nrc_settings.precision, r->nrc_plane, stream)) !=
+                RGI_OK)
+                return fe;
+            surface_light = r->nrc_plane;
+            fg.multiple_bounces = 0;
+            fg.lighting_surface = 1.0f;
+            // The composite's diffuse light: the cache's prediction itself,
+            // which covers DI and GI only when the training target holds both.
+            if (r->nrc_settings.resolve && s.multiple_bounces) {
+                resolved_diffuse = r->nrc_plane;
+                if (emission) {
+                    if ((fe = nrc_infer_gbuffer(r->nrc, f, *g, nullptr, r->nrc_settings.precision, r->nrc_resolved,
+                                                stream)) != RGI_OK)
+                        return fe;
+                    resolved_diffuse = r->nrc_resolved;
+                }
             }
         }
-    }
-    rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
-                      surface_light, shadow_map, r->grid[read]};
+        rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
+                          surface_light, shadow_map, r->grid[read]};
+        rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
+        return gi_temporal(fg, r->scene, *g, tin, res, r->ctx->tuning, stream);
+    };
+    if (e == RGI_OK) e = front();
+    if (fork_grid) hipStreamWaitEvent(stream, r->side_join, 0);
+    if (e != RGI_OK) return e;
     rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
-    if ((e = gi_temporal(fg, r->scene, *g, tin, res, r->ctx->tuning, stream)) != RGI_OK) return e;
     rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                       surface_light, shadow_map, r->grid[write]};
     if ((e = gi_spatial(fg, r->scene, *g, sin, res, r->color[write], r->moments[write], r->ctx->tuning, stream)) !=
@@ -491,6 +528,11 @@ int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     ++r->frame_number;
     r->last = f;
     return RGI_OK;
+}
+
+int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
+{
+    return render_frame(r, view, g, stream_ptr, true);
 }
 
 int rgi_renderer_render_graph(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
@@ -732,7 +774,8 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
     int e = RGI_OK;
     if (!b->warm) {
         // First frame eagerly, on the caller's stream (allocations happen here).
-        for (size_t i = 0; i < n && e == RGI_OK; ++i) e = rgi_renderer_render(b->renderers[i], &views[i], &gbuffers[i], stream);
+        for (size_t i = 0; i < n && e == RGI_OK; ++i)
+            e = render_frame(b->renderers[i], &views[i], &gbuffers[i], stream, false);
         if (e == RGI_OK) b->warm = true;
         return e;
     }
@@ -741,7 +784,9 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
     hipEventRecord(b->fork, stream);
     for (size_t i = 0; i < n && e == RGI_OK; ++i) {
         hipStreamWaitEvent(b->streams[i], b->fork, 0);
-        e = rgi_renderer_render(b->renderers[i], &views[i], &gbuffers[i], b->streams[i]);
+        // Each renderer's frame is already a branch of its own; its grid
+        // stays on that branch (no nested fork inside the batch's capture).
+        e = render_frame(b->renderers[i], &views[i], &gbuffers[i], b->streams[i], false);
         hipEventRecord(b->joined[i], b->streams[i]);
         hipStreamWaitEvent(stream, b->joined[i], 0);
     }
