@@ -151,7 +151,13 @@ int rgi_nrc_gather_gbuffer(const rgi_frame* frame, const rgi_gbuffer* gbuffer, c
  *     stratified over the image, each with target DI diffuse + albedo * the
  *     next frame's GI history colour (the ReSTIR estimate of the light the
  *     surface reflects, which already contains the cache's own prediction at
- *     the sample points: self-training), and takes one Adam step.
+ *     the sample points: self-training), and takes one Adam step. The step
+ *     runs on a stream of the renderer's own, forked from the caller's: in
+ *     rgi_renderer_render it may still run when the call's work on `stream`
+ *     is done (beside the next frame's DI passes), and the next frame waits
+ *     for it before it reads the cache; rgi_renderer_render_graph and
+ *     batches join it within the frame. rgi_renderer_nrc_state waits for it
+ *     on the host, so the state it returns is the last frame's.
  * With `resolve` set (and multiple bounces on, so that the target is the
  * surface's whole diffuse light), the composite also takes the query of
  * step 1 at each pixel as the pixel's diffuse light, in place of the DI

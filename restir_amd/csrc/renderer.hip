@@ -45,6 +45,11 @@ struct rgi_renderer {
     // and GI temporal (default priority), with its fork and join events.
     hipStream_t side = nullptr;
     hipEvent_t side_fork = nullptr, side_join = nullptr;
+    // The cache's training step runs on its own stream (default priority):
+    // `nrc_pending` marks a step that the caller's stream has not joined yet.
+    hipStream_t nrc_stream = nullptr;
+    hipEvent_t nrc_fork = nullptr, nrc_done = nullptr;
+    bool nrc_pending = false;
     // The graph form of a frame (rgi_renderer_render_graph).
     hipGraphExec_t graph_exec = nullptr;
     bool graph_warm = false;
@@ -98,6 +103,7 @@ void release_nrc(rgi_renderer* r)
     release(r->nrc_targets);
     release(r->nrc_loss);
     r->nrc_trained = 0;
+    r->nrc_pending = false;
 }
 
 void free_all(rgi_renderer* r)
@@ -123,6 +129,12 @@ void free_all(rgi_renderer* r)
         release(r->filter[k]);
     }
     release_nrc(r);
+    if (r->nrc_fork) hipEventDestroy(r->nrc_fork);
+    if (r->nrc_done) hipEventDestroy(r->nrc_done);
+    if (r->nrc_stream) hipStreamDestroy(r->nrc_stream);
+    r->nrc_fork = r->nrc_done = nullptr;
+    r->nrc_stream = nullptr;
+    r->nrc_pending = false;
     if (r->side_fork) hipEventDestroy(r->side_fork);
     if (r->side_join) hipEventDestroy(r->side_join);
     if (r->side) hipStreamDestroy(r->side);
@@ -307,7 +319,10 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
     }
     ok = ok && hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&r->side_fork, hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&r->side_join, hipEventDisableTiming) == hipSuccess;
+         hipEventCreateWithFlags(&r->side_join, hipEventDisableTiming) == hipSuccess &&
+         hipStreamCreateWithFlags(&r->nrc_stream, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&r->nrc_fork, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&r->nrc_done, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         free_all(r);
         rgi_destroy(r->ctx);
@@ -346,8 +361,21 @@ void rgi_renderer_invalidate_history(rgi_renderer* r)
     r->di_valid = false;
 }
 
-// One frame; `fork_grid` builds the hash grid on the renderer's side stream.
-static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr, bool fork_grid)
+// The caller's stream waits for a training step still running on the
+// cache's stream.
+static void join_nrc(rgi_renderer* r, hipStream_t stream)
+{
+    if (r->nrc_pending) hipStreamWaitEvent(stream, r->nrc_done, 0);
+    r->nrc_pending = false;
+}
+
+// One frame. `fork_grid` builds the hash grid on the renderer's side stream
+// and trains the cache on the cache's stream; `cross_frame` leaves that
+// training running past the end of the frame, joined where the next frame
+// first touches the cache (eager frames only: a captured frame joins every
+// branch before its capture ends).
+static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr, bool fork_grid,
+                        bool cross_frame)
 {
     if (!r || !view || !g) return fail(RGI_E_INVALID, "renderer_render: NULL argument");
     if (g->width != r->width || g->height != r->height)
@@ -426,6 +454,7 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
             r->di_valid = false;
         }
         if (nrc_on && r->nrc_trained >= r->nrc_settings.warmup) {
+            join_nrc(r, stream);
             const uint16_t* emission = (!s.restir_di && s.emissive_lighting) ? g->emission : nullptr;
             if ((fe = nrc_infer_gbuffer(r->nrc, f, *g, emission, r->nrc_settings.precision, r->nrc_plane, stream)) !=
                 RGI_OK)
@@ -494,15 +523,32 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
 
     // One training step of the cache on this frame's surfaces: target = DI
     // diffuse + albedo * next frame's GI history colour.
+    // The gather reads this frame's planes on `stream`; the step itself reads
+    // only the gathered records and the cache, so with `fork_grid` it runs on
+    // the cache's stream beside the denoiser and composite (and, with
+    // `cross_frame`, the next frame's DI passes and grid).
     if (nrc_on) {
+        join_nrc(r, stream);
         const uint16_t* next_history = s.denoiser ? r->filtered[write] : r->color[write];
         if ((e = nrc_gather(r->nrc, f, *g, di_active ? r->diffuse : nullptr, s.multiple_bounces ? next_history : nullptr,
                             r->nrc_settings.batch, f.frame_index ^ 0x4e5243u, r->nrc_records, r->nrc_targets,
                             stream)) != RGI_OK)
             return e;
-        if ((e = rgi_nrc_train(r->nrc, r->nrc_records, r->nrc_targets, r->nrc_settings.batch, r->nrc_loss, stream)) !=
-            RGI_OK)
+        hipStream_t train_stream = stream;
+        if (fork_grid) {
+            hipEventRecord(r->nrc_fork, stream);
+            hipStreamWaitEvent(r->nrc_stream, r->nrc_fork, 0);
+            train_stream = r->nrc_stream;
+        }
+        e = rgi_nrc_train(r->nrc, r->nrc_records, r->nrc_targets, r->nrc_settings.batch, r->nrc_loss, train_stream);
+        if (fork_grid) {
+            hipEventRecord(r->nrc_done, r->nrc_stream);
+            r->nrc_pending = true;
+        }
+        if (e != RGI_OK) {
+            join_nrc(r, stream);
             return e;
+        }
         ++r->nrc_trained;
     }
 
@@ -527,12 +573,13 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     r->history_valid = true;
     ++r->frame_number;
     r->last = f;
+    if (!cross_frame) join_nrc(r, stream);
     return RGI_OK;
 }
 
 int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
 {
-    return render_frame(r, view, g, stream_ptr, true);
+    return render_frame(r, view, g, stream_ptr, true, true);
 }
 
 int rgi_renderer_render_graph(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
@@ -542,14 +589,15 @@ int rgi_renderer_render_graph(rgi_renderer* r, const rgi_view* view, const rgi_g
     // The first frame runs eagerly: scratch buffers and the shadow atlas
     // reach their final state outside capture.
     if (!r->graph_warm) {
-        int e = rgi_renderer_render(r, view, g, stream_ptr);
+        int e = render_frame(r, view, g, stream_ptr, true, false);
         if (e == RGI_OK) r->graph_warm = true;
         return e;
     }
     hipStream_t stream = as_stream(stream_ptr);
+    join_nrc(r, stream); // an eager frame's training, joined outside the capture
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return fail(RGI_E_DEVICE, "render_graph: stream capture could not begin");
-    int e = rgi_renderer_render(r, view, g, stream_ptr);
+    int e = render_frame(r, view, g, stream_ptr, true, false);
     hipGraph_t graph = nullptr;
     hipError_t ce = hipStreamEndCapture(stream, &graph);
     if (e != RGI_OK || ce != hipSuccess) {
@@ -700,6 +748,9 @@ int rgi_renderer_nrc_state(const rgi_renderer* r, rgi_nrc** cache, const uint16_
                            uint32_t* trained)
 {
     if (!r) return fail(RGI_E_INVALID, "renderer is NULL");
+    // A training step still running on the cache's stream ends first, so the
+    // state returned is the last frame's.
+    if (r->nrc_pending) hipEventSynchronize(r->nrc_done);
     if (cache) *cache = r->nrc;
     if (radiance) *radiance = r->nrc ? r->nrc_plane : nullptr;
     if (loss) *loss = r->nrc ? r->nrc_loss : nullptr;
@@ -775,10 +826,11 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
     if (!b->warm) {
         // First frame eagerly, on the caller's stream (allocations happen here).
         for (size_t i = 0; i < n && e == RGI_OK; ++i)
-            e = render_frame(b->renderers[i], &views[i], &gbuffers[i], stream, false);
+            e = render_frame(b->renderers[i], &views[i], &gbuffers[i], stream, false, false);
         if (e == RGI_OK) b->warm = true;
         return e;
     }
+    for (size_t i = 0; i < n; ++i) join_nrc(b->renderers[i], stream); // eager frames' training, outside the capture
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return fail(RGI_E_DEVICE, "batch_render: stream capture could not begin");
     hipEventRecord(b->fork, stream);
@@ -786,7 +838,7 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
         hipStreamWaitEvent(b->streams[i], b->fork, 0);
         // Each renderer's frame is already a branch of its own; its grid
         // stays on that branch (no nested fork inside the batch's capture).
-        e = render_frame(b->renderers[i], &views[i], &gbuffers[i], b->streams[i], false);
+        e = render_frame(b->renderers[i], &views[i], &gbuffers[i], b->streams[i], false, false);
         hipEventRecord(b->joined[i], b->streams[i]);
         hipStreamWaitEvent(stream, b->joined[i], 0);
     }
