@@ -81,6 +81,19 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
     float depth = inside ? fetch_depth(p, uv) : 1.0f;
     bool working = inside && depth < kSkyDepth;
+    // A wave with no surface pixel writes its empty reservoirs and leaves
+    // (the rest of the pass is wave-local: ballots, the LDS queue and the
+    // 4 x 8 group sums never reach another wave).
+    if (__ballot(working) == 0ull) {
+        if (inside) {
+            uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+            F4 zero = F4{0.0f, 0.0f, 0.0f, 0.0f};
+            store_f4(out.sample, o, zero);
+            store_f4(out.receiver, o, zero);
+            store_f4(out.normal, o, zero);
+        }
+        return;
+    }
     F3 position = reconstruct_position(f, uv, depth);
     F3 n = decode_normal(p, uv);
     F3 albedo = diffuse_albedo(p, uv);

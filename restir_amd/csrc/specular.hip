@@ -111,6 +111,12 @@ __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
     float depth = inside ? fetch_depth(p, uv) : 1.0f;
     bool working = inside && depth < kSkyDepth;
+    // A wave with no surface pixel writes zeros and leaves (the rest of the
+    // pass is wave-local).
+    if (__ballot(working) == 0ull) {
+        if (inside) store_h4(out, (uint32_t)y * (uint32_t)g.width + (uint32_t)x, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        return;
+    }
     F3 position = reconstruct_position(f, uv, depth);
     F3 n = decode_normal(p, uv);
     uint32_t t = tix(p, uv);
