@@ -932,6 +932,20 @@ __global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, r
         int y = (int)(blockIdx.y * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
         surface[b] = gbuffer_point(gp, f, gb, texel, min(x, gb.width - 1), min(y, gb.height - 1), pts[b]);
     }
+    // A wave whose pixels are all sky writes zeros and leaves: the forward
+    // pass is wave-local.
+    bool any_surface = false;
+#pragma unroll
+    for (int b = 0; b < B; ++b) any_surface = any_surface || surface[b];
+    if (__ballot(any_surface) == 0ull) {
+        if (g != 0 || x >= gb.width) return;
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            int y = (int)(blockIdx.y * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
+            if (y < gb.height) store_h4(out, (uint32_t)y * (uint32_t)gb.width + (uint32_t)x, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        }
+        return;
+    }
     f32x4 yv[B];
     forward<P, B>(n, gp, L, pts, yv);
     if (g != 0 || x >= gb.width) return;
