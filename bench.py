@@ -15,7 +15,7 @@ scale) and reports Mrays/s.
 
 Beside the metric it reports the neural radiance cache: the metric's frame
 with the cache on, its kernels alone (2M per-pixel queries in BF16, FP16 and
-FP8; one training step of 65536 records) with the MLP's matrix-core
+FP8; one training step of the default 16384 surface records) with the MLP's matrix-core
 throughput, and the GI and composite MSE of the plain path and the cache
 against the same 4096-frame reference.
 
@@ -351,7 +351,8 @@ def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, step
     """Cost of the cache: the metric's frame with the cache on (same path,
     same settings), and its kernels alone with HIP events: 2M queries per
     precision (the 1080p frame's per-pixel query count) and one training
-    step of 65536 records, with the MLP's matrix-core throughput."""
+    step of the default batch of surface records, with the MLP's matrix-core
+    throughput."""
     import torch
     from restir_amd import nrc as rnrc
     rend = ctypes.c_void_p()
@@ -401,12 +402,15 @@ def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, step
         return e0.elapsed_time(e1) / reps
 
     def gather():
-        ra.check(L.rgi_nrc_gather_gbuffer(ctypes.byref(frame), ctypes.byref(gbufs[0].gb), outs.di_diffuse, outs.color,
-                                          batch, 7, train_rec.data_ptr(), train_tgt.data_ptr(), s))
+        ra.check(L.rgi_nrc_gather_gbuffer(cache.handle, ctypes.byref(frame), ctypes.byref(gbufs[0].gb), outs.di_diffuse,
+                                          outs.color, batch, 7, train_rec.data_ptr(), train_tgt.data_ptr(), s))
 
     gather_ms = timed(gather)
     train_ms = timed(lambda: cache.train(train_rec, train_tgt))
     queries = W * H
+    # Waves whose 16 pixels are all sky skip the encoding and the MLP, so the
+    # useful work is the surface pixels' queries.
+    surface_queries = int((gbufs[0].depth.view(torch.float32) < 0.99999).sum().item())
     infer_ms = {}
     for name, prec in (("bf16", ra.RGI_NRC_BF16), ("fp16", ra.RGI_NRC_FP16), ("fp8", ra.RGI_NRC_FP8)):
         infer_ms[name] = timed(lambda: ra.check(L.rgi_nrc_query_gbuffer(cache.handle, ctypes.byref(frame),
@@ -414,10 +418,11 @@ def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, step
                                                                         radiance.data_ptr(), s)))
     L.rgi_renderer_destroy(plain)
     cache.close()
-    tflops = queries * NRC_FLOP_PER_QUERY / (infer_ms["bf16"] * 1e-3) / 1e12
+    tflops = surface_queries * NRC_FLOP_PER_QUERY / (infer_ms["bf16"] * 1e-3) / 1e12
     return {"frame_ms_with_cache": round(frame_ms, 5), "frame_ms_plain": round(plain_ms, 5),
             "gather_ms": round(gather_ms, 5), "train_ms": round(train_ms, 5), "train_batch": batch,
             "query_ms": {k: round(v, 5) for k, v in infer_ms.items()}, "queries": queries,
+            "surface_queries": surface_queries,
             "mfma": {"kernel": "k_nrc_infer_gbuffer (bf16)", "achieved": round(tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 6), "flop_per_query": NRC_FLOP_PER_QUERY,
                      "bound": "hash-table gathers (L1 address processing), not the matrix cores"}}

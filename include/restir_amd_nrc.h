@@ -127,13 +127,17 @@ int rgi_nrc_query_gbuffer(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuffe
                           int precision, uint16_t* out_radiance, void* stream);
 
 /* Training records of a G-buffer, as the frame driver gathers them: `batch`
- * pixels stratified over the image (record i from pixel floor((i * N + u_i) /
- * batch), N pixels, u_i in [0, N) from `seed`), each with its surface inputs
- * and the target di_diffuse + albedo * color (either plane may be NULL);
- * weight 0 for sky pixels. records[12 * batch], targets[4 * batch]. */
-int rgi_nrc_gather_gbuffer(const rgi_frame* frame, const rgi_gbuffer* gbuffer, const uint16_t* di_diffuse,
-                           const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
-                           void* stream);
+ * records stratified over the N surface pixels (depth below the far plane)
+ * in row-major order, record i from surface pixel floor((i * N + u_i) /
+ * batch) with u_i in [0, N) from `seed`, each with its surface inputs and
+ * the target di_diffuse + albedo * color (either plane may be NULL), weight
+ * 1; when the G-buffer has no surface pixel every record has weight 0.
+ * records[12 * batch], targets[4 * batch]. `nrc` owns the scratch (the
+ * surface list, grown on the first call at a larger size, after a device
+ * synchronization). */
+int rgi_nrc_gather_gbuffer(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuffer* gbuffer,
+                           const uint16_t* di_diffuse, const uint16_t* color, uint32_t batch, uint32_t seed,
+                           float* records, float* targets, void* stream);
 
 /* ---------------------------------------------------------------------
  * The cache in the frame driver (include/restir_amd_renderer.h).
@@ -147,9 +151,9 @@ int rgi_nrc_gather_gbuffer(const rgi_frame* frame, const rgi_gbuffer* gbuffer, c
  *     its emission when the G-buffer emission is the surface light), where
  *     the reference reads the DI light plus last frame's indirect light. The
  *     GI target function and the shaded samples both use it;
- *  2. after the denoiser, gathers `batch` training records from pixels
- *     stratified over the image, each with target DI diffuse + albedo * the
- *     next frame's GI history colour (the ReSTIR estimate of the light the
+ *  2. after the denoiser, gathers `batch` training records from surface
+ *     pixels stratified over the image, each with target DI diffuse +
+ *     albedo * the next frame's GI history colour (the ReSTIR estimate of the light the
  *     surface reflects, which already contains the cache's own prediction at
  *     the sample points: self-training), and takes one Adam step. The step
  *     runs on a stream of the renderer's own, forked from the caller's: in
@@ -174,7 +178,7 @@ typedef struct rgi_renderer rgi_renderer;
 
 typedef struct rgi_nrc_settings {
     rgi_nrc_config cache;     /* aabb: the scene's bounds */
-    uint32_t batch;           /* 65536: training records per frame */
+    uint32_t batch;           /* 16384: training records per frame (surface pixels) */
     uint32_t warmup;          /* 4: training frames before the GI passes read the cache */
     int32_t precision;        /* RGI_NRC_BF16: precision of the per-pixel queries */
     int32_t resolve;          /* 0; 1: the composite takes the cache's prediction as each pixel's diffuse light */

@@ -239,7 +239,7 @@ def lib():
     L.rgi_nrc_set_params.argtypes = [P, P, P, P]
     L.rgi_nrc_steps.argtypes = [P, ctypes.POINTER(U32)]
     L.rgi_nrc_query_gbuffer.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, ctypes.c_int, P, P]
-    L.rgi_nrc_gather_gbuffer.argtypes = [ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, P, U32, U32, P, P, P]
+    L.rgi_nrc_gather_gbuffer.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, P, U32, U32, P, P, P]
     L.rgi_nrc_settings_default.restype = None
     L.rgi_nrc_settings_default.argtypes = [ctypes.POINTER(NrcSettings)]
     L.rgi_renderer_set_nrc.argtypes = [P, ctypes.POINTER(NrcSettings)]

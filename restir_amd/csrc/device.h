@@ -81,6 +81,12 @@ inline dim3 tile_grid(const TileMap& m)
     return dim3(128u * ((blocks + 7u) / 8u));
 }
 
+// Lanes below this one whose bit is set in `mask` (a wave-64 ballot).
+RGI_DEV uint32_t rank_in(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
 RGI_DEV F3 f3(float x, float y, float z) { return F3{x, y, z}; }
 RGI_DEV F3 splat3(float s) { return F3{s, s, s}; }
 RGI_DEV F3 xyz(F4 v) { return F3{v.x, v.y, v.z}; }

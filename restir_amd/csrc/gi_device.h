@@ -123,11 +123,6 @@ RGI_DEV void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-RGI_DEV uint32_t rank_in(uint64_t mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
 // Pixel of this thread: 16 x 16 per block, two 4 x 8 groups per wave;
 // false when the workgroup has no tile.
 RGI_DEV bool group_pixel(const TileMap& tiles, int& x, int& y)

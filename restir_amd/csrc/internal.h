@@ -72,7 +72,7 @@ int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
 // radiance at every G-buffer pixel, and one frame's training records.
 int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* emission, int precision,
                       uint16_t* out, hipStream_t stream);
-int nrc_gather(const rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
+int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
                const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
                hipStream_t stream);
 

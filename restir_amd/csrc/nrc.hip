@@ -55,6 +55,12 @@ struct rgi_nrc {
     __bf16* deltas = nullptr;     // [6][tiles][64][256]
     float* partial = nullptr;     // [chunks][RGI_NRC_MLP_PARAMS]
     uint32_t capacity_tiles = 0;
+    // Gather scratch, grown on demand: surface pixel counts per 1024-pixel
+    // block, their exclusive offsets (the total last) and the surface list.
+    uint32_t* surface_counts = nullptr;
+    uint32_t* surface_offsets = nullptr;
+    uint32_t* surface_list = nullptr;
+    uint32_t capacity_pixels = 0;
 };
 
 namespace rgi {
@@ -966,19 +972,91 @@ __global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, r
     }
 }
 
-// Training records of one frame: `batch` pixels stratified over the image
-// with a random offset per record, each with its surface inputs (as the
-// G-buffer query computes them) and the target radiance di_diffuse + albedo
-// * color (either term optional); weight 0 for sky pixels.
+// The surface pixels of a G-buffer, in row-major order, as a list: a
+// 256-thread block owns 1024 consecutive pixels (four chunks of 256), counts
+// them, one workgroup turns the counts into offsets, and each block writes
+// its surface pixels' indices at its offset.
+constexpr uint32_t kSurfaceBlock = 1024;
+
+__global__ void __launch_bounds__(256) k_nrc_surface_count(const float* depth, uint32_t npix, uint32_t* counts)
+{
+    __shared__ uint32_t per_wave[4];
+    uint32_t base = blockIdx.x * kSurfaceBlock, c = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        uint32_t p = base + k * 256u + threadIdx.x;
+        c += (uint32_t)__popcll(__ballot(p < npix && depth[p] < kSkyDepth));
+    }
+    if ((threadIdx.x & 63u) == 0) per_wave[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = ((per_wave[0] + per_wave[1]) + per_wave[2]) + per_wave[3];
+}
+
+__global__ void __launch_bounds__(1024) k_nrc_surface_scan(const uint32_t* counts, uint32_t blocks, uint32_t* offsets)
+{
+    __shared__ uint32_t part[1024];
+    uint32_t per = (blocks + 1023u) / 1024u, lo = threadIdx.x * per, hi = min(lo + per, blocks), s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += counts[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t step = 1; step < 1024u; step <<= 1) {
+        uint32_t v = threadIdx.x >= step ? part[threadIdx.x - step] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        offsets[i] = run;
+        run += counts[i];
+    }
+    if (threadIdx.x == 1023u) offsets[blocks] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_nrc_surface_list(const float* depth, uint32_t npix, const uint32_t* offsets,
+                                                          uint32_t* list)
+{
+    __shared__ uint32_t per_wave[4][4];
+    const uint32_t wave = threadIdx.x >> 6, base = blockIdx.x * kSurfaceBlock;
+    bool surface[4];
+    uint64_t mask[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        uint32_t p = base + k * 256u + threadIdx.x;
+        surface[k] = p < npix && depth[p] < kSkyDepth;
+        mask[k] = __ballot(surface[k]);
+        if ((threadIdx.x & 63u) == 0) per_wave[k][wave] = (uint32_t)__popcll(mask[k]);
+    }
+    __syncthreads();
+    uint32_t run = offsets[blockIdx.x];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        uint32_t before = run;
+        for (uint32_t w = 0; w < wave; ++w) before += per_wave[k][w];
+        if (surface[k]) list[before + rank_in(mask[k])] = base + k * 256u + threadIdx.x;
+        run += ((per_wave[k][0] + per_wave[k][1]) + per_wave[k][2]) + per_wave[k][3];
+    }
+}
+
+// Training records of one frame: `batch` records stratified over the Ns
+// surface pixels in row-major order (record i from surface pixel floor((i Ns
+// + u_i) / batch), u_i in [0, Ns) from `seed`), each with its surface inputs
+// (as the G-buffer query computes them) and the target radiance di_diffuse +
+// albedo * color (either term optional); weight 0 for every record when the
+// G-buffer has no surface pixel.
 __global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_t* di_diffuse, const uint16_t* color,
-                             uint32_t batch, uint32_t seed, float* records, float* targets)
+                             const uint32_t* surface_total, const uint32_t* surface_list, uint32_t batch, uint32_t seed,
+                             float* records, float* targets)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= batch) return;
-    uint64_t npix = (uint64_t)g.width * (uint64_t)g.height;
+    uint64_t ns = *surface_total;
     uint32_t h = pcg(i + pcg(seed));
-    uint64_t offset = ((uint64_t)(h >> 8) * npix) >> 24;
-    uint32_t p = (uint32_t)(((uint64_t)i * npix + offset) / batch);
+    uint32_t p = 0;
+    if (ns > 0) {
+        uint64_t offset = ((uint64_t)(h >> 8) * ns) >> 24;
+        p = surface_list[(uint32_t)(((uint64_t)i * ns + offset) / batch)];
+    }
     int x = (int)(p % (uint32_t)g.width), y = (int)(p / (uint32_t)g.width);
     uint32_t t = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     float depth = g.depth[t];
@@ -1033,14 +1111,34 @@ int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, cons
     return check_launch("nrc infer (G-buffer)");
 }
 
-int nrc_gather(const rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
+int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
                const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
                hipStream_t stream)
 {
-    (void)n;
+    uint32_t npix = (uint32_t)g.width * (uint32_t)g.height;
+    uint32_t blocks = (npix + nrc::kSurfaceBlock - 1) / nrc::kSurfaceBlock;
+    if (npix > n->capacity_pixels) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(n->surface_counts);
+        (void)hipFree(n->surface_offsets);
+        (void)hipFree(n->surface_list);
+        n->surface_counts = n->surface_offsets = n->surface_list = nullptr;
+        n->capacity_pixels = 0;
+        uint32_t cap_blocks = blocks;
+        if (hipMalloc((void**)&n->surface_counts, (size_t)cap_blocks * 4) != hipSuccess ||
+            hipMalloc((void**)&n->surface_offsets, (size_t)(cap_blocks + 1) * 4) != hipSuccess ||
+            hipMalloc((void**)&n->surface_list, (size_t)npix * 4) != hipSuccess)
+            return fail(RGI_E_NOMEM, "nrc gather: surface list of %u pixels could not be allocated", npix);
+        n->capacity_pixels = npix;
+    }
+    hipLaunchKernelGGL(nrc::k_nrc_surface_count, dim3(blocks), dim3(256), 0, stream, g.depth, npix, n->surface_counts);
+    hipLaunchKernelGGL(nrc::k_nrc_surface_scan, dim3(1), dim3(1024), 0, stream, n->surface_counts, blocks,
+                       n->surface_offsets);
+    hipLaunchKernelGGL(nrc::k_nrc_surface_list, dim3(blocks), dim3(256), 0, stream, g.depth, npix, n->surface_offsets,
+                       n->surface_list);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     hipLaunchKernelGGL(nrc::k_nrc_gather, dim3((batch + 255) / 256), dim3(256), 0, stream, f, g, texel, di_diffuse,
-                       color, batch, seed, records, targets);
+                       color, n->surface_offsets + blocks, n->surface_list, batch, seed, records, targets);
     return check_launch("nrc gather");
 }
 
@@ -1258,7 +1356,7 @@ void rgi_nrc_destroy(rgi_nrc* n)
     (void)hipDeviceSynchronize();
     void* ptrs[] = {n->table, n->table_m, n->table_v, n->table_grad, n->table_h, n->mlp, n->mlp_m, n->mlp_v,
                     n->mlp_grad, n->w_fwd, n->w_bwd, n->w_fwd_h, n->w_fwd_8, n->scales, n->act_max, n->loss, n->levels,
-                    n->acts, n->deltas, n->partial};
+                    n->acts, n->deltas, n->partial, n->surface_counts, n->surface_offsets, n->surface_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete n;
@@ -1361,13 +1459,15 @@ int rgi_nrc_query_gbuffer(rgi_nrc* n, const rgi_frame* f, const rgi_gbuffer* g, 
     return nrc_infer_gbuffer(n, *f, *g, emission, precision, out, as_stream(stream));
 }
 
-int rgi_nrc_gather_gbuffer(const rgi_frame* f, const rgi_gbuffer* g, const uint16_t* di_diffuse, const uint16_t* color,
-                           uint32_t batch, uint32_t seed, float* records, float* targets, void* stream)
+int rgi_nrc_gather_gbuffer(rgi_nrc* n, const rgi_frame* f, const rgi_gbuffer* g, const uint16_t* di_diffuse,
+                           const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
+                           void* stream)
 {
-    if (!f || !g || !records || !targets || batch == 0) return fail(RGI_E_INVALID, "nrc_gather_gbuffer: bad argument");
+    if (!n || !f || !g || !records || !targets || batch == 0)
+        return fail(RGI_E_INVALID, "nrc_gather_gbuffer: bad argument");
     if (g->width <= 0 || g->height <= 0 || !g->depth || !g->normal || !g->albedo)
         return fail(RGI_E_INVALID, "nrc_gather_gbuffer: depth, normal and albedo planes are required");
-    return nrc_gather(nullptr, *f, *g, di_diffuse, color, batch, seed, records, targets, as_stream(stream));
+    return nrc_gather(n, *f, *g, di_diffuse, color, batch, seed, records, targets, as_stream(stream));
 }
 
 int rgi_nrc_steps(const rgi_nrc* n, uint32_t* steps)

@@ -712,7 +712,7 @@ void rgi_nrc_settings_default(rgi_nrc_settings* s)
     if (!s) return;
     memset(s, 0, sizeof(*s));
     rgi_nrc_config_default(&s->cache);
-    s->batch = 65536;
+    s->batch = 16384;
     s->warmup = 4;
     s->precision = RGI_NRC_BF16;
 }

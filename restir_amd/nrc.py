@@ -96,7 +96,7 @@ class NeuralCache:
         return s.value
 
 
-def renderer_settings(hs, batch=65536, warmup=4, precision=ra.RGI_NRC_BF16, resolve=0, margin=0.02, **kw):
+def renderer_settings(hs, batch=16384, warmup=4, precision=ra.RGI_NRC_BF16, resolve=0, margin=0.02, **kw):
     """rgi_nrc_settings for a HostScene: the cache's bounds are the scene's
     triangle bounds, widened by `margin` of their extent."""
     st = ra.NrcSettings()

@@ -354,9 +354,10 @@ def test_renderer_resolve_lowers_composite_error():
 
 def test_gbuffer_query_matches_records_query():
     """The per-pixel query reads a pixel's inputs exactly as the training
-    gather records them: gathering every pixel (batch = pixel count puts
-    record i on pixel i) and querying those records gives the G-buffer
-    query's radiance, bit for bit after its clamp and FP16 store; the
+    gather records them: gathering one record per surface pixel (batch = the
+    surface count puts record i on the i-th surface pixel in row-major order)
+    and querying those records gives the G-buffer query's radiance at those
+    pixels, bit for bit after its clamp and FP16 store, and 0 on the sky; the
     records query is itself checked against the oracle above."""
     from restir_amd import driver
     W, H = 320, 240
@@ -375,21 +376,69 @@ def test_gbuffer_query_matches_records_query():
     mlp[20480:] = rng.uniform(-1, 1, 192) * np.sqrt(6.0 / 67.0)
     table = rng.uniform(-0.5, 0.5, nc.table_floats).astype(np.float32)
     nc.set_params(torch.from_numpy(mlp).cuda(), torch.from_numpy(table).cuda())
-    rec = torch.empty((n, 12), dtype=torch.float32, device="cuda")
-    tgt = torch.empty((n, 4), dtype=torch.float32, device="cuda")
-    ra.check(L.rgi_nrc_gather_gbuffer(ctypes.byref(frame), ctypes.byref(gbuf.gb), None, None, n, 5, rec.data_ptr(),
-                                      tgt.data_ptr(), s))
+    surface = gbuf.depth.view(torch.float32) < 0.99999
+    ns = int(surface.sum())
+    assert n // 10 < ns < n  # the box covers about a fifth of this view
+    rec = torch.empty((ns, 12), dtype=torch.float32, device="cuda")
+    tgt = torch.empty((ns, 4), dtype=torch.float32, device="cuda")
+    ra.check(L.rgi_nrc_gather_gbuffer(nc.handle, ctypes.byref(frame), ctypes.byref(gbuf.gb), None, None, ns, 5,
+                                      rec.data_ptr(), tgt.data_ptr(), s))
+    assert torch.all(tgt[:, 3] == 1)
     for prec in (ra.RGI_NRC_BF16, ra.RGI_NRC_FP16, ra.RGI_NRC_FP8):
         print("gbuffer query precision", prec, flush=True)
         rad = torch.empty(n * 4, dtype=torch.float16, device="cuda")
         ra.check(L.rgi_nrc_query_gbuffer(nc.handle, ctypes.byref(frame), ctypes.byref(gbuf.gb), None, prec,
                                          rad.data_ptr(), s))
         y = nc.infer(rec, prec)
-        surface = tgt[:, 3] > 0
-        want = torch.where(surface[:, None], torch.clamp(y[:, :3], min=0.0), torch.zeros_like(y[:, :3])).half()
+        want = torch.clamp(y[:, :3], min=0.0).half()
         got = rad.view(n, 4)
-        assert torch.equal(got[:, :3].view(torch.int16), want.view(torch.int16)), prec
-        assert torch.equal(got[:, 3], surface.half())
-    assert int(surface.sum()) > n // 10  # the box covers about a fifth of this view
+        assert torch.equal(got[surface][:, :3].view(torch.int16), want.view(torch.int16)), prec
+        assert torch.all(got[surface][:, 3] == 1)
+        assert torch.all(got[~surface] == 0)
+    nc.close()
+    L.rgi_destroy(ctx)
+
+
+def test_gather_stratifies_over_surface_pixels():
+    """A smaller batch: every record is a surface pixel's record (one of the
+    full gather's), records follow the surface pixels' row-major order, and
+    the stratification leaves no gap wider than two strata; a G-buffer with
+    no surface pixel gives weight-0 records."""
+    from restir_amd import driver
+    W, H = 320, 240
+    st = ra.Settings(grid_table_size=1 << 16)
+    hs, gscene, ctx, gbuf, view = _setup_scene(W, H, st)
+    L = ra.lib()
+    s = driver.stream()
+    eye, vp, inv = ra.orbit_camera(hs, 0, W, H, yaw_of=ra.swing_yaw)
+    frame = ra.make_frame(st, hs, 0, eye, vp, inv, eye, vp, W, H, False)
+    nc = rnrc.NeuralCache(rnrc.renderer_settings(hs).cache)
+    ns = int((gbuf.depth.view(torch.float32) < 0.99999).sum())
+    full = torch.empty((ns, 12), dtype=torch.float32, device="cuda")
+    full_t = torch.empty((ns, 4), dtype=torch.float32, device="cuda")
+    ra.check(L.rgi_nrc_gather_gbuffer(nc.handle, ctypes.byref(frame), ctypes.byref(gbuf.gb), None, None, ns, 1,
+                                      full.data_ptr(), full_t.data_ptr(), s))
+    batch = 1000
+    rec = torch.empty((batch, 12), dtype=torch.float32, device="cuda")
+    tgt = torch.empty((batch, 4), dtype=torch.float32, device="cuda")
+    ra.check(L.rgi_nrc_gather_gbuffer(nc.handle, ctypes.byref(frame), ctypes.byref(gbuf.gb), None, None, batch, 9,
+                                      rec.data_ptr(), tgt.data_ptr(), s))
+    assert torch.all(tgt[:, 3] == 1)
+    f, r = full.cpu().numpy(), rec.cpu().numpy()
+    index = {row.tobytes(): i for i, row in enumerate(f)}
+    assert len(index) == ns  # surface pixels have distinct records
+    picked = np.array([index[row.tobytes()] for row in r])
+    assert np.all(np.diff(picked) >= 0)
+    stratum = ns / batch
+    assert picked[0] < stratum and picked[-1] >= ns - stratum - 1
+    assert np.max(np.diff(picked)) < 2 * stratum + 1
+    # No surface at all: a G-buffer whose depth is the far plane everywhere.
+    sky = torch.ones_like(gbuf.depth.view(torch.float32))
+    gb = ra.GBuffer(W, H, sky.data_ptr(), gbuf.albedo.data_ptr(), gbuf.normal.data_ptr(), gbuf.shading.data_ptr(),
+                    gbuf.emission.data_ptr())
+    ra.check(L.rgi_nrc_gather_gbuffer(nc.handle, ctypes.byref(frame), ctypes.byref(gb), None, None, batch, 9,
+                                      rec.data_ptr(), tgt.data_ptr(), s))
+    torch.cuda.synchronize()
+    assert torch.all(tgt[:, 3] == 0)
     nc.close()
     L.rgi_destroy(ctx)
