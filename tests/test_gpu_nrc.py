@@ -352,6 +352,53 @@ def test_renderer_resolve_lowers_composite_error():
         L.rgi_destroy(ctx)
 
 
+def test_renderer_cache_in_graph_frames():
+    """Graph frames with a resolving cache: the training step forks onto the
+    cache's stream and joins inside each captured frame, so every frame
+    trains once, the state read afterwards is the last frame's, and the
+    resolved image is finite and close to the eager renderer's mean (the
+    table gradient's FP32 atomics make the two caches differ in their last
+    bits, so the check is statistical)."""
+    from restir_amd import driver
+    W, H = 160, 120
+    n = W * H
+    st = ra.Settings(grid_table_size=1 << 16, grid_mode=ra.RGI_GRID_DETERMINISTIC)
+    hs, gscene, ctx, gbuf, view = _setup_scene(W, H, st)
+    L = ra.lib()
+    s = driver.stream()
+    settings = rnrc.renderer_settings(hs, batch=4096, warmup=2, resolve=1)
+    side = torch.cuda.Stream()  # capture needs a stream other than the null stream
+    gs = ctypes.c_void_p(side.cuda_stream)
+    torch.cuda.synchronize()
+    graph = _renderer(hs, gscene, st, W, H, settings)
+    eager = _renderer(hs, gscene, st, W, H, settings)
+    out = ra.RendererOutputs()
+    try:
+        frames = 12
+        for frame in range(frames):
+            ra.check(L.rgi_renderer_render_graph(graph, ctypes.byref(view), ctypes.byref(gbuf.gb), gs))
+            ra.check(L.rgi_renderer_render(eager, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        torch.cuda.synchronize()
+        means = []
+        for r in (graph, eager):
+            cache, rad_ptr, loss_ptr, trained = rnrc.renderer_state(r)
+            assert cache and rad_ptr and trained == frames
+            loss = torch.empty(1, dtype=torch.float32, device="cuda")
+            L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            assert L.hipMemcpy(loss.data_ptr(), loss_ptr, 4, 3) == 0
+            assert np.isfinite(float(loss.item())) and float(loss.item()) > 0
+            ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(out)))
+            hdr = _half_plane(out.hdr, n)
+            assert np.all(np.isfinite(hdr))
+            means.append(hdr[:, :3].mean())
+        print("graph / eager resolved mean: %.5f / %.5f" % tuple(means), flush=True)
+        assert abs(means[0] - means[1]) < 0.05 * means[1]
+    finally:
+        L.rgi_renderer_destroy(graph)
+        L.rgi_renderer_destroy(eager)
+        L.rgi_destroy(ctx)
+
+
 def test_gbuffer_query_matches_records_query():
     """The per-pixel query reads a pixel's inputs exactly as the training
     gather records them: gathering one record per surface pixel (batch = the
