@@ -307,6 +307,52 @@ def nrc_motion_quality(L, hs, gscene, views, gbufs, T, mode, ctx, s, ref_frames=
     return result
 
 
+def resolve_quality(L, hs, gscene, gbuf, view, T, mode, ctx, s, ref_frames=1024, frames=256):
+    """The resolved cache against the plain path on another scene: the
+    composite of each after `frames` frames at one static camera (default
+    settings, denoiser on), against an FP64 accumulation of `ref_frames`
+    reference frames of that camera."""
+    import torch
+    from restir_amd import nrc as rnrc
+    W, H = gbuf.gb.width, gbuf.gb.height
+    settings = ra.Settings(grid_table_size=T, grid_mode=mode)
+    acc = torch.zeros(3 * W * H, dtype=torch.float64, device="cuda")
+    res = torch.zeros(5, dtype=torch.float64, device="cuda")
+    out = ra.RendererOutputs()
+
+    def renderer(cache):
+        rend = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                       ctypes.byref(gscene.scene), hs.radius))
+        if cache:
+            ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs, resolve=1))))
+        return rend
+
+    rend = renderer(False)
+    ra.check(L.rgi_renderer_set_reference_mode(rend, 1))
+    for _ in range(ref_frames):
+        ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+        ra.check(L.rgi_reference_accumulate_composite(ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), s))
+    L.rgi_renderer_destroy(rend)
+    result = {"reference_frames": ref_frames, "frames": frames, "camera": "static, view 0 of the swing path"}
+    for name in ("plain", "resolved"):
+        rend = renderer(name == "resolved")
+        for _ in range(frames):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+        ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), ref_frames,
+                                                 res.data_ptr(), s))
+        r = res.cpu().numpy()
+        result[name] = {"composite_mse_rgb": float(r[1]), "relative_mse": float(r[0]), "mean_luminance": float(r[2]),
+                        "reference_mean_luminance": float(r[3])}
+        L.rgi_renderer_destroy(rend)
+    result["composite_mse_plain_over_resolved"] = round(result["plain"]["composite_mse_rgb"] /
+                                                        result["resolved"]["composite_mse_rgb"], 3)
+    del acc
+    return result
+
+
 def cache_frames(hs, W, H, T, frames_in_path, steps):
     """The metric's swing path and settings with the neural radiance cache on:
     16 warm-up frames, then `steps` timed frames."""
@@ -852,6 +898,10 @@ def main():
                   "triangles": ahs.triangle_count, "bvh_nodes": ahs.node_count, "di_lights": ahs.light_count,
                   "grid_table_size": 1 << 22, "steps": args.atrium_steps, "camera_distance_radii": 0.7,
                   "tuning": arun["tuning"]}
+        if args.nrc and args.reference_frames > 0 and nrc is not None:
+            # The resolved cache on the atrium (static camera, view 0).
+            nrc["quality_atrium"] = resolve_quality(L, ahs, arun["gscene"], arun["gbufs"][0], arun["views"][0],
+                                                    1 << 22, mode, arun["drv"].ctx, s)
         L.rgi_renderer_destroy(arun["rend"])
         arun["drv"].close()
 

@@ -823,7 +823,12 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
     hipStream_t stream = as_stream(stream_ptr);
     size_t n = b->renderers.size();
     int e = RGI_OK;
-    if (!b->warm) {
+    // A cache that has not trained yet allocates its gather and training
+    // scratch on its first step, which must not happen inside a capture.
+    bool warm = b->warm;
+    for (size_t i = 0; i < n; ++i)
+        if (b->renderers[i]->nrc && b->renderers[i]->nrc_trained == 0) warm = false;
+    if (!warm) {
         // First frame eagerly, on the caller's stream (allocations happen here).
         for (size_t i = 0; i < n && e == RGI_OK; ++i)
             e = render_frame(b->renderers[i], &views[i], &gbuffers[i], stream, false, false);
