@@ -474,6 +474,54 @@ def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, step
                      "bound": "hash-table gathers (L1 address processing), not the matrix cores"}}
 
 
+def trace_benchmark(L, ctx, scene, rays, s):
+    """rgi_trace over `rays` (closest hit): both launch forms timed with HIP
+    events (5 repetitions each), the faster kept on the context, then the
+    watertight triangle test on the same rays: its time and how often its hit
+    differs from the reference's test. Options are left at their defaults,
+    except the launch form."""
+    import torch
+    count = rays.shape[0]
+    rays_t = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()
+    hit = torch.empty(count * 4, dtype=torch.uint8, device="cuda")
+    dist = torch.empty(count * 4, dtype=torch.uint8, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def trace():
+        ra.check(L.rgi_trace(ctx, ctypes.byref(scene), rays_t.data_ptr(), count, hit.data_ptr(), dist.data_ptr(), 0, s))
+
+    def timed(reps=5):
+        trace()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            trace()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    variants = {}
+    for persistent in (0, 1):
+        ra.check(L.rgi_context_set_option(ctx, ra.RGI_OPT_TRACE_PERSISTENT, persistent))
+        variants[persistent] = timed()
+    best = min(variants, key=variants.get)
+    ra.check(L.rgi_context_set_option(ctx, ra.RGI_OPT_TRACE_PERSISTENT, best))
+    trace()
+    ms = variants[best]
+    hit_rate = float((hit.view(torch.int32) >= 0).float().mean().item())
+    mt_hit = hit.view(torch.int32).clone()
+    ra.check(L.rgi_context_set_option(ctx, ra.RGI_OPT_TRIANGLE_TEST, 1))
+    wt_ms = timed()
+    wt_hit = hit.view(torch.int32)
+    watertight = {"ms": round(wt_ms, 4), "mrays_per_s": round(count / (wt_ms * 1e-3) / 1e6, 2),
+                  "hit_rate": round(float((wt_hit >= 0).float().mean().item()), 5),
+                  "differs_from_moeller_trumbore": round(float((wt_hit != mt_hit).float().mean().item()), 7)}
+    ra.check(L.rgi_context_set_option(ctx, ra.RGI_OPT_TRIANGLE_TEST, 0))
+    del rays_t, hit, dist, mt_hit
+    return {"rays": count, "persistent": best, "ms_by_variant": {str(k): round(v, 4) for k, v in variants.items()},
+            "ms": ms, "mrays_per_s": count / (ms * 1e-3) / 1e6, "hit_rate": hit_rate, "watertight": watertight}
+
+
 def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames, distance=3.17):
     """One workload through the C++ frame driver: the swing camera path's
     G-buffers produced once and resident in HBM, the autotuner, `steps` timed
@@ -805,53 +853,10 @@ def main():
     composite_ms = e0.elapsed_time(e1) / 10
 
     # Traversal microbenchmark.
-    rays = surface_rays(hs, args.rays, seed=7)
-    rays_t = torch.from_numpy(rays.view(np.uint8).reshape(-1)).cuda()
-    hit = torch.empty(args.rays * 4, dtype=torch.uint8, device="cuda")
-    dist = torch.empty(args.rays * 4, dtype=torch.uint8, device="cuda")
-    def trace():
-        ra.check(L.rgi_trace(drv.ctx, ctypes.byref(gscene.scene), rays_t.data_ptr(), args.rays, hit.data_ptr(),
-                             dist.data_ptr(), 0, s))
-
-    trace_variants = {}
-    for persistent in (0, 1):
-        ra.check(L.rgi_context_set_option(drv.ctx, ra.RGI_OPT_TRACE_PERSISTENT, persistent))
-        trace()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = 5
-        e0.record()
-        for _ in range(reps):
-            trace()
-        e1.record()
-        torch.cuda.synchronize()
-        trace_variants[persistent] = e0.elapsed_time(e1) / reps
-    best_persistent = min(trace_variants, key=trace_variants.get)
-    ra.check(L.rgi_context_set_option(drv.ctx, ra.RGI_OPT_TRACE_PERSISTENT, best_persistent))
-    trace()
-    trace_ms = trace_variants[best_persistent]
-    tuning["trace_persistent"] = best_persistent
-    tuning["trace_ms"] = {str(k): round(v, 4) for k, v in trace_variants.items()}
-    mrays = args.rays / (trace_ms * 1e-3) / 1e6
-    hit_rate = float((hit.view(torch.int32) >= 0).float().mean().item())
-    # The same rays with the watertight triangle test (RGI_OPT_TRIANGLE_TEST):
-    # its speed, and how often its hit differs from the reference's test.
-    mt_hit = hit.view(torch.int32).clone()
-    ra.check(L.rgi_context_set_option(drv.ctx, ra.RGI_OPT_TRIANGLE_TEST, 1))
-    trace()
-    torch.cuda.synchronize()
-    e0.record()
-    for _ in range(5):
-        trace()
-    e1.record()
-    torch.cuda.synchronize()
-    wt_ms = e0.elapsed_time(e1) / 5
-    wt_hit = hit.view(torch.int32)
-    watertight = {"ms": round(wt_ms, 4), "mrays_per_s": round(args.rays / (wt_ms * 1e-3) / 1e6, 2),
-                  "hit_rate": round(float((wt_hit >= 0).float().mean().item()), 5),
-                  "differs_from_moeller_trumbore": round(float((wt_hit != mt_hit).float().mean().item()), 7)}
-    ra.check(L.rgi_context_set_option(drv.ctx, ra.RGI_OPT_TRIANGLE_TEST, 0))
-    del mt_hit
+    tr = trace_benchmark(L, drv.ctx, gscene.scene, surface_rays(hs, args.rays, seed=7), s)
+    tuning["trace_persistent"] = tr["persistent"]
+    tuning["trace_ms"] = tr["ms_by_variant"]
+    trace_ms, mrays, hit_rate, watertight = tr["ms"], tr["mrays_per_s"], tr["hit_rate"], tr["watertight"]
 
     dominant = max((k for k in pass_ms if k in BYTES_PER_PIXEL), key=lambda k: pass_ms[k])
     traffic = BYTES_PER_PIXEL[dominant] * W * H
@@ -898,6 +903,12 @@ def main():
                   "triangles": ahs.triangle_count, "bvh_nodes": ahs.node_count, "di_lights": ahs.light_count,
                   "grid_table_size": 1 << 22, "steps": args.atrium_steps, "camera_distance_radii": 0.7,
                   "tuning": arun["tuning"]}
+        # Traversal over the atrium's BVH: the same ray count, cosine rays from
+        # points uniform over its area.
+        atr = trace_benchmark(L, arun["drv"].ctx, arun["gscene"].scene, surface_rays(ahs, args.rays, seed=7), s)
+        atrium["trace"] = {"rays": atr["rays"], "ms": round(atr["ms"], 4), "mrays_per_s": round(atr["mrays_per_s"], 2),
+                           "hit_rate": round(atr["hit_rate"], 4), "persistent": atr["persistent"],
+                           "ms_by_variant": atr["ms_by_variant"], "watertight": atr["watertight"]}
         if args.nrc and args.reference_frames > 0 and nrc is not None:
             # The resolved cache on the atrium (static camera, view 0).
             nrc["quality_atrium"] = resolve_quality(L, ahs, arun["gscene"], arun["gbufs"][0], arun["views"][0],
