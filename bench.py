@@ -564,6 +564,7 @@ def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames,
     tuning = {"gi_temporal_waves": options[ra.RGI_OPT_GI_TEMPORAL_WAVES],
               "gi_spatial_waves": options[ra.RGI_OPT_GI_SPATIAL_WAVES],
               "tile_order": options[ra.RGI_OPT_TILE_ORDER],
+              "tile_shape": "32x8" if options[ra.RGI_OPT_TILE_SHAPE] else "16x16",
               "trial_ms": [round(float(v), 5) for v in tune_ms]}
     for k in range(warmup):
         render(k)

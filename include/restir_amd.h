@@ -191,6 +191,9 @@ void rgi_destroy(rgi_context* context);
  * shared edge or vertex of two triangles. The resampling passes and the
  * G-buffer and shadow producers always use the reference's test. */
 #define RGI_OPT_TRIANGLE_TEST 5
+/* Screen passes' workgroup shape: 0 16 x 16 pixels, 1 32 x 8 pixels (both 256
+ * threads; GI keeps the reference's 4 x 8 groups either way). */
+#define RGI_OPT_TILE_SHAPE 6
 int rgi_context_set_option(rgi_context* context, int option, int value);
 int rgi_context_get_option(const rgi_context* context, int option);
 

@@ -135,16 +135,17 @@ void rgi_batch_destroy(rgi_batch* batch);
  * `views` and their G-buffers), times each variant with HIP events on
  * `stream`, and keeps the fastest. Variants are bit-exact with each other,
  * so tuning never changes an image. The search is coordinate descent over
- * three knobs, each trial keeping the best value found so far for the
+ * four knobs, each trial keeping the best value found so far for the
  * others: trials 0-1 tile order 0/2; trials 2-3 GI temporal waves 0/4;
- * trials 4-5 GI spatial waves 0/4. `out_ms` (may be NULL) receives the mean
+ * trials 4-5 GI spatial waves 0/4; trials 6-7 tile shape 16 x 16 / 32 x 8. `out_ms` (may be NULL) receives the mean
  * frame time of each of the `RGI_TUNE_TRIALS` trials. Returns the chosen
  * configuration as a bit set (RGI_TUNE_* below), which stays set on the
  * renderer, or a negative code. */
-#define RGI_TUNE_TRIALS 6
+#define RGI_TUNE_TRIALS 8
 #define RGI_TUNE_TEMPORAL_WAVES4 1
 #define RGI_TUNE_SPATIAL_WAVES4 2
 #define RGI_TUNE_TILE_ORDER 4 /* RGI_OPT_TILE_ORDER 2 */
+#define RGI_TUNE_TILE_SHAPE 8 /* RGI_OPT_TILE_SHAPE 1 */
 int rgi_renderer_autotune(rgi_renderer* renderer, const rgi_view* views, const rgi_gbuffer* gbuffers, uint32_t count,
                           uint32_t frames, float* out_ms, void* stream);
 

@@ -24,8 +24,9 @@ RGI_SCENE_CORNELL, RGI_SCENE_ATRIUM, RGI_SCENE_BOXFIELD = 0, 1, 2
 RGI_OPT_GI_TEMPORAL_WAVES, RGI_OPT_GI_SPATIAL_WAVES, RGI_OPT_TRACE_PERSISTENT = 1, 2, 3
 RGI_OPT_TILE_ORDER = 4
 RGI_OPT_TRIANGLE_TEST = 5
-RGI_TUNE_TRIALS = 6
-RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4, RGI_TUNE_TILE_ORDER = 1, 2, 4
+RGI_OPT_TILE_SHAPE = 6
+RGI_TUNE_TRIALS = 8
+RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4, RGI_TUNE_TILE_ORDER, RGI_TUNE_TILE_SHAPE = 1, 2, 4, 8
 
 
 def tuning_of(config):
@@ -34,6 +35,7 @@ def tuning_of(config):
         RGI_OPT_GI_TEMPORAL_WAVES: 4 if config & RGI_TUNE_TEMPORAL_WAVES4 else 0,
         RGI_OPT_GI_SPATIAL_WAVES: 4 if config & RGI_TUNE_SPATIAL_WAVES4 else 0,
         RGI_OPT_TILE_ORDER: 2 if config & RGI_TUNE_TILE_ORDER else 0,
+        RGI_OPT_TILE_SHAPE: 1 if config & RGI_TUNE_TILE_SHAPE else 0,
     }
 
 F, I32, U32, P = ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_void_p

@@ -152,6 +152,10 @@ int rgi_context_set_option(rgi_context* ctx, int option, int value)
         if (value < 0 || value > 2) return fail(RGI_E_INVALID, "tile order takes 0, 1 or 2, not %d", value);
         ctx->tuning.tile_order = value;
         return RGI_OK;
+    case RGI_OPT_TILE_SHAPE:
+        if (value != 0 && value != 1) return fail(RGI_E_INVALID, "tile shape takes 0 or 1, not %d", value);
+        ctx->tuning.tile_shape = value;
+        return RGI_OK;
     case RGI_OPT_TRIANGLE_TEST:
         if (value != 0 && value != 1) return fail(RGI_E_INVALID, "triangle test takes 0 or 1, not %d", value);
         ctx->tuning.watertight = value;
@@ -169,6 +173,7 @@ int rgi_context_get_option(const rgi_context* ctx, int option)
     case RGI_OPT_GI_SPATIAL_WAVES: return ctx->tuning.gi_spatial_waves;
     case RGI_OPT_TRACE_PERSISTENT: return ctx->tuning.trace_persistent;
     case RGI_OPT_TILE_ORDER: return ctx->tuning.tile_order;
+    case RGI_OPT_TILE_SHAPE: return ctx->tuning.tile_shape;
     case RGI_OPT_TRIANGLE_TEST: return ctx->tuning.watertight;
     default: return fail(RGI_E_INVALID, "unknown option %d", option);
     }

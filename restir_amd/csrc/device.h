@@ -25,7 +25,7 @@ struct F2 { float x, y; };
 struct F3 { float x, y, z; };
 struct F4 { float x, y, z, w; };
 
-// Workgroup -> 16 x 16 screen tile.
+// Workgroup -> screen tile.
 // Order 0 is row-major in workgroup id.
 // Order 1 hands each XCD a contiguous band of rows: the dispatcher places
 // workgroup b on XCD b % 8, so numbering that XCD's workgroups b / 8 = 0, 1,
@@ -39,8 +39,10 @@ struct F4 { float x, y, z, w; };
 // per block slot; slots past the screen edge have no tile.
 // Every order visits every tile exactly once; false means this workgroup has
 // no tile (order 2 only), and the whole workgroup leaves.
+// A tile is 16 x 16 pixels, or 32 x 8 with `wide` (the same 256 threads).
 struct TileMap {
     uint32_t order;
+    uint32_t wide;
     uint32_t tiles_x, tiles_y;
     uint32_t blocks_x; // order 2: 4 x 4-tile blocks per row
 };
@@ -64,12 +66,13 @@ RGI_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty)
 }
 
 // Host side: the map and the grid of a width x height screen pass.
-inline TileMap tile_map(int order, uint32_t width, uint32_t height)
+inline TileMap tile_map(int order, int wide, uint32_t width, uint32_t height)
 {
     TileMap m;
     m.order = (uint32_t)order;
-    m.tiles_x = (width + 15u) / 16u;
-    m.tiles_y = (height + 15u) / 16u;
+    m.wide = wide ? 1u : 0u;
+    m.tiles_x = wide ? (width + 31u) / 32u : (width + 15u) / 16u;
+    m.tiles_y = wide ? (height + 7u) / 8u : (height + 15u) / 16u;
     m.blocks_x = (m.tiles_x + 3u) / 4u;
     return m;
 }

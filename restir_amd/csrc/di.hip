@@ -134,8 +134,13 @@ RGI_DEV bool block_pixel(const TileMap& tiles, int& x, int& y)
 {
     uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, tx, ty;
     if (!block_tile(tiles, tx, ty)) return false;
-    x = (int)(tx * 16u + (wave & 1u) * 8u + (lane & 7u));
-    y = (int)(ty * 16u + (wave >> 1) * 8u + (lane >> 3));
+    if (tiles.wide) {
+        x = (int)(tx * 32u + wave * 8u + (lane & 7u));
+        y = (int)(ty * 8u + (lane >> 3));
+    } else {
+        x = (int)(tx * 16u + (wave & 1u) * 8u + (lane & 7u));
+        y = (int)(ty * 16u + (wave >> 1) * 8u + (lane >> 3));
+    }
     return true;
 }
 
@@ -369,7 +374,7 @@ __global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rg
 int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
                 const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     hipLaunchKernelGGL(k_di_temporal, tile_grid(tiles), dim3(256), 0, stream, f, s, g, prev, out,
                        texel, tiles);
@@ -379,7 +384,7 @@ int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
 int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
                uint16_t* diffuse, uint16_t* specular, const Tuning& tuning, hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     hipLaunchKernelGGL(k_di_spatial, tile_grid(tiles), dim3(256), 0, stream, f, s, g, res,
                        diffuse, specular, texel, tiles);

@@ -376,7 +376,7 @@ k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_g
 int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                 const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     auto kernel = tuning.gi_temporal_waves == 4 ? k_gi_temporal_w4 : k_gi_temporal;
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, out, texel, tiles);
@@ -387,7 +387,7 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
                const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
                hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     auto kernel = tuning.gi_spatial_waves == 4 ? k_gi_spatial_w4 : k_gi_spatial;
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, res, color,

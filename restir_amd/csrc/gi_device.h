@@ -123,14 +123,21 @@ RGI_DEV void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Pixel of this thread: 16 x 16 per block, two 4 x 8 groups per wave;
-// false when the workgroup has no tile.
+// Pixel of this thread: two 4 x 8 groups per wave, eight per block, as a
+// 4 x 2 arrangement of groups (16 x 16 tile) or 8 x 1 (32 x 8 wide tile);
+// every group stays on the reference's 4 x 8 grid. False when the workgroup
+// has no tile.
 RGI_DEV bool group_pixel(const TileMap& tiles, int& x, int& y)
 {
     uint32_t group = threadIdx.x >> 5, t = threadIdx.x & 31u, tx, ty;
     if (!block_tile(tiles, tx, ty)) return false;
-    x = (int)(tx * 16u + (group & 3u) * 4u + (t & 3u));
-    y = (int)(ty * 16u + (group >> 2) * 8u + (t >> 2));
+    if (tiles.wide) {
+        x = (int)(tx * 32u + group * 4u + (t & 3u));
+        y = (int)(ty * 8u + (t >> 2));
+    } else {
+        x = (int)(tx * 16u + (group & 3u) * 4u + (t & 3u));
+        y = (int)(ty * 16u + (group >> 2) * 8u + (t >> 2));
+    }
     return true;
 }
 

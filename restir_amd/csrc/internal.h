@@ -16,6 +16,7 @@ struct Tuning {
     int gi_spatial_waves = 0;
     int trace_persistent = 0;
     int tile_order = 0;
+    int tile_shape = 0; // 1: 32 x 8-pixel tiles (RGI_OPT_TILE_SHAPE)
     int watertight = 0; // rgi_trace's triangle test (RGI_OPT_TRIANGLE_TEST)
 };
 } // namespace rgi

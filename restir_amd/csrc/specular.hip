@@ -218,7 +218,7 @@ int specular_gi(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
                 const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out, const Tuning& tuning,
                 hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, g.width, g.height);
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     hipLaunchKernelGGL(k_specular, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, previous_hdr, history, out,
                        texel, tiles);

@@ -241,7 +241,7 @@ def test_svgf(dev, cornell_seq, step):
         assert_same("atrous", dev.get(atr_t, want_a), want_a)
 
 
-@pytest.mark.parametrize("case", ["default", "bvh_rays_sky", "xcd_blocks"])
+@pytest.mark.parametrize("case", ["default", "bvh_rays_sky", "xcd_blocks", "wide_tiles"])
 def test_specular_gi(dev, cornell_seq, case):
     """rgi_specular_gi against the oracle, bit-exact, with last frame's
     composite and specular history."""
@@ -267,7 +267,8 @@ def test_specular_gi(dev, cornell_seq, case):
     dgin = ra.GiInputs(None, d["rec"].data_ptr(), d["nor"].data_ptr(), d["col"].data_ptr(), d["mom"].data_ptr(),
                        d["sl"].data_ptr(), None, d["grid"].data_ptr())
     o = dev.empty(n * 8)
-    with options(dev, {ra.RGI_OPT_TILE_ORDER: 2} if case == "xcd_blocks" else {}):
+    opts = {"xcd_blocks": XCD_BLOCKS, "wide_tiles": WIDE_TILES}.get(case, {})
+    with options(dev, opts):
         ra.check(dev.L.rgi_specular_gi(dev.ctx, ctypes.byref(f), ctypes.byref(dscene), ctypes.byref(gb),
                                        ctypes.byref(dgin), d["hdr"].data_ptr(), d["spec"].data_ptr(), o.data_ptr(),
                                        dev.stream))
@@ -432,6 +433,7 @@ def oracle_gi_spatial(hs, f, g, hist, res, grid, surface_light):
 
 XCD_TILES = {ra.RGI_OPT_TILE_ORDER: 1}
 XCD_BLOCKS = {ra.RGI_OPT_TILE_ORDER: 2}
+WIDE_TILES = {ra.RGI_OPT_TILE_SHAPE: 1}
 
 # Per case: settings overrides, and "options" = launch variants of the
 # context (temporal waves also select spatial waves in the spatial test).
@@ -441,6 +443,8 @@ GI_CASES = {
     "default_waves4_xcd": {"options": {**XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
     "default_xcd_tiles": {"options": XCD_TILES},
     "default_xcd_blocks": {"options": XCD_BLOCKS},
+    "default_wide_tiles": {"options": WIDE_TILES},
+    "default_wide_xcd_blocks": {"options": {**WIDE_TILES, **XCD_BLOCKS}},
     "three_candidates": {"initial_candidates": 3},
     "short_march": {"trace_steps": 5},
     "bvh_rays": {"ray_mode": ra.RGI_RAY_BVH},
@@ -575,14 +579,14 @@ def di_both(dev, hs, f, g, prev):
     return got, w
 
 
-@pytest.mark.parametrize("frame_no,extra,order", [(0, 4, 0), (2, 4, 0), (2, 0, 0), (2, 4, 1), (2, 4, 2)])
+@pytest.mark.parametrize("frame_no,extra,order", [(0, 4, 0), (2, 4, 0), (2, 0, 0), (2, 4, 1), (2, 4, 2), (2, 4, "wide")])
 def test_di(dev, frame_no, extra, order):
     hs = common.cornell(extra_lights=extra)
     s = ra.Settings()
     seq = common.oracle_sequence(hs, s, W, H, frame_no + 1, yaw_step=2 * np.pi / 64)
     f, g, hist, out = seq[frame_no]
-    say("case di frame=%d extra=%d lights=%d tile_order=%d" % (frame_no, extra, hs.light_count, order))
-    with options(dev, {ra.RGI_OPT_TILE_ORDER: order}):
+    say("case di frame=%d extra=%d lights=%d tiles=%s" % (frame_no, extra, hs.light_count, order))
+    with options(dev, WIDE_TILES if order == "wide" else {ra.RGI_OPT_TILE_ORDER: order}):
         got, want = di_both(dev, hs, f, g, hist)
     for k in want:
         assert_same(k, got[k], want[k], rel=1e-4)

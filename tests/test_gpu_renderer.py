@@ -121,6 +121,65 @@ def run_renderer_case(torch, frames, kw, variants, scene, W, H):
         L.rgi_renderer_destroy(r)
 
 
+def test_autotune_keeps_images_bit_exact(torch):
+    """rgi_renderer_autotune times every trial (RGI_TUNE_TRIALS of them, each
+    > 0 ms), leaves its chosen configuration set on the renderer, and the
+    frames rendered afterwards are bit-identical to a default renderer's."""
+    say("case autotune")
+    L = ra.lib()
+    W2, H2 = 160, 96
+    hs = common.cornell(extra_lights=4)
+    s = ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC)
+    bvh, lights = upload(torch, hs.bvh), upload(torch, hs.lights)
+    scene = hs.rgi_scene(bvh.data_ptr(), lights.data_ptr())
+    yaw = 2 * np.pi / 64
+    views, gbs, keep = [], [], []
+    for i in range(4):
+        eye, vp, inv = ra.orbit_camera(hs, i, W2, H2, yaw_step=yaw)
+        f = common.frame_for(hs, s, W2, H2, i, i > 0, yaw_step=yaw)
+        planes = {k: upload(torch, v) for k, v in common.oracle_gbuffer(hs, f).items()}
+        keep.append(planes)
+        gbs.append(ra.GBuffer(W2, H2, planes["depth"].data_ptr(), planes["albedo"].data_ptr(),
+                              planes["normal"].data_ptr(), planes["shading"].data_ptr(), planes["emission"].data_ptr()))
+        views.append(ra.make_view(eye, vp, inv))
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rs = []
+    for _ in range(2):
+        r = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(r), W2, H2, ctypes.byref(ra.c_settings(s)), ctypes.byref(scene),
+                                       hs.radius))
+        rs.append(r)
+    try:
+        ms = (ctypes.c_float * ra.RGI_TUNE_TRIALS)()
+        frames = 2
+        chosen = L.rgi_renderer_autotune(rs[0], (ra.View * 4)(*views), (ra.GBuffer * 4)(*gbs), 4, frames, ms, stream)
+        assert chosen >= 0, chosen
+        assert all(v > 0.0 for v in ms), list(ms)
+        # The default renderer replays the frames the trials rendered (one
+        # untimed and `frames` timed per trial, cycling the views): the two
+        # renderers then hold the same history only if every trial's launch
+        # variants rendered the same images.
+        for k in range(ra.RGI_TUNE_TRIALS * (1 + frames)):
+            ra.check(L.rgi_renderer_render(rs[1], ctypes.byref(views[k % 4]), ctypes.byref(gbs[k % 4]), stream))
+        for i in range(4):
+            for r in rs:
+                ra.check(L.rgi_renderer_render(r, ctypes.byref(views[i]), ctypes.byref(gbs[i]), stream))
+            outs = []
+            for r in rs:
+                o = ra.RendererOutputs()
+                ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(o)))
+                outs.append(o)
+            for name in ("gi_sample", "color", "hdr"):
+                like = np.zeros((W2 * H2, 4), np.float32 if name == "gi_sample" else np.uint16)
+                a = download(torch, getattr(outs[0], name), like)
+                b = download(torch, getattr(outs[1], name), like)
+                assert np.array_equal(a, b), "frame %d: %s differs after tuning (config %d)" % (i, name, chosen)
+        say("  autotune chose %d, trials %s" % (chosen, [round(v, 4) for v in ms]))
+    finally:
+        for r in rs:
+            L.rgi_renderer_destroy(r)
+
+
 def test_reference_accumulate_and_error(torch):
     say("case reference accumulate/error")
     hs = common.cornell(extra_lights=4)
