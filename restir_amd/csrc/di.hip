@@ -156,12 +156,30 @@ RGI_DEV DiPass make_pass(const rgi_frame& f, const rgi_scene& s, const rgi_gbuff
     return p;
 }
 
+// Light records staged in LDS per workgroup when the scene's fit: the light
+// choice's binary search over the CDF and every light evaluation then read
+// LDS instead of a chain of dependent global loads. The records are the
+// same words, so results do not change.
+constexpr uint32_t kLdsLights = 256; // 16 KB
+
+RGI_DEV void stage_lights(DiPass& p, const rgi_frame& f, const rgi_scene& s, float4* lds)
+{
+    const uint32_t n = s.light_count;
+    if (f.di_light_count == 0 || n == 0 || n > kLdsLights) return; // the same for the whole workgroup
+    const float4* src = reinterpret_cast<const float4*>(s.lights);
+    for (uint32_t i = threadIdx.x; i < 4u * n; i += blockDim.x) lds[i] = src[i];
+    __syncthreads();
+    p.lights = lds;
+}
+
 __global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs prev,
                                                      rgi_di_reservoirs out, F2 texel, TileMap tiles)
 {
+    __shared__ float4 lights[4 * kLdsLights];
     DiPass p = make_pass(f, s, g, texel);
     int x, y;
-    if (!block_pixel(tiles, x, y)) return;
+    if (!block_pixel(tiles, x, y)) return; // the whole workgroup, before the barrier
+    stage_lights(p, f, s, lights);
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -271,9 +289,11 @@ __global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rg
                                                     uint16_t* out_diffuse, uint16_t* out_specular, F2 texel,
                                                     TileMap tiles)
 {
+    __shared__ float4 lights[4 * kLdsLights];
     DiPass p = make_pass(f, s, g, texel);
     int x, y;
-    if (!block_pixel(tiles, x, y)) return;
+    if (!block_pixel(tiles, x, y)) return; // the whole workgroup, before the barrier
+    stage_lights(p, f, s, lights);
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
