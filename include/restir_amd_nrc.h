@@ -27,6 +27,11 @@
  *             + 0.01), times the sample's weight, averaged over the batch.
  *   optimiser Adam with bias correction on FP32 master weights; a table entry
  *             whose gradient is exactly zero keeps its value and moments.
+ *   average   after each step, every parameter's moving average e = d e +
+ *             (1 - d) w with d = min(ema_decay, (1 + t) / (10 + t)) at step
+ *             t, in FP32 (both products rounded, then their sum); inference
+ *             reads the averaged weights and table, training the parameters.
+ *             Creation and rgi_nrc_set_params restart it at the parameters.
  *
  * Precision: training runs the MLP in BF16 on the matrix cores (products of
  * BF16 operands, FP32 accumulation); activations and back-propagated
@@ -79,7 +84,8 @@ typedef struct rgi_nrc_config {
     float epsilon;            /* 1e-8 */
     float aabb_min[3];        /* positions map linearly onto [0, 1]^3, clamped */
     float aabb_max[3];
-    int32_t reserved[4];
+    float ema_decay;          /* 0.95: inference reads a moving average of the parameters (0: the parameters) */
+    int32_t reserved[3];
 } rgi_nrc_config;
 
 void rgi_nrc_config_default(rgi_nrc_config* config);
@@ -114,6 +120,10 @@ int rgi_nrc_gradients(rgi_nrc* nrc, const float* records, const float* targets, 
  * clears the Adam moments and the step count. */
 int rgi_nrc_get_params(const rgi_nrc* nrc, float* mlp, float* table, void* stream);
 int rgi_nrc_set_params(rgi_nrc* nrc, const float* mlp, const float* table, void* stream);
+
+/* The moving average of the parameters, laid out as rgi_nrc_get_params (the
+ * parameters themselves when ema_decay is 0). */
+int rgi_nrc_get_average(const rgi_nrc* nrc, float* mlp, float* table, void* stream);
 
 /* Adam steps taken since creation or the last rgi_nrc_set_params. */
 int rgi_nrc_steps(const rgi_nrc* nrc, uint32_t* steps);

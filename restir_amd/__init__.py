@@ -132,7 +132,7 @@ class NrcConfig(ctypes.Structure):
     """rgi_nrc_config (include/restir_amd_nrc.h)."""
     _fields_ = [("log2_table", U32), ("base_resolution", U32), ("max_resolution", U32), ("seed", U32),
                 ("learning_rate", F), ("beta1", F), ("beta2", F), ("epsilon", F),
-                ("aabb_min", F * 3), ("aabb_max", F * 3), ("reserved", I32 * 4)]
+                ("aabb_min", F * 3), ("aabb_max", F * 3), ("ema_decay", F), ("reserved", I32 * 3)]
 
 
 class NrcSettings(ctypes.Structure):
@@ -238,6 +238,7 @@ def lib():
     L.rgi_nrc_infer.argtypes = [P, P, U32, ctypes.c_int, P, P]
     L.rgi_nrc_gradients.argtypes = [P, P, P, U32, P, P, P, P]
     L.rgi_nrc_get_params.argtypes = [P, P, P, P]
+    L.rgi_nrc_get_average.argtypes = [P, P, P, P]
     L.rgi_nrc_set_params.argtypes = [P, P, P, P]
     L.rgi_nrc_steps.argtypes = [P, ctypes.POINTER(U32)]
     L.rgi_nrc_query_gbuffer.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, ctypes.c_int, P, P]

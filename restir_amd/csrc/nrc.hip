@@ -49,6 +49,16 @@ struct rgi_nrc {
     float* scales = nullptr;      // [0..5] weight scales, [8..13] activation scales, [16..21] 1/(sa*sw)
     uint32_t* act_max = nullptr;  // max |a_k| of the last training batch, float bits, k = 0..5
     uint32_t* levels = nullptr;   // res[16], stride[16] (res + 1 for a densely indexed level, else 0)
+    // Moving average of the parameters and its device copies: what inference
+    // reads when config.ema_decay > 0 (training reads the parameters).
+    float* ema_table = nullptr;
+    uint32_t* ema_table_h = nullptr;
+    float* ema_mlp = nullptr;
+    __bf16* ema_w_fwd = nullptr;
+    __bf16* ema_w_bwd = nullptr;
+    _Float16* ema_w_fwd_h = nullptr;
+    uint8_t* ema_w_fwd_8 = nullptr;
+    float* ema_scales = nullptr;
     float* loss = nullptr;        // internal loss accumulator
     // Training scratch, grown on demand.
     __bf16* acts = nullptr;       // [6][tiles][64][256]
@@ -682,6 +692,49 @@ __global__ void k_nrc_copies(const float* mlp, const float* scales, __bf16* w_fw
     }
 }
 
+// The parameters' moving average after a step: e = d e + (1 - d) w, one
+// workgroup per weight matrix, with the FP8 weight scale of the averaged
+// matrix (the activation scales are the trained ones).
+__global__ void __launch_bounds__(256) k_nrc_ema_mlp(const float* mlp, float* ema, float d, float one_minus_d,
+                                                     const float* scales, float* ema_scales)
+{
+    __shared__ float red[4];
+    const int layer = (int)blockIdx.x;
+    const int lo = layer * 4096, hi = layer == 5 ? 20480 + 192 : lo + 4096;
+    float wmax = 0.0f;
+    for (int p = lo + (int)threadIdx.x; p < hi; p += 256) {
+        float e = d * ema[p] + one_minus_d * mlp[p];
+        ema[p] = e;
+        wmax = fmaxf(wmax, fabsf(e));
+    }
+    wmax = wave_max(wmax);
+    if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = wmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        wmax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        float sw = wmax > 0.0f ? 448.0f / wmax : 1.0f;
+        float sa = scales[8 + layer];
+        ema_scales[layer] = sw;
+        ema_scales[8 + layer] = sa;
+        ema_scales[16 + layer] = 1.0f / (sa * sw);
+    }
+}
+
+// The table's moving average, four values per thread, and its FP16 copy.
+__global__ void k_nrc_ema_table(const float* table, float* ema, uint32_t* ema_h, float d, float one_minus_d,
+                                size_t quads)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= quads) return;
+    float4 w = reinterpret_cast<const float4*>(table)[i];
+    float4 e = reinterpret_cast<float4*>(ema)[i];
+    e = make_float4(d * e.x + one_minus_d * w.x, d * e.y + one_minus_d * w.y, d * e.z + one_minus_d * w.z,
+                    d * e.w + one_minus_d * w.w);
+    reinterpret_cast<float4*>(ema)[i] = e;
+    reinterpret_cast<uint2*>(ema_h)[i] = make_uint2((uint32_t)float_to_half(e.x) | ((uint32_t)float_to_half(e.y) << 16),
+                                                    (uint32_t)float_to_half(e.z) | ((uint32_t)float_to_half(e.w) << 16));
+}
+
 __global__ void k_nrc_clear_max(uint32_t* act_max, float* loss)
 {
     if (threadIdx.x < 6) act_max[threadIdx.x] = 0u;
@@ -1085,25 +1138,27 @@ namespace {
 
 using namespace nrc;
 Grid grid_of(const rgi_nrc& n);
+rgi_nrc inference_view(const rgi_nrc& n);
 } // namespace
 
 int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* emission, int precision,
                       uint16_t* out, hipStream_t stream)
 {
     nrc::Grid gp = grid_of(*n);
+    const rgi_nrc view = inference_view(*n);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 4 * nrc::kInferB - 1) / (4u * nrc::kInferB));
     switch (precision) {
     case RGI_NRC_BF16:
-        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_BF16>, blocks, dim3(256), 0, stream, *n, gp, f, g, texel,
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_BF16>, blocks, dim3(256), 0, stream, view, gp, f, g, texel,
                            emission, out);
         break;
     case RGI_NRC_FP16:
-        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_FP16>, blocks, dim3(256), 0, stream, *n, gp, f, g, texel,
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_FP16>, blocks, dim3(256), 0, stream, view, gp, f, g, texel,
                            emission, out);
         break;
     case RGI_NRC_FP8:
-        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_FP8>, blocks, dim3(256), 0, stream, *n, gp, f, g, texel,
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer<RGI_NRC_FP8>, blocks, dim3(256), 0, stream, view, gp, f, g, texel,
                            emission, out);
         break;
     default: return fail(RGI_E_INVALID, "nrc: unknown precision %d", precision);
@@ -1245,6 +1300,63 @@ int refresh_copies(rgi_nrc* n, int step, hipStream_t stream)
     return check_launch("nrc weight copies");
 }
 
+// The moving average's decay at step t: the configured decay, held below
+// (1 + t) / (10 + t) over the first steps so that the initial parameters
+// fade out of the average quickly.
+float ema_decay_at(const rgi_nrc& n, uint32_t t)
+{
+    float warm = (float)(1.0 + (double)t) / (float)(10.0 + (double)t);
+    return fminf(n.config.ema_decay, warm);
+}
+
+// One step of the moving average (after the parameters' Adam step) and the
+// inference copies of the averaged weights.
+int refresh_average(rgi_nrc* n, hipStream_t stream)
+{
+    if (!(n->config.ema_decay > 0.0f)) return RGI_OK;
+    float d = ema_decay_at(*n, n->steps), om = 1.0f - d;
+    hipLaunchKernelGGL(k_nrc_ema_mlp, dim3(6), dim3(256), 0, stream, n->mlp, n->ema_mlp, d, om, n->scales,
+                       n->ema_scales);
+    hipLaunchKernelGGL(k_nrc_copies, dim3(((kFwdBlocks + kBwdBlocks) * kFrag + 255) / 256), dim3(256), 0, stream,
+                       n->ema_mlp, n->ema_scales, n->ema_w_fwd, n->ema_w_bwd, n->ema_w_fwd_h, n->ema_w_fwd_8);
+    size_t quads = n->table_floats / 4;
+    hipLaunchKernelGGL(k_nrc_ema_table, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, stream, n->table,
+                       n->ema_table, n->ema_table_h, d, om, quads);
+    return check_launch("nrc moving average");
+}
+
+// The average restarts at the parameters (after creation or set_params):
+// copies of the parameters and of their device copies, bit for bit.
+int reset_average(rgi_nrc* n, hipStream_t stream)
+{
+    if (!(n->config.ema_decay > 0.0f)) return RGI_OK;
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    bool ok = hipMemcpyAsync(n->ema_table, n->table, n->table_floats * 4, k, stream) == hipSuccess &&
+              hipMemcpyAsync(n->ema_table_h, n->table_h, n->table_floats * 2, k, stream) == hipSuccess &&
+              hipMemcpyAsync(n->ema_mlp, n->mlp, RGI_NRC_MLP_PARAMS * 4, k, stream) == hipSuccess &&
+              hipMemcpyAsync(n->ema_w_fwd, n->w_fwd, (size_t)kFwdBlocks * kFrag * 2, k, stream) == hipSuccess &&
+              hipMemcpyAsync(n->ema_w_bwd, n->w_bwd, (size_t)kBwdBlocks * kFrag * 2, k, stream) == hipSuccess &&
+              hipMemcpyAsync(n->ema_w_fwd_h, n->w_fwd_h, (size_t)kFwdBlocks * kFrag * 2, k, stream) == hipSuccess &&
+              hipMemcpyAsync(n->ema_w_fwd_8, n->w_fwd_8, (size_t)kFwdBlocks * kFrag, k, stream) == hipSuccess &&
+              hipMemcpyAsync(n->ema_scales, n->scales, 32 * 4, k, stream) == hipSuccess;
+    return ok ? RGI_OK : fail(RGI_E_DEVICE, "nrc: moving average reset failed");
+}
+
+// The cache as inference sees it: the averaged weights and table when the
+// average is on.
+rgi_nrc inference_view(const rgi_nrc& n)
+{
+    rgi_nrc v = n;
+    if (n.config.ema_decay > 0.0f) {
+        v.table_h = n.ema_table_h;
+        v.w_fwd = n.ema_w_fwd;
+        v.w_fwd_h = n.ema_w_fwd_h;
+        v.w_fwd_8 = n.ema_w_fwd_8;
+        v.scales = n.ema_scales;
+    }
+    return v;
+}
+
 } // namespace
 } // namespace rgi
 
@@ -1264,6 +1376,7 @@ void rgi_nrc_config_default(rgi_nrc_config* c)
     c->beta1 = 0.9f;
     c->beta2 = 0.99f;
     c->epsilon = 1e-8f;
+    c->ema_decay = 0.95f;
     for (int i = 0; i < 3; ++i) {
         c->aabb_min[i] = -1.0f;
         c->aabb_max[i] = 1.0f;
@@ -1293,6 +1406,8 @@ int rgi_nrc_create(rgi_nrc** out, const rgi_nrc_config* config)
     if (!(config->learning_rate > 0.0f) || !(config->beta1 >= 0.0f && config->beta1 < 1.0f) ||
         !(config->beta2 >= 0.0f && config->beta2 < 1.0f) || !(config->epsilon > 0.0f))
         return fail(RGI_E_INVALID, "nrc_create: Adam hyperparameters out of range");
+    if (!(config->ema_decay >= 0.0f && config->ema_decay < 1.0f))
+        return fail(RGI_E_INVALID, "nrc_create: ema_decay must be in [0, 1)");
     rgi_nrc* n = new rgi_nrc();
     n->config = *config;
     for (uint32_t l = 0; l < 16; ++l) {
@@ -1313,6 +1428,15 @@ int rgi_nrc_create(rgi_nrc** out, const rgi_nrc_config* config)
         (e = alloc(&n->w_bwd, (size_t)kBwdBlocks * kFrag)) || (e = alloc(&n->w_fwd_h, (size_t)kFwdBlocks * kFrag)) ||
         (e = alloc(&n->w_fwd_8, (size_t)kFwdBlocks * kFrag)) || (e = alloc(&n->scales, 32)) ||
         (e = alloc(&n->act_max, 8)) || (e = alloc(&n->loss, 1)) || (e = alloc(&n->levels, 32))) {
+        rgi_nrc_destroy(n);
+        return e;
+    }
+    if (config->ema_decay > 0.0f &&
+        ((e = alloc(&n->ema_table, n->table_floats)) || (e = alloc(&n->ema_table_h, n->table_floats / 2)) ||
+         (e = alloc(&n->ema_mlp, RGI_NRC_MLP_PARAMS)) || (e = alloc(&n->ema_w_fwd, (size_t)kFwdBlocks * kFrag)) ||
+         (e = alloc(&n->ema_w_bwd, (size_t)kBwdBlocks * kFrag)) ||
+         (e = alloc(&n->ema_w_fwd_h, (size_t)kFwdBlocks * kFrag)) ||
+         (e = alloc(&n->ema_w_fwd_8, (size_t)kFwdBlocks * kFrag)) || (e = alloc(&n->ema_scales, 32)))) {
         rgi_nrc_destroy(n);
         return e;
     }
@@ -1338,7 +1462,7 @@ int rgi_nrc_create(rgi_nrc** out, const rgi_nrc_config* config)
     hipLaunchKernelGGL(k_nrc_init_table, dim3((unsigned)((entries_total + 255) / 256)), dim3(256), 0, 0, n->table,
                        n->table_h, config->seed, entries_total);
     hipLaunchKernelGGL(k_nrc_init_mlp, dim3((RGI_NRC_MLP_PARAMS + 255) / 256), dim3(256), 0, 0, n->mlp, config->seed);
-    if ((e = refresh_copies(n, 0, 0)) || (e = check_launch("nrc init"))) {
+    if ((e = refresh_copies(n, 0, 0)) || (e = check_launch("nrc init")) || (e = reset_average(n, 0))) {
         rgi_nrc_destroy(n);
         return e;
     }
@@ -1356,7 +1480,9 @@ void rgi_nrc_destroy(rgi_nrc* n)
     (void)hipDeviceSynchronize();
     void* ptrs[] = {n->table, n->table_m, n->table_v, n->table_grad, n->table_h, n->mlp, n->mlp_m, n->mlp_v,
                     n->mlp_grad, n->w_fwd, n->w_bwd, n->w_fwd_h, n->w_fwd_8, n->scales, n->act_max, n->loss, n->levels,
-                    n->acts, n->deltas, n->partial, n->surface_counts, n->surface_offsets, n->surface_list};
+                    n->acts, n->deltas, n->partial, n->surface_counts, n->surface_offsets, n->surface_list,
+                    n->ema_table, n->ema_table_h, n->ema_mlp, n->ema_w_fwd, n->ema_w_bwd, n->ema_w_fwd_h,
+                    n->ema_w_fwd_8, n->ema_scales};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete n;
@@ -1375,6 +1501,7 @@ int rgi_nrc_train(rgi_nrc* n, const float* records, const float* targets, uint32
     hipLaunchKernelGGL(k_nrc_adam_table, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, s, n->table, n->table_m,
                        n->table_v, n->table_grad, n->table_h, h, quads);
     if ((e = check_launch("nrc adam (table)"))) return e;
+    if ((e = refresh_average(n, s))) return e;
     if (loss) hipLaunchKernelGGL(k_nrc_copy_loss, dim3(1), dim3(64), 0, s, n->loss, loss);
     return check_launch("nrc train");
 }
@@ -1402,16 +1529,17 @@ int rgi_nrc_infer(rgi_nrc* n, const float* records, uint32_t count, int precisio
     if (count == 0) return RGI_OK;
     hipStream_t s = as_stream(stream);
     Grid gp = grid_of(*n);
+    const rgi_nrc view = inference_view(*n);
     dim3 blocks((count + 64 * kInferB - 1) / (64 * kInferB));
     switch (precision) {
     case RGI_NRC_BF16:
-        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_BF16>, blocks, dim3(256), 0, s, *n, gp, records, count, out);
+        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_BF16>, blocks, dim3(256), 0, s, view, gp, records, count, out);
         break;
     case RGI_NRC_FP16:
-        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP16>, blocks, dim3(256), 0, s, *n, gp, records, count, out);
+        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP16>, blocks, dim3(256), 0, s, view, gp, records, count, out);
         break;
     case RGI_NRC_FP8:
-        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP8>, blocks, dim3(256), 0, s, *n, gp, records, count, out);
+        hipLaunchKernelGGL(k_nrc_infer<RGI_NRC_FP8>, blocks, dim3(256), 0, s, view, gp, records, count, out);
         break;
     default: return fail(RGI_E_INVALID, "nrc_infer: unknown precision %d", precision);
     }
@@ -1447,7 +1575,8 @@ int rgi_nrc_set_params(rgi_nrc* n, const float* mlp, const float* table, void* s
     (void)hipMemsetAsync(n->mlp_m, 0, RGI_NRC_MLP_PARAMS * 4, s);
     (void)hipMemsetAsync(n->mlp_v, 0, RGI_NRC_MLP_PARAMS * 4, s);
     n->steps = 0;
-    return refresh_copies(n, 0, s);
+    int e = refresh_copies(n, 0, s);
+    return e ? e : reset_average(n, s);
 }
 
 int rgi_nrc_query_gbuffer(rgi_nrc* n, const rgi_frame* f, const rgi_gbuffer* g, const uint16_t* emission,
@@ -1468,6 +1597,18 @@ int rgi_nrc_gather_gbuffer(rgi_nrc* n, const rgi_frame* f, const rgi_gbuffer* g,
     if (g->width <= 0 || g->height <= 0 || !g->depth || !g->normal || !g->albedo)
         return fail(RGI_E_INVALID, "nrc_gather_gbuffer: depth, normal and albedo planes are required");
     return nrc_gather(n, *f, *g, di_diffuse, color, batch, seed, records, targets, as_stream(stream));
+}
+
+int rgi_nrc_get_average(const rgi_nrc* n, float* mlp, float* table, void* stream)
+{
+    if (!n) return fail(RGI_E_INVALID, "nrc_get_average: nrc is NULL");
+    if (!(n->config.ema_decay > 0.0f)) return rgi_nrc_get_params(n, mlp, table, stream);
+    hipStream_t s = as_stream(stream);
+    if (mlp && hipMemcpyAsync(mlp, n->ema_mlp, RGI_NRC_MLP_PARAMS * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(RGI_E_DEVICE, "nrc_get_average: copy failed");
+    if (table && hipMemcpyAsync(table, n->ema_table, n->table_floats * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(RGI_E_DEVICE, "nrc_get_average: copy failed");
+    return RGI_OK;
 }
 
 int rgi_nrc_steps(const rgi_nrc* n, uint32_t* steps)

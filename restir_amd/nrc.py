@@ -86,6 +86,14 @@ class NeuralCache:
         ra.check(self.L.rgi_nrc_get_params(self.handle, mlp.data_ptr(), tab.data_ptr(), _stream()))
         return mlp, tab
 
+    def average(self):
+        """The moving average of the parameters (what inference reads)."""
+        import torch
+        mlp = torch.empty(ra.RGI_NRC_MLP_PARAMS, dtype=torch.float32, device="cuda")
+        tab = torch.empty(self.table_floats, dtype=torch.float32, device="cuda")
+        ra.check(self.L.rgi_nrc_get_average(self.handle, mlp.data_ptr(), tab.data_ptr(), _stream()))
+        return mlp, tab
+
     def set_params(self, mlp=None, table=None):
         ra.check(self.L.rgi_nrc_set_params(self.handle, mlp.data_ptr() if mlp is not None else None,
                                            table.data_ptr() if table is not None else None, _stream()))

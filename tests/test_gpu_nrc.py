@@ -74,10 +74,11 @@ def test_nrc_infer_fp16_fp8_close_to_exact():
     exact = model.forward(rec, emulate=0)
     scale = np.sqrt(np.mean(exact ** 2))
     # FP8 activation scales come from a training batch: take one step first,
-    # then compare against the updated parameters.
+    # then compare against the function inference evaluates, that of the
+    # parameters' moving average.
     tg = targets(rng, 8192)
     nc.train(torch.from_numpy(rec).cuda(), torch.from_numpy(tg).cuda())
-    mlp, table = nc.params()
+    mlp, table = nc.average()
     model.mlp[:] = mlp.cpu().numpy()
     model.table[:] = table.cpu().numpy()
     exact = model.forward(rec, emulate=0)
@@ -147,6 +148,46 @@ def smooth_radiance(rec):
     g = 0.5 + 0.3 * np.cos(3.0 * p[:, 2]) + 0.2 * p[:, 0] * p[:, 1]
     b = 0.4 + 0.3 * rec[:, 8] + 0.2 * np.sin(4.0 * p[:, 1] + p[:, 2])
     return np.stack([r, g, b], axis=1).astype(np.float32)
+
+
+def test_nrc_moving_average_follows_its_closed_form():
+    """After set_params the moving average is the parameters; each training
+    step then moves it to d e + (1 - d) w with d = min(ema_decay, (1 + t) /
+    (10 + t)) at step t, both products rounded in FP32 and then summed, bit
+    for bit; and inference reads it: BF16 and FP16 predictions equal those of
+    a cache without the average given the averaged parameters."""
+    nc, model, rng = setup_cache(31)
+    decay = np.float32(nc.cfg.ema_decay)
+    assert decay > 0
+    pm, pt = nc.params()
+    em, et = nc.average()
+    assert torch.equal(pm.view(torch.int32), em.view(torch.int32))
+    assert torch.equal(pt.view(torch.int32), et.view(torch.int32))
+    prev_m, prev_t = em.cpu().numpy(), et.cpu().numpy()
+    rec = torch.from_numpy(records(rng, 4096)).cuda()
+    tgt = torch.from_numpy(targets(rng, 4096)).cuda()
+    for t in (1, 2):
+        nc.train(rec, tgt)
+        wm, wt = (x.cpu().numpy() for x in nc.params())
+        em, et = (x.cpu().numpy() for x in nc.average())
+        d = np.minimum(decay, np.float32(1.0 + t) / np.float32(10.0 + t))
+        om = np.float32(1.0) - d
+        want_m = d * prev_m + om * wm
+        want_t = d * prev_t + om * wt
+        assert want_m.dtype == np.float32 and want_t.dtype == np.float32
+        assert np.array_equal(em.view(np.uint32), want_m.view(np.uint32)), t
+        assert np.array_equal(et.view(np.uint32), want_t.view(np.uint32)), t
+        assert not np.array_equal(em, wm)  # the average lags the parameters
+        prev_m, prev_t = em, et
+    plain = rnrc.NeuralCache(log2_table=nc.cfg.log2_table, seed=31, aabb_min=(-1, -1, -1), aabb_max=(1, 1, 1),
+                             ema_decay=0.0)
+    plain.set_params(torch.from_numpy(prev_m).cuda(), torch.from_numpy(prev_t).cuda())
+    q = torch.from_numpy(records(rng, 1000)).cuda()
+    for prec in (ra.RGI_NRC_BF16, ra.RGI_NRC_FP16):
+        a, b = nc.infer(q, prec), plain.infer(q, prec)
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32)), prec
+    plain.close()
+    nc.close()
 
 
 def test_nrc_training_learns_a_smooth_field():
