@@ -39,7 +39,8 @@ from restir_amd import driver  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per pixel of each pass (compulsory traffic with the
-# reference's formats; see DESIGN.md for the derivation).
+# reference's formats: each pass's reads and writes of its input and output
+# planes, reservoirs and grid entries, once per pixel).
 BYTES_PER_PIXEL = {
     "grid_insert": 20.0,
     "gi_temporal": 220.0,

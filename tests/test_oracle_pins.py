@@ -4,7 +4,9 @@ a closed form, an invariant, or brute force on a small input — chosen so that
 a plausible mistake (a dropped term, a wrong sign, index or constant) fails.
 
 The reference ships no tests, golden vectors or fixtures for this path, so
-these are the pins; see DESIGN.md for what each one covers.
+these are the pins. Each test's name or docstring says which oracle function
+it covers and what it is checked against; a few tests also check the host
+library's exports, ABI and argument checks without a GPU.
 """
 import ctypes
 import math
