@@ -56,7 +56,13 @@ typedef struct rgi_settings {
     int32_t atrous_iterations;    /* 5 (kMaxAtrousIterations), 0..5 */
     int32_t composite;            /* 1: the composite pass into an RGBA16F HDR image every frame */
     int32_t specular_gi;          /* 1: indirect specular (fs_specular.sc), composited with weight 1 */
-    int32_t reserved[3];
+    /* 2: the GI size is the window size over this divisor, floored, at least
+     * 1 (kGiResolutions, settings.h:18-23; giResolution = 1 -> Half,
+     * settings.h:218; renderer.cpp:519-520). 1, 2 or 4. The GI passes, the
+     * grid, the denoiser and indirect specular run at the GI size over the
+     * window-size G-buffer; DI and the composite at the window size. */
+    int32_t gi_divisor;
+    int32_t reserved[2];
 } rgi_settings;
 
 /* One frame's camera: eye and (jittered) view-projection and its inverse. */
@@ -87,15 +93,18 @@ typedef struct rgi_renderer rgi_renderer;
 void rgi_settings_default(rgi_settings* out);
 uint32_t rgi_settings_sizeof(void);
 
-/* Allocates every per-frame buffer for a width x height frame (GI at the same
- * size). `scene_radius` is describeScene's radius (scene_scale = radius /
- * 1.75, renderer.cpp:52, 501). */
+/* Allocates every per-frame buffer for a width x height window: G-buffer, DI,
+ * surface light and composite planes at that size, GI reservoirs, colour,
+ * moments, denoiser and indirect specular planes at the GI size
+ * (settings->gi_divisor). `scene_radius` is describeScene's radius
+ * (scene_scale = radius / 1.75, renderer.cpp:52, 501). */
 int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const rgi_settings* settings,
                         const rgi_scene* scene, float scene_radius);
 void rgi_renderer_destroy(rgi_renderer* renderer);
 
 /* Renders one frame of the hot path on `stream` from a G-buffer of the
- * renderer's size (device planes, including shading_normal and emission). */
+ * renderer's window size (device planes, including shading_normal and
+ * emission). */
 int rgi_renderer_render(rgi_renderer* renderer, const rgi_view* view, const rgi_gbuffer* gbuffer, void* stream);
 
 /* setReferenceAccumulation (renderer.cpp:1127-1142): while on, reuse,
@@ -150,8 +159,10 @@ int rgi_renderer_autotune(rgi_renderer* renderer, const rgi_view* views, const r
                           uint32_t frames, float* out_ms, void* stream);
 
 /* Packs the uniforms of one frame as Renderer::render does
- * (renderer.cpp:497-531, 614-621, 632-640, 693-731). `scene` supplies the
- * light count; `effective` settings are used as given (apply the reference
+ * (renderer.cpp:497-531, 614-621, 632-640, 693-731). `width` x `height` is
+ * the window (u_screenSize); the GI size (u_screenInfo, and u_gridInfo.y's
+ * pixel size) is that over effective->gi_divisor. `scene` supplies the light
+ * count; `effective` settings are used as given (apply the reference
  * override before calling). */
 int rgi_frame_pack(const rgi_settings* effective, const rgi_scene* scene, float scene_radius, uint32_t frame_number,
                    const rgi_view* view, const rgi_view* previous_view, int32_t width, int32_t height,

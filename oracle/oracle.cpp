@@ -175,6 +175,7 @@ struct Pass {
     const rgi_gbuffer* g;
     V2 texel_size;   // u_screenInfo.zw / u_screenSize.zw
     V2 size;         // u_screenInfo.xy
+    int gi_w, gi_h;  // GI-size planes (reservoirs, history): u_screenInfo.xy
     V3 grid_camera;  // u_gridCamera of this pass
     const uint32_t* grid;
     const rgi_gi_inputs* in;
@@ -334,7 +335,7 @@ uint32_t grid_pick_entry(const rgi_frame* f, const uint32_t* grid, V3 camera, V3
 bool insert_key(const rgi_frame* f, const rgi_gbuffer* g, int x, int y, uint32_t& slot, uint32_t& checksum,
                 float& cell_size)
 {
-    V2 ts = V2{1.0f / (float)g->width, 1.0f / (float)g->height};
+    V2 ts = V2{1.0f / (float)f->gi_width, 1.0f / (float)f->gi_height}; // u_screenInfo.zw
     V2 uv = V2{((float)x + 0.5f) * ts.x, ((float)y + 0.5f) * ts.y};
     float depth = g->depth[texel(uv, g->width, g->height)];
     if (depth >= kSkyDepth) return false;
@@ -703,14 +704,22 @@ float encode_sample_id(const rgi_scene* s, int tri, V3 dir)
     return dot(pn, dir) <= 0.0f ? (float)(tri + 1) : -(float)(tri + 1);
 }
 
+// Reservoir and history planes: GI size (the G-buffer is the window's size).
 V4 fetch_f4(const float* plane, const Pass& p, V2 uv)
 {
-    return float4(plane + texel(uv, p.g->width, p.g->height) * 4);
+    return float4(plane + texel(uv, p.gi_w, p.gi_h) * 4);
 }
 
 V4 fetch_h4(const uint16_t* plane, const Pass& p, V2 uv)
 {
-    return half4(plane + texel(uv, p.g->width, p.g->height) * 4);
+    return half4(plane + texel(uv, p.gi_w, p.gi_h) * 4);
+}
+
+// s_surfaceLight (renderer.cpp:690-695): the DI diffuse light or the
+// G-buffer emission, both window-size planes.
+V3 fetch_surface_light(const Pass& p, V2 uv)
+{
+    return xyz(half4(p.in->surface_light + texel(uv, p.g->width, p.g->height) * 4));
 }
 
 // The same-surface test of previousIndirect / historyIrradiance / reprojectHistory
@@ -882,7 +891,7 @@ bool evaluate_sample(const Pass& p, V3 rpos, V3 rn, V3 ralbedo, V3 spos, float s
     if (visible_surface) {
         V3 albedo = diffuse_albedo(p, suv);
         radiance = (world_direct(p, albedo, spos, snormal) + previous_indirect(p, spos, snormal, albedo)) +
-                   xyz(fetch_h4(p.in->surface_light, p, suv)) * f->lighting_surface;
+                   fetch_surface_light(p, suv) * f->lighting_surface;
     } else {
         V3 albedo = bvh_triangle_albedo(p.s, tri);
         radiance = (world_direct(p, albedo, spos, snormal) + previous_indirect(p, spos, snormal, albedo)) +
@@ -1090,7 +1099,7 @@ float finish_mis(const Pass& p, Mis& m, V3 position, V3 n, const Reservoir& c, f
 // giPixelUv (restir.sh:180-184).
 V2 gi_pixel_uv(const Pass& p, uint32_t index)
 {
-    uint32_t w = (uint32_t)p.g->width;
+    uint32_t w = (uint32_t)p.gi_w;
     return V2{((float)(index % w) + 0.5f) * p.texel_size.x, ((float)(index / w) + 0.5f) * p.texel_size.y};
 }
 
@@ -1101,12 +1110,25 @@ Pass make_pass(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g, con
     p.s = s;
     p.g = g;
     p.in = in;
-    p.size = V2{(float)g->width, (float)g->height};
-    p.texel_size = V2{1.0f / (float)g->width, 1.0f / (float)g->height};
+    p.gi_w = f->gi_width;
+    p.gi_h = f->gi_height;
+    p.size = V2{(float)f->gi_width, (float)f->gi_height};
+    p.texel_size = V2{1.0f / (float)f->gi_width, 1.0f / (float)f->gi_height};
     const float* cam = previous_camera ? f->prev_camera_position : f->camera_position;
     p.grid_camera = V3{cam[0], cam[1], cam[2]};
     p.grid = in ? in->grid : nullptr;
     p.grid_mask = f->grid_table_size - 1u;
+    return p;
+}
+
+// The DI passes run at the window size (u_screenSize, cs_di_temporal.sc:28-30).
+Pass make_di_pass(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g)
+{
+    Pass p = make_pass(f, s, g, nullptr, false);
+    p.gi_w = g->width;
+    p.gi_h = g->height;
+    p.size = V2{(float)g->width, (float)g->height};
+    p.texel_size = V2{1.0f / (float)g->width, 1.0f / (float)g->height};
     return p;
 }
 
@@ -1175,7 +1197,7 @@ V3 specular_hit_radiance(const Pass& p, const uint16_t* previous_hdr, V3 hit, fl
         V3 n = decode_normal(p, uv);
         V3 albedo = diffuse_albedo(p, uv);
         return (world_direct(p, albedo, hit, n) + previous_indirect(p, hit, n, albedo)) +
-               xyz(fetch_h4(p.in->surface_light, p, uv)) * f->lighting_surface;
+               fetch_surface_light(p, uv) * f->lighting_surface;
     }
     int tri = to_int(fabsf(id) + 0.5f) - 1;
     if (tri < 0 || !bvh_available(p.s)) return v3s(0.0f);
@@ -1558,14 +1580,14 @@ void orc_specular_albedo(const float* f0, float roughness, float nv, float* out)
     out[2] = r.z;
 }
 
-// fs_composite.sc:82-143 over gbuffer->width x height pixels; GI planes at
-// frame->width x height.
+// fs_composite.sc:82-143 over gbuffer->width x height pixels; GI and indirect
+// specular planes at frame->gi_width x gi_height (u_screenInfo).
 void orc_composite(const rgi_frame* f, const rgi_gbuffer* g, const rgi_composite_inputs* in, uint16_t* out)
 {
     int w = g->width, h = g->height;
     V2 ts = V2{1.0f / (float)w, 1.0f / (float)h};
-    V2 gi_size = V2{(float)f->width, (float)f->height};
-    V2 gi_texel = V2{1.0f / (float)f->width, 1.0f / (float)f->height};
+    V2 gi_size = V2{(float)f->gi_width, (float)f->gi_height};
+    V2 gi_texel = V2{1.0f / (float)f->gi_width, 1.0f / (float)f->gi_height};
     V3 camera = V3{f->camera_position[0], f->camera_position[1], f->camera_position[2]};
     auto h4 = [](const uint16_t* plane, size_t index) {
         return plane ? half4(plane + index * 4) : V4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -1629,12 +1651,12 @@ void orc_composite(const rgi_frame* f, const rgi_gbuffer* g, const rgi_composite
             float plane_step = 0.03f * f->scene_scale >= fabsf(dot(tap_position - position, surface_normal)) ? 1.0f : 0.0f;
             float normal_step = dot(tap_normal, surface_normal) >= 0.9f ? 1.0f : 0.0f;
             float weight = ((wx * wy + 0.01f) * plane_step) * normal_step;
-            size_t it = texel(tuv, f->width, f->height);
+            size_t it = texel(tuv, f->gi_width, f->gi_height);
             sum = sum + xyz(h4(in->gi, it)) * weight;
             specular_sum = specular_sum + xyz(h4(in->specular, it)) * weight;
             weight_sum += weight;
         }
-        size_t own = texel(uv, f->width, f->height);
+        size_t own = texel(uv, f->gi_width, f->gi_height);
         V3 irradiance = weight_sum > 0.0f ? sum / weight_sum : xyz(h4(in->gi, own));
         V3 specular_radiance = weight_sum > 0.0f ? specular_sum / weight_sum : xyz(h4(in->specular, own));
         V3 modulate = in->options[2] > 0.5f ? (albedo * (1.0f - metallic)) * eo.w : v3s(1.0f);
@@ -1673,10 +1695,11 @@ void orc_specular_gi(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* 
 {
     Pass p = make_pass(f, s, g, in, false);
     float sc = f->scene_scale;
-    int w = g->width, h = g->height;
+    int w = g->width, h = g->height;        // G-buffer
+    int gw = f->gi_width, gh = f->gi_height; // this pass's pixels and planes
     V3 camera = V3{f->camera_position[0], f->camera_position[1], f->camera_position[2]};
-    parallel_for((int64_t)w * h, [&](int64_t i) {
-        int x = (int)(i % w), y = (int)(i / w);
+    parallel_for((int64_t)gw * gh, [&](int64_t i) {
+        int x = (int)(i % gw), y = (int)(i / gw);
         V2 uv = V2{((float)x + 0.5f) * p.texel_size.x, ((float)y + 0.5f) * p.texel_size.y};
         float depth = fetch_depth(p, uv);
         if (depth >= kSkyDepth) {
@@ -1749,7 +1772,7 @@ void orc_specular_gi(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* 
 void orc_svgf_variance(const rgi_frame* f, const uint16_t* color, const uint16_t* moments, const float* receiver,
                        const float* normal, uint16_t* out)
 {
-    int w = f->width, h = f->height;
+    int w = f->gi_width, h = f->gi_height;
     Planes t{w, h};
     V2 ts = V2{1.0f / (float)w, 1.0f / (float)h};
     float scale = f->scene_scale;
@@ -1794,7 +1817,7 @@ void orc_svgf_variance(const rgi_frame* f, const uint16_t* color, const uint16_t
 void orc_svgf_atrous(const rgi_frame* f, const uint16_t* input, const float* receiver, const float* normal,
                      int step, uint16_t* out)
 {
-    int w = f->width, h = f->height;
+    int w = f->gi_width, h = f->gi_height;
     Planes t{w, h};
     V2 ts = V2{1.0f / (float)w, 1.0f / (float)h};
     float scale = f->scene_scale;
@@ -1855,9 +1878,10 @@ uint16_t orc_half_bits(float value) { return rgi_half_bits(value); }
 void orc_grid_keys(const rgi_frame* frame, const rgi_gbuffer* g, uint32_t* out_slot, uint32_t* out_checksum,
                    float* out_cell_size)
 {
-    int64_t n = (int64_t)g->width * g->height;
+    // One key per GI pixel (frame->gi_width x gi_height) from the window-size G-buffer.
+    int64_t n = (int64_t)frame->gi_width * frame->gi_height;
     parallel_for(n, [&](int64_t i) {
-        int x = (int)(i % g->width), y = (int)(i / g->width);
+        int x = (int)(i % frame->gi_width), y = (int)(i / frame->gi_width);
         uint32_t slot = 0, checksum = 0;
         float cell = 0.0f;
         if (!insert_key(frame, g, x, y, slot, checksum, cell)) {
@@ -1883,7 +1907,7 @@ void orc_grid_clear(uint32_t* grid, uint32_t table_size)
 void orc_grid_insert(const rgi_frame* frame, const rgi_gbuffer* g, uint32_t* grid)
 {
     uint32_t table = frame->grid_table_size, mask = table - 1u;
-    int64_t n = (int64_t)g->width * g->height;
+    int64_t n = (int64_t)frame->gi_width * frame->gi_height;
     std::vector<uint32_t> slot0((size_t)n), checksum((size_t)n);
     orc_grid_keys(frame, g, slot0.data(), checksum.data(), nullptr);
     std::vector<uint32_t> resolved((size_t)n, kGridInvalid);
@@ -1952,7 +1976,7 @@ void orc_gi_temporal(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* 
                      const rgi_gi_reservoirs* out)
 {
     Pass p = make_pass(f, s, g, in, true);
-    int groups_x = (g->width + 3) / 4, groups_y = (g->height + 7) / 8;
+    int groups_x = (p.gi_w + 3) / 4, groups_y = (p.gi_h + 7) / 8;
     parallel_for((int64_t)groups_x * groups_y, [&](int64_t gi) {
         int gx = (int)(gi % groups_x), gy = (int)(gi / groups_x);
         TemporalThread th[32];
@@ -1985,7 +2009,7 @@ void orc_gi_temporal(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* 
             }
             if (!h.inside) continue;
             int x = gx * 4 + (t & 3), y = gy * 8 + (t >> 2);
-            size_t o = ((size_t)y * g->width + x) * 4;
+            size_t o = ((size_t)y * p.gi_w + x) * 4;
             if (h.working) {
                 store_float4(out->sample + o, V4{h.mis.combined.position.x, h.mis.combined.position.y,
                                                  h.mis.combined.position.z, h.mis.total_count});
@@ -2005,9 +2029,9 @@ void orc_gi_spatial(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g
                     const rgi_gi_reservoirs* res, uint16_t* out_color, uint16_t* out_moments)
 {
     Pass p = make_pass(f, s, g, in, false);
-    int64_t n = (int64_t)g->width * g->height;
+    int64_t n = (int64_t)p.gi_w * p.gi_h;
     parallel_for(n, [&](int64_t i) {
-        int x = (int)(i % g->width), y = (int)(i / g->width);
+        int x = (int)(i % p.gi_w), y = (int)(i / p.gi_w);
         V2 uv = V2{((float)x + 0.5f) * p.texel_size.x, ((float)y + 0.5f) * p.texel_size.y};
         uint16_t* oc = out_color + (size_t)i * 4;
         uint16_t* om = out_moments + (size_t)i * 4;
@@ -2021,7 +2045,7 @@ void orc_gi_spatial(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g
         V3 n3 = decode_normal(p, uv);
         V3 albedo = diffuse_albedo(p, uv);
         float px = floorf(uv.x * p.size.x), py = floorf(uv.y * p.size.y);
-        uint32_t pixel_index = (uint32_t)to_int(py) * (uint32_t)g->width + (uint32_t)to_int(px);
+        uint32_t pixel_index = (uint32_t)to_int(py) * (uint32_t)p.gi_w + (uint32_t)to_int(px);
         uint32_t rng = pcg((uint32_t)to_int(px) + pcg((uint32_t)to_int(py) + pcg(f->frame_index ^ 0x68e31da4u)));
 
         V4 own_sample = fetch_f4(res->sample, p, uv);
@@ -2116,7 +2140,7 @@ void orc_gi_spatial(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g
 void orc_di_temporal(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g, const rgi_di_reservoirs* prev,
                      const rgi_di_reservoirs* out)
 {
-    Pass p = make_pass(f, s, g, nullptr, false);
+    Pass p = make_di_pass(f, s, g);
     int64_t n = (int64_t)g->width * g->height;
     parallel_for(n, [&](int64_t i) {
         int x = (int)(i % g->width), y = (int)(i / g->width);
@@ -2232,7 +2256,7 @@ void orc_di_temporal(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* 
 void orc_di_spatial(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g, const rgi_di_reservoirs* res,
                     uint16_t* out_diffuse, uint16_t* out_specular)
 {
-    Pass p = make_pass(f, s, g, nullptr, false);
+    Pass p = make_di_pass(f, s, g);
     int64_t n = (int64_t)g->width * g->height;
     parallel_for(n, [&](int64_t i) {
         int x = (int)(i % g->width), y = (int)(i / g->width);

@@ -96,7 +96,7 @@ class CSettings(ctypes.Structure):
         ("grid_table_size", U32), ("sky_zenith", F * 3), ("sky_horizon", F * 3), ("sky_ground", F * 3),
         ("sky_intensity", F), ("light_position", F * 3), ("light_color", F * 3), ("origin_bottom_left", I32),
         ("homogeneous_depth", I32), ("shadow_face_size", I32), ("denoiser", I32), ("atrous_iterations", I32),
-        ("composite", I32), ("specular_gi", I32), ("reserved", I32 * 3),
+        ("composite", I32), ("specular_gi", I32), ("gi_divisor", I32), ("reserved", I32 * 2),
     ]
 
 
@@ -400,6 +400,11 @@ class Settings:
         self.specular_gi = True        # fs_specular.sc (settings.h:196)
         self.grid_mode = RGI_GRID_ATOMIC
         self.grid_table_size = 65536
+        # GI at the window size over this divisor (kGiResolutions,
+        # settings.h:18-23). The reference's default is 2 (Half) and so is
+        # rgi_settings_default's; the benchmark configurations give the GI
+        # size as the G-buffer size, so the default here is 1.
+        self.gi_divisor = 1
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -441,6 +446,7 @@ def c_settings(settings):
     c.atrous_iterations = int(s.atrous_iterations)
     c.composite = int(bool(s.composite))
     c.specular_gi = int(bool(s.specular_gi))
+    c.gi_divisor = int(s.gi_divisor)
     return c
 
 

@@ -64,6 +64,27 @@ static int check_frame(const rgi_frame* f)
     return RGI_OK;
 }
 
+// The GI size (u_screenInfo.xy) of a pass over GI-size planes.
+static int check_gi_size(const rgi_frame* f)
+{
+    if (f->gi_width <= 0 || f->gi_height <= 0 || (int64_t)f->gi_width * f->gi_height >= (1ll << 29))
+        return fail(RGI_E_INVALID, "frame GI size %d x %d out of range", f->gi_width, f->gi_height);
+    return RGI_OK;
+}
+
+// The GI passes read a window-size G-buffer at GI-pixel centres; the
+// reference's GI size is the window size over a divisor (renderer.cpp:519-520),
+// so a G-buffer smaller than the GI size is a caller error.
+static int check_gi_gbuffer(const rgi_frame* f, const rgi_gbuffer* g)
+{
+    int e = check_gi_size(f);
+    if (e != RGI_OK) return e;
+    if (g->width < f->gi_width || g->height < f->gi_height)
+        return fail(RGI_E_INVALID, "gbuffer %d x %d is smaller than the frame's GI size %d x %d", g->width, g->height,
+                    f->gi_width, f->gi_height);
+    return RGI_OK;
+}
+
 static int check_gbuffer(const rgi_gbuffer* g, bool need_albedo, bool need_shading)
 {
     if (!g) return fail(RGI_E_INVALID, "gbuffer is NULL");
@@ -193,6 +214,7 @@ int rgi_grid_insert(rgi_context* ctx, const rgi_frame* frame, const rgi_gbuffer*
     int e;
     if ((e = check_frame(frame)) != RGI_OK) return e;
     if ((e = check_gbuffer(g, false, false)) != RGI_OK) return e;
+    if ((e = check_gi_gbuffer(frame, g)) != RGI_OK) return e;
     if (!grid) return fail(RGI_E_INVALID, "grid is NULL");
     if (mode == RGI_GRID_ATOMIC) return grid_insert_atomic(*frame, *g, grid, as_stream(stream));
     if (mode == RGI_GRID_DETERMINISTIC) {
@@ -215,8 +237,7 @@ int rgi_gi_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* s
         return fail(RGI_E_INVALID, "gi temporal: history planes, surface light and previous grid are required");
     if (frame->shadows_enabled && !in->shadow_map) return fail(RGI_E_INVALID, "shadows enabled but shadow_map is NULL");
     if (!out->sample || !out->receiver || !out->normal) return fail(RGI_E_INVALID, "gi temporal: output planes are required");
-    if (frame->gi_width != g->width || frame->gi_height != g->height)
-        return fail(RGI_E_INVALID, "frame GI size %d x %d != gbuffer %d x %d", frame->gi_width, frame->gi_height, g->width, g->height);
+    if ((e = check_gi_gbuffer(frame, g)) != RGI_OK) return e;
     return gi_temporal(*frame, *scene, *g, *in, *out, ctx ? ctx->tuning : Tuning{}, as_stream(stream));
 }
 
@@ -234,8 +255,7 @@ int rgi_gi_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* sc
         return fail(RGI_E_INVALID, "gi spatial: history planes, surface light and grid are required");
     if (frame->shadows_enabled && !in->shadow_map) return fail(RGI_E_INVALID, "shadows enabled but shadow_map is NULL");
     if (!res->sample || !res->receiver || !res->normal) return fail(RGI_E_INVALID, "gi spatial: reservoir planes are required");
-    if (frame->gi_width != g->width || frame->gi_height != g->height)
-        return fail(RGI_E_INVALID, "frame GI size %d x %d != gbuffer %d x %d", frame->gi_width, frame->gi_height, g->width, g->height);
+    if ((e = check_gi_gbuffer(frame, g)) != RGI_OK) return e;
     return gi_spatial(*frame, *scene, *g, *in, *res, out_color, out_moments, ctx ? ctx->tuning : Tuning{},
                       as_stream(stream));
 }
@@ -313,6 +333,7 @@ int rgi_specular_gi(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* s
     if ((e = check_frame(frame)) != RGI_OK) return e;
     if ((e = check_scene(scene, false)) != RGI_OK) return e;
     if ((e = check_gbuffer(g, true, false)) != RGI_OK) return e;
+    if ((e = check_gi_gbuffer(frame, g)) != RGI_OK) return e;
     if (!in || !out) return fail(RGI_E_INVALID, "specular_gi: inputs and output are required");
     if (frame->history_valid && (!in->history_receiver || !in->history_normal || !in->history_color))
         return fail(RGI_E_INVALID, "specular_gi: history planes are required when history_valid");
@@ -330,6 +351,7 @@ int rgi_composite(rgi_context* ctx, const rgi_frame* frame, const rgi_gbuffer* g
     if (!frame || !in || !out) return fail(RGI_E_INVALID, "composite: frame, inputs and output are required");
     if (frame->width <= 0 || frame->height <= 0) return fail(RGI_E_INVALID, "frame width/height must be positive");
     if ((e = check_gbuffer(g, true, true)) != RGI_OK) return e;
+    if ((e = check_gi_gbuffer(frame, g)) != RGI_OK) return e;
     if (!g->emission) return fail(RGI_E_INVALID, "composite: the emission plane is required");
     if (!in->gi || !aligned(in->gi, 8)) return fail(RGI_E_INVALID, "composite: the GI plane is required (8-byte aligned)");
     if (frame->shadows_enabled && in->shadow_map && (frame->shadow_width <= 0 || frame->shadow_height <= 0))
@@ -342,7 +364,8 @@ int rgi_svgf_variance(rgi_context* ctx, const rgi_frame* frame, const uint16_t* 
 {
     (void)ctx;
     if (!frame) return fail(RGI_E_INVALID, "frame is NULL");
-    if (frame->width <= 0 || frame->height <= 0) return fail(RGI_E_INVALID, "frame width/height must be positive");
+    int e = check_gi_size(frame);
+    if (e != RGI_OK) return e;
     if (!color || !moments || !receiver || !normal || !out) return fail(RGI_E_INVALID, "svgf_variance: NULL plane");
     if (!aligned(color, 8) || !aligned(moments, 8) || !aligned(out, 8) || !aligned(receiver, 16) || !aligned(normal, 16))
         return fail(RGI_E_INVALID, "svgf_variance: misaligned plane");
@@ -354,7 +377,8 @@ int rgi_svgf_atrous(rgi_context* ctx, const rgi_frame* frame, const uint16_t* in
 {
     (void)ctx;
     if (!frame) return fail(RGI_E_INVALID, "frame is NULL");
-    if (frame->width <= 0 || frame->height <= 0) return fail(RGI_E_INVALID, "frame width/height must be positive");
+    int e = check_gi_size(frame);
+    if (e != RGI_OK) return e;
     if (!input || !receiver || !normal || !out) return fail(RGI_E_INVALID, "svgf_atrous: NULL plane");
     if (input == out) return fail(RGI_E_INVALID, "svgf_atrous: out must not alias input");
     if (step < 1 || step > 1 << 20) return fail(RGI_E_INVALID, "svgf_atrous: step %d out of range", step);

@@ -23,14 +23,16 @@ RGI_DEV F3 gbuffer_normal(const uint32_t* plane, uint32_t index)
 }
 
 // upsampleIndirect (fs_composite.sc:48-80): the four GI texels around the
-// pixel, bilinear weights plus 0.01, kept only on the pixel's surface.
+// pixel, bilinear weights plus 0.01, kept only on the pixel's surface. The GI
+// and indirect specular planes are GI size (u_screenInfo), the G-buffer the
+// window's size.
 RGI_DEV void upsample_indirect(const rgi_frame& f, const rgi_gbuffer& g, const rgi_composite_inputs& in, F2 uv,
                                F3 position, F3 n, F3& diffuse, F3& specular)
 {
-    float tcx = uv.x * (float)f.width - 0.5f, tcy = uv.y * (float)f.height - 0.5f;
+    float tcx = uv.x * (float)f.gi_width - 0.5f, tcy = uv.y * (float)f.gi_height - 0.5f;
     float bx = floorf(tcx), by = floorf(tcy);
     float blx = tcx - bx, bly = tcy - by;
-    F2 gi_texel = F2{1.0f / (float)f.width, 1.0f / (float)f.height};
+    F2 gi_texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
     F3 sum = splat3(0.0f), specular_sum = splat3(0.0f);
     float weight_sum = 0.0f;
     for (int k = 0; k < 4; ++k) {
@@ -45,12 +47,12 @@ RGI_DEV void upsample_indirect(const rgi_frame& f, const rgi_gbuffer& g, const r
         float plane_step = 0.03f * f.scene_scale >= fabsf(dot(tap_position - position, n)) ? 1.0f : 0.0f;
         float normal_step = dot(tap_normal, n) >= 0.9f ? 1.0f : 0.0f;
         float weight = ((wx * wy + 0.01f) * plane_step) * normal_step;
-        uint32_t it = texel_index(tuv, f.width, f.height);
+        uint32_t it = texel_index(tuv, f.gi_width, f.gi_height);
         sum = sum + xyz(load_h4(in.gi, it)) * weight;
         specular_sum = specular_sum + xyz(h4_or_zero(in.specular, it)) * weight;
         weight_sum += weight;
     }
-    uint32_t own = texel_index(uv, f.width, f.height);
+    uint32_t own = texel_index(uv, f.gi_width, f.gi_height);
     diffuse = weight_sum > 0.0f ? sum / weight_sum : xyz(load_h4(in.gi, own));
     specular = weight_sum > 0.0f ? specular_sum / weight_sum : xyz(h4_or_zero(in.specular, own));
 }

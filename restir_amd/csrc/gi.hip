@@ -69,15 +69,14 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     p.v = scene_view(s);
     p.g = g;
     p.in = in;
-    p.texel = texel;
-    p.size = F2{(float)g.width, (float)g.height};
+    set_gi_size(p, f);
     p.grid_camera = F3{f.prev_camera_position[0], f.prev_camera_position[1], f.prev_camera_position[2]};
     p.sc = f.scene_scale;
 
     int x, y;
     if (!group_pixel(tiles, x, y)) return;
     uint32_t lane = threadIdx.x & 63u, qbase = threadIdx.x & ~63u;
-    bool inside = x < g.width && y < g.height;
+    bool inside = x < p.hw && y < p.hh;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
     float depth = inside ? fetch_depth(p, uv) : 1.0f;
     bool working = inside && depth < kSkyDepth;
@@ -86,7 +85,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     // 4 x 8 group sums never reach another wave).
     if (__ballot(working) == 0ull) {
         if (inside) {
-            uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+            uint32_t o = (uint32_t)y * (uint32_t)p.hw + (uint32_t)x;
             F4 zero = F4{0.0f, 0.0f, 0.0f, 0.0f};
             store_f4(out.sample, o, zero);
             store_f4(out.receiver, o, zero);
@@ -209,7 +208,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
         if (estimate > limit) combined_weight = combined_weight * (limit / estimate);
     }
     if (!inside) return;
-    uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    uint32_t o = (uint32_t)y * (uint32_t)p.hw + (uint32_t)x;
     if (working) {
         store_f4(out.sample, o, F4{mis.combined.position.x, mis.combined.position.y, mis.combined.position.z, mis.total});
         store_f4(out.receiver, o, F4{position.x, position.y, position.z, combined_weight});
@@ -231,15 +230,14 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     p.v = scene_view(s);
     p.g = g;
     p.in = in;
-    p.texel = texel;
-    p.size = F2{(float)g.width, (float)g.height};
+    set_gi_size(p, f);
     p.grid_camera = F3{f.camera_position[0], f.camera_position[1], f.camera_position[2]};
     p.sc = f.scene_scale;
 
     int x, y;
     if (!group_pixel(tiles, x, y)) return;
-    if (x >= g.width || y >= g.height) return;
-    uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    if (x >= p.hw || y >= p.hh) return;
+    uint32_t o = (uint32_t)y * (uint32_t)p.hw + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
     float depth = fetch_depth(p, uv);
     if (depth >= kSkyDepth) {
@@ -251,10 +249,10 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     F3 n = decode_normal(p, uv);
     F3 albedo = diffuse_albedo(p, uv);
     float px = floorf(uv.x * p.size.x), py = floorf(uv.y * p.size.y);
-    uint32_t pixel_index = (uint32_t)to_int(py) * (uint32_t)g.width + (uint32_t)to_int(px);
+    uint32_t pixel_index = (uint32_t)to_int(py) * (uint32_t)p.hw + (uint32_t)to_int(px);
     uint32_t rng = pcg((uint32_t)to_int(px) + pcg((uint32_t)to_int(py) + pcg(f.frame_index ^ 0x68e31da4u)));
 
-    uint32_t t0 = tix(p, uv);
+    uint32_t t0 = hix(p, uv);
     F4 own_sample = load_f4(res.sample, t0);
     float cweight = load_f4(res.receiver, t0).w;
     float own_id = load_f4(res.normal, t0).w;
@@ -376,8 +374,8 @@ k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_g
 int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                 const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
-    F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
+    F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
     auto kernel = tuning.gi_temporal_waves == 4 ? k_gi_temporal_w4 : k_gi_temporal;
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, out, texel, tiles);
     return check_launch("gi temporal");
@@ -387,8 +385,8 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
                const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
                hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
-    F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
+    F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
     auto kernel = tuning.gi_spatial_waves == 4 ? k_gi_spatial_w4 : k_gi_spatial;
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, res, color,
                        moments, texel, tiles);

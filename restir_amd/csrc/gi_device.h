@@ -9,19 +9,33 @@
 
 namespace rgi {
 
-// Everything a GI pass reads besides its per-pixel state.
+// Everything a GI pass reads besides its per-pixel state. The G-buffer and
+// the surface light are window-size planes; the reservoir, history and
+// colour planes are GI-size planes (u_screenInfo.xy), smaller by the GI
+// divisor (settings.h:18-23). Each is point-sampled at its own size.
 struct GiPass {
     const rgi_frame* f;      // kernel-argument frame
     SceneView v;
     rgi_gbuffer g;
     rgi_gi_inputs in;
-    F2 texel;                // 1 / GI size
-    F2 size;                 // GI size
+    F2 texel;                // 1 / GI size (u_screenInfo.zw)
+    F2 size;                 // GI size (u_screenInfo.xy)
+    int hw, hh;              // GI size as integers
     F3 grid_camera;          // u_gridCamera of this pass
     float sc;                // scene scale
 };
 
+// Texel of a G-buffer (window-size) plane and of a GI-size plane at uv.
 RGI_DEV uint32_t tix(const GiPass& p, F2 uv) { return texel_index(uv, p.g.width, p.g.height); }
+RGI_DEV uint32_t hix(const GiPass& p, F2 uv) { return texel_index(uv, p.hw, p.hh); }
+
+RGI_DEV void set_gi_size(GiPass& p, const rgi_frame& f)
+{
+    p.hw = f.gi_width;
+    p.hh = f.gi_height;
+    p.size = F2{(float)f.gi_width, (float)f.gi_height};
+    p.texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
+}
 RGI_DEV float fetch_depth(const GiPass& p, F2 uv) { return p.g.depth[tix(p, uv)]; }
 
 RGI_DEV F3 decode_normal(const GiPass& p, F2 uv)
@@ -87,8 +101,11 @@ RGI_DEV float sample_id_of(const SceneView& v, int tri, F3 dir)
     return dot(pn, dir) <= 0.0f ? (float)(tri + 1) : -(float)(tri + 1);
 }
 
-RGI_DEV F4 fetch_f4(const float* plane, const GiPass& p, F2 uv) { return load_f4(plane, tix(p, uv)); }
-RGI_DEV F4 fetch_h4(const uint16_t* plane, const GiPass& p, F2 uv) { return load_h4(plane, tix(p, uv)); }
+// GI-size planes (reservoirs, history receiver/normal/colour/moments).
+RGI_DEV F4 fetch_f4(const float* plane, const GiPass& p, F2 uv) { return load_f4(plane, hix(p, uv)); }
+RGI_DEV F4 fetch_h4(const uint16_t* plane, const GiPass& p, F2 uv) { return load_h4(plane, hix(p, uv)); }
+// The surface light (DI diffuse or the G-buffer emission): window size.
+RGI_DEV F3 fetch_light(const GiPass& p, F2 uv) { return xyz(load_h4(p.in.surface_light, tix(p, uv))); }
 
 // Whether history texel uv held the same surface as (position, n).
 RGI_DEV bool same_history_surface(const GiPass& p, F2 uv, F3 position, F3 n)
@@ -305,7 +322,7 @@ RGI_DEV bool shade_sample(const GiPass& p, F3 rpos, F3 rn, F3 ralbedo, F3 spos, 
     if (visible_surface) {
         F3 albedo = diffuse_albedo(p, suv);
         radiance = (point_light_direct(p, albedo, spos, snormal) + previous_indirect(p, spos, snormal, albedo)) +
-                   xyz(fetch_h4(p.in.surface_light, p, suv)) * f.lighting_surface;
+                   fetch_light(p, suv) * f.lighting_surface;
     } else {
         F3 albedo = tri_albedo(p.v, tri);
         radiance = (point_light_direct(p, albedo, spos, snormal) + previous_indirect(p, spos, snormal, albedo)) +
@@ -496,7 +513,7 @@ RGI_DEV float finish_pairwise(const GiPass& p, Pairwise& m, F3 position, F3 n, c
 
 RGI_DEV F2 pixel_uv(const GiPass& p, uint32_t index)
 {
-    uint32_t w = (uint32_t)p.g.width;
+    uint32_t w = (uint32_t)p.hw;
     return F2{((float)(index % w) + 0.5f) * p.texel.x, ((float)(index / w) + 0.5f) * p.texel.y};
 }
 

@@ -57,8 +57,8 @@ __global__ void __launch_bounds__(256) k_grid_insert_atomic(rgi_frame f, rgi_gbu
     block_pixel(x, y);
     uint32_t table = f.grid_table_size, mask = table - 1u;
     uint32_t slot0 = 0, checksum = 0;
-    bool active = x < g.width && y < g.height && insert_key(f, g, texel, x, y, slot0, checksum);
-    uint32_t pixel = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    bool active = x < f.gi_width && y < f.gi_height && insert_key(f, g, texel, x, y, slot0, checksum);
+    uint32_t pixel = (uint32_t)y * (uint32_t)f.gi_width + (uint32_t)x;
     uint32_t lane = lane_id();
     uint64_t pending = __ballot(active);
     while (pending) {
@@ -112,9 +112,9 @@ RGI_DEV uint32_t claim_tag(uint32_t round, uint32_t pixel) { return ((7u - round
 __global__ void __launch_bounds__(256) k_det_keys(rgi_frame f, rgi_gbuffer g, F2 texel, DetScratch d)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = (uint32_t)g.width * (uint32_t)g.height;
+    uint32_t n = (uint32_t)f.gi_width * (uint32_t)f.gi_height;
     if (i >= n) return;
-    int x = (int)(i % (uint32_t)g.width), y = (int)(i / (uint32_t)g.width);
+    int x = (int)(i % (uint32_t)f.gi_width), y = (int)(i / (uint32_t)f.gi_width);
     uint32_t slot = 0, checksum = 0;
     if (!insert_key(f, g, texel, x, y, slot, checksum)) checksum = 0;
     d.slot0[i] = slot;
@@ -289,19 +289,20 @@ int grid_clear(uint32_t* grid, uint32_t table_size, hipStream_t stream)
     return check_launch("grid clear");
 }
 
-static F2 texel_of(const rgi_gbuffer& g) { return F2{1.0f / (float)g.width, 1.0f / (float)g.height}; }
+// u_screenInfo.zw: the GI pixels' texel size; the G-buffer is read at its own size.
+static F2 texel_of(const rgi_frame& f) { return F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height}; }
 
 int grid_insert_atomic(const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid, hipStream_t stream)
 {
-    dim3 blocks((uint32_t)(g.width + 15) / 16u, (uint32_t)(g.height + 15) / 16u);
-    hipLaunchKernelGGL(k_grid_insert_atomic, blocks, dim3(256), 0, stream, f, g, texel_of(g), grid);
+    dim3 blocks((uint32_t)(f.gi_width + 15) / 16u, (uint32_t)(f.gi_height + 15) / 16u);
+    hipLaunchKernelGGL(k_grid_insert_atomic, blocks, dim3(256), 0, stream, f, g, texel_of(f), grid);
     return check_launch("grid insert");
 }
 
 int grid_insert_deterministic(rgi_context* ctx, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
                               hipStream_t stream)
 {
-    uint32_t n = (uint32_t)g.width * (uint32_t)g.height;
+    uint32_t n = (uint32_t)f.gi_width * (uint32_t)f.gi_height;
     uint32_t table = f.grid_table_size;
     uint32_t cells = table < n ? table : n;
     size_t bytes = align256((size_t)n * 4) * 6 + align256((size_t)table * 4) + align256((size_t)cells * 4) * 3 + 256;
@@ -325,7 +326,7 @@ int grid_insert_deterministic(rgi_context* ctx, const rgi_frame& f, const rgi_gb
     d.fill = take(cells);
     d.counters = take(64);
     uint32_t blocks = (n + 255u) / 256u;
-    hipLaunchKernelGGL(k_det_keys, dim3(blocks), dim3(256), 0, stream, f, g, texel_of(g), d);
+    hipLaunchKernelGGL(k_det_keys, dim3(blocks), dim3(256), 0, stream, f, g, texel_of(f), d);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, stream, d.claim, table, 0xffffffffu);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, stream, d.counters, 64u, 0u);
     hipLaunchKernelGGL(k_det_claim0, dim3(blocks), dim3(256), 0, stream, n, table - 1u, grid, d);

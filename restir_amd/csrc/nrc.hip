@@ -1122,7 +1122,8 @@ __global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_
     bool surface = depth < kSkyDepth;
     F3 target = splat3(0.0f);
     if (di_diffuse) target = target + xyz(load_h4(di_diffuse, t));
-    if (color) target = target + xyz(load_h4(color, t)) * albedo;
+    // The GI colour is a GI-size plane, point-sampled at the pixel's uv.
+    if (color) target = target + xyz(load_h4(color, texel_index(uv, f.gi_width, f.gi_height))) * albedo;
     float4* r = reinterpret_cast<float4*>(records) + (size_t)i * 3;
     r[0] = surface ? make_float4(pos.x, pos.y, pos.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     r[1] = surface ? make_float4(n.x, n.y, n.z, 0.0f) : make_float4(0.0f, 0.0f, 1.0f, 0.0f);

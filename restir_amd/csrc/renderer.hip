@@ -12,7 +12,8 @@
 #include <vector>
 
 struct rgi_renderer {
-    int32_t width = 0, height = 0;
+    int32_t width = 0, height = 0;       // window (G-buffer, DI, composite)
+    int32_t gi_width = 0, gi_height = 0; // GI planes: window / gi_divisor
     rgi_settings settings{};
     rgi_scene scene{};
     float radius = 1.0f;
@@ -201,6 +202,15 @@ void rgi_settings_default(rgi_settings* s)
     s->atrous_iterations = 5;
     s->composite = 1;
     s->specular_gi = 1;
+    s->gi_divisor = 2;
+}
+
+// The GI size of a window (renderer.cpp:519-520, 1329-1331).
+static void gi_size(const rgi_settings* s, int32_t width, int32_t height, int32_t& gi_width, int32_t& gi_height)
+{
+    int32_t d = s->gi_divisor > 0 ? s->gi_divisor : 1;
+    gi_width = width / d > 1 ? width / d : 1;
+    gi_height = height / d > 1 ? height / d : 1;
 }
 
 uint32_t rgi_settings_sizeof(void) { return (uint32_t)sizeof(rgi_settings); }
@@ -211,6 +221,8 @@ int rgi_frame_pack(const rgi_settings* s, const rgi_scene* scene, float radius, 
 {
     if (!s || !view || !prev || !out) return fail(RGI_E_INVALID, "frame_pack: NULL argument");
     if (width <= 0 || height <= 0) return fail(RGI_E_INVALID, "frame_pack: size %d x %d", width, height);
+    if (s->gi_divisor != 1 && s->gi_divisor != 2 && s->gi_divisor != 4)
+        return fail(RGI_E_INVALID, "frame_pack: gi_divisor %d is not 1, 2 or 4", s->gi_divisor);
     rgi_frame f;
     memset(&f, 0, sizeof(f));
     memcpy(f.inv_view_proj, view->inv_view_proj, sizeof(f.inv_view_proj));
@@ -231,8 +243,7 @@ int rgi_frame_pack(const rgi_settings* s, const rgi_scene* scene, float radius, 
     f.origin_bottom_left = s->origin_bottom_left;
     f.multiple_bounces = s->multiple_bounces ? 1 : 0;
     f.homogeneous_depth = s->homogeneous_depth;
-    f.gi_width = width;
-    f.gi_height = height;
+    gi_size(s, width, height, f.gi_width, f.gi_height);
     f.width = width;
     f.height = height;
     f.scene_scale = scale;
@@ -249,7 +260,7 @@ int rgi_frame_pack(const rgi_settings* s, const rgi_scene* scene, float radius, 
     // u_gridInfo (renderer.cpp:714-720): kGridMinCellSize 0.02, kGridCellPixels 6, fovY 48.
     const float pi = 3.14159265358979323846f;
     f.grid_min_cell = 0.02f * scale;
-    f.grid_pixel_size = 2.0f * tanf(0.5f * (48.0f * (pi / 180.0f))) / (float)height;
+    f.grid_pixel_size = 2.0f * tanf(0.5f * (48.0f * (pi / 180.0f))) / (float)f.gi_height;
     f.grid_cell_pixels = 6.0f;
     f.world_space_reuse = s->world_space_reuse ? 1 : 0;
     f.grid_table_size = s->grid_table_size;
@@ -289,10 +300,13 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         return fail(RGI_E_INVALID, "atrous_iterations %d is not in 0..5", settings->atrous_iterations);
     if (settings->shadows && shadow_face_size(settings) == 0)
         return fail(RGI_E_INVALID, "shadow_face_size index %d is not in 0..3", settings->shadow_face_size);
+    if (settings->gi_divisor != 1 && settings->gi_divisor != 2 && settings->gi_divisor != 4)
+        return fail(RGI_E_INVALID, "gi_divisor %d is not 1, 2 or 4", settings->gi_divisor);
     rgi_renderer* r = new (std::nothrow) rgi_renderer();
     if (!r) return fail(RGI_E_NOMEM, "renderer allocation failed");
     r->width = width;
     r->height = height;
+    gi_size(settings, width, height, r->gi_width, r->gi_height);
     r->settings = *settings;
     r->scene = *scene;
     r->radius = radius;
@@ -301,18 +315,19 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         delete r;
         return e;
     }
-    size_t n = (size_t)width * height;
+    size_t n = (size_t)width * height;                    // window-size planes
+    size_t ng = (size_t)r->gi_width * (size_t)r->gi_height; // GI-size planes
     bool ok = true;
     for (int k = 0; k < 2 && ok; ++k) {
-        for (int j = 0; j < 3; ++j) ok = ok && alloc(r->gi[k][j], 4 * n);
-        ok = ok && alloc(r->color[k], 4 * n) && alloc(r->moments[k], 4 * n) && alloc(r->grid[k], (size_t)34 * T);
+        for (int j = 0; j < 3; ++j) ok = ok && alloc(r->gi[k][j], 4 * ng);
+        ok = ok && alloc(r->color[k], 4 * ng) && alloc(r->moments[k], 4 * ng) && alloc(r->grid[k], (size_t)34 * T);
         ok = ok && alloc(r->di_sample[k], 4 * n) && alloc(r->di_receiver[k], 4 * n) && alloc(r->di_normal[k], 4 * n);
     }
     ok = ok && alloc(r->diffuse, 4 * n) && alloc(r->specular, 4 * n);
-    for (int k = 0; k < 2 && ok; ++k) ok = alloc(r->filtered[k], 4 * n) && alloc(r->filter[k], 4 * n);
+    for (int k = 0; k < 2 && ok; ++k) ok = alloc(r->filtered[k], 4 * ng) && alloc(r->filter[k], 4 * ng);
     if (settings->composite) ok = ok && alloc(r->hdr, 4 * n);
     if (settings->restir_di) ok = ok && alloc(r->black, 4 * n);
-    if (settings->specular_gi) ok = ok && alloc(r->specular_gi[0], 4 * n) && alloc(r->specular_gi[1], 4 * n);
+    if (settings->specular_gi) ok = ok && alloc(r->specular_gi[0], 4 * ng) && alloc(r->specular_gi[1], 4 * ng);
     if (settings->shadows) {
         size_t F = (size_t)shadow_face_size(settings);
         ok = ok && alloc(r->atlas, 6 * F * F);

@@ -76,7 +76,7 @@ RGI_DEV F3 hit_radiance(const GiPass& p, const uint16_t* previous_hdr, F3 hit, f
         F3 n = decode_normal(p, uv);
         F3 albedo = diffuse_albedo(p, uv);
         return (point_light_direct(p, albedo, hit, n) + previous_indirect(p, hit, n, albedo)) +
-               xyz(fetch_h4(p.in.surface_light, p, uv)) * f.lighting_surface;
+               fetch_light(p, uv) * f.lighting_surface;
     }
     int tri = to_int(fabsf(id) + 0.5f) - 1;
     if (tri < 0 || !bvh_ready(p.v)) return splat3(0.0f);
@@ -99,22 +99,21 @@ __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_
     p.v = scene_view(s);
     p.g = g;
     p.in = in;
-    p.texel = texel;
-    p.size = F2{(float)g.width, (float)g.height};
+    set_gi_size(p, f);
     p.grid_camera = F3{f.camera_position[0], f.camera_position[1], f.camera_position[2]};
     p.sc = f.scene_scale;
 
     int x, y;
     if (!group_pixel(tiles, x, y)) return;
     uint32_t lane = threadIdx.x & 63u, qbase = threadIdx.x & ~63u;
-    bool inside = x < g.width && y < g.height;
+    bool inside = x < p.hw && y < p.hh;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
     float depth = inside ? fetch_depth(p, uv) : 1.0f;
     bool working = inside && depth < kSkyDepth;
     // A wave with no surface pixel writes zeros and leaves (the rest of the
     // pass is wave-local).
     if (__ballot(working) == 0ull) {
-        if (inside) store_h4(out, (uint32_t)y * (uint32_t)g.width + (uint32_t)x, F4{0.0f, 0.0f, 0.0f, 0.0f});
+        if (inside) store_h4(out, (uint32_t)y * (uint32_t)p.hw + (uint32_t)x, F4{0.0f, 0.0f, 0.0f, 0.0f});
         return;
     }
     F3 position = reconstruct_position(f, uv, depth);
@@ -185,7 +184,7 @@ __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_
                    smith_g1_ggx(normal_light, alpha);
     }
     if (!inside) return;
-    uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
+    uint32_t o = (uint32_t)y * (uint32_t)p.hw + (uint32_t)x;
     if (!working) {
         store_h4(out, o, F4{0.0f, 0.0f, 0.0f, 0.0f});
         return;
@@ -218,8 +217,8 @@ int specular_gi(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
                 const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out, const Tuning& tuning,
                 hipStream_t stream)
 {
-    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
-    F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
+    F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
     hipLaunchKernelGGL(k_specular, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, previous_hdr, history, out,
                        texel, tiles);
     return check_launch("specular gi");

@@ -135,10 +135,13 @@ __global__ void __launch_bounds__(256) k_svgf_atrous(SvgfPlanes t, float scale, 
 
 SvgfPlanes planes_of(const rgi_frame& f)
 {
-    return SvgfPlanes{f.width, f.height, F2{1.0f / (float)f.width, 1.0f / (float)f.height}};
+    return SvgfPlanes{f.gi_width, f.gi_height, F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height}};
 }
 
-dim3 grid_of(const rgi_frame& f) { return dim3((uint32_t)(f.width + 15) / 16u, (uint32_t)(f.height + 15) / 16u); }
+dim3 grid_of(const rgi_frame& f)
+{
+    return dim3((uint32_t)(f.gi_width + 15) / 16u, (uint32_t)(f.gi_height + 15) / 16u);
+}
 
 } // namespace
 

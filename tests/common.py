@@ -31,16 +31,23 @@ def frame_for(hs, settings, width, height, frame_index, history_valid, di_histor
                          di_history_valid)
 
 
-def empty_history(width, height, table_size):
+def gi_size(width, height, divisor):
+    """The GI size of a window (renderer.cpp:519-520)."""
+    return max(1, width // divisor), max(1, height // divisor)
+
+
+def empty_history(width, height, table_size, divisor=1):
     n = width * height
+    gw, gh = gi_size(width, height, divisor)
+    ng = gw * gh
     return {
-        "gi_sample": np.zeros((n, 4), np.float32),
-        "gi_receiver": np.zeros((n, 4), np.float32),
-        "gi_normal": np.zeros((n, 4), np.float32),
-        "color": np.zeros((n, 4), np.uint16),
-        "moments": np.zeros((n, 4), np.uint16),
-        "filtered": np.zeros((n, 4), np.uint16),
-        "specular_gi": np.zeros((n, 4), np.uint16),
+        "gi_sample": np.zeros((ng, 4), np.float32),
+        "gi_receiver": np.zeros((ng, 4), np.float32),
+        "gi_normal": np.zeros((ng, 4), np.float32),
+        "color": np.zeros((ng, 4), np.uint16),
+        "moments": np.zeros((ng, 4), np.uint16),
+        "filtered": np.zeros((ng, 4), np.uint16),
+        "specular_gi": np.zeros((ng, 4), np.uint16),
         "hdr": np.zeros((n, 4), np.uint16),
         "grid": np.zeros(34 * table_size, np.uint32),
         "di_sample": np.zeros((n, 4), np.float32),
@@ -90,7 +97,7 @@ def oracle_denoise(frame, out, atrous_iterations):
     """The denoiser of Renderer::render (renderer.cpp:857-907) on the oracle:
     returns (filtered history, denoised)."""
     L = orc.lib()
-    n = frame.width * frame.height
+    n = frame.gi_width * frame.gi_height
     targets = [np.zeros((n, 4), np.uint16), np.zeros((n, 4), np.uint16)]
     filtered = None
     rec, nor = out["gi_receiver"], out["gi_normal"]
@@ -149,7 +156,8 @@ def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterati
     under its function name."""
     L = _Timed(orc.lib(), times)
     w, h, T = frame.width, frame.height, frame.grid_table_size
-    n = w * h
+    n = w * h                              # window-size planes: G-buffer, DI, composite
+    ng = frame.gi_width * frame.gi_height  # GI-size planes
     scene = orc.host_scene(hs)
     gb = host_gbuffer(g, w, h)
     out = {}
@@ -184,13 +192,13 @@ def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterati
                       history_color.ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, shadow_ptr,
                       hist["grid"].ctypes.data)
     for k in GI_PLANES_F32:
-        out["gi_" + k] = np.zeros((n, 4), np.float32)
+        out["gi_" + k] = np.zeros((ng, 4), np.float32)
     gres = ra.GiReservoirs(out["gi_sample"].ctypes.data, out["gi_receiver"].ctypes.data, out["gi_normal"].ctypes.data)
     L.orc_gi_temporal(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(gin), ctypes.byref(gres))
     sin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data, history_color.ctypes.data,
                       hist["moments"].ctypes.data, surface_light.ctypes.data, shadow_ptr, grid.ctypes.data)
-    out["color"] = np.zeros((n, 4), np.uint16)
-    out["moments"] = np.zeros((n, 4), np.uint16)
+    out["color"] = np.zeros((ng, 4), np.uint16)
+    out["moments"] = np.zeros((ng, 4), np.uint16)
     L.orc_gi_spatial(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(sin), ctypes.byref(gres),
                      orc.ptr(out["color"]), orc.ptr(out["moments"]))
     out["surface_light"] = surface_light
@@ -201,7 +209,7 @@ def oracle_frame(hs, frame, g, hist, insert=True, denoiser=False, atrous_iterati
         spin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
                            history_color.ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data,
                            shadow_ptr, grid.ctypes.data)
-        out["specular_gi"] = np.zeros((n, 4), np.uint16)
+        out["specular_gi"] = np.zeros((ng, 4), np.uint16)
         prev_hdr = hist["hdr"] if composite else None
         L.orc_specular_gi(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(spin),
                           prev_hdr.ctypes.data if prev_hdr is not None else None, hist["specular_gi"].ctypes.data,
@@ -233,7 +241,7 @@ def next_history(hist, out):
 
 def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0, distance=3.17):
     """Runs `frames` frames on the oracle; returns (frames, gbuffers, histories, outputs)."""
-    hist = empty_history(width, height, settings.grid_table_size)
+    hist = empty_history(width, height, settings.grid_table_size, settings.gi_divisor)
     seq = []
     di_on = settings.restir_di and hs.light_count > 0
     for i in range(frames):
@@ -302,3 +310,44 @@ def grid_view(grid, table):
     listed = np.minimum(cnt, 32)
     entries[np.arange(32)[None, :] >= listed[:, None]] = 0
     return cs, cnt, entries
+
+
+def replicate(plane, width, height, k):
+    """A width x height plane (rows of texels, any trailing shape) with every
+    texel repeated k x k times: the same image at k times the resolution."""
+    a = plane.reshape((height, width) + plane.shape[1:])
+    a = np.repeat(np.repeat(a, k, axis=0), k, axis=1)
+    return np.ascontiguousarray(a.reshape((height * k * width * k,) + plane.shape[1:]))
+
+
+def gi_passes_oracle(hs, frame, g, hist, surface_light, previous_hdr):
+    """Grid insert, GI temporal, GI spatial, indirect specular and the
+    denoiser's first two steps on the oracle for one frame; returns the
+    GI-size outputs."""
+    L = orc.lib()
+    w, h, T = frame.width, frame.height, frame.grid_table_size
+    ng = frame.gi_width * frame.gi_height
+    scene = orc.host_scene(hs)
+    gb = host_gbuffer(g, w, h)
+    out = {"grid": np.zeros(34 * T, np.uint32)}
+    L.orc_grid_clear(orc.ptr(out["grid"]), T)
+    L.orc_grid_insert(ctypes.byref(frame), ctypes.byref(gb), orc.ptr(out["grid"]))
+    gin = ra.GiInputs(hist["gi_sample"].ctypes.data, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
+                      hist["filtered"].ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, None,
+                      hist["grid"].ctypes.data)
+    for k in GI_PLANES_F32:
+        out["gi_" + k] = np.zeros((ng, 4), np.float32)
+    gres = ra.GiReservoirs(out["gi_sample"].ctypes.data, out["gi_receiver"].ctypes.data, out["gi_normal"].ctypes.data)
+    L.orc_gi_temporal(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(gin), ctypes.byref(gres))
+    sin = ra.GiInputs(None, hist["gi_receiver"].ctypes.data, hist["gi_normal"].ctypes.data,
+                      hist["filtered"].ctypes.data, hist["moments"].ctypes.data, surface_light.ctypes.data, None,
+                      out["grid"].ctypes.data)
+    out["color"] = np.zeros((ng, 4), np.uint16)
+    out["moments"] = np.zeros((ng, 4), np.uint16)
+    L.orc_gi_spatial(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(sin), ctypes.byref(gres),
+                     orc.ptr(out["color"]), orc.ptr(out["moments"]))
+    out["specular_gi"] = np.zeros((ng, 4), np.uint16)
+    L.orc_specular_gi(ctypes.byref(frame), ctypes.byref(scene), ctypes.byref(gb), ctypes.byref(sin),
+                      previous_hdr.ctypes.data, hist["specular_gi"].ctypes.data, orc.ptr(out["specular_gi"]))
+    out["filtered"], out["denoised"] = oracle_denoise(frame, out, 2)
+    return out

@@ -58,16 +58,19 @@ def atrium():
     (3, {"ray_mode": ra.RGI_RAY_BVH}, {}, "cornell"), (4, {}, ALL_VARIANTS, "cornell"),
     (3, {"shadows": True, "light_color": (1.0, 0.9, 0.8), "light_position": (0.1, 1.2, -0.2),
          "shadow_face_size": 0}, {}, "cornell"),
-    (3, {}, ALL_VARIANTS, "atrium")])
+    (3, {}, ALL_VARIANTS, "atrium"),
+    (4, {"gi_divisor": 2}, {}, "cornell"), (3, {"gi_divisor": 4}, ALL_VARIANTS, "cornell"),
+    (3, {"gi_divisor": 2}, ALL_VARIANTS, "atrium")])
 def test_renderer_matches_oracle_sequence(torch, frames, kw, variants, scene):
     run_renderer_case(torch, frames, kw, variants, scene, W, H)
 
 
-@pytest.mark.parametrize("width,height", [(97, 61), (7, 5)])
-def test_renderer_matches_oracle_at_ragged_sizes(torch, width, height):
+@pytest.mark.parametrize("width,height,divisor", [(97, 61, 1), (7, 5, 1), (97, 61, 2), (7, 5, 4)])
+def test_renderer_matches_oracle_at_ragged_sizes(torch, width, height, divisor):
     """Sizes that are no multiple of the 16 x 16 blocks, the 4 x 8 groups or
-    the 8 x 8 insert tiles: every pass's edge handling, all launch variants."""
-    run_renderer_case(torch, 3, {}, ALL_VARIANTS, "cornell", width, height)
+    the 8 x 8 insert tiles: every pass's edge handling, all launch variants;
+    at divisors 2 and 4 also GI sizes that are floors (48 x 30, 1 x 1)."""
+    run_renderer_case(torch, 3, {"gi_divisor": divisor}, ALL_VARIANTS, "cornell", width, height)
 
 
 def run_renderer_case(torch, frames, kw, variants, scene, W, H):
