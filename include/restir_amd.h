@@ -214,6 +214,22 @@ int rgi_grid_clear(uint32_t* grid, uint32_t table_size, void* stream);
 int rgi_grid_insert(rgi_context* context, const rgi_frame* frame, const rgi_gbuffer* gbuffer,
                     uint32_t* grid, int mode, void* stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). rgi_grid_insert in RGI_GRID_DETERMINISTIC mode, the
+ * same kernels and the same result, with its scratch in a fixed device array
+ * of the library (RGI_NOALLOC_SCRATCH_BYTES, 88 MiB plus 256 bytes, held by
+ * each GPU the library runs on until the process ends) instead of a
+ * context's: it needs no context and calls no allocator. GI sizes up to
+ * RGI_NOALLOC_MAX_PIXELS pixels (1920 x 1080 fits) and tables up to
+ * RGI_NOALLOC_MAX_TABLE cells; larger inputs return RGI_E_INVALID. Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialized by the caller.
+ * The first call on a device looks up the array's address on it. */
+#define RGI_NOALLOC_MAX_PIXELS (1u << 21)
+#define RGI_NOALLOC_MAX_TABLE (1u << 22)
+int rgi_grid_insert_noalloc(const rgi_frame* frame, const rgi_gbuffer* gbuffer, uint32_t* grid, void* stream);
+
 /* cs_restir_temporal (shaders/cs_restir_temporal.sc:57-297, dispatched at
  * renderer.cpp:807-817): fresh candidates, temporal reuse through last
  * frame's grid (inputs->grid), boiling filter; writes this frame's GI

@@ -171,6 +171,7 @@ def lib():
     L.rgi_destroy.restype = None
     L.rgi_grid_clear.argtypes = [P, U32, P]
     L.rgi_grid_insert.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, ctypes.c_int, P]
+    L.rgi_grid_insert_noalloc.argtypes = [ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, P]
     L.rgi_gi_temporal.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), ctypes.POINTER(GBuffer),
                                   ctypes.POINTER(GiInputs), ctypes.POINTER(GiReservoirs), P]
     L.rgi_gi_spatial.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(Scene), ctypes.POINTER(GBuffer),

@@ -4,11 +4,47 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 
 #include "internal.h"
 
+// rgi_grid_insert_noalloc's scratch: the deterministic insert's scratch at
+// its largest admitted inputs, one copy per GPU. In host code this name is a
+// host-side handle, not a device address: kernels and copies get the address
+// looked up per device by noalloc_scratch() below.
+#define RGI_NOALLOC_SCRATCH_BYTES (88ull * 1024 * 1024 + 256)
+static __device__ __align__(256) unsigned char scratch[RGI_NOALLOC_SCRATCH_BYTES];
+
 namespace rgi {
+
+static_assert(RGI_NOALLOC_SCRATCH_BYTES == 6 * 4 * (size_t)RGI_NOALLOC_MAX_PIXELS + 4 * (size_t)RGI_NOALLOC_MAX_TABLE +
+                                                3 * 4 * (size_t)RGI_NOALLOC_MAX_PIXELS + 256,
+              "the no-alloc scratch holds the deterministic insert's scratch at the largest admitted inputs");
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+static void* noalloc_scratch(int& error)
+{
+    static std::mutex mu;
+    static void* address[64] = {};
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) {
+        error = fail(RGI_E_DEVICE, "grid_insert_noalloc: no current HIP device (or device index >= 64)");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!address[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+        if (e != hipSuccess || !p) {
+            error = fail(RGI_E_DEVICE, "grid_insert_noalloc: scratch address lookup failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        address[device] = p;
+    }
+    return address[device];
+}
 
 static thread_local char g_error[512] = "";
 
@@ -28,7 +64,7 @@ int check_launch(const char* what)
     return RGI_OK;
 }
 
-void* scratch(rgi_context* ctx, size_t bytes)
+void* context_scratch(rgi_context* ctx, size_t bytes)
 {
     if (ctx->scratch_bytes >= bytes) return ctx->scratch;
     if (ctx->scratch) {
@@ -222,6 +258,26 @@ int rgi_grid_insert(rgi_context* ctx, const rgi_frame* frame, const rgi_gbuffer*
         return grid_insert_deterministic(ctx, *frame, *g, grid, as_stream(stream));
     }
     return fail(RGI_E_INVALID, "unknown grid insert mode %d", mode);
+}
+
+int rgi_grid_insert_noalloc(const rgi_frame* frame, const rgi_gbuffer* g, uint32_t* grid, void* stream)
+{
+    int e;
+    if ((e = check_frame(frame)) != RGI_OK) return e;
+    if ((e = check_gbuffer(g, false, false)) != RGI_OK) return e;
+    if ((e = check_gi_gbuffer(frame, g)) != RGI_OK) return e;
+    if (!grid) return fail(RGI_E_INVALID, "grid is NULL");
+    uint64_t pixels = (uint64_t)frame->gi_width * (uint64_t)frame->gi_height;
+    if (pixels > RGI_NOALLOC_MAX_PIXELS || frame->grid_table_size > RGI_NOALLOC_MAX_TABLE)
+        return fail(RGI_E_INVALID,
+                    "grid_insert_noalloc: %llu GI pixels and a table of %u cells exceed its fixed scratch (at most %u "
+                    "pixels and %u cells); use rgi_grid_insert",
+                    (unsigned long long)pixels, frame->grid_table_size, RGI_NOALLOC_MAX_PIXELS, RGI_NOALLOC_MAX_TABLE);
+    if (det_scratch_bytes((uint32_t)pixels, frame->grid_table_size) > RGI_NOALLOC_SCRATCH_BYTES)
+        return fail(RGI_E_INVALID, "grid_insert_noalloc: scratch too small");
+    void* base = noalloc_scratch(e);
+    if (!base) return e;
+    return grid_insert_deterministic_in(base, *frame, *g, grid, as_stream(stream));
 }
 
 int rgi_gi_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,

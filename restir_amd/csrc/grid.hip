@@ -299,15 +299,28 @@ int grid_insert_atomic(const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
     return check_launch("grid insert");
 }
 
+size_t det_scratch_bytes(uint32_t pixels, uint32_t table)
+{
+    uint32_t cells = table < pixels ? table : pixels;
+    return align256((size_t)pixels * 4) * 6 + align256((size_t)table * 4) + align256((size_t)cells * 4) * 3 + 256;
+}
+
 int grid_insert_deterministic(rgi_context* ctx, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
                               hipStream_t stream)
 {
     uint32_t n = (uint32_t)f.gi_width * (uint32_t)f.gi_height;
+    char* base = static_cast<char*>(context_scratch(ctx, det_scratch_bytes(n, f.grid_table_size)));
+    if (!base) return RGI_E_NOMEM;
+    return grid_insert_deterministic_in(base, f, g, grid, stream);
+}
+
+int grid_insert_deterministic_in(void* scratch_base, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
+                                 hipStream_t stream)
+{
+    uint32_t n = (uint32_t)f.gi_width * (uint32_t)f.gi_height;
     uint32_t table = f.grid_table_size;
     uint32_t cells = table < n ? table : n;
-    size_t bytes = align256((size_t)n * 4) * 6 + align256((size_t)table * 4) + align256((size_t)cells * 4) * 3 + 256;
-    char* base = static_cast<char*>(scratch(ctx, bytes));
-    if (!base) return RGI_E_NOMEM;
+    char* base = static_cast<char*>(scratch_base);
     DetScratch d;
     auto take = [&](size_t count) {
         uint32_t* p = reinterpret_cast<uint32_t*>(base);

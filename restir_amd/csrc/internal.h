@@ -34,7 +34,7 @@ namespace rgi {
 int fail(int code, const char* fmt, ...);
 int check_launch(const char* what);
 // Scratch of at least `bytes`, grown (after a device sync) when too small.
-void* scratch(rgi_context* ctx, size_t bytes);
+void* context_scratch(rgi_context* ctx, size_t bytes);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -44,6 +44,12 @@ int grid_clear(uint32_t* grid, uint32_t table_size, hipStream_t stream);
 int grid_insert_atomic(const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid, hipStream_t stream);
 int grid_insert_deterministic(rgi_context* ctx, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
                               hipStream_t stream);
+// The deterministic insert's scratch: its size for a GI-pixel count and a
+// table size, and the insert over caller-provided scratch of that size (every
+// word of it is written by the insert's own kernels before it is read).
+size_t det_scratch_bytes(uint32_t pixels, uint32_t table);
+int grid_insert_deterministic_in(void* scratch_base, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
+                                 hipStream_t stream);
 int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* hit, float* dist, int any_hit,
                const Tuning& tuning, uint32_t* counter, hipStream_t stream);
 int gbuffer_primary(const rgi_frame& f, const rgi_scene& s, const float* materials, uint32_t material_count,

@@ -129,7 +129,7 @@ int error(rgi_context* ctx, const rgi_gbuffer* g, const uint16_t* color, const u
     if (g->width <= 0 || g->height <= 0) return fail(RGI_E_INVALID, "reference_error: empty frame");
     uint32_t n = (uint32_t)g->width * (uint32_t)g->height;
     uint32_t blocks = (n + kBlock - 1) / kBlock;
-    double* partial = static_cast<double*>(scratch(ctx, (size_t)blocks * kTerms * sizeof(double)));
+    double* partial = static_cast<double*>(context_scratch(ctx, (size_t)blocks * kTerms * sizeof(double)));
     if (!partial) return RGI_E_NOMEM;
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_ref_error, dim3(blocks), dim3(kBlock), 0, s, *g, color, dd, ds, composited, accum,
