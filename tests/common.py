@@ -241,18 +241,28 @@ def next_history(hist, out):
 
 def oracle_sequence(hs, settings, width, height, frames, yaw_step=0.0, distance=3.17):
     """Runs `frames` frames on the oracle; returns (frames, gbuffers, histories, outputs)."""
+    return list(oracle_frames(hs, settings, width, height, frames, yaw_step=yaw_step, distance=distance))
+
+
+def camera(hs, i, width, height, yaw_step=0.0, distance=3.17, yaw_of=None):
+    """Frame i's camera of a path: a constant yaw step, or yaw_of(i)."""
+    return ra.orbit_camera(hs, i, width, height, yaw_step=yaw_step, distance=distance, yaw_of=yaw_of)
+
+
+def oracle_frames(hs, settings, width, height, frames, yaw_step=0.0, distance=3.17, yaw_of=None):
+    """oracle_sequence as a generator (each frame's arrays can be dropped
+    once compared): yields (frame, gbuffer, history, outputs)."""
     hist = empty_history(width, height, settings.grid_table_size, settings.gi_divisor)
-    seq = []
     di_on = settings.restir_di and hs.light_count > 0
     for i in range(frames):
-        f = frame_for(hs, settings, width, height, i, history_valid=i > 0, di_history_valid=i > 0 and di_on,
-                      yaw_step=yaw_step, distance=distance)
+        eye, vp, inv = camera(hs, i, width, height, yaw_step, distance, yaw_of)
+        peye, pvp, _ = camera(hs, i - 1, width, height, yaw_step, distance, yaw_of) if i > 0 else (eye, vp, inv)
+        f = ra.make_frame(settings, hs, i, eye, vp, inv, peye, pvp, width, height, i > 0, i > 0 and di_on)
         g = oracle_gbuffer(hs, f)
         out = oracle_frame(hs, f, g, hist, denoiser=settings.denoiser, atrous_iterations=settings.atrous_iterations,
                            composite=settings.composite, specular=settings.specular_gi)
-        seq.append((f, g, hist, out))
+        yield f, g, hist, out
         hist = next_history(hist, out)
-    return seq
 
 
 # ---------------------------------------------------------------- device
