@@ -167,6 +167,24 @@ uint32_t checksum_hash(uint32_t value)
     return value;
 }
 
+// ------------------------------------------------------------- planted mistakes
+
+// Test infrastructure for the oracle's own pins: orc_plant(k) makes the
+// oracle commit one plausible mistake (0: none), so that a pin can show it
+// fails on it. Never set outside those tests.
+enum Plant {
+    kPlantNone = 0,
+    kPlantNoJacobian = 1,    // reconnectionJacobian taken as 1
+    kPlantMisHere = 2,       // a neighbour's pairwise density evaluated at this receiver
+    kPlantPdfAtOrigin = 3,   // candidate pdf left at the ray origin (no conversion to the receiver)
+    kPlantThickness = 4,     // march thickness ten times the reference's
+    kPlantBisection = 5,     // two bisection rounds instead of five
+    kPlantDiTemporalMis = 6, // DI temporal: canonical sample without its MIS weight
+    kPlantDiSpatialMis = 7,  // DI spatial: canonical takes every pair's whole share
+    kPlantSkipShare = 8,     // a skipped GI tap does not hand its pair share to the canonical sample
+};
+static int g_plant = kPlantNone;
+
 // ------------------------------------------------------------- pass context
 
 struct Pass {
@@ -827,7 +845,7 @@ bool trace_screen_space_ray(const Pass& p, V3 origin, V3 direction, float max_di
         if (!in_front && previous_in_front) {
             float near_d = previous_distance;
             float far_d = ray_distance;
-            for (int r = 0; r < 5; ++r) {
+            for (int r = 0; r < (g_plant == kPlantBisection ? 2 : 5); ++r) {
                 float middle = 0.5f * (near_d + far_d);
                 V2 muv;
                 V3 msurf;
@@ -842,6 +860,7 @@ bool trace_screen_space_ray(const Pass& p, V3 origin, V3 direction, float max_di
                 }
             }
             float thickness = 0.03f * sc + 0.03f * far_d;
+            if (g_plant == kPlantThickness) thickness = 0.3f * sc + 0.3f * far_d;
             if (behind < thickness) {
                 hit_position = surface;
                 return true;
@@ -956,6 +975,7 @@ float reconnection_jacobian(const Pass& p, V3 here, V3 there, V3 spos, V3 sn, fl
     float ct = dot(sn, to_there) * inversesqrt(ts);
     if (ct <= 0.001f) return 0.0f;
     float j = fmax_((ch / ct) * (ts / hs), 0.0f);
+    if (g_plant == kPlantNoJacobian) j = 1.0f;
     float clamp_value = p.f->jacobian_clamp;
     return clamp_value > 0.0f ? fmin_(j, clamp_value) : j;
 }
@@ -1045,7 +1065,21 @@ Mis begin_mis(const Pass& p, V3 position, V3 n, const Reservoir& c, float ccount
     return m;
 }
 
-void skip_neighbour(Mis& m) { m.canonical_share += m.pair_canonical; }
+void skip_neighbour(Mis& m)
+{
+    if (g_plant != kPlantSkipShare) m.canonical_share += m.pair_canonical;
+}
+
+// A neighbour's pairwise MIS weight (restir.sh:886-892), times M_total, of
+// its sample with normal `sn` shifted to this receiver.
+float neighbour_mis(const Pass& p, const Mis& m, V3 position, V3 n, const Neighbour& nb, V3 sn, float pair_weight)
+{
+    float hg = geometry_term(p, position, n, nb.sample_position, sn, nb.sample_id);
+    float tg = geometry_term(p, nb.receiver_position, nb.receiver_normal, nb.sample_position, sn, nb.sample_id);
+    float nd = nb.count * (g_plant == kPlantMisHere ? hg : tg);
+    float den = nd + m.pair_canonical * hg;
+    return den > 0.0f ? (pair_weight * nd) / den : 0.0f;
+}
 
 void add_neighbour(const Pass& p, Mis& m, V3 position, V3 n, V3 albedo, const Neighbour& nb, float random)
 {
@@ -1071,11 +1105,7 @@ void add_neighbour(const Pass& p, Mis& m, V3 position, V3 n, V3 albedo, const Ne
     if (shifted) {
         float target = luminance(contribution);
         float j = reconnection_jacobian(p, position, nb.receiver_position, nb.sample_position, sn, nb.sample_id);
-        float hg = geometry_term(p, position, n, nb.sample_position, sn, nb.sample_id);
-        float tg = geometry_term(p, nb.receiver_position, nb.receiver_normal, nb.sample_position, sn, nb.sample_id);
-        float nd = ncount * tg;
-        float den = nd + m.pair_canonical * hg;
-        float mis = den > 0.0f ? (pair_weight * nd) / den : 0.0f;
+        float mis = neighbour_mis(p, m, position, n, nb, sn, pair_weight);
         float w = ((mis * target) * nb.weight) * j;
         if (w > 0.0f && update_reservoir(m.combined, contribution, nb.sample_position, sn, nb.sample_id, target, w, random))
             m.neighbour_selected = true;
@@ -1276,6 +1306,7 @@ void temporal_thread(const Pass& p, int x, int y, TemporalThread& t)
                 float hco = fmax_(-dot(hit_normal, from_origin), 0.0f) * inversesqrt(os);
                 float hcr = -dot(hit_normal, from_receiver) * inversesqrt(rs);
                 source_pdf = hcr > 0.0001f ? (source_pdf * (hco / os)) * (rs / hcr) : 0.0f;
+                if (g_plant == kPlantPdfAtOrigin) source_pdf = dot(t.normal, dir) / kPi;
             }
             if (target > 0.0f && source_pdf > 0.0001f)
                 update_reservoir(canonical, contribution, hit_position, hit_normal, hit_id, target, target / source_pdf, rsel);
@@ -1538,6 +1569,75 @@ void parallel_for(int64_t n, F&& fn)
 extern "C" {
 
 void orc_set_threads(int threads) { thread_count = threads; }
+void orc_plant(int mistake) { g_plant = mistake; }
+
+// traceScreenSpaceRay (restir.sh:364-436) of one ray over a G-buffer, for
+// the march's pins: returns whether it hit, and the hit surface point.
+int orc_screen_march(const rgi_frame* f, const rgi_gbuffer* g, const float* origin3, const float* dir3,
+                     float max_distance, float* hit3)
+{
+    Pass p = make_pass(f, nullptr, g, nullptr, false);
+    V3 hit;
+    bool h = trace_screen_space_ray(p, V3{origin3[0], origin3[1], origin3[2]}, V3{dir3[0], dir3[1], dir3[2]},
+                                    max_distance, hit);
+    hit3[0] = hit.x;
+    hit3[1] = hit.y;
+    hit3[2] = hit.z;
+    return h ? 1 : 0;
+}
+
+// reconnectionJacobian (restir.sh:623-649) of a sample point (position,
+// normal, not the sky) shifted from receiver `there` to receiver `here`.
+float orc_reconnection_jacobian(const rgi_frame* f, const float* here3, const float* there3, const float* sample3,
+                                const float* normal3)
+{
+    Pass p = make_pass(f, nullptr, nullptr, nullptr, false);
+    return reconnection_jacobian(p, V3{here3[0], here3[1], here3[2]}, V3{there3[0], there3[1], there3[2]},
+                                 V3{sample3[0], sample3[1], sample3[2]}, V3{normal3[0], normal3[1], normal3[2]}, 1.0f);
+}
+
+// The pairwise MIS of restir.sh:789-931 over one sample point x (position,
+// normal) that the canonical reservoir and `taps` neighbour reservoirs all
+// hold: begin_mis, then per neighbour add_neighbour (a negative count marks
+// a tap that found no reservoir: skip_neighbour), with every visibility test
+// passing (`scene` must see nothing between the points). Writes the
+// canonical share m_c(x) M_total, each neighbour's m_i(x) M_total (0 for a
+// skipped tap) and M_total. Neighbours: `taps` rows of 7 floats, receiver
+// position, receiver normal, count.
+void orc_gi_pairwise_weights(const rgi_frame* f, const rgi_scene* s, const float* receiver6, float canonical_count,
+                             const float* sample6, int taps, const float* neighbours, float* out_canonical_share,
+                             float* out_neighbour_mis, float* out_total)
+{
+    Pass p = make_pass(f, s, nullptr, nullptr, false);
+    V3 position = V3{receiver6[0], receiver6[1], receiver6[2]}, n = V3{receiver6[3], receiver6[4], receiver6[5]};
+    Reservoir c;
+    c.position = V3{sample6[0], sample6[1], sample6[2]};
+    c.normal = V3{sample6[3], sample6[4], sample6[5]};
+    c.target = 1.0f;
+    c.id = 1.0f;
+    Mis m = begin_mis(p, position, n, c, canonical_count, taps);
+    for (int i = 0; i < taps; ++i) {
+        const float* r = neighbours + (size_t)i * 7;
+        out_neighbour_mis[i] = 0.0f;
+        if (r[6] < 0.0f) {
+            skip_neighbour(m);
+            continue;
+        }
+        // The neighbour's MIS weight of x as add_neighbour forms it (its
+        // shift to this receiver is replaced by x itself).
+        Neighbour nb;
+        nb.sample_position = c.position;
+        nb.sample_id = c.id;
+        nb.receiver_position = V3{r[0], r[1], r[2]};
+        nb.receiver_normal = V3{r[3], r[4], r[5]};
+        nb.count = r[6];
+        nb.weight = 0.0f; // add_neighbour then only updates the canonical share and M_total
+        add_neighbour(p, m, position, n, v3s(0.0f), nb, 0.5f);
+        out_neighbour_mis[i] = neighbour_mis(p, m, position, n, nb, c.normal, nb.count + m.pair_canonical);
+    }
+    *out_canonical_share = m.canonical_share;
+    *out_total = m.total_count;
+}
 int orc_get_threads(void)
 {
 #ifdef _OPENMP
@@ -2222,6 +2322,7 @@ void orc_di_temporal(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* 
             }
             float den = ccount * ctarget + tcount * ptarget;
             float w = (((ccount * ctarget) / den) * ctarget) * cweight;
+            if (g_plant == kPlantDiTemporalMis) w = ctarget * cweight;
             wsum += w;
             chosen_point = cpoint;
             chosen_light = clight;
@@ -2316,7 +2417,7 @@ void orc_di_spatial(const rgi_frame* f, const rgi_scene* s, const rgi_gbuffer* g
                 if (light_visible(p, nb, cpoint)) ntarget = light_target(p, nb, clight, cpoint);
                 float cd = pair_canonical * ctarget;
                 float den = ncount * ntarget + cd;
-                share += (pair_weight * cd) / den;
+                share += g_plant == kPlantDiSpatialMis ? pair_weight : (pair_weight * cd) / den;
             }
             if (light >= 0 && light < f->di_light_count && rc.w > 0.0f) {
                 float target = light_target(p, surface, light, xyz(sd));

@@ -29,6 +29,30 @@ extern "C" {
 void orc_set_threads(int threads);
 int orc_get_threads(void);
 
+/* For the oracle's own pins only: commit planted mistake `mistake` (see
+ * enum Plant in oracle.cpp; 0 restores the faithful oracle), so a pin can
+ * show that it fails on it. */
+void orc_plant(int mistake);
+
+/* traceScreenSpaceRay (restir.sh:364-436) of one ray over a G-buffer at the
+ * frame's GI size; returns 1 on a hit and writes the hit surface point. */
+int orc_screen_march(const rgi_frame* frame, const rgi_gbuffer* gbuffer, const float* origin3, const float* dir3,
+                     float max_distance, float* hit3);
+
+/* reconnectionJacobian (restir.sh:623-649): a sample point (position,
+ * normal) shifted from receiver `there` to receiver `here`. */
+float orc_reconnection_jacobian(const rgi_frame* frame, const float* here3, const float* there3,
+                                const float* sample3, const float* normal3);
+
+/* The pairwise MIS bookkeeping of restir.sh:789-931 for one sample point held
+ * by the canonical reservoir and `taps` neighbours (rows of receiver
+ * position, normal, count; a negative count is a tap with no reservoir):
+ * the canonical share m_c(x) M_total, each neighbour's m_i(x) M_total, and
+ * M_total. Visibility goes through `scene`'s BVH (frame ray mode BVH). */
+void orc_gi_pairwise_weights(const rgi_frame* frame, const rgi_scene* scene, const float* receiver6,
+                             float canonical_count, const float* sample6, int taps, const float* neighbours,
+                             float* out_canonical_share, float* out_neighbour_mis, float* out_total);
+
 /* Scalar primitives, for known-answer tests. */
 uint32_t orc_pcg(uint32_t value);                       /* gridHash / pcg, hashgrid.sh:61-67 */
 uint32_t orc_checksum_hash(uint32_t value);             /* gridChecksumHash, hashgrid.sh:69-78 */

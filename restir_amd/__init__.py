@@ -282,21 +282,35 @@ def _ptr(a):
 class HostScene:
     """A procedural scene on the host: mesh, BVH (bvh.h layout), light list."""
 
-    def __init__(self, kind=RGI_SCENE_CORNELL, seed=1, size=0, extra_lights=0, extra_intensity=0.3, light_seed=1):
+    def __init__(self, kind=RGI_SCENE_CORNELL, seed=1, size=0, extra_lights=0, extra_intensity=0.3, light_seed=1,
+                 mesh=None):
         L = lib()
-        count = L.rgi_scene_generate(kind, seed, size, None, 0)
-        if count < 0:
-            raise RestirError("unknown scene kind %d" % kind)
-        self.positions = np.zeros((count, 9), np.float32)
-        self.albedo = np.zeros((count, 3), np.float32)
-        self.emission = np.zeros((count, 3), np.float32)
-        self.material = np.zeros(count, np.uint32)
-        self.materials = np.zeros((16, 4), np.float32)
-        mesh = Mesh(0, self.positions.ctypes.data, self.albedo.ctypes.data, self.emission.ctypes.data,
-                    self.material.ctypes.data, 0, self.materials.ctypes.data)
-        got = L.rgi_scene_generate(kind, seed, size, ctypes.byref(mesh), count)
-        assert got == count
-        self.material_count = min(mesh.material_count, 16)
+        if mesh is None:
+            count = L.rgi_scene_generate(kind, seed, size, None, 0)
+            if count < 0:
+                raise RestirError("unknown scene kind %d" % kind)
+            self.positions = np.zeros((count, 9), np.float32)
+            self.albedo = np.zeros((count, 3), np.float32)
+            self.emission = np.zeros((count, 3), np.float32)
+            self.material = np.zeros(count, np.uint32)
+            self.materials = np.zeros((16, 4), np.float32)
+            mesh = Mesh(0, self.positions.ctypes.data, self.albedo.ctypes.data, self.emission.ctypes.data,
+                        self.material.ctypes.data, 0, self.materials.ctypes.data)
+            got = L.rgi_scene_generate(kind, seed, size, ctypes.byref(mesh), count)
+            assert got == count
+            self.material_count = min(mesh.material_count, 16)
+        else:
+            positions, albedo, emission, materials = mesh
+            count = len(positions)
+            self.positions = np.ascontiguousarray(positions, np.float32).reshape(count, 9)
+            self.albedo = np.ascontiguousarray(albedo, np.float32).reshape(count, 3)
+            self.emission = np.ascontiguousarray(emission, np.float32).reshape(count, 3)
+            self.material = np.zeros(count, np.uint32)
+            self.materials = np.zeros((16, 4), np.float32)
+            self.materials[0, :len(materials)] = materials
+            self.material_count = 1
+            mesh = Mesh(count, self.positions.ctypes.data, self.albedo.ctypes.data, self.emission.ctypes.data,
+                        self.material.ctypes.data, 1, self.materials.ctypes.data)
         self.triangle_count = int(count)
         self.bounds_min = np.zeros(3, np.float32)
         self.bounds_max = np.zeros(3, np.float32)
@@ -325,6 +339,15 @@ class HostScene:
                            _ptr(self.lights), max(nl, 1))
         self.light_count = int(nl)
         self.lights = self.lights[: max(nl, 1)]
+
+    @classmethod
+    def from_mesh(cls, positions, albedo, emission, material=(1.0, 0.0, 0.0, 0.0), extra_lights=0,
+                  extra_intensity=0.3, light_seed=1):
+        """A scene of given triangles (n x 9 vertex coordinates), per-triangle
+        albedo and emission (n x 3), and one material (roughness, metallic)
+        for all of them, through the same BVH and light-list builders."""
+        return cls(extra_lights=extra_lights, extra_intensity=extra_intensity, light_seed=light_seed,
+                   mesh=(positions, albedo, emission, np.asarray(material, np.float32)))
 
     def _light_scale(self):
         s = np.float32(self.scene_scale)

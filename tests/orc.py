@@ -36,6 +36,14 @@ def lib():
     L.orc_set_threads.argtypes = [ctypes.c_int]
     L.orc_set_threads.restype = None
     L.orc_get_threads.restype = ctypes.c_int
+    L.orc_plant.argtypes = [ctypes.c_int]
+    L.orc_plant.restype = None
+    L.orc_screen_march.restype = ctypes.c_int
+    L.orc_screen_march.argtypes = [PF, PG, P, P, F, P]
+    L.orc_reconnection_jacobian.restype = F
+    L.orc_reconnection_jacobian.argtypes = [PF, P, P, P, P]
+    L.orc_gi_pairwise_weights.restype = None
+    L.orc_gi_pairwise_weights.argtypes = [PF, PS, P, F, P, ctypes.c_int, P, P, P, P]
     L.orc_pcg.restype = U32
     L.orc_pcg.argtypes = [U32]
     L.orc_checksum_hash.restype = U32
