@@ -675,6 +675,42 @@ def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offse
             "gi_pixels_per_s": round(sum(pixels) / (batch_ms * 1e-3), 1), "steps": steps, "nrc": nrc}
 
 
+def atrium_workload(args, W, H, mode):
+    """The atrium workload: ~256k triangles, 2^22-cell grid, DI over 64
+    point lights and the emissive triangles; the same swing path, with the
+    camera inside the courtyard (0.7 scene radii from the centre); then
+    traversal over its BVH (the same ray count, cosine rays from points
+    uniform over its area). Returns its bench entry and the live run."""
+    L = ra.lib()
+    ahs = ra.HostScene(ra.RGI_SCENE_ATRIUM, seed=1, size=262144, extra_lights=64, light_seed=1)
+    aset = ra.Settings(grid_table_size=1 << 22, grid_mode=mode, denoiser=False, composite=False, specular_gi=False)
+    arun = run_workload(ahs, aset, W, H, args.frames, args.atrium_steps, 8, 4, distance=0.7)
+    atrium = {"ms_per_frame": round(arun["ms_per_frame"], 5),
+              "pass_ms": {k: round(v, 5) for k, v in sorted(arun["pass_ms"].items())},
+              "triangles": ahs.triangle_count, "bvh_nodes": ahs.node_count, "di_lights": ahs.light_count,
+              "grid_table_size": 1 << 22, "steps": args.atrium_steps, "camera_distance_radii": 0.7,
+              "tuning": arun["tuning"]}
+    atr = trace_benchmark(L, arun["drv"].ctx, arun["gscene"].scene, surface_rays(ahs, args.rays, seed=7),
+                          driver.stream())
+    atrium["trace"] = {"rays": atr["rays"], "ms": round(atr["ms"], 4), "mrays_per_s": round(atr["mrays_per_s"], 2),
+                       "hit_rate": round(atr["hit_rate"], 4), "persistent": atr["persistent"],
+                       "ms_by_variant": atr["ms_by_variant"], "watertight": atr["watertight"]}
+    arun["hs"] = ahs
+    return atrium, arun
+
+
+def frame_variant(hs, settings, W, H, frames, steps):
+    """The metric's frame with other settings (its own renderer, autotuned
+    on the same path): frame time and per-pass times."""
+    L = ra.lib()
+    r = run_workload(hs, settings, W, H, frames, steps, 8, 4)
+    out = {"ms_per_frame": round(r["ms_per_frame"], 5),
+           "pass_ms": {k: round(v, 5) for k, v in sorted(r["pass_ms"].items())}, "tuning": r["tuning"]}
+    L.rgi_renderer_destroy(r["rend"])
+    r["drv"].close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -695,6 +731,11 @@ def main():
     ap.add_argument("--no-batch", dest="batch", action="store_false", help="skip the uneven batch workload")
     ap.add_argument("--no-nrc", dest="nrc", action="store_false", help="skip the neural radiance cache measurements")
     ap.add_argument("--nrc-frames", type=int, default=256, help="frames rendered before the cache's MSE is measured")
+    ap.add_argument("--atrium-only", action="store_true",
+                    help="run only the atrium workload (--atrium-steps timed frames and its traversal) and print it: "
+                         "the workload of the atrium's kernel profile")
+    ap.add_argument("--no-variants", dest="variants", action="store_false",
+                    help="skip the deterministic-insert and half-resolution GI frames")
     ap.add_argument("--cache-frames-only", action="store_true",
                     help="render only the metric's path with the neural radiance cache on (16 warm-up + --steps "
                          "frames) and print their frame time: the workload of the cache's kernel profile")
@@ -719,6 +760,9 @@ def main():
         print(json.dumps(cache_frames(hs, W, H, T, args.frames, args.steps)), flush=True)
         return
     mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
+    if args.atrium_only:
+        print(json.dumps(atrium_workload(args, W, H, mode)[0]), flush=True)
+        return
     # The metric's frame: grid + resample + trace. The denoiser that follows
     # it in the reference is timed on its own below.
     settings = ra.Settings(grid_table_size=T, grid_mode=mode, denoiser=False, composite=False, specular_gi=False)
@@ -810,6 +854,16 @@ def main():
     # than the process's 4 hardware queues, the cache's training stream can
     # share a queue with the frame's stream and run behind it, not beside it.
     L.rgi_renderer_destroy(rend)
+    variants = None
+    if args.variants:
+        # The same frame with the deterministic (bit-exact) grid insert, and
+        # at the reference's default GI resolution (half the window each way,
+        # settings.h:218) over the full-resolution G-buffer and DI.
+        det = ra.Settings(**dict(settings.__dict__, grid_mode=ra.RGI_GRID_DETERMINISTIC))
+        half = ra.Settings(**dict(settings.__dict__, gi_divisor=2))
+        variants = {"deterministic_insert": frame_variant(hs, det, W, H, args.frames, args.steps),
+                    "half_resolution_gi": dict(frame_variant(hs, half, W, H, args.frames, args.steps),
+                                               gi_width=W // 2, gi_height=H // 2)}
     # The neural radiance cache's cost on the metric's path.
     nrc = nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, args.frames, 64, ms_per_frame, s) if args.nrc else None
 
@@ -848,27 +902,10 @@ def main():
 
     atrium = None
     if args.atrium_steps > 0:
-        # The atrium workload: ~256k triangles, 2^22-cell grid, DI over 64
-        # point lights and the emissive triangles; the same swing path, with
-        # the camera inside the courtyard (0.7 scene radii from the centre).
-        ahs = ra.HostScene(ra.RGI_SCENE_ATRIUM, seed=1, size=262144, extra_lights=64, light_seed=1)
-        aset = ra.Settings(grid_table_size=1 << 22, grid_mode=mode, denoiser=False, composite=False,
-                           specular_gi=False)
-        arun = run_workload(ahs, aset, W, H, args.frames, args.atrium_steps, 8, 4, distance=0.7)
-        atrium = {"ms_per_frame": round(arun["ms_per_frame"], 5),
-                  "pass_ms": {k: round(v, 5) for k, v in sorted(arun["pass_ms"].items())},
-                  "triangles": ahs.triangle_count, "bvh_nodes": ahs.node_count, "di_lights": ahs.light_count,
-                  "grid_table_size": 1 << 22, "steps": args.atrium_steps, "camera_distance_radii": 0.7,
-                  "tuning": arun["tuning"]}
-        # Traversal over the atrium's BVH: the same ray count, cosine rays from
-        # points uniform over its area.
-        atr = trace_benchmark(L, arun["drv"].ctx, arun["gscene"].scene, surface_rays(ahs, args.rays, seed=7), s)
-        atrium["trace"] = {"rays": atr["rays"], "ms": round(atr["ms"], 4), "mrays_per_s": round(atr["mrays_per_s"], 2),
-                           "hit_rate": round(atr["hit_rate"], 4), "persistent": atr["persistent"],
-                           "ms_by_variant": atr["ms_by_variant"], "watertight": atr["watertight"]}
+        atrium, arun = atrium_workload(args, W, H, mode)
         if args.nrc and args.reference_frames > 0 and nrc is not None:
             # The resolved cache on the atrium (static camera, view 0).
-            nrc["quality_atrium"] = resolve_quality(L, ahs, arun["gscene"], arun["gbufs"][0], arun["views"][0],
+            nrc["quality_atrium"] = resolve_quality(L, arun["hs"], arun["gscene"], arun["gbufs"][0], arun["views"][0],
                                                     1 << 22, mode, arun["drv"].ctx, s)
         L.rgi_renderer_destroy(arun["rend"])
         arun["drv"].close()
@@ -912,6 +949,7 @@ def main():
         "tuning": tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "variants": variants,
         "atrium_1080p": atrium,
         "uneven_batch": uneven,
         "atrium_8_views_batch": views8,

@@ -218,7 +218,7 @@ int rgi_grid_insert(rgi_context* context, const rgi_frame* frame, const rgi_gbuf
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). rgi_grid_insert in RGI_GRID_DETERMINISTIC mode, the
  * same kernels and the same result, with its scratch in a fixed device array
- * of the library (RGI_NOALLOC_SCRATCH_BYTES, 88 MiB plus 256 bytes, held by
+ * of the library (RGI_NOALLOC_SCRATCH_BYTES, 88 MiB plus 8,704 bytes, held by
  * each GPU the library runs on until the process ends) instead of a
  * context's: it needs no context and calls no allocator. GI sizes up to
  * RGI_NOALLOC_MAX_PIXELS pixels (1920 x 1080 fits) and tables up to

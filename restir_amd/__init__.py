@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librestir_amd.so")
 
 RGI_OK = 0
+RGI_E_INVALID, RGI_E_DEVICE, RGI_E_NOMEM = -1, -2, -3
 RGI_RAY_SCREEN, RGI_RAY_HYBRID, RGI_RAY_BVH = 0, 1, 2
 RGI_GRID_ATOMIC, RGI_GRID_DETERMINISTIC = 0, 1
 RGI_SCENE_CORNELL, RGI_SCENE_ATRIUM, RGI_SCENE_BOXFIELD = 0, 1, 2

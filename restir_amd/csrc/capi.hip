@@ -13,13 +13,14 @@
 // its largest admitted inputs, one copy per GPU. In host code this name is a
 // host-side handle, not a device address: kernels and copies get the address
 // looked up per device by noalloc_scratch() below.
-#define RGI_NOALLOC_SCRATCH_BYTES (88ull * 1024 * 1024 + 256)
+#define RGI_NOALLOC_SCRATCH_BYTES (88ull * 1024 * 1024 + 256 + 8448)
 static __device__ __align__(256) unsigned char scratch[RGI_NOALLOC_SCRATCH_BYTES];
 
 namespace rgi {
 
 static_assert(RGI_NOALLOC_SCRATCH_BYTES == 6 * 4 * (size_t)RGI_NOALLOC_MAX_PIXELS + 4 * (size_t)RGI_NOALLOC_MAX_TABLE +
-                                                3 * 4 * (size_t)RGI_NOALLOC_MAX_PIXELS + 256,
+                                                3 * 4 * (size_t)RGI_NOALLOC_MAX_PIXELS + 256 +
+                                                ((4 * ((size_t)RGI_NOALLOC_MAX_PIXELS / 1024 + 1) + 255) & ~(size_t)255),
               "the no-alloc scratch holds the deterministic insert's scratch at the largest admitted inputs");
 
 // The device address of `scratch` on the current device, looked up once per

@@ -55,6 +55,7 @@ RGI_DEV void stream_candidate(const GiPass& p, F3 position, F3 n, F3 albedo, F3 
     }
 }
 
+template <bool kSameSize>
 RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                               const rgi_gi_reservoirs& out, F2 texel, const TileMap& tiles, TemporalLds& lds)
 {
@@ -69,7 +70,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     p.v = scene_view(s);
     p.g = g;
     p.in = in;
-    set_gi_size(p, f);
+    set_gi_size<kSameSize>(p, f, g, texel);
     p.grid_camera = F3{f.prev_camera_position[0], f.prev_camera_position[1], f.prev_camera_position[2]};
     p.sc = f.scene_scale;
 
@@ -221,6 +222,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     }
 }
 
+template <bool kSameSize>
 RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                              const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel,
                              const TileMap& tiles)
@@ -230,7 +232,7 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     p.v = scene_view(s);
     p.g = g;
     p.in = in;
-    set_gi_size(p, f);
+    set_gi_size<kSameSize>(p, f, g, texel);
     p.grid_camera = F3{f.camera_position[0], f.camera_position[1], f.camera_position[2]};
     p.sc = f.scene_scale;
 
@@ -337,36 +339,41 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
 }
 
 // The kernels: each pass with the compiler's occupancy, and capped at
-// 128 VGPRs for 4 waves per SIMD.
+// 128 VGPRs for 4 waves per SIMD; each for a G-buffer of the GI size and for
+// a larger (window-size) one.
 #define RGI_WAVES4 __attribute__((amdgpu_waves_per_eu(4, 4)))
 
+template <bool kSameSize>
 __global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                      rgi_gi_reservoirs out, F2 texel, TileMap tiles)
 {
     __shared__ TemporalLds lds;
-    gi_temporal_body(f, s, g, in, out, texel, tiles, lds);
+    gi_temporal_body<kSameSize>(f, s, g, in, out, texel, tiles, lds);
 }
 
+template <bool kSameSize>
 __global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs out, F2 texel,
                  TileMap tiles)
 {
     __shared__ TemporalLds lds;
-    gi_temporal_body(f, s, g, in, out, texel, tiles, lds);
+    gi_temporal_body<kSameSize>(f, s, g, in, out, texel, tiles, lds);
 }
 
+template <bool kSameSize>
 __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                     rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
                                                     F2 texel, TileMap tiles)
 {
-    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, tiles);
+    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles);
 }
 
+template <bool kSameSize>
 __global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
                 uint16_t* out_moments, F2 texel, TileMap tiles)
 {
-    gi_spatial_body(f, s, g, in, res, out_color, out_moments, texel, tiles);
+    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles);
 }
 
 } // namespace
@@ -376,7 +383,9 @@ int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
 {
     TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
     F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
-    auto kernel = tuning.gi_temporal_waves == 4 ? k_gi_temporal_w4 : k_gi_temporal;
+    const bool same = g.width == f.gi_width && g.height == f.gi_height;
+    auto kernel = tuning.gi_temporal_waves == 4 ? (same ? k_gi_temporal_w4<true> : k_gi_temporal_w4<false>)
+                                                : (same ? k_gi_temporal<true> : k_gi_temporal<false>);
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, out, texel, tiles);
     return check_launch("gi temporal");
 }
@@ -387,7 +396,9 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
 {
     TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
     F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
-    auto kernel = tuning.gi_spatial_waves == 4 ? k_gi_spatial_w4 : k_gi_spatial;
+    const bool same = g.width == f.gi_width && g.height == f.gi_height;
+    auto kernel = tuning.gi_spatial_waves == 4 ? (same ? k_gi_spatial_w4<true> : k_gi_spatial_w4<false>)
+                                               : (same ? k_gi_spatial<true> : k_gi_spatial<false>);
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, res, color,
                        moments, texel, tiles);
     return check_launch("gi spatial");

@@ -29,12 +29,18 @@ struct GiPass {
 RGI_DEV uint32_t tix(const GiPass& p, F2 uv) { return texel_index(uv, p.g.width, p.g.height); }
 RGI_DEV uint32_t hix(const GiPass& p, F2 uv) { return texel_index(uv, p.hw, p.hh); }
 
-RGI_DEV void set_gi_size(GiPass& p, const rgi_frame& f)
+// `texel` is 1 / GI size, computed once on the host and passed as a kernel
+// argument (a uniform value: dividing it out per lane costs registers). With
+// kSameSize (a G-buffer of the GI size, divisor 1) the GI-plane size is taken
+// from the G-buffer's own fields, so the compiler sees one texel index where
+// both kinds of plane are fetched at the same uv.
+template <bool kSameSize>
+RGI_DEV void set_gi_size(GiPass& p, const rgi_frame& f, const rgi_gbuffer& g, F2 texel)
 {
-    p.hw = f.gi_width;
-    p.hh = f.gi_height;
-    p.size = F2{(float)f.gi_width, (float)f.gi_height};
-    p.texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
+    p.hw = kSameSize ? g.width : f.gi_width;
+    p.hh = kSameSize ? g.height : f.gi_height;
+    p.size = F2{(float)p.hw, (float)p.hh};
+    p.texel = texel;
 }
 RGI_DEV float fetch_depth(const GiPass& p, F2 uv) { return p.g.depth[tix(p, uv)]; }
 

@@ -103,7 +103,10 @@ struct DetScratch {
     uint32_t* offset;    // [min(T, n)] CSR offsets per cell
     uint32_t* fill;      // [min(T, n)] CSR fill counters per cell
     uint32_t* counters;  // [0] pending count, [1] occupied count, [2] resolved count
+    uint32_t* block_sum; // [min(T, n) / 1024 + 1] listed pixels per 1024 occupied cells, then their offsets
 };
+
+constexpr uint32_t kScanCells = 1024; // occupied cells per workgroup of the offsets scan
 
 constexpr uint32_t kTagShift = 29;
 constexpr uint32_t kIndexMask = (1u << kTagShift) - 1u;
@@ -135,42 +138,128 @@ __global__ void __launch_bounds__(256) k_det_claim0(uint32_t n, uint32_t mask, c
     if (grid[s] == 0u) atomicMin(&d.claim[s], claim_tag(0u, i));
 }
 
-// Resolution of pixel i in round p; true when it found its slot.
-RGI_DEV bool det_resolve(uint32_t i, uint32_t round, uint32_t table, uint32_t* grid, DetScratch& d)
+// Resolution of pixel i in round p without its counter updates: true when it
+// found its slot `s`; `claimed` when it is the slot's new owner (it writes
+// the slot's checksum).
+RGI_DEV bool det_resolve_core(uint32_t i, uint32_t round, uint32_t table, uint32_t* grid, DetScratch& d,
+                              uint32_t& s, bool& claimed)
 {
     uint32_t mask = table - 1u;
     uint32_t cs = d.checksum[i];
-    uint32_t s = (d.slot0[i] + round) & mask;
+    s = (d.slot0[i] + round) & mask;
     uint32_t c = d.claim[s];
     bool done;
+    claimed = false;
     if (c != 0xffffffffu && (c >> kTagShift) == 7u - round) {
         // Empty at the start of this round: the lowest claimant took it.
         uint32_t winner = c & kIndexMask;
         done = d.checksum[winner] == cs;
         if (winner == i) {
             grid[s] = cs;
-            uint32_t k = atomicAdd(&d.counters[1], 1u);
-            d.occupied[k] = s;
+            claimed = true;
         }
     } else {
         done = grid[s] == cs;
     }
+    if (done) d.resolved[i] = s;
+    return done;
+}
+
+// The same with one atomic per counter update (the tail's lanes diverge).
+RGI_DEV bool det_resolve(uint32_t i, uint32_t round, uint32_t table, uint32_t* grid, DetScratch& d)
+{
+    uint32_t s;
+    bool claimed;
+    bool done = det_resolve_core(i, round, table, grid, d, s, claimed);
+    if (claimed) d.occupied[atomicAdd(&d.counters[1], 1u)] = s;
     if (done) {
-        d.resolved[i] = s;
         atomicAdd(&grid[table + s], 1u);
         atomicAdd(&d.counters[2], 1u);
     }
     return done;
 }
 
-__global__ void __launch_bounds__(256) k_det_resolve0(uint32_t n, uint32_t table, uint32_t* grid, DetScratch d)
+// Wave-aggregated fetch-and-add of 1 per active lane to *addr (addresses may
+// differ between lanes): one atomicAdd per distinct address in the wave, each
+// lane receiving the old value plus its rank among the lanes sharing its
+// address. All lanes of the wave must call it. The sums do not depend on the
+// order of the atomics, and every consumer of the returned positions (the
+// occupied-cell and pending lists, a cell's gathered pixels) is order-free:
+// pending pixels claim by atomicMin, CSR offsets follow whatever order the
+// cells come in, and each cell's pixels are sorted afterwards.
+RGI_DEV uint32_t aggregated_add(bool active, uint32_t* addr)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || d.checksum[i] == 0u) return;
-    if (!det_resolve(i, 0u, table, grid, d)) {
-        uint32_t k = atomicAdd(&d.counters[0], 1u);
-        d.pending[k] = i;
+    const uint32_t lane = lane_id();
+    const uint64_t a = reinterpret_cast<uint64_t>(addr);
+    uint64_t pending = __ballot(active);
+    uint32_t result = 0;
+    while (pending) {
+        uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+        // readlane returns an int: widen each half through uint32_t, never
+        // by sign extension.
+        uint32_t lead_hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(a >> 32), (int)leader);
+        uint32_t lead_lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)a, (int)leader);
+        uint64_t lead = ((uint64_t)lead_hi << 32) | (uint64_t)lead_lo;
+        // The leader is always its own peer, so every pass retires at least
+        // one lane and the loop ends after at most 64 passes.
+        bool peer = lane == leader || (((pending >> lane) & 1ull) && a == lead);
+        uint64_t peers = __ballot(peer);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(addr, (uint32_t)__popcll(peers));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if (peer) result = base + rank_in(peers);
+        pending &= ~peers;
     }
+    return result;
+}
+
+// Pixel index of this thread over the GI pixels (gw x gh) in 16 x 16 blocks
+// of 8 x 8 tiles per wave, so that a wave's pixels share few cells; false
+// past the edge.
+RGI_DEV bool tile_index(int gw, int gh, uint32_t& i)
+{
+    int x, y;
+    block_pixel(x, y);
+    i = (uint32_t)y * (uint32_t)gw + (uint32_t)x;
+    return x < gw && y < gh;
+}
+
+// Positions for the lanes of this workgroup with `flag` set in a list whose
+// length is *counter: each wave's lanes are ranked by ballot, the waves by an
+// LDS counter, the workgroup by one global atomicAdd (not one per wave: a
+// global counter that every wave increments serializes in L2). All threads
+// of the workgroup must call it.
+RGI_DEV uint32_t workgroup_append(bool flag, uint32_t* counter, uint32_t* lds_count, uint32_t* lds_base)
+{
+    const uint32_t lane = lane_id();
+    if (threadIdx.x == 0) *lds_count = 0u;
+    __syncthreads();
+    uint64_t m = __ballot(flag);
+    uint32_t wave_base = 0;
+    if (lane == 0u && m != 0ull) wave_base = atomicAdd(lds_count, (uint32_t)__popcll(m));
+    wave_base = __builtin_amdgcn_readlane(wave_base, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) *lds_base = *lds_count ? atomicAdd(counter, *lds_count) : 0u;
+    __syncthreads();
+    return *lds_base + wave_base + rank_in(m);
+}
+
+__global__ void __launch_bounds__(256) k_det_resolve0(int gw, int gh, uint32_t table, uint32_t* grid, DetScratch d)
+{
+    __shared__ uint32_t lds_count, lds_base;
+    uint32_t i;
+    bool active = tile_index(gw, gh, i);
+    active = active && d.checksum[i] != 0u;
+    uint32_t s = 0;
+    bool claimed = false, done = false;
+    if (active) done = det_resolve_core(i, 0u, table, grid, d, s, claimed);
+    uint32_t k = workgroup_append(claimed, &d.counters[1], &lds_count, &lds_base);
+    if (claimed) d.occupied[k] = s;
+    aggregated_add(done, &grid[table + s]);
+    workgroup_append(done, &d.counters[2], &lds_count, &lds_base);
+    bool left = active && !done;
+    uint32_t p = workgroup_append(left, &d.counters[0], &lds_count, &lds_base);
+    if (left) d.pending[p] = i;
 }
 
 // Rounds 1..7 over the pixels round 0 left, in one workgroup: a block
@@ -203,43 +292,103 @@ __global__ void __launch_bounds__(1024) k_det_tail(uint32_t table, uint32_t* gri
     }
 }
 
-// CSR offsets of the occupied cells (one workgroup; cells in claim order).
-__global__ void __launch_bounds__(1024) k_det_offsets(uint32_t table, const uint32_t* grid, DetScratch d)
+// CSR offsets of the occupied cells (cells in claim order), a three-step
+// scan: per workgroup of kScanCells cells the sum of their counts; one
+// workgroup's exclusive scan over those sums; per workgroup the exclusive
+// scan of its cells' counts plus its sum's offset. Each cell's offset is the
+// pixels listed before it, its fill counter starts at 0, and its slot maps to
+// its cell index.
+RGI_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t& total)
 {
-    __shared__ uint32_t partial[1024];
-    uint32_t cells = d.counters[1];
-    uint32_t per = (cells + blockDim.x - 1u) / blockDim.x;
-    uint32_t begin = threadIdx.x * per;
-    uint32_t end = begin + per < cells ? begin + per : cells;
-    uint32_t sum = 0;
-    for (uint32_t k = begin; k < end; ++k) sum += grid[table + d.occupied[k]];
-    partial[threadIdx.x] = sum;
+    // 256 threads: Hillis-Steele over LDS.
+    uint32_t t = threadIdx.x;
+    lds[t] = v;
     __syncthreads();
-    for (uint32_t stride = 1; stride < blockDim.x; stride <<= 1) {
-        uint32_t v = threadIdx.x >= stride ? partial[threadIdx.x - stride] : 0u;
+    for (uint32_t stride = 1; stride < 256u; stride <<= 1) {
+        uint32_t add = t >= stride ? lds[t - stride] : 0u;
         __syncthreads();
-        partial[threadIdx.x] += v;
+        lds[t] += add;
         __syncthreads();
     }
-    uint32_t running = partial[threadIdx.x] - sum;
-    for (uint32_t k = begin; k < end; ++k) {
-        uint32_t s = d.occupied[k];
-        d.offset[k] = running;
-        d.fill[k] = 0u;
-        d.claim[s] = k;
-        running += grid[table + s];
+    total = lds[255];
+    uint32_t inclusive = lds[t];
+    __syncthreads();
+    return inclusive - v;
+}
+
+__global__ void __launch_bounds__(256) k_det_block_sums(uint32_t table, const uint32_t* grid, DetScratch d)
+{
+    __shared__ uint32_t lds[256];
+    const uint32_t cells = d.counters[1];
+    const uint32_t first = blockIdx.x * kScanCells;
+    if (first >= cells) return; // the same for the whole workgroup
+    uint32_t sum = 0;
+    for (uint32_t k = first + threadIdx.x; k < first + kScanCells && k < cells; k += 256u)
+        sum += grid[table + d.occupied[k]];
+    uint32_t total;
+    block_exclusive_scan(sum, lds, total);
+    if (threadIdx.x == 0) d.block_sum[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256) k_det_block_scan(DetScratch d)
+{
+    __shared__ uint32_t lds[256];
+    const uint32_t blocks = (d.counters[1] + kScanCells - 1u) / kScanCells;
+    // Up to 16 block sums per thread, in order.
+    const uint32_t per = (blocks + 255u) / 256u;
+    const uint32_t begin = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t b = begin; b < begin + per && b < blocks; ++b) sum += d.block_sum[b];
+    uint32_t total;
+    uint32_t running = block_exclusive_scan(sum, lds, total);
+    for (uint32_t b = begin; b < begin + per && b < blocks; ++b) {
+        uint32_t v = d.block_sum[b];
+        d.block_sum[b] = running;
+        running += v;
     }
 }
 
-__global__ void __launch_bounds__(256) k_det_scatter(uint32_t n, DetScratch d)
+__global__ void __launch_bounds__(256) k_det_offsets(uint32_t table, const uint32_t* grid, DetScratch d)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t s = d.resolved[i];
-    if (s == kGridInvalid) return;
-    uint32_t k = d.claim[s];
-    uint32_t pos = d.offset[k] + atomicAdd(&d.fill[k], 1u);
-    if (pos < d.counters[2]) d.list[pos] = i;
+    __shared__ uint32_t lds[256];
+    const uint32_t cells = d.counters[1];
+    const uint32_t first = blockIdx.x * kScanCells;
+    if (first >= cells) return; // the same for the whole workgroup
+    // Thread t owns cells first + 4t .. first + 4t + 3.
+    constexpr uint32_t kPer = kScanCells / 256u;
+    uint32_t counts[kPer], sum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        uint32_t k = first + threadIdx.x * kPer + j;
+        counts[j] = k < cells ? grid[table + d.occupied[k]] : 0u;
+        sum += counts[j];
+    }
+    uint32_t total;
+    uint32_t running = d.block_sum[blockIdx.x] + block_exclusive_scan(sum, lds, total);
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        uint32_t k = first + threadIdx.x * kPer + j;
+        if (k < cells) {
+            d.offset[k] = running;
+            d.fill[k] = 0u;
+            d.claim[d.occupied[k]] = k;
+        }
+        running += counts[j];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_det_scatter(int gw, int gh, DetScratch d)
+{
+    uint32_t i;
+    bool inside = tile_index(gw, gh, i);
+    uint32_t s = inside ? d.resolved[i] : kGridInvalid;
+    bool active = s != kGridInvalid;
+    uint32_t k = active ? d.claim[s] : 0u;
+    uint32_t rank = aggregated_add(active, &d.fill[k]);
+    if (active) {
+        uint32_t pos = d.offset[k] + rank;
+        if (pos < d.counters[2]) d.list[pos] = i;
+    }
 }
 
 RGI_DEV uint32_t bitonic_sort64(uint32_t v, uint32_t lane)
@@ -302,7 +451,8 @@ int grid_insert_atomic(const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
 size_t det_scratch_bytes(uint32_t pixels, uint32_t table)
 {
     uint32_t cells = table < pixels ? table : pixels;
-    return align256((size_t)pixels * 4) * 6 + align256((size_t)table * 4) + align256((size_t)cells * 4) * 3 + 256;
+    return align256((size_t)pixels * 4) * 6 + align256((size_t)table * 4) + align256((size_t)cells * 4) * 3 + 256 +
+           align256((size_t)(cells / kScanCells + 1) * 4);
 }
 
 int grid_insert_deterministic(rgi_context* ctx, const rgi_frame& f, const rgi_gbuffer& g, uint32_t* grid,
@@ -338,15 +488,20 @@ int grid_insert_deterministic_in(void* scratch_base, const rgi_frame& f, const r
     d.offset = take(cells);
     d.fill = take(cells);
     d.counters = take(64);
+    d.block_sum = take(cells / kScanCells + 1);
     uint32_t blocks = (n + 255u) / 256u;
     hipLaunchKernelGGL(k_det_keys, dim3(blocks), dim3(256), 0, stream, f, g, texel_of(f), d);
     hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, stream, d.claim, table, 0xffffffffu);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, stream, d.counters, 64u, 0u);
     hipLaunchKernelGGL(k_det_claim0, dim3(blocks), dim3(256), 0, stream, n, table - 1u, grid, d);
-    hipLaunchKernelGGL(k_det_resolve0, dim3(blocks), dim3(256), 0, stream, n, table, grid, d);
+    const dim3 tiles((uint32_t)(f.gi_width + 15) / 16u, (uint32_t)(f.gi_height + 15) / 16u);
+    hipLaunchKernelGGL(k_det_resolve0, tiles, dim3(256), 0, stream, f.gi_width, f.gi_height, table, grid, d);
     hipLaunchKernelGGL(k_det_tail, dim3(1), dim3(1024), 0, stream, table, grid, d);
-    hipLaunchKernelGGL(k_det_offsets, dim3(1), dim3(1024), 0, stream, table, grid, d);
-    hipLaunchKernelGGL(k_det_scatter, dim3(blocks), dim3(256), 0, stream, n, d);
+    const uint32_t scan_blocks = cells / kScanCells + 1; // covers every occupied cell (at most `cells`)
+    hipLaunchKernelGGL(k_det_block_sums, dim3(scan_blocks), dim3(256), 0, stream, table, grid, d);
+    hipLaunchKernelGGL(k_det_block_scan, dim3(1), dim3(256), 0, stream, d);
+    hipLaunchKernelGGL(k_det_offsets, dim3(scan_blocks), dim3(256), 0, stream, table, grid, d);
+    hipLaunchKernelGGL(k_det_scatter, tiles, dim3(256), 0, stream, f.gi_width, f.gi_height, d);
     hipLaunchKernelGGL(k_det_select, dim3(2048), dim3(256), 0, stream, table, grid, d);
     return check_launch("grid insert (deterministic)");
 }

@@ -81,7 +81,10 @@ typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
 constexpr int kTrainB = 2;              // 16-sample column blocks per wave in training
-static_assert(kTrainB >= 1 && 16 * kTrainB <= 64, "the ReLU masks hold 16 bits per column block in 64 bits");
+// The ReLU masks hold 16 bits per column block in 64 bits, so B <= 4 fits;
+// only B = 2 has run on the GPU against the oracle's gradients, and the
+// training kernels are built for that value alone.
+static_assert(kTrainB == 2, "k_nrc_train is validated at two 16-sample column blocks per wave");
 constexpr int kTile = 64 * kTrainB;     // samples per training workgroup (4 waves)
 constexpr int kChunk = 1024;            // samples per weight-gradient workgroup
 constexpr int kChunkTiles = kChunk / kTile;

@@ -376,6 +376,41 @@ void rgi_renderer_invalidate_history(rgi_renderer* r)
     r->di_valid = false;
 }
 
+// What render_frame commits on the host (ping-pong parity, validity flags,
+// frame count, cameras, the cache's step count): saved before a capture so
+// that a capture, instantiation or launch that fails leaves the renderer as
+// it was, its GPU buffers untouched by work that never ran.
+struct HostState {
+    int read, specular_read;
+    bool history_valid, di_valid, have_frame, nrc_pending, atlas_valid;
+    uint32_t frame_number, nrc_trained;
+    rgi_view previous;
+    rgi_frame last;
+    const uint16_t* denoised;
+};
+
+static HostState save_state(const rgi_renderer* r)
+{
+    return HostState{r->read, r->specular_read, r->history_valid, r->di_valid, r->have_frame, r->nrc_pending,
+                     r->atlas_valid, r->frame_number, r->nrc_trained, r->previous, r->last, r->denoised};
+}
+
+static void restore_state(rgi_renderer* r, const HostState& h)
+{
+    r->read = h.read;
+    r->specular_read = h.specular_read;
+    r->history_valid = h.history_valid;
+    r->di_valid = h.di_valid;
+    r->have_frame = h.have_frame;
+    r->nrc_pending = h.nrc_pending;
+    r->atlas_valid = h.atlas_valid;
+    r->frame_number = h.frame_number;
+    r->nrc_trained = h.nrc_trained;
+    r->previous = h.previous;
+    r->last = h.last;
+    r->denoised = h.denoised;
+}
+
 // The caller's stream waits for a training step still running on the
 // cache's stream.
 static void join_nrc(rgi_renderer* r, hipStream_t stream)
@@ -610,6 +645,7 @@ int rgi_renderer_render_graph(rgi_renderer* r, const rgi_view* view, const rgi_g
     }
     hipStream_t stream = as_stream(stream_ptr);
     join_nrc(r, stream); // an eager frame's training, joined outside the capture
+    const HostState before = save_state(r);
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return fail(RGI_E_DEVICE, "render_graph: stream capture could not begin");
     int e = render_frame(r, view, g, stream_ptr, true, false);
@@ -617,6 +653,7 @@ int rgi_renderer_render_graph(rgi_renderer* r, const rgi_view* view, const rgi_g
     hipError_t ce = hipStreamEndCapture(stream, &graph);
     if (e != RGI_OK || ce != hipSuccess) {
         if (graph) hipGraphDestroy(graph);
+        restore_state(r, before);
         return e != RGI_OK ? e : fail(RGI_E_DEVICE, "render_graph: capture failed: %s", hipGetErrorString(ce));
     }
     // Same launches as last frame with new arguments: update in place;
@@ -633,10 +670,14 @@ int rgi_renderer_render_graph(rgi_renderer* r, const rgi_view* view, const rgi_g
     if (!r->graph_exec && hipGraphInstantiate(&r->graph_exec, graph, nullptr, nullptr, 0) != hipSuccess) {
         r->graph_exec = nullptr;
         hipGraphDestroy(graph);
+        restore_state(r, before);
         return fail(RGI_E_DEVICE, "render_graph: instantiation failed");
     }
     hipGraphDestroy(graph);
-    if (hipGraphLaunch(r->graph_exec, stream) != hipSuccess) return fail(RGI_E_DEVICE, "render_graph: launch failed");
+    if (hipGraphLaunch(r->graph_exec, stream) != hipSuccess) {
+        restore_state(r, before);
+        return fail(RGI_E_DEVICE, "render_graph: launch failed");
+    }
     return RGI_OK;
 }
 
@@ -854,6 +895,11 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
         return e;
     }
     for (size_t i = 0; i < n; ++i) join_nrc(b->renderers[i], stream); // eager frames' training, outside the capture
+    std::vector<HostState> before;
+    for (size_t i = 0; i < n; ++i) before.push_back(save_state(b->renderers[i]));
+    auto restore_all = [&]() {
+        for (size_t i = 0; i < n; ++i) restore_state(b->renderers[i], before[i]);
+    };
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return fail(RGI_E_DEVICE, "batch_render: stream capture could not begin");
     hipEventRecord(b->fork, stream);
@@ -869,6 +915,7 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
     hipError_t ce = hipStreamEndCapture(stream, &graph);
     if (e != RGI_OK || ce != hipSuccess) {
         if (graph) hipGraphDestroy(graph);
+        restore_all();
         return e != RGI_OK ? e : fail(RGI_E_DEVICE, "batch_render: capture failed: %s", hipGetErrorString(ce));
     }
     if (b->exec) {
@@ -883,10 +930,14 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
     if (!b->exec && hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
         b->exec = nullptr;
         hipGraphDestroy(graph);
+        restore_all();
         return fail(RGI_E_DEVICE, "batch_render: instantiation failed");
     }
     hipGraphDestroy(graph);
-    if (hipGraphLaunch(b->exec, stream) != hipSuccess) return fail(RGI_E_DEVICE, "batch_render: launch failed");
+    if (hipGraphLaunch(b->exec, stream) != hipSuccess) {
+        restore_all();
+        return fail(RGI_E_DEVICE, "batch_render: launch failed");
+    }
     return RGI_OK;
 }
 

@@ -88,7 +88,9 @@ RGI_DEV F3 hit_radiance(const GiPass& p, const uint16_t* previous_hdr, F3 hit, f
            tri_emission(p.v, tri) * f.lighting_emission;
 }
 
-// fs_specular.sc:127-226.
+// fs_specular.sc:127-226 (kSameSize: a G-buffer of the GI size, as in the GI
+// passes).
+template <bool kSameSize>
 __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                   const uint16_t* previous_hdr, const uint16_t* history,
                                                   uint16_t* out, F2 texel, TileMap tiles)
@@ -99,7 +101,7 @@ __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_
     p.v = scene_view(s);
     p.g = g;
     p.in = in;
-    set_gi_size(p, f);
+    set_gi_size<kSameSize>(p, f, g, texel);
     p.grid_camera = F3{f.camera_position[0], f.camera_position[1], f.camera_position[2]};
     p.sc = f.scene_scale;
 
@@ -219,8 +221,9 @@ int specular_gi(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, co
 {
     TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
     F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
-    hipLaunchKernelGGL(k_specular, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, previous_hdr, history, out,
-                       texel, tiles);
+    const bool same = g.width == f.gi_width && g.height == f.gi_height;
+    hipLaunchKernelGGL(same ? k_specular<true> : k_specular<false>, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in,
+                       previous_hdr, history, out, texel, tiles);
     return check_launch("specular gi");
 }
 

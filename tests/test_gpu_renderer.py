@@ -304,7 +304,8 @@ def test_render_graph_matches_eager(torch):
 def test_batch_matches_single_renderers(torch):
     """rgi_batch_render (renderers of different scenes and sizes in one HIP
     graph, each on its own stream) against each renderer rendered alone:
-    bit-identical outputs on every frame."""
+    bit-identical outputs on every frame, also after a captured batch frame
+    that failed part-way."""
     say("case batch vs single renderers")
     L = ra.lib()
     cases = [(common.cornell(extra_lights=4), 96, 64),
@@ -347,6 +348,16 @@ def test_batch_matches_single_renderers(torch):
                 ra.check(L.rgi_renderer_render(r, ctypes.byref(v), ctypes.byref(gb), es))
             va = (ra.View * len(views))(*views)
             ga = (ra.GBuffer * len(gbs))(*gbs)
+            if i == 2:
+                # A captured batch frame that fails part-way (the last
+                # renderer's G-buffer has the wrong size, after the others
+                # were captured) leaves every renderer as it was: the retry
+                # below must still match the renderers alone.
+                bad = list(gbs)
+                bad[-1] = ra.GBuffer(cases[-1][1] - 1, cases[-1][2], *(getattr(gbs[-1], k) for k in (
+                    "depth", "albedo", "normal", "shading_normal", "emission")))
+                assert L.rgi_batch_render(batch, va, (ra.GBuffer * len(bad))(*bad), bs) == ra.RGI_E_INVALID
+                torch.cuda.synchronize()
             ra.check(L.rgi_batch_render(batch, va, ga, bs))
             torch.cuda.synchronize()
             for k, ((hs, w, h), rb, rsg) in enumerate(zip(cases, batch_r, single_r)):
