@@ -55,7 +55,7 @@ def measured_traffic(pass_name, options, width, height):
     """HBM bytes per launch of the kernel variant the autotuner chose, from
     the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload
     (calibrated against a copy of known size), or None when none matches."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_cornell1080p.json")
+    path = os.path.join(ROOT, "profiles", "cornell1080p_kernels.json")
     if (width, height) != (1920, 1080) or not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -67,7 +67,7 @@ def measured_traffic(pass_name, options, width, height):
     if k is None:
         return None
     return {"bytes": round(k["fetch_bytes"] + k["write_bytes"]), "fetch_bytes": round(k["fetch_bytes"]),
-            "write_bytes": round(k["write_bytes"]), "kernel": kernel, "source": "profiles/hbm_traffic_cornell1080p.txt"}
+            "write_bytes": round(k["write_bytes"]), "kernel": kernel, "source": "profiles/cornell1080p_kernels.txt"}
 
 
 def cpu_model():
