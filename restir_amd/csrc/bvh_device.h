@@ -236,6 +236,53 @@ RGI_DEV int walk_short_stack(const SceneView& v, const Tri& test, F3 o, F3 d, fl
 // traceBvh's any-hit form: -1 when nothing is hit within max_distance;
 // `dist` ends at the first hit found (or max_distance). The reference's
 // Moeller-Trumbore test unless `watertight`.
+// walk_stackless's any-hit walk with the reference's triangle test, as a
+// resumable state: any_begin sets a ray up, each any_step visits one node;
+// `active` goes false when the walk ends, with `hit` set if a triangle is
+// closer than the max distance. The same nodes in the same order as
+// trace_any, so the answer is the same whichever lane runs the steps.
+struct AnyWalk {
+    F3 o, d, inv;
+    float dist;
+    int node, step;
+    bool active, hit;
+};
+
+RGI_DEV void any_begin(const SceneView& v, AnyWalk& w, F3 o, F3 d, float max_distance)
+{
+    w.o = o;
+    w.d = d;
+    w.inv = inverse_dir(d);
+    w.dist = max_distance;
+    w.node = 0;
+    w.step = 0;
+    w.hit = false;
+    w.active = bvh_ready(v);
+}
+
+RGI_DEV void any_step(const SceneView& v, AnyWalk& w)
+{
+    if (w.step >= kMaxSteps || w.node >= (int)v.node_count) {
+        w.active = false;
+        return;
+    }
+    const MollerTrumbore test{w.o, w.d};
+    F4 n0 = sv(v, (uint32_t)w.node * 2u);
+    F4 n1 = sv(v, (uint32_t)w.node * 2u + 1u);
+    bool leaf = n1.w >= 0.0f;
+    if (test.box(w.inv, n0, n1, w.dist) < 0.0f) {
+        w.node = leaf ? w.node + 1 : to_int(-n1.w) - 1;
+    } else {
+        if (leaf && leaf_hits(v, test, to_int(n0.w), to_int(n1.w), w.dist, -1) >= 0) {
+            w.hit = true;
+            w.active = false;
+            return;
+        }
+        w.node = w.node + 1;
+    }
+    ++w.step;
+}
+
 RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance, float& dist, bool watertight = false)
 {
     dist = max_distance;

@@ -181,6 +181,8 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
             }
             add_pair(p, mis, position, n, albedo, nb, rsel);
         }
+    }
+    if (working) {
         float rfinal = next_random(rng);
         combined_weight = finish_pairwise(p, mis, position, n, canonical, canonical_weight, rfinal);
     }
@@ -225,7 +227,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
 template <bool kSameSize>
 RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                              const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel,
-                             const TileMap& tiles)
+                             const TileMap& tiles, VisLds& vis)
 {
     GiPass p;
     p.f = &f;
@@ -272,30 +274,41 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     }
     float radius = reuse_radius(p, position);
     Pairwise mis = begin_pairwise(p, position, n, canonical, ccount, f.spatial_taps);
-    for (int tap = 0; tap < f.spatial_taps; ++tap) {
-        float r0 = next_random(rng);
-        float r1 = next_random(rng);
-        float r2 = next_random(rng);
-        float rsel = next_random(rng);
-        F2 nuv = F2{-1.0f, -1.0f};
-        if (f.world_space_reuse) {
-            uint32_t entry = grid_pick(f, in.grid, p.grid_camera, position, n, r0, r1, r2);
-            if (entry != kGridInvalid && entry != pixel_index) nuv = pixel_uv(p, entry);
-        } else {
-            float disk = 12.0f * sqrtf(r0);
-            float angle = (2.0f * kPi) * r1;
-            float sa, ca;
-            rgi_sincos(angle, &sa, &ca);
-            float nx = floorf(px + ca * disk), ny = floorf(py + sa * disk);
-            if (nx != px || ny != py) nuv = F2{(nx + 0.5f) * texel.x, (ny + 0.5f) * texel.y};
+    // Taps in chunks whose visibility marches run together (march_queue).
+    for (int t0 = 0; t0 < f.spatial_taps; t0 += kVisChunk) {
+        const int taps = min(kVisChunk, f.spatial_taps - t0);
+        uint32_t marches = 0u, walks = 0u;
+#pragma unroll
+        for (int j = 0; j < kVisChunk; ++j) {
+            if (j >= taps) break;
+            float r0 = next_random(rng);
+            float r1 = next_random(rng);
+            float r2 = next_random(rng);
+            float rsel = next_random(rng);
+            F2 nuv = F2{-1.0f, -1.0f};
+            if (f.world_space_reuse) {
+                uint32_t entry = grid_pick(f, in.grid, p.grid_camera, position, n, r0, r1, r2);
+                if (entry != kGridInvalid && entry != pixel_index) nuv = pixel_uv(p, entry);
+            } else {
+                float disk = 12.0f * sqrtf(r0);
+                float angle = (2.0f * kPi) * r1;
+                float sa, ca;
+                rgi_sincos(angle, &sa, &ca);
+                float nx = floorf(px + ca * disk), ny = floorf(py + sa * disk);
+                if (nx != px || ny != py) nuv = F2{(nx + 0.5f) * texel.x, (ny + 0.5f) * texel.y};
+            }
+            float ncount = -1.0f, ng = 0.0f, q_max = 0.0f;
+            int queue = kVisKnown;
+            F3 q_origin = splat3(0.0f), q_dir = splat3(0.0f);
+            Nb nb;
+            if (on_screen(nuv) && accept(p, fetch_f4(res.sample, p, nuv), fetch_f4(res.receiver, p, nuv),
+                                         fetch_f4(res.normal, p, nuv), position, n, radius, nb)) {
+                ncount = nb.count;
+                add_pair_deferred(p, mis, position, n, albedo, nb, rsel, ng, queue, q_origin, q_dir, q_max);
+            }
+            queue_tap(vis, j, marches, walks, queue, q_origin, q_dir, q_max, ncount, ng);
         }
-        Nb nb;
-        if (!on_screen(nuv) || !accept(p, fetch_f4(res.sample, p, nuv), fetch_f4(res.receiver, p, nuv),
-                                       fetch_f4(res.normal, p, nuv), position, n, radius, nb)) {
-            mis.c_share += mis.pair_c;
-            continue;
-        }
-        add_pair(p, mis, position, n, albedo, nb, rsel);
+        finish_chunk(p, vis, mis, taps, marches, walks);
     }
     float rfinal = next_random(rng);
     float cw = finish_pairwise(p, mis, position, n, canonical, cweight, rfinal);
@@ -365,7 +378,8 @@ __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rg
                                                     rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
                                                     F2 texel, TileMap tiles)
 {
-    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles);
+    __shared__ VisLds vis;
+    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, vis);
 }
 
 template <bool kSameSize>
@@ -373,7 +387,8 @@ __global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
                 uint16_t* out_moments, F2 texel, TileMap tiles)
 {
-    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles);
+    __shared__ VisLds vis;
+    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, vis);
 }
 
 } // namespace

@@ -216,11 +216,104 @@ RGI_DEV float depth_behind(const GiPass& p, const ClipRay& m, float t, F3& surfa
 // and a hit if the bisected depth is within the thickness; the ray stops on
 // a hit, on leaving the frustum, or past its max distance.
 //
-// Written as one depth lookup per loop iteration: a lane is either stepping
+// Written as one depth lookup per step: a lane is either stepping
 // (round < 0) or in bisection round `round`. Each lane evaluates the same
 // lookups in the same order as the nested step/bisection loops, so the
 // result is the same; a wave then iterates as often as its busiest lane,
-// instead of running every lane's bisections one after another.
+// instead of running every lane's bisections one after another. The state
+// is resumable, so a lane can also take over another lane's ray between
+// steps (march_queue). With kPosition the hit surface point is kept.
+struct MarchState {
+    ClipRay m;
+    float max_distance, previous, near_d, far_d, step_d, behind_b;
+    int i, round;
+    bool previous_front, hit, active;
+    F3 surface_b;
+};
+
+RGI_DEV float march_step_scale(const GiPass& p) { return 32.0f / (float)p.f->trace_steps; }
+
+RGI_DEV void march_begin(const GiPass& p, MarchState& s, F3 origin, F3 dir, float max_distance)
+{
+    s.m = clip_ray(p, origin, dir);
+    s.max_distance = max_distance;
+    s.i = 1;
+    s.round = -1;
+    s.previous = 0.0f;
+    s.near_d = 0.0f;
+    s.far_d = 0.0f;
+    s.step_d = 0.0f;
+    s.behind_b = 0.0f;
+    s.previous_front = true;
+    s.hit = false;
+    s.surface_b = splat3(0.0f);
+    s.active = s.i <= p.f->trace_steps && !(s.previous >= max_distance);
+}
+
+// One depth lookup of an active ray.
+template <bool kPosition>
+RGI_DEV void march_step(const GiPass& p, MarchState& s, float step_scale)
+{
+    const int steps = p.f->trace_steps;
+    const float sc = p.sc;
+    float t;
+    if (s.round < 0) {
+        float sp = (float)s.i * step_scale;
+        t = minf_((0.02f + (sp * sp) * 0.0035f) * sc, s.max_distance);
+    } else {
+        t = 0.5f * (s.near_d + s.far_d);
+    }
+    F3 surface;
+    bool visible;
+    float behind = depth_behind(p, s.m, t, surface, visible);
+    bool next_step = false;
+    if (s.round < 0) {
+        if (!visible) {
+            s.active = false;
+        } else {
+            bool front = behind <= 0.0f;
+            if (!front && s.previous_front) {
+                s.near_d = s.previous;
+                s.far_d = t;
+                s.step_d = t;
+                s.behind_b = behind;
+                if (kPosition) s.surface_b = surface;
+                s.round = 0;
+            } else {
+                s.previous_front = front;
+                s.previous = t;
+                next_step = true;
+            }
+        }
+    } else {
+        if (visible && behind > 0.0f) {
+            s.far_d = t;
+            s.behind_b = behind;
+            if (kPosition) s.surface_b = surface;
+        } else {
+            s.near_d = t;
+        }
+        if (++s.round == 5) {
+            if (s.behind_b < 0.03f * sc + 0.03f * s.far_d) {
+                s.hit = true;
+                s.active = false;
+            } else {
+                s.round = -1;
+                s.previous_front = false;
+                s.previous = s.step_d;
+                next_step = true;
+            }
+        }
+    }
+    if (next_step) {
+        ++s.i;
+        s.active = s.i <= steps && !(s.previous >= s.max_distance);
+    }
+}
+
+// The march of one ray to its end, per lane: the same lookups and decisions
+// as march_begin / march_step, written as one loop (the per-lane form keeps
+// the GI temporal pass at 4 waves per SIMD).
 RGI_DEV bool march(const GiPass& p, F3 origin, F3 dir, float max_distance, F3& hit_position)
 {
     ClipRay m = clip_ray(p, origin, dir);
@@ -350,6 +443,39 @@ RGI_DEV float geometry_term(const GiPass& p, F3 rpos, F3 rn, F3 spos, F3 sn, flo
     return rc > 0.0f && scos > 0.0f ? (rc * scos) / d2 : 0.0f;
 }
 
+// isVisible (restir.sh:564-606) as a ray to answer: kVisBvh for a BVH
+// any-hit ray (sky samples, and every ray in BVH mode; a hit means occluded),
+// kVisMarch for a march (a hit means occluded), or kVisKnown with `visible`
+// already set (a sky sample without a BVH is occluded; a march that takes no
+// step misses). `origin`, `dir` and `max_distance` are the ray's.
+constexpr int kVisKnown = 0, kVisMarch = 1, kVisBvh = 2;
+
+RGI_DEV int visibility_ray(const GiPass& p, F3 rpos, F3 rn, F3 spos, float sid, bool& visible, F3& origin, F3& dir,
+                           float& max_distance)
+{
+    float bias = 0.02f * p.sc;
+    float max_bvh = 4.0f * p.sc;
+    origin = rpos + rn * bias;
+    bool sky = is_sky(sid);
+    F3 to_sample = sky ? spos * max_bvh : spos - origin;
+    float sd = length(to_sample);
+    dir = to_sample / maxf_(sd, 0.0001f);
+    visible = true;
+    max_distance = 0.0f;
+    if (sky) {
+        visible = false;
+        max_distance = max_bvh;
+        return bvh_ready(p.v) ? kVisBvh : kVisKnown;
+    }
+    if (p.f->ray_mode == RGI_RAY_BVH && bvh_ready(p.v)) {
+        max_distance = sd - bias;
+        return kVisBvh;
+    }
+    max_distance = sd * 0.95f - 0.05f * p.sc;
+    // march_begin's first test: such a ray ends before its first lookup.
+    return 1 <= p.f->trace_steps && !(0.0f >= max_distance) ? kVisMarch : kVisKnown;
+}
+
 RGI_DEV bool visible_between(const GiPass& p, F3 rpos, F3 rn, F3 spos, float sid)
 {
     float bias = 0.02f * p.sc;
@@ -471,6 +597,73 @@ RGI_DEV Pairwise begin_pairwise(const GiPass& p, F3 position, F3 n, const Res& c
     return m;
 }
 
+// One accepted neighbour of the pairwise MIS (addNeighbour, restir.sh:833-893),
+// with the visibility of the canonical sample at the neighbour (the
+// canonical share's isVisible) left open: `ng` returns that term's geometry
+// factor, and when its visibility needs a ray `queue` is its kind (kVisMarch
+// or kVisBvh) with the ray (otherwise `ng` already includes the visibility). finish_pair_share adds
+// the term once the visibility is known. The shift of the neighbour's
+// sample to this pixel does not depend on it.
+RGI_DEV void add_pair_deferred(const GiPass& p, Pairwise& m, F3 position, F3 n, F3 albedo, const Nb& nb, float random,
+                               float& ng_out, int& queue, F3& q_origin, F3& q_dir, float& q_max)
+{
+    float ncount = nb.count;
+    float pair_weight = ncount + m.pair_c;
+    m.total += ncount;
+    // The shift first: it touches neither the canonical share nor the
+    // canonical sample, so the order does not change a result, and the
+    // visibility ray is then built just before it is queued.
+    F3 contribution = splat3(0.0f);
+    F3 sn = F3{0.0f, 0.0f, 1.0f};
+    bool shifted = false;
+    if (nb.weight > 0.0f) shifted = shade_sample(p, position, n, albedo, nb.sample_position, nb.sample_id, contribution, sn);
+    if (shifted) {
+        float target = luminance(contribution);
+        float j = jacobian(p, position, nb.receiver_position, nb.sample_position, sn, nb.sample_id);
+        float hg = geometry_term(p, position, n, nb.sample_position, sn, nb.sample_id);
+        float tg = geometry_term(p, nb.receiver_position, nb.receiver_normal, nb.sample_position, sn, nb.sample_id);
+        float nd = ncount * tg;
+        float den = nd + m.pair_c * hg;
+        float mis = den > 0.0f ? (pair_weight * nd) / den : 0.0f;
+        float w = ((mis * target) * nb.weight) * j;
+        if (w > 0.0f && stream_in(m.combined, contribution, nb.sample_position, sn, nb.sample_id, target, w, random))
+            m.neighbour_selected = true;
+    }
+    ng_out = 0.0f;
+    queue = kVisKnown;
+    if (m.has_c) {
+        float ng = geometry_term(p, nb.receiver_position, nb.receiver_normal, m.c_position, m.c_normal, m.c_id);
+        if (ng > 0.0f) {
+            bool visible;
+            queue = visibility_ray(p, nb.receiver_position, nb.receiver_normal, m.c_position, m.c_id, visible, q_origin,
+                                   q_dir, q_max);
+            if (queue == kVisKnown && !visible) ng = 0.0f;
+        }
+        ng_out = ng;
+    }
+}
+
+// The canonical sample's MIS share from one tap, in tap order: a rejected tap
+// (ncount < 0) gives its whole pair weight; an accepted one the balance
+// between the canonical and the neighbour's density, with `ng` zero when the
+// canonical sample is occluded at the neighbour.
+RGI_DEV void finish_pair_share(Pairwise& m, float ncount, float ng)
+{
+    if (ncount < 0.0f) {
+        m.c_share += m.pair_c;
+        return;
+    }
+    if (!m.has_c) return;
+    float pair_weight = ncount + m.pair_c;
+    float cd = m.pair_c * m.c_geometry;
+    float den = ncount * ng + cd;
+    m.c_share += den > 0.0f ? (pair_weight * cd) / den : pair_weight;
+}
+
+// One accepted neighbour of the pairwise MIS (addNeighbour,
+// restir.sh:833-893) with its visibility answered at once: the temporal pass,
+// whose waves hold too few visibility rays to gain from answering them
+// together (add_pair_deferred).
 RGI_DEV void add_pair(const GiPass& p, Pairwise& m, F3 position, F3 n, F3 albedo, const Nb& nb, float random)
 {
     float ncount = nb.count;
@@ -501,6 +694,110 @@ RGI_DEV void add_pair(const GiPass& p, Pairwise& m, F3 position, F3 n, F3 albedo
         if (w > 0.0f && stream_in(m.combined, contribution, nb.sample_position, sn, nb.sample_id, target, w, random))
             m.neighbour_selected = true;
     }
+}
+
+// The visibility rays of up to kVisChunk taps of a wave, answered together:
+// march rays fill each wave's queue from the front, BVH rays from the back.
+constexpr int kVisChunk = 2;
+constexpr int kVisQueue = 64 * kVisChunk;
+constexpr uint16_t kNoSlot = 0xffffu;
+
+struct VisLds {
+    float4 origin[4][kVisQueue];      // per wave of the block: ray origin, max distance
+    float4 dir[4][kVisQueue];         // direction; w = hit (written when the ray is answered)
+    float2 rec[kVisChunk][256];       // per tap and thread: neighbour count (-1: rejected), geometry factor
+    uint16_t slot[kVisChunk][256];    // per tap and thread: the queued ray, or kNoSlot
+};
+
+// Answers this wave's queued rays [first, first + count) of one kind with
+// every active lane: a lane whose ray ends takes the next ray not yet
+// started, so the wave takes about (all the rays' steps) / 64 iterations
+// instead of the sum over taps of each tap's longest ray. Each ray's answer
+// is the same whichever lane computes it; its hit goes to dir[k].w. `first`
+// and `count` are wave-uniform.
+template <int kKind>
+RGI_DEV void answer_queue(const GiPass& p, float4* origin, float4* dir, uint32_t first, uint32_t count)
+{
+    const float step_scale = march_step_scale(p);
+    MarchState s;
+    AnyWalk w;
+    s.active = false;
+    s.hit = false;
+    w.active = false;
+    w.hit = false;
+    uint32_t k = 0u, next = 0u;
+    bool busy = false;
+    for (;;) {
+        bool active = kKind == kVisMarch ? s.active : w.active;
+        if (busy && !active) {
+            dir[first + k].w = (kKind == kVisMarch ? s.hit : w.hit) ? 1.0f : 0.0f;
+            busy = false;
+        }
+        if (next < count) {
+            uint64_t idle = __ballot(!busy);
+            uint32_t slot = next + rank_in(idle);
+            if (!busy && slot < count) {
+                k = slot;
+                busy = true;
+                float4 o = origin[first + k], d = dir[first + k];
+                if (kKind == kVisMarch) march_begin(p, s, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, o.w);
+                else any_begin(p.v, w, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, o.w);
+            }
+            next += (uint32_t)__popcll(idle);
+        }
+        if (__ballot(busy) == 0ull) break;
+        if (kKind == kVisMarch) {
+            if (busy && s.active) march_step<false>(p, s, step_scale);
+        } else {
+            if (busy && w.active) any_step(p.v, w);
+        }
+    }
+}
+
+// Queues one tap's visibility ray (every active lane of the wave calls it,
+// with or without a ray; `marches` and `walks` are the wave's queue counts,
+// uniform) and records the tap.
+RGI_DEV void queue_tap(VisLds& vis, int j, uint32_t& marches, uint32_t& walks, int kind, F3 o, F3 d,
+                       float max_distance, float ncount, float ng)
+{
+    const uint32_t wave = threadIdx.x >> 6;
+    uint64_t want_march = __ballot(kind == kVisMarch), want_walk = __ballot(kind == kVisBvh);
+    uint32_t slot = kNoSlot;
+    if (kind == kVisMarch) slot = marches + rank_in(want_march);
+    if (kind == kVisBvh) slot = (uint32_t)kVisQueue - 1u - (walks + rank_in(want_walk));
+    if (slot != kNoSlot) {
+        vis.origin[wave][slot] = make_float4(o.x, o.y, o.z, max_distance);
+        vis.dir[wave][slot] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
+    marches += (uint32_t)__popcll(want_march);
+    walks += (uint32_t)__popcll(want_walk);
+    vis.rec[j][threadIdx.x] = make_float2(ncount, ng);
+    vis.slot[j][threadIdx.x] = (uint16_t)slot;
+}
+
+// Answers the chunk's queued rays and adds the chunk's MIS shares in tap
+// order.
+RGI_DEV void finish_chunk(const GiPass& p, VisLds& vis, Pairwise& m, int taps, uint32_t marches, uint32_t walks)
+{
+    const uint32_t wave = threadIdx.x >> 6;
+    if (marches + walks != 0u) {
+        wave_sync();
+        if (walks != 0u)
+            answer_queue<kVisBvh>(p, vis.origin[wave], vis.dir[wave], (uint32_t)kVisQueue - walks, walks);
+        if (marches != 0u) answer_queue<kVisMarch>(p, vis.origin[wave], vis.dir[wave], 0u, marches);
+        wave_sync();
+    }
+#pragma unroll
+    for (int j = 0; j < kVisChunk; ++j) {
+        if (j < taps) {
+            float2 r = vis.rec[j][threadIdx.x];
+            uint32_t slot = vis.slot[j][threadIdx.x];
+            float ng = r.y;
+            if (slot != kNoSlot && vis.dir[wave][slot].w != 0.0f) ng = 0.0f;
+            finish_pair_share(m, r.x, ng);
+        }
+    }
+    wave_sync();
 }
 
 RGI_DEV float finish_pairwise(const GiPass& p, Pairwise& m, F3 position, F3 n, const Res& c, float cweight, float random)
