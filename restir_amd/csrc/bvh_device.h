@@ -127,7 +127,6 @@ RGI_DEV float watertight_hit(const WatertightRay& r, F3 v0, F3 v1, F3 v2)
     return t;
 }
 
-// The two triangle tests as traversal functors.
 // The two triangle tests as traversal functors, each with its box test.
 struct MollerTrumbore {
     F3 o, d;
@@ -142,6 +141,11 @@ struct Watertight {
         return box_enter_conservative(r.o, inv, lo, hi, max_distance);
     }
 };
+
+// A triangle test set up for one ray.
+template <class Tri> RGI_DEV Tri triangle_test(F3 o, F3 d);
+template <> RGI_DEV MollerTrumbore triangle_test<MollerTrumbore>(F3 o, F3 d) { return MollerTrumbore{o, d}; }
+template <> RGI_DEV Watertight triangle_test<Watertight>(F3 o, F3 d) { return Watertight{watertight_ray(o, d)}; }
 
 RGI_DEV float safe_dir(float d) { return fabsf(d) > 1e-8f ? d : (d >= 0.0f ? 1e-8f : -1e-8f); }
 RGI_DEV F3 inverse_dir(F3 d) { return F3{1.0f / safe_dir(d.x), 1.0f / safe_dir(d.y), 1.0f / safe_dir(d.z)}; }
@@ -233,56 +237,163 @@ RGI_DEV int walk_short_stack(const SceneView& v, const Tri& test, F3 o, F3 d, fl
     return hit;
 }
 
-// traceBvh's any-hit form: -1 when nothing is hit within max_distance;
-// `dist` ends at the first hit found (or max_distance). The reference's
-// Moeller-Trumbore test unless `watertight`.
-// walk_stackless's any-hit walk with the reference's triangle test, as a
-// resumable state: any_begin sets a ray up, each any_step visits one node;
-// `active` goes false when the walk ends, with `hit` set if a triangle is
-// closer than the max distance. The same nodes in the same order as
-// trace_any, so the answer is the same whichever lane runs the steps.
+// The two walks as resumable states, so that a lane can take up another
+// ray between node visits (a queue of rays answered by every lane of a wave).
+// Each visits the same nodes in the same order, and keeps the same distance
+// and hit, as walk_stackless / walk_short_stack, whichever lane runs it.
+//
+// AnyWalk: walk_stackless's any-hit walk. any_begin sets a ray up, each
+// any_step visits one node; `active` goes false when the walk ends, `hit` is
+// the triangle found (or -1) and `dist` the distance as trace_any leaves it.
+template <class Tri>
 struct AnyWalk {
-    F3 o, d, inv;
+    Tri test;
+    F3 inv;
     float dist;
-    int node, step;
-    bool active, hit;
+    int node, step, hit;
+    bool active;
 };
 
-RGI_DEV void any_begin(const SceneView& v, AnyWalk& w, F3 o, F3 d, float max_distance)
+template <class Tri>
+RGI_DEV void any_begin(const SceneView& v, AnyWalk<Tri>& w, const Tri& test, F3 d, float max_distance)
 {
-    w.o = o;
-    w.d = d;
+    w.test = test;
     w.inv = inverse_dir(d);
     w.dist = max_distance;
     w.node = 0;
     w.step = 0;
-    w.hit = false;
+    w.hit = -1;
     w.active = bvh_ready(v);
 }
 
-RGI_DEV void any_step(const SceneView& v, AnyWalk& w)
+template <class Tri>
+RGI_DEV void any_step(const SceneView& v, AnyWalk<Tri>& w)
 {
     if (w.step >= kMaxSteps || w.node >= (int)v.node_count) {
         w.active = false;
         return;
     }
-    const MollerTrumbore test{w.o, w.d};
     F4 n0 = sv(v, (uint32_t)w.node * 2u);
     F4 n1 = sv(v, (uint32_t)w.node * 2u + 1u);
     bool leaf = n1.w >= 0.0f;
-    if (test.box(w.inv, n0, n1, w.dist) < 0.0f) {
+    ++w.step;
+    if (w.test.box(w.inv, n0, n1, w.dist) < 0.0f) {
         w.node = leaf ? w.node + 1 : to_int(-n1.w) - 1;
-    } else {
-        if (leaf && leaf_hits(v, test, to_int(n0.w), to_int(n1.w), w.dist, -1) >= 0) {
-            w.hit = true;
+        return;
+    }
+    if (leaf) {
+        w.hit = leaf_hits(v, w.test, to_int(n0.w), to_int(n1.w), w.dist, w.hit);
+        if (w.hit >= 0) {
             w.active = false;
             return;
         }
-        w.node = w.node + 1;
     }
-    ++w.step;
+    w.node = w.node + 1;
 }
 
+// ClosestWalk: walk_short_stack's closest-hit walk, then its stackless
+// re-pass when the ring dropped an entry. closest_step visits one node of
+// whichever phase is running; `ring` and `stride` as for walk_short_stack.
+template <class Tri>
+struct ClosestWalk {
+    Tri test;
+    F3 inv;
+    float dist;
+    int hit, node, step, top, held;
+    bool dropped, repass, active;
+};
+
+template <class Tri>
+RGI_DEV void closest_begin(const SceneView& v, ClosestWalk<Tri>& w, const Tri& test, F3 d, float max_distance)
+{
+    w.test = test;
+    w.inv = inverse_dir(d);
+    w.dist = max_distance;
+    w.hit = -1;
+    w.node = 0;
+    w.step = 0;
+    w.top = 0;
+    w.held = 0;
+    w.dropped = false;
+    w.repass = false;
+    w.active = bvh_ready(v) && test.box(w.inv, sv(v, 0), sv(v, 1), w.dist) >= 0.0f;
+}
+
+template <class Tri>
+RGI_DEV void closest_end_stack(ClosestWalk<Tri>& w)
+{
+    if (w.dropped) {
+        w.repass = true;
+        w.node = 0;
+        w.step = 0;
+    } else {
+        w.active = false;
+    }
+}
+
+template <class Tri>
+RGI_DEV void closest_step(const SceneView& v, ClosestWalk<Tri>& w, int* ring, int stride)
+{
+    if (w.repass) {
+        // walk_stackless, closest hit, continuing from the stack walk's
+        // distance and hit.
+        if (w.step >= kMaxSteps || w.node >= (int)v.node_count) {
+            w.active = false;
+            return;
+        }
+        F4 n0 = sv(v, (uint32_t)w.node * 2u);
+        F4 n1 = sv(v, (uint32_t)w.node * 2u + 1u);
+        bool leaf = n1.w >= 0.0f;
+        ++w.step;
+        if (w.test.box(w.inv, n0, n1, w.dist) < 0.0f) {
+            w.node = leaf ? w.node + 1 : to_int(-n1.w) - 1;
+            return;
+        }
+        if (leaf) w.hit = leaf_hits(v, w.test, to_int(n0.w), to_int(n1.w), w.dist, w.hit);
+        w.node = w.node + 1;
+        return;
+    }
+    if (w.step >= kMaxSteps) {
+        closest_end_stack(w);
+        return;
+    }
+    ++w.step;
+    F4 n0 = sv(v, (uint32_t)w.node * 2u);
+    F4 n1 = sv(v, (uint32_t)w.node * 2u + 1u);
+    if (n1.w >= 0.0f) {
+        w.hit = leaf_hits(v, w.test, to_int(n0.w), to_int(n1.w), w.dist, w.hit);
+    } else {
+        int left = w.node + 1;
+        int right = to_int(n0.w);
+        float ld = w.test.box(w.inv, sv(v, (uint32_t)left * 2u), sv(v, (uint32_t)left * 2u + 1u), w.dist);
+        float rd = w.test.box(w.inv, sv(v, (uint32_t)right * 2u), sv(v, (uint32_t)right * 2u + 1u), w.dist);
+        bool vl = ld >= 0.0f, vr = rd >= 0.0f;
+        if (vl && vr) {
+            bool left_first = ld <= rd;
+            ring[(w.top & (kStackRing - 1)) * stride] = left_first ? right : left;
+            ++w.top;
+            if (w.held == kStackRing) w.dropped = true;
+            else ++w.held;
+            w.node = left_first ? left : right;
+            return;
+        }
+        if (vl || vr) {
+            w.node = vl ? left : right;
+            return;
+        }
+    }
+    if (w.held == 0) {
+        closest_end_stack(w);
+        return;
+    }
+    --w.top;
+    --w.held;
+    w.node = ring[(w.top & (kStackRing - 1)) * stride];
+}
+
+// traceBvh's any-hit form: -1 when nothing is hit within max_distance;
+// `dist` ends at the first hit found (or max_distance). The reference's
+// Moeller-Trumbore test unless `watertight`.
 RGI_DEV int trace_any(const SceneView& v, F3 o, F3 d, float max_distance, float& dist, bool watertight = false)
 {
     dist = max_distance;

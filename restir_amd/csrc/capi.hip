@@ -202,7 +202,7 @@ int rgi_context_set_option(rgi_context* ctx, int option, int value)
         return RGI_OK;
     case RGI_OPT_TRACE_PERSISTENT:
         if (value != 0 && value != 1) return fail(RGI_E_INVALID, "persistent option takes 0 or 1, not %d", value);
-        if (value && !ctx->counter && hipMalloc((void**)&ctx->counter, 256) != hipSuccess)
+        if (value && !ctx->counter && hipMalloc((void**)&ctx->counter, 1024) != hipSuccess)
             return fail(RGI_E_NOMEM, "work counter allocation failed");
         ctx->tuning.trace_persistent = value;
         return RGI_OK;

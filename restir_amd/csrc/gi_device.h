@@ -720,17 +720,17 @@ RGI_DEV void answer_queue(const GiPass& p, float4* origin, float4* dir, uint32_t
 {
     const float step_scale = march_step_scale(p);
     MarchState s;
-    AnyWalk w;
+    AnyWalk<MollerTrumbore> w;
     s.active = false;
     s.hit = false;
     w.active = false;
-    w.hit = false;
+    w.hit = -1;
     uint32_t k = 0u, next = 0u;
     bool busy = false;
     for (;;) {
         bool active = kKind == kVisMarch ? s.active : w.active;
         if (busy && !active) {
-            dir[first + k].w = (kKind == kVisMarch ? s.hit : w.hit) ? 1.0f : 0.0f;
+            dir[first + k].w = (kKind == kVisMarch ? s.hit : w.hit >= 0) ? 1.0f : 0.0f;
             busy = false;
         }
         if (next < count) {
@@ -741,7 +741,7 @@ RGI_DEV void answer_queue(const GiPass& p, float4* origin, float4* dir, uint32_t
                 busy = true;
                 float4 o = origin[first + k], d = dir[first + k];
                 if (kKind == kVisMarch) march_begin(p, s, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, o.w);
-                else any_begin(p.v, w, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, o.w);
+                else any_begin(p.v, w, MollerTrumbore{F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}}, F3{d.x, d.y, d.z}, o.w);
             }
             next += (uint32_t)__popcll(idle);
         }

@@ -1,11 +1,16 @@
 // Ray batches through the BVH (rgi_trace) and the primary-ray G-buffer
 // producer (rgi_gbuffer_primary).
 //
-// rgi_trace keeps every lane busy with a persistent loop: each wave takes the
-// next 64 rays from a global counter when all its lanes are done, so waves
-// that drew short rays move on instead of idling behind long ones. Closest-hit
-// rays keep their 8-entry ring stack in LDS, laid out slot-major so that the
-// 64 lanes of a wave hit 64 different banks.
+// rgi_trace has two launch forms, bit-exact with each other:
+//   - one ray per lane;
+//   - persistent waves (RGI_OPT_TRACE_PERSISTENT): each wave claims chunks of
+//     rays and answers them with every lane, a lane whose ray ends taking the
+//     next ray of the wave's chunk at once, so no lane idles behind a long
+//     traversal while rays remain. Chunks are dealt to 8 counters (workgroup
+//     b uses counter b mod 8), since one counter serves only about one claim
+//     per 13 ns.
+// Closest-hit rays keep their 8-entry ring stack in LDS, laid out slot-major
+// so that the 64 lanes of a wave hit 64 different banks.
 #include "bvh_device.h"
 #include "internal.h"
 
@@ -34,30 +39,80 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4
     dist[i] = t;
 }
 
-// Persistent form: waves pull 64-ray batches until the counter passes count;
-// every wave reaches the exit test each iteration, so the grid always drains.
+// Persistent form. Group g (workgroups b = g mod kTraceGroups) claims
+// chunks g, g + kTraceGroups, ... of kTraceChunk rays from its own counter
+// (each on a 128-byte line) until they pass `count`; every wave reaches that
+// exit test, so the grid always drains. The counters are zeroed before the
+// launch.
+constexpr uint32_t kTraceGroups = 8, kTraceChunk = 256, kTraceLine = 32;
+
+// Lanes keep traversing while at least kRefillBelow of them are busy; below
+// that the wave stops to hand the idle lanes new rays (Aila and Laine 2009's
+// dynamic ray fetch), so the hand-out costs little per node visited.
+constexpr int kRefillBelow = 40;
+
+template <class Tri, bool kAny>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_persistent(rgi_scene s, const float4* rays, uint32_t count,
-                                                                  int32_t* hit, float* dist, int any_hit,
-                                                                  int watertight, uint32_t* next_batch)
+                                                                  int32_t* hit, float* dist, uint32_t* counters)
 {
     __shared__ int ring[kStackRing * kTraceBlock];
     SceneView v = scene_view(s);
-    uint32_t lane = threadIdx.x & 63u;
+    const uint32_t group = blockIdx.x % kTraceGroups;
+    uint32_t* counter = counters + group * kTraceLine;
+    AnyWalk<Tri> aw;
+    ClosestWalk<Tri> cw;
+    aw.active = false;
+    cw.active = false;
+    uint32_t k = 0u, cur = 0u, end = 0u;
+    bool busy = false, exhausted = false;
     for (;;) {
-        uint32_t batch = 0;
-        if (lane == 0) batch = atomicAdd(next_batch, 64u);
-        batch = __builtin_amdgcn_readfirstlane(batch);
-        if (batch >= count) break;
-        uint32_t i = batch + lane;
-        if (i < count) {
-            float4 a = rays[2u * i], b = rays[2u * i + 1u];
-            F3 o = F3{a.x, a.y, a.z}, d = F3{b.x, b.y, b.z};
-            float t = a.w;
-            int h;
-            if (any_hit) h = trace_any(v, o, d, a.w, t, watertight != 0);
-            else h = trace_closest(v, o, d, a.w, t, ring + threadIdx.x, kTraceBlock, watertight != 0);
-            hit[i] = h;
-            dist[i] = t;
+        // Hand the idle lanes the next rays of the wave's chunk (claiming a
+        // new chunk when it is used up).
+        uint64_t idle = __ballot(!busy);
+        while (idle != 0ull && !exhausted) {
+            if (cur >= end) {
+                uint32_t c = 0u;
+                if ((threadIdx.x & 63u) == 0u) c = atomicAdd(counter, 1u);
+                uint32_t first = ((uint32_t)__shfl((int)c, 0) * kTraceGroups + group) * kTraceChunk;
+                if (first >= count) {
+                    exhausted = true;
+                    break;
+                }
+                cur = first;
+                end = count - first < kTraceChunk ? count : first + kTraceChunk;
+            }
+            uint32_t ray = cur + rank_in(idle);
+            bool take = !busy && ray < end;
+            if (take) {
+                k = ray;
+                busy = true;
+                float4 a = rays[2u * k], b = rays[2u * k + 1u];
+                F3 o = F3{a.x, a.y, a.z}, d = F3{b.x, b.y, b.z};
+                Tri test = triangle_test<Tri>(o, d);
+                if (kAny) any_begin(v, aw, test, d, a.w);
+                else closest_begin(v, cw, test, d, a.w);
+            }
+            uint32_t taken = (uint32_t)__popcll(idle);
+            cur = end - cur < taken ? end : cur + taken;
+            idle = __ballot(!busy);
+        }
+        const int keep = exhausted ? 1 : kRefillBelow;
+        if (__popcll(__ballot(busy)) == 0) break;
+        // Traverse until fewer than `keep` lanes are busy.
+        for (;;) {
+            if (busy) {
+                if (kAny) {
+                    if (aw.active) any_step(v, aw);
+                } else {
+                    if (cw.active) closest_step(v, cw, ring + threadIdx.x, kTraceBlock);
+                }
+                if (!(kAny ? aw.active : cw.active)) {
+                    hit[k] = kAny ? aw.hit : cw.hit;
+                    dist[k] = kAny ? aw.dist : cw.dist;
+                    busy = false;
+                }
+            }
+            if (__popcll(__ballot(busy)) < keep) break;
         }
     }
 }
@@ -160,7 +215,10 @@ __global__ void __launch_bounds__(256) k_shadow_atlas(rgi_shadow sh, rgi_scene s
 
 } // namespace
 
-__global__ void k_zero_counter(uint32_t* counter) { *counter = 0u; }
+__global__ void k_zero_counters(uint32_t* counters)
+{
+    if (threadIdx.x < kTraceGroups) counters[threadIdx.x * kTraceLine] = 0u;
+}
 
 int shadow_atlas(const rgi_shadow& sh, const rgi_scene& s, float* atlas, hipStream_t stream)
 {
@@ -175,17 +233,21 @@ int trace_rays(const rgi_scene& s, const float* rays, uint32_t count, int32_t* h
     if (count == 0) return RGI_OK;
     const float4* r = reinterpret_cast<const float4*>(rays);
     if (tuning.trace_persistent && counter) {
-        // Enough waves to fill every SIMD of the 256 CUs several times over;
-        // each exits once the batch counter passes `count`.
-        int device = 0, cus = 256;
-        hipGetDevice(&device);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-        uint32_t blocks = (uint32_t)cus * 8u;
-        uint32_t needed = (count + kTraceBlock - 1u) / kTraceBlock;
-        blocks = blocks < needed ? blocks : needed;
-        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
-        hipLaunchKernelGGL(k_trace_persistent, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist,
-                           any_hit, tuning.watertight, counter);
+        // As many workgroups as run at once (at least one per group).
+        auto kernel = any_hit ? (tuning.watertight ? k_trace_persistent<Watertight, true>
+                                                   : k_trace_persistent<MollerTrumbore, true>)
+                              : (tuning.watertight ? k_trace_persistent<Watertight, false>
+                                                   : k_trace_persistent<MollerTrumbore, false>);
+        int device = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&device) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, 0) != hipSuccess ||
+            cus <= 0 || per_cu <= 0)
+            return fail(RGI_E_DEVICE, "trace: occupancy query failed");
+        uint32_t blocks = (uint32_t)cus * (uint32_t)per_cu;
+        if (blocks < kTraceGroups) blocks = kTraceGroups;
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, stream, counter);
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist, counter);
     } else {
         uint32_t blocks = (count + kTraceBlock - 1u) / kTraceBlock;
         hipLaunchKernelGGL(k_trace, dim3(blocks), dim3(kTraceBlock), 0, stream, s, r, count, hit, dist, any_hit,

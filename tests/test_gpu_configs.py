@@ -222,8 +222,9 @@ def test_c2_atomic_insert_1080p_invariants(torch):
 
 def test_c2_traversal_2pow24_rays(torch):
     """The traversal microbenchmark's 2^24 cosine rays from points uniform
-    over the Cornell box's area, closest hit: hit triangle ids and
-    distances bit-exact against the oracle."""
+    over the Cornell box's area, closest hit, in both launch forms (one ray
+    per lane, persistent waves): hit triangle ids and distances bit-exact
+    against the oracle."""
     say("case C2 traversal 2^24 rays")
     hs = common.cornell(extra_lights=4)
     rays = surface_rays(hs, 1 << 24, seed=7)
@@ -234,14 +235,20 @@ def test_c2_traversal_2pow24_rays(torch):
     try:
         scene = d.scene(hs)
         rt = d.put(rays)
-        ht, hd = d.empty(4 * n), d.empty(4 * n)
-        ra.check(d.L.rgi_trace(d.ctx, ctypes.byref(scene), rt.data_ptr(), n, ht.data_ptr(), hd.data_ptr(), 0,
-                               d.stream))
-        got_t, got_d = d.get(ht, want_t), d.get(hd, want_d)
+        got = []
+        for persistent in (0, 1):
+            ra.check(d.L.rgi_context_set_option(d.ctx, ra.RGI_OPT_TRACE_PERSISTENT, persistent))
+            ht, hd = d.empty(4 * n), d.empty(4 * n)
+            ra.check(d.L.rgi_trace(d.ctx, ctypes.byref(scene), rt.data_ptr(), n, ht.data_ptr(), hd.data_ptr(), 0,
+                                   d.stream))
+            got.append((d.get(ht, want_t), d.get(hd, want_d)))
+            del ht, hd
     finally:
         d.close()
-    assert np.array_equal(got_t, want_t), "%d hit ids differ" % int((got_t != want_t).sum())
-    assert np.array_equal(got_d, want_d)
+    for persistent, (got_t, got_d) in enumerate(got):
+        assert np.array_equal(got_t, want_t), "persistent=%d: %d hit ids differ" % (persistent,
+                                                                                    int((got_t != want_t).sum()))
+        assert np.array_equal(got_d, want_d), "persistent=%d" % persistent
     say("  hit rate %.4f" % (want_t >= 0).mean())
 
 

@@ -145,7 +145,8 @@ def trace_both(dev, hs, rays, any_hit, watertight=False):
     (ra.RGI_SCENE_CORNELL, 0, False, 0), (ra.RGI_SCENE_CORNELL, 0, True, 0),
     (ra.RGI_SCENE_BOXFIELD, 3000, False, 0), (ra.RGI_SCENE_BOXFIELD, 3000, True, 0),
     (ra.RGI_SCENE_ATRIUM, 60000, False, 0), (ra.RGI_SCENE_ATRIUM, 60000, True, 0),
-    (ra.RGI_SCENE_BOXFIELD, 3000, False, 1), (ra.RGI_SCENE_CORNELL, 0, True, 1)])
+    (ra.RGI_SCENE_BOXFIELD, 3000, False, 1), (ra.RGI_SCENE_CORNELL, 0, True, 1), (ra.RGI_SCENE_CORNELL, 0, False, 1),
+    (ra.RGI_SCENE_ATRIUM, 60000, False, 1), (ra.RGI_SCENE_ATRIUM, 60000, True, 1)])
 def test_trace(dev, kind, size, any_hit, persistent):
     say("case trace kind=%d size=%d any_hit=%d persistent=%d" % (kind, size, any_hit, persistent))
     hs = ra.HostScene(kind, seed=5, size=size)
@@ -162,7 +163,7 @@ def test_trace(dev, kind, size, any_hit, persistent):
 
 @pytest.mark.parametrize("kind,size,any_hit,persistent", [
     (ra.RGI_SCENE_CORNELL, 0, False, 0), (ra.RGI_SCENE_BOXFIELD, 3000, True, 0),
-    (ra.RGI_SCENE_ATRIUM, 60000, False, 1)])
+    (ra.RGI_SCENE_ATRIUM, 60000, False, 1), (ra.RGI_SCENE_ATRIUM, 60000, True, 1)])
 def test_trace_watertight(dev, kind, size, any_hit, persistent):
     """rgi_trace with the watertight triangle test: hit ids and distances
     bit-exact against the oracle's twin of the test."""
