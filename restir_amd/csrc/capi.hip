@@ -85,6 +85,34 @@ void* context_scratch(rgi_context* ctx, size_t bytes)
     return ctx->scratch;
 }
 
+uint32_t* context_tile_list(rgi_context* ctx, size_t words)
+{
+    if (ctx->tile_list_words >= words) return ctx->tile_list;
+    if (ctx->tile_list) {
+        hipDeviceSynchronize();
+        hipFree(ctx->tile_list);
+        ctx->tile_list = nullptr;
+        ctx->tile_list_words = 0;
+    }
+    if (hipMalloc(&ctx->tile_list, words * sizeof(uint32_t)) != hipSuccess) {
+        ctx->tile_list = nullptr;
+        fail(RGI_E_NOMEM, "tile list allocation of %zu words failed", words);
+        return nullptr;
+    }
+    ctx->tile_list_words = words;
+    return ctx->tile_list;
+}
+
+int context_compute_units(rgi_context* ctx)
+{
+    if (ctx->compute_units <= 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || n <= 0) n = 256;
+        ctx->compute_units = n;
+    }
+    return ctx->compute_units;
+}
+
 static bool pow2(uint32_t v) { return v != 0 && (v & (v - 1)) == 0; }
 static bool aligned(const void* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; }
 
@@ -185,8 +213,10 @@ int rgi_create(rgi_context** out)
 void rgi_destroy(rgi_context* ctx)
 {
     if (!ctx) return;
-    if (ctx->scratch || ctx->counter) hipDeviceSynchronize();
+    if (ctx->scratch || ctx->counter || ctx->tile_list || ctx->cand) hipDeviceSynchronize();
     if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->tile_list) hipFree(ctx->tile_list);
+    if (ctx->cand) hipFree(ctx->cand);
     if (ctx->counter) hipFree(ctx->counter);
     delete ctx;
 }
@@ -295,7 +325,7 @@ int rgi_gi_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* s
     if (frame->shadows_enabled && !in->shadow_map) return fail(RGI_E_INVALID, "shadows enabled but shadow_map is NULL");
     if (!out->sample || !out->receiver || !out->normal) return fail(RGI_E_INVALID, "gi temporal: output planes are required");
     if ((e = check_gi_gbuffer(frame, g)) != RGI_OK) return e;
-    return gi_temporal(*frame, *scene, *g, *in, *out, ctx ? ctx->tuning : Tuning{}, as_stream(stream));
+    return gi_temporal(ctx, *frame, *scene, *g, *in, *out, ctx ? ctx->tuning : Tuning{}, as_stream(stream));
 }
 
 int rgi_gi_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,

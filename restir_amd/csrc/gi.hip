@@ -18,14 +18,23 @@
 namespace rgi {
 namespace {
 
-// Per-block LDS of the temporal pass: the closest-hit stack rings and the
-// per-wave ray queue.
-struct TemporalLds {
-    int ring[kStackRing * 256];
-    float4 q_origin[256];
-    float4 q_dir[256];
-    int q_tri[256];
-    float q_dist[256];
+// Per-block LDS of the temporal pass for a workgroup of kWaves waves: the
+// closest-hit stack rings, the per-wave ray queue (the candidates' rays,
+// then the taps' visibility rays) and the per-wave list of candidates left
+// to the BVH.
+// kTrace: the candidates are traced in the pass (the LDS below); otherwise
+// only the taps' visibility queue.
+template <int kWaves, bool kTrace>
+struct TemporalLds;
+template <int kWaves>
+struct TemporalLds<kWaves, true> {
+    int ring[kStackRing * 64 * kWaves];
+    VisLds<kWaves> vis;
+    uint8_t walk[kWaves][kVisQueue];
+};
+template <int kWaves>
+struct TemporalLds<kWaves, false> {
+    VisLds<kWaves> vis;
 };
 
 // Source density of a candidate hit, converted from the ray origin to the
@@ -55,16 +64,138 @@ RGI_DEV void stream_candidate(const GiPass& p, F3 position, F3 n, F3 albedo, F3 
     }
 }
 
-template <bool kSameSize>
-RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                              const rgi_gi_reservoirs& out, F2 texel, const TileMap& tiles, TemporalLds& lds)
+// The fresh candidates traced in the pass itself, kVisChunk per pixel at a
+// time: each lane draws its rays in candidate order and queues them; the
+// wave marches the queue, walks the rays the march left open through the
+// BVH, then streams each pixel's candidates into its reservoir in order.
+template <int kWaves>
+RGI_DEV void candidates_in_place(const GiPass& p, F3 position, F3 n, F3 albedo, F3 ray_origin, bool working,
+                                 uint32_t& rng, TemporalLds<kWaves, true>& lds, Res& canonical)
 {
-    int* ring = lds.ring;
-    float4* q_origin = lds.q_origin;
-    float4* q_dir = lds.q_dir;
-    int* q_tri = lds.q_tri;
-    float* q_dist = lds.q_dist;
+    const rgi_frame& f = *p.f;
+    const uint32_t wave = threadIdx.x >> 6;
+    float4* q_origin = lds.vis.origin[wave];
+    float4* q_dir = lds.vis.dir[wave];
+    uint8_t* walk_list = lds.walk[wave];
+    for (int k0 = 0; k0 < f.initial_candidates; k0 += kVisChunk) {
+        const int cands = min(kVisChunk, f.initial_candidates - k0);
+        uint32_t slot[kVisChunk];
+        float rsel[kVisChunk];
+        uint32_t queued = 0u;
+#pragma unroll
+        for (int j = 0; j < kVisChunk; ++j) {
+            slot[j] = kNoSlot;
+            rsel[j] = 0.0f;
+            if (j >= cands) continue;
+            float r0 = next_random(rng);
+            float r1 = next_random(rng);
+            rsel[j] = next_random(rng);
+            F3 dir = cosine_lobe(n, r0, r1);
+            canonical.count += 1.0f;
+            uint64_t q = __ballot(working);
+            if (working) {
+                slot[j] = queued + rank_in(q);
+                q_origin[slot[j]] = make_float4(ray_origin.x, ray_origin.y, ray_origin.z, 0.0f);
+                q_dir[slot[j]] = make_float4(dir.x, dir.y, dir.z, kCandOpen);
+            }
+            queued += (uint32_t)__popcll(q);
+        }
+        if (queued != 0u) {
+            wave_sync();
+            if (f.ray_mode != RGI_RAY_BVH) march_candidates(p, q_origin, q_dir, queued, 3.6f * p.sc);
+            if (f.ray_mode != RGI_RAY_SCREEN) {
+                wave_sync();
+                uint32_t walks = 0u;
+#pragma unroll
+                for (int j = 0; j < kVisChunk; ++j) {
+                    bool need = slot[j] != kNoSlot && q_dir[slot[j]].w == kCandOpen;
+                    uint64_t q = __ballot(need);
+                    if (need) walk_list[walks + rank_in(q)] = (uint8_t)slot[j];
+                    walks += (uint32_t)__popcll(q);
+                }
+                wave_sync();
+                if (walks != 0u)
+                    walk_candidates(p, q_origin, q_dir, walk_list, walks, 4.0f * p.sc, lds.ring + threadIdx.x, 64 * kWaves);
+            }
+            wave_sync();
+        }
+#pragma unroll
+        for (int j = 0; j < kVisChunk; ++j) {
+            if (j >= cands) continue;
+            F3 hit_position = splat3(0.0f), dir = splat3(0.0f);
+            float hit_id = 0.0f;
+            bool hit = false;
+            if (slot[j] != kNoSlot) {
+                float4 a = q_origin[slot[j]], d = q_dir[slot[j]];
+                dir = F3{d.x, d.y, d.z};
+                if (d.w == kCandMarched) {
+                    hit = true;
+                    hit_position = F3{a.x, a.y, a.z};
+                } else if (d.w == kCandWalked) {
+                    int tri = __float_as_int(a.y);
+                    if (tri >= 0) {
+                        hit = true;
+                        hit_position = ray_origin + dir * a.x;
+                        hit_id = sample_id_of(p.v, tri, dir);
+                    } else if (sky_on(f)) {
+                        hit = true;
+                        hit_position = dir;
+                        hit_id = kSkySampleId;
+                    }
+                }
+            }
+            stream_candidate(p, position, n, albedo, ray_origin, dir, hit, hit_position, hit_id, rsel[j], canonical);
+        }
+        wave_sync();
+    }
+}
 
+// The fresh candidates from the records k_gi_candidates and k_gi_walks left
+// for GI pixel o, streamed into its reservoir in candidate order.
+RGI_DEV void candidates_from_records(const GiPass& p, F3 position, F3 n, F3 albedo, F3 ray_origin, bool working,
+                                     uint32_t o, uint32_t& rng, const CandRays& c, Res& canonical)
+{
+    const rgi_frame& f = *p.f;
+    for (int k = 0; k < f.initial_candidates; ++k) {
+        float r0 = next_random(rng);
+        float r1 = next_random(rng);
+        float rsel = next_random(rng);
+        F3 dir = cosine_lobe(n, r0, r1);
+        canonical.count += 1.0f;
+        F3 hit_position = splat3(0.0f);
+        float hit_id = 0.0f;
+        bool hit = false;
+        if (working) {
+            float4 a = c.rec[(uint32_t)k * c.pixels + o];
+            if (a.w == kCandMarched) {
+                hit = true;
+                hit_position = F3{a.x, a.y, a.z};
+            } else if (f.ray_mode != RGI_RAY_SCREEN) {
+                uint32_t slot = __float_as_uint(a.x);
+                int tri = c.hit[slot];
+                if (tri >= 0) {
+                    hit = true;
+                    hit_position = ray_origin + dir * c.dist[slot];
+                    hit_id = sample_id_of(p.v, tri, dir);
+                } else if (sky_on(f)) {
+                    hit = true;
+                    hit_position = dir;
+                    hit_id = kSkySampleId;
+                }
+            }
+        }
+        stream_candidate(p, position, n, albedo, ray_origin, dir, hit, hit_position, hit_id, rsel, canonical);
+    }
+}
+
+// One pixel (x, y) of the temporal pass per lane; the wave holds two of the
+// reference's 4 x 8 groups, one per half-wave. kRecords: the candidates come
+// from `cand` (the listed form); otherwise they are traced here with `lds`.
+template <bool kSameSize, int kWaves, bool kRecords>
+RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+                              const rgi_gi_reservoirs& out, F2 texel, int x, int y,
+                              TemporalLds<kWaves, !kRecords>& lds, const CandRays& cand)
+{
     GiPass p;
     p.f = &f;
     p.v = scene_view(s);
@@ -74,9 +205,6 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     p.grid_camera = F3{f.prev_camera_position[0], f.prev_camera_position[1], f.prev_camera_position[2]};
     p.sc = f.scene_scale;
 
-    int x, y;
-    if (!group_pixel(tiles, x, y)) return;
-    uint32_t lane = threadIdx.x & 63u, qbase = threadIdx.x & ~63u;
     bool inside = x < p.hw && y < p.hh;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
     float depth = inside ? fetch_depth(p, uv) : 1.0f;
@@ -102,52 +230,11 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     uint32_t rng = pcg((uint32_t)to_int(gx) + pcg((uint32_t)to_int(gy) + pcg(f.frame_index)));
 
     Res canonical = empty_res();
-    for (int k = 0; k < f.initial_candidates; ++k) {
-        float r0 = next_random(rng);
-        float r1 = next_random(rng);
-        float rsel = next_random(rng);
-        F3 dir = cosine_lobe(n, r0, r1);
-        canonical.count += 1.0f;
-        F3 hit_position = splat3(0.0f);
-        float hit_id = 0.0f;
-        bool hit = false;
-        if (working && f.ray_mode != RGI_RAY_BVH) hit = march(p, ray_origin, dir, 3.6f * p.sc, hit_position);
-        bool need = working && !hit && f.ray_mode != RGI_RAY_SCREEN;
-        uint64_t queue = __ballot(need);
-        if (queue != 0ull) {
-            uint32_t slot = rank_in(queue);
-            if (need) {
-                q_origin[qbase + slot] = make_float4(ray_origin.x, ray_origin.y, ray_origin.z, 0.0f);
-                q_dir[qbase + slot] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-            }
-            wave_sync();
-            uint32_t queued = (uint32_t)__popcll(queue);
-            if (lane < queued) {
-                float4 qo = q_origin[qbase + lane], qd = q_dir[qbase + lane];
-                float dist;
-                int tri = trace_closest(p.v, F3{qo.x, qo.y, qo.z}, F3{qd.x, qd.y, qd.z}, 4.0f * p.sc, dist,
-                                        ring + threadIdx.x, 256);
-                q_tri[qbase + lane] = tri;
-                q_dist[qbase + lane] = dist;
-            }
-            wave_sync();
-            if (need) {
-                int tri = q_tri[qbase + slot];
-                float dist = q_dist[qbase + slot];
-                if (tri >= 0) {
-                    hit = true;
-                    hit_position = ray_origin + dir * dist;
-                    hit_id = sample_id_of(p.v, tri, dir);
-                } else if (sky_on(f)) {
-                    hit = true;
-                    hit_position = dir;
-                    hit_id = kSkySampleId;
-                }
-            }
-            wave_sync();
-        }
-        stream_candidate(p, position, n, albedo, ray_origin, dir, hit, hit_position, hit_id, rsel, canonical);
-    }
+    if constexpr (kRecords)
+        candidates_from_records(p, position, n, albedo, ray_origin, working, (uint32_t)y * (uint32_t)p.hw + (uint32_t)x,
+                                rng, cand, canonical);
+    else
+        candidates_in_place<kWaves>(p, position, n, albedo, ray_origin, working, rng, lds, canonical);
     float canonical_weight = canonical.target > 0.000001f ? canonical.wsum / (canonical.count * canonical.target) : 0.0f;
 
     bool history_valid = f.history_valid != 0;
@@ -158,29 +245,42 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
     float radius = reuse_radius(p, position);
     Pairwise mis = begin_pairwise(p, position, n, canonical, canonical.count, f.temporal_taps);
     float combined_weight = 0.0f;
-    if (working) {
-        for (int tap = 0; tap < f.temporal_taps; ++tap) {
-            float r0 = next_random(rng);
-            float r1 = next_random(rng);
-            float r2 = next_random(rng);
-            float rsel = next_random(rng);
-            F2 huv = F2{-1.0f, -1.0f};
-            if (history_valid) {
-                if (f.world_space_reuse) {
-                    uint32_t entry = grid_pick(f, in.grid, p.grid_camera, position, n, r0, r1, r2);
-                    if (entry != kGridInvalid) huv = pixel_uv(p, entry);
-                } else if (tap == 0 && clip_prev.w > 0.0001f) {
-                    huv = previous_uv;
+    // Temporal taps, kVisChunk at a time, their visibility rays answered
+    // per wave (finish_chunk); every lane takes part in the wave's queue,
+    // pixels without a surface with no ray.
+    for (int t0 = 0; t0 < f.temporal_taps; t0 += kVisChunk) {
+        const int taps = min(kVisChunk, f.temporal_taps - t0);
+        uint32_t marches = 0u, walks = 0u;
+#pragma unroll
+        for (int j = 0; j < kVisChunk; ++j) {
+            if (j >= taps) break;
+            float ncount = -1.0f, ng = 0.0f, v_max = 0.0f;
+            int queue = kVisKnown;
+            F3 v_origin = splat3(0.0f), v_dir = splat3(0.0f);
+            if (working) {
+                float r0 = next_random(rng);
+                float r1 = next_random(rng);
+                float r2 = next_random(rng);
+                float rsel = next_random(rng);
+                F2 huv = F2{-1.0f, -1.0f};
+                if (history_valid) {
+                    if (f.world_space_reuse) {
+                        uint32_t entry = grid_pick(f, in.grid, p.grid_camera, position, n, r0, r1, r2);
+                        if (entry != kGridInvalid) huv = pixel_uv(p, entry);
+                    } else if (t0 + j == 0 && clip_prev.w > 0.0001f) {
+                        huv = previous_uv;
+                    }
+                }
+                Nb nb;
+                if (on_screen(huv) && accept(p, fetch_f4(in.history_sample, p, huv), fetch_f4(in.history_receiver, p, huv),
+                                             fetch_f4(in.history_normal, p, huv), position, n, radius, nb)) {
+                    ncount = nb.count;
+                    add_pair_deferred(p, mis, position, n, albedo, nb, rsel, ng, queue, v_origin, v_dir, v_max);
                 }
             }
-            Nb nb;
-            if (!on_screen(huv) || !accept(p, fetch_f4(in.history_sample, p, huv), fetch_f4(in.history_receiver, p, huv),
-                                           fetch_f4(in.history_normal, p, huv), position, n, radius, nb)) {
-                mis.c_share += mis.pair_c;
-                continue;
-            }
-            add_pair(p, mis, position, n, albedo, nb, rsel);
+            queue_tap(lds.vis, j, marches, walks, queue, v_origin, v_dir, v_max, ncount, ng);
         }
+        finish_chunk(p, lds.vis, mis, taps, marches, walks);
     }
     if (working) {
         float rfinal = next_random(rng);
@@ -227,7 +327,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
 template <bool kSameSize>
 RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                              const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel,
-                             const TileMap& tiles, VisLds& vis)
+                             const TileMap& tiles, VisLds<4>& vis)
 {
     GiPass p;
     p.f = &f;
@@ -360,8 +460,10 @@ template <bool kSameSize>
 __global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                      rgi_gi_reservoirs out, F2 texel, TileMap tiles)
 {
-    __shared__ TemporalLds lds;
-    gi_temporal_body<kSameSize>(f, s, g, in, out, texel, tiles, lds);
+    __shared__ TemporalLds<4, true> lds;
+    int x, y;
+    if (!group_pixel(tiles, x, y)) return;
+    gi_temporal_body<kSameSize, 4, false>(f, s, g, in, out, texel, x, y, lds, CandRays{});
 }
 
 template <bool kSameSize>
@@ -369,8 +471,151 @@ __global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs out, F2 texel,
                  TileMap tiles)
 {
-    __shared__ TemporalLds lds;
-    gi_temporal_body<kSameSize>(f, s, g, in, out, texel, tiles, lds);
+    __shared__ TemporalLds<4, true> lds;
+    int x, y;
+    if (!group_pixel(tiles, x, y)) return;
+    gi_temporal_body<kSameSize, 4, false>(f, s, g, in, out, texel, x, y, lds, CandRays{});
+}
+
+// The fresh candidates' rays of the listed tiles, recorded for the listed
+// form of the pass (cs_restir_temporal.sc:85-150): each lane draws its
+// pixel's candidate directions in order, marches each ray through the depth
+// buffer and records the hit; the rays the march leaves open are listed for
+// the BVH (one list append per workgroup). Apart from the resampling, the
+// march needs few registers, so many waves per SIMD hide its chain of
+// dependent depth loads. Every workgroup runs the same number of rounds of
+// 4 tiles (one per wave).
+template <bool kSameSize>
+__global__ void __launch_bounds__(256) k_gi_candidates(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list,
+                                                       CandRays c)
+{
+    __shared__ uint32_t wave_base[4];
+    GiPass p;
+    p.f = &f;
+    p.g = g;
+    set_gi_size<kSameSize>(p, f, g, texel);
+    p.sc = f.scene_scale;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t total = list.words[0] + list.words[1];
+    const int K = f.initial_candidates;
+    const bool trace_open = f.ray_mode != RGI_RAY_SCREEN;
+    for (uint32_t round = blockIdx.x * 4u; round < total; round += gridDim.x * 4u) {
+        const uint32_t item = round + wave;
+        int x = 0, y = 0;
+        if (item < total) tile_pixel(list, listed_tile(list, item), x, y);
+        const bool inside = item < total && x < p.hw && y < p.hh;
+        F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+        const float depth = inside ? fetch_depth(p, uv) : 1.0f;
+        const bool working = inside && depth < kSkyDepth;
+        const uint32_t o = (uint32_t)y * (uint32_t)p.hw + (uint32_t)x;
+        F3 n = splat3(0.0f), ray_origin = splat3(0.0f);
+        uint32_t rng = 0u;
+        if (working) {
+            F3 position = reconstruct_position(f, uv, depth);
+            n = decode_normal(p, uv);
+            ray_origin = position + n * (0.02f * p.sc);
+            float gx = floorf(uv.x * p.size.x), gy = floorf(uv.y * p.size.y);
+            rng = pcg((uint32_t)to_int(gx) + pcg((uint32_t)to_int(gy) + pcg(f.frame_index)));
+        }
+        const uint32_t rng0 = rng;
+        uint32_t open = 0u; // bit k: candidate k left to the BVH
+        for (int k = 0; k < K; ++k) {
+            float r0 = next_random(rng);
+            float r1 = next_random(rng);
+            next_random(rng);
+            if (!working) continue;
+            F3 dir = cosine_lobe(n, r0, r1);
+            F3 hit_position = splat3(0.0f);
+            bool hit = false;
+            if (f.ray_mode != RGI_RAY_BVH) hit = march(p, ray_origin, dir, 3.6f * p.sc, hit_position);
+            if (hit) {
+                c.rec[(uint32_t)k * c.pixels + o] = make_float4(hit_position.x, hit_position.y, hit_position.z, kCandMarched);
+            } else if (trace_open) {
+                open |= 1u << k;
+            } else {
+                c.rec[(uint32_t)k * c.pixels + o] = make_float4(0.0f, 0.0f, 0.0f, kCandOpen);
+            }
+        }
+        // List the open rays: ranks per candidate within the wave, one
+        // append for the workgroup.
+        uint32_t wave_open = 0u;
+        for (int k = 0; k < K; ++k) wave_open += (uint32_t)__popcll(__ballot((open >> k) & 1u));
+        if ((threadIdx.x & 63u) == 0u) wave_base[wave] = wave_open;
+        __syncthreads();
+        if (threadIdx.x == 0u) {
+            uint32_t sum = 0u;
+            for (int w = 0; w < 4; ++w) {
+                uint32_t v = wave_base[w];
+                wave_base[w] = sum;
+                sum += v;
+            }
+            uint32_t base = sum != 0u ? atomicAdd(list.words + kTileRayWord, sum) : 0u;
+            for (int w = 0; w < 4; ++w) wave_base[w] += base;
+        }
+        __syncthreads();
+        uint32_t next = wave_base[wave];
+        rng = rng0;
+        for (int k = 0; k < K; ++k) {
+            float r0 = next_random(rng);
+            float r1 = next_random(rng);
+            next_random(rng);
+            const bool mine = ((open >> k) & 1u) != 0u;
+            const uint64_t b = __ballot(mine);
+            if (mine) {
+                uint32_t slot = next + rank_in(b);
+                F3 dir = cosine_lobe(n, r0, r1);
+                c.rays[2u * slot] = make_float4(ray_origin.x, ray_origin.y, ray_origin.z, 4.0f * p.sc);
+                c.rays[2u * slot + 1u] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+                c.rec[(uint32_t)k * c.pixels + o] = make_float4(__uint_as_float(slot), 0.0f, 0.0f, kCandOpen);
+            }
+            next += (uint32_t)__popcll(b);
+        }
+        __syncthreads();
+    }
+}
+
+// Closest hits of the candidates' listed rays (traceBvh, bvh.sh:216-295),
+// answered by persistent waves with per-lane refill, in chunks of 64 rays (a
+// frame's few hundred thousand rays then still reach every wave).
+__global__ void __launch_bounds__(256) k_gi_walks(rgi_scene s, TileList list, CandRays c)
+{
+    __shared__ int ring[kStackRing * 256];
+    trace_persistent<MollerTrumbore, false>(scene_view(s), c.rays, list.words[kTileRayWord], c.hit, c.dist,
+                                            list.words + kWalkClaimWord, ring + threadIdx.x, 256, 64u);
+}
+
+// The listed form: one-wave workgroups that claim the listed tiles
+// (heavy ones first) until the list is used up; every wave reaches that
+// exit test. A wave's LDS is its own, so a long tile holds no other wave's.
+template <bool kSameSize>
+RGI_DEV void gi_temporal_listed(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+                                const rgi_gi_reservoirs& out, F2 texel, const TileList& list, const CandRays& c,
+                                TemporalLds<1, false>& lds)
+{
+    uint32_t t;
+    while (claim_tile(list, t)) {
+        int x, y;
+        tile_pixel(list, t, x, y);
+        gi_temporal_body<kSameSize, 1, true>(f, s, g, in, out, texel, x, y, lds, c);
+        wave_sync();
+    }
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(64) k_gi_temporal_listed(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
+                                                           rgi_gi_reservoirs out, F2 texel, TileList list, CandRays c)
+{
+    __shared__ TemporalLds<1, false> lds;
+    gi_temporal_listed<kSameSize>(f, s, g, in, out, texel, list, c, lds);
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(64) RGI_WAVES4
+k_gi_temporal_listed_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs out, F2 texel,
+                        TileList list, CandRays c)
+{
+    __shared__ TemporalLds<1, false> lds;
+    gi_temporal_listed<kSameSize>(f, s, g, in, out, texel, list, c, lds);
 }
 
 template <bool kSameSize>
@@ -378,7 +623,7 @@ __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rg
                                                     rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
                                                     F2 texel, TileMap tiles)
 {
-    __shared__ VisLds vis;
+    __shared__ VisLds<4> vis;
     gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, vis);
 }
 
@@ -387,18 +632,166 @@ __global__ void __launch_bounds__(256) RGI_WAVES4
 k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
                 uint16_t* out_moments, F2 texel, TileMap tiles)
 {
-    __shared__ VisLds vis;
+    __shared__ VisLds<4> vis;
     gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, vis);
+}
+
+// Zeroes a list's counters before k_gi_tiles fills it.
+__global__ void __launch_bounds__(kTileListWord) k_gi_tiles_reset(TileList list)
+{
+    list.words[threadIdx.x] = 0u;
+}
+
+// Lists the surface tiles of a GI pass, one tile per wave and 16 per
+// workgroup (one append per kind and workgroup), and writes the empty
+// reservoirs of every GI pixel of the tiles without a surface: the pixels
+// that the temporal pass writes as zeros without reading anything else.
+template <bool kSameSize>
+__global__ void __launch_bounds__(1024) k_gi_tiles(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list,
+                                                   rgi_gi_reservoirs out)
+{
+    __shared__ uint32_t counts[2], bases[2];
+    if (threadIdx.x < 2u) counts[threadIdx.x] = 0u;
+    __syncthreads();
+    GiPass p;
+    p.f = &f;
+    p.g = g;
+    set_gi_size<kSameSize>(p, f, g, texel);
+    const uint32_t t = blockIdx.x * 16u + (threadIdx.x >> 6);
+    const bool valid = t < list.tiles;
+    int x = 0, y = 0;
+    if (valid) tile_pixel(list, t, x, y);
+    const bool inside = valid && x < p.hw && y < p.hh;
+    F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+    const bool working = inside && fetch_depth(p, uv) < kSkyDepth;
+    const uint64_t b = __ballot(working);
+    const uint32_t kind = b == 0ull ? 2u : (__popcll(b) >= kHeavyTile ? 0u : 1u);
+    const bool leader = (threadIdx.x & 63u) == 0u && valid && kind < 2u;
+    uint32_t rank = 0u;
+    if (leader) rank = atomicAdd(&counts[kind], 1u);
+    if (kind == 2u && inside) {
+        uint32_t o = (uint32_t)y * (uint32_t)p.hw + (uint32_t)x;
+        F4 zero = F4{0.0f, 0.0f, 0.0f, 0.0f};
+        store_f4(out.sample, o, zero);
+        store_f4(out.receiver, o, zero);
+        store_f4(out.normal, o, zero);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2u && counts[threadIdx.x] != 0u) bases[threadIdx.x] = atomicAdd(list.words + threadIdx.x, counts[threadIdx.x]);
+    __syncthreads();
+    if (leader) {
+        uint32_t i = bases[kind] + rank;
+        list.words[kTileListWord + (kind == 0u ? i : list.tiles - 1u - i)] = t;
+    }
+}
+
+// Workgroups of a listed pass: as many one-wave workgroups as the chip holds
+// at 4 waves per SIMD (16 per CU), at most one per tile.
+uint32_t listed_grid(rgi_context* ctx, const TileList& list)
+{
+    uint32_t most = 16u * (uint32_t)context_compute_units(ctx);
+    return list.tiles < most ? list.tiles : most;
 }
 
 } // namespace
 
-int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream)
+int context_cand_rays(rgi_context* ctx, uint32_t candidates, CandRays& c)
 {
-    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
+    // rec: 16 B per candidate and pixel; rays: 32 B, hit and dist: 4 B each
+    // per listed ray, at most one per candidate and pixel.
+    const size_t n = (size_t)candidates * c.pixels;
+    const size_t rec = align256(n * sizeof(float4)), rays = align256(2 * n * sizeof(float4)),
+                 hits = align256(n * sizeof(int32_t)), dists = align256(n * sizeof(float));
+    const size_t bytes = rec + rays + hits + dists;
+    if (ctx->cand_bytes < bytes) {
+        if (ctx->cand) {
+            hipDeviceSynchronize();
+            hipFree(ctx->cand);
+            ctx->cand = nullptr;
+            ctx->cand_bytes = 0;
+        }
+        if (hipMalloc(&ctx->cand, bytes) != hipSuccess) {
+            ctx->cand = nullptr;
+            return fail(RGI_E_NOMEM, "candidate records: allocation of %zu bytes failed", bytes);
+        }
+        ctx->cand_bytes = bytes;
+    }
+    char* base = static_cast<char*>(ctx->cand);
+    c.rec = reinterpret_cast<float4*>(base);
+    c.rays = reinterpret_cast<float4*>(base + rec);
+    c.hit = reinterpret_cast<int32_t*>(base + rec + rays);
+    c.dist = reinterpret_cast<float*>(base + rec + rays + hits);
+    return RGI_OK;
+}
+
+bool gi_temporal_split(const rgi_context* ctx, const rgi_frame& f)
+{
+    return ctx != nullptr && f.initial_candidates <= kMaxCandRecords;
+}
+
+// The listed form's descriptors for frame f: the context's tile list and
+// candidate records (neither is reallocated between the two halves of one
+// frame, whose sizes are equal).
+static int listed_buffers(rgi_context* ctx, const rgi_frame& f, TileList& list, CandRays& c)
+{
+    list.tiles_x = ((uint32_t)f.gi_width + 7u) / 8u;
+    list.tiles = list.tiles_x * (((uint32_t)f.gi_height + 7u) / 8u);
+    list.words = context_tile_list(ctx, kTileListWord + list.tiles);
+    if (!list.words) return RGI_E_NOMEM;
+    c.pixels = (uint32_t)f.gi_width * (uint32_t)f.gi_height;
+    return context_cand_rays(ctx, (uint32_t)f.initial_candidates, c);
+}
+
+int gi_temporal_trace(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                      const rgi_gi_reservoirs& out, hipStream_t stream)
+{
+    TileList list;
+    CandRays c;
+    int e = listed_buffers(ctx, f, list, c);
+    if (e != RGI_OK) return e;
     F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
     const bool same = g.width == f.gi_width && g.height == f.gi_height;
+    hipLaunchKernelGGL(k_gi_tiles_reset, dim3(1), dim3(kTileListWord), 0, stream, list);
+    hipLaunchKernelGGL(same ? k_gi_tiles<true> : k_gi_tiles<false>, dim3((list.tiles + 15u) / 16u), dim3(1024), 0,
+                       stream, f, g, texel, list, out);
+    const uint32_t cus = (uint32_t)context_compute_units(ctx);
+    if (f.initial_candidates > 0) {
+        const uint32_t rounds = (list.tiles + 3u) / 4u, most = 8u * cus;
+        hipLaunchKernelGGL(same ? k_gi_candidates<true> : k_gi_candidates<false>, dim3(rounds < most ? rounds : most),
+                           dim3(256), 0, stream, f, g, texel, list, c);
+        // Without a BVH every walk ends at once with no hit, as trace_closest does.
+        if (f.ray_mode != RGI_RAY_SCREEN)
+            hipLaunchKernelGGL(k_gi_walks, dim3(4u * cus), dim3(256), 0, stream, s, list, c);
+    }
+    return check_launch("gi temporal tracing");
+}
+
+int gi_temporal_resample(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                         const rgi_gi_inputs& in, const rgi_gi_reservoirs& out, const Tuning& tuning,
+                         hipStream_t stream)
+{
+    TileList list;
+    CandRays c;
+    int e = listed_buffers(ctx, f, list, c);
+    if (e != RGI_OK) return e;
+    F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
+    const bool same = g.width == f.gi_width && g.height == f.gi_height;
+    auto kernel = tuning.gi_temporal_waves == 4 ? (same ? k_gi_temporal_listed_w4<true> : k_gi_temporal_listed_w4<false>)
+                                                : (same ? k_gi_temporal_listed<true> : k_gi_temporal_listed<false>);
+    hipLaunchKernelGGL(kernel, dim3(listed_grid(ctx, list)), dim3(64), 0, stream, f, s, g, in, out, texel, list, c);
+    return check_launch("gi temporal");
+}
+
+int gi_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                const rgi_gi_inputs& in, const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream)
+{
+    if (gi_temporal_split(ctx, f)) {
+        int e = gi_temporal_trace(ctx, f, s, g, out, stream);
+        return e != RGI_OK ? e : gi_temporal_resample(ctx, f, s, g, in, out, tuning, stream);
+    }
+    F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
+    const bool same = g.width == f.gi_width && g.height == f.gi_height;
+    TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
     auto kernel = tuning.gi_temporal_waves == 4 ? (same ? k_gi_temporal_w4<true> : k_gi_temporal_w4<false>)
                                                 : (same ? k_gi_temporal<true> : k_gi_temporal<false>);
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, out, texel, tiles);

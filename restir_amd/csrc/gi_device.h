@@ -136,6 +136,71 @@ RGI_DEV F3 previous_indirect(const GiPass& p, F3 position, F3 n, F3 albedo)
     return xyz(fetch_h4(p.in.history_color, p, uv)) * albedo;
 }
 
+// ------------------------------------------------------------ surface tiles
+
+// The surface tiles of a GI pass: 8 x 8-pixel tiles (two of the reference's
+// 4 x 8 groups side by side) that hold at least one GI pixel with a surface,
+// listed each pass so that only they are launched. words: [0] heavy tiles (at
+// least 48 surface pixels), [1] light tiles, [2] the temporal pass's listed
+// candidate rays, the tile claim counters at kTileClaimWord + 32 c and the
+// ray claim counters at kWalkClaimWord + 32 c for c = 0..7 (one 128-byte line
+// each), and the list from kTileListWord: heavy tiles from its front, light
+// ones from its back, so that the heavy ones are claimed first.
+struct TileList {
+    uint32_t* words;
+    uint32_t tiles_x;   // tiles per row
+    uint32_t tiles;     // tiles on the screen: the list's capacity
+};
+constexpr uint32_t kTileRayWord = 2u, kTileClaimWord = 32u, kWalkClaimWord = 32u + 8u * 32u,
+                   kTileListWord = 32u + 16u * 32u;
+constexpr int kHeavyTile = 48;
+
+// The temporal pass's candidate records (the listed form): for candidate k
+// and GI pixel o, rec[k * pixels + o] holds the march's hit point (w =
+// kCandMarched) or, for a ray the march leaves open, (slot bits, 0, 0,
+// kCandOpen), the ray being listed at `slot` for the BVH when the ray mode
+// traces such rays. Listed ray i is rays[2 i] (origin, max distance) and
+// rays[2 i + 1] (direction); its closest hit goes to hit[i] and dist[i].
+struct CandRays {
+    float4* rec;
+    float4* rays;
+    int32_t* hit;
+    float* dist;
+    uint32_t pixels;
+};
+constexpr int kMaxCandRecords = 8;
+
+// This lane's pixel of tile t: lanes [0, 32) the left 4 x 8 group, [32, 64)
+// the right one, each row-major.
+RGI_DEV void tile_pixel(const TileList& l, uint32_t t, int& x, int& y)
+{
+    uint32_t lane = threadIdx.x & 63u;
+    x = (int)((t % l.tiles_x) * 8u + (lane >> 5) * 4u + (lane & 3u));
+    y = (int)((t / l.tiles_x) * 8u + ((lane & 31u) >> 2));
+}
+
+// Listed item i (below the list's count): the heavy tiles, then the light.
+RGI_DEV uint32_t listed_tile(const TileList& l, uint32_t i)
+{
+    uint32_t heavy = l.words[0];
+    return i < heavy ? l.words[kTileListWord + i] : l.words[kTileListWord + l.tiles - 1u - (i - heavy)];
+}
+
+// The next listed tile for this wave (the result is wave-uniform). Workgroup
+// b, which runs on XCD b mod 8, claims from counter c = b mod 8 the items
+// c, c + 8, c + 16, ...: eight counters spread the claims of the chip's
+// waves, and each XCD still takes every eighth tile of the heavy-first order.
+RGI_DEV bool claim_tile(const TileList& l, uint32_t& t)
+{
+    const uint32_t c = blockIdx.x & 7u;
+    uint32_t k = 0u;
+    if ((threadIdx.x & 63u) == 0u) k = atomicAdd(l.words + kTileClaimWord + 32u * c, 1u);
+    uint32_t item = (uint32_t)__shfl((int)k, 0) * 8u + c;
+    if (item >= l.words[0] + l.words[1]) return false;
+    t = listed_tile(l, item);
+    return true;
+}
+
 // ------------------------------------------------------------ waves
 
 // Orders this wave's LDS writes before its reads (the wave runs in lockstep).
@@ -702,11 +767,13 @@ constexpr int kVisChunk = 2;
 constexpr int kVisQueue = 64 * kVisChunk;
 constexpr uint16_t kNoSlot = 0xffffu;
 
+// For a workgroup of kWaves waves.
+template <int kWaves>
 struct VisLds {
-    float4 origin[4][kVisQueue];      // per wave of the block: ray origin, max distance
-    float4 dir[4][kVisQueue];         // direction; w = hit (written when the ray is answered)
-    float2 rec[kVisChunk][256];       // per tap and thread: neighbour count (-1: rejected), geometry factor
-    uint16_t slot[kVisChunk][256];    // per tap and thread: the queued ray, or kNoSlot
+    float4 origin[kWaves][kVisQueue];         // per wave of the block: ray origin, max distance
+    float4 dir[kWaves][kVisQueue];            // direction; w = hit (written when the ray is answered)
+    float2 rec[kVisChunk][64 * kWaves];       // per tap and thread: neighbour count (-1: rejected), geometry factor
+    uint16_t slot[kVisChunk][64 * kWaves];    // per tap and thread: the queued ray, or kNoSlot
 };
 
 // Answers this wave's queued rays [first, first + count) of one kind with
@@ -757,7 +824,8 @@ RGI_DEV void answer_queue(const GiPass& p, float4* origin, float4* dir, uint32_t
 // Queues one tap's visibility ray (every active lane of the wave calls it,
 // with or without a ray; `marches` and `walks` are the wave's queue counts,
 // uniform) and records the tap.
-RGI_DEV void queue_tap(VisLds& vis, int j, uint32_t& marches, uint32_t& walks, int kind, F3 o, F3 d,
+template <int kWaves>
+RGI_DEV void queue_tap(VisLds<kWaves>& vis, int j, uint32_t& marches, uint32_t& walks, int kind, F3 o, F3 d,
                        float max_distance, float ncount, float ng)
 {
     const uint32_t wave = threadIdx.x >> 6;
@@ -777,7 +845,8 @@ RGI_DEV void queue_tap(VisLds& vis, int j, uint32_t& marches, uint32_t& walks, i
 
 // Answers the chunk's queued rays and adds the chunk's MIS shares in tap
 // order.
-RGI_DEV void finish_chunk(const GiPass& p, VisLds& vis, Pairwise& m, int taps, uint32_t marches, uint32_t walks)
+template <int kWaves>
+RGI_DEV void finish_chunk(const GiPass& p, VisLds<kWaves>& vis, Pairwise& m, int taps, uint32_t marches, uint32_t walks)
 {
     const uint32_t wave = threadIdx.x >> 6;
     if (marches + walks != 0u) {
@@ -798,6 +867,83 @@ RGI_DEV void finish_chunk(const GiPass& p, VisLds& vis, Pairwise& m, int taps, u
         }
     }
     wave_sync();
+}
+
+// The temporal pass's fresh candidates, up to kVisChunk per pixel at a time,
+// traced per wave from the VisLds queue: every working pixel's ray is
+// queued, marched with lane refill (march_candidates), and the rays the
+// march leaves open are walked through the BVH with lane refill
+// (walk_candidates). Per queued ray k: origin[k] holds the origin, then the
+// march's hit point or the walk's (distance, triangle bits); dir[k] holds the
+// direction, with w the ray's state below. Whichever lane answers a ray, its
+// answer is the one march() / trace_closest() give.
+constexpr float kCandOpen = 0.0f, kCandMarched = 1.0f, kCandWalked = 2.0f;
+
+RGI_DEV void march_candidates(const GiPass& p, float4* origin, float4* dir, uint32_t count, float max_distance)
+{
+    const float step_scale = march_step_scale(p);
+    MarchState s;
+    s.active = false;
+    s.hit = false;
+    uint32_t k = 0u, next = 0u;
+    bool busy = false;
+    for (;;) {
+        if (busy && !s.active) {
+            if (s.hit) {
+                origin[k] = make_float4(s.surface_b.x, s.surface_b.y, s.surface_b.z, 0.0f);
+                dir[k].w = kCandMarched;
+            }
+            busy = false;
+        }
+        if (next < count) {
+            uint64_t idle = __ballot(!busy);
+            uint32_t slot = next + rank_in(idle);
+            if (!busy && slot < count) {
+                k = slot;
+                busy = true;
+                float4 o = origin[k], d = dir[k];
+                march_begin(p, s, F3{o.x, o.y, o.z}, F3{d.x, d.y, d.z}, max_distance);
+            }
+            next += (uint32_t)__popcll(idle);
+        }
+        if (__ballot(busy) == 0ull) break;
+        if (busy && s.active) march_step<true>(p, s, step_scale);
+    }
+}
+
+// Closest-hit walks of the queued rays list[0, count); `ring` is this lane's
+// ring stack, its entries `stride` ints apart (slot-major over the block's
+// lanes).
+RGI_DEV void walk_candidates(const GiPass& p, float4* origin, float4* dir, const uint8_t* list, uint32_t count,
+                             float max_distance, int* ring, int stride)
+{
+    ClosestWalk<MollerTrumbore> w;
+    w.active = false;
+    w.hit = -1;
+    w.dist = 0.0f;
+    uint32_t k = 0u, next = 0u;
+    bool busy = false;
+    for (;;) {
+        if (busy && !w.active) {
+            origin[k] = make_float4(w.dist, __int_as_float(w.hit), 0.0f, 0.0f);
+            dir[k].w = kCandWalked;
+            busy = false;
+        }
+        if (next < count) {
+            uint64_t idle = __ballot(!busy);
+            uint32_t i = next + rank_in(idle);
+            if (!busy && i < count) {
+                k = list[i];
+                busy = true;
+                float4 o = origin[k], d = dir[k];
+                F3 d3 = F3{d.x, d.y, d.z};
+                closest_begin(p.v, w, MollerTrumbore{F3{o.x, o.y, o.z}, d3}, d3, max_distance);
+            }
+            next += (uint32_t)__popcll(idle);
+        }
+        if (__ballot(busy) == 0ull) break;
+        if (busy && w.active) closest_step(p.v, w, ring, stride);
+    }
 }
 
 RGI_DEV float finish_pairwise(const GiPass& p, Pairwise& m, F3 position, F3 n, const Res& c, float cweight, float random)

@@ -27,6 +27,11 @@ struct rgi_context {
     uint32_t* counter = nullptr;   // persistent-launch work counter
     void* scratch = nullptr;       // grid insert (deterministic) and other per-call scratch
     size_t scratch_bytes = 0;
+    uint32_t* tile_list = nullptr; // the GI passes' surface tiles (apart from `scratch`: the renderer's
+    size_t tile_list_words = 0;    // grid insert runs on a side stream beside GI temporal)
+    void* cand = nullptr;          // the GI temporal pass's candidate records and rays
+    size_t cand_bytes = 0;
+    int compute_units = 0;
 };
 
 namespace rgi {
@@ -35,6 +40,15 @@ int fail(int code, const char* fmt, ...);
 int check_launch(const char* what);
 // Scratch of at least `bytes`, grown (after a device sync) when too small.
 void* context_scratch(rgi_context* ctx, size_t bytes);
+// The tile list of at least `words` words, grown (after a device sync) when
+// too small; NULL (with the error set) when it cannot be allocated.
+uint32_t* context_tile_list(rgi_context* ctx, size_t words);
+// The device's compute units, looked up once per context.
+int context_compute_units(rgi_context* ctx);
+struct CandRays;
+// Points `c` (with c.pixels set) at the context's candidate records and rays
+// for `candidates` candidates per pixel, grown when too small.
+int context_cand_rays(rgi_context* ctx, uint32_t candidates, CandRays& c);
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -65,8 +79,21 @@ int svgf_variance(const rgi_frame& f, const uint16_t* color, const uint16_t* mom
                   const float* normal, uint16_t* out, hipStream_t stream);
 int svgf_atrous(const rgi_frame& f, const uint16_t* input, const float* receiver, const float* normal, int step,
                 uint16_t* out, hipStream_t stream);
-int gi_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
-                const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream);
+// GI temporal. With a context (and at most 8 candidates) in two halves:
+// gi_temporal_trace lists the surface tiles (writing the sky tiles' empty
+// reservoirs), marches the candidates' rays and walks the open ones through
+// the BVH, reading only the G-buffer and the scene; gi_temporal_resample
+// then resamples the listed tiles. A caller may run other work between the
+// two (the DI passes), on other streams. Without a context, one pass over
+// every tile of the screen.
+bool gi_temporal_split(const rgi_context* ctx, const rgi_frame& f);
+int gi_temporal_trace(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                      const rgi_gi_reservoirs& out, hipStream_t stream);
+int gi_temporal_resample(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                         const rgi_gi_inputs& in, const rgi_gi_reservoirs& out, const Tuning& tuning,
+                         hipStream_t stream);
+int gi_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                const rgi_gi_inputs& in, const rgi_gi_reservoirs& out, const Tuning& tuning, hipStream_t stream);
 int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
                hipStream_t stream);

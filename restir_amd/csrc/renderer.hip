@@ -42,10 +42,11 @@ struct rgi_renderer {
     uint16_t* black = nullptr; // m_blackTexture: DI light when ReSTIR DI is selected but has no lights
     uint16_t* specular_gi[2] = {}; // m_specularTextures, RGBA16F, own parity m_specularRead
     int specular_read = 0;
-    // The side stream that builds the frame's hash grid beside the DI passes
-    // and GI temporal (default priority), with its fork and join events.
+    // The side stream that traces GI temporal's candidates and builds the
+    // frame's hash grid beside the DI passes (default priority), with its
+    // fork event and the events that end its two parts.
     hipStream_t side = nullptr;
-    hipEvent_t side_fork = nullptr, side_join = nullptr;
+    hipEvent_t side_fork = nullptr, side_trace = nullptr, side_join = nullptr;
     // The cache's training step runs on its own stream (default priority):
     // `nrc_pending` marks a step that the caller's stream has not joined yet.
     hipStream_t nrc_stream = nullptr;
@@ -137,9 +138,10 @@ void free_all(rgi_renderer* r)
     r->nrc_stream = nullptr;
     r->nrc_pending = false;
     if (r->side_fork) hipEventDestroy(r->side_fork);
+    if (r->side_trace) hipEventDestroy(r->side_trace);
     if (r->side_join) hipEventDestroy(r->side_join);
     if (r->side) hipStreamDestroy(r->side);
-    r->side_fork = r->side_join = nullptr;
+    r->side_fork = r->side_trace = r->side_join = nullptr;
     r->side = nullptr;
 }
 
@@ -334,6 +336,7 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
     }
     ok = ok && hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&r->side_fork, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&r->side_trace, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&r->side_join, hipEventDisableTiming) == hipSuccess &&
          hipStreamCreateWithFlags(&r->nrc_stream, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&r->nrc_fork, hipEventDisableTiming) == hipSuccess &&
@@ -458,18 +461,27 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         }
         shadow_map = r->atlas;
     }
-    // This frame's hash grid (renderer.cpp:735-750). Only GI spatial and the
-    // specular pass read it (GI temporal reads last frame's), so with
-    // `fork_grid` it is built on the side stream beside the DI passes and GI
-    // temporal, and joined before GI spatial. The grid buffer it clears was
-    // last read by last frame's GI temporal, which precedes the fork on
-    // `stream`.
+    // GI temporal's tracing half reads only the G-buffer and the scene (and
+    // writes the sky tiles' empty reservoirs into this frame's GI planes,
+    // which nothing reads before GI temporal), so with `fork_grid` it runs
+    // on the side stream beside the DI passes, joined before GI temporal's
+    // resampling. This frame's hash grid (renderer.cpp:735-750) follows it
+    // there: only GI spatial and the specular pass read the grid (GI temporal
+    // reads last frame's), so it is joined before GI spatial. The buffers
+    // both write were last read by last frame's passes, which precede the
+    // fork on `stream`.
     hipStream_t grid_stream = fork_grid ? r->side : stream;
+    const bool split = gi_temporal_split(r->ctx, f);
+    rgi_gi_reservoirs gi_out{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
     if (fork_grid) {
         hipEventRecord(r->side_fork, stream);
         hipStreamWaitEvent(r->side, r->side_fork, 0);
+        if (split) {
+            e = gi_temporal_trace(r->ctx, f, r->scene, *g, gi_out, r->side);
+            hipEventRecord(r->side_trace, r->side);
+        }
     }
-    e = grid_clear(r->grid[write], f.grid_table_size, grid_stream);
+    if (e == RGI_OK) e = grid_clear(r->grid[write], f.grid_table_size, grid_stream);
     if (e == RGI_OK) {
         if (s.grid_mode == RGI_GRID_DETERMINISTIC)
             e = grid_insert_deterministic(r->ctx, f, *g, r->grid[write], grid_stream);
@@ -526,11 +538,18 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         }
         rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
                           surface_light, shadow_map, r->grid[read]};
-        rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
-        return gi_temporal(fg, r->scene, *g, tin, res, r->ctx->tuning, stream);
+        if (fork_grid && split) {
+            hipStreamWaitEvent(stream, r->side_trace, 0);
+            return gi_temporal_resample(r->ctx, fg, r->scene, *g, tin, gi_out, r->ctx->tuning, stream);
+        }
+        return gi_temporal(r->ctx, fg, r->scene, *g, tin, gi_out, r->ctx->tuning, stream);
     };
     if (e == RGI_OK) e = front();
-    if (fork_grid) hipStreamWaitEvent(stream, r->side_join, 0);
+    if (fork_grid) {
+        // Every branch of the side stream is joined, also when a pass failed.
+        if (split) hipStreamWaitEvent(stream, r->side_trace, 0);
+        hipStreamWaitEvent(stream, r->side_join, 0);
+    }
     if (e != RGI_OK) return e;
     rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
     rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],

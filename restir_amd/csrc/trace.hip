@@ -39,82 +39,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace(rgi_scene s, const float4
     dist[i] = t;
 }
 
-// Persistent form. Group g (workgroups b = g mod kTraceGroups) claims
-// chunks g, g + kTraceGroups, ... of kTraceChunk rays from its own counter
-// (each on a 128-byte line) until they pass `count`; every wave reaches that
-// exit test, so the grid always drains. The counters are zeroed before the
-// launch.
-constexpr uint32_t kTraceGroups = 8, kTraceChunk = 256, kTraceLine = 32;
-
-// Lanes keep traversing while at least kRefillBelow of them are busy; below
-// that the wave stops to hand the idle lanes new rays (Aila and Laine 2009's
-// dynamic ray fetch), so the hand-out costs little per node visited.
-constexpr int kRefillBelow = 40;
-
+// Persistent form (trace_persistent): workgroups claim chunks of rays from
+// kTraceGroups counters until the rays run out.
 template <class Tri, bool kAny>
 __global__ void __launch_bounds__(kTraceBlock) k_trace_persistent(rgi_scene s, const float4* rays, uint32_t count,
                                                                   int32_t* hit, float* dist, uint32_t* counters)
 {
     __shared__ int ring[kStackRing * kTraceBlock];
-    SceneView v = scene_view(s);
-    const uint32_t group = blockIdx.x % kTraceGroups;
-    uint32_t* counter = counters + group * kTraceLine;
-    AnyWalk<Tri> aw;
-    ClosestWalk<Tri> cw;
-    aw.active = false;
-    cw.active = false;
-    uint32_t k = 0u, cur = 0u, end = 0u;
-    bool busy = false, exhausted = false;
-    for (;;) {
-        // Hand the idle lanes the next rays of the wave's chunk (claiming a
-        // new chunk when it is used up).
-        uint64_t idle = __ballot(!busy);
-        while (idle != 0ull && !exhausted) {
-            if (cur >= end) {
-                uint32_t c = 0u;
-                if ((threadIdx.x & 63u) == 0u) c = atomicAdd(counter, 1u);
-                uint32_t first = ((uint32_t)__shfl((int)c, 0) * kTraceGroups + group) * kTraceChunk;
-                if (first >= count) {
-                    exhausted = true;
-                    break;
-                }
-                cur = first;
-                end = count - first < kTraceChunk ? count : first + kTraceChunk;
-            }
-            uint32_t ray = cur + rank_in(idle);
-            bool take = !busy && ray < end;
-            if (take) {
-                k = ray;
-                busy = true;
-                float4 a = rays[2u * k], b = rays[2u * k + 1u];
-                F3 o = F3{a.x, a.y, a.z}, d = F3{b.x, b.y, b.z};
-                Tri test = triangle_test<Tri>(o, d);
-                if (kAny) any_begin(v, aw, test, d, a.w);
-                else closest_begin(v, cw, test, d, a.w);
-            }
-            uint32_t taken = (uint32_t)__popcll(idle);
-            cur = end - cur < taken ? end : cur + taken;
-            idle = __ballot(!busy);
-        }
-        const int keep = exhausted ? 1 : kRefillBelow;
-        if (__popcll(__ballot(busy)) == 0) break;
-        // Traverse until fewer than `keep` lanes are busy.
-        for (;;) {
-            if (busy) {
-                if (kAny) {
-                    if (aw.active) any_step(v, aw);
-                } else {
-                    if (cw.active) closest_step(v, cw, ring + threadIdx.x, kTraceBlock);
-                }
-                if (!(kAny ? aw.active : cw.active)) {
-                    hit[k] = kAny ? aw.hit : cw.hit;
-                    dist[k] = kAny ? aw.dist : cw.dist;
-                    busy = false;
-                }
-            }
-            if (__popcll(__ballot(busy)) < keep) break;
-        }
-    }
+    trace_persistent<Tri, kAny>(scene_view(s), rays, count, hit, dist, counters, ring + threadIdx.x, kTraceBlock,
+                                kTraceChunk);
 }
 
 RGI_DEV uint32_t unorm_byte(float v)
