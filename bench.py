@@ -62,9 +62,16 @@ def measured_traffic(pass_name, options, width, height):
         summary = json.load(fh)
     waves = {"gi_temporal": options.get(ra.RGI_OPT_GI_TEMPORAL_WAVES, 0),
              "gi_spatial": options.get(ra.RGI_OPT_GI_SPATIAL_WAVES, 0)}.get(pass_name, 0)
-    kernel = {"grid_insert": "k_grid_insert_atomic"}.get(pass_name, "k_" + pass_name) + ("_w4" if waves == 4 else "")
-    k = summary["kernels"].get(kernel)
-    if k is None:
+    suffix = "_w4" if waves == 4 else ""
+    # GI temporal is several kernels (surface-tile listing, candidate march,
+    # BVH walks, resampling): its traffic is their sum.
+    k = summary.get("passes", {}).get(pass_name + suffix)
+    if k is not None:
+        kernel = " + ".join(k["kernels"])
+    else:
+        kernel = {"grid_insert": "k_grid_insert_atomic"}.get(pass_name, "k_" + pass_name) + suffix
+        k = summary["kernels"].get(kernel)
+    if k is None or "fetch_bytes" not in k or "write_bytes" not in k:
         return None
     return {"bytes": round(k["fetch_bytes"] + k["write_bytes"]), "fetch_bytes": round(k["fetch_bytes"]),
             "write_bytes": round(k["write_bytes"]), "kernel": kernel, "source": "profiles/cornell1080p_kernels.txt"}

@@ -169,7 +169,9 @@ int rgi_abi_version(void);
 uint32_t rgi_abi_sizeof(int which);
 
 /* Library state: scratch buffers of the deterministic grid insert and of the
- * traversal queue, grown on demand. One context per host thread. */
+ * traversal queue, and GI temporal's surface-tile list and candidate records
+ * (about 56 bytes per GI pixel and candidate), grown on demand. One context
+ * per host thread; calls that share a context run in stream order. */
 typedef struct rgi_context rgi_context;
 int rgi_create(rgi_context** out_context);
 void rgi_destroy(rgi_context* context);
@@ -233,7 +235,10 @@ int rgi_grid_insert_noalloc(const rgi_frame* frame, const rgi_gbuffer* gbuffer, 
 /* cs_restir_temporal (shaders/cs_restir_temporal.sc:57-297, dispatched at
  * renderer.cpp:807-817): fresh candidates, temporal reuse through last
  * frame's grid (inputs->grid), boiling filter; writes this frame's GI
- * reservoirs. */
+ * reservoirs. With a context (and at most 8 candidates) the pass lists its
+ * surface tiles, marches the candidates and walks the rays the march leaves
+ * open in kernels of their own before resampling; without one it runs as
+ * one kernel. Both give the same result. */
 int rgi_gi_temporal(rgi_context* context, const rgi_frame* frame, const rgi_scene* scene,
                     const rgi_gbuffer* gbuffer, const rgi_gi_inputs* inputs,
                     const rgi_gi_reservoirs* out, void* stream);

@@ -482,12 +482,12 @@ k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_
 // pixel's candidate directions in order, marches each ray through the depth
 // buffer and records the hit; the rays the march leaves open are listed for
 // the BVH (one list append per workgroup). Apart from the resampling, the
-// march needs few registers, so many waves per SIMD hide its chain of
-// dependent depth loads. Every workgroup runs the same number of rounds of
-// 4 tiles (one per wave).
+// march needs few registers, so 8 waves per SIMD (64 VGPRs) hide its chain
+// of dependent depth loads. Every workgroup runs the same number of rounds
+// of 4 tiles (one per wave).
 template <bool kSameSize>
-__global__ void __launch_bounds__(256) k_gi_candidates(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list,
-                                                       CandRays c)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_gi_candidates(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list, CandRays c)
 {
     __shared__ uint32_t wave_base[4];
     GiPass p;
