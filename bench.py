@@ -191,47 +191,72 @@ def nrc_quality(L, hs, gscene, gbuf, view, W, H, T, mode, acc, acc_gi, ref_frame
     `frames` frames from an empty history at the static camera with the
     reference's default settings, measured against the same `ref_frames`
     reference accumulations: the GI estimate alone (the GI frame's output,
-    colour x albedo) and the composite. Variants: plain; the cache standing in
-    for path continuation; the same with the composite resolved from the
-    cache; the plain path with PLAIN_HEAVY's extra samples. Each variant's
-    full frame (denoiser and composite included) is timed over its last 32
-    frames."""
+    colour x albedo) and the composite, each the mean of the MSE of the last
+    16 frames (one frame's MSE varies by about 15% from run to run). Variants:
+    plain; the cache standing in for path continuation (its default records,
+    65,536 records, records at pixels, mixed records); the cache with the
+    composite resolved from it (pixel and path-vertex records); the plain path
+    with PLAIN_HEAVY's extra samples. Each variant's full frame (denoiser
+    and composite included) is timed over the 32 frames before the measured
+    ones."""
     import torch
     from restir_amd import nrc as rnrc
     res = torch.zeros(5, dtype=torch.float64, device="cuda")
     out = ra.RendererOutputs()
     result = {"frames": frames, "reference_frames": ref_frames, "plain_heavy_settings": PLAIN_HEAVY}
-    for name in ("plain", "cache", "resolved", "plain_heavy"):
+    # By default the cache trains on 16,384 records per frame at ReSTIR GI path
+    # vertices (at surface pixels when it resolves the composite); beside it
+    # the same at the fourth configuration's 65,536 records, with records at
+    # surface pixels, every other one at a path vertex, and the resolved
+    # composite from path-vertex records.
+    variants = {"cache": {}, "resolved": {"resolve": 1}, "cache_65536": {"batch": 65536},
+                "cache_pixel_records": {"records": ra.RGI_NRC_RECORDS_PIXELS},
+                "cache_mixed_records": {"records": ra.RGI_NRC_RECORDS_MIXED},
+                "resolved_vertex_records": {"resolve": 1, "records": ra.RGI_NRC_RECORDS_VERTICES}}
+    measured = 16
+    for name in ("plain", "cache", "resolved", "plain_heavy", "cache_65536", "cache_pixel_records",
+                 "cache_mixed_records", "resolved_vertex_records"):
         settings = ra.Settings(grid_table_size=T, grid_mode=mode, **(PLAIN_HEAVY if name == "plain_heavy" else {}))
         rend = ctypes.c_void_p()
         ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
                                        ctypes.byref(gscene.scene), hs.radius))
-        if name in ("cache", "resolved"):
-            ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs, resolve=int(name == "resolved")))))
-        timed = min(32, frames)
-        for k in range(frames):
-            if k == frames - timed:
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
+        if name in variants:
+            ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs, **variants[name]))))
+        # frames - 48 frames to settle, 32 timed, then 16 measured.
+        timed = 32
+        for k in range(frames - measured - timed):
             ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
         torch.cuda.synchronize()
-        line = {"ms_per_frame": round((time.perf_counter() - t0) * 1e3 / timed, 4)}
-        ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
-        ra.check(L.rgi_reference_error(ctx, ctypes.byref(gbuf.gb), out.color, None, None, acc_gi.data_ptr(),
-                                       ref_frames, res.data_ptr(), s))
-        r = res.cpu().numpy()
+        t0 = time.perf_counter()
+        for k in range(timed):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / timed
+        gi_sum, comp_sum = np.zeros(4), np.zeros(4)
+        for k in range(measured):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+            ra.check(L.rgi_renderer_outputs_get(rend, ctypes.byref(out)))
+            ra.check(L.rgi_reference_error(ctx, ctypes.byref(gbuf.gb), out.color, None, None, acc_gi.data_ptr(),
+                                           ref_frames, res.data_ptr(), s))
+            gi_sum += res.cpu().numpy()[:4]
+            ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), ref_frames,
+                                                     res.data_ptr(), s))
+            comp_sum += res.cpu().numpy()[:4]
+        line = {"ms_per_frame": round(ms, 4)}
+        r = gi_sum / measured
         line["gi"] = {"mse_rgb": float(r[1]), "relative_mse": float(r[0]), "mean_luminance": float(r[2]),
                       "reference_mean_luminance": float(r[3])}
-        ra.check(L.rgi_reference_error_composite(ctx, ctypes.byref(gbuf.gb), out.hdr, acc.data_ptr(), ref_frames,
-                                                 res.data_ptr(), s))
-        r = res.cpu().numpy()
+        r = comp_sum / measured
         line["composite"] = {"mse_rgb": float(r[1]), "relative_mse": float(r[0]), "mean_luminance": float(r[2]),
                              "reference_mean_luminance": float(r[3])}
         result[name] = line
         L.rgi_renderer_destroy(rend)
     comp = lambda k: result[k]["composite"]["mse_rgb"]
     result["gi_mse_plain_over_cache"] = round(result["plain"]["gi"]["mse_rgb"] / result["cache"]["gi"]["mse_rgb"], 4)
+    for k in ("cache_65536", "cache_pixel_records", "cache_mixed_records"):
+        result["gi_mse_plain_over_" + k] = round(result["plain"]["gi"]["mse_rgb"] / result[k]["gi"]["mse_rgb"], 4)
     result["composite_mse_plain_over_resolved"] = round(comp("plain") / comp("resolved"), 3)
+    result["composite_mse_plain_over_resolved_vertex_records"] = round(comp("plain") / comp("resolved_vertex_records"), 3)
     result["composite_mse_plain_heavy_over_resolved"] = round(comp("plain_heavy") / comp("resolved"), 3)
     result["plain_heavy_frame_at_least_as_long"] = result["plain_heavy"]["ms_per_frame"] >= result["resolved"]["ms_per_frame"]
     return result
@@ -374,27 +399,34 @@ def cache_frames(hs, W, H, T, frames_in_path, steps):
 
 def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, steps, plain_ms, s):
     """Cost of the cache: the metric's frame with the cache on (same path,
-    same settings), and its kernels alone with HIP events: 2M queries per
-    precision (the 1080p frame's per-pixel query count) and one training
-    step of the default batch of surface records, with the MLP's matrix-core
-    throughput."""
+    same settings; the default 16,384 and 65,536 training records per frame),
+    and its kernels alone with HIP events: 2M queries per precision (the 1080p
+    frame's per-pixel query count) and one training step of 16,384 and of
+    65,536 records, with the MLP's matrix-core throughput."""
     import torch
     from restir_amd import nrc as rnrc
-    rend = ctypes.c_void_p()
-    ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
-                                   ctypes.byref(gscene.scene), hs.radius))
-    ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(rnrc.renderer_settings(hs))))
-    for k in range(16):
-        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[k % frames_in_path]),
-                                       ctypes.byref(gbufs[k % frames_in_path].gb), s))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[k % frames_in_path]),
-                                       ctypes.byref(gbufs[k % frames_in_path].gb), s))
-    torch.cuda.synchronize()
-    frame_ms = (time.perf_counter() - t0) * 1e3 / steps
-    L.rgi_renderer_destroy(rend)
+
+    def cache_frame_ms(nrc_settings):
+        rend = ctypes.c_void_p()
+        ra.check(L.rgi_renderer_create(ctypes.byref(rend), W, H, ctypes.byref(ra.c_settings(settings)),
+                                       ctypes.byref(gscene.scene), hs.radius))
+        ra.check(L.rgi_renderer_set_nrc(rend, ctypes.byref(nrc_settings)))
+        for k in range(16):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[k % frames_in_path]),
+                                           ctypes.byref(gbufs[k % frames_in_path].gb), s))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ra.check(L.rgi_renderer_render(rend, ctypes.byref(views[k % frames_in_path]),
+                                           ctypes.byref(gbufs[k % frames_in_path].gb), s))
+        torch.cuda.synchronize()
+        L.rgi_renderer_destroy(rend)
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    frame_ms = cache_frame_ms(rnrc.renderer_settings(hs))
+    # The cache as the fourth configuration sets it: 65,536 training records
+    # per frame (ReSTIR GI path vertices, as by default).
+    frame_ms_65536 = cache_frame_ms(rnrc.renderer_settings(hs, batch=65536))
 
     # The kernels alone, on the frame's own data: the per-pixel query of the
     # first camera's G-buffer and one training step on records gathered from
@@ -432,6 +464,12 @@ def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, step
 
     gather_ms = timed(gather)
     train_ms = timed(lambda: cache.train(train_rec, train_tgt))
+    big_rec = torch.empty((65536, 12), dtype=torch.float32, device="cuda")
+    big_tgt = torch.empty((65536, 4), dtype=torch.float32, device="cuda")
+    gi_res = ra.GiReservoirs(outs.gi_sample, outs.gi_receiver, outs.gi_normal)
+    ra.check(L.rgi_nrc_gather_vertices(cache.handle, ctypes.byref(frame), ctypes.byref(gbufs[0].gb), ctypes.byref(gi_res),
+                                       outs.di_diffuse, outs.color, 65536, 7, big_rec.data_ptr(), big_tgt.data_ptr(), s))
+    train_65536_ms = timed(lambda: cache.train(big_rec, big_tgt))
     queries = W * H
     # Waves whose 16 pixels are all sky skip the encoding and the MLP, so the
     # useful work is the surface pixels' queries.
@@ -445,7 +483,9 @@ def nrc_timing(L, hs, settings, views, gbufs, gscene, W, H, frames_in_path, step
     cache.close()
     tflops = surface_queries * NRC_FLOP_PER_QUERY / (infer_ms["bf16"] * 1e-3) / 1e12
     return {"frame_ms_with_cache": round(frame_ms, 5), "frame_ms_plain": round(plain_ms, 5),
+            "frame_ms_with_cache_65536": round(frame_ms_65536, 5),
             "gather_ms": round(gather_ms, 5), "train_ms": round(train_ms, 5), "train_batch": batch,
+            "train_ms_65536": round(train_65536_ms, 5),
             "query_ms": {k: round(v, 5) for k, v in infer_ms.items()}, "queries": queries,
             "surface_queries": surface_queries,
             "mfma": {"kernel": "k_nrc_infer_gbuffer (bf16)", "achieved": round(tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",

@@ -164,7 +164,7 @@ const char* rgi_last_error(void);
 int rgi_abi_version(void);
 /* sizeof of the ABI structs (0: rgi_frame, 1: rgi_scene, 2: rgi_gbuffer,
  * 3: rgi_gi_reservoirs, 4: rgi_gi_inputs, 5: rgi_di_reservoirs, 6: rgi_shadow,
- * 7: rgi_composite_inputs), so that
+ * 7: rgi_composite_inputs, 8: rgi_nrc_config, 9: rgi_nrc_settings), so that
  * bindings can check their mirrors. */
 uint32_t rgi_abi_sizeof(int which);
 

@@ -149,6 +149,19 @@ int rgi_nrc_gather_gbuffer(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuff
                            const uint16_t* di_diffuse, const uint16_t* color, uint32_t batch, uint32_t seed,
                            float* records, float* targets, void* stream);
 
+/* The same records at the frame's ReSTIR GI path vertices: record i is drawn
+ * from the same stratified surface pixel, which is then replaced by the
+ * pixel where that pixel's GI reservoir sample lies (reservoirs: the frame's
+ * temporal reservoirs, GI-size planes; sample = point xyz and M, normal.w =
+ * sample id), when the sample is a surface seen on screen (it projects onto
+ * the screen and the depth buffer there reconstructs it within 0.1 scene
+ * units, evaluateSample's test, restir.sh:454-461); a pixel whose sample is
+ * hidden, the sky or absent (M = 0) keeps its own record. The target is taken
+ * at the record's pixel as for rgi_nrc_gather_gbuffer. */
+int rgi_nrc_gather_vertices(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuffer* gbuffer,
+                            const rgi_gi_reservoirs* reservoirs, const uint16_t* di_diffuse, const uint16_t* color,
+                            uint32_t batch, uint32_t seed, float* records, float* targets, void* stream);
+
 /* ---------------------------------------------------------------------
  * The cache in the frame driver (include/restir_amd_renderer.h).
  *
@@ -161,8 +174,11 @@ int rgi_nrc_gather_gbuffer(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuff
  *     its emission when the G-buffer emission is the surface light), where
  *     the reference reads the DI light plus last frame's indirect light. The
  *     GI target function and the shaded samples both use it;
- *  2. after the denoiser, gathers `batch` training records from surface
- *     pixels stratified over the image, each with target DI diffuse +
+ *  2. after the denoiser, gathers `batch` training records at the ReSTIR GI
+ *     path vertices that the reservoirs of surface pixels stratified over
+ *     the image hold (RGI_NRC_RECORDS_VERTICES, see rgi_nrc_gather_vertices;
+ *     _PIXELS keeps the pixels themselves, _MIXED moves every other record),
+ *     each with target DI diffuse +
  *     albedo * the next frame's GI history colour (the ReSTIR estimate of the light the
  *     surface reflects, which already contains the cache's own prediction at
  *     the sample points: self-training), and takes one Adam step. The step
@@ -172,6 +188,9 @@ int rgi_nrc_gather_gbuffer(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuff
  *     for it before it reads the cache; rgi_renderer_render_graph and
  *     batches join it within the frame. rgi_renderer_nrc_state waits for it
  *     on the host, so the state it returns is the last frame's.
+ * With `resolve` set the cache is queried at every pixel, so records at the
+ * pixels (RGI_NRC_RECORDS_PIXELS) suit it better than path vertices, which
+ * crowd where GI samples land.
  * With `resolve` set (and multiple bounces on, so that the target is the
  * surface's whole diffuse light), the composite also takes the query of
  * step 1 at each pixel as the pixel's diffuse light, in place of the DI
@@ -186,12 +205,19 @@ int rgi_nrc_gather_gbuffer(rgi_nrc* nrc, const rgi_frame* frame, const rgi_gbuff
  * accumulates, the frame is the plain ReSTIR path. */
 typedef struct rgi_renderer rgi_renderer;
 
+/* Where the frame driver draws its training records. */
+#define RGI_NRC_RECORDS_PIXELS 0   /* surface pixels (rgi_nrc_gather_gbuffer) */
+#define RGI_NRC_RECORDS_VERTICES 1 /* ReSTIR GI path vertices (rgi_nrc_gather_vertices) */
+#define RGI_NRC_RECORDS_MIXED 2    /* the two in turn: even records at path vertices, odd ones at pixels */
+
 typedef struct rgi_nrc_settings {
     rgi_nrc_config cache;     /* aabb: the scene's bounds */
-    uint32_t batch;           /* 16384: training records per frame (surface pixels) */
+    uint32_t batch;           /* 16384: training records per frame */
     uint32_t warmup;          /* 4: training frames before the GI passes read the cache */
     int32_t precision;        /* RGI_NRC_BF16: precision of the per-pixel queries */
     int32_t resolve;          /* 0; 1: the composite takes the cache's prediction as each pixel's diffuse light */
+    int32_t records;          /* RGI_NRC_RECORDS_VERTICES (default), _PIXELS (better with resolve) or _MIXED */
+    int32_t reserved[3];
 } rgi_nrc_settings;
 
 void rgi_nrc_settings_default(rgi_nrc_settings* settings);

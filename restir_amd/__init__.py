@@ -138,15 +138,18 @@ class NrcConfig(ctypes.Structure):
 
 class NrcSettings(ctypes.Structure):
     """rgi_nrc_settings (include/restir_amd_nrc.h): the cache in the frame driver."""
-    _fields_ = [("cache", NrcConfig), ("batch", U32), ("warmup", U32), ("precision", I32), ("resolve", I32)]
+    _fields_ = [("cache", NrcConfig), ("batch", U32), ("warmup", U32), ("precision", I32), ("resolve", I32),
+                ("records", I32), ("reserved", I32 * 3)]
 
 
 RGI_NRC_BF16, RGI_NRC_FP16, RGI_NRC_FP8 = 0, 1, 2
+RGI_NRC_RECORDS_PIXELS, RGI_NRC_RECORDS_VERTICES, RGI_NRC_RECORDS_MIXED = 0, 1, 2
 RGI_NRC_MLP_PARAMS = 5 * 64 * 64 + 3 * 64
 RGI_NRC_LEVELS = 16
 
 
-ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs, Shadow, CompositeInputs]
+ABI_STRUCTS = [Frame, Scene, GBuffer, GiReservoirs, GiInputs, DiReservoirs, Shadow, CompositeInputs, NrcConfig,
+               NrcSettings]
 
 
 class RestirError(RuntimeError):
@@ -245,6 +248,8 @@ def lib():
     L.rgi_nrc_steps.argtypes = [P, ctypes.POINTER(U32)]
     L.rgi_nrc_query_gbuffer.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, ctypes.c_int, P, P]
     L.rgi_nrc_gather_gbuffer.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), P, P, U32, U32, P, P, P]
+    L.rgi_nrc_gather_vertices.argtypes = [P, ctypes.POINTER(Frame), ctypes.POINTER(GBuffer), ctypes.POINTER(GiReservoirs),
+                                          P, P, U32, U32, P, P, P]
     L.rgi_nrc_settings_default.restype = None
     L.rgi_nrc_settings_default.argtypes = [ctypes.POINTER(NrcSettings)]
     L.rgi_renderer_set_nrc.argtypes = [P, ctypes.POINTER(NrcSettings)]

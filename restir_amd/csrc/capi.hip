@@ -193,6 +193,8 @@ uint32_t rgi_abi_sizeof(int which)
     case 5: return (uint32_t)sizeof(rgi_di_reservoirs);
     case 6: return (uint32_t)sizeof(rgi_shadow);
     case 7: return (uint32_t)sizeof(rgi_composite_inputs);
+    case 8: return (uint32_t)sizeof(rgi_nrc_config);
+    case 9: return (uint32_t)sizeof(rgi_nrc_settings);
     default: return 0;
     }
 }

@@ -106,8 +106,10 @@ int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
 // radiance at every G-buffer pixel, and one frame's training records.
 int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* emission, int precision,
                       uint16_t* out, hipStream_t stream);
+// With gi_sample and gi_normal, every vertex_every-th record is taken at
+// the ReSTIR GI path vertex of its pixel (rgi_nrc_gather_vertices).
 int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
-               const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
-               hipStream_t stream);
+               const uint16_t* color, const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
+               uint32_t batch, uint32_t seed, float* records, float* targets, hipStream_t stream);
 
 } // namespace rgi

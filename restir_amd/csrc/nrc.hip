@@ -1100,7 +1100,17 @@ __global__ void __launch_bounds__(256) k_nrc_surface_list(const float* depth, ui
 // (as the G-buffer query computes them) and the target radiance di_diffuse +
 // albedo * color (either term optional); weight 0 for every record when the
 // G-buffer has no surface pixel.
+//
+// With the GI reservoirs (gi_sample, gi_normal: GI-size planes, the frame's
+// temporal reservoirs), the record of every `vertex_every`-th i (from i = 0)
+// is the path vertex that ReSTIR GI chose for that pixel instead: its
+// reservoir's sample point, when the sample is a surface seen on screen
+// (evaluateSample's visibility test, restir.sh:454-461: the point projects
+// onto the screen and the depth buffer there reconstructs it within 0.1 scene
+// units), taken at the pixel where it lies; a pixel whose sample is hidden,
+// the sky or absent keeps its own record.
 __global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_t* di_diffuse, const uint16_t* color,
+                             const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
                              const uint32_t* surface_total, const uint32_t* surface_list, uint32_t batch, uint32_t seed,
                              float* records, float* targets)
 {
@@ -1114,6 +1124,24 @@ __global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_
         p = surface_list[(uint32_t)(((uint64_t)i * ns + offset) / batch)];
     }
     int x = (int)(p % (uint32_t)g.width), y = (int)(p / (uint32_t)g.width);
+    if (gi_sample && ns > 0 && i % vertex_every == 0u) {
+        // The pixel's GI reservoir (GI-size planes, point-sampled at its uv).
+        F2 puv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
+        uint32_t gt = texel_index(puv, f.gi_width, f.gi_height);
+        F4 sample = load_f4(gi_sample, gt);
+        float id = load_f4(gi_normal, gt).w;
+        F3 sp = xyz(sample);
+        F2 suv;
+        float sview;
+        if (sample.w > 0.0f && !(id > kSkySampleId * 0.5f) && project_to_screen(f, sp, suv, sview)) {
+            uint32_t st = texel_index(suv, g.width, g.height);
+            float sdepth = g.depth[st];
+            if (sdepth < kSkyDepth && length(reconstruct_position(f, suv, sdepth) - sp) <= 0.1f * f.scene_scale) {
+                x = (int)(st % (uint32_t)g.width);
+                y = (int)(st / (uint32_t)g.width);
+            }
+        }
+    }
     uint32_t t = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     float depth = g.depth[t];
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -1171,8 +1199,8 @@ int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, cons
 }
 
 int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
-               const uint16_t* color, uint32_t batch, uint32_t seed, float* records, float* targets,
-               hipStream_t stream)
+               const uint16_t* color, const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
+               uint32_t batch, uint32_t seed, float* records, float* targets, hipStream_t stream)
 {
     uint32_t npix = (uint32_t)g.width * (uint32_t)g.height;
     uint32_t blocks = (npix + nrc::kSurfaceBlock - 1) / nrc::kSurfaceBlock;
@@ -1197,7 +1225,8 @@ int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint1
                        n->surface_list);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
     hipLaunchKernelGGL(nrc::k_nrc_gather, dim3((batch + 255) / 256), dim3(256), 0, stream, f, g, texel, di_diffuse,
-                       color, n->surface_offsets + blocks, n->surface_list, batch, seed, records, targets);
+                       color, gi_sample, gi_normal, vertex_every, n->surface_offsets + blocks, n->surface_list, batch,
+                       seed, records, targets);
     return check_launch("nrc gather");
 }
 
@@ -1600,7 +1629,23 @@ int rgi_nrc_gather_gbuffer(rgi_nrc* n, const rgi_frame* f, const rgi_gbuffer* g,
         return fail(RGI_E_INVALID, "nrc_gather_gbuffer: bad argument");
     if (g->width <= 0 || g->height <= 0 || !g->depth || !g->normal || !g->albedo)
         return fail(RGI_E_INVALID, "nrc_gather_gbuffer: depth, normal and albedo planes are required");
-    return nrc_gather(n, *f, *g, di_diffuse, color, batch, seed, records, targets, as_stream(stream));
+    return nrc_gather(n, *f, *g, di_diffuse, color, nullptr, nullptr, 1u, batch, seed, records, targets,
+                      as_stream(stream));
+}
+
+int rgi_nrc_gather_vertices(rgi_nrc* n, const rgi_frame* f, const rgi_gbuffer* g, const rgi_gi_reservoirs* reservoirs,
+                            const uint16_t* di_diffuse, const uint16_t* color, uint32_t batch, uint32_t seed,
+                            float* records, float* targets, void* stream)
+{
+    if (!n || !f || !g || !reservoirs || !records || !targets || batch == 0)
+        return fail(RGI_E_INVALID, "nrc_gather_vertices: bad argument");
+    if (g->width <= 0 || g->height <= 0 || !g->depth || !g->normal || !g->albedo)
+        return fail(RGI_E_INVALID, "nrc_gather_vertices: depth, normal and albedo planes are required");
+    if (!reservoirs->sample || !reservoirs->normal) return fail(RGI_E_INVALID, "nrc_gather_vertices: reservoir planes are required");
+    if (f->gi_width <= 0 || f->gi_height <= 0 || f->gi_width > g->width || f->gi_height > g->height)
+        return fail(RGI_E_INVALID, "nrc_gather_vertices: GI size %d x %d out of range", f->gi_width, f->gi_height);
+    return nrc_gather(n, *f, *g, di_diffuse, color, reservoirs->sample, reservoirs->normal, 1u, batch, seed, records,
+                      targets, as_stream(stream));
 }
 
 int rgi_nrc_get_average(const rgi_nrc* n, float* mlp, float* table, void* stream)

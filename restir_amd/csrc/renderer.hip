@@ -599,9 +599,14 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     if (nrc_on) {
         join_nrc(r, stream);
         const uint16_t* next_history = s.denoiser ? r->filtered[write] : r->color[write];
+        // Records at the frame's ReSTIR GI path vertices (the temporal
+        // reservoirs' sample points), at its surface pixels, or alternately.
+        const int rec = r->nrc_settings.records;
+        const bool vertices = rec == RGI_NRC_RECORDS_VERTICES || rec == RGI_NRC_RECORDS_MIXED;
         if ((e = nrc_gather(r->nrc, f, *g, di_active ? r->diffuse : nullptr, s.multiple_bounces ? next_history : nullptr,
-                            r->nrc_settings.batch, f.frame_index ^ 0x4e5243u, r->nrc_records, r->nrc_targets,
-                            stream)) != RGI_OK)
+                            vertices ? r->gi[write][0] : nullptr, vertices ? r->gi[write][2] : nullptr,
+                            rec == RGI_NRC_RECORDS_MIXED ? 2u : 1u, r->nrc_settings.batch, f.frame_index ^ 0x4e5243u,
+                            r->nrc_records, r->nrc_targets, stream)) != RGI_OK)
             return e;
         hipStream_t train_stream = stream;
         if (fork_grid) {
@@ -793,6 +798,7 @@ void rgi_nrc_settings_default(rgi_nrc_settings* s)
     s->batch = 16384;
     s->warmup = 4;
     s->precision = RGI_NRC_BF16;
+    s->records = RGI_NRC_RECORDS_VERTICES;
 }
 
 int rgi_renderer_set_nrc(rgi_renderer* r, const rgi_nrc_settings* settings)
@@ -804,6 +810,8 @@ int rgi_renderer_set_nrc(rgi_renderer* r, const rgi_nrc_settings* settings)
         return fail(RGI_E_INVALID, "set_nrc: unknown precision %d", settings->precision);
     if (settings && settings->resolve != 0 && settings->resolve != 1)
         return fail(RGI_E_INVALID, "set_nrc: resolve takes 0 or 1, not %d", settings->resolve);
+    if (settings && (settings->records < RGI_NRC_RECORDS_PIXELS || settings->records > RGI_NRC_RECORDS_MIXED))
+        return fail(RGI_E_INVALID, "set_nrc: unknown record source %d", settings->records);
     (void)hipDeviceSynchronize();
     release_nrc(r);
     if (!settings) return RGI_OK;

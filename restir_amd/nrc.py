@@ -104,16 +104,21 @@ class NeuralCache:
         return s.value
 
 
-def renderer_settings(hs, batch=16384, warmup=4, precision=ra.RGI_NRC_BF16, resolve=0, margin=0.02, **kw):
+def renderer_settings(hs, batch=16384, warmup=4, precision=ra.RGI_NRC_BF16, resolve=0, margin=0.02, records=None,
+                      **kw):
     """rgi_nrc_settings for a HostScene: the cache's bounds are the scene's
-    triangle bounds, widened by `margin` of their extent."""
+    triangle bounds, widened by `margin` of their extent. Records default to
+    ReSTIR GI path vertices, or to surface pixels with `resolve` (the
+    resolved image queries the cache at every pixel)."""
+    if records is None:
+        records = ra.RGI_NRC_RECORDS_PIXELS if resolve else ra.RGI_NRC_RECORDS_VERTICES
     st = ra.NrcSettings()
     ra.lib().rgi_nrc_settings_default(ctypes.byref(st))
     p = hs.positions.reshape(-1, 3)
     lo, hi = p.min(axis=0), p.max(axis=0)
     pad = (hi - lo) * margin + 1e-3
     st.cache = config(aabb_min=lo - pad, aabb_max=hi + pad, **kw)
-    st.batch, st.warmup, st.precision, st.resolve = batch, warmup, precision, resolve
+    st.batch, st.warmup, st.precision, st.resolve, st.records = batch, warmup, precision, resolve, records
     return st
 
 

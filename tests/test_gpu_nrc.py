@@ -530,3 +530,66 @@ def test_gather_stratifies_over_surface_pixels():
     assert torch.all(tgt[:, 3] == 0)
     nc.close()
     L.rgi_destroy(ctx)
+
+
+def test_gather_at_path_vertices():
+    """Records at ReSTIR GI path vertices: with one record per surface pixel,
+    a pixel whose reservoir holds the surface point of another pixel (its
+    reconstructed position, as the pixel gather records it) gives that
+    pixel's record, target included; a pixel whose reservoir is empty (M = 0),
+    holds a sky sample or a point that the depth buffer does not show keeps
+    its own record."""
+    from restir_amd import driver
+    W, H = 320, 240
+    n = W * H
+    st = ra.Settings(grid_table_size=1 << 16)
+    hs, gscene, ctx, gbuf, view = _setup_scene(W, H, st)
+    L = ra.lib()
+    s = driver.stream()
+    eye, vp, inv = ra.orbit_camera(hs, 0, W, H, yaw_of=ra.swing_yaw)
+    frame = ra.make_frame(st, hs, 0, eye, vp, inv, eye, vp, W, H, False)
+    nc = rnrc.NeuralCache(rnrc.renderer_settings(hs).cache)
+    depth = gbuf.depth.view(torch.float32)
+    surface = (depth < 0.99999).cpu().numpy()
+    ns = int(surface.sum())
+    # Targets from a colour plane that differs per pixel, so that a record's
+    # target names its pixel as well as its inputs do.
+    color = torch.from_numpy(np.random.default_rng(3).uniform(0, 1, (n, 4)).astype(np.float16)).cuda()
+    full = torch.empty((ns, 12), dtype=torch.float32, device="cuda")
+    full_t = torch.empty((ns, 4), dtype=torch.float32, device="cuda")
+    ra.check(L.rgi_nrc_gather_gbuffer(nc.handle, ctypes.byref(frame), ctypes.byref(gbuf.gb), None, color.data_ptr(), ns,
+                                      1, full.data_ptr(), full_t.data_ptr(), s))
+    torch.cuda.synchronize()
+    f = full.cpu().numpy()
+    pix = np.flatnonzero(surface)                  # record i <-> surface pixel pix[i]
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(ns)                     # pixel pix[i]'s sample: pixel pix[perm[i]]'s point
+    kind = rng.integers(0, 4, ns)                  # 0, 1: moved; 2: empty reservoir; 3: sky sample
+    sample = np.zeros((n, 4), np.float32)
+    normal = np.zeros((n, 4), np.float32)
+    sample[pix, :3] = f[perm, :3]
+    sample[pix, 3] = np.where(kind == 2, 0.0, 3.0)
+    normal[pix, 3] = np.where(kind == 3, 30000000.0, 5.0)
+    # A few moved samples pushed 50 scene units further along the eye's ray
+    # through them: they project onto the same pixel, whose depth
+    # reconstructs a point 50 units away, so the depth buffer does not show
+    # them.
+    off = (kind == 1) & (rng.uniform(size=ns) < 0.25)
+    ray = f[perm[off], :3] - np.asarray(eye, np.float32)[None, :]
+    ray /= np.linalg.norm(ray, axis=1, keepdims=True)
+    sample[pix[off], :3] += np.float32(50.0 * frame.scene_scale) * ray
+    ts, tn = torch.from_numpy(sample).cuda(), torch.from_numpy(normal).cuda()
+    res = ra.GiReservoirs(ts.data_ptr(), ts.data_ptr(), tn.data_ptr())
+    rec = torch.empty((ns, 12), dtype=torch.float32, device="cuda")
+    tgt = torch.empty((ns, 4), dtype=torch.float32, device="cuda")
+    ra.check(L.rgi_nrc_gather_vertices(nc.handle, ctypes.byref(frame), ctypes.byref(gbuf.gb), ctypes.byref(res), None,
+                                       color.data_ptr(), ns, 1, rec.data_ptr(), tgt.data_ptr(), s))
+    torch.cuda.synchronize()
+    r, t, ft = rec.cpu().numpy(), tgt.cpu().numpy(), full_t.cpu().numpy()
+    moved = (kind <= 1) & ~off
+    assert moved.sum() > ns // 3
+    assert np.array_equal(r[moved], f[perm[moved]]) and np.array_equal(t[moved], ft[perm[moved]])
+    kept = ~moved
+    assert np.array_equal(r[kept], f[kept]) and np.array_equal(t[kept], ft[kept])
+    nc.close()
+    L.rgi_destroy(ctx)
