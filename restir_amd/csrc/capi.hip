@@ -85,22 +85,22 @@ void* context_scratch(rgi_context* ctx, size_t bytes)
     return ctx->scratch;
 }
 
-uint32_t* context_tile_list(rgi_context* ctx, size_t words)
+void* context_buffer(rgi_context* ctx, ContextBuffer which, size_t bytes)
 {
-    if (ctx->tile_list_words >= words) return ctx->tile_list;
-    if (ctx->tile_list) {
+    if (ctx->buf_bytes[which] >= bytes) return ctx->bufs[which];
+    if (ctx->bufs[which]) {
         hipDeviceSynchronize();
-        hipFree(ctx->tile_list);
-        ctx->tile_list = nullptr;
-        ctx->tile_list_words = 0;
+        hipFree(ctx->bufs[which]);
+        ctx->bufs[which] = nullptr;
+        ctx->buf_bytes[which] = 0;
     }
-    if (hipMalloc(&ctx->tile_list, words * sizeof(uint32_t)) != hipSuccess) {
-        ctx->tile_list = nullptr;
-        fail(RGI_E_NOMEM, "tile list allocation of %zu words failed", words);
+    if (hipMalloc(&ctx->bufs[which], bytes) != hipSuccess) {
+        ctx->bufs[which] = nullptr;
+        fail(RGI_E_NOMEM, "context buffer %d: allocation of %zu bytes failed", (int)which, bytes);
         return nullptr;
     }
-    ctx->tile_list_words = words;
-    return ctx->tile_list;
+    ctx->buf_bytes[which] = bytes;
+    return ctx->bufs[which];
 }
 
 int context_compute_units(rgi_context* ctx)
@@ -215,10 +215,12 @@ int rgi_create(rgi_context** out)
 void rgi_destroy(rgi_context* ctx)
 {
     if (!ctx) return;
-    if (ctx->scratch || ctx->counter || ctx->tile_list || ctx->cand) hipDeviceSynchronize();
+    bool any = ctx->scratch || ctx->counter;
+    for (void* b : ctx->bufs) any = any || b;
+    if (any) hipDeviceSynchronize();
     if (ctx->scratch) hipFree(ctx->scratch);
-    if (ctx->tile_list) hipFree(ctx->tile_list);
-    if (ctx->cand) hipFree(ctx->cand);
+    for (void* b : ctx->bufs)
+        if (b) hipFree(b);
     if (ctx->counter) hipFree(ctx->counter);
     delete ctx;
 }
@@ -361,7 +363,7 @@ int rgi_di_temporal(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* s
         return fail(RGI_E_INVALID, "di_light_count %d exceeds scene light_count %u", frame->di_light_count, scene->light_count);
     if (!prev || !out || !prev->sample || !prev->receiver || !prev->normal || !out->sample || !out->receiver || !out->normal)
         return fail(RGI_E_INVALID, "di temporal: reservoir planes are required");
-    return di_temporal(*frame, *scene, *g, *prev, *out, ctx ? ctx->tuning : Tuning{}, as_stream(stream));
+    return di_temporal(ctx, *frame, *scene, *g, *prev, *out, ctx ? ctx->tuning : Tuning{}, as_stream(stream));
 }
 
 int rgi_di_spatial(rgi_context* ctx, const rgi_frame* frame, const rgi_scene* scene, const rgi_gbuffer* g,

@@ -17,6 +17,8 @@ struct DiPass {
     const float4* lights;
     F2 texel;
     float sc;
+    const uint32_t* guide;    // the CDF's guide table (k_light_guide), or NULL
+    uint32_t guide_buckets;   // its K, a power of two <= 2^16
 };
 
 struct Surf {
@@ -53,10 +55,20 @@ RGI_DEV F4 light_row(const DiPass& p, int light, int row)
 
 RGI_DEV bool point_light(const DiPass& p, int light) { return light_row(p, light, 0).w > 0.5f; }
 
-// Power-proportional choice: binary search of the CDF (di.sh:123-142).
+// Power-proportional choice: binary search of the CDF (di.sh:123-142), for
+// the first light whose CDF value reaches `random`. With a guide table the
+// search starts from the bracket [guide[k], guide[k + 1]], k = floor(random
+// K), which holds that light (k_light_guide), and so ends on the same light
+// in fewer steps. `random` is a nextRandom value, a multiple of 2^-24, so
+// random K is exact.
 RGI_DEV int pick_light(const DiPass& p, float random, float& pmf)
 {
     int low = 0, high = p.f->di_light_count - 1;
+    if (p.guide) {
+        uint32_t k = (uint32_t)(random * (float)p.guide_buckets);
+        low = (int)p.guide[k];
+        high = (int)p.guide[k + 1u];
+    }
     for (int step = 0; step < 24 && low < high; ++step) {
         int middle = (low + high) >> 1;
         if (light_row(p, middle, 2).w < random) low = middle + 1;
@@ -153,7 +165,30 @@ RGI_DEV DiPass make_pass(const rgi_frame& f, const rgi_scene& s, const rgi_gbuff
     p.lights = reinterpret_cast<const float4*>(s.lights);
     p.texel = texel;
     p.sc = f.scene_scale;
+    p.guide = nullptr;
+    p.guide_buckets = 0u;
     return p;
+}
+
+// The CDF's guide table (the cutpoint method of Chen and Asau, 1974): entry k
+// of K + 1 is the first light whose CDF value reaches k / K (the last light
+// when none does), found by the same search as pick_light. A random r in
+// [k / K, (k + 1) / K) then picks a light between entries k and k + 1: every
+// light before entry k has a CDF value below k / K <= r, and entry k + 1's
+// reaches (k + 1) / K > r (or is the last light). K is a power of two, so k /
+// K is exact.
+__global__ void __launch_bounds__(256) k_light_guide(const float4* lights, int n, uint32_t buckets, uint32_t* guide)
+{
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > buckets) return;
+    float t = (float)k / (float)buckets;
+    int low = 0, high = n - 1;
+    for (int step = 0; step < 24 && low < high; ++step) {
+        int middle = (low + high) >> 1;
+        if (lights[middle * 4 + 2].w < t) low = middle + 1;
+        else high = middle;
+    }
+    guide[k] = (uint32_t)low;
 }
 
 // Light records staged in LDS per workgroup when the scene's fit: the light
@@ -173,10 +208,13 @@ RGI_DEV void stage_lights(DiPass& p, const rgi_frame& f, const rgi_scene& s, flo
 }
 
 __global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs prev,
-                                                     rgi_di_reservoirs out, F2 texel, TileMap tiles)
+                                                     rgi_di_reservoirs out, F2 texel, TileMap tiles,
+                                                     const uint32_t* guide, uint32_t guide_buckets)
 {
     __shared__ float4 lights[4 * kLdsLights];
     DiPass p = make_pass(f, s, g, texel);
+    p.guide = guide;
+    p.guide_buckets = guide_buckets;
     int x, y;
     if (!block_pixel(tiles, x, y)) return; // the whole workgroup, before the barrier
     stage_lights(p, f, s, lights);
@@ -391,13 +429,27 @@ __global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rg
 
 } // namespace
 
-int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
-                const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream)
+int di_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                const rgi_di_reservoirs& prev, const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream)
 {
     TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    // More lights than the workgroups stage in LDS: the light choice searches
+    // the CDF in memory, so it starts from a guide table built here (with a
+    // context to hold it).
+    uint32_t* guide = nullptr;
+    uint32_t buckets = 0;
+    const int n = f.di_light_count;
+    if (ctx && n > (int)kLdsLights && n <= (1 << 24)) {
+        buckets = 1024u;
+        while (buckets < (uint32_t)n && buckets < (1u << 16)) buckets <<= 1;
+        guide = static_cast<uint32_t*>(context_buffer(ctx, kBufLightGuide, (buckets + 1u) * sizeof(uint32_t)));
+        if (!guide) return RGI_E_NOMEM;
+        hipLaunchKernelGGL(k_light_guide, dim3((buckets + 256u) / 256u), dim3(256), 0, stream,
+                           reinterpret_cast<const float4*>(s.lights), n, buckets, guide);
+    }
     hipLaunchKernelGGL(k_di_temporal, tile_grid(tiles), dim3(256), 0, stream, f, s, g, prev, out,
-                       texel, tiles);
+                       texel, tiles, guide, buckets);
     return check_launch("di temporal");
 }
 

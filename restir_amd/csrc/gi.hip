@@ -700,23 +700,17 @@ int context_cand_rays(rgi_context* ctx, uint32_t candidates, CandRays& c)
     // rec: 16 B per candidate and pixel; rays: 32 B, hit and dist: 4 B each
     // per listed ray, at most one per candidate and pixel.
     const size_t n = (size_t)candidates * c.pixels;
+    if (n == 0) {
+        c.rec = c.rays = nullptr;
+        c.hit = nullptr;
+        c.dist = nullptr;
+        return RGI_OK;
+    }
     const size_t rec = align256(n * sizeof(float4)), rays = align256(2 * n * sizeof(float4)),
                  hits = align256(n * sizeof(int32_t)), dists = align256(n * sizeof(float));
-    const size_t bytes = rec + rays + hits + dists;
-    if (ctx->cand_bytes < bytes) {
-        if (ctx->cand) {
-            hipDeviceSynchronize();
-            hipFree(ctx->cand);
-            ctx->cand = nullptr;
-            ctx->cand_bytes = 0;
-        }
-        if (hipMalloc(&ctx->cand, bytes) != hipSuccess) {
-            ctx->cand = nullptr;
-            return fail(RGI_E_NOMEM, "candidate records: allocation of %zu bytes failed", bytes);
-        }
-        ctx->cand_bytes = bytes;
-    }
-    char* base = static_cast<char*>(ctx->cand);
+    void* buffer = context_buffer(ctx, kBufCandidates, rec + rays + hits + dists);
+    if (!buffer) return RGI_E_NOMEM;
+    char* base = static_cast<char*>(buffer);
     c.rec = reinterpret_cast<float4*>(base);
     c.rays = reinterpret_cast<float4*>(base + rec);
     c.hit = reinterpret_cast<int32_t*>(base + rec + rays);
@@ -736,7 +730,8 @@ static int listed_buffers(rgi_context* ctx, const rgi_frame& f, TileList& list, 
 {
     list.tiles_x = ((uint32_t)f.gi_width + 7u) / 8u;
     list.tiles = list.tiles_x * (((uint32_t)f.gi_height + 7u) / 8u);
-    list.words = context_tile_list(ctx, kTileListWord + list.tiles);
+    list.words = static_cast<uint32_t*>(
+        context_buffer(ctx, kBufTileList, (size_t)(kTileListWord + list.tiles) * sizeof(uint32_t)));
     if (!list.words) return RGI_E_NOMEM;
     c.pixels = (uint32_t)f.gi_width * (uint32_t)f.gi_height;
     return context_cand_rays(ctx, (uint32_t)f.initial_candidates, c);

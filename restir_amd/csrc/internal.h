@@ -27,10 +27,11 @@ struct rgi_context {
     uint32_t* counter = nullptr;   // persistent-launch work counter
     void* scratch = nullptr;       // grid insert (deterministic) and other per-call scratch
     size_t scratch_bytes = 0;
-    uint32_t* tile_list = nullptr; // the GI passes' surface tiles (apart from `scratch`: the renderer's
-    size_t tile_list_words = 0;    // grid insert runs on a side stream beside GI temporal)
-    void* cand = nullptr;          // the GI temporal pass's candidate records and rays
-    size_t cand_bytes = 0;
+    // Buffers of single passes, apart from `scratch` (the renderer's grid
+    // insert runs on a side stream beside GI temporal and the DI passes):
+    // rgi::kBuf* name them.
+    void* bufs[3] = {};
+    size_t buf_bytes[3] = {};
     int compute_units = 0;
 };
 
@@ -40,9 +41,10 @@ int fail(int code, const char* fmt, ...);
 int check_launch(const char* what);
 // Scratch of at least `bytes`, grown (after a device sync) when too small.
 void* context_scratch(rgi_context* ctx, size_t bytes);
-// The tile list of at least `words` words, grown (after a device sync) when
-// too small; NULL (with the error set) when it cannot be allocated.
-uint32_t* context_tile_list(rgi_context* ctx, size_t words);
+// A context buffer of at least `bytes`, grown (after a device sync) when too
+// small; NULL (with the error set) when it cannot be allocated.
+enum ContextBuffer { kBufTileList = 0, kBufCandidates = 1, kBufLightGuide = 2 };
+void* context_buffer(rgi_context* ctx, ContextBuffer which, size_t bytes);
 // The device's compute units, looked up once per context.
 int context_compute_units(rgi_context* ctx);
 struct CandRays;
@@ -97,8 +99,10 @@ int gi_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const 
 int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
                hipStream_t stream);
-int di_temporal(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
-                const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream);
+// With a context, scenes with more lights than fit in LDS search the CDF
+// from a guide table that the context holds.
+int di_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
+                const rgi_di_reservoirs& prev, const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream);
 int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
                uint16_t* diffuse, uint16_t* specular, const Tuning& tuning, hipStream_t stream);
 

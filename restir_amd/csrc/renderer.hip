@@ -506,7 +506,8 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
         if (di_active) {
             rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
             rgi_di_reservoirs next_di{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
-            if ((fe = di_temporal(f, r->scene, *g, prev_di, next_di, r->ctx->tuning, stream)) != RGI_OK) return fe;
+            if ((fe = di_temporal(r->ctx, f, r->scene, *g, prev_di, next_di, r->ctx->tuning, stream)) != RGI_OK)
+                return fe;
             if ((fe = di_spatial(f, r->scene, *g, next_di, r->diffuse, r->specular, r->ctx->tuning, stream)) !=
                 RGI_OK)
                 return fe;

@@ -492,13 +492,15 @@ def test_c5_scene_matches_oracle(torch, case):
 
 # ------------------------------------------------------------ DI lights
 
-@pytest.mark.parametrize("order", [0, "wide"])
-def test_di_over_many_lights(torch, order):
+@pytest.mark.parametrize("order,guide", [(0, True), ("wide", True), (0, False)])
+def test_di_over_many_lights(torch, order, guide):
     """DI temporal and spatial through the C ABI on a box field lit by more
     than 256 lights (above the LDS light table's capacity: the light list
-    is read from global memory), two frames, against the oracle."""
+    is read from global memory), two frames, against the oracle; with a
+    context the light choice starts from the CDF's guide table, without one
+    it searches the whole CDF."""
     hs = ra.HostScene(ra.RGI_SCENE_BOXFIELD, seed=1, size=840, extra_lights=4, light_seed=1)
-    say("case DI over %d lights order=%s" % (hs.light_count, order))
+    say("case DI over %d lights order=%s guide=%s" % (hs.light_count, order, guide))
     assert hs.light_count > 256
     w, h = 640, 360
     s = ra.Settings()
@@ -528,7 +530,7 @@ def test_di_over_many_lights(torch, order):
             dp = {k: d.put(v) for k, v in prev.items()}
             do = {k: d.empty(v.nbytes) for k, v in want.items()}
             dd, dsp = d.empty(n * 8), d.empty(n * 8)
-            ra.check(d.L.rgi_di_temporal(d.ctx, ctypes.byref(f), ctypes.byref(scene), ctypes.byref(gb),
+            ra.check(d.L.rgi_di_temporal(d.ctx if guide else None, ctypes.byref(f), ctypes.byref(scene), ctypes.byref(gb),
                                          ctypes.byref(ra.DiReservoirs(*(dp[k].data_ptr() for k in
                                                                         ("sample", "receiver", "normal")))),
                                          ctypes.byref(ra.DiReservoirs(*(do[k].data_ptr() for k in
