@@ -633,8 +633,10 @@ def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames,
 def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offsets=None):
     """Renderers of `cases` as one batch frame: each renderer alone (its own
     frame time), all of them eagerly one after another on one stream, then
-    all of them as one graph (rgi_batch_render), each on a stream of its
-    own. With `nrc`, every renderer has its own neural radiance cache."""
+    all of them as one rgi_batch_render call in both of its forms: each pass
+    over every view in one launch (RGI_BATCH_PASSES) and one graph with each
+    renderer on a stream of its own (RGI_BATCH_GRAPH). With `nrc`, every
+    renderer has its own neural radiance cache."""
     import torch
     from restir_amd import nrc as rnrc
     L = ra.lib()
@@ -704,22 +706,28 @@ def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offse
         ga = (ra.GBuffer * len(rends))(*[g[k % views_per_case].gb for _, g in frames])
         ra.check(L.rgi_batch_render(batch, va, ga, bs))
 
-    for k in range(warmup):
-        batched(k)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        batched(k)
-    torch.cuda.synchronize()
-    batch_ms = (time.perf_counter() - t0) * 1e3 / steps
+    batch_ms = {}
+    for mode in (ra.RGI_BATCH_GRAPH, ra.RGI_BATCH_PASSES):
+        ra.check(L.rgi_batch_set_mode(batch, mode))
+        for k in range(warmup):
+            batched(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            batched(k)
+        torch.cuda.synchronize()
+        batch_ms[mode] = (time.perf_counter() - t0) * 1e3 / steps
     L.rgi_batch_destroy(batch)
     for r in rends:
         L.rgi_renderer_destroy(r)
     L.rgi_destroy(ctx)
     return {"scenes": names, "triangles": tris, "per_scene_ms": [round(v, 4) for v in per_scene],
             "sum_of_scene_ms": round(sum(per_scene), 4), "eager_ms_per_batch_frame": round(eager_ms, 5),
-            "graph_batch_ms_per_batch_frame": round(batch_ms, 5), "pixels": sum(pixels),
-            "gi_pixels_per_s": round(sum(pixels) / (batch_ms * 1e-3), 1), "steps": steps, "nrc": nrc}
+            "graph_batch_ms_per_batch_frame": round(batch_ms[ra.RGI_BATCH_GRAPH], 5),
+            "passes_batch_ms_per_batch_frame": round(batch_ms[ra.RGI_BATCH_PASSES], 5),
+            "passes_vs_eager": round(batch_ms[ra.RGI_BATCH_PASSES] / eager_ms, 4), "pixels": sum(pixels),
+            "gi_pixels_per_s": round(sum(pixels) / (batch_ms[ra.RGI_BATCH_PASSES] * 1e-3), 1), "steps": steps,
+            "nrc": nrc}
 
 
 def atrium_workload(args, W, H, mode):

@@ -127,16 +127,34 @@ int rgi_renderer_set_option(rgi_renderer* renderer, int option, int value);
  * any capture. Results are those of rgi_renderer_render. */
 int rgi_renderer_render_graph(rgi_renderer* renderer, const rgi_view* view, const rgi_gbuffer* gbuffer, void* stream);
 
-/* A batch of renderers (any sizes and scenes) rendered together: one HIP
- * graph per call holds every renderer's frame, each on a stream of its own
- * forked from and joined to `stream` (not the null stream), so small views
- * overlap on the GPU. views[i] and gbuffers[i] go to renderers[i]. The first
- * call renders eagerly, one renderer after another; later calls capture,
- * update the batch's executable graph in place and launch it. Results are
+/* A batch of up to 64 distinct renderers (any sizes and scenes) rendered
+ * together; views[i] and gbuffers[i] go to renderers[i], and results are
  * those of rgi_renderer_render on each renderer. The renderers stay owned by
- * the caller and must outlive the batch. */
+ * the caller and must outlive the batch; consecutive frames of a batch go to
+ * one stream (or the caller orders them).
+ *
+ * RGI_BATCH_PASSES (the default): each screen pass of the frame (DI temporal
+ * and spatial, the hash grids' clear and insert, GI temporal's tile list,
+ * candidate rays, BVH walks and resampling, GI spatial) runs once for every
+ * view, its kernels reading each view's frame, scene and planes from a
+ * device table that the call uploads from pinned host memory; the passes
+ * with per-renderer state (the cache's query and training, indirect
+ * specular, denoiser, composite) run per view. GI temporal's tracing and the
+ * grids run on the batch's side stream, forked from and joined back to
+ * `stream` (which may be the null stream). Every view is checked before the
+ * first launch: a view that fails its checks leaves every renderer as it
+ * was.
+ *
+ * RGI_BATCH_GRAPH: one HIP graph per call holds every renderer's frame, each
+ * on a stream of its own forked from and joined to `stream` (not the null
+ * stream). The first call renders eagerly, one renderer after another; later
+ * calls capture, update the batch's executable graph in place and launch it;
+ * a capture that fails leaves every renderer as it was. */
+#define RGI_BATCH_PASSES 0
+#define RGI_BATCH_GRAPH 1
 typedef struct rgi_batch rgi_batch;
 int rgi_batch_create(rgi_batch** out, rgi_renderer* const* renderers, uint32_t count);
+int rgi_batch_set_mode(rgi_batch* batch, int mode);
 int rgi_batch_render(rgi_batch* batch, const rgi_view* views, const rgi_gbuffer* gbuffers, void* stream);
 void rgi_batch_destroy(rgi_batch* batch);
 

@@ -21,6 +21,7 @@ RGI_OK = 0
 RGI_E_INVALID, RGI_E_DEVICE, RGI_E_NOMEM = -1, -2, -3
 RGI_RAY_SCREEN, RGI_RAY_HYBRID, RGI_RAY_BVH = 0, 1, 2
 RGI_GRID_ATOMIC, RGI_GRID_DETERMINISTIC = 0, 1
+RGI_BATCH_PASSES, RGI_BATCH_GRAPH = 0, 1
 RGI_SCENE_CORNELL, RGI_SCENE_ATRIUM, RGI_SCENE_BOXFIELD = 0, 1, 2
 RGI_OPT_GI_TEMPORAL_WAVES, RGI_OPT_GI_SPATIAL_WAVES, RGI_OPT_TRACE_PERSISTENT = 1, 2, 3
 RGI_OPT_TILE_ORDER = 4
@@ -218,6 +219,7 @@ def lib():
     L.rgi_renderer_render_graph.argtypes = [P, ctypes.POINTER(View), ctypes.POINTER(GBuffer), P]
     L.rgi_batch_create.argtypes = [ctypes.POINTER(P), P, U32]
     L.rgi_batch_render.argtypes = [P, P, P, P]
+    L.rgi_batch_set_mode.argtypes = [P, ctypes.c_int]
     L.rgi_batch_destroy.argtypes = [P]
     L.rgi_batch_destroy.restype = None
     L.rgi_renderer_set_reference_mode.argtypes = [P, ctypes.c_int]

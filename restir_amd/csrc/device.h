@@ -38,7 +38,8 @@ struct F4 { float x, y, z, w; };
 // spreads over all eight XCDs. Its grid is one-dimensional, 16 workgroups
 // per block slot; slots past the screen edge have no tile.
 // Every order visits every tile exactly once; false means this workgroup has
-// no tile (order 2 only), and the whole workgroup leaves.
+// no tile (order 2, or a batch slice's padding), and the whole workgroup
+// leaves.
 // A tile is 16 x 16 pixels, or 32 x 8 with `wide` (the same 256 threads).
 struct TileMap {
     uint32_t order;
@@ -47,23 +48,34 @@ struct TileMap {
     uint32_t blocks_x; // order 2: 4 x 4-tile blocks per row
 };
 
-RGI_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty)
+// A workgroup's place in its view's grid: its linear index b and the grid's
+// dimensions (the launch's own, or a slice of a batch launch).
+struct BlockAt {
+    uint32_t b, gx, gy;
+};
+RGI_DEV BlockAt this_block() { return BlockAt{blockIdx.y * gridDim.x + blockIdx.x, gridDim.x, gridDim.y}; }
+
+RGI_DEV bool block_tile(const TileMap& m, const BlockAt& at, uint32_t& tx, uint32_t& ty)
 {
-    uint32_t b = blockIdx.y * gridDim.x + blockIdx.x;
+    uint32_t b = at.b;
     if (m.order == 2u) {
         uint32_t k = b >> 3, s = (b & 7u) + 8u * (k >> 4), j = k & 15u;
         tx = (s % m.blocks_x) * 4u + (j & 3u);
         ty = (s / m.blocks_x) * 4u + (j >> 2);
         return tx < m.tiles_x && ty < m.tiles_y;
     }
+    // A batch pads each view's slice to whole rounds of 8 workgroups; the
+    // padding has no tile.
+    if (b >= at.gx * at.gy) return false;
     if (m.order == 1u) {
-        uint32_t n = gridDim.x * gridDim.y, q = n >> 3, r = n & 7u, xcd = b & 7u, i = b >> 3;
+        uint32_t n = at.gx * at.gy, q = n >> 3, r = n & 7u, xcd = b & 7u, i = b >> 3;
         b = xcd < r ? xcd * (q + 1u) + i : r * (q + 1u) + (xcd - r) * q + i;
     }
-    tx = b % gridDim.x;
-    ty = b / gridDim.x;
+    tx = b % at.gx;
+    ty = b / at.gx;
     return true;
 }
+RGI_DEV bool block_tile(const TileMap& m, uint32_t& tx, uint32_t& ty) { return block_tile(m, this_block(), tx, ty); }
 
 // Host side: the map and the grid of a width x height screen pass.
 inline TileMap tile_map(int order, int wide, uint32_t width, uint32_t height)

@@ -5,6 +5,7 @@
 // the BVH. One pixel per lane, 16 x 16 pixels per 256-thread block.
 #include "bvh_device.h"
 #include "shading.h"
+#include "batch.h"
 #include "internal.h"
 
 namespace rgi {
@@ -142,10 +143,10 @@ RGI_DEV bool same_surface(const DiPass& p, F3 position, F3 n, F3 other_position,
            fabsf(dot(off, n)) <= 0.03f * p.sc && dot(normalize(other_normal), n) >= 0.9f;
 }
 
-RGI_DEV bool block_pixel(const TileMap& tiles, int& x, int& y)
+RGI_DEV bool block_pixel(const TileMap& tiles, const BlockAt& at, int& x, int& y)
 {
     uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u, tx, ty;
-    if (!block_tile(tiles, tx, ty)) return false;
+    if (!block_tile(tiles, at, tx, ty)) return false;
     if (tiles.wide) {
         x = (int)(tx * 32u + wave * 8u + (lane & 7u));
         y = (int)(ty * 8u + (lane >> 3));
@@ -207,16 +208,16 @@ RGI_DEV void stage_lights(DiPass& p, const rgi_frame& f, const rgi_scene& s, flo
     p.lights = lds;
 }
 
-__global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs prev,
-                                                     rgi_di_reservoirs out, F2 texel, TileMap tiles,
-                                                     const uint32_t* guide, uint32_t guide_buckets)
+// One workgroup of DI temporal: the tile at `at` of the view's grid.
+RGI_DEV void di_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& prev,
+                              const rgi_di_reservoirs& out, F2 texel, const TileMap& tiles, const BlockAt& at,
+                              const uint32_t* guide, uint32_t guide_buckets, float4* lights)
 {
-    __shared__ float4 lights[4 * kLdsLights];
     DiPass p = make_pass(f, s, g, texel);
     p.guide = guide;
     p.guide_buckets = guide_buckets;
     int x, y;
-    if (!block_pixel(tiles, x, y)) return; // the whole workgroup, before the barrier
+    if (!block_pixel(tiles, at, x, y)) return; // the whole workgroup, before the barrier
     stage_lights(p, f, s, lights);
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
@@ -323,14 +324,14 @@ __global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, r
     store_h4(out.normal, o, F4{surface.normal.x, surface.normal.y, surface.normal.z, ccount + tcount});
 }
 
-__global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs res,
-                                                    uint16_t* out_diffuse, uint16_t* out_specular, F2 texel,
-                                                    TileMap tiles)
+// One workgroup of DI spatial: the tile at `at` of the view's grid.
+RGI_DEV void di_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
+                             uint16_t* out_diffuse, uint16_t* out_specular, F2 texel, const TileMap& tiles,
+                             const BlockAt& at, float4* lights)
 {
-    __shared__ float4 lights[4 * kLdsLights];
     DiPass p = make_pass(f, s, g, texel);
     int x, y;
-    if (!block_pixel(tiles, x, y)) return; // the whole workgroup, before the barrier
+    if (!block_pixel(tiles, at, x, y)) return; // the whole workgroup, before the barrier
     stage_lights(p, f, s, lights);
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
@@ -427,10 +428,48 @@ __global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rg
     store_h4(out_specular, o, F4{sp.x, sp.y, sp.z, 1.0f});
 }
 
+__global__ void __launch_bounds__(256) k_di_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs prev,
+                                                     rgi_di_reservoirs out, F2 texel, TileMap tiles,
+                                                     const uint32_t* guide, uint32_t guide_buckets)
+{
+    __shared__ float4 lights[4 * kLdsLights];
+    di_temporal_body(f, s, g, prev, out, texel, tiles, this_block(), guide, guide_buckets, lights);
+}
+
+__global__ void __launch_bounds__(256) k_di_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_di_reservoirs res,
+                                                    uint16_t* out_diffuse, uint16_t* out_specular, F2 texel,
+                                                    TileMap tiles)
+{
+    __shared__ float4 lights[4 * kLdsLights];
+    di_spatial_body(f, s, g, res, out_diffuse, out_specular, texel, tiles, this_block(), lights);
+}
+
+// The batched forms: workgroups [first[v], first[v + 1]) run view v's grid.
+__global__ void __launch_bounds__(256) k_di_temporal_batch(const BatchView* views, const uint32_t* first, uint32_t count)
+{
+    __shared__ float4 lights[4 * kLdsLights];
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    di_temporal_body(v.f, v.s, v.g, v.di_prev, v.di_next, v.texel, v.di_tiles, BlockAt{b, v.di_gx, v.di_gy}, v.guide,
+                     v.guide_buckets, lights);
+}
+
+// At most 128 VGPRs, as the compiler gives the single-view kernel: 4 waves
+// per SIMD.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) k_di_spatial_batch(const BatchView* views, const uint32_t* first, uint32_t count)
+{
+    __shared__ float4 lights[4 * kLdsLights];
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    di_spatial_body(v.f, v.s, v.g, v.di_next, v.di_diffuse, v.di_specular, v.texel, v.di_tiles,
+                    BlockAt{b, v.di_gx, v.di_gy}, lights);
+}
+
 } // namespace
 
 int di_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
-                const rgi_di_reservoirs& prev, const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream)
+                const rgi_di_reservoirs& prev, const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream,
+                bool fixed_lights)
 {
     TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, g.width, g.height);
     F2 texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
@@ -438,16 +477,9 @@ int di_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const 
     // the CDF in memory, so it starts from a guide table built here (with a
     // context to hold it).
     uint32_t* guide = nullptr;
-    uint32_t buckets = 0;
-    const int n = f.di_light_count;
-    if (ctx && n > (int)kLdsLights && n <= (1 << 24)) {
-        buckets = 1024u;
-        while (buckets < (uint32_t)n && buckets < (1u << 16)) buckets <<= 1;
-        guide = static_cast<uint32_t*>(context_buffer(ctx, kBufLightGuide, (buckets + 1u) * sizeof(uint32_t)));
-        if (!guide) return RGI_E_NOMEM;
-        hipLaunchKernelGGL(k_light_guide, dim3((buckets + 256u) / 256u), dim3(256), 0, stream,
-                           reinterpret_cast<const float4*>(s.lights), n, buckets, guide);
-    }
+    uint32_t buckets = 0u;
+    const int e = di_light_guide(ctx, f, s, &guide, &buckets, stream, fixed_lights);
+    if (e != RGI_OK) return e;
     hipLaunchKernelGGL(k_di_temporal, tile_grid(tiles), dim3(256), 0, stream, f, s, g, prev, out,
                        texel, tiles, guide, buckets);
     return check_launch("di temporal");
@@ -461,6 +493,44 @@ int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
     hipLaunchKernelGGL(k_di_spatial, tile_grid(tiles), dim3(256), 0, stream, f, s, g, res,
                        diffuse, specular, texel, tiles);
     return check_launch("di spatial");
+}
+
+int di_light_guide(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, uint32_t** guide, uint32_t* buckets,
+                   hipStream_t stream, bool fixed_lights)
+{
+    *guide = nullptr;
+    *buckets = 0u;
+    const int n = f.di_light_count;
+    if (!ctx || n <= (int)kLdsLights || n > (1 << 24)) return RGI_OK;
+    uint32_t k = 1024u;
+    while (k < (uint32_t)n && k < (1u << 16)) k <<= 1;
+    *guide = static_cast<uint32_t*>(context_buffer(ctx, kBufLightGuide, (k + 1u) * sizeof(uint32_t)));
+    if (!*guide) return RGI_E_NOMEM;
+    *buckets = k;
+    // The bucket count follows from n, so the same table and n find the
+    // same guide in the same buffer.
+    if (fixed_lights && ctx->guide_lights == s.lights && ctx->guide_count == n && ctx->guide_buffer == *guide)
+        return RGI_OK;
+    hipLaunchKernelGGL(k_light_guide, dim3((k + 256u) / 256u), dim3(256), 0, stream,
+                       reinterpret_cast<const float4*>(s.lights), n, k, *guide);
+    ctx->guide_lights = fixed_lights ? s.lights : nullptr;
+    ctx->guide_count = n;
+    ctx->guide_buffer = *guide;
+    return check_launch("light guide");
+}
+
+int di_temporal_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, hipStream_t stream)
+{
+    if (blocks == 0u) return RGI_OK;
+    hipLaunchKernelGGL(k_di_temporal_batch, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    return check_launch("di temporal (batch)");
+}
+
+int di_spatial_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, hipStream_t stream)
+{
+    if (blocks == 0u) return RGI_OK;
+    hipLaunchKernelGGL(k_di_spatial_batch, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    return check_launch("di spatial (batch)");
 }
 
 } // namespace rgi

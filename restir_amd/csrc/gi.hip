@@ -13,6 +13,7 @@
 // reference's group queue does (cs_restir_temporal.sc:108-149); which lane
 // traces a ray does not change its result.
 #include "gi_device.h"
+#include "batch.h"
 #include "internal.h"
 
 namespace rgi {
@@ -327,7 +328,7 @@ RGI_DEV void gi_temporal_body(const rgi_frame& f, const rgi_scene& s, const rgi_
 template <bool kSameSize>
 RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                              const rgi_gi_reservoirs& res, uint16_t* out_color, uint16_t* out_moments, F2 texel,
-                             const TileMap& tiles, VisLds<4>& vis)
+                             const TileMap& tiles, const BlockAt& at, VisLds<4>& vis)
 {
     GiPass p;
     p.f = &f;
@@ -339,7 +340,7 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     p.sc = f.scene_scale;
 
     int x, y;
-    if (!group_pixel(tiles, x, y)) return;
+    if (!group_pixel(tiles, at, x, y)) return;
     if (x >= p.hw || y >= p.hh) return;
     uint32_t o = (uint32_t)y * (uint32_t)p.hw + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -486,10 +487,9 @@ k_gi_temporal_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_
 // of dependent depth loads. Every workgroup runs the same number of rounds
 // of 4 tiles (one per wave).
 template <bool kSameSize>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k_gi_candidates(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list, CandRays c)
+RGI_DEV void gi_candidates(const rgi_frame& f, const rgi_gbuffer& g, F2 texel, const TileList& list, const CandRays& c,
+                           uint32_t first, uint32_t stride, uint32_t* wave_base)
 {
-    __shared__ uint32_t wave_base[4];
     GiPass p;
     p.f = &f;
     p.g = g;
@@ -499,7 +499,7 @@ k_gi_candidates(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list, CandRays c)
     const uint32_t total = list.words[0] + list.words[1];
     const int K = f.initial_candidates;
     const bool trace_open = f.ray_mode != RGI_RAY_SCREEN;
-    for (uint32_t round = blockIdx.x * 4u; round < total; round += gridDim.x * 4u) {
+    for (uint32_t round = first * 4u; round < total; round += stride * 4u) {
         const uint32_t item = round + wave;
         int x = 0, y = 0;
         if (item < total) tile_pixel(list, listed_tile(list, item), x, y);
@@ -574,6 +574,14 @@ k_gi_candidates(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list, CandRays c)
     }
 }
 
+template <bool kSameSize>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_gi_candidates(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list, CandRays c)
+{
+    __shared__ uint32_t wave_base[4];
+    gi_candidates<kSameSize>(f, g, texel, list, c, blockIdx.x, gridDim.x, wave_base);
+}
+
 // Closest hits of the candidates' listed rays (traceBvh, bvh.sh:216-295),
 // answered by persistent waves with per-lane refill, in chunks of 64 rays (a
 // frame's few hundred thousand rays then still reach every wave).
@@ -582,6 +590,18 @@ __global__ void __launch_bounds__(256) k_gi_walks(rgi_scene s, TileList list, Ca
     __shared__ int ring[kStackRing * 256];
     trace_persistent<MollerTrumbore, false>(scene_view(s), c.rays, list.words[kTileRayWord], c.hit, c.dist,
                                             list.words + kWalkClaimWord, ring + threadIdx.x, 256, 64u);
+}
+
+__global__ void __launch_bounds__(256) k_gi_walks_batch(const BatchView* __restrict__ views, uint32_t count)
+{
+    __shared__ int ring[kStackRing * 256];
+    for (uint32_t i = 0; i < count; ++i) {
+        const BatchView& v = views[i];
+        if (!v.split || v.f.initial_candidates <= 0 || v.f.ray_mode == RGI_RAY_SCREEN) continue;
+        trace_persistent<MollerTrumbore, false>(scene_view(v.s), v.cand.rays, v.list.words[kTileRayWord], v.cand.hit,
+                                                v.cand.dist, v.list.words + kWalkClaimWord, ring + threadIdx.x, 256,
+                                                64u);
+    }
 }
 
 // The listed form: one-wave workgroups that claim the listed tiles
@@ -619,12 +639,37 @@ k_gi_temporal_listed_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs i
 }
 
 template <bool kSameSize>
+RGI_DEV void gi_temporal_listed_views(const BatchView* __restrict__ views, uint32_t count, TemporalLds<1, false>& lds)
+{
+    for (uint32_t i = 0; i < count; ++i) {
+        const BatchView& v = views[i];
+        if (!v.split || v.same != (kSameSize ? 1u : 0u)) continue;
+        gi_temporal_listed<kSameSize>(v.fg, v.s, v.g, v.tin, v.res, v.gi_texel, v.list, v.cand, lds);
+    }
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(64) k_gi_temporal_listed_batch(const BatchView* __restrict__ views, uint32_t count)
+{
+    __shared__ TemporalLds<1, false> lds;
+    gi_temporal_listed_views<kSameSize>(views, count, lds);
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(64) RGI_WAVES4
+k_gi_temporal_listed_batch_w4(const BatchView* __restrict__ views, uint32_t count)
+{
+    __shared__ TemporalLds<1, false> lds;
+    gi_temporal_listed_views<kSameSize>(views, count, lds);
+}
+
+template <bool kSameSize>
 __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
                                                     rgi_gi_reservoirs res, uint16_t* out_color, uint16_t* out_moments,
                                                     F2 texel, TileMap tiles)
 {
     __shared__ VisLds<4> vis;
-    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, vis);
+    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, this_block(), vis);
 }
 
 template <bool kSameSize>
@@ -633,7 +678,34 @@ k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_g
                 uint16_t* out_moments, F2 texel, TileMap tiles)
 {
     __shared__ VisLds<4> vis;
-    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, vis);
+    gi_spatial_body<kSameSize>(f, s, g, in, res, out_color, out_moments, texel, tiles, this_block(), vis);
+}
+
+template <bool kSameSize>
+RGI_DEV void gi_spatial_view(const BatchView* __restrict__ views, const uint32_t* __restrict__ first, uint32_t count,
+                             VisLds<4>& vis)
+{
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    if (v.same != (kSameSize ? 1u : 0u)) return;
+    gi_spatial_body<kSameSize>(v.fg, v.s, v.g, v.sin, v.res, v.color, v.moments, v.gi_texel, v.gi_tiles,
+                               BlockAt{b, v.gi_gx, v.gi_gy}, vis);
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(256) k_gi_spatial_batch(const BatchView* __restrict__ views,
+                                                          const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ VisLds<4> vis;
+    gi_spatial_view<kSameSize>(views, first, count, vis);
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(256) RGI_WAVES4
+k_gi_spatial_batch_w4(const BatchView* __restrict__ views, const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ VisLds<4> vis;
+    gi_spatial_view<kSameSize>(views, first, count, vis);
 }
 
 // Zeroes a list's counters before k_gi_tiles fills it.
@@ -647,17 +719,16 @@ __global__ void __launch_bounds__(kTileListWord) k_gi_tiles_reset(TileList list)
 // reservoirs of every GI pixel of the tiles without a surface: the pixels
 // that the temporal pass writes as zeros without reading anything else.
 template <bool kSameSize>
-__global__ void __launch_bounds__(1024) k_gi_tiles(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list,
-                                                   rgi_gi_reservoirs out)
+RGI_DEV void gi_tiles_body(const rgi_frame& f, const rgi_gbuffer& g, F2 texel, const TileList& list,
+                           const rgi_gi_reservoirs& out, uint32_t block, uint32_t* counts, uint32_t* bases)
 {
-    __shared__ uint32_t counts[2], bases[2];
     if (threadIdx.x < 2u) counts[threadIdx.x] = 0u;
     __syncthreads();
     GiPass p;
     p.f = &f;
     p.g = g;
     set_gi_size<kSameSize>(p, f, g, texel);
-    const uint32_t t = blockIdx.x * 16u + (threadIdx.x >> 6);
+    const uint32_t t = block * 16u + (threadIdx.x >> 6);
     const bool valid = t < list.tiles;
     int x = 0, y = 0;
     if (valid) tile_pixel(list, t, x, y);
@@ -682,6 +753,59 @@ __global__ void __launch_bounds__(1024) k_gi_tiles(rgi_frame f, rgi_gbuffer g, F
     if (leader) {
         uint32_t i = bases[kind] + rank;
         list.words[kTileListWord + (kind == 0u ? i : list.tiles - 1u - i)] = t;
+    }
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(1024) k_gi_tiles(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list,
+                                                   rgi_gi_reservoirs out)
+{
+    __shared__ uint32_t counts[2], bases[2];
+    gi_tiles_body<kSameSize>(f, g, texel, list, out, blockIdx.x, counts, bases);
+}
+
+// ------------------------------------------------------------------ batches
+//
+// The passes of a batch of views in one launch each, over the device table
+// of the views (BatchView). Screen passes give each view a slice of the
+// grid (workgroups [first[v], first[v + 1])); the listed and persistent
+// passes loop over the views, each view's tiles or rays claimed from its own
+// counters, so that a workgroup done with one view moves on to the next.
+// Each kernel works on the views whose G-buffer size matches its kSameSize
+// (a batch that mixes both launches both forms); the views' results are
+// those of their single-view passes, the same bodies on the same inputs.
+
+__global__ void __launch_bounds__(kTileListWord) k_gi_tiles_reset_batch(const BatchView* __restrict__ views)
+{
+    const BatchView& v = views[blockIdx.x];
+    if (v.split) v.list.words[threadIdx.x] = 0u;
+}
+
+template <bool kSameSize>
+__global__ void __launch_bounds__(1024) k_gi_tiles_batch(const BatchView* __restrict__ views,
+                                                         const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ uint32_t counts[2], bases[2];
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    if (v.same != (kSameSize ? 1u : 0u)) return; // the whole workgroup, before the barriers
+    gi_tiles_body<kSameSize>(v.f, v.g, v.gi_texel, v.list, v.res, b, counts, bases);
+}
+
+// A view's rounds of 4 tiles continue the previous views' rounds, so that
+// the workgroups share the whole batch's rounds evenly.
+template <bool kSameSize>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_gi_candidates_batch(const BatchView* __restrict__ views, uint32_t count)
+{
+    __shared__ uint32_t wave_base[4];
+    uint32_t before = 0u;
+    for (uint32_t i = 0; i < count; ++i) {
+        const BatchView& v = views[i];
+        if (!v.split || v.same != (kSameSize ? 1u : 0u) || v.f.initial_candidates <= 0) continue;
+        const uint32_t first = (blockIdx.x + gridDim.x - before % gridDim.x) % gridDim.x;
+        gi_candidates<kSameSize>(v.f, v.g, v.gi_texel, v.list, v.cand, first, gridDim.x, wave_base);
+        before += (v.list.words[0] + v.list.words[1] + 3u) / 4u;
     }
 }
 
@@ -812,6 +936,66 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, res, color,
                        moments, texel, tiles);
     return check_launch("gi spatial");
+}
+
+// ------------------------------------------------------------------ batches
+
+int gi_trace_batch(const BatchLaunch& b, hipStream_t stream)
+{
+    if (!b.split) return RGI_OK;
+    hipLaunchKernelGGL(k_gi_tiles_reset_batch, dim3(b.count), dim3(kTileListWord), 0, stream, b.views);
+    const uint32_t* first = b.first[kPassTiles];
+    const uint32_t blocks = b.blocks[kPassTiles];
+    if (b.same) hipLaunchKernelGGL(k_gi_tiles_batch<true>, dim3(blocks), dim3(1024), 0, stream, b.views, first, b.count);
+    if (b.larger) hipLaunchKernelGGL(k_gi_tiles_batch<false>, dim3(blocks), dim3(1024), 0, stream, b.views, first, b.count);
+    const uint32_t cus = (uint32_t)b.compute_units;
+    if (b.rounds > 0u) {
+        const uint32_t grid = b.rounds < 8u * cus ? b.rounds : 8u * cus;
+        if (b.same) hipLaunchKernelGGL(k_gi_candidates_batch<true>, dim3(grid), dim3(256), 0, stream, b.views, b.count);
+        if (b.larger) hipLaunchKernelGGL(k_gi_candidates_batch<false>, dim3(grid), dim3(256), 0, stream, b.views, b.count);
+        if (b.trace) {
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gi_walks_batch, 256, 0) != hipSuccess ||
+                per_cu <= 0)
+                per_cu = 4;
+            hipLaunchKernelGGL(k_gi_walks_batch, dim3((uint32_t)per_cu * cus), dim3(256), 0, stream, b.views, b.count);
+        }
+    }
+    return check_launch("gi temporal tracing (batch)");
+}
+
+int gi_resample_batch(const BatchLaunch& b, hipStream_t stream)
+{
+    if (!b.split) return RGI_OK;
+    const uint32_t most = 16u * (uint32_t)b.compute_units, grid = b.tiles < most ? b.tiles : most;
+    const bool w4 = b.gi_temporal_waves == 4;
+    if (b.same)
+        hipLaunchKernelGGL(w4 ? k_gi_temporal_listed_batch_w4<true> : k_gi_temporal_listed_batch<true>, dim3(grid),
+                           dim3(64), 0, stream, b.views, b.count);
+    if (b.larger)
+        hipLaunchKernelGGL(w4 ? k_gi_temporal_listed_batch_w4<false> : k_gi_temporal_listed_batch<false>, dim3(grid),
+                           dim3(64), 0, stream, b.views, b.count);
+    return check_launch("gi temporal (batch)");
+}
+
+int gi_spatial_batch(const BatchLaunch& b, hipStream_t stream)
+{
+    const uint32_t* first = b.first[kPassGiSpatial];
+    const uint32_t blocks = b.blocks[kPassGiSpatial];
+    if (blocks == 0u) return RGI_OK;
+    const bool w4 = b.gi_spatial_waves == 4;
+    if (b.same)
+        hipLaunchKernelGGL(w4 ? k_gi_spatial_batch_w4<true> : k_gi_spatial_batch<true>, dim3(blocks), dim3(256), 0,
+                           stream, b.views, first, b.count);
+    if (b.larger)
+        hipLaunchKernelGGL(w4 ? k_gi_spatial_batch_w4<false> : k_gi_spatial_batch<false>, dim3(blocks), dim3(256), 0,
+                           stream, b.views, first, b.count);
+    return check_launch("gi spatial (batch)");
+}
+
+int gi_listed_buffers(rgi_context* ctx, const rgi_frame& f, TileList& list, CandRays& c)
+{
+    return listed_buffers(ctx, f, list, c);
 }
 
 } // namespace rgi

@@ -23,6 +23,7 @@
 // count is every pixel resolved to it and its entries are its 32 lowest pixel
 // indices, ascending, found by a CSR gather and a wave bitonic sort per cell.
 #include "device.h"
+#include "batch.h"
 #include "internal.h"
 
 namespace rgi {
@@ -43,18 +44,23 @@ RGI_DEV uint32_t rank_in(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// 16 x 16 pixels per block, one 8 x 8 tile per wave.
-RGI_DEV void block_pixel(int& x, int& y)
+// 16 x 16 pixels per block (block bx, by of the grid), one 8 x 8 tile per
+// wave.
+RGI_DEV void block_pixel(uint32_t bx, uint32_t by, int& x, int& y)
 {
     uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    x = (int)(blockIdx.x * 16u + (wave & 1u) * 8u + (lane & 7u));
-    y = (int)(blockIdx.y * 16u + (wave >> 1) * 8u + (lane >> 3));
+    x = (int)(bx * 16u + (wave & 1u) * 8u + (lane & 7u));
+    y = (int)(by * 16u + (wave >> 1) * 8u + (lane >> 3));
 }
+RGI_DEV void block_pixel(int& x, int& y) { block_pixel(blockIdx.x, blockIdx.y, x, y); }
 
-__global__ void __launch_bounds__(256) k_grid_insert_atomic(rgi_frame f, rgi_gbuffer g, F2 texel, uint32_t* grid)
+// gridInsert with its atomics aggregated per wave, for the workgroup at
+// (bx, by) of the 16 x 16-pixel grid.
+RGI_DEV void grid_insert_body(const rgi_frame& f, const rgi_gbuffer& g, F2 texel, uint32_t* grid, uint32_t bx,
+                              uint32_t by)
 {
     int x, y;
-    block_pixel(x, y);
+    block_pixel(bx, by, x, y);
     uint32_t table = f.grid_table_size, mask = table - 1u;
     uint32_t slot0 = 0, checksum = 0;
     bool active = x < f.gi_width && y < f.gi_height && insert_key(f, g, texel, x, y, slot0, checksum);
@@ -87,6 +93,30 @@ __global__ void __launch_bounds__(256) k_grid_insert_atomic(rgi_frame f, rgi_gbu
         }
         pending &= ~peers;
     }
+}
+
+__global__ void __launch_bounds__(256) k_grid_insert_atomic(rgi_frame f, rgi_gbuffer g, F2 texel, uint32_t* grid)
+{
+    grid_insert_body(f, g, texel, grid, blockIdx.x, blockIdx.y);
+}
+
+// The batched forms: workgroups [first[v], first[v + 1]) work on view v.
+__global__ void __launch_bounds__(256) k_grid_clear_batch(const BatchView* views, const uint32_t* first, uint32_t count)
+{
+    uint32_t b;
+    const uint32_t vi = batch_view(first, count, blockIdx.x, b);
+    const BatchView& v = views[vi];
+    const uint32_t blocks = first[vi + 1u] - first[vi];
+    const uint32_t vec = (2u * v.f.grid_table_size) >> 2;
+    uint4* g4 = reinterpret_cast<uint4*>(v.grid);
+    for (uint32_t i = b * blockDim.x + threadIdx.x; i < vec; i += blocks * blockDim.x) g4[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void __launch_bounds__(256) k_grid_insert_batch(const BatchView* views, const uint32_t* first, uint32_t count)
+{
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    grid_insert_body(v.f, v.g, v.gi_texel, v.grid, b % v.grid_gx, b / v.grid_gx);
 }
 
 // ------------------------------------------------------- deterministic mode
@@ -432,9 +462,7 @@ __global__ void __launch_bounds__(256) k_det_select(uint32_t table, uint32_t* gr
 int grid_clear(uint32_t* grid, uint32_t table_size, hipStream_t stream)
 {
     uint32_t words = 2u * table_size;
-    uint32_t blocks = (words / 4u + 255u) / 256u;
-    blocks = blocks > 4096u ? 4096u : (blocks == 0 ? 1u : blocks);
-    hipLaunchKernelGGL(k_grid_clear, dim3(blocks), dim3(256), 0, stream, grid, words);
+    hipLaunchKernelGGL(k_grid_clear, dim3(grid_clear_blocks(table_size)), dim3(256), 0, stream, grid, words);
     return check_launch("grid clear");
 }
 
@@ -504,6 +532,27 @@ int grid_insert_deterministic_in(void* scratch_base, const rgi_frame& f, const r
     hipLaunchKernelGGL(k_det_scatter, tiles, dim3(256), 0, stream, f.gi_width, f.gi_height, d);
     hipLaunchKernelGGL(k_det_select, dim3(2048), dim3(256), 0, stream, table, grid, d);
     return check_launch("grid insert (deterministic)");
+}
+
+uint32_t grid_clear_blocks(uint32_t table_size)
+{
+    uint32_t blocks = (2u * table_size / 4u + 255u) / 256u;
+    return blocks > 4096u ? 4096u : (blocks == 0 ? 1u : blocks);
+}
+
+int grid_clear_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, hipStream_t stream)
+{
+    if (blocks == 0u) return RGI_OK;
+    hipLaunchKernelGGL(k_grid_clear_batch, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    return check_launch("grid clear (batch)");
+}
+
+int grid_insert_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks,
+                      hipStream_t stream)
+{
+    if (blocks == 0u) return RGI_OK;
+    hipLaunchKernelGGL(k_grid_insert_batch, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    return check_launch("grid insert (batch)");
 }
 
 } // namespace rgi

@@ -33,6 +33,11 @@ struct rgi_context {
     void* bufs[3] = {};
     size_t buf_bytes[3] = {};
     int compute_units = 0;
+    // The light table and size that the guide buffer last held a guide for
+    // (a renderer's lights are fixed for its lifetime, so it builds it once).
+    const void* guide_lights = nullptr;
+    int guide_count = 0;
+    const void* guide_buffer = nullptr;
 };
 
 namespace rgi {
@@ -100,11 +105,36 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
                const rgi_gi_reservoirs& res, uint16_t* color, uint16_t* moments, const Tuning& tuning,
                hipStream_t stream);
 // With a context, scenes with more lights than fit in LDS search the CDF
-// from a guide table that the context holds.
+// from a guide table that the context holds, built on every call unless
+// `fixed_lights` (the caller's light table never changes, as a renderer's):
+// then once per table.
 int di_temporal(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g,
-                const rgi_di_reservoirs& prev, const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream);
+                const rgi_di_reservoirs& prev, const rgi_di_reservoirs& out, const Tuning& tuning, hipStream_t stream,
+                bool fixed_lights = false);
 int di_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_di_reservoirs& res,
                uint16_t* diffuse, uint16_t* specular, const Tuning& tuning, hipStream_t stream);
+
+// Batches (batch.h): each screen pass over every view of a batch in one
+// launch. The guide table of DI temporal's light choice for frame f (no
+// table, and 0 buckets, when the scene's lights fit in LDS or there is no
+// context), the listed GI temporal's buffers, and the passes.
+struct TileList;
+struct BatchView;
+struct BatchLaunch;
+int di_light_guide(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, uint32_t** guide, uint32_t* buckets,
+                   hipStream_t stream, bool fixed_lights = false);
+int gi_listed_buffers(rgi_context* ctx, const rgi_frame& f, TileList& list, CandRays& c);
+uint32_t grid_clear_blocks(uint32_t table_size);
+int grid_clear_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, hipStream_t stream);
+int grid_insert_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks,
+                      hipStream_t stream);
+int di_temporal_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks,
+                      hipStream_t stream);
+int di_spatial_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks,
+                     hipStream_t stream);
+int gi_trace_batch(const BatchLaunch& b, hipStream_t stream);
+int gi_resample_batch(const BatchLaunch& b, hipStream_t stream);
+int gi_spatial_batch(const BatchLaunch& b, hipStream_t stream);
 
 // The neural radiance cache's frame-driver passes (nrc.hip): the cache's
 // radiance at every G-buffer pixel, and one frame's training records.

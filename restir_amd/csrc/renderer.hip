@@ -7,6 +7,7 @@
 #include <new>
 
 #include "../../include/restir_amd_renderer.h"
+#include "batch.h"
 #include "internal.h"
 
 #include <vector>
@@ -422,13 +423,33 @@ static void join_nrc(rgi_renderer* r, hipStream_t stream)
     r->nrc_pending = false;
 }
 
-// One frame. `fork_grid` builds the hash grid on the renderer's side stream
-// and trains the cache on the cache's stream; `cross_frame` leaves that
-// training running past the end of the frame, joined where the next frame
-// first touches the cache (eager frames only: a captured frame joins every
-// branch before its capture ends).
-static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr, bool fork_grid,
-                        bool cross_frame)
+// Everything one frame of a renderer settles on the host before its first
+// launch: the packed frame, the ping-pong parity, and the planes that each
+// pass reads and writes.
+struct FrameJob {
+    rgi_settings s;
+    rgi_frame f;  // DI, grid, GI tracing, denoiser, composite
+    rgi_frame fg; // GI resampling, spatial and specular: f with the cache's lighting
+    int read, write;
+    bool di_active, split, nrc_on, nrc_query, shadow_pass;
+    const float* shadow_map;
+    // The GI passes' surface light (renderer.cpp:690-695): the DI diffuse
+    // light with ReSTIR DI (black when it has no lights), else the emission;
+    // the cache's prediction once it has trained.
+    const uint16_t* surface_light;
+    // With the denoiser, the GI history colour is its first filtered
+    // iteration (renderer.cpp:774-776).
+    const uint16_t* history_color;
+    const uint16_t* query_emission;   // the emission the cache's query adds, or NULL
+    const uint16_t* resolved_diffuse; // the composite's diffuse light from the cache, or NULL
+    rgi_di_reservoirs prev_di, next_di;
+    rgi_gi_reservoirs gi_out;
+    rgi_gi_inputs tin, sin;
+};
+
+// Checks the frame's arguments and plans it (no GPU work). The only state
+// it changes is the reference mode's reset of DI history.
+static int plan_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, FrameJob& j)
 {
     if (!r || !view || !g) return fail(RGI_E_INVALID, "renderer_render: NULL argument");
     if (g->width != r->width || g->height != r->height)
@@ -436,137 +457,107 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
                     r->width, r->height);
     if (!g->depth || !g->normal || !g->albedo || !g->shading_normal || !g->emission)
         return fail(RGI_E_INVALID, "renderer_render: every G-buffer plane is required");
-    hipStream_t stream = as_stream(stream_ptr);
-    rgi_settings s = effective(r);
+    j.s = effective(r);
+    const rgi_settings& s = j.s;
     if (r->reference) r->di_valid = false; // no DI temporal reuse either (renderer.cpp:451)
     const rgi_view& prev = r->have_frame ? r->previous : *view;
-    rgi_frame f;
-    bool di_active = s.restir_di && r->scene.light_count > 0 && r->scene.node_count > 0;
+    j.di_active = s.restir_di && r->scene.light_count > 0 && r->scene.node_count > 0;
     int e = rgi_frame_pack(&s, &r->scene, r->radius, r->frame_number, view, &prev, r->width, r->height,
-                           r->history_valid, r->di_valid, &f);
+                           r->history_valid, r->di_valid, &j.f);
     if (e != RGI_OK) return e;
-    if (!di_active) f.di_light_count = 0;
-    const int read = r->read, write = 1 - r->read;
+    if (!j.di_active) j.f.di_light_count = 0;
+    j.read = r->read;
+    j.write = 1 - r->read;
+    const int read = j.read, write = j.write;
+    j.shadow_map = s.shadows ? r->atlas : nullptr;
+    j.shadow_pass = s.shadows && !r->atlas_valid;
+    j.split = gi_temporal_split(r->ctx, j.f);
+    j.nrc_on = r->nrc && !r->reference;
+    j.nrc_query = j.nrc_on && r->nrc_trained >= r->nrc_settings.warmup;
+    j.fg = j.f;
+    j.surface_light = s.restir_di ? r->black : g->emission;
+    if (j.di_active) j.surface_light = r->diffuse;
+    j.query_emission = nullptr;
+    j.resolved_diffuse = nullptr;
+    if (j.nrc_query) {
+        j.query_emission = (!s.restir_di && s.emissive_lighting) ? g->emission : nullptr;
+        j.surface_light = r->nrc_plane;
+        j.fg.multiple_bounces = 0;
+        j.fg.lighting_surface = 1.0f;
+        // The composite's diffuse light: the cache's prediction itself,
+        // which covers DI and GI only when the training target holds both.
+        if (r->nrc_settings.resolve && s.multiple_bounces)
+            j.resolved_diffuse = j.query_emission ? r->nrc_resolved : r->nrc_plane;
+    }
+    j.history_color = s.denoiser ? r->filtered[read] : r->color[read];
+    j.prev_di = rgi_di_reservoirs{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
+    j.next_di = rgi_di_reservoirs{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
+    j.gi_out = rgi_gi_reservoirs{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
+    j.tin = rgi_gi_inputs{r->gi[read][0], r->gi[read][1], r->gi[read][2], j.history_color, r->moments[read],
+                          j.surface_light, j.shadow_map, r->grid[read]};
+    j.sin = rgi_gi_inputs{nullptr, r->gi[read][1], r->gi[read][2], j.history_color, r->moments[read],
+                          j.surface_light, j.shadow_map, r->grid[write]};
+    return RGI_OK;
+}
 
-    const float* shadow_map = nullptr;
-    if (s.shadows) {
-        if (!r->atlas_valid) {
-            // The shadow pass (renderer.cpp:527-596).
-            rgi_shadow sh;
-            if ((e = rgi_shadow_setup(s.light_position, r->radius, shadow_face_size(&s), s.homogeneous_depth,
-                                      s.origin_bottom_left, &sh)) != RGI_OK)
-                return fail(e, "shadow setup failed");
-            if ((e = shadow_atlas(sh, r->scene, r->atlas, stream)) != RGI_OK) return e;
-            r->atlas_valid = true;
-        }
-        shadow_map = r->atlas;
-    }
-    // GI temporal's tracing half reads only the G-buffer and the scene (and
-    // writes the sky tiles' empty reservoirs into this frame's GI planes,
-    // which nothing reads before GI temporal), so with `fork_grid` it runs
-    // on the side stream beside the DI passes, joined before GI temporal's
-    // resampling. This frame's hash grid (renderer.cpp:735-750) follows it
-    // there: only GI spatial and the specular pass read the grid (GI temporal
-    // reads last frame's), so it is joined before GI spatial. The buffers
-    // both write were last read by last frame's passes, which precede the
-    // fork on `stream`.
-    hipStream_t grid_stream = fork_grid ? r->side : stream;
-    const bool split = gi_temporal_split(r->ctx, f);
-    rgi_gi_reservoirs gi_out{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
-    if (fork_grid) {
-        hipEventRecord(r->side_fork, stream);
-        hipStreamWaitEvent(r->side, r->side_fork, 0);
-        if (split) {
-            e = gi_temporal_trace(r->ctx, f, r->scene, *g, gi_out, r->side);
-            hipEventRecord(r->side_trace, r->side);
-        }
-    }
-    if (e == RGI_OK) e = grid_clear(r->grid[write], f.grid_table_size, grid_stream);
-    if (e == RGI_OK) {
-        if (s.grid_mode == RGI_GRID_DETERMINISTIC)
-            e = grid_insert_deterministic(r->ctx, f, *g, r->grid[write], grid_stream);
-        else
-            e = grid_insert_atomic(f, *g, r->grid[write], grid_stream);
-    }
-    if (fork_grid) hipEventRecord(r->side_join, r->side);
-    // The GI passes' surface light (renderer.cpp:690-695): the DI diffuse
-    // light with ReSTIR DI (black when it has no lights), else the emission.
-    const uint16_t* surface_light = s.restir_di ? r->black : g->emission;
-    // With the denoiser, the GI history colour is its first filtered
-    // iteration (renderer.cpp:774-776).
-    const uint16_t* history_color = s.denoiser ? r->filtered[read] : r->color[read];
-    // The neural radiance cache stands in for the surface light and last
-    // frame's indirect light of on-screen sample points once it has trained.
-    const bool nrc_on = r->nrc && !r->reference;
-    rgi_frame fg = f;
-    const uint16_t* resolved_diffuse = nullptr;
-    // The DI passes, the cache's query and GI temporal, beside the grid.
-    auto front = [&]() -> int {
-        int fe;
-        if (di_active) {
-            rgi_di_reservoirs prev_di{r->di_sample[read], r->di_receiver[read], r->di_normal[read]};
-            rgi_di_reservoirs next_di{r->di_sample[write], r->di_receiver[write], r->di_normal[write]};
-            if ((fe = di_temporal(r->ctx, f, r->scene, *g, prev_di, next_di, r->ctx->tuning, stream)) != RGI_OK)
-                return fe;
-            if ((fe = di_spatial(f, r->scene, *g, next_di, r->diffuse, r->specular, r->ctx->tuning, stream)) !=
-                RGI_OK)
-                return fe;
-            surface_light = r->diffuse;
-            r->di_valid = true;
-        } else {
-            r->di_valid = false;
-        }
-        if (nrc_on && r->nrc_trained >= r->nrc_settings.warmup) {
-            join_nrc(r, stream);
-            const uint16_t* emission = (!s.restir_di && s.emissive_lighting) ? g->emission : nullptr;
-            if ((fe = nrc_infer_gbuffer(r->nrc, f, *g, emission, r->nrc_settings.precision, r->nrc_plane, stream)) !=
-                RGI_OK)
-                return fe;
-            surface_light = r->nrc_plane;
-            fg.multiple_bounces = 0;
-            fg.lighting_surface = 1.0f;
-            // The composite's diffuse light: the cache's prediction itself,
-            // which covers DI and GI only when the training target holds both.
-            if (r->nrc_settings.resolve && s.multiple_bounces) {
-                resolved_diffuse = r->nrc_plane;
-                if (emission) {
-                    if ((fe = nrc_infer_gbuffer(r->nrc, f, *g, nullptr, r->nrc_settings.precision, r->nrc_resolved,
-                                                stream)) != RGI_OK)
-                        return fe;
-                    resolved_diffuse = r->nrc_resolved;
-                }
-            }
-        }
-        rgi_gi_inputs tin{r->gi[read][0], r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
-                          surface_light, shadow_map, r->grid[read]};
-        if (fork_grid && split) {
-            hipStreamWaitEvent(stream, r->side_trace, 0);
-            return gi_temporal_resample(r->ctx, fg, r->scene, *g, tin, gi_out, r->ctx->tuning, stream);
-        }
-        return gi_temporal(r->ctx, fg, r->scene, *g, tin, gi_out, r->ctx->tuning, stream);
-    };
-    if (e == RGI_OK) e = front();
-    if (fork_grid) {
-        // Every branch of the side stream is joined, also when a pass failed.
-        if (split) hipStreamWaitEvent(stream, r->side_trace, 0);
-        hipStreamWaitEvent(stream, r->side_join, 0);
-    }
+// The shadow pass (renderer.cpp:527-596), on the first frame with shadows.
+static int frame_shadow(rgi_renderer* r, const FrameJob& j, hipStream_t stream)
+{
+    if (!j.shadow_pass) return RGI_OK;
+    rgi_shadow sh;
+    int e = rgi_shadow_setup(j.s.light_position, r->radius, shadow_face_size(&j.s), j.s.homogeneous_depth,
+                             j.s.origin_bottom_left, &sh);
+    if (e != RGI_OK) return fail(e, "shadow setup failed");
+    if ((e = shadow_atlas(sh, r->scene, r->atlas, stream)) != RGI_OK) return e;
+    r->atlas_valid = true;
+    return RGI_OK;
+}
+
+// This frame's hash grid (renderer.cpp:735-750).
+static int frame_grid(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream)
+{
+    int e = grid_clear(r->grid[j.write], j.f.grid_table_size, stream);
     if (e != RGI_OK) return e;
-    rgi_gi_reservoirs res{r->gi[write][0], r->gi[write][1], r->gi[write][2]};
-    rgi_gi_inputs sin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
-                      surface_light, shadow_map, r->grid[write]};
-    if ((e = gi_spatial(fg, r->scene, *g, sin, res, r->color[write], r->moments[write], r->ctx->tuning, stream)) !=
-        RGI_OK)
-        return e;
+    if (j.s.grid_mode == RGI_GRID_DETERMINISTIC) return grid_insert_deterministic(r->ctx, j.f, g, r->grid[j.write], stream);
+    return grid_insert_atomic(j.f, g, r->grid[j.write], stream);
+}
 
+// The cache's query at every pixel once it has trained, on a stream that
+// follows the cache's last training step.
+static int nrc_query_planes(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream)
+{
+    if (!j.nrc_query) return RGI_OK;
+    int e = nrc_infer_gbuffer(r->nrc, j.f, g, j.query_emission, r->nrc_settings.precision, r->nrc_plane, stream);
+    if (e != RGI_OK || !j.query_emission || !j.resolved_diffuse) return e;
+    return nrc_infer_gbuffer(r->nrc, j.f, g, nullptr, r->nrc_settings.precision, r->nrc_resolved, stream);
+}
+
+static int frame_nrc_query(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream)
+{
+    if (!j.nrc_query) return RGI_OK;
+    join_nrc(r, stream);
+    return nrc_query_planes(r, j, g, stream);
+}
+
+// Everything after GI spatial: indirect specular, the denoiser, the cache's
+// training step, the composite, and the end of the frame. A non-null
+// `train_on` runs the training step on that stream (forked from `stream`);
+// `cross_frame` leaves it running past the end of the frame.
+static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, const rgi_gbuffer& g,
+                      hipStream_t stream, hipStream_t train_on, bool cross_frame)
+{
+    const rgi_settings& s = j.s;
+    const rgi_frame& f = j.f;
+    const int read = j.read, write = j.write;
+    int e;
     // Indirect specular (renderer.cpp:836-855): last frame's composite is
     // still in the HDR target.
     const uint16_t* specular = nullptr;
     if (s.specular_gi && r->specular_gi[0]) {
         const int sw = 1 - r->specular_read;
-        rgi_gi_inputs spin{nullptr, r->gi[read][1], r->gi[read][2], history_color, r->moments[read],
-                           surface_light, shadow_map, r->grid[write]};
-        if ((e = specular_gi(fg, r->scene, *g, spin, r->hdr, r->specular_gi[r->specular_read], r->specular_gi[sw],
+        rgi_gi_inputs spin{nullptr, r->gi[read][1], r->gi[read][2], j.history_color, r->moments[read],
+                           j.surface_light, j.shadow_map, r->grid[write]};
+        if ((e = specular_gi(j.fg, r->scene, g, spin, r->hdr, r->specular_gi[r->specular_read], r->specular_gi[sw],
                              r->ctx->tuning, stream)) != RGI_OK)
             return e;
         r->specular_read = sw;
@@ -594,30 +585,30 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     // One training step of the cache on this frame's surfaces: target = DI
     // diffuse + albedo * next frame's GI history colour.
     // The gather reads this frame's planes on `stream`; the step itself reads
-    // only the gathered records and the cache, so with `fork_grid` it runs on
-    // the cache's stream beside the denoiser and composite (and, with
-    // `cross_frame`, the next frame's DI passes and grid).
-    if (nrc_on) {
+    // only the gathered records and the cache, so with `train_on` it runs
+    // beside the denoiser and composite (and, with `cross_frame`, the next
+    // frame's DI passes and grid).
+    if (j.nrc_on) {
         join_nrc(r, stream);
         const uint16_t* next_history = s.denoiser ? r->filtered[write] : r->color[write];
         // Records at the frame's ReSTIR GI path vertices (the temporal
         // reservoirs' sample points), at its surface pixels, or alternately.
         const int rec = r->nrc_settings.records;
         const bool vertices = rec == RGI_NRC_RECORDS_VERTICES || rec == RGI_NRC_RECORDS_MIXED;
-        if ((e = nrc_gather(r->nrc, f, *g, di_active ? r->diffuse : nullptr, s.multiple_bounces ? next_history : nullptr,
+        if ((e = nrc_gather(r->nrc, f, g, j.di_active ? r->diffuse : nullptr, s.multiple_bounces ? next_history : nullptr,
                             vertices ? r->gi[write][0] : nullptr, vertices ? r->gi[write][2] : nullptr,
                             rec == RGI_NRC_RECORDS_MIXED ? 2u : 1u, r->nrc_settings.batch, f.frame_index ^ 0x4e5243u,
                             r->nrc_records, r->nrc_targets, stream)) != RGI_OK)
             return e;
         hipStream_t train_stream = stream;
-        if (fork_grid) {
+        if (train_on) {
             hipEventRecord(r->nrc_fork, stream);
-            hipStreamWaitEvent(r->nrc_stream, r->nrc_fork, 0);
-            train_stream = r->nrc_stream;
+            hipStreamWaitEvent(train_on, r->nrc_fork, 0);
+            train_stream = train_on;
         }
         e = rgi_nrc_train(r->nrc, r->nrc_records, r->nrc_targets, r->nrc_settings.batch, r->nrc_loss, train_stream);
-        if (fork_grid) {
-            hipEventRecord(r->nrc_done, r->nrc_stream);
+        if (train_on) {
+            hipEventRecord(r->nrc_done, train_on);
             r->nrc_pending = true;
         }
         if (e != RGI_OK) {
@@ -631,17 +622,18 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     if (s.composite && r->hdr) {
         rgi_composite_inputs ci{};
         ci.gi = r->denoised;
-        ci.di_diffuse = di_active ? r->diffuse : nullptr;
-        ci.di_specular = di_active ? r->specular : nullptr;
+        ci.di_diffuse = j.di_active ? r->diffuse : nullptr;
+        ci.di_specular = j.di_active ? r->specular : nullptr;
         ci.specular = specular;
-        ci.shadow_map = shadow_map;
-        ci.resolved_diffuse = resolved_diffuse;
+        ci.shadow_map = j.shadow_map;
+        ci.resolved_diffuse = j.resolved_diffuse;
         for (int k = 0; k < 4; ++k) ci.options[k] = 1.0f;
         ci.specular_weight = specular ? 1.0f : 0.0f;
-        if ((e = composite(f, *g, ci, r->hdr, stream)) != RGI_OK) return e;
+        if ((e = composite(f, g, ci, r->hdr, stream)) != RGI_OK) return e;
     }
 
     // End of frame (renderer.cpp:1119-1124).
+    r->di_valid = j.di_active;
     r->previous = *view;
     r->have_frame = true;
     r->read = write;
@@ -650,6 +642,69 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     r->last = f;
     if (!cross_frame) join_nrc(r, stream);
     return RGI_OK;
+}
+
+// One frame. `fork_grid` traces GI temporal's candidates and builds the hash
+// grid on the renderer's side stream and trains the cache on the cache's
+// stream; `cross_frame` leaves that training running past the end of the
+// frame, joined where the next frame first touches the cache (eager frames
+// only: a captured frame joins every branch before its capture ends).
+static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr, bool fork_grid,
+                        bool cross_frame)
+{
+    FrameJob j;
+    int e = plan_frame(r, view, g, j);
+    if (e != RGI_OK) return e;
+    hipStream_t stream = as_stream(stream_ptr);
+    if ((e = frame_shadow(r, j, stream)) != RGI_OK) return e;
+    // GI temporal's tracing half reads only the G-buffer and the scene (and
+    // writes the sky tiles' empty reservoirs into this frame's GI planes,
+    // which nothing reads before GI temporal), so with `fork_grid` it runs
+    // on the side stream beside the DI passes, joined before GI temporal's
+    // resampling. This frame's hash grid follows it there: only GI spatial
+    // and the specular pass read the grid (GI temporal reads last frame's),
+    // so it is joined before GI spatial. The buffers both write were last
+    // read by last frame's passes, which precede the fork on `stream`.
+    hipStream_t grid_stream = fork_grid ? r->side : stream;
+    if (fork_grid) {
+        hipEventRecord(r->side_fork, stream);
+        hipStreamWaitEvent(r->side, r->side_fork, 0);
+        if (j.split) {
+            e = gi_temporal_trace(r->ctx, j.f, r->scene, *g, j.gi_out, r->side);
+            hipEventRecord(r->side_trace, r->side);
+        }
+    }
+    if (e == RGI_OK) e = frame_grid(r, j, *g, grid_stream);
+    if (fork_grid) hipEventRecord(r->side_join, r->side);
+    // The DI passes, the cache's query and GI temporal, beside the grid.
+    auto front = [&]() -> int {
+        int fe;
+        if (j.di_active) {
+            if ((fe = di_temporal(r->ctx, j.f, r->scene, *g, j.prev_di, j.next_di, r->ctx->tuning, stream, true)) !=
+                RGI_OK)
+                return fe;
+            if ((fe = di_spatial(j.f, r->scene, *g, j.next_di, r->diffuse, r->specular, r->ctx->tuning, stream)) !=
+                RGI_OK)
+                return fe;
+        }
+        if ((fe = frame_nrc_query(r, j, *g, stream)) != RGI_OK) return fe;
+        if (fork_grid && j.split) {
+            hipStreamWaitEvent(stream, r->side_trace, 0);
+            return gi_temporal_resample(r->ctx, j.fg, r->scene, *g, j.tin, j.gi_out, r->ctx->tuning, stream);
+        }
+        return gi_temporal(r->ctx, j.fg, r->scene, *g, j.tin, j.gi_out, r->ctx->tuning, stream);
+    };
+    if (e == RGI_OK) e = front();
+    if (fork_grid) {
+        // Every branch of the side stream is joined, also when a pass failed.
+        if (j.split) hipStreamWaitEvent(stream, r->side_trace, 0);
+        hipStreamWaitEvent(stream, r->side_join, 0);
+    }
+    if (e != RGI_OK) return e;
+    if ((e = gi_spatial(j.fg, r->scene, *g, j.sin, j.gi_out, r->color[j.write], r->moments[j.write], r->ctx->tuning,
+                        stream)) != RGI_OK)
+        return e;
+    return frame_tail(r, j, view, *g, stream, fork_grid ? r->nrc_stream : nullptr, cross_frame);
 }
 
 int rgi_renderer_render(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer* g, void* stream_ptr)
@@ -855,12 +910,30 @@ int rgi_renderer_last_frame(const rgi_renderer* r, rgi_frame* out)
 
 // ------------------------------------------------------------------ batch
 
-// Renderers of any sizes and scenes whose frames go out together as one
-// graph: each renderer's frame is captured on a stream of its own, forked
-// from and joined back to the caller's stream, so small views overlap on the
-// GPU instead of running one after another.
+// Renderers of any sizes and scenes whose frames go out together. The
+// default (pass-major) form launches each screen pass once for every view:
+// the kernels read each view's frame, scene and planes from a device table
+// of BatchView entries, uploaded once per frame from a pinned staging slot,
+// each view taking its own slice of the workgroups (screen passes) or its
+// own tiles and rays from its own counters (the listed and persistent
+// passes). Passes with per-renderer state (the cache's query and training,
+// specular, denoiser, composite) stay per view. The graph form captures each
+// renderer's frame on a stream of its own.
 struct rgi_batch {
     std::vector<rgi_renderer*> renderers;
+    int mode = RGI_BATCH_PASSES;
+    // Pass-major form: the side stream that traces and builds the grids, its
+    // fork and join events, and the view table (two pinned staging slots,
+    // each reused once the copy out of it has run, and the device copy).
+    hipStream_t side = nullptr;
+    hipEvent_t side_fork = nullptr, traced = nullptr, gridded = nullptr;
+    char* staging[2] = {};
+    hipEvent_t copied[2] = {};
+    bool copy_pending[2] = {};
+    int slot = 0;
+    char* table = nullptr;
+    size_t table_bytes = 0;
+    // Graph form.
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> joined;
     hipEvent_t fork = nullptr;
@@ -871,22 +944,38 @@ struct rgi_batch {
 int rgi_batch_create(rgi_batch** out, rgi_renderer* const* renderers, uint32_t count)
 {
     if (!out || !renderers || count == 0) return fail(RGI_E_INVALID, "batch_create: renderers are required");
+    if (count > kMaxBatchViews) return fail(RGI_E_INVALID, "batch_create: %u renderers (at most %u)", count, kMaxBatchViews);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!renderers[i]) return fail(RGI_E_INVALID, "batch_create: renderer %u is NULL", i);
+        for (uint32_t k = 0; k < i; ++k)
+            if (renderers[k] == renderers[i])
+                return fail(RGI_E_INVALID, "batch_create: renderer %u appears twice (also at %u)", i, k);
+    }
     rgi_batch* b = new (std::nothrow) rgi_batch();
     if (!b) return fail(RGI_E_NOMEM, "batch allocation failed");
     b->renderers.assign(renderers, renderers + count);
-    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    b->table_bytes = align256(count * sizeof(BatchView)) + (size_t)kBatchPasses * (count + 1) * sizeof(uint32_t);
+    bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+              hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->side_fork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->traced, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->gridded, hipEventDisableTiming) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&b->table), b->table_bytes) == hipSuccess;
+    for (int k = 0; k < 2 && ok; ++k)
+        ok = hipHostMalloc(reinterpret_cast<void**>(&b->staging[k]), b->table_bytes, hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipEventCreateWithFlags(&b->copied[k], hipEventDisableTiming) == hipSuccess;
     for (uint32_t i = 0; i < count && ok; ++i) {
-        if (!renderers[i]) ok = false;
         hipStream_t st = nullptr;
         hipEvent_t ev = nullptr;
-        ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
+        ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
         if (st) b->streams.push_back(st);
         ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
         if (ev) b->joined.push_back(ev);
     }
     if (!ok) {
         rgi_batch_destroy(b);
-        return fail(RGI_E_DEVICE, "batch_create: streams, events or renderers missing");
+        return fail(RGI_E_DEVICE, "batch_create: streams, events or the view table could not be created");
     }
     *out = b;
     return RGI_OK;
@@ -900,14 +989,40 @@ void rgi_batch_destroy(rgi_batch* b)
     for (hipStream_t st : b->streams) hipStreamDestroy(st);
     for (hipEvent_t ev : b->joined) hipEventDestroy(ev);
     if (b->fork) hipEventDestroy(b->fork);
+    if (b->side) hipStreamDestroy(b->side);
+    if (b->side_fork) hipEventDestroy(b->side_fork);
+    if (b->traced) hipEventDestroy(b->traced);
+    if (b->gridded) hipEventDestroy(b->gridded);
+    for (int k = 0; k < 2; ++k) {
+        if (b->staging[k]) hipHostFree(b->staging[k]);
+        if (b->copied[k]) hipEventDestroy(b->copied[k]);
+    }
+    if (b->table) hipFree(b->table);
     delete b;
 }
 
-int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffers, void* stream_ptr)
+int rgi_batch_set_mode(rgi_batch* b, int mode)
 {
-    if (!b || !views || !gbuffers) return fail(RGI_E_INVALID, "batch_render: NULL argument");
-    if (!stream_ptr) return fail(RGI_E_INVALID, "batch_render: capture needs a stream other than the null stream");
-    hipStream_t stream = as_stream(stream_ptr);
+    if (!b) return fail(RGI_E_INVALID, "batch_set_mode: NULL batch");
+    if (mode != RGI_BATCH_PASSES && mode != RGI_BATCH_GRAPH)
+        return fail(RGI_E_INVALID, "batch_set_mode: mode %d is not RGI_BATCH_PASSES or RGI_BATCH_GRAPH", mode);
+    b->mode = mode;
+    return RGI_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// Workgroups of a slice, padded to whole rounds of 8 so that every view's
+// slice starts on XCD 0 (its workgroup b runs on XCD b mod 8, as in a launch
+// of its own).
+uint32_t slice(uint32_t blocks) { return (blocks + 7u) & ~7u; }
+
+// The graph form: each renderer's frame captured on a stream of its own.
+int render_graph_branches(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffers, hipStream_t stream)
+{
+    if (!stream) return fail(RGI_E_INVALID, "batch_render: capture needs a stream other than the null stream");
     size_t n = b->renderers.size();
     int e = RGI_OK;
     // A cache that has not trained yet allocates its gather and training
@@ -967,6 +1082,181 @@ int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbu
         return fail(RGI_E_DEVICE, "batch_render: launch failed");
     }
     return RGI_OK;
+}
+
+// The pass-major form. Every view is checked and planned before the first
+// launch, so a view that fails its checks leaves every renderer as it was.
+int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffers, hipStream_t stream)
+{
+    const uint32_t n = (uint32_t)b->renderers.size();
+    std::vector<FrameJob> jobs(n);
+    std::vector<HostState> before;
+    for (uint32_t i = 0; i < n; ++i) before.push_back(save_state(b->renderers[i]));
+    auto restore_all = [&]() {
+        for (uint32_t i = 0; i < n; ++i) restore_state(b->renderers[i], before[i]);
+    };
+    int e = RGI_OK;
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) e = plan_frame(b->renderers[i], &views[i], &gbuffers[i], jobs[i]);
+    if (e != RGI_OK) {
+        restore_all();
+        return e;
+    }
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) e = frame_shadow(b->renderers[i], jobs[i], stream);
+    if (e != RGI_OK) {
+        restore_all();
+        return e;
+    }
+
+    // The view table, in the staging slot whose last copy has run.
+    const int slot = b->slot;
+    if (b->copy_pending[slot] && hipEventSynchronize(b->copied[slot]) != hipSuccess)
+        return fail(RGI_E_DEVICE, "batch_render: the view table's last upload did not finish");
+    b->copy_pending[slot] = false;
+    char* host = b->staging[slot];
+    memset(host, 0, b->table_bytes);
+    BatchView* hv = reinterpret_cast<BatchView*>(host);
+    uint32_t* hfirst = reinterpret_cast<uint32_t*>(host + align256(n * sizeof(BatchView)));
+    BatchLaunch L{};
+    L.count = n;
+    L.views = reinterpret_cast<const BatchView*>(b->table);
+    for (int p = 0; p < kBatchPasses; ++p) {
+        L.first[p] = reinterpret_cast<const uint32_t*>(b->table + align256(n * sizeof(BatchView))) + p * (n + 1);
+        hfirst[p * (n + 1)] = 0u;
+    }
+    const Tuning& tuning0 = b->renderers[0]->ctx->tuning;
+    L.gi_temporal_waves = tuning0.gi_temporal_waves;
+    L.gi_spatial_waves = tuning0.gi_spatial_waves;
+    L.compute_units = context_compute_units(b->renderers[0]->ctx);
+    bool any_di = false;
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) {
+        rgi_renderer* r = b->renderers[i];
+        const FrameJob& j = jobs[i];
+        const rgi_gbuffer& g = gbuffers[i];
+        BatchView& v = hv[i];
+        v.f = j.f;
+        v.fg = j.fg;
+        v.s = r->scene;
+        v.g = g;
+        v.texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+        v.gi_texel = F2{1.0f / (float)j.f.gi_width, 1.0f / (float)j.f.gi_height};
+        const Tuning& t = r->ctx->tuning;
+        v.di_tiles = tile_map(t.tile_order, t.tile_shape, g.width, g.height);
+        const dim3 dg = tile_grid(v.di_tiles);
+        v.di_gx = dg.x;
+        v.di_gy = dg.y;
+        v.di_prev = j.prev_di;
+        v.di_next = j.next_di;
+        v.di_diffuse = r->diffuse;
+        v.di_specular = r->specular;
+        if (j.di_active) {
+            uint32_t* guide = nullptr;
+            e = di_light_guide(r->ctx, j.f, r->scene, &guide, &v.guide_buckets, stream, true);
+            v.guide = guide;
+            any_di = true;
+        }
+        v.grid = r->grid[j.write];
+        v.grid_gx = ((uint32_t)j.f.gi_width + 15u) / 16u;
+        v.grid_gy = ((uint32_t)j.f.gi_height + 15u) / 16u;
+        v.tin = j.tin;
+        v.sin = j.sin;
+        v.res = j.gi_out;
+        v.color = r->color[j.write];
+        v.moments = r->moments[j.write];
+        v.gi_tiles = tile_map(t.tile_order, t.tile_shape, j.f.gi_width, j.f.gi_height);
+        const dim3 gg = tile_grid(v.gi_tiles);
+        v.gi_gx = gg.x;
+        v.gi_gy = gg.y;
+        v.same = g.width == j.f.gi_width && g.height == j.f.gi_height ? 1u : 0u;
+        v.split = j.split ? 1u : 0u;
+        if (j.split && e == RGI_OK) {
+            e = gi_listed_buffers(r->ctx, j.f, v.list, v.cand);
+            L.split = true;
+            L.tiles += v.list.tiles;
+            if (j.f.initial_candidates > 0) {
+                L.rounds += (v.list.tiles + 3u) / 4u;
+                if (j.f.ray_mode != RGI_RAY_SCREEN) L.trace = true;
+            }
+        }
+        (v.same ? L.same : L.larger) = true;
+        const uint32_t blocks[kBatchPasses] = {
+            j.di_active ? slice(dg.x * dg.y) : 0u,
+            slice(grid_clear_blocks(j.f.grid_table_size)),
+            j.s.grid_mode == RGI_GRID_DETERMINISTIC ? 0u : slice(v.grid_gx * v.grid_gy),
+            j.split ? slice((v.list.tiles + 15u) / 16u) : 0u,
+            slice(gg.x * gg.y)};
+        for (int p = 0; p < kBatchPasses; ++p) hfirst[p * (n + 1) + i + 1] = hfirst[p * (n + 1) + i] + blocks[p];
+    }
+    if (e != RGI_OK) {
+        restore_all();
+        return e;
+    }
+    for (int p = 0; p < kBatchPasses; ++p) L.blocks[p] = hfirst[p * (n + 1) + n];
+    if (hipMemcpyAsync(b->table, host, b->table_bytes, hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipEventRecord(b->copied[slot], stream) != hipSuccess) {
+        restore_all();
+        return fail(RGI_E_DEVICE, "batch_render: the view table could not be uploaded");
+    }
+    b->copy_pending[slot] = true;
+    b->slot = 1 - slot;
+
+    // The side stream, beside the DI passes on `stream`: the caches'
+    // queries (after their last training steps, which the previous frame
+    // left on this stream or a renderer's own frame on the cache's stream),
+    // GI temporal's tracing half, then the grids; joined before GI temporal's
+    // resampling and GI spatial.
+    hipEventRecord(b->side_fork, stream);
+    hipStreamWaitEvent(b->side, b->side_fork, 0);
+    int se = RGI_OK;
+    for (uint32_t i = 0; i < n && se == RGI_OK; ++i) {
+        rgi_renderer* r = b->renderers[i];
+        if (!jobs[i].nrc_query) continue;
+        if (r->nrc_pending) hipStreamWaitEvent(b->side, r->nrc_done, 0);
+        se = nrc_query_planes(r, jobs[i], gbuffers[i], b->side);
+    }
+    if (se == RGI_OK) se = gi_trace_batch(L, b->side);
+    hipEventRecord(b->traced, b->side);
+    if (se == RGI_OK) se = grid_clear_batch(L.views, L.first[kPassGridClear], n, L.blocks[kPassGridClear], b->side);
+    if (se == RGI_OK) se = grid_insert_batch(L.views, L.first[kPassGrid], n, L.blocks[kPassGrid], b->side);
+    for (uint32_t i = 0; i < n && se == RGI_OK; ++i)
+        if (jobs[i].s.grid_mode == RGI_GRID_DETERMINISTIC)
+            se = grid_insert_deterministic(b->renderers[i]->ctx, jobs[i].f, gbuffers[i], b->renderers[i]->grid[jobs[i].write],
+                                           b->side);
+    hipEventRecord(b->gridded, b->side);
+
+    if (any_di) e = di_temporal_batch(L.views, L.first[kPassDi], n, L.blocks[kPassDi], stream);
+    if (any_di && e == RGI_OK) e = di_spatial_batch(L.views, L.first[kPassDi], n, L.blocks[kPassDi], stream);
+    hipStreamWaitEvent(stream, b->traced, 0);
+    // The side stream's queries followed every pending training step.
+    for (uint32_t i = 0; i < n; ++i)
+        if (jobs[i].nrc_query) b->renderers[i]->nrc_pending = false;
+    if (e == RGI_OK) e = gi_resample_batch(L, stream);
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) {
+        rgi_renderer* r = b->renderers[i];
+        if (!jobs[i].split)
+            e = gi_temporal(r->ctx, jobs[i].fg, r->scene, gbuffers[i], jobs[i].tin, jobs[i].gi_out, r->ctx->tuning,
+                            stream);
+    }
+    hipStreamWaitEvent(stream, b->gridded, 0);
+    if (e == RGI_OK) e = se;
+    if (e == RGI_OK) e = gi_spatial_batch(L, stream);
+    // The rest of each view's frame. The caches train on the side stream,
+    // past the end of the call (as rgi_renderer_render leaves a training
+    // step): beside the next frame's DI passes, ahead of its queries.
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
+        e = frame_tail(b->renderers[i], jobs[i], &views[i], gbuffers[i], stream, b->side, true);
+    return e;
+}
+
+} // namespace
+
+extern "C" {
+
+int rgi_batch_render(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffers, void* stream_ptr)
+{
+    if (!b || !views || !gbuffers) return fail(RGI_E_INVALID, "batch_render: NULL argument");
+    hipStream_t stream = as_stream(stream_ptr);
+    if (b->mode == RGI_BATCH_GRAPH) return render_graph_branches(b, views, gbuffers, stream);
+    return render_passes(b, views, gbuffers, stream);
 }
 
 } // extern "C"

@@ -301,12 +301,13 @@ def test_render_graph_matches_eager(torch):
             L.rgi_renderer_destroy(r)
 
 
-def test_batch_matches_single_renderers(torch):
-    """rgi_batch_render (renderers of different scenes and sizes in one HIP
-    graph, each on its own stream) against each renderer rendered alone:
-    bit-identical outputs on every frame, also after a captured batch frame
-    that failed part-way."""
-    say("case batch vs single renderers")
+@pytest.mark.parametrize("mode", [ra.RGI_BATCH_PASSES, ra.RGI_BATCH_GRAPH])
+def test_batch_matches_single_renderers(torch, mode):
+    """rgi_batch_render (renderers of different scenes and sizes: each pass
+    over every view in one launch, or one HIP graph with each renderer on its
+    own stream) against each renderer rendered alone: bit-identical outputs
+    on every frame, also after a batch frame that failed part-way."""
+    say("case batch vs single renderers, mode %d" % mode)
     L = ra.lib()
     cases = [(common.cornell(extra_lights=4), 96, 64),
              (ra.HostScene(ra.RGI_SCENE_BOXFIELD, seed=3, size=60, extra_lights=4), 80, 48),
@@ -330,6 +331,7 @@ def test_batch_matches_single_renderers(torch):
                 lst.append(r)
         arr = (ctypes.c_void_p * len(batch_r))(*[r.value for r in batch_r])
         ra.check(L.rgi_batch_create(ctypes.byref(batch), arr, len(batch_r)))
+        ra.check(L.rgi_batch_set_mode(batch, mode))
         yaw = 2 * np.pi / 64
         for i in range(4):
             views, gbs = [], []
@@ -349,10 +351,10 @@ def test_batch_matches_single_renderers(torch):
             va = (ra.View * len(views))(*views)
             ga = (ra.GBuffer * len(gbs))(*gbs)
             if i == 2:
-                # A captured batch frame that fails part-way (the last
-                # renderer's G-buffer has the wrong size, after the others
-                # were captured) leaves every renderer as it was: the retry
-                # below must still match the renderers alone.
+                # A batch frame that fails part-way (the last renderer's
+                # G-buffer has the wrong size: found after the others were
+                # captured, or planned) leaves every renderer as it was: the
+                # retry below must still match the renderers alone.
                 bad = list(gbs)
                 bad[-1] = ra.GBuffer(cases[-1][1] - 1, cases[-1][2], *(getattr(gbs[-1], k) for k in (
                     "depth", "albedo", "normal", "shading_normal", "emission")))
