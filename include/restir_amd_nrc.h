@@ -105,6 +105,18 @@ void rgi_nrc_destroy(rgi_nrc* nrc);
 int rgi_nrc_train(rgi_nrc* nrc, const float* records, const float* targets, uint32_t count, float* loss,
                   void* stream);
 
+/* One Adam step of each of `count` distinct caches, caches[i] on counts[i]
+ * records and targets (losses may be NULL, or hold NULL entries): each
+ * kernel of rgi_nrc_train's chain runs once for all the caches, each cache
+ * taking its own slice of the workgroups, so that small steps fill the chip
+ * together. Each cache ends where rgi_nrc_train would leave it (the table
+ * gradient accumulates with FP32 atomics in either, so the table's last bits
+ * follow the order the GPU runs them). Returns after the steps have run (its
+ * argument table is allocated and freed per call); a renderer batch trains
+ * its caches the same way from its own table without that wait. */
+int rgi_nrc_train_batch(rgi_nrc* const* caches, const float* const* records, const float* const* targets,
+                        const uint32_t* counts, float* const* losses, uint32_t count, void* stream);
+
 /* Radiance predictions for `count` records into out[4 * count]. */
 int rgi_nrc_infer(rgi_nrc* nrc, const float* records, uint32_t count, int precision, float* out, void* stream);
 

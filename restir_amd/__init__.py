@@ -242,6 +242,7 @@ def lib():
     L.rgi_nrc_destroy.restype = None
     L.rgi_nrc_destroy.argtypes = [P]
     L.rgi_nrc_train.argtypes = [P, P, P, U32, P, P]
+    L.rgi_nrc_train_batch.argtypes = [P, P, P, P, P, U32, P]
     L.rgi_nrc_infer.argtypes = [P, P, U32, ctypes.c_int, P, P]
     L.rgi_nrc_gradients.argtypes = [P, P, P, U32, P, P, P, P]
     L.rgi_nrc_get_params.argtypes = [P, P, P, P]

@@ -45,6 +45,14 @@ struct BatchView {
     uint32_t split;                 // GI temporal in its listed form (list and cand set)
 };
 
+// A cache's training step in a batch's table (nrc.hip fills it), and the
+// kernels of the training chain that give each cache a slice of a launch.
+constexpr size_t kNrcStepBytes = 512;
+enum NrcPass {
+    kNrcTrain = 0, kNrcWgrad, kNrcReduce, kNrcAdamMlp, kNrcCopies, kNrcAdamTable, kNrcEmaMlp, kNrcEmaCopies,
+    kNrcEmaTable, kNrcPasses
+};
+
 // The batched screen passes, each with its own slices of a launch.
 enum BatchPass { kPassDi = 0, kPassGridClear, kPassGrid, kPassTiles, kPassGiSpatial, kBatchPasses };
 

@@ -135,6 +135,19 @@ int di_spatial_batch(const BatchView* views, const uint32_t* first, uint32_t cou
 int gi_trace_batch(const BatchLaunch& b, hipStream_t stream);
 int gi_resample_batch(const BatchLaunch& b, hipStream_t stream);
 int gi_spatial_batch(const BatchLaunch& b, hipStream_t stream);
+// A batch of caches' training steps: nrc_step_plan writes the next step of
+// cache n on `count` records into a kNrcStepBytes slot (growing the cache's
+// scratch; nothing is launched) with each kernel's workgroups in blocks[];
+// nrc_train_batch runs the steps of a device table of such slots, each
+// kernel of rgi_nrc_train's chain once for all of them (first: per kernel,
+// count + 1 slice starts; blocks: per kernel, the launch's workgroups), with
+// results identical to rgi_nrc_train on each cache; nrc_step_taken then
+// advances each cache's step count.
+int nrc_step_plan(rgi_nrc* n, const float* records, const float* targets, uint32_t count, float* loss, void* slot,
+                  uint32_t* blocks);
+int nrc_train_batch(const void* steps, const uint32_t* first, uint32_t count, const uint32_t* blocks,
+                    hipStream_t stream);
+void nrc_step_taken(rgi_nrc* n);
 
 // The neural radiance cache's frame-driver passes (nrc.hip): the cache's
 // radiance at every G-buffer pixel, and one frame's training records.

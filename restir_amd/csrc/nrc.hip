@@ -23,6 +23,7 @@
 
 #include "../../include/restir_amd_nrc.h"
 #include "device.h"
+#include "batch.h"
 #include "internal.h"
 
 struct rgi_nrc {
@@ -307,11 +308,10 @@ struct TrainArgs {
 // Forward, loss and backward of 16 * kTrainB samples per wave: stores the
 // activations and back-propagated gradients for the weight-gradient GEMMs
 // and scatters the input gradient into the table gradient.
-__global__ void __launch_bounds__(256) k_nrc_train(TrainArgs a)
+RGI_DEV void nrc_train_body(const TrainArgs& a, const uint32_t tile)
 {
     constexpr int B = kTrainB;
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u), g = lane >> 4, s16 = lane & 15;
-    const uint32_t tile = blockIdx.x;
     const size_t layer_stride = (size_t)a.tiles * 64 * kTile;
     const size_t tile_base = (size_t)tile * 64 * kTile;
     const LaneLevels L = lane_levels(a.gp, g);
@@ -511,16 +511,16 @@ __global__ void __launch_bounds__(256) k_nrc_train(TrainArgs a)
     }
 }
 
+__global__ void __launch_bounds__(256) k_nrc_train(TrainArgs a) { nrc_train_body(a, blockIdx.x); }
+
 // Weight gradients: dW_k[o][i] = sum over samples of delta_k[o][s] a_{k-1}[i][s]
 // over one chunk of kChunk samples; one workgroup per (chunk, layer). For
 // W1..W5, wave w owns output rows 16w..16w+15; for W6, column block w of
 // rows 0..15.
-__global__ void __launch_bounds__(256) k_nrc_wgrad(const __bf16* acts, const __bf16* deltas, uint32_t tiles,
-                                                   float* partial)
+RGI_DEV void nrc_wgrad_body(const __bf16* acts, const __bf16* deltas, uint32_t tiles, float* partial,
+                            const uint32_t chunk, const int layer /* 0..5 -> W1..W6 */)
 {
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u), g = lane >> 4, s16 = lane & 15;
-    const int layer = (int)blockIdx.y; // 0..5 -> W1..W6
-    const uint32_t chunk = blockIdx.x;
     const size_t layer_stride = (size_t)tiles * 64 * kTile;
     const __bf16* dl = deltas + (size_t)layer * layer_stride;
     const __bf16* al = acts + (size_t)layer * layer_stride;
@@ -572,14 +572,25 @@ __global__ void __launch_bounds__(256) k_nrc_wgrad(const __bf16* acts, const __b
     }
 }
 
-// Sum of the chunks' partial gradients, in chunk order.
-__global__ void k_nrc_reduce(const float* partial, uint32_t chunks, float* grad)
+__global__ void __launch_bounds__(256) k_nrc_wgrad(const __bf16* acts, const __bf16* deltas, uint32_t tiles,
+                                                   float* partial)
 {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    nrc_wgrad_body(acts, deltas, tiles, partial, blockIdx.x, (int)blockIdx.y);
+}
+
+// Sum of the chunks' partial gradients, in chunk order.
+RGI_DEV void nrc_reduce_body(const float* partial, uint32_t chunks, float* grad, uint32_t block)
+{
+    uint32_t p = block * blockDim.x + threadIdx.x;
     if (p >= RGI_NRC_MLP_PARAMS) return;
     float s = 0.0f;
     for (uint32_t c = 0; c < chunks; ++c) s = s + partial[(size_t)c * RGI_NRC_MLP_PARAMS + p];
     grad[p] = s;
+}
+
+__global__ void k_nrc_reduce(const float* partial, uint32_t chunks, float* grad)
+{
+    nrc_reduce_body(partial, chunks, grad, blockIdx.x);
 }
 
 // ------------------------------------------------------------------- Adam
@@ -642,11 +653,9 @@ RGI_DEV uint8_t to_fp8(float x)
 
 // One workgroup per weight matrix: Adam on its parameters (when `step`), its
 // largest magnitude, and its FP8 weight and activation scales.
-__global__ void __launch_bounds__(256) k_nrc_adam_mlp(float* mlp, float* m, float* v, const float* grad, AdamArgs h,
-                                                      float* scales, const uint32_t* act_max, int step)
+RGI_DEV void nrc_adam_mlp_body(float* mlp, float* m, float* v, const float* grad, const AdamArgs& h, float* scales,
+                               const uint32_t* act_max, int step, const int layer, float* red)
 {
-    __shared__ float red[4];
-    const int layer = (int)blockIdx.x;
     const int lo = layer * 4096, hi = layer == 5 ? 20480 + 192 : lo + 4096;
     float wmax = 0.0f;
     for (int p = lo + (int)threadIdx.x; p < hi; p += 256) {
@@ -674,12 +683,19 @@ __global__ void __launch_bounds__(256) k_nrc_adam_mlp(float* mlp, float* m, floa
     }
 }
 
+__global__ void __launch_bounds__(256) k_nrc_adam_mlp(float* mlp, float* m, float* v, const float* grad, AdamArgs h,
+                                                      float* scales, const uint32_t* act_max, int step)
+{
+    __shared__ float red[4];
+    nrc_adam_mlp_body(mlp, m, v, grad, h, scales, act_max, step, (int)blockIdx.x, red);
+}
+
 // Every fragment slot of the forward (BF16, FP16, FP8) and transposed (BF16)
 // copies, one per thread.
-__global__ void k_nrc_copies(const float* mlp, const float* scales, __bf16* w_fwd, __bf16* w_bwd, _Float16* w_fwd_h,
-                             uint8_t* w_fwd_8)
+RGI_DEV void nrc_copies_body(const float* mlp, const float* scales, __bf16* w_fwd, __bf16* w_bwd, _Float16* w_fwd_h,
+                             uint8_t* w_fwd_8, uint32_t block)
 {
-    int e = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    int e = (int)(block * blockDim.x + threadIdx.x);
     if (e < kFwdBlocks * kFrag) {
         int block = e / kFrag, lane = (e % kFrag) / 8, q = e % 8;
         float w = fwd_weight(mlp, block, lane, q);
@@ -695,14 +711,18 @@ __global__ void k_nrc_copies(const float* mlp, const float* scales, __bf16* w_fw
     }
 }
 
+__global__ void k_nrc_copies(const float* mlp, const float* scales, __bf16* w_fwd, __bf16* w_bwd, _Float16* w_fwd_h,
+                             uint8_t* w_fwd_8)
+{
+    nrc_copies_body(mlp, scales, w_fwd, w_bwd, w_fwd_h, w_fwd_8, blockIdx.x);
+}
+
 // The parameters' moving average after a step: e = d e + (1 - d) w, one
 // workgroup per weight matrix, with the FP8 weight scale of the averaged
 // matrix (the activation scales are the trained ones).
-__global__ void __launch_bounds__(256) k_nrc_ema_mlp(const float* mlp, float* ema, float d, float one_minus_d,
-                                                     const float* scales, float* ema_scales)
+RGI_DEV void nrc_ema_mlp_body(const float* mlp, float* ema, float d, float one_minus_d, const float* scales,
+                              float* ema_scales, const int layer, float* red)
 {
-    __shared__ float red[4];
-    const int layer = (int)blockIdx.x;
     const int lo = layer * 4096, hi = layer == 5 ? 20480 + 192 : lo + 4096;
     float wmax = 0.0f;
     for (int p = lo + (int)threadIdx.x; p < hi; p += 256) {
@@ -723,11 +743,18 @@ __global__ void __launch_bounds__(256) k_nrc_ema_mlp(const float* mlp, float* em
     }
 }
 
-// The table's moving average, four values per thread, and its FP16 copy.
-__global__ void k_nrc_ema_table(const float* table, float* ema, uint32_t* ema_h, float d, float one_minus_d,
-                                size_t quads)
+__global__ void __launch_bounds__(256) k_nrc_ema_mlp(const float* mlp, float* ema, float d, float one_minus_d,
+                                                     const float* scales, float* ema_scales)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float red[4];
+    nrc_ema_mlp_body(mlp, ema, d, one_minus_d, scales, ema_scales, (int)blockIdx.x, red);
+}
+
+// The table's moving average, four values per thread, and its FP16 copy.
+RGI_DEV void nrc_ema_table_body(const float* table, float* ema, uint32_t* ema_h, float d, float one_minus_d,
+                                size_t quads, uint32_t block)
+{
+    size_t i = (size_t)block * blockDim.x + threadIdx.x;
     if (i >= quads) return;
     float4 w = reinterpret_cast<const float4*>(table)[i];
     float4 e = reinterpret_cast<float4*>(ema)[i];
@@ -736,6 +763,12 @@ __global__ void k_nrc_ema_table(const float* table, float* ema, uint32_t* ema_h,
     reinterpret_cast<float4*>(ema)[i] = e;
     reinterpret_cast<uint2*>(ema_h)[i] = make_uint2((uint32_t)float_to_half(e.x) | ((uint32_t)float_to_half(e.y) << 16),
                                                     (uint32_t)float_to_half(e.z) | ((uint32_t)float_to_half(e.w) << 16));
+}
+
+__global__ void k_nrc_ema_table(const float* table, float* ema, uint32_t* ema_h, float d, float one_minus_d,
+                                size_t quads)
+{
+    nrc_ema_table_body(table, ema, ema_h, d, one_minus_d, quads, blockIdx.x);
 }
 
 __global__ void k_nrc_clear_max(uint32_t* act_max, float* loss)
@@ -751,10 +784,10 @@ __global__ void k_nrc_copy_loss(const float* from, float* to)
 
 // Adam on the table: four floats (two entries) per thread; an entry pair
 // whose gradient is all zero is left alone. The gradient is cleared.
-__global__ void k_nrc_adam_table(float* table, float* m, float* v, float* grad, uint32_t* table_h, AdamArgs h,
-                                 size_t quads)
+RGI_DEV void nrc_adam_table_body(float* table, float* m, float* v, float* grad, uint32_t* table_h, const AdamArgs& h,
+                                 size_t quads, uint32_t block)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t i = (size_t)block * blockDim.x + threadIdx.x;
     if (i >= quads) return;
     float4 gq = reinterpret_cast<const float4*>(grad)[i];
     if (gq.x == 0.0f && gq.y == 0.0f && gq.z == 0.0f && gq.w == 0.0f) return;
@@ -773,6 +806,138 @@ __global__ void k_nrc_adam_table(float* table, float* m, float* v, float* grad, 
     uint2 hv = make_uint2((uint32_t)float_to_half(ps[0]) | ((uint32_t)float_to_half(ps[1]) << 16),
                           (uint32_t)float_to_half(ps[2]) | ((uint32_t)float_to_half(ps[3]) << 16));
     reinterpret_cast<uint2*>(table_h)[i] = hv;
+}
+
+__global__ void k_nrc_adam_table(float* table, float* m, float* v, float* grad, uint32_t* table_h, AdamArgs h,
+                                 size_t quads)
+{
+    nrc_adam_table_body(table, m, v, grad, table_h, h, quads, blockIdx.x);
+}
+
+// ------------------------------------------------------- batched training
+
+// One cache's training step in a batch of caches: the arguments of every
+// kernel of rgi_nrc_train's chain. Each kernel of the chain runs once for
+// the whole batch, each cache taking its own slice of the workgroups, so the
+// chain's many small launches fill the chip together.
+struct NrcStep {
+    TrainArgs a;
+    uint32_t active;  // the cache trains in this batch
+    uint32_t chunks;
+    uint32_t ema;     // a moving average is kept
+    float d, om;      // its decay this step
+    float* partial;
+    float* mlp;
+    float* mlp_m;
+    float* mlp_v;
+    float* mlp_grad;
+    AdamArgs h;
+    float* scales;
+    __bf16* w_fwd;
+    __bf16* w_bwd;
+    _Float16* w_fwd_h;
+    uint8_t* w_fwd_8;
+    float* table;
+    float* table_m;
+    float* table_v;
+    float* table_grad;
+    uint32_t* table_h;
+    size_t quads;
+    float* ema_mlp;
+    float* ema_scales;
+    __bf16* ema_w_fwd;
+    __bf16* ema_w_bwd;
+    _Float16* ema_w_fwd_h;
+    uint8_t* ema_w_fwd_8;
+    float* ema_table;
+    uint32_t* ema_table_h;
+    float* loss_out;  // where the step's loss goes, or NULL
+};
+static_assert(sizeof(NrcStep) <= kNrcStepBytes, "a cache's step must fit its slot of a batch's table");
+
+// Step i of a table: the steps lie kNrcStepBytes apart (not sizeof(NrcStep)).
+RGI_DEV const NrcStep& step_at(const uint8_t* steps, uint32_t i)
+{
+    return *reinterpret_cast<const NrcStep*>(steps + (size_t)i * kNrcStepBytes);
+}
+
+#define RGI_NRC_STEP(steps, first, count)                                                                              \
+    uint32_t b;                                                                                                        \
+    const NrcStep& st = step_at(steps, batch_view(first, count, blockIdx.x, b))
+
+__global__ void k_nrc_clear_max_batch(const uint8_t* __restrict__ steps)
+{
+    const NrcStep& st = step_at(steps, blockIdx.x);
+    if (!st.active) return;
+    if (threadIdx.x < 6) st.a.act_max[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) *st.a.loss = 0.0f;
+}
+
+__global__ void __launch_bounds__(256) k_nrc_train_batch(const uint8_t* __restrict__ steps,
+                                                         const uint32_t* __restrict__ first, uint32_t count)
+{
+    RGI_NRC_STEP(steps, first, count);
+    nrc_train_body(st.a, b);
+}
+
+__global__ void __launch_bounds__(256) k_nrc_wgrad_batch(const uint8_t* __restrict__ steps,
+                                                         const uint32_t* __restrict__ first, uint32_t count)
+{
+    RGI_NRC_STEP(steps, first, count);
+    nrc_wgrad_body(st.a.acts, st.a.deltas, st.a.tiles, st.partial, b % st.chunks, (int)(b / st.chunks));
+}
+
+__global__ void k_nrc_reduce_batch(const uint8_t* __restrict__ steps, const uint32_t* __restrict__ first,
+                                   uint32_t count)
+{
+    RGI_NRC_STEP(steps, first, count);
+    nrc_reduce_body(st.partial, st.chunks, st.mlp_grad, b);
+}
+
+__global__ void __launch_bounds__(256) k_nrc_adam_mlp_batch(const uint8_t* __restrict__ steps,
+                                                            const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ float red[4];
+    RGI_NRC_STEP(steps, first, count);
+    nrc_adam_mlp_body(st.mlp, st.mlp_m, st.mlp_v, st.mlp_grad, st.h, st.scales, st.a.act_max, 1, (int)b, red);
+}
+
+// The parameters' copies (ema = 0) or the moving average's (ema = 1).
+template <int kEma>
+__global__ void k_nrc_copies_batch(const uint8_t* __restrict__ steps, const uint32_t* __restrict__ first,
+                                   uint32_t count)
+{
+    RGI_NRC_STEP(steps, first, count);
+    if (kEma) nrc_copies_body(st.ema_mlp, st.ema_scales, st.ema_w_fwd, st.ema_w_bwd, st.ema_w_fwd_h, st.ema_w_fwd_8, b);
+    else nrc_copies_body(st.mlp, st.scales, st.w_fwd, st.w_bwd, st.w_fwd_h, st.w_fwd_8, b);
+}
+
+__global__ void k_nrc_adam_table_batch(const uint8_t* __restrict__ steps, const uint32_t* __restrict__ first,
+                                       uint32_t count)
+{
+    RGI_NRC_STEP(steps, first, count);
+    nrc_adam_table_body(st.table, st.table_m, st.table_v, st.table_grad, st.table_h, st.h, st.quads, b);
+}
+
+__global__ void __launch_bounds__(256) k_nrc_ema_mlp_batch(const uint8_t* __restrict__ steps,
+                                                           const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ float red[4];
+    RGI_NRC_STEP(steps, first, count);
+    nrc_ema_mlp_body(st.mlp, st.ema_mlp, st.d, st.om, st.scales, st.ema_scales, (int)b, red);
+}
+
+__global__ void k_nrc_ema_table_batch(const uint8_t* __restrict__ steps, const uint32_t* __restrict__ first,
+                                      uint32_t count)
+{
+    RGI_NRC_STEP(steps, first, count);
+    nrc_ema_table_body(st.table, st.ema_table, st.ema_table_h, st.d, st.om, st.quads, b);
+}
+
+__global__ void k_nrc_copy_loss_batch(const uint8_t* __restrict__ steps)
+{
+    const NrcStep& st = step_at(steps, blockIdx.x);
+    if (st.active && st.loss_out && threadIdx.x == 0) *st.loss_out = *st.a.loss;
 }
 
 // Table gradient out (for the gradient entry point), then cleared.
@@ -1391,6 +1556,106 @@ rgi_nrc inference_view(const rgi_nrc& n)
 }
 
 } // namespace
+
+int nrc_step_plan(rgi_nrc* n, const float* records, const float* targets, uint32_t count, float* loss, void* slot,
+                  uint32_t* blocks)
+{
+    if (!n || !records || !targets || count == 0) return fail(RGI_E_INVALID, "nrc step: bad argument");
+    const uint32_t tiles = (count + kTile - 1) / kTile;
+    int e = ensure_scratch(n, tiles);
+    if (e) return e;
+    NrcStep st;
+    memset(&st, 0, sizeof(st));
+    st.a.gp = grid_of(*n);
+    st.a.records = records;
+    st.a.targets = targets;
+    st.a.count = count;
+    st.a.inv_count = 1.0f / (float)count;
+    st.a.w_fwd = n->w_fwd;
+    st.a.w_bwd = n->w_bwd;
+    st.a.table_h = n->table_h;
+    st.a.table_grad = n->table_grad;
+    st.a.acts = n->acts;
+    st.a.deltas = n->deltas;
+    st.a.tiles = tiles;
+    st.a.loss = n->loss;
+    st.a.act_max = n->act_max;
+    st.active = 1u;
+    st.chunks = (tiles + kChunkTiles - 1) / kChunkTiles;
+    // The step about to be taken: rgi_nrc_train's step count after its
+    // increment.
+    const uint32_t step = n->steps + 1;
+    st.ema = n->config.ema_decay > 0.0f ? 1u : 0u;
+    st.d = st.ema ? ema_decay_at(*n, step) : 0.0f;
+    st.om = 1.0f - st.d;
+    st.partial = n->partial;
+    st.mlp = n->mlp;
+    st.mlp_m = n->mlp_m;
+    st.mlp_v = n->mlp_v;
+    st.mlp_grad = n->mlp_grad;
+    st.h = adam_args(*n, step);
+    st.scales = n->scales;
+    st.w_fwd = n->w_fwd;
+    st.w_bwd = n->w_bwd;
+    st.w_fwd_h = n->w_fwd_h;
+    st.w_fwd_8 = n->w_fwd_8;
+    st.table = n->table;
+    st.table_m = n->table_m;
+    st.table_v = n->table_v;
+    st.table_grad = n->table_grad;
+    st.table_h = n->table_h;
+    st.quads = n->table_floats / 4;
+    st.ema_mlp = n->ema_mlp;
+    st.ema_scales = n->ema_scales;
+    st.ema_w_fwd = n->ema_w_fwd;
+    st.ema_w_bwd = n->ema_w_bwd;
+    st.ema_w_fwd_h = n->ema_w_fwd_h;
+    st.ema_w_fwd_8 = n->ema_w_fwd_8;
+    st.ema_table = n->ema_table;
+    st.ema_table_h = n->ema_table_h;
+    st.loss_out = loss;
+    memcpy(slot, &st, sizeof(st));
+    const uint32_t copies = ((kFwdBlocks + kBwdBlocks) * kFrag + 255) / 256,
+                   quads = (uint32_t)((st.quads + 255) / 256);
+    blocks[kNrcTrain] = tiles;
+    blocks[kNrcWgrad] = st.chunks * 6u;
+    blocks[kNrcReduce] = (RGI_NRC_MLP_PARAMS + 255) / 256;
+    blocks[kNrcAdamMlp] = 6u;
+    blocks[kNrcCopies] = copies;
+    blocks[kNrcAdamTable] = quads;
+    blocks[kNrcEmaMlp] = st.ema ? 6u : 0u;
+    blocks[kNrcEmaCopies] = st.ema ? copies : 0u;
+    blocks[kNrcEmaTable] = st.ema ? quads : 0u;
+    return RGI_OK;
+}
+
+void nrc_step_taken(rgi_nrc* n) { n->steps += 1; }
+
+int nrc_train_batch(const void* steps, const uint32_t* first, uint32_t count, const uint32_t* blocks,
+                    hipStream_t stream)
+{
+    const uint8_t* st = static_cast<const uint8_t*>(steps);
+    auto at = [&](int p) { return first + (size_t)p * (count + 1); };
+    hipLaunchKernelGGL(k_nrc_clear_max_batch, dim3(count), dim3(64), 0, stream, st);
+    hipLaunchKernelGGL(k_nrc_train_batch, dim3(blocks[kNrcTrain]), dim3(256), 0, stream, st, at(kNrcTrain), count);
+    hipLaunchKernelGGL(k_nrc_wgrad_batch, dim3(blocks[kNrcWgrad]), dim3(256), 0, stream, st, at(kNrcWgrad), count);
+    hipLaunchKernelGGL(k_nrc_reduce_batch, dim3(blocks[kNrcReduce]), dim3(256), 0, stream, st, at(kNrcReduce), count);
+    hipLaunchKernelGGL(k_nrc_adam_mlp_batch, dim3(blocks[kNrcAdamMlp]), dim3(256), 0, stream, st, at(kNrcAdamMlp),
+                       count);
+    hipLaunchKernelGGL(k_nrc_copies_batch<0>, dim3(blocks[kNrcCopies]), dim3(256), 0, stream, st, at(kNrcCopies), count);
+    hipLaunchKernelGGL(k_nrc_adam_table_batch, dim3(blocks[kNrcAdamTable]), dim3(256), 0, stream, st, at(kNrcAdamTable),
+                       count);
+    if (blocks[kNrcEmaMlp] > 0u) {
+        hipLaunchKernelGGL(k_nrc_ema_mlp_batch, dim3(blocks[kNrcEmaMlp]), dim3(256), 0, stream, st, at(kNrcEmaMlp),
+                           count);
+        hipLaunchKernelGGL(k_nrc_copies_batch<1>, dim3(blocks[kNrcEmaCopies]), dim3(256), 0, stream, st,
+                           at(kNrcEmaCopies), count);
+        hipLaunchKernelGGL(k_nrc_ema_table_batch, dim3(blocks[kNrcEmaTable]), dim3(256), 0, stream, st,
+                           at(kNrcEmaTable), count);
+    }
+    hipLaunchKernelGGL(k_nrc_copy_loss_batch, dim3(count), dim3(64), 0, stream, st);
+    return check_launch("nrc train (batch)");
+}
 } // namespace rgi
 
 using namespace rgi;
@@ -1537,6 +1802,44 @@ int rgi_nrc_train(rgi_nrc* n, const float* records, const float* targets, uint32
     if ((e = refresh_average(n, s))) return e;
     if (loss) hipLaunchKernelGGL(k_nrc_copy_loss, dim3(1), dim3(64), 0, s, n->loss, loss);
     return check_launch("nrc train");
+}
+
+int rgi_nrc_train_batch(rgi_nrc* const* caches, const float* const* records, const float* const* targets,
+                        const uint32_t* counts, float* const* losses, uint32_t count, void* stream)
+{
+    if (!caches || !records || !targets || !counts || count == 0 || count > 1024)
+        return fail(RGI_E_INVALID, "nrc_train_batch: bad argument (count %u, at most 1024)", count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!caches[i] || !records[i] || !targets[i] || counts[i] == 0)
+            return fail(RGI_E_INVALID, "nrc_train_batch: cache %u has a NULL argument or no records", i);
+        for (uint32_t k = 0; k < i; ++k)
+            if (caches[k] == caches[i]) return fail(RGI_E_INVALID, "nrc_train_batch: cache %u appears twice", i);
+    }
+    hipStream_t s = as_stream(stream);
+    const size_t steps_bytes = (size_t)count * kNrcStepBytes;
+    std::vector<uint8_t> host(steps_bytes + (size_t)kNrcPasses * (count + 1) * sizeof(uint32_t), 0);
+    uint32_t* first = reinterpret_cast<uint32_t*>(host.data() + steps_bytes);
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t blocks[kNrcPasses] = {};
+        int e = nrc_step_plan(caches[i], records[i], targets[i], counts[i], losses ? losses[i] : nullptr,
+                              host.data() + (size_t)i * kNrcStepBytes, blocks);
+        if (e) return e;
+        for (int p = 0; p < kNrcPasses; ++p) first[p * (count + 1) + i + 1] = first[p * (count + 1) + i] + blocks[p];
+    }
+    uint32_t totals[kNrcPasses];
+    for (int p = 0; p < kNrcPasses; ++p) totals[p] = first[p * (count + 1) + count];
+    void* table = nullptr;
+    if (hipMalloc(&table, host.size()) != hipSuccess) return fail(RGI_E_NOMEM, "nrc_train_batch: table allocation");
+    int e = hipMemcpy(table, host.data(), host.size(), hipMemcpyHostToDevice) == hipSuccess
+                ? nrc_train_batch(table, reinterpret_cast<const uint32_t*>(static_cast<uint8_t*>(table) + steps_bytes),
+                                  count, totals, s)
+                : fail(RGI_E_DEVICE, "nrc_train_batch: table upload");
+    const bool ran = hipStreamSynchronize(s) == hipSuccess;
+    (void)hipFree(table);
+    if (e) return e;
+    if (!ran) return fail(RGI_E_DEVICE, "nrc_train_batch: the steps failed");
+    for (uint32_t i = 0; i < count; ++i) nrc_step_taken(caches[i]);
+    return RGI_OK;
 }
 
 int rgi_nrc_gradients(rgi_nrc* n, const float* records, const float* targets, uint32_t count, float* mlp_grad,

@@ -539,12 +539,10 @@ static int frame_nrc_query(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer
     return nrc_query_planes(r, j, g, stream);
 }
 
-// Everything after GI spatial: indirect specular, the denoiser, the cache's
-// training step, the composite, and the end of the frame. A non-null
-// `train_on` runs the training step on that stream (forked from `stream`);
-// `cross_frame` leaves it running past the end of the frame.
-static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, const rgi_gbuffer& g,
-                      hipStream_t stream, hipStream_t train_on, bool cross_frame)
+// After GI spatial: indirect specular (its plane for the composite goes to
+// *specular_out, NULL without it) and the denoiser.
+static int frame_post(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream,
+                      const uint16_t** specular_out)
 {
     const rgi_settings& s = j.s;
     const rgi_frame& f = j.f;
@@ -553,6 +551,7 @@ static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, 
     // Indirect specular (renderer.cpp:836-855): last frame's composite is
     // still in the HDR target.
     const uint16_t* specular = nullptr;
+    *specular_out = nullptr;
     if (s.specular_gi && r->specular_gi[0]) {
         const int sw = 1 - r->specular_read;
         rgi_gi_inputs spin{nullptr, r->gi[read][1], r->gi[read][2], j.history_color, r->moments[read],
@@ -562,6 +561,7 @@ static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, 
             return e;
         r->specular_read = sw;
         specular = r->specular_gi[sw];
+        *specular_out = specular;
     }
 
     // Denoiser (renderer.cpp:857-907): variance into filter target 0;
@@ -581,43 +581,37 @@ static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, 
             r->denoised = output;
         }
     }
+    return RGI_OK;
+}
 
-    // One training step of the cache on this frame's surfaces: target = DI
-    // diffuse + albedo * next frame's GI history colour.
-    // The gather reads this frame's planes on `stream`; the step itself reads
-    // only the gathered records and the cache, so with `train_on` it runs
-    // beside the denoiser and composite (and, with `cross_frame`, the next
-    // frame's DI passes and grid).
-    if (j.nrc_on) {
-        join_nrc(r, stream);
-        const uint16_t* next_history = s.denoiser ? r->filtered[write] : r->color[write];
-        // Records at the frame's ReSTIR GI path vertices (the temporal
-        // reservoirs' sample points), at its surface pixels, or alternately.
-        const int rec = r->nrc_settings.records;
-        const bool vertices = rec == RGI_NRC_RECORDS_VERTICES || rec == RGI_NRC_RECORDS_MIXED;
-        if ((e = nrc_gather(r->nrc, f, g, j.di_active ? r->diffuse : nullptr, s.multiple_bounces ? next_history : nullptr,
-                            vertices ? r->gi[write][0] : nullptr, vertices ? r->gi[write][2] : nullptr,
-                            rec == RGI_NRC_RECORDS_MIXED ? 2u : 1u, r->nrc_settings.batch, f.frame_index ^ 0x4e5243u,
-                            r->nrc_records, r->nrc_targets, stream)) != RGI_OK)
-            return e;
-        hipStream_t train_stream = stream;
-        if (train_on) {
-            hipEventRecord(r->nrc_fork, stream);
-            hipStreamWaitEvent(train_on, r->nrc_fork, 0);
-            train_stream = train_on;
-        }
-        e = rgi_nrc_train(r->nrc, r->nrc_records, r->nrc_targets, r->nrc_settings.batch, r->nrc_loss, train_stream);
-        if (train_on) {
-            hipEventRecord(r->nrc_done, train_on);
-            r->nrc_pending = true;
-        }
-        if (e != RGI_OK) {
-            join_nrc(r, stream);
-            return e;
-        }
-        ++r->nrc_trained;
-    }
+// The cache's training records of this frame's surfaces: target = DI
+// diffuse + albedo * next frame's GI history colour; after `stream` has
+// joined the cache's last training step, which reads the same records.
+static int frame_gather(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream)
+{
+    const rgi_settings& s = j.s;
+    const rgi_frame& f = j.f;
+    const int write = j.write;
+    join_nrc(r, stream);
+    const uint16_t* next_history = s.denoiser ? r->filtered[write] : r->color[write];
+    // Records at the frame's ReSTIR GI path vertices (the temporal
+    // reservoirs' sample points), at its surface pixels, or alternately.
+    const int rec = r->nrc_settings.records;
+    const bool vertices = rec == RGI_NRC_RECORDS_VERTICES || rec == RGI_NRC_RECORDS_MIXED;
+    return nrc_gather(r->nrc, f, g, j.di_active ? r->diffuse : nullptr, s.multiple_bounces ? next_history : nullptr,
+                      vertices ? r->gi[write][0] : nullptr, vertices ? r->gi[write][2] : nullptr,
+                      rec == RGI_NRC_RECORDS_MIXED ? 2u : 1u, r->nrc_settings.batch, f.frame_index ^ 0x4e5243u,
+                      r->nrc_records, r->nrc_targets, stream);
+}
 
+// The composite and the end of the frame; `cross_frame` leaves a training
+// step of the cache running past it.
+static int frame_finish(rgi_renderer* r, const FrameJob& j, const rgi_view* view, const rgi_gbuffer& g,
+                        const uint16_t* specular, hipStream_t stream, bool cross_frame)
+{
+    const rgi_settings& s = j.s;
+    const rgi_frame& f = j.f;
+    int e;
     // Composite (renderer.cpp:967-990): the final image's weights.
     if (s.composite && r->hdr) {
         rgi_composite_inputs ci{};
@@ -636,12 +630,45 @@ static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, 
     r->di_valid = j.di_active;
     r->previous = *view;
     r->have_frame = true;
-    r->read = write;
+    r->read = j.write;
     r->history_valid = true;
     ++r->frame_number;
     r->last = f;
     if (!cross_frame) join_nrc(r, stream);
     return RGI_OK;
+}
+
+// Everything after GI spatial: indirect specular, the denoiser, the cache's
+// training step, the composite, and the end of the frame. A non-null
+// `train_on` runs the training step on that stream (forked from `stream`)
+// beside the composite (and, with `cross_frame`, the next frame's DI passes
+// and grid); `cross_frame` leaves it running past the end of the frame.
+static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, const rgi_gbuffer& g,
+                      hipStream_t stream, hipStream_t train_on, bool cross_frame)
+{
+    const uint16_t* specular = nullptr;
+    int e = frame_post(r, j, g, stream, &specular);
+    if (e != RGI_OK) return e;
+    if (j.nrc_on) {
+        if ((e = frame_gather(r, j, g, stream)) != RGI_OK) return e;
+        hipStream_t train_stream = stream;
+        if (train_on) {
+            hipEventRecord(r->nrc_fork, stream);
+            hipStreamWaitEvent(train_on, r->nrc_fork, 0);
+            train_stream = train_on;
+        }
+        e = rgi_nrc_train(r->nrc, r->nrc_records, r->nrc_targets, r->nrc_settings.batch, r->nrc_loss, train_stream);
+        if (train_on) {
+            hipEventRecord(r->nrc_done, train_on);
+            r->nrc_pending = true;
+        }
+        if (e != RGI_OK) {
+            join_nrc(r, stream);
+            return e;
+        }
+        ++r->nrc_trained;
+    }
+    return frame_finish(r, j, view, g, specular, stream, cross_frame);
 }
 
 // One frame. `fork_grid` traces GI temporal's candidates and builds the hash
@@ -919,6 +946,24 @@ int rgi_renderer_last_frame(const rgi_renderer* r, rgi_frame* out)
 // passes). Passes with per-renderer state (the cache's query and training,
 // specular, denoiser, composite) stay per view. The graph form captures each
 // renderer's frame on a stream of its own.
+namespace {
+// The view table of a batch of n views: the views, each pass's slice
+// starts, each cache's training step, and the training kernels' slice
+// starts.
+struct TableLayout {
+    size_t first, steps, nrc_first, bytes;
+};
+TableLayout table_layout(size_t n)
+{
+    TableLayout t;
+    t.first = align256(n * sizeof(BatchView));
+    t.steps = align256(t.first + (size_t)kBatchPasses * (n + 1) * sizeof(uint32_t));
+    t.nrc_first = t.steps + n * kNrcStepBytes;
+    t.bytes = t.nrc_first + (size_t)kNrcPasses * (n + 1) * sizeof(uint32_t);
+    return t;
+}
+} // namespace
+
 struct rgi_batch {
     std::vector<rgi_renderer*> renderers;
     int mode = RGI_BATCH_PASSES;
@@ -926,7 +971,7 @@ struct rgi_batch {
     // fork and join events, and the view table (two pinned staging slots,
     // each reused once the copy out of it has run, and the device copy).
     hipStream_t side = nullptr;
-    hipEvent_t side_fork = nullptr, traced = nullptr, gridded = nullptr;
+    hipEvent_t side_fork = nullptr, traced = nullptr, gridded = nullptr, gathered = nullptr;
     char* staging[2] = {};
     hipEvent_t copied[2] = {};
     bool copy_pending[2] = {};
@@ -954,12 +999,13 @@ int rgi_batch_create(rgi_batch** out, rgi_renderer* const* renderers, uint32_t c
     rgi_batch* b = new (std::nothrow) rgi_batch();
     if (!b) return fail(RGI_E_NOMEM, "batch allocation failed");
     b->renderers.assign(renderers, renderers + count);
-    b->table_bytes = align256(count * sizeof(BatchView)) + (size_t)kBatchPasses * (count + 1) * sizeof(uint32_t);
+    b->table_bytes = table_layout(count).bytes;
     bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
               hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
               hipEventCreateWithFlags(&b->side_fork, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&b->traced, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&b->gridded, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&b->gathered, hipEventDisableTiming) == hipSuccess &&
               hipMalloc(reinterpret_cast<void**>(&b->table), b->table_bytes) == hipSuccess;
     for (int k = 0; k < 2 && ok; ++k)
         ok = hipHostMalloc(reinterpret_cast<void**>(&b->staging[k]), b->table_bytes, hipHostMallocDefault) ==
@@ -993,6 +1039,7 @@ void rgi_batch_destroy(rgi_batch* b)
     if (b->side_fork) hipEventDestroy(b->side_fork);
     if (b->traced) hipEventDestroy(b->traced);
     if (b->gridded) hipEventDestroy(b->gridded);
+    if (b->gathered) hipEventDestroy(b->gathered);
     for (int k = 0; k < 2; ++k) {
         if (b->staging[k]) hipHostFree(b->staging[k]);
         if (b->copied[k]) hipEventDestroy(b->copied[k]);
@@ -1114,13 +1161,14 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     b->copy_pending[slot] = false;
     char* host = b->staging[slot];
     memset(host, 0, b->table_bytes);
+    const TableLayout T = table_layout(n);
     BatchView* hv = reinterpret_cast<BatchView*>(host);
-    uint32_t* hfirst = reinterpret_cast<uint32_t*>(host + align256(n * sizeof(BatchView)));
+    uint32_t* hfirst = reinterpret_cast<uint32_t*>(host + T.first);
     BatchLaunch L{};
     L.count = n;
     L.views = reinterpret_cast<const BatchView*>(b->table);
     for (int p = 0; p < kBatchPasses; ++p) {
-        L.first[p] = reinterpret_cast<const uint32_t*>(b->table + align256(n * sizeof(BatchView))) + p * (n + 1);
+        L.first[p] = reinterpret_cast<const uint32_t*>(b->table + T.first) + p * (n + 1);
         hfirst[p * (n + 1)] = 0u;
     }
     const Tuning& tuning0 = b->renderers[0]->ctx->tuning;
@@ -1186,6 +1234,22 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
             slice(gg.x * gg.y)};
         for (int p = 0; p < kBatchPasses; ++p) hfirst[p * (n + 1) + i + 1] = hfirst[p * (n + 1) + i] + blocks[p];
     }
+    // The caches' training steps of this frame, planned now (growing their
+    // scratch before the first launch) and launched after the gathers.
+    uint32_t* nfirst = reinterpret_cast<uint32_t*>(host + T.nrc_first);
+    uint32_t nrc_blocks[kNrcPasses] = {};
+    bool any_train = false;
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) {
+        rgi_renderer* r = b->renderers[i];
+        uint32_t blocks[kNrcPasses] = {};
+        if (jobs[i].nrc_on) {
+            e = nrc_step_plan(r->nrc, r->nrc_records, r->nrc_targets, r->nrc_settings.batch, r->nrc_loss,
+                              host + T.steps + i * kNrcStepBytes, blocks);
+            any_train = true;
+        }
+        for (int p = 0; p < kNrcPasses; ++p) nfirst[p * (n + 1) + i + 1] = nfirst[p * (n + 1) + i] + blocks[p];
+    }
+    for (int p = 0; p < kNrcPasses; ++p) nrc_blocks[p] = nfirst[p * (n + 1) + n];
     if (e != RGI_OK) {
         restore_all();
         return e;
@@ -1239,11 +1303,32 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     hipStreamWaitEvent(stream, b->gridded, 0);
     if (e == RGI_OK) e = se;
     if (e == RGI_OK) e = gi_spatial_batch(L, stream);
-    // The rest of each view's frame. The caches train on the side stream,
-    // past the end of the call (as rgi_renderer_render leaves a training
-    // step): beside the next frame's DI passes, ahead of its queries.
+    // The rest of each view's frame. The caches' steps train in one chain of
+    // launches on the side stream, past the end of the call (as
+    // rgi_renderer_render leaves a training step): beside the composites and
+    // the next frame's DI passes, ahead of its queries.
+    std::vector<const uint16_t*> specular(n, nullptr);
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) e = frame_post(b->renderers[i], jobs[i], gbuffers[i], stream, &specular[i]);
     for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
-        e = frame_tail(b->renderers[i], jobs[i], &views[i], gbuffers[i], stream, b->side, true);
+        if (jobs[i].nrc_on) e = frame_gather(b->renderers[i], jobs[i], gbuffers[i], stream);
+    if (any_train && e == RGI_OK) {
+        hipEventRecord(b->gathered, stream);
+        hipStreamWaitEvent(b->side, b->gathered, 0);
+        e = nrc_train_batch(b->table + T.steps, reinterpret_cast<const uint32_t*>(b->table + T.nrc_first), n,
+                            nrc_blocks, b->side);
+        for (uint32_t i = 0; i < n; ++i) {
+            rgi_renderer* r = b->renderers[i];
+            if (!jobs[i].nrc_on) continue;
+            hipEventRecord(r->nrc_done, b->side);
+            r->nrc_pending = true;
+            if (e == RGI_OK) {
+                nrc_step_taken(r->nrc);
+                ++r->nrc_trained;
+            }
+        }
+    }
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
+        e = frame_finish(b->renderers[i], jobs[i], &views[i], gbuffers[i], specular[i], stream, true);
     return e;
 }
 

@@ -593,3 +593,61 @@ def test_gather_at_path_vertices():
     assert np.array_equal(r[kept], f[kept]) and np.array_equal(t[kept], ft[kept])
     nc.close()
     L.rgi_destroy(ctx)
+
+
+def test_train_batch_matches_single_steps():
+    """rgi_nrc_train_batch (one Adam step of several caches of different
+    table sizes and record counts in one chain of launches) against
+    rgi_nrc_train on each cache from the same state: after the first step
+    the MLP parameters and their moving average are bit-identical (the MLP
+    gradient is a fixed-order reduction) and the table within its last bits
+    (its gradient accumulates with FP32 atomics in no fixed order); after
+    three steps both stay within a small tolerance, and every step count
+    advanced."""
+    L = ra.lib()
+    cases = [(21, 14, 4096), (22, 15, 1000), (23, 13, 17)]
+    batch_side, single_side, data = [], [], []
+    for seed, log2_table, n in cases:
+        a, _, rng = setup_cache(seed, log2_table)
+        b, _, _ = setup_cache(seed, log2_table)
+        batch_side.append(a)
+        single_side.append(b)
+        data.append([(torch.from_numpy(records(rng, n)).cuda(), torch.from_numpy(targets(rng, n)).cuda())
+                     for _ in range(3)])
+    P = ctypes.c_void_p
+    losses = [torch.zeros(1, dtype=torch.float32, device="cuda") for _ in cases]
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for step in range(3):
+        print("nrc train batch, step", step, flush=True)
+        caches = (P * 3)(*[c.handle.value for c in batch_side])
+        recs = (P * 3)(*[d[step][0].data_ptr() for d in data])
+        tgts = (P * 3)(*[d[step][1].data_ptr() for d in data])
+        counts = (ra.U32 * 3)(*[n for _, _, n in cases])
+        lp = (P * 3)(*[l.data_ptr() for l in losses])
+        ra.check(L.rgi_nrc_train_batch(caches, recs, tgts, counts, lp, 3, s))
+        for c, d in zip(single_side, data):
+            c.train(d[step][0], d[step][1])
+        torch.cuda.synchronize()
+        for k, (a, b) in enumerate(zip(batch_side, single_side)):
+            (ma, ta), (mb, tb) = a.params(), b.params()
+            (ea, fa), (eb, fb) = a.average(), b.average()
+            ma, ta, mb, tb = (x.cpu().numpy() for x in (ma, ta, mb, tb))
+            ea, eb = ea.cpu().numpy(), eb.cpu().numpy()
+            # Entries whose summed gradient lands on zero in one order and
+            # not in the other take or skip a whole Adam step: a few per
+            # hundred thousand at most.
+            d = np.abs(ta - tb)
+            if step == 0:
+                assert np.array_equal(ma, mb), "cache %d: MLP differs after one step" % k
+                assert np.array_equal(ea, eb), "cache %d: MLP average differs after one step" % k
+                assert np.count_nonzero(d > 1e-6) <= max(4, d.size // 100000), (k, np.count_nonzero(d > 1e-6))
+            else:
+                assert np.max(np.abs(ma - mb)) < 1e-3 * np.max(np.abs(mb)), (k, step)
+                assert np.count_nonzero(d > 1e-4) <= max(8, d.size // 10000), (k, step, np.count_nonzero(d > 1e-4))
+            assert float(losses[k].cpu()[0]) > 0.0
+            sa, sb = ra.U32(), ra.U32()
+            ra.check(L.rgi_nrc_steps(a.handle, ctypes.byref(sa)))
+            ra.check(L.rgi_nrc_steps(b.handle, ctypes.byref(sb)))
+            assert sa.value == sb.value == step + 1
+    for c in batch_side + single_side:
+        c.close()
