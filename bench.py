@@ -630,13 +630,15 @@ def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames,
             "drv": drv, "rend": rend, "gbufs": gbufs, "views": views}
 
 
-def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offsets=None):
+def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offsets=None,
+              forms=(ra.RGI_BATCH_GRAPH, ra.RGI_BATCH_PASSES), singles=True):
     """Renderers of `cases` as one batch frame: each renderer alone (its own
     frame time), all of them eagerly one after another on one stream, then
     all of them as one rgi_batch_render call in both of its forms: each pass
     over every view in one launch (RGI_BATCH_PASSES) and one graph with each
     renderer on a stream of its own (RGI_BATCH_GRAPH). With `nrc`, every
-    renderer has its own neural radiance cache."""
+    renderer has its own neural radiance cache. `forms` and `singles` limit
+    the run to some of the batch forms, without the single renderers."""
     import torch
     from restir_amd import nrc as rnrc
     L = ra.lib()
@@ -677,8 +679,8 @@ def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offse
         ra.check(L.rgi_renderer_render(r, ctypes.byref(views[k % views_per_case]),
                                        ctypes.byref(gbs[k % views_per_case].gb), es))
 
-    per_scene = []
-    for r, (views, gbs) in zip(rends, frames):
+    per_scene, eager_ms = [], float("nan")
+    for r, (views, gbs) in zip(rends, frames if singles else []):
         for k in range(warmup):
             one(r, views, gbs, k)
         torch.cuda.synchronize()
@@ -687,13 +689,14 @@ def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offse
             one(r, views, gbs, k)
         torch.cuda.synchronize()
         per_scene.append((time.perf_counter() - t0) * 1e3 / steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(steps):
-        for r, (views, gbs) in zip(rends, frames):
-            one(r, views, gbs, k)
-    torch.cuda.synchronize()
-    eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+    if singles:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            for r, (views, gbs) in zip(rends, frames):
+                one(r, views, gbs, k)
+        torch.cuda.synchronize()
+        eager_ms = (time.perf_counter() - t0) * 1e3 / steps
 
     batch = ctypes.c_void_p()
     arr = (ctypes.c_void_p * len(rends))(*[r.value for r in rends])
@@ -706,8 +709,8 @@ def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offse
         ga = (ra.GBuffer * len(rends))(*[g[k % views_per_case].gb for _, g in frames])
         ra.check(L.rgi_batch_render(batch, va, ga, bs))
 
-    batch_ms = {}
-    for mode in (ra.RGI_BATCH_GRAPH, ra.RGI_BATCH_PASSES):
+    batch_ms = {m: float("nan") for m in (ra.RGI_BATCH_GRAPH, ra.RGI_BATCH_PASSES)}
+    for mode in forms:
         ra.check(L.rgi_batch_set_mode(batch, mode))
         for k in range(warmup):
             batched(k)
@@ -791,6 +794,9 @@ def main():
                          "the workload of the atrium's kernel profile")
     ap.add_argument("--no-variants", dest="variants", action="store_false",
                     help="skip the deterministic-insert and half-resolution GI frames")
+    ap.add_argument("--batch-only", choices=["uneven", "views8"],
+                    help="run only one batch workload (--steps timed batch frames after --warmup), pass-major form "
+                         "only, and print it: the workload of the batch kernel profiles")
     ap.add_argument("--cache-frames-only", action="store_true",
                     help="render only the metric's path with the neural radiance cache on (16 warm-up + --steps "
                          "frames) and print their frame time: the workload of the cache's kernel profile")
@@ -817,6 +823,17 @@ def main():
     mode = ra.RGI_GRID_ATOMIC if args.grid_mode == "atomic" else ra.RGI_GRID_DETERMINISTIC
     if args.atrium_only:
         print(json.dumps(atrium_workload(args, W, H, mode)[0]), flush=True)
+        return
+    if args.batch_only:
+        if args.batch_only == "uneven":
+            cases, kw = UNEVEN_SCENES, {"nrc": args.nrc}
+        else:
+            cases = [("atrium_256k_view%d" % v, ra.RGI_SCENE_ATRIUM, 262144, 1, W, H, 0.7) for v in range(8)]
+            kw = {"yaw_offsets": [v * math.pi / 4 for v in range(8)]}
+        out = run_batch(cases, mode, args.steps, args.warmup, forms=(ra.RGI_BATCH_PASSES,), singles=False, **kw)
+        print(json.dumps({"batch": args.batch_only, "passes_batch_ms_per_batch_frame":
+                          out["passes_batch_ms_per_batch_frame"], "steps": args.steps, "warmup": args.warmup}),
+              flush=True)
         return
     # The metric's frame: grid + resample + trace. The denoiser that follows
     # it in the reference is timed on its own below.
