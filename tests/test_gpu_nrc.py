@@ -651,3 +651,63 @@ def test_train_batch_matches_single_steps():
             assert sa.value == sb.value == step + 1
     for c in batch_side + single_side:
         c.close()
+
+
+def test_renderer_caches_in_pass_major_batch():
+    """Three renderers with resolving caches as one pass-major batch (the
+    caches' queries and one chain of training steps on the batch's side
+    stream, left running across calls) against the same renderers rendered
+    alone: every batch frame trains each cache once, and the resolved images
+    are finite and close to the eager renderers' mean (the table gradient's
+    FP32 atomics make the caches differ in their last bits, so the check is
+    statistical)."""
+    W, H = 160, 120
+    n = W * H
+    st = ra.Settings(grid_table_size=1 << 16, grid_mode=ra.RGI_GRID_DETERMINISTIC)
+    hs, gscene, ctx, gbuf, view = _setup_scene(W, H, st)
+    L = ra.lib()
+    from restir_amd import driver
+    s = driver.stream()
+    settings = rnrc.renderer_settings(hs, batch=4096, warmup=2, resolve=1)
+    batched = [_renderer(hs, gscene, st, W, H, settings) for _ in range(3)]
+    alone = [_renderer(hs, gscene, st, W, H, settings) for _ in range(3)]
+    batch = ctypes.c_void_p()
+    out = ra.RendererOutputs()
+    try:
+        arr = (ctypes.c_void_p * 3)(*[r.value for r in batched])
+        ra.check(L.rgi_batch_create(ctypes.byref(batch), arr, 3))
+        ra.check(L.rgi_batch_set_mode(batch, ra.RGI_BATCH_PASSES))
+        frames = 12
+        views = (ra.View * 3)(view, view, view)
+        gbufs = (ra.GBuffer * 3)(gbuf.gb, gbuf.gb, gbuf.gb)
+        for frame in range(frames):
+            print("pass-major batch with caches, frame", frame, flush=True)
+            ra.check(L.rgi_batch_render(batch, views, gbufs, s))
+            for r in alone:
+                ra.check(L.rgi_renderer_render(r, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+        torch.cuda.synchronize()
+        means = []
+        for r in batched + alone:
+            cache, rad_ptr, loss_ptr, trained = rnrc.renderer_state(r)
+            assert cache and rad_ptr and trained == frames
+            loss = torch.empty(1, dtype=torch.float32, device="cuda")
+            L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            assert L.hipMemcpy(loss.data_ptr(), loss_ptr, 4, 3) == 0
+            assert np.isfinite(float(loss.item())) and float(loss.item()) > 0
+            steps = ra.U32()
+            ra.check(L.rgi_nrc_steps(ctypes.c_void_p(cache), ctypes.byref(steps)))
+            assert steps.value == frames
+            ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(out)))
+            hdr = _half_plane(out.hdr, n)
+            assert np.all(np.isfinite(hdr))
+            means.append(hdr[:, :3].mean())
+        print("batch / alone resolved means:", ["%.5f" % m for m in means], flush=True)
+        ref = float(np.mean(means[3:]))
+        for m in means:
+            assert abs(m - ref) < 0.05 * ref
+    finally:
+        if batch:
+            L.rgi_batch_destroy(batch)
+        for r in batched + alone:
+            L.rgi_renderer_destroy(r)
+        L.rgi_destroy(ctx)
