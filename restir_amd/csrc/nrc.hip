@@ -305,40 +305,6 @@ struct TrainArgs {
     uint32_t* act_max;
 };
 
-// Adds v0 and v1 to level[2 idx] and level[2 idx + 1] for the lanes with
-// `on`. A lane's 16-lane group works on one level; where `merge` (a dense
-// coarse level, whose few cells every wave's samples share) the group's lanes
-// with equal indices first sum their values, so that each address takes one
-// atomic per wave instead of one per sample. Every lane of the wave calls it.
-RGI_DEV void scatter_pair(float* level, uint32_t idx, float v0, float v1, bool on, bool merge)
-{
-    const uint32_t lane = threadIdx.x & 63u, g16 = lane & ~15u;
-    if (on && !merge) {
-        if (v0 != 0.0f) unsafeAtomicAdd(level + 2 * idx, v0);
-        if (v1 != 0.0f) unsafeAtomicAdd(level + 2 * idx + 1, v1);
-    }
-    uint64_t pending = __ballot(on && merge);
-    // Each pass retires at least the leader of every group with lanes left,
-    // so the loop ends after at most 16 passes.
-    while (pending != 0ull) {
-        const uint32_t mine = (uint32_t)(pending >> g16) & 0xffffu;
-        const uint32_t leader = g16 + (mine != 0u ? (uint32_t)__builtin_ctz(mine) : 0u);
-        const uint32_t lead_idx = (uint32_t)__shfl((int)idx, (int)leader);
-        const bool peer = ((pending >> lane) & 1ull) != 0ull && idx == lead_idx;
-        float s0 = peer ? v0 : 0.0f, s1 = peer ? v1 : 0.0f;
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
-            s0 += __shfl_xor(s0, o, 16);
-            s1 += __shfl_xor(s1, o, 16);
-        }
-        if (lane == leader && mine != 0u) {
-            if (s0 != 0.0f) unsafeAtomicAdd(level + 2 * lead_idx, s0);
-            if (s1 != 0.0f) unsafeAtomicAdd(level + 2 * lead_idx + 1, s1);
-        }
-        pending &= ~__ballot(peer);
-    }
-}
-
 // Forward, loss and backward of 16 * kTrainB samples per wave: stores the
 // activations and back-propagated gradients for the weight-gradient GEMMs
 // and scatters the input gradient into the table gradient.
@@ -528,7 +494,7 @@ RGI_DEV void nrc_train_body(const TrainArgs& a, const uint32_t tile)
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             float d0 = dx[m >> 1][2 * (m & 1)], d1 = dx[m >> 1][2 * (m & 1) + 1];
-            const bool on = d0 != 0.0f || d1 != 0.0f;
+            if (d0 == 0.0f && d1 == 0.0f) continue;
             uint32_t c[3];
             float w[3];
             level_cell(L, m, pts[b].p, c, w);
@@ -538,8 +504,8 @@ RGI_DEV void nrc_train_body(const TrainArgs& a, const uint32_t tile)
                 uint32_t idx;
                 float wt;
                 corner_of(L.stride[m], a.gp.mask, c, w, k, idx, wt);
-                scatter_pair(level, idx, d0 != 0.0f ? wt * d0 : 0.0f, d1 != 0.0f ? wt * d1 : 0.0f, on,
-                             L.stride[m] != 0u);
+                if (d0 != 0.0f) unsafeAtomicAdd(level + 2 * idx, wt * d0);
+                if (d1 != 0.0f) unsafeAtomicAdd(level + 2 * idx + 1, wt * d1);
             }
         }
     }
