@@ -48,6 +48,8 @@ struct rgi_renderer {
     // fork event and the events that end its two parts.
     hipStream_t side = nullptr;
     hipEvent_t side_fork = nullptr, side_trace = nullptr, side_join = nullptr;
+    hipStream_t grid_side = nullptr; // the hash grid's stream, beside the side stream
+
     // The cache's training step runs on its own stream (default priority):
     // `nrc_pending` marks a step that the caller's stream has not joined yet.
     hipStream_t nrc_stream = nullptr;
@@ -142,6 +144,8 @@ void free_all(rgi_renderer* r)
     if (r->side_trace) hipEventDestroy(r->side_trace);
     if (r->side_join) hipEventDestroy(r->side_join);
     if (r->side) hipStreamDestroy(r->side);
+    if (r->grid_side) hipStreamDestroy(r->grid_side);
+    r->grid_side = nullptr;
     r->side_fork = r->side_trace = r->side_join = nullptr;
     r->side = nullptr;
 }
@@ -336,6 +340,7 @@ int rgi_renderer_create(rgi_renderer** out, int32_t width, int32_t height, const
         ok = ok && alloc(r->atlas, 6 * F * F);
     }
     ok = ok && hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking) == hipSuccess &&
+         hipStreamCreateWithFlags(&r->grid_side, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&r->side_fork, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&r->side_trace, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&r->side_join, hipEventDisableTiming) == hipSuccess &&
@@ -672,7 +677,7 @@ static int frame_tail(rgi_renderer* r, const FrameJob& j, const rgi_view* view, 
 }
 
 // One frame. `fork_grid` traces GI temporal's candidates and builds the hash
-// grid on the renderer's side stream and trains the cache on the cache's
+// grid on the renderer's side streams and trains the cache on the cache's
 // stream; `cross_frame` leaves that training running past the end of the
 // frame, joined where the next frame first touches the cache (eager frames
 // only: a captured frame joins every branch before its capture ends).
@@ -688,21 +693,24 @@ static int render_frame(rgi_renderer* r, const rgi_view* view, const rgi_gbuffer
     // writes the sky tiles' empty reservoirs into this frame's GI planes,
     // which nothing reads before GI temporal), so with `fork_grid` it runs
     // on the side stream beside the DI passes, joined before GI temporal's
-    // resampling. This frame's hash grid follows it there: only GI spatial
-    // and the specular pass read the grid (GI temporal reads last frame's),
-    // so it is joined before GI spatial. The buffers both write were last
-    // read by last frame's passes, which precede the fork on `stream`.
-    hipStream_t grid_stream = fork_grid ? r->side : stream;
+    // resampling. This frame's hash grid is built on a third stream from the
+    // same fork (beside both, not after the tracing, where it would slow the
+    // resampling): only GI spatial and the specular pass read the grid (GI
+    // temporal reads last frame's), so it is joined before GI spatial. The
+    // buffers both write were last read by last frame's passes, which
+    // precede the fork on `stream`.
+    hipStream_t grid_stream = fork_grid ? r->grid_side : stream;
     if (fork_grid) {
         hipEventRecord(r->side_fork, stream);
         hipStreamWaitEvent(r->side, r->side_fork, 0);
+        hipStreamWaitEvent(r->grid_side, r->side_fork, 0);
         if (j.split) {
             e = gi_temporal_trace(r->ctx, j.f, r->scene, *g, j.gi_out, r->side);
             hipEventRecord(r->side_trace, r->side);
         }
     }
     if (e == RGI_OK) e = frame_grid(r, j, *g, grid_stream);
-    if (fork_grid) hipEventRecord(r->side_join, r->side);
+    if (fork_grid) hipEventRecord(r->side_join, r->grid_side);
     // The DI passes, the cache's query and GI temporal, beside the grid.
     auto front = [&]() -> int {
         int fe;
