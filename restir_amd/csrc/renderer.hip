@@ -980,6 +980,10 @@ struct rgi_batch {
     // each reused once the copy out of it has run, and the device copy).
     hipStream_t side = nullptr;
     hipEvent_t side_fork = nullptr, traced = nullptr, gridded = nullptr, gathered = nullptr;
+    // The caches' stream: each frame's training chain, then the next frame's
+    // queries (which need it), beside both other streams.
+    hipStream_t cache_side = nullptr;
+    hipEvent_t queried = nullptr;
     char* staging[2] = {};
     hipEvent_t copied[2] = {};
     bool copy_pending[2] = {};
@@ -1010,6 +1014,8 @@ int rgi_batch_create(rgi_batch** out, rgi_renderer* const* renderers, uint32_t c
     b->table_bytes = table_layout(count).bytes;
     bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
               hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&b->cache_side, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&b->queried, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&b->side_fork, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&b->traced, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&b->gridded, hipEventDisableTiming) == hipSuccess &&
@@ -1044,6 +1050,8 @@ void rgi_batch_destroy(rgi_batch* b)
     for (hipEvent_t ev : b->joined) hipEventDestroy(ev);
     if (b->fork) hipEventDestroy(b->fork);
     if (b->side) hipStreamDestroy(b->side);
+    if (b->cache_side) hipStreamDestroy(b->cache_side);
+    if (b->queried) hipEventDestroy(b->queried);
     if (b->side_fork) hipEventDestroy(b->side_fork);
     if (b->traced) hipEventDestroy(b->traced);
     if (b->gridded) hipEventDestroy(b->gridded);
@@ -1271,20 +1279,25 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     b->copy_pending[slot] = true;
     b->slot = 1 - slot;
 
-    // The side stream, beside the DI passes on `stream`: the caches'
-    // queries (after their last training steps, which the previous frame
-    // left on this stream or a renderer's own frame on the cache's stream),
-    // GI temporal's tracing half, then the grids; joined before GI temporal's
-    // resampling and GI spatial.
+    // Beside the DI passes on `stream`: on the side stream GI temporal's
+    // tracing half, then the grids, joined before GI temporal's resampling
+    // and GI spatial; on the caches' stream the caches' queries, after their
+    // last training steps (which the previous frame left on that stream, or
+    // a renderer's own frame on the cache's stream), joined before the
+    // resampling.
     hipEventRecord(b->side_fork, stream);
     hipStreamWaitEvent(b->side, b->side_fork, 0);
     int se = RGI_OK;
+    bool any_query = false;
     for (uint32_t i = 0; i < n && se == RGI_OK; ++i) {
         rgi_renderer* r = b->renderers[i];
         if (!jobs[i].nrc_query) continue;
-        if (r->nrc_pending) hipStreamWaitEvent(b->side, r->nrc_done, 0);
-        se = nrc_query_planes(r, jobs[i], gbuffers[i], b->side);
+        if (!any_query) hipStreamWaitEvent(b->cache_side, b->side_fork, 0);
+        any_query = true;
+        if (r->nrc_pending) hipStreamWaitEvent(b->cache_side, r->nrc_done, 0);
+        se = nrc_query_planes(r, jobs[i], gbuffers[i], b->cache_side);
     }
+    if (any_query) hipEventRecord(b->queried, b->cache_side);
     if (se == RGI_OK) se = gi_trace_batch(L, b->side);
     hipEventRecord(b->traced, b->side);
     if (se == RGI_OK) se = grid_clear_batch(L.views, L.first[kPassGridClear], n, L.blocks[kPassGridClear], b->side);
@@ -1298,7 +1311,8 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     if (any_di) e = di_temporal_batch(L.views, L.first[kPassDi], n, L.blocks[kPassDi], stream);
     if (any_di && e == RGI_OK) e = di_spatial_batch(L.views, L.first[kPassDi], n, L.blocks[kPassDi], stream);
     hipStreamWaitEvent(stream, b->traced, 0);
-    // The side stream's queries followed every pending training step.
+    if (any_query) hipStreamWaitEvent(stream, b->queried, 0);
+    // The caches' queries followed every pending training step.
     for (uint32_t i = 0; i < n; ++i)
         if (jobs[i].nrc_query) b->renderers[i]->nrc_pending = false;
     if (e == RGI_OK) e = gi_resample_batch(L, stream);
@@ -1312,22 +1326,22 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     if (e == RGI_OK) e = se;
     if (e == RGI_OK) e = gi_spatial_batch(L, stream);
     // The rest of each view's frame. The caches' steps train in one chain of
-    // launches on the side stream, past the end of the call (as
+    // launches on the caches' stream, past the end of the call (as
     // rgi_renderer_render leaves a training step): beside the composites and
-    // the next frame's DI passes, ahead of its queries.
+    // the next frame's DI passes and tracing, ahead of its queries.
     std::vector<const uint16_t*> specular(n, nullptr);
     for (uint32_t i = 0; i < n && e == RGI_OK; ++i) e = frame_post(b->renderers[i], jobs[i], gbuffers[i], stream, &specular[i]);
     for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
         if (jobs[i].nrc_on) e = frame_gather(b->renderers[i], jobs[i], gbuffers[i], stream);
     if (any_train && e == RGI_OK) {
         hipEventRecord(b->gathered, stream);
-        hipStreamWaitEvent(b->side, b->gathered, 0);
+        hipStreamWaitEvent(b->cache_side, b->gathered, 0);
         e = nrc_train_batch(b->table + T.steps, reinterpret_cast<const uint32_t*>(b->table + T.nrc_first), n,
-                            nrc_blocks, b->side);
+                            nrc_blocks, b->cache_side);
         for (uint32_t i = 0; i < n; ++i) {
             rgi_renderer* r = b->renderers[i];
             if (!jobs[i].nrc_on) continue;
-            hipEventRecord(r->nrc_done, b->side);
+            hipEventRecord(r->nrc_done, b->cache_side);
             r->nrc_pending = true;
             if (e == RGI_OK) {
                 nrc_step_taken(r->nrc);
