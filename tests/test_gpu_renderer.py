@@ -376,3 +376,123 @@ def test_batch_matches_single_renderers(torch, mode):
             L.rgi_batch_destroy(batch)
         for r in batch_r + single_r:
             L.rgi_renderer_destroy(r)
+
+
+def _batch_views(torch, cases, settings, i, yaw, keep):
+    views, gbs = [], []
+    for (hs, w, h, div), s in zip(cases, settings):
+        eye, vp, inv = ra.orbit_camera(hs, i, w, h, yaw_step=yaw)
+        f = common.frame_for(hs, s, w, h, i, i > 0, yaw_step=yaw)
+        g = common.oracle_gbuffer(hs, f)
+        planes = {k: upload(torch, v) for k, v in g.items()}
+        keep.append(planes)
+        gbs.append(ra.GBuffer(w, h, planes["depth"].data_ptr(), planes["albedo"].data_ptr(),
+                              planes["normal"].data_ptr(), planes["shading"].data_ptr(), planes["emission"].data_ptr()))
+        views.append(ra.make_view(eye, vp, inv))
+    return views, gbs
+
+
+def _batch_and_single(torch, L, cases, settings, keep):
+    batch_r, single_r = [], []
+    for (hs, w, h, div), s in zip(cases, settings):
+        bvh, lights = upload(torch, hs.bvh), upload(torch, hs.lights)
+        keep += [bvh, lights]
+        scene = hs.rgi_scene(bvh.data_ptr(), lights.data_ptr())
+        for lst in (batch_r, single_r):
+            r = ctypes.c_void_p()
+            ra.check(L.rgi_renderer_create(ctypes.byref(r), w, h, ctypes.byref(ra.c_settings(s)),
+                                           ctypes.byref(scene), hs.radius))
+            lst.append(r)
+    return batch_r, single_r
+
+
+def test_batch_mixes_gi_sizes(torch):
+    """A pass-major batch whose views have GI at the window size and at the
+    reference's half and quarter size (both forms of every batched GI kernel
+    in one frame, the box field's DI through the light guide), against each
+    renderer alone: bit-identical outputs on every frame."""
+    say("case batch with mixed GI sizes")
+    L = ra.lib()
+    cases = [(common.cornell(extra_lights=4), 96, 64, 1),
+             (ra.HostScene(ra.RGI_SCENE_BOXFIELD, seed=3, size=60, extra_lights=300), 80, 48, 2),
+             (atrium(), 128, 96, 4),
+             (common.cornell(extra_lights=4), 64, 64, 2)]
+    keep = []
+    settings = [ra.Settings(grid_mode=ra.RGI_GRID_DETERMINISTIC, gi_divisor=div) for _, _, _, div in cases]
+    es = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    side = torch.cuda.Stream()
+    bs = ctypes.c_void_p(side.cuda_stream)
+    batch = ctypes.c_void_p()
+    batch_r, single_r = [], []
+    try:
+        batch_r, single_r = _batch_and_single(torch, L, cases, settings, keep)
+        arr = (ctypes.c_void_p * len(batch_r))(*[r.value for r in batch_r])
+        ra.check(L.rgi_batch_create(ctypes.byref(batch), arr, len(batch_r)))
+        yaw = 2 * np.pi / 64
+        for i in range(3):
+            views, gbs = _batch_views(torch, cases, settings, i, yaw, keep)
+            torch.cuda.synchronize()
+            for r, v, gb in zip(single_r, views, gbs):
+                ra.check(L.rgi_renderer_render(r, ctypes.byref(v), ctypes.byref(gb), es))
+            ra.check(L.rgi_batch_render(batch, (ra.View * len(views))(*views), (ra.GBuffer * len(gbs))(*gbs), bs))
+            torch.cuda.synchronize()
+            for k, ((hs, w, h, div), rb, rsg) in enumerate(zip(cases, batch_r, single_r)):
+                ob, osg = ra.RendererOutputs(), ra.RendererOutputs()
+                ra.check(L.rgi_renderer_outputs_get(rb, ctypes.byref(ob)))
+                ra.check(L.rgi_renderer_outputs_get(rsg, ctypes.byref(osg)))
+                gw, gh = max(w // div, 1), max(h // div, 1)
+                for name, n, dt in (("gi_sample", gw * gh, np.float32), ("color", gw * gh, np.uint16),
+                                    ("di_diffuse", w * h, np.uint16), ("hdr", w * h, np.uint16),
+                                    ("denoised", gw * gh, np.uint16)):
+                    like = np.zeros((n, 4), dt)
+                    a = download(torch, getattr(ob, name), like)
+                    b = download(torch, getattr(osg, name), like)
+                    assert np.array_equal(a, b), "frame %d view %d (divisor %d): %s differs" % (i, k, div, name)
+    finally:
+        if batch:
+            L.rgi_batch_destroy(batch)
+        for r in batch_r + single_r:
+            L.rgi_renderer_destroy(r)
+
+
+def test_batch_atomic_grids(torch):
+    """The batched atomic insert (one launch for every view's grid) against
+    each renderer's own atomic insert on the same G-buffers, first frame:
+    the same cells with the same checksums and counts, and each cell of at
+    most 32 pixels lists the same pixels (the order of the atomics places
+    them in any order, and decides which pixels a fuller cell keeps)."""
+    say("case batch atomic grids")
+    L = ra.lib()
+    cases = [(common.cornell(extra_lights=4), 96, 64, 1), (atrium(), 128, 96, 2), (common.cornell(), 64, 48, 4)]
+    settings = [ra.Settings(grid_mode=ra.RGI_GRID_ATOMIC, gi_divisor=div) for _, _, _, div in cases]
+    keep = []
+    es = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    batch = ctypes.c_void_p()
+    batch_r, single_r = [], []
+    try:
+        batch_r, single_r = _batch_and_single(torch, L, cases, settings, keep)
+        arr = (ctypes.c_void_p * len(batch_r))(*[r.value for r in batch_r])
+        ra.check(L.rgi_batch_create(ctypes.byref(batch), arr, len(batch_r)))
+        views, gbs = _batch_views(torch, cases, settings, 0, 0.0, keep)
+        torch.cuda.synchronize()
+        for r, v, gb in zip(single_r, views, gbs):
+            ra.check(L.rgi_renderer_render(r, ctypes.byref(v), ctypes.byref(gb), es))
+        ra.check(L.rgi_batch_render(batch, (ra.View * len(views))(*views), (ra.GBuffer * len(gbs))(*gbs), es))
+        torch.cuda.synchronize()
+        T = settings[0].grid_table_size
+        for k, (rb, rsg) in enumerate(zip(batch_r, single_r)):
+            cells = []
+            for r in (rb, rsg):
+                o = ra.RendererOutputs()
+                ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(o)))
+                cs, cnt, entries = common.grid_view(download(torch, o.grid, np.zeros(34 * T, np.uint32)), T)
+                occ = np.nonzero(cs)[0]
+                cells.append({int(cs[s_]): (int(cnt[s_]), tuple(sorted(entries[s_, :cnt[s_]])) if cnt[s_] <= 32 else None)
+                              for s_ in occ})
+            assert cells[0] == cells[1], "view %d: the batched atomic insert's cells differ" % k
+            assert len(cells[0]) > 0
+    finally:
+        if batch:
+            L.rgi_batch_destroy(batch)
+        for r in batch_r + single_r:
+            L.rgi_renderer_destroy(r)
