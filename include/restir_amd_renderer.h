@@ -137,13 +137,17 @@ int rgi_renderer_render_graph(rgi_renderer* renderer, const rgi_view* view, cons
  * and spatial, the hash grids' clear and insert, GI temporal's tile list,
  * candidate rays, BVH walks and resampling, GI spatial) runs once for every
  * view, its kernels reading each view's frame, scene and planes from a
- * device table that the call uploads from pinned host memory; the passes
- * with per-renderer state (the cache's query and training, indirect
- * specular, denoiser, composite) run per view. GI temporal's tracing and the
- * grids run on the batch's side stream, forked from and joined back to
- * `stream` (which may be the null stream). Every view is checked before the
- * first launch: a view that fails its checks leaves every renderer as it
- * was.
+ * device table that the call uploads from pinned host memory; the caches'
+ * training steps run as one chain of launches for all of them (as
+ * rgi_nrc_train_batch), and the passes with per-renderer state (the cache's
+ * query and gather, indirect specular, denoiser, composite) run per view.
+ * GI temporal's tracing and the grids run on the batch's side stream, the
+ * caches' queries and training on a third stream, forked from and joined
+ * back to `stream` (which may be the null stream); as with
+ * rgi_renderer_render, a training step may still run when the call returns,
+ * and the renderer's next use of its cache waits for it. Every view is
+ * checked before the first launch: a view that fails its checks leaves
+ * every renderer as it was.
  *
  * RGI_BATCH_GRAPH: one HIP graph per call holds every renderer's frame, each
  * on a stream of its own forked from and joined to `stream` (not the null
