@@ -138,9 +138,10 @@ int rgi_renderer_render_graph(rgi_renderer* renderer, const rgi_view* view, cons
  * candidate rays, BVH walks and resampling, GI spatial) runs once for every
  * view, its kernels reading each view's frame, scene and planes from a
  * device table that the call uploads from pinned host memory; the caches'
- * training steps run as one chain of launches for all of them (as
- * rgi_nrc_train_batch), and the passes with per-renderer state (the cache's
- * query and gather, indirect specular, denoiser, composite) run per view.
+ * queries run as one launch per precision and their training steps as one
+ * chain of launches for all of them (as rgi_nrc_train_batch), and the passes
+ * with per-renderer state (the cache's gather, indirect specular, denoiser,
+ * composite) run per view.
  * GI temporal's tracing and the grids run on the batch's side stream, the
  * caches' queries and training on a third stream, forked from and joined
  * back to `stream` (which may be the null stream); as with

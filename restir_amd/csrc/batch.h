@@ -48,6 +48,8 @@ struct BatchView {
 // A cache's training step in a batch's table (nrc.hip fills it), and the
 // kernels of the training chain that give each cache a slice of a launch.
 constexpr size_t kNrcStepBytes = 512;
+// A cache's query of one G-buffer plane in a batch's table (nrc.hip fills it).
+constexpr size_t kNrcQueryBytes = 2048;
 enum NrcPass {
     kNrcTrain = 0, kNrcWgrad, kNrcReduce, kNrcAdamMlp, kNrcCopies, kNrcAdamTable, kNrcEmaMlp, kNrcEmaCopies,
     kNrcEmaTable, kNrcPasses

@@ -148,6 +148,17 @@ int nrc_step_plan(rgi_nrc* n, const float* records, const float* targets, uint32
 int nrc_train_batch(const void* steps, const uint32_t* first, uint32_t count, const uint32_t* blocks,
                     hipStream_t stream);
 void nrc_step_taken(rgi_nrc* n);
+// A batch of caches' G-buffer queries: nrc_query_plan writes the query of
+// nrc_infer_gbuffer's arguments into a kNrcQueryBytes slot (nothing is
+// launched), with its workgroups in *blocks; nrc_query_batch runs a device
+// table of them, one launch per precision present (precisions[p] for
+// RGI_NRC_BF16..FP8), each query on its own slice (first: count + 1 slice
+// starts; blocks: their total), with results identical to
+// nrc_infer_gbuffer's.
+int nrc_query_plan(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* emission, int precision,
+                   uint16_t* out, void* slot, uint32_t* blocks);
+int nrc_query_batch(const void* queries, const uint32_t* first, uint32_t count, uint32_t blocks,
+                    const bool* precisions, hipStream_t stream);
 
 // The neural radiance cache's frame-driver passes (nrc.hip): the cache's
 // radiance at every G-buffer pixel, and one frame's training records.

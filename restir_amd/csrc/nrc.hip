@@ -1145,18 +1145,18 @@ RGI_DEV bool gbuffer_point(const Grid& gp, const rgi_frame& f, const rgi_gbuffer
 // cache's radiance, clamped to >= 0, plus the G-buffer emission when
 // `emission` is set, as RGBA16F (alpha 1); sky pixels get 0.
 template <int P>
-__global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, rgi_frame f, rgi_gbuffer gb,
-                                                           F2 texel, const uint16_t* emission, uint16_t* out)
+RGI_DEV void infer_gbuffer_body(const rgi_nrc& n, const Grid& gp, const rgi_frame& f, const rgi_gbuffer& gb,
+                                F2 texel, const uint16_t* emission, uint16_t* out, const uint32_t bx, const uint32_t by)
 {
     constexpr int B = kInferB;
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u), g = lane >> 4, s16 = lane & 15;
     const LaneLevels L = lane_levels(gp, g);
-    const int x = (int)(blockIdx.x * 16u) + (wave & 1) * 8 + (s16 & 7);
+    const int x = (int)(bx * 16u) + (wave & 1) * 8 + (s16 & 7);
     Point pts[B];
     bool surface[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        int y = (int)(blockIdx.y * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
+        int y = (int)(by * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
         surface[b] = gbuffer_point(gp, f, gb, texel, min(x, gb.width - 1), min(y, gb.height - 1), pts[b]);
     }
     // A wave whose pixels are all sky writes zeros and leaves: the forward
@@ -1168,7 +1168,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, r
         if (g != 0 || x >= gb.width) return;
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            int y = (int)(blockIdx.y * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
+            int y = (int)(by * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
             if (y < gb.height) store_h4(out, (uint32_t)y * (uint32_t)gb.width + (uint32_t)x, F4{0.0f, 0.0f, 0.0f, 0.0f});
         }
         return;
@@ -1178,7 +1178,7 @@ __global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, r
     if (g != 0 || x >= gb.width) return;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        int y = (int)(blockIdx.y * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
+        int y = (int)(by * (4u * B)) + (wave >> 1) * (2 * B) + 2 * b + (s16 >> 3);
         if (y >= gb.height) continue;
         uint32_t o = (uint32_t)y * (uint32_t)gb.width + (uint32_t)x;
         F4 r = F4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -1191,6 +1191,40 @@ __global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, r
         }
         store_h4(out, o, r);
     }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) k_nrc_infer_gbuffer(rgi_nrc n, Grid gp, rgi_frame f, rgi_gbuffer gb,
+                                                           F2 texel, const uint16_t* emission, uint16_t* out)
+{
+    infer_gbuffer_body<P>(n, gp, f, gb, texel, emission, out, blockIdx.x, blockIdx.y);
+}
+
+// One query of a batch of G-buffer queries (each its own cache, frame,
+// G-buffer and output plane): the batched launch gives query q the
+// workgroups [first[q], first[q + 1]), which cover its grid of bx columns.
+struct NrcQuery {
+    rgi_nrc n;       // the cache's inference view (its moving average's copies)
+    Grid gp;
+    rgi_frame f;
+    rgi_gbuffer gb;
+    F2 texel;
+    const uint16_t* emission;
+    uint16_t* out;
+    uint32_t bx;
+    int precision;
+};
+static_assert(sizeof(NrcQuery) <= kNrcQueryBytes, "a query must fit its slot of a batch's table");
+
+template <int P>
+__global__ void __launch_bounds__(256) k_nrc_infer_gbuffer_batch(const uint8_t* __restrict__ queries,
+                                                                 const uint32_t* __restrict__ first, uint32_t count)
+{
+    uint32_t b;
+    const NrcQuery& q =
+        *reinterpret_cast<const NrcQuery*>(queries + (size_t)batch_view(first, count, blockIdx.x, b) * kNrcQueryBytes);
+    if (q.precision != P) return; // the whole workgroup: the pass has no barrier
+    infer_gbuffer_body<P>(q.n, q.gp, q.f, q.gb, q.texel, q.emission, q.out, b % q.bx, b / q.bx);
 }
 
 // The surface pixels of a G-buffer, in row-major order, as a list: a
@@ -1361,6 +1395,40 @@ int nrc_infer_gbuffer(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, cons
     default: return fail(RGI_E_INVALID, "nrc: unknown precision %d", precision);
     }
     return check_launch("nrc infer (G-buffer)");
+}
+
+int nrc_query_plan(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* emission, int precision,
+                   uint16_t* out, void* slot, uint32_t* blocks)
+{
+    if (precision < RGI_NRC_BF16 || precision > RGI_NRC_FP8) return fail(RGI_E_INVALID, "nrc: unknown precision %d", precision);
+    nrc::NrcQuery q;
+    memset(&q, 0, sizeof(q));
+    q.n = inference_view(*n);
+    q.gp = grid_of(*n);
+    q.f = f;
+    q.gb = g;
+    q.texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    q.emission = emission;
+    q.out = out;
+    q.bx = (uint32_t)(g.width + 15) / 16u;
+    q.precision = precision;
+    memcpy(slot, &q, sizeof(q));
+    *blocks = q.bx * ((uint32_t)(g.height + 4 * nrc::kInferB - 1) / (4u * nrc::kInferB));
+    return RGI_OK;
+}
+
+int nrc_query_batch(const void* queries, const uint32_t* first, uint32_t count, uint32_t blocks,
+                    const bool* precisions, hipStream_t stream)
+{
+    const uint8_t* q = static_cast<const uint8_t*>(queries);
+    if (blocks == 0u) return RGI_OK;
+    if (precisions[RGI_NRC_BF16])
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer_batch<RGI_NRC_BF16>, dim3(blocks), dim3(256), 0, stream, q, first, count);
+    if (precisions[RGI_NRC_FP16])
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer_batch<RGI_NRC_FP16>, dim3(blocks), dim3(256), 0, stream, q, first, count);
+    if (precisions[RGI_NRC_FP8])
+        hipLaunchKernelGGL(nrc::k_nrc_infer_gbuffer_batch<RGI_NRC_FP8>, dim3(blocks), dim3(256), 0, stream, q, first, count);
+    return check_launch("nrc infer (G-buffer, batch)");
 }
 
 int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,

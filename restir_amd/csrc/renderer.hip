@@ -956,10 +956,11 @@ int rgi_renderer_last_frame(const rgi_renderer* r, rgi_frame* out)
 // renderer's frame on a stream of its own.
 namespace {
 // The view table of a batch of n views: the views, each pass's slice
-// starts, each cache's training step, and the training kernels' slice
-// starts.
+// starts, each cache's training step, the training kernels' slice starts,
+// each view's two cache queries (the query plane, and the resolved plane
+// when the query adds emission) and their slice starts.
 struct TableLayout {
-    size_t first, steps, nrc_first, bytes;
+    size_t first, steps, nrc_first, queries, query_first, bytes;
 };
 TableLayout table_layout(size_t n)
 {
@@ -967,7 +968,9 @@ TableLayout table_layout(size_t n)
     t.first = align256(n * sizeof(BatchView));
     t.steps = align256(t.first + (size_t)kBatchPasses * (n + 1) * sizeof(uint32_t));
     t.nrc_first = t.steps + n * kNrcStepBytes;
-    t.bytes = t.nrc_first + (size_t)kNrcPasses * (n + 1) * sizeof(uint32_t);
+    t.queries = align256(t.nrc_first + (size_t)kNrcPasses * (n + 1) * sizeof(uint32_t));
+    t.query_first = t.queries + 2 * n * kNrcQueryBytes;
+    t.bytes = t.query_first + (2 * n + 1) * sizeof(uint32_t);
     return t;
 }
 } // namespace
@@ -1266,6 +1269,26 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
         for (int p = 0; p < kNrcPasses; ++p) nfirst[p * (n + 1) + i + 1] = nfirst[p * (n + 1) + i] + blocks[p];
     }
     for (int p = 0; p < kNrcPasses; ++p) nrc_blocks[p] = nfirst[p * (n + 1) + n];
+    // The caches' queries: the query plane of every cache that has trained,
+    // and its resolved plane where the query adds emission.
+    uint32_t* qfirst = reinterpret_cast<uint32_t*>(host + T.query_first);
+    bool precisions[3] = {false, false, false};
+    bool any_query = false;
+    for (uint32_t q = 0; q < 2 * n && e == RGI_OK; ++q) {
+        const uint32_t i = q / 2;
+        rgi_renderer* r = b->renderers[i];
+        const FrameJob& j = jobs[i];
+        const bool resolved = (q & 1u) != 0u;
+        uint32_t blocks = 0;
+        if (j.nrc_query && (!resolved || (j.query_emission && j.resolved_diffuse))) {
+            e = nrc_query_plan(r->nrc, j.f, gbuffers[i], resolved ? nullptr : j.query_emission,
+                               r->nrc_settings.precision, resolved ? r->nrc_resolved : r->nrc_plane,
+                               host + T.queries + q * kNrcQueryBytes, &blocks);
+            precisions[r->nrc_settings.precision] = true;
+            any_query = true;
+        }
+        qfirst[q + 1] = qfirst[q] + blocks;
+    }
     if (e != RGI_OK) {
         restore_all();
         return e;
@@ -1288,16 +1311,15 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     hipEventRecord(b->side_fork, stream);
     hipStreamWaitEvent(b->side, b->side_fork, 0);
     int se = RGI_OK;
-    bool any_query = false;
-    for (uint32_t i = 0; i < n && se == RGI_OK; ++i) {
-        rgi_renderer* r = b->renderers[i];
-        if (!jobs[i].nrc_query) continue;
-        if (!any_query) hipStreamWaitEvent(b->cache_side, b->side_fork, 0);
-        any_query = true;
-        if (r->nrc_pending) hipStreamWaitEvent(b->cache_side, r->nrc_done, 0);
-        se = nrc_query_planes(r, jobs[i], gbuffers[i], b->cache_side);
+    if (any_query) {
+        hipStreamWaitEvent(b->cache_side, b->side_fork, 0);
+        for (uint32_t i = 0; i < n; ++i)
+            if (jobs[i].nrc_query && b->renderers[i]->nrc_pending)
+                hipStreamWaitEvent(b->cache_side, b->renderers[i]->nrc_done, 0);
+        se = nrc_query_batch(b->table + T.queries, reinterpret_cast<const uint32_t*>(b->table + T.query_first), 2 * n,
+                             qfirst[2 * n], precisions, b->cache_side);
+        hipEventRecord(b->queried, b->cache_side);
     }
-    if (any_query) hipEventRecord(b->queried, b->cache_side);
     if (se == RGI_OK) se = gi_trace_batch(L, b->side);
     hipEventRecord(b->traced, b->side);
     if (se == RGI_OK) se = grid_clear_batch(L.views, L.first[kPassGridClear], n, L.blocks[kPassGridClear], b->side);

@@ -711,3 +711,71 @@ def test_renderer_caches_in_pass_major_batch():
         for r in batched + alone:
             L.rgi_renderer_destroy(r)
         L.rgi_destroy(ctx)
+
+
+def test_batch_cache_queries_match_single_renderers():
+    """A pass-major batch queries all its caches in one launch per precision
+    (the query plane, and the resolved plane where the query adds emission):
+    with every cache given the same parameters after the warm-up frames (so
+    that the batch's and the single renderers' caches are equal), the first
+    frame that queries them is bit-identical between the batch and the
+    renderers alone: query planes, GI reservoirs and colour, composite."""
+    W, H = 160, 120
+    n = W * H
+    base = ra.Settings(grid_table_size=1 << 16, grid_mode=ra.RGI_GRID_DETERMINISTIC)
+    hs, gscene, ctx, gbuf, view = _setup_scene(W, H, base)
+    L = ra.lib()
+    from restir_amd import driver
+    s = driver.stream()
+    variants = [(base, rnrc.renderer_settings(hs, batch=4096, warmup=3, resolve=1)),
+                (ra.Settings(grid_table_size=1 << 16, grid_mode=ra.RGI_GRID_DETERMINISTIC, restir_di=False),
+                 rnrc.renderer_settings(hs, batch=4096, warmup=3, resolve=1)),
+                (base, rnrc.renderer_settings(hs, batch=2048, warmup=3, precision=ra.RGI_NRC_FP16))]
+    batched = [_renderer(hs, gscene, st, W, H, ns) for st, ns in variants]
+    alone = [_renderer(hs, gscene, st, W, H, ns) for st, ns in variants]
+    batch = ctypes.c_void_p()
+    try:
+        arr = (ctypes.c_void_p * 3)(*[r.value for r in batched])
+        ra.check(L.rgi_batch_create(ctypes.byref(batch), arr, 3))
+        views = (ra.View * 3)(view, view, view)
+        gbufs = (ra.GBuffer * 3)(gbuf.gb, gbuf.gb, gbuf.gb)
+
+        def frame():
+            ra.check(L.rgi_batch_render(batch, views, gbufs, s))
+            for r in alone:
+                ra.check(L.rgi_renderer_render(r, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+            torch.cuda.synchronize()
+
+        for _ in range(3):
+            frame()
+        probe = rnrc.NeuralCache(variants[0][1].cache)
+        rng = np.random.default_rng(9)
+        mlp = torch.from_numpy((rng.uniform(-1, 1, ra.RGI_NRC_MLP_PARAMS) * 0.15).astype(np.float32)).cuda()
+        table = torch.from_numpy((rng.uniform(-1, 1, probe.table_floats) * 0.05).astype(np.float32)).cuda()
+        probe.close()
+        for r in batched + alone:
+            cache = rnrc.renderer_state(r)[0]
+            ra.check(L.rgi_nrc_set_params(ctypes.c_void_p(cache), mlp.data_ptr(), table.data_ptr(), s))
+        torch.cuda.synchronize()
+        print("batch cache queries: the querying frame", flush=True)
+        frame()
+        for k, (rb, ra_) in enumerate(zip(batched, alone)):
+            planes = []
+            for r in (rb, ra_):
+                cache, rad, loss, trained = rnrc.renderer_state(r)
+                assert trained == 4
+                o = ra.RendererOutputs()
+                ra.check(L.rgi_renderer_outputs_get(r, ctypes.byref(o)))
+                planes.append({"query": _half_plane(rad, n), "color": _half_plane(o.color, n),
+                               "hdr": _half_plane(o.hdr, n)})
+            for name in ("query", "color", "hdr"):
+                a, b = planes[0][name], planes[1][name]
+                assert np.array_equal(a, b), "renderer %d: %s differs between the batch and the renderer alone" % (
+                    k, name)
+            assert np.any(planes[0]["query"][:, :3] != 0.0)
+    finally:
+        if batch:
+            L.rgi_batch_destroy(batch)
+        for r in batched + alone:
+            L.rgi_renderer_destroy(r)
+        L.rgi_destroy(ctx)
