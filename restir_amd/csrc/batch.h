@@ -43,6 +43,13 @@ struct BatchView {
     CandRays cand;
     uint32_t same;                  // the G-buffer has the GI size
     uint32_t split;                 // GI temporal in its listed form (list and cand set)
+    // The denoiser: the variance into den_out[0], a-trous iteration it from
+    // den_out[it] into den_out[it + 1] (planes of the GI size).
+    uint32_t atrous;                // iterations, 0 without the denoiser
+    uint16_t* den_out[6];
+    // The composite into hdr (the window size).
+    rgi_composite_inputs ci;
+    uint16_t* hdr;
 };
 
 // A cache's training step in a batch's table (nrc.hip fills it), and the
@@ -56,7 +63,7 @@ enum NrcPass {
 };
 
 // The batched screen passes, each with its own slices of a launch.
-enum BatchPass { kPassDi = 0, kPassGridClear, kPassGrid, kPassTiles, kPassGiSpatial, kBatchPasses };
+enum BatchPass { kPassDi = 0, kPassGridClear, kPassGrid, kPassTiles, kPassGiSpatial, kPassDenoise, kPassComposite, kBatchPasses };
 
 // Host side: a batch frame's device table and the sizes of its launches.
 struct BatchLaunch {

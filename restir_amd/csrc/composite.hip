@@ -5,6 +5,7 @@
 // emission; the sky where the depth is the far plane. One thread per pixel,
 // 16 x 16 pixels per block. A resolved diffuse plane, when given, replaces
 // the DI diffuse light and the re-modulated GI irradiance together.
+#include "batch.h"
 #include "internal.h"
 #include "shading.h"
 
@@ -57,11 +58,11 @@ RGI_DEV void upsample_indirect(const rgi_frame& f, const rgi_gbuffer& g, const r
     specular = weight_sum > 0.0f ? specular_sum / weight_sum : xyz(h4_or_zero(in.specular, own));
 }
 
-// fs_composite.sc:82-143.
-__global__ void __launch_bounds__(256) k_composite(rgi_frame f, rgi_gbuffer g, rgi_composite_inputs in, F2 texel,
-                                                   uint16_t* out)
+// fs_composite.sc:82-143, for the 16 x 16-pixel block (bx, by).
+RGI_DEV void composite_body(const rgi_frame& f, const rgi_gbuffer& g, const rgi_composite_inputs& in, F2 texel,
+                            uint16_t* out, uint32_t bx, uint32_t by)
 {
-    int x = (int)(blockIdx.x * 16u + (threadIdx.x & 15u)), y = (int)(blockIdx.y * 16u + (threadIdx.x >> 4));
+    int x = (int)(bx * 16u + (threadIdx.x & 15u)), y = (int)(by * 16u + (threadIdx.x >> 4));
     if (x >= g.width || y >= g.height) return;
     uint32_t o = (uint32_t)y * (uint32_t)g.width + (uint32_t)x;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -111,7 +112,30 @@ __global__ void __launch_bounds__(256) k_composite(rgi_frame f, rgi_gbuffer g, r
     store_h4(out, o, F4{color.x, color.y, color.z, 1.0f});
 }
 
+__global__ void __launch_bounds__(256) k_composite(rgi_frame f, rgi_gbuffer g, rgi_composite_inputs in, F2 texel,
+                                                   uint16_t* out)
+{
+    composite_body(f, g, in, texel, out, blockIdx.x, blockIdx.y);
+}
+
+// The batched form: workgroups [first[v], first[v + 1]) run view v's blocks.
+__global__ void __launch_bounds__(256) k_composite_batch(const BatchView* __restrict__ views,
+                                                         const uint32_t* __restrict__ first, uint32_t count)
+{
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    const uint32_t gx = ((uint32_t)v.g.width + 15u) / 16u;
+    composite_body(v.f, v.g, v.ci, v.texel, v.hdr, b % gx, b / gx);
+}
+
 } // namespace
+
+int composite_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, hipStream_t stream)
+{
+    if (blocks == 0u) return RGI_OK;
+    hipLaunchKernelGGL(k_composite_batch, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    return check_launch("composite (batch)");
+}
 
 int composite(const rgi_frame& f, const rgi_gbuffer& g, const rgi_composite_inputs& in, uint16_t* out,
               hipStream_t stream)

@@ -135,6 +135,12 @@ int di_spatial_batch(const BatchView* views, const uint32_t* first, uint32_t cou
 int gi_trace_batch(const BatchLaunch& b, hipStream_t stream);
 int gi_resample_batch(const BatchLaunch& b, hipStream_t stream);
 int gi_spatial_batch(const BatchLaunch& b, hipStream_t stream);
+// The denoiser (variance, then `iterations` a-trous steps) and the composite
+// over every view of a batch; svgf_blocks is a view's denoiser grid.
+uint32_t svgf_blocks(const rgi_frame& f);
+int svgf_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, int iterations,
+               hipStream_t stream);
+int composite_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, hipStream_t stream);
 // A batch of caches' training steps: nrc_step_plan writes the next step of
 // cache n on `count` records into a kNrcStepBytes slot (growing the cache's
 // scratch; nothing is launched) with each kernel's workgroups in blocks[];

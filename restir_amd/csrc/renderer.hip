@@ -546,11 +546,10 @@ static int frame_nrc_query(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer
 
 // After GI spatial: indirect specular (its plane for the composite goes to
 // *specular_out, NULL without it) and the denoiser.
-static int frame_post(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream,
-                      const uint16_t** specular_out)
+static int frame_specular(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream,
+                          const uint16_t** specular_out)
 {
     const rgi_settings& s = j.s;
-    const rgi_frame& f = j.f;
     const int read = j.read, write = j.write;
     int e;
     // Indirect specular (renderer.cpp:836-855): last frame's composite is
@@ -568,10 +567,34 @@ static int frame_post(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, 
         specular = r->specular_gi[sw];
         *specular_out = specular;
     }
+    return RGI_OK;
+}
 
-    // Denoiser (renderer.cpp:857-907): variance into filter target 0;
-    // iteration 0 writes the filtered history, iteration 1 target 1, then
-    // 1 -> 0, 0 -> 1, 1 -> 0.
+// The indirect specular plane frame_specular will write for this frame
+// (NULL without the pass).
+static const uint16_t* planned_specular(const rgi_renderer* r, const FrameJob& j)
+{
+    return j.s.specular_gi && r->specular_gi[0] ? r->specular_gi[1 - r->specular_read] : nullptr;
+}
+
+// The denoiser's planes (renderer.cpp:857-907): the variance into filter
+// target 0; iteration 0 writes the filtered history, iteration 1 target 1,
+// then 1 -> 0, 0 -> 1, 1 -> 0. out[0] is the variance's output and out[it + 1]
+// iteration it's; the denoised plane is out[iterations].
+static void denoiser_planes(rgi_renderer* r, const FrameJob& j, uint16_t* out[6])
+{
+    out[0] = r->filter[0];
+    for (int it = 0; it < 5; ++it) out[it + 1] = it == 0 ? r->filtered[j.write] : r->filter[it & 1];
+}
+
+static int frame_post(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g, hipStream_t stream,
+                      const uint16_t** specular_out)
+{
+    const rgi_settings& s = j.s;
+    const rgi_frame& f = j.f;
+    const int write = j.write;
+    int e = frame_specular(r, j, g, stream, specular_out);
+    if (e != RGI_OK) return e;
     r->denoised = r->color[write];
     if (s.denoiser) {
         const float* receiver = r->gi[write][1];
@@ -611,35 +634,44 @@ static int frame_gather(rgi_renderer* r, const FrameJob& j, const rgi_gbuffer& g
 
 // The composite and the end of the frame; `cross_frame` leaves a training
 // step of the cache running past it.
-static int frame_finish(rgi_renderer* r, const FrameJob& j, const rgi_view* view, const rgi_gbuffer& g,
-                        const uint16_t* specular, hipStream_t stream, bool cross_frame)
+// The composite's inputs (renderer.cpp:967-990): the final image's weights.
+static rgi_composite_inputs composite_inputs(const rgi_renderer* r, const FrameJob& j, const uint16_t* denoised,
+                                             const uint16_t* specular)
 {
-    const rgi_settings& s = j.s;
-    const rgi_frame& f = j.f;
-    int e;
-    // Composite (renderer.cpp:967-990): the final image's weights.
-    if (s.composite && r->hdr) {
-        rgi_composite_inputs ci{};
-        ci.gi = r->denoised;
-        ci.di_diffuse = j.di_active ? r->diffuse : nullptr;
-        ci.di_specular = j.di_active ? r->specular : nullptr;
-        ci.specular = specular;
-        ci.shadow_map = j.shadow_map;
-        ci.resolved_diffuse = j.resolved_diffuse;
-        for (int k = 0; k < 4; ++k) ci.options[k] = 1.0f;
-        ci.specular_weight = specular ? 1.0f : 0.0f;
-        if ((e = composite(f, g, ci, r->hdr, stream)) != RGI_OK) return e;
-    }
+    rgi_composite_inputs ci{};
+    ci.gi = denoised;
+    ci.di_diffuse = j.di_active ? r->diffuse : nullptr;
+    ci.di_specular = j.di_active ? r->specular : nullptr;
+    ci.specular = specular;
+    ci.shadow_map = j.shadow_map;
+    ci.resolved_diffuse = j.resolved_diffuse;
+    for (int k = 0; k < 4; ++k) ci.options[k] = 1.0f;
+    ci.specular_weight = specular ? 1.0f : 0.0f;
+    return ci;
+}
 
-    // End of frame (renderer.cpp:1119-1124).
+// End of frame (renderer.cpp:1119-1124).
+static void frame_commit(rgi_renderer* r, const FrameJob& j, const rgi_view* view, hipStream_t stream, bool cross_frame)
+{
     r->di_valid = j.di_active;
     r->previous = *view;
     r->have_frame = true;
     r->read = j.write;
     r->history_valid = true;
     ++r->frame_number;
-    r->last = f;
+    r->last = j.f;
     if (!cross_frame) join_nrc(r, stream);
+}
+
+static int frame_finish(rgi_renderer* r, const FrameJob& j, const rgi_view* view, const rgi_gbuffer& g,
+                        const uint16_t* specular, hipStream_t stream, bool cross_frame)
+{
+    if (j.s.composite && r->hdr) {
+        int e = composite(j.f, g, composite_inputs(r, j, r->denoised, specular), r->hdr, stream);
+        if (e != RGI_OK) return e;
+    }
+
+    frame_commit(r, j, view, stream, cross_frame);
     return RGI_OK;
 }
 
@@ -1195,6 +1227,7 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     L.gi_spatial_waves = tuning0.gi_spatial_waves;
     L.compute_units = context_compute_units(b->renderers[0]->ctx);
     bool any_di = false;
+    uint32_t max_atrous = 0u;
     for (uint32_t i = 0; i < n && e == RGI_OK; ++i) {
         rgi_renderer* r = b->renderers[i];
         const FrameJob& j = jobs[i];
@@ -1245,12 +1278,26 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
             }
         }
         (v.same ? L.same : L.larger) = true;
+        // The denoiser's and the composite's planes (as frame_post and
+        // frame_finish use them).
+        const bool denoise = j.s.denoiser != 0;
+        v.atrous = denoise ? (uint32_t)j.s.atrous_iterations : 0u;
+        denoiser_planes(r, j, v.den_out);
+        if (v.atrous > max_atrous) max_atrous = v.atrous;
+        const bool composited = j.s.composite && r->hdr;
+        if (composited) {
+            v.ci = composite_inputs(r, j, v.atrous > 0u ? v.den_out[v.atrous] : r->color[j.write],
+                                    planned_specular(r, j));
+            v.hdr = r->hdr;
+        }
         const uint32_t blocks[kBatchPasses] = {
             j.di_active ? slice(dg.x * dg.y) : 0u,
             slice(grid_clear_blocks(j.f.grid_table_size)),
             j.s.grid_mode == RGI_GRID_DETERMINISTIC ? 0u : slice(v.grid_gx * v.grid_gy),
             j.split ? slice((v.list.tiles + 15u) / 16u) : 0u,
-            slice(gg.x * gg.y)};
+            slice(gg.x * gg.y),
+            denoise ? slice(svgf_blocks(j.f)) : 0u,
+            composited ? slice(((uint32_t)g.width + 15u) / 16u * (((uint32_t)g.height + 15u) / 16u)) : 0u};
         for (int p = 0; p < kBatchPasses; ++p) hfirst[p * (n + 1) + i + 1] = hfirst[p * (n + 1) + i] + blocks[p];
     }
     // The caches' training steps of this frame, planned now (growing their
@@ -1352,7 +1399,11 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     // rgi_renderer_render leaves a training step): beside the composites and
     // the next frame's DI passes and tracing, ahead of its queries.
     std::vector<const uint16_t*> specular(n, nullptr);
-    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) e = frame_post(b->renderers[i], jobs[i], gbuffers[i], stream, &specular[i]);
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
+        e = frame_specular(b->renderers[i], jobs[i], gbuffers[i], stream, &specular[i]);
+    if (e == RGI_OK) e = svgf_batch(L.views, L.first[kPassDenoise], n, L.blocks[kPassDenoise], (int)max_atrous, stream);
+    for (uint32_t i = 0; i < n; ++i)
+        b->renderers[i]->denoised = hv[i].atrous > 0u ? hv[i].den_out[hv[i].atrous] : b->renderers[i]->color[jobs[i].write];
     for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
         if (jobs[i].nrc_on) e = frame_gather(b->renderers[i], jobs[i], gbuffers[i], stream);
     if (any_train && e == RGI_OK) {
@@ -1371,8 +1422,8 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
             }
         }
     }
-    for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
-        e = frame_finish(b->renderers[i], jobs[i], &views[i], gbuffers[i], specular[i], stream, true);
+    if (e == RGI_OK) e = composite_batch(L.views, L.first[kPassComposite], n, L.blocks[kPassComposite], stream);
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) frame_commit(b->renderers[i], jobs[i], &views[i], stream, true);
     return e;
 }
 

@@ -4,6 +4,7 @@
 // normal planes. Full-screen passes over the GI planes: one thread per
 // texel, 16 x 16 texels per block, every tap a point fetch with clamp.
 #include "device.h"
+#include "batch.h"
 #include "internal.h"
 #include "rgi_math.h"
 
@@ -37,20 +38,20 @@ RGI_DEV float spline_weight(int offset)
     return d == 0 ? 0.375f : (d == 1 ? 0.25f : 0.0625f);
 }
 
-RGI_DEV bool block_texel(int w, int h, int& x, int& y)
+// Texel of this thread in the 16 x 16-texel block (bx, by).
+RGI_DEV bool block_texel(int w, int h, uint32_t bx, uint32_t by, int& x, int& y)
 {
-    x = (int)(blockIdx.x * 16u + (threadIdx.x & 15u));
-    y = (int)(blockIdx.y * 16u + (threadIdx.x >> 4));
+    x = (int)(bx * 16u + (threadIdx.x & 15u));
+    y = (int)(by * 16u + (threadIdx.x >> 4));
     return x < w && y < h;
 }
 
 // fs_svgf_variance.sc:24-73.
-__global__ void __launch_bounds__(256) k_svgf_variance(SvgfPlanes t, float scale, const uint16_t* color,
-                                                       const uint16_t* moments, const float* receiver,
-                                                       const float* normal, uint16_t* out)
+RGI_DEV void variance_body(const SvgfPlanes& t, float scale, const uint16_t* color, const uint16_t* moments,
+                           const float* receiver, const float* normal, uint16_t* out, uint32_t bx, uint32_t by)
 {
     int x, y;
-    if (!block_texel(t.w, t.h, x, y)) return;
+    if (!block_texel(t.w, t.h, bx, by, x, y)) return;
     uint32_t o = (uint32_t)y * (uint32_t)t.w + (uint32_t)x;
     F2 uv0 = F2{((float)x + 0.5f) * t.ts.x, ((float)y + 0.5f) * t.ts.y};
     F3 n = xyz(t.f4(normal, uv0));
@@ -86,12 +87,11 @@ __global__ void __launch_bounds__(256) k_svgf_variance(SvgfPlanes t, float scale
 }
 
 // fs_svgf_atrous.sc:35-101.
-__global__ void __launch_bounds__(256) k_svgf_atrous(SvgfPlanes t, float scale, const uint16_t* input,
-                                                     const float* receiver, const float* normal, F2 spacing,
-                                                     uint16_t* out)
+RGI_DEV void atrous_body(const SvgfPlanes& t, float scale, const uint16_t* input, const float* receiver,
+                         const float* normal, F2 spacing, uint16_t* out, uint32_t bx, uint32_t by)
 {
     int x, y;
-    if (!block_texel(t.w, t.h, x, y)) return;
+    if (!block_texel(t.w, t.h, bx, by, x, y)) return;
     uint32_t o = (uint32_t)y * (uint32_t)t.w + (uint32_t)x;
     F2 uv0 = F2{((float)x + 0.5f) * t.ts.x, ((float)y + 0.5f) * t.ts.y};
     F3 n = xyz(t.f4(normal, uv0));
@@ -133,9 +133,49 @@ __global__ void __launch_bounds__(256) k_svgf_atrous(SvgfPlanes t, float scale, 
     store_h4(out, o, F4{c.x, c.y, c.z, vsum / (wsum * wsum)});
 }
 
-SvgfPlanes planes_of(const rgi_frame& f)
+__global__ void __launch_bounds__(256) k_svgf_variance(SvgfPlanes t, float scale, const uint16_t* color,
+                                                       const uint16_t* moments, const float* receiver,
+                                                       const float* normal, uint16_t* out)
+{
+    variance_body(t, scale, color, moments, receiver, normal, out, blockIdx.x, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) k_svgf_atrous(SvgfPlanes t, float scale, const uint16_t* input,
+                                                     const float* receiver, const float* normal, F2 spacing,
+                                                     uint16_t* out)
+{
+    atrous_body(t, scale, input, receiver, normal, spacing, out, blockIdx.x, blockIdx.y);
+}
+
+__host__ __device__ inline SvgfPlanes planes_of(const rgi_frame& f)
 {
     return SvgfPlanes{f.gi_width, f.gi_height, F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height}};
+}
+
+// The batched forms: workgroups [first[v], first[v + 1]) run view v's
+// 16 x 16-texel blocks of its GI planes (views without the denoiser have no
+// workgroups; a-trous iteration `it` skips views with fewer iterations).
+__global__ void __launch_bounds__(256) k_svgf_variance_batch(const BatchView* __restrict__ views,
+                                                             const uint32_t* __restrict__ first, uint32_t count)
+{
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    const uint32_t gx = ((uint32_t)v.f.gi_width + 15u) / 16u;
+    variance_body(planes_of(v.f), v.f.scene_scale, v.color, v.moments, v.res.receiver, v.res.normal, v.den_out[0],
+                  b % gx, b / gx);
+}
+
+__global__ void __launch_bounds__(256) k_svgf_atrous_batch(const BatchView* __restrict__ views,
+                                                           const uint32_t* __restrict__ first, uint32_t count, int it)
+{
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    if (it >= (int)v.atrous) return;
+    const SvgfPlanes t = planes_of(v.f);
+    const float step = (float)(1 << it);
+    const uint32_t gx = ((uint32_t)v.f.gi_width + 15u) / 16u;
+    atrous_body(t, v.f.scene_scale, v.den_out[it], v.res.receiver, v.res.normal, F2{step * t.ts.x, step * t.ts.y},
+                v.den_out[it + 1], b % gx, b / gx);
 }
 
 dim3 grid_of(const rgi_frame& f)
@@ -144,6 +184,22 @@ dim3 grid_of(const rgi_frame& f)
 }
 
 } // namespace
+
+uint32_t svgf_blocks(const rgi_frame& f)
+{
+    const dim3 g = grid_of(f);
+    return g.x * g.y;
+}
+
+int svgf_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, int iterations,
+               hipStream_t stream)
+{
+    if (blocks == 0u) return RGI_OK;
+    hipLaunchKernelGGL(k_svgf_variance_batch, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    for (int it = 0; it < iterations; ++it)
+        hipLaunchKernelGGL(k_svgf_atrous_batch, dim3(blocks), dim3(256), 0, stream, views, first, count, it);
+    return check_launch("svgf (batch)");
+}
 
 int svgf_variance(const rgi_frame& f, const uint16_t* color, const uint16_t* moments, const float* receiver,
                   const float* normal, uint16_t* out, hipStream_t stream)
