@@ -50,6 +50,11 @@ struct BatchView {
     // The composite into hdr (the window size).
     rgi_composite_inputs ci;
     uint16_t* hdr;
+    // Indirect specular: last frame's composite and the history in, the
+    // other history plane out (GI spatial's inputs and tile map).
+    const uint16_t* spec_previous;
+    const uint16_t* spec_history;
+    uint16_t* spec_out;
 };
 
 // A cache's training step in a batch's table (nrc.hip fills it), and the
@@ -63,7 +68,10 @@ enum NrcPass {
 };
 
 // The batched screen passes, each with its own slices of a launch.
-enum BatchPass { kPassDi = 0, kPassGridClear, kPassGrid, kPassTiles, kPassGiSpatial, kPassDenoise, kPassComposite, kBatchPasses };
+enum BatchPass {
+    kPassDi = 0, kPassGridClear, kPassGrid, kPassTiles, kPassGiSpatial, kPassDenoise, kPassComposite, kPassSpecular,
+    kBatchPasses
+};
 
 // Host side: a batch frame's device table and the sizes of its launches.
 struct BatchLaunch {

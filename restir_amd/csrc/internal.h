@@ -141,6 +141,10 @@ uint32_t svgf_blocks(const rgi_frame& f);
 int svgf_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, int iterations,
                hipStream_t stream);
 int composite_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, hipStream_t stream);
+// Indirect specular over every view of a batch (`same`, `larger`: views whose
+// G-buffer has the GI size, or is larger).
+int specular_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, bool same,
+                   bool larger, hipStream_t stream);
 // A batch of caches' training steps: nrc_step_plan writes the next step of
 // cache n on `count` records into a kNrcStepBytes slot (growing the cache's
 // scratch; nothing is launched) with each kernel's workgroups in blocks[];

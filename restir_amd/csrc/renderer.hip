@@ -1290,6 +1290,13 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
                                     planned_specular(r, j));
             v.hdr = r->hdr;
         }
+        // Indirect specular (as frame_specular runs it).
+        const bool specular_on = planned_specular(r, j) != nullptr;
+        if (specular_on) {
+            v.spec_previous = r->hdr;
+            v.spec_history = r->specular_gi[r->specular_read];
+            v.spec_out = r->specular_gi[1 - r->specular_read];
+        }
         const uint32_t blocks[kBatchPasses] = {
             j.di_active ? slice(dg.x * dg.y) : 0u,
             slice(grid_clear_blocks(j.f.grid_table_size)),
@@ -1297,7 +1304,8 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
             j.split ? slice((v.list.tiles + 15u) / 16u) : 0u,
             slice(gg.x * gg.y),
             denoise ? slice(svgf_blocks(j.f)) : 0u,
-            composited ? slice(((uint32_t)g.width + 15u) / 16u * (((uint32_t)g.height + 15u) / 16u)) : 0u};
+            composited ? slice(((uint32_t)g.width + 15u) / 16u * (((uint32_t)g.height + 15u) / 16u)) : 0u,
+            specular_on ? slice(gg.x * gg.y) : 0u};
         for (int p = 0; p < kBatchPasses; ++p) hfirst[p * (n + 1) + i + 1] = hfirst[p * (n + 1) + i] + blocks[p];
     }
     // The caches' training steps of this frame, planned now (growing their
@@ -1398,9 +1406,9 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     // launches on the caches' stream, past the end of the call (as
     // rgi_renderer_render leaves a training step): beside the composites and
     // the next frame's DI passes and tracing, ahead of its queries.
-    std::vector<const uint16_t*> specular(n, nullptr);
-    for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
-        e = frame_specular(b->renderers[i], jobs[i], gbuffers[i], stream, &specular[i]);
+    if (e == RGI_OK) e = specular_batch(L.views, L.first[kPassSpecular], n, L.blocks[kPassSpecular], L.same, L.larger, stream);
+    for (uint32_t i = 0; i < n; ++i)
+        if (hv[i].spec_out) b->renderers[i]->specular_read = 1 - b->renderers[i]->specular_read;
     if (e == RGI_OK) e = svgf_batch(L.views, L.first[kPassDenoise], n, L.blocks[kPassDenoise], (int)max_atrous, stream);
     for (uint32_t i = 0; i < n; ++i)
         b->renderers[i]->denoised = hv[i].atrous > 0u ? hv[i].den_out[hv[i].atrous] : b->renderers[i]->color[jobs[i].write];

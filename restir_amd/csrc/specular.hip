@@ -8,6 +8,7 @@
 // depth buffer cannot answer compacted per wave into an LDS queue and traced
 // by the first lanes of the wave.
 #include "gi_device.h"
+#include "batch.h"
 #include "internal.h"
 
 namespace rgi {
@@ -91,11 +92,10 @@ RGI_DEV F3 hit_radiance(const GiPass& p, const uint16_t* previous_hdr, F3 hit, f
 // fs_specular.sc:127-226 (kSameSize: a G-buffer of the GI size, as in the GI
 // passes).
 template <bool kSameSize>
-__global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
-                                                  const uint16_t* previous_hdr, const uint16_t* history,
-                                                  uint16_t* out, F2 texel, TileMap tiles)
+RGI_DEV void specular_body(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
+                           const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out, F2 texel,
+                           const TileMap& tiles, const BlockAt& at, SpecularLds& lds)
 {
-    __shared__ SpecularLds lds;
     GiPass p;
     p.f = &f;
     p.v = scene_view(s);
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_
     p.sc = f.scene_scale;
 
     int x, y;
-    if (!group_pixel(tiles, x, y)) return;
+    if (!group_pixel(tiles, at, x, y)) return;
     uint32_t lane = threadIdx.x & 63u, qbase = threadIdx.x & ~63u;
     bool inside = x < p.hw && y < p.hh;
     F2 uv = F2{((float)x + 0.5f) * texel.x, ((float)y + 0.5f) * texel.y};
@@ -213,7 +213,40 @@ __global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_
     store_h4(out, o, F4{color.x, color.y, color.z, history_length});
 }
 
+template <bool kSameSize>
+__global__ void __launch_bounds__(256) k_specular(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
+                                                  const uint16_t* previous_hdr, const uint16_t* history,
+                                                  uint16_t* out, F2 texel, TileMap tiles)
+{
+    __shared__ SpecularLds lds;
+    specular_body<kSameSize>(f, s, g, in, previous_hdr, history, out, texel, tiles, this_block(), lds);
+}
+
+// The batched form: workgroups [first[v], first[v + 1]) run view v's tiles
+// (its GI spatial tile map), for the views whose G-buffer size matches
+// kSameSize.
+template <bool kSameSize>
+__global__ void __launch_bounds__(256) k_specular_batch(const BatchView* __restrict__ views,
+                                                        const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ SpecularLds lds;
+    uint32_t b;
+    const BatchView& v = views[batch_view(first, count, blockIdx.x, b)];
+    if (v.same != (kSameSize ? 1u : 0u)) return; // the whole workgroup, before the barriers
+    specular_body<kSameSize>(v.fg, v.s, v.g, v.sin, v.spec_previous, v.spec_history, v.spec_out, v.gi_texel,
+                             v.gi_tiles, BlockAt{b, v.gi_gx, v.gi_gy}, lds);
+}
+
 } // namespace
+
+int specular_batch(const BatchView* views, const uint32_t* first, uint32_t count, uint32_t blocks, bool same,
+                   bool larger, hipStream_t stream)
+{
+    if (blocks == 0u) return RGI_OK;
+    if (same) hipLaunchKernelGGL(k_specular_batch<true>, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    if (larger) hipLaunchKernelGGL(k_specular_batch<false>, dim3(blocks), dim3(256), 0, stream, views, first, count);
+    return check_launch("specular gi (batch)");
+}
 
 int specular_gi(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, const rgi_gi_inputs& in,
                 const uint16_t* previous_hdr, const uint16_t* history, uint16_t* out, const Tuning& tuning,
