@@ -631,14 +631,16 @@ def run_workload(hs, settings, W, H, frames_in_path, steps, warmup, tune_frames,
 
 
 def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offsets=None,
-              forms=(ra.RGI_BATCH_GRAPH, ra.RGI_BATCH_PASSES), singles=True):
+              forms=(ra.RGI_BATCH_GRAPH, ra.RGI_BATCH_PASSES), singles=True, full=False):
     """Renderers of `cases` as one batch frame: each renderer alone (its own
     frame time), all of them eagerly one after another on one stream, then
     all of them as one rgi_batch_render call in both of its forms: each pass
     over every view in one launch (RGI_BATCH_PASSES) and one graph with each
     renderer on a stream of its own (RGI_BATCH_GRAPH). With `nrc`, every
     renderer has its own neural radiance cache. `forms` and `singles` limit
-    the run to some of the batch forms, without the single renderers."""
+    the run to some of the batch forms, without the single renderers. With
+    `full`, the renderers also run indirect specular, the denoiser and the
+    composite."""
     import torch
     from restir_amd import nrc as rnrc
     L = ra.lib()
@@ -655,7 +657,7 @@ def run_batch(cases, mode, steps, warmup, nrc=False, views_per_case=4, yaw_offse
             scenes[key] = (hs, driver.GpuScene(hs))
         hs, gscene = scenes[key]
         table = 1 << 20 if w * h <= (1 << 21) else 1 << 21
-        st = ra.Settings(grid_table_size=table, grid_mode=mode, denoiser=False, composite=False, specular_gi=False)
+        st = ra.Settings(grid_table_size=table, grid_mode=mode, denoiser=full, composite=full, specular_gi=full)
         yaw0 = yaw_offsets[c] if yaw_offsets else 0.0
         views, gbs = [], []
         for i in range(views_per_case):
@@ -797,6 +799,10 @@ def main():
     ap.add_argument("--batch-only", choices=["uneven", "views8"],
                     help="run only one batch workload (--steps timed batch frames after --warmup), pass-major form "
                          "only, and print it: the workload of the batch kernel profiles")
+    ap.add_argument("--batch-full", action="store_true",
+                    help="with --batch-only: the renderers also run indirect specular, the denoiser and the composite")
+    ap.add_argument("--batch-all-forms", action="store_true",
+                    help="with --batch-only: also time each renderer alone, all of them eagerly, and the graph form")
     ap.add_argument("--cache-frames-only", action="store_true",
                     help="render only the metric's path with the neural radiance cache on (16 warm-up + --steps "
                          "frames) and print their frame time: the workload of the cache's kernel profile")
@@ -830,10 +836,16 @@ def main():
         else:
             cases = [("atrium_256k_view%d" % v, ra.RGI_SCENE_ATRIUM, 262144, 1, W, H, 0.7) for v in range(8)]
             kw = {"yaw_offsets": [v * math.pi / 4 for v in range(8)]}
-        out = run_batch(cases, mode, args.steps, args.warmup, forms=(ra.RGI_BATCH_PASSES,), singles=False, **kw)
-        print(json.dumps({"batch": args.batch_only, "passes_batch_ms_per_batch_frame":
-                          out["passes_batch_ms_per_batch_frame"], "steps": args.steps, "warmup": args.warmup}),
-              flush=True)
+        if args.batch_all_forms:
+            out = run_batch(cases, mode, args.steps, args.warmup, full=args.batch_full, **kw)
+        else:
+            out = run_batch(cases, mode, args.steps, args.warmup, forms=(ra.RGI_BATCH_PASSES,), singles=False,
+                            full=args.batch_full, **kw)
+        keep = ("eager_ms_per_batch_frame", "graph_batch_ms_per_batch_frame", "passes_batch_ms_per_batch_frame",
+                "passes_vs_eager", "sum_of_scene_ms")
+        line = {"batch": args.batch_only, "full": args.batch_full, "steps": args.steps, "warmup": args.warmup}
+        line.update({k: out[k] for k in keep if out[k] == out[k]})
+        print(json.dumps(line), flush=True)
         return
     # The metric's frame: grid + resample + trace. The denoiser that follows
     # it in the reference is timed on its own below.
