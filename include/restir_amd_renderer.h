@@ -141,8 +141,8 @@ int rgi_renderer_render_graph(rgi_renderer* renderer, const rgi_view* view, cons
  * its kernels reading each view's frame, scene and planes from a device
  * table that the call uploads from pinned host memory; the caches' queries
  * run as one launch per precision and their training steps as one chain of
- * launches for all of them (as rgi_nrc_train_batch); the cache's gather and
- * the deterministic insert run per view.
+ * launches for all of them (as rgi_nrc_train_batch), their training-record
+ * gathers as one chain; only the deterministic insert runs per view.
  * GI temporal's tracing and the grids run on the batch's side stream, the
  * caches' queries and training on a third stream, forked from and joined
  * back to `stream` (which may be the null stream); as with

@@ -62,6 +62,8 @@ struct BatchView {
 constexpr size_t kNrcStepBytes = 512;
 // A cache's query of one G-buffer plane in a batch's table (nrc.hip fills it).
 constexpr size_t kNrcQueryBytes = 2048;
+// A cache's training-record gather in a batch's table (nrc.hip fills it).
+constexpr size_t kNrcGatherBytes = 1024;
 enum NrcPass {
     kNrcTrain = 0, kNrcWgrad, kNrcReduce, kNrcAdamMlp, kNrcCopies, kNrcAdamTable, kNrcEmaMlp, kNrcEmaCopies,
     kNrcEmaTable, kNrcPasses

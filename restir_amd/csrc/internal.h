@@ -169,6 +169,18 @@ int nrc_query_plan(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const u
                    uint16_t* out, void* slot, uint32_t* blocks);
 int nrc_query_batch(const void* queries, const uint32_t* first, uint32_t count, uint32_t blocks,
                     const bool* precisions, hipStream_t stream);
+// A batch of caches' training-record gathers: nrc_gather_plan writes
+// nrc_gather's arguments into a kNrcGatherBytes slot (growing the cache's
+// surface list; nothing is launched), with the workgroups of its surface
+// passes in blocks[0] and of its gather in blocks[1]; nrc_gather_batch runs
+// a device table of them (first: count + 1 slice starts for the surface
+// passes, then count + 1 for the gather; blocks: the two totals), with
+// results identical to nrc_gather's.
+int nrc_gather_plan(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
+                    const uint16_t* color, const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
+                    uint32_t batch, uint32_t seed, float* records, float* targets, void* slot, uint32_t* blocks);
+int nrc_gather_batch(const void* gathers, const uint32_t* first, uint32_t count, const uint32_t* blocks,
+                     hipStream_t stream);
 
 // The neural radiance cache's frame-driver passes (nrc.hip): the cache's
 // radiance at every G-buffer pixel, and one frame's training records.

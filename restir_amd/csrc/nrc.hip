@@ -1233,10 +1233,10 @@ __global__ void __launch_bounds__(256) k_nrc_infer_gbuffer_batch(const uint8_t* 
 // its surface pixels' indices at its offset.
 constexpr uint32_t kSurfaceBlock = 1024;
 
-__global__ void __launch_bounds__(256) k_nrc_surface_count(const float* depth, uint32_t npix, uint32_t* counts)
+RGI_DEV void surface_count_body(const float* depth, uint32_t npix, uint32_t* counts, uint32_t block,
+                                uint32_t* per_wave)
 {
-    __shared__ uint32_t per_wave[4];
-    uint32_t base = blockIdx.x * kSurfaceBlock, c = 0;
+    uint32_t base = block * kSurfaceBlock, c = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         uint32_t p = base + k * 256u + threadIdx.x;
@@ -1244,12 +1244,17 @@ __global__ void __launch_bounds__(256) k_nrc_surface_count(const float* depth, u
     }
     if ((threadIdx.x & 63u) == 0) per_wave[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) counts[blockIdx.x] = ((per_wave[0] + per_wave[1]) + per_wave[2]) + per_wave[3];
+    if (threadIdx.x == 0) counts[block] = ((per_wave[0] + per_wave[1]) + per_wave[2]) + per_wave[3];
 }
 
-__global__ void __launch_bounds__(1024) k_nrc_surface_scan(const uint32_t* counts, uint32_t blocks, uint32_t* offsets)
+__global__ void __launch_bounds__(256) k_nrc_surface_count(const float* depth, uint32_t npix, uint32_t* counts)
 {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t per_wave[4];
+    surface_count_body(depth, npix, counts, blockIdx.x, per_wave);
+}
+
+RGI_DEV void surface_scan_body(const uint32_t* counts, uint32_t blocks, uint32_t* offsets, uint32_t* part)
+{
     uint32_t per = (blocks + 1023u) / 1024u, lo = threadIdx.x * per, hi = min(lo + per, blocks), s = 0;
     for (uint32_t i = lo; i < hi; ++i) s += counts[i];
     part[threadIdx.x] = s;
@@ -1268,11 +1273,16 @@ __global__ void __launch_bounds__(1024) k_nrc_surface_scan(const uint32_t* count
     if (threadIdx.x == 1023u) offsets[blocks] = part[1023];
 }
 
-__global__ void __launch_bounds__(256) k_nrc_surface_list(const float* depth, uint32_t npix, const uint32_t* offsets,
-                                                          uint32_t* list)
+__global__ void __launch_bounds__(1024) k_nrc_surface_scan(const uint32_t* counts, uint32_t blocks, uint32_t* offsets)
 {
-    __shared__ uint32_t per_wave[4][4];
-    const uint32_t wave = threadIdx.x >> 6, base = blockIdx.x * kSurfaceBlock;
+    __shared__ uint32_t part[1024];
+    surface_scan_body(counts, blocks, offsets, part);
+}
+
+RGI_DEV void surface_list_body(const float* depth, uint32_t npix, const uint32_t* offsets, uint32_t* list,
+                               uint32_t block, uint32_t (*per_wave)[4])
+{
+    const uint32_t wave = threadIdx.x >> 6, base = block * kSurfaceBlock;
     bool surface[4];
     uint64_t mask[4];
 #pragma unroll
@@ -1283,7 +1293,7 @@ __global__ void __launch_bounds__(256) k_nrc_surface_list(const float* depth, ui
         if ((threadIdx.x & 63u) == 0) per_wave[k][wave] = (uint32_t)__popcll(mask[k]);
     }
     __syncthreads();
-    uint32_t run = offsets[blockIdx.x];
+    uint32_t run = offsets[block];
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         uint32_t before = run;
@@ -1291,6 +1301,13 @@ __global__ void __launch_bounds__(256) k_nrc_surface_list(const float* depth, ui
         if (surface[k]) list[before + rank_in(mask[k])] = base + k * 256u + threadIdx.x;
         run += ((per_wave[k][0] + per_wave[k][1]) + per_wave[k][2]) + per_wave[k][3];
     }
+}
+
+__global__ void __launch_bounds__(256) k_nrc_surface_list(const float* depth, uint32_t npix, const uint32_t* offsets,
+                                                          uint32_t* list)
+{
+    __shared__ uint32_t per_wave[4][4];
+    surface_list_body(depth, npix, offsets, list, blockIdx.x, per_wave);
 }
 
 // Training records of one frame: `batch` records stratified over the Ns
@@ -1308,12 +1325,12 @@ __global__ void __launch_bounds__(256) k_nrc_surface_list(const float* depth, ui
 // onto the screen and the depth buffer there reconstructs it within 0.1 scene
 // units), taken at the pixel where it lies; a pixel whose sample is hidden,
 // the sky or absent keeps its own record.
-__global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_t* di_diffuse, const uint16_t* color,
-                             const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
-                             const uint32_t* surface_total, const uint32_t* surface_list, uint32_t batch, uint32_t seed,
-                             float* records, float* targets)
+RGI_DEV void gather_body(const rgi_frame& f, const rgi_gbuffer& g, F2 texel, const uint16_t* di_diffuse,
+                         const uint16_t* color, const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
+                         const uint32_t* surface_total, const uint32_t* surface_list, uint32_t batch, uint32_t seed,
+                         float* records, float* targets, uint32_t block)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = block * blockDim.x + threadIdx.x;
     if (i >= batch) return;
     uint64_t ns = *surface_total;
     uint32_t h = pcg(i + pcg(seed));
@@ -1359,6 +1376,76 @@ __global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_
     r[1] = surface ? make_float4(n.x, n.y, n.z, 0.0f) : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
     r[2] = surface ? make_float4(albedo.x, albedo.y, albedo.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     reinterpret_cast<float4*>(targets)[i] = make_float4(target.x, target.y, target.z, surface ? 1.0f : 0.0f);
+}
+
+__global__ void k_nrc_gather(rgi_frame f, rgi_gbuffer g, F2 texel, const uint16_t* di_diffuse, const uint16_t* color,
+                             const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
+                             const uint32_t* surface_total, const uint32_t* surface_list, uint32_t batch, uint32_t seed,
+                             float* records, float* targets)
+{
+    gather_body(f, g, texel, di_diffuse, color, gi_sample, gi_normal, vertex_every, surface_total, surface_list, batch,
+                seed, records, targets, blockIdx.x);
+}
+
+// One cache's training-record gather in a batch: nrc_gather's arguments and
+// the cache's surface-list scratch. The batched launches give gather q the
+// workgroups [first[q], first[q + 1]) of each slice pass (the surface count
+// and list, the gather); the scan runs one workgroup per gather.
+struct NrcGather {
+    rgi_frame f;
+    rgi_gbuffer g;
+    F2 texel;
+    const uint16_t* di_diffuse;
+    const uint16_t* color;
+    const float* gi_sample;
+    const float* gi_normal;
+    uint32_t vertex_every, batch, seed, npix, blocks, active;
+    uint32_t* counts;
+    uint32_t* offsets;
+    uint32_t* list;
+    float* records;
+    float* targets;
+};
+static_assert(sizeof(NrcGather) <= kNrcGatherBytes, "a gather must fit its slot of a batch's table");
+
+RGI_DEV const NrcGather& gather_at(const uint8_t* gathers, uint32_t i)
+{
+    return *reinterpret_cast<const NrcGather*>(gathers + (size_t)i * kNrcGatherBytes);
+}
+
+__global__ void __launch_bounds__(256) k_nrc_surface_count_batch(const uint8_t* __restrict__ gathers,
+                                                                 const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ uint32_t per_wave[4];
+    uint32_t b;
+    const NrcGather& q = gather_at(gathers, batch_view(first, count, blockIdx.x, b));
+    surface_count_body(q.g.depth, q.npix, q.counts, b, per_wave);
+}
+
+__global__ void __launch_bounds__(1024) k_nrc_surface_scan_batch(const uint8_t* __restrict__ gathers)
+{
+    __shared__ uint32_t part[1024];
+    const NrcGather& q = gather_at(gathers, blockIdx.x);
+    if (!q.active) return;
+    surface_scan_body(q.counts, q.blocks, q.offsets, part);
+}
+
+__global__ void __launch_bounds__(256) k_nrc_surface_list_batch(const uint8_t* __restrict__ gathers,
+                                                                const uint32_t* __restrict__ first, uint32_t count)
+{
+    __shared__ uint32_t per_wave[4][4];
+    uint32_t b;
+    const NrcGather& q = gather_at(gathers, batch_view(first, count, blockIdx.x, b));
+    surface_list_body(q.g.depth, q.npix, q.offsets, q.list, b, per_wave);
+}
+
+__global__ void __launch_bounds__(256) k_nrc_gather_batch(const uint8_t* __restrict__ gathers,
+                                                          const uint32_t* __restrict__ first, uint32_t count)
+{
+    uint32_t b;
+    const NrcGather& q = gather_at(gathers, batch_view(first, count, blockIdx.x, b));
+    gather_body(q.f, q.g, q.texel, q.di_diffuse, q.color, q.gi_sample, q.gi_normal, q.vertex_every,
+                q.offsets + q.blocks, q.list, q.batch, q.seed, q.records, q.targets, b);
 }
 
 } // namespace nrc
@@ -1431,12 +1518,11 @@ int nrc_query_batch(const void* queries, const uint32_t* first, uint32_t count, 
     return check_launch("nrc infer (G-buffer, batch)");
 }
 
-int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
-               const uint16_t* color, const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
-               uint32_t batch, uint32_t seed, float* records, float* targets, hipStream_t stream)
+// The cache's surface-list scratch for npix pixels (grown after a device
+// sync).
+static int ensure_surface_list(rgi_nrc* n, uint32_t npix)
 {
-    uint32_t npix = (uint32_t)g.width * (uint32_t)g.height;
-    uint32_t blocks = (npix + nrc::kSurfaceBlock - 1) / nrc::kSurfaceBlock;
+    const uint32_t blocks = (npix + nrc::kSurfaceBlock - 1) / nrc::kSurfaceBlock;
     if (npix > n->capacity_pixels) {
         (void)hipDeviceSynchronize();
         (void)hipFree(n->surface_counts);
@@ -1451,6 +1537,64 @@ int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint1
             return fail(RGI_E_NOMEM, "nrc gather: surface list of %u pixels could not be allocated", npix);
         n->capacity_pixels = npix;
     }
+    return RGI_OK;
+}
+
+int nrc_gather_plan(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
+                    const uint16_t* color, const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
+                    uint32_t batch, uint32_t seed, float* records, float* targets, void* slot, uint32_t* blocks)
+{
+    const uint32_t npix = (uint32_t)g.width * (uint32_t)g.height;
+    int e = ensure_surface_list(n, npix);
+    if (e != RGI_OK) return e;
+    nrc::NrcGather q;
+    memset(&q, 0, sizeof(q));
+    q.f = f;
+    q.g = g;
+    q.texel = F2{1.0f / (float)g.width, 1.0f / (float)g.height};
+    q.di_diffuse = di_diffuse;
+    q.color = color;
+    q.gi_sample = gi_sample;
+    q.gi_normal = gi_normal;
+    q.vertex_every = vertex_every;
+    q.batch = batch;
+    q.seed = seed;
+    q.npix = npix;
+    q.blocks = (npix + nrc::kSurfaceBlock - 1) / nrc::kSurfaceBlock;
+    q.active = 1u;
+    q.counts = n->surface_counts;
+    q.offsets = n->surface_offsets;
+    q.list = n->surface_list;
+    q.records = records;
+    q.targets = targets;
+    memcpy(slot, &q, sizeof(q));
+    blocks[0] = q.blocks;
+    blocks[1] = (batch + 255u) / 256u;
+    return RGI_OK;
+}
+
+int nrc_gather_batch(const void* gathers, const uint32_t* first, uint32_t count, const uint32_t* blocks,
+                     hipStream_t stream)
+{
+    if (blocks[0] == 0u) return RGI_OK;
+    const uint8_t* q = static_cast<const uint8_t*>(gathers);
+    const uint32_t* surface_first = first;
+    const uint32_t* gather_first = first + (count + 1);
+    hipLaunchKernelGGL(nrc::k_nrc_surface_count_batch, dim3(blocks[0]), dim3(256), 0, stream, q, surface_first, count);
+    hipLaunchKernelGGL(nrc::k_nrc_surface_scan_batch, dim3(count), dim3(1024), 0, stream, q);
+    hipLaunchKernelGGL(nrc::k_nrc_surface_list_batch, dim3(blocks[0]), dim3(256), 0, stream, q, surface_first, count);
+    hipLaunchKernelGGL(nrc::k_nrc_gather_batch, dim3(blocks[1]), dim3(256), 0, stream, q, gather_first, count);
+    return check_launch("nrc gather (batch)");
+}
+
+int nrc_gather(rgi_nrc* n, const rgi_frame& f, const rgi_gbuffer& g, const uint16_t* di_diffuse,
+               const uint16_t* color, const float* gi_sample, const float* gi_normal, uint32_t vertex_every,
+               uint32_t batch, uint32_t seed, float* records, float* targets, hipStream_t stream)
+{
+    uint32_t npix = (uint32_t)g.width * (uint32_t)g.height;
+    uint32_t blocks = (npix + nrc::kSurfaceBlock - 1) / nrc::kSurfaceBlock;
+    int e = ensure_surface_list(n, npix);
+    if (e != RGI_OK) return e;
     hipLaunchKernelGGL(nrc::k_nrc_surface_count, dim3(blocks), dim3(256), 0, stream, g.depth, npix, n->surface_counts);
     hipLaunchKernelGGL(nrc::k_nrc_surface_scan, dim3(1), dim3(1024), 0, stream, n->surface_counts, blocks,
                        n->surface_offsets);

@@ -990,9 +990,10 @@ namespace {
 // The view table of a batch of n views: the views, each pass's slice
 // starts, each cache's training step, the training kernels' slice starts,
 // each view's two cache queries (the query plane, and the resolved plane
-// when the query adds emission) and their slice starts.
+// when the query adds emission) and their slice starts, each cache's
+// training-record gather and its passes' slice starts.
 struct TableLayout {
-    size_t first, steps, nrc_first, queries, query_first, bytes;
+    size_t first, steps, nrc_first, queries, query_first, gathers, gather_first, bytes;
 };
 TableLayout table_layout(size_t n)
 {
@@ -1002,7 +1003,9 @@ TableLayout table_layout(size_t n)
     t.nrc_first = t.steps + n * kNrcStepBytes;
     t.queries = align256(t.nrc_first + (size_t)kNrcPasses * (n + 1) * sizeof(uint32_t));
     t.query_first = t.queries + 2 * n * kNrcQueryBytes;
-    t.bytes = t.query_first + (2 * n + 1) * sizeof(uint32_t);
+    t.gathers = align256(t.query_first + (2 * n + 1) * sizeof(uint32_t));
+    t.gather_first = t.gathers + n * kNrcGatherBytes;
+    t.bytes = t.gather_first + 2 * (n + 1) * sizeof(uint32_t);
     return t;
 }
 } // namespace
@@ -1324,6 +1327,26 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
         for (int p = 0; p < kNrcPasses; ++p) nfirst[p * (n + 1) + i + 1] = nfirst[p * (n + 1) + i] + blocks[p];
     }
     for (int p = 0; p < kNrcPasses; ++p) nrc_blocks[p] = nfirst[p * (n + 1) + n];
+    // The caches' training-record gathers (as frame_gather runs them).
+    uint32_t* gfirst = reinterpret_cast<uint32_t*>(host + T.gather_first);
+    for (uint32_t i = 0; i < n && e == RGI_OK; ++i) {
+        rgi_renderer* r = b->renderers[i];
+        const FrameJob& j = jobs[i];
+        uint32_t blocks[2] = {0u, 0u};
+        if (j.nrc_on) {
+            const uint16_t* next_history = j.s.denoiser ? r->filtered[j.write] : r->color[j.write];
+            const int rec = r->nrc_settings.records;
+            const bool vertices = rec == RGI_NRC_RECORDS_VERTICES || rec == RGI_NRC_RECORDS_MIXED;
+            e = nrc_gather_plan(r->nrc, j.f, gbuffers[i], j.di_active ? r->diffuse : nullptr,
+                                j.s.multiple_bounces ? next_history : nullptr, vertices ? r->gi[j.write][0] : nullptr,
+                                vertices ? r->gi[j.write][2] : nullptr, rec == RGI_NRC_RECORDS_MIXED ? 2u : 1u,
+                                r->nrc_settings.batch, j.f.frame_index ^ 0x4e5243u, r->nrc_records, r->nrc_targets,
+                                host + T.gathers + i * kNrcGatherBytes, blocks);
+        }
+        gfirst[i + 1] = gfirst[i] + blocks[0];
+        gfirst[(n + 1) + i + 1] = gfirst[(n + 1) + i] + blocks[1];
+    }
+    const uint32_t gather_blocks[2] = {gfirst[n], gfirst[(n + 1) + n]};
     // The caches' queries: the query plane of every cache that has trained,
     // and its resolved plane where the query adds emission.
     uint32_t* qfirst = reinterpret_cast<uint32_t*>(host + T.query_first);
@@ -1412,8 +1435,13 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
     if (e == RGI_OK) e = svgf_batch(L.views, L.first[kPassDenoise], n, L.blocks[kPassDenoise], (int)max_atrous, stream);
     for (uint32_t i = 0; i < n; ++i)
         b->renderers[i]->denoised = hv[i].atrous > 0u ? hv[i].den_out[hv[i].atrous] : b->renderers[i]->color[jobs[i].write];
-    for (uint32_t i = 0; i < n && e == RGI_OK; ++i)
-        if (jobs[i].nrc_on) e = frame_gather(b->renderers[i], jobs[i], gbuffers[i], stream);
+    // The gathers overwrite the records that each cache's last training step
+    // reads: `stream` joins the steps still pending.
+    for (uint32_t i = 0; i < n; ++i)
+        if (jobs[i].nrc_on) join_nrc(b->renderers[i], stream);
+    if (any_train && e == RGI_OK)
+        e = nrc_gather_batch(b->table + T.gathers, reinterpret_cast<const uint32_t*>(b->table + T.gather_first), n,
+                             gather_blocks, stream);
     if (any_train && e == RGI_OK) {
         hipEventRecord(b->gathered, stream);
         hipStreamWaitEvent(b->cache_side, b->gathered, 0);

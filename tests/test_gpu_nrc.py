@@ -685,6 +685,21 @@ def test_renderer_caches_in_pass_major_batch():
             ra.check(L.rgi_batch_render(batch, views, gbufs, s))
             for r in alone:
                 ra.check(L.rgi_renderer_render(r, ctypes.byref(view), ctypes.byref(gbuf.gb), s))
+            if frame == 0:
+                # The first step trains equal caches on the frame's gathered
+                # records: the batch's one gather chain and the renderers'
+                # own gathers give the same records exactly when the MLPs
+                # after that step are bit-identical (its gradient is a
+                # fixed-order reduction).
+                torch.cuda.synchronize()
+                for rb, rs in zip(batched, alone):
+                    mb = torch.empty(ra.RGI_NRC_MLP_PARAMS, dtype=torch.float32, device="cuda")
+                    ms = torch.empty(ra.RGI_NRC_MLP_PARAMS, dtype=torch.float32, device="cuda")
+                    cb, cs = rnrc.renderer_state(rb)[0], rnrc.renderer_state(rs)[0]
+                    ra.check(L.rgi_nrc_get_params(ctypes.c_void_p(cb), mb.data_ptr(), None, s))
+                    ra.check(L.rgi_nrc_get_params(ctypes.c_void_p(cs), ms.data_ptr(), None, s))
+                    torch.cuda.synchronize()
+                    assert torch.equal(mb, ms), "the batch's first training step differs from the renderer's own"
         torch.cuda.synchronize()
         means = []
         for r in batched + alone:
