@@ -62,7 +62,7 @@ def measured_traffic(pass_name, options, width, height):
         summary = json.load(fh)
     waves = {"gi_temporal": options.get(ra.RGI_OPT_GI_TEMPORAL_WAVES, 0),
              "gi_spatial": options.get(ra.RGI_OPT_GI_SPATIAL_WAVES, 0)}.get(pass_name, 0)
-    suffix = "_w4" if waves == 4 else ""
+    suffix = "_w%d" % waves if waves else ""
     # GI temporal is several kernels (surface-tile listing, candidate march,
     # BVH walks, resampling): its traffic is their sum.
     k = summary.get("passes", {}).get(pass_name + suffix)

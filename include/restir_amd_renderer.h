@@ -176,7 +176,7 @@ void rgi_batch_destroy(rgi_batch* batch);
  * renderer, or a negative code. */
 #define RGI_TUNE_TRIALS 8
 #define RGI_TUNE_TEMPORAL_WAVES4 1
-#define RGI_TUNE_SPATIAL_WAVES4 2
+#define RGI_TUNE_SPATIAL_WAVES5 2
 #define RGI_TUNE_TILE_ORDER 4 /* RGI_OPT_TILE_ORDER 2 */
 #define RGI_TUNE_TILE_SHAPE 8 /* RGI_OPT_TILE_SHAPE 1 */
 int rgi_renderer_autotune(rgi_renderer* renderer, const rgi_view* views, const rgi_gbuffer* gbuffers, uint32_t count,

@@ -28,14 +28,14 @@ RGI_OPT_TILE_ORDER = 4
 RGI_OPT_TRIANGLE_TEST = 5
 RGI_OPT_TILE_SHAPE = 6
 RGI_TUNE_TRIALS = 8
-RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4, RGI_TUNE_TILE_ORDER, RGI_TUNE_TILE_SHAPE = 1, 2, 4, 8
+RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES5, RGI_TUNE_TILE_ORDER, RGI_TUNE_TILE_SHAPE = 1, 2, 4, 8
 
 
 def tuning_of(config):
     """The context options of an autotuner configuration bit set."""
     return {
         RGI_OPT_GI_TEMPORAL_WAVES: 4 if config & RGI_TUNE_TEMPORAL_WAVES4 else 0,
-        RGI_OPT_GI_SPATIAL_WAVES: 4 if config & RGI_TUNE_SPATIAL_WAVES4 else 0,
+        RGI_OPT_GI_SPATIAL_WAVES: 5 if config & RGI_TUNE_SPATIAL_WAVES5 else 0,
         RGI_OPT_TILE_ORDER: 2 if config & RGI_TUNE_TILE_ORDER else 0,
         RGI_OPT_TILE_SHAPE: 1 if config & RGI_TUNE_TILE_SHAPE else 0,
     }

@@ -230,9 +230,12 @@ int rgi_context_set_option(rgi_context* ctx, int option, int value)
     if (!ctx) return fail(RGI_E_INVALID, "context is NULL");
     switch (option) {
     case RGI_OPT_GI_TEMPORAL_WAVES:
+        if (value != 0 && value != 4) return fail(RGI_E_INVALID, "GI temporal waves option takes 0 or 4, not %d", value);
+        ctx->tuning.gi_temporal_waves = value;
+        return RGI_OK;
     case RGI_OPT_GI_SPATIAL_WAVES:
-        if (value != 0 && value != 4) return fail(RGI_E_INVALID, "waves option takes 0 or 4, not %d", value);
-        (option == RGI_OPT_GI_TEMPORAL_WAVES ? ctx->tuning.gi_temporal_waves : ctx->tuning.gi_spatial_waves) = value;
+        if (value != 0 && value != 5) return fail(RGI_E_INVALID, "GI spatial waves option takes 0 or 5, not %d", value);
+        ctx->tuning.gi_spatial_waves = value;
         return RGI_OK;
     case RGI_OPT_TRACE_PERSISTENT:
         if (value != 0 && value != 1) return fail(RGI_E_INVALID, "persistent option takes 0 or 1, not %d", value);

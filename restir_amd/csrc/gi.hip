@@ -452,10 +452,13 @@ RGI_DEV void gi_spatial_body(const rgi_frame& f, const rgi_scene& s, const rgi_g
     store_h4(out_moments, o, F4{m0, m1, history_length, 0.0f});
 }
 
-// The kernels: each pass with the compiler's occupancy, and capped at
-// 128 VGPRs for 4 waves per SIMD; each for a G-buffer of the GI size and for
-// a larger (window-size) one.
+// The kernels: each pass with the compiler's occupancy, and capped: GI
+// temporal at 128 VGPRs for 4 waves per SIMD, GI spatial at 96 VGPRs for 5
+// (its surface waves then fit the chip in one round on a mostly-sky frame;
+// the spills to scratch cost less than the second round); each for a
+// G-buffer of the GI size and for a larger (window-size) one.
 #define RGI_WAVES4 __attribute__((amdgpu_waves_per_eu(4, 4)))
+#define RGI_WAVES5 __attribute__((amdgpu_waves_per_eu(5, 5)))
 
 template <bool kSameSize>
 __global__ void __launch_bounds__(256) k_gi_temporal(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in,
@@ -673,8 +676,8 @@ __global__ void __launch_bounds__(256) k_gi_spatial(rgi_frame f, rgi_scene s, rg
 }
 
 template <bool kSameSize>
-__global__ void __launch_bounds__(256) RGI_WAVES4
-k_gi_spatial_w4(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
+__global__ void __launch_bounds__(256) RGI_WAVES5
+k_gi_spatial_w5(rgi_frame f, rgi_scene s, rgi_gbuffer g, rgi_gi_inputs in, rgi_gi_reservoirs res, uint16_t* out_color,
                 uint16_t* out_moments, F2 texel, TileMap tiles)
 {
     __shared__ VisLds<4> vis;
@@ -701,8 +704,8 @@ __global__ void __launch_bounds__(256) k_gi_spatial_batch(const BatchView* __res
 }
 
 template <bool kSameSize>
-__global__ void __launch_bounds__(256) RGI_WAVES4
-k_gi_spatial_batch_w4(const BatchView* __restrict__ views, const uint32_t* __restrict__ first, uint32_t count)
+__global__ void __launch_bounds__(256) RGI_WAVES5
+k_gi_spatial_batch_w5(const BatchView* __restrict__ views, const uint32_t* __restrict__ first, uint32_t count)
 {
     __shared__ VisLds<4> vis;
     gi_spatial_view<kSameSize>(views, first, count, vis);
@@ -931,7 +934,7 @@ int gi_spatial(const rgi_frame& f, const rgi_scene& s, const rgi_gbuffer& g, con
     TileMap tiles = tile_map(tuning.tile_order, tuning.tile_shape, f.gi_width, f.gi_height);
     F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
     const bool same = g.width == f.gi_width && g.height == f.gi_height;
-    auto kernel = tuning.gi_spatial_waves == 4 ? (same ? k_gi_spatial_w4<true> : k_gi_spatial_w4<false>)
+    auto kernel = tuning.gi_spatial_waves == 5 ? (same ? k_gi_spatial_w5<true> : k_gi_spatial_w5<false>)
                                                : (same ? k_gi_spatial<true> : k_gi_spatial<false>);
     hipLaunchKernelGGL(kernel, tile_grid(tiles), dim3(256), 0, stream, f, s, g, in, res, color,
                        moments, texel, tiles);
@@ -983,12 +986,12 @@ int gi_spatial_batch(const BatchLaunch& b, hipStream_t stream)
     const uint32_t* first = b.first[kPassGiSpatial];
     const uint32_t blocks = b.blocks[kPassGiSpatial];
     if (blocks == 0u) return RGI_OK;
-    const bool w4 = b.gi_spatial_waves == 4;
+    const bool w5 = b.gi_spatial_waves == 5;
     if (b.same)
-        hipLaunchKernelGGL(w4 ? k_gi_spatial_batch_w4<true> : k_gi_spatial_batch<true>, dim3(blocks), dim3(256), 0,
+        hipLaunchKernelGGL(w5 ? k_gi_spatial_batch_w5<true> : k_gi_spatial_batch<true>, dim3(blocks), dim3(256), 0,
                            stream, b.views, first, b.count);
     if (b.larger)
-        hipLaunchKernelGGL(w4 ? k_gi_spatial_batch_w4<false> : k_gi_spatial_batch<false>, dim3(blocks), dim3(256), 0,
+        hipLaunchKernelGGL(w5 ? k_gi_spatial_batch_w5<false> : k_gi_spatial_batch<false>, dim3(blocks), dim3(256), 0,
                            stream, b.views, first, b.count);
     return check_launch("gi spatial (batch)");
 }

@@ -846,7 +846,7 @@ int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuf
     auto apply = [&](int config) {
         rgi::Tuning& t = r->ctx->tuning;
         t.gi_temporal_waves = (config & RGI_TUNE_TEMPORAL_WAVES4) ? 4 : 0;
-        t.gi_spatial_waves = (config & RGI_TUNE_SPATIAL_WAVES4) ? 4 : 0;
+        t.gi_spatial_waves = (config & RGI_TUNE_SPATIAL_WAVES5) ? 5 : 0;
         t.tile_order = (config & RGI_TUNE_TILE_ORDER) ? 2 : 0;
         t.tile_shape = (config & RGI_TUNE_TILE_SHAPE) ? 1 : 0;
     };
@@ -867,9 +867,9 @@ int rgi_renderer_autotune(rgi_renderer* r, const rgi_view* views, const rgi_gbuf
         ms /= (float)frames;
     };
     // Knob t: the bits it owns and the values it tries.
-    const int knob_mask[4] = {RGI_TUNE_TILE_ORDER, RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES4,
+    const int knob_mask[4] = {RGI_TUNE_TILE_ORDER, RGI_TUNE_TEMPORAL_WAVES4, RGI_TUNE_SPATIAL_WAVES5,
                               RGI_TUNE_TILE_SHAPE};
-    const int knob_values[4][2] = {{0, RGI_TUNE_TILE_ORDER}, {0, RGI_TUNE_TEMPORAL_WAVES4}, {0, RGI_TUNE_SPATIAL_WAVES4},
+    const int knob_values[4][2] = {{0, RGI_TUNE_TILE_ORDER}, {0, RGI_TUNE_TEMPORAL_WAVES4}, {0, RGI_TUNE_SPATIAL_WAVES5},
                                    {0, RGI_TUNE_TILE_SHAPE}};
     int trial = 0;
     for (int knob = 0; knob < 4 && e == RGI_OK; ++knob) {

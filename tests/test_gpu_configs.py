@@ -77,7 +77,7 @@ def atrium_settings(**kw):
                        composite=False, specular_gi=False, **kw)
 
 
-ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES4 | ra.RGI_TUNE_TILE_ORDER |
+ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES5 | ra.RGI_TUNE_TILE_ORDER |
                             ra.RGI_TUNE_TILE_SHAPE)
 
 

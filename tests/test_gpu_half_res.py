@@ -116,7 +116,7 @@ def test_half_res_gi_passes_match_oracle(dev, half_seq, variant):
     say("case half-res GI passes %s" % variant)
     opts = {"default": {},
             "xcd_blocks_waves4": {ra.RGI_OPT_TILE_ORDER: 2, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4,
-                                  ra.RGI_OPT_GI_SPATIAL_WAVES: 4},
+                                  ra.RGI_OPT_GI_SPATIAL_WAVES: 5},
             "wide_tiles": {ra.RGI_OPT_TILE_SHAPE: 1}}[variant]
     want = common.gi_passes_oracle(hs, f, g, hist, out["surface_light"], hist["hdr"])
     # The helper's frame is the sequence's own (the same passes, same inputs).

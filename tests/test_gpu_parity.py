@@ -440,8 +440,8 @@ WIDE_TILES = {ra.RGI_OPT_TILE_SHAPE: 1}
 # context (temporal waves also select spatial waves in the spatial test).
 GI_CASES = {
     "default": {},
-    "default_waves4": {"options": {ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
-    "default_waves4_xcd": {"options": {**XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 4}},
+    "default_waves4": {"options": {ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 5}},
+    "default_waves4_xcd": {"options": {**XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 5}},
     "default_xcd_tiles": {"options": XCD_TILES},
     "default_xcd_blocks": {"options": XCD_BLOCKS},
     "default_wide_tiles": {"options": WIDE_TILES},

@@ -42,7 +42,7 @@ def download(torch, ptr, like):
     return out
 
 
-ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES4 | ra.RGI_TUNE_TILE_ORDER)
+ALL_VARIANTS = ra.tuning_of(ra.RGI_TUNE_TEMPORAL_WAVES4 | ra.RGI_TUNE_SPATIAL_WAVES5 | ra.RGI_TUNE_TILE_ORDER)
 
 
 ATRIUM_DISTANCE = 0.7  # camera inside the courtyard: about 70% of pixels hit geometry
