@@ -717,8 +717,8 @@ __global__ void __launch_bounds__(kTileListWord) k_gi_tiles_reset(TileList list)
     list.words[threadIdx.x] = 0u;
 }
 
-// Lists the surface tiles of a GI pass, one tile per wave and 16 per
-// workgroup (one append per kind and workgroup), and writes the empty
+// Lists the surface tiles of a GI pass, one tile per wave and
+// kListTilesPerGroup per workgroup (one append per kind and workgroup), and writes the empty
 // reservoirs of every GI pixel of the tiles without a surface: the pixels
 // that the temporal pass writes as zeros without reading anything else.
 template <bool kSameSize>
@@ -731,7 +731,7 @@ RGI_DEV void gi_tiles_body(const rgi_frame& f, const rgi_gbuffer& g, F2 texel, c
     p.f = &f;
     p.g = g;
     set_gi_size<kSameSize>(p, f, g, texel);
-    const uint32_t t = block * 16u + (threadIdx.x >> 6);
+    const uint32_t t = block * kListTilesPerGroup + (threadIdx.x >> 6);
     const bool valid = t < list.tiles;
     int x = 0, y = 0;
     if (valid) tile_pixel(list, t, x, y);
@@ -760,7 +760,7 @@ RGI_DEV void gi_tiles_body(const rgi_frame& f, const rgi_gbuffer& g, F2 texel, c
 }
 
 template <bool kSameSize>
-__global__ void __launch_bounds__(1024) k_gi_tiles(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list,
+__global__ void __launch_bounds__(64 * kListTilesPerGroup) k_gi_tiles(rgi_frame f, rgi_gbuffer g, F2 texel, TileList list,
                                                    rgi_gi_reservoirs out)
 {
     __shared__ uint32_t counts[2], bases[2];
@@ -785,7 +785,7 @@ __global__ void __launch_bounds__(kTileListWord) k_gi_tiles_reset_batch(const Ba
 }
 
 template <bool kSameSize>
-__global__ void __launch_bounds__(1024) k_gi_tiles_batch(const BatchView* __restrict__ views,
+__global__ void __launch_bounds__(64 * kListTilesPerGroup) k_gi_tiles_batch(const BatchView* __restrict__ views,
                                                          const uint32_t* __restrict__ first, uint32_t count)
 {
     __shared__ uint32_t counts[2], bases[2];
@@ -874,8 +874,9 @@ int gi_temporal_trace(rgi_context* ctx, const rgi_frame& f, const rgi_scene& s, 
     F2 texel = F2{1.0f / (float)f.gi_width, 1.0f / (float)f.gi_height};
     const bool same = g.width == f.gi_width && g.height == f.gi_height;
     hipLaunchKernelGGL(k_gi_tiles_reset, dim3(1), dim3(kTileListWord), 0, stream, list);
-    hipLaunchKernelGGL(same ? k_gi_tiles<true> : k_gi_tiles<false>, dim3((list.tiles + 15u) / 16u), dim3(1024), 0,
-                       stream, f, g, texel, list, out);
+    hipLaunchKernelGGL(same ? k_gi_tiles<true> : k_gi_tiles<false>,
+                       dim3((list.tiles + kListTilesPerGroup - 1u) / kListTilesPerGroup), dim3(64 * kListTilesPerGroup),
+                       0, stream, f, g, texel, list, out);
     const uint32_t cus = (uint32_t)context_compute_units(ctx);
     if (f.initial_candidates > 0) {
         const uint32_t rounds = (list.tiles + 3u) / 4u, most = 8u * cus;
@@ -949,8 +950,9 @@ int gi_trace_batch(const BatchLaunch& b, hipStream_t stream)
     hipLaunchKernelGGL(k_gi_tiles_reset_batch, dim3(b.count), dim3(kTileListWord), 0, stream, b.views);
     const uint32_t* first = b.first[kPassTiles];
     const uint32_t blocks = b.blocks[kPassTiles];
-    if (b.same) hipLaunchKernelGGL(k_gi_tiles_batch<true>, dim3(blocks), dim3(1024), 0, stream, b.views, first, b.count);
-    if (b.larger) hipLaunchKernelGGL(k_gi_tiles_batch<false>, dim3(blocks), dim3(1024), 0, stream, b.views, first, b.count);
+    const dim3 threads(64 * kListTilesPerGroup);
+    if (b.same) hipLaunchKernelGGL(k_gi_tiles_batch<true>, dim3(blocks), threads, 0, stream, b.views, first, b.count);
+    if (b.larger) hipLaunchKernelGGL(k_gi_tiles_batch<false>, dim3(blocks), threads, 0, stream, b.views, first, b.count);
     const uint32_t cus = (uint32_t)b.compute_units;
     if (b.rounds > 0u) {
         const uint32_t grid = b.rounds < 8u * cus ? b.rounds : 8u * cus;

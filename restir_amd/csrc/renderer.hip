@@ -1304,7 +1304,7 @@ int render_passes(rgi_batch* b, const rgi_view* views, const rgi_gbuffer* gbuffe
             j.di_active ? slice(dg.x * dg.y) : 0u,
             slice(grid_clear_blocks(j.f.grid_table_size)),
             j.s.grid_mode == RGI_GRID_DETERMINISTIC ? 0u : slice(v.grid_gx * v.grid_gy),
-            j.split ? slice((v.list.tiles + 15u) / 16u) : 0u,
+            j.split ? slice((v.list.tiles + kListTilesPerGroup - 1u) / kListTilesPerGroup) : 0u,
             slice(gg.x * gg.y),
             denoise ? slice(svgf_blocks(j.f)) : 0u,
             composited ? slice(((uint32_t)g.width + 15u) / 16u * (((uint32_t)g.height + 15u) / 16u)) : 0u,

@@ -23,6 +23,12 @@ struct TileList {
 constexpr uint32_t kTileRayWord = 2u, kTileClaimWord = 32u, kWalkClaimWord = 32u + 8u * 32u,
                    kTileListWord = 32u + 16u * 32u;
 constexpr int kHeavyTile = 48;
+// Tiles per workgroup of the listing pass (one wave each). The listing runs
+// beside the DI passes: a 1024-thread workgroup waits for a CU with four free
+// wave slots on every SIMD, while smaller workgroups fit beside the DI waves;
+// but each workgroup appends to the list's two counters, so smaller ones
+// contend on them. 512 threads balance the two.
+constexpr uint32_t kListTilesPerGroup = 8u;
 
 // The temporal pass's candidate records (the listed form): for candidate k
 // and GI pixel o, rec[k * pixels + o] holds the march's hit point (w =
