@@ -181,7 +181,7 @@ void rgi_destroy(rgi_context* context);
  * bit-exact with every other. rgi_autotune-style callers time them on their
  * workload and keep the fastest. */
 #define RGI_OPT_GI_TEMPORAL_WAVES 1 /* occupancy target of GI temporal: 0 (compiler's choice) or 4 waves per SIMD */
-#define RGI_OPT_GI_SPATIAL_WAVES 2  /* occupancy target of GI spatial: 0 (compiler's choice) or 5 waves per SIMD */
+#define RGI_OPT_GI_SPATIAL_WAVES 2  /* occupancy target of GI spatial: 0 (compiler's choice) or 5 waves per SIMD (default) */
 #define RGI_OPT_TRACE_PERSISTENT 3  /* rgi_trace: 0 one ray per lane, 1 persistent 64-ray batches per wave */
 /* Screen passes' 16 x 16 tiles: 0 row-major; 1 a contiguous band of tiles per
  * XCD; 2 64 x 64-pixel blocks dealt to the 8 XCDs in turn. */

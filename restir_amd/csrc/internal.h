@@ -13,7 +13,7 @@ namespace rgi {
 // Launch variants (RGI_OPT_*), all bit-exact with each other.
 struct Tuning {
     int gi_temporal_waves = 0;
-    int gi_spatial_waves = 0;
+    int gi_spatial_waves = 5; // faster on every measured workload (RGI_OPT_GI_SPATIAL_WAVES)
     int trace_persistent = 0;
     int tile_order = 0;
     int tile_shape = 0; // 1: 32 x 8-pixel tiles (RGI_OPT_TILE_SHAPE)

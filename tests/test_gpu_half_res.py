@@ -49,6 +49,7 @@ def gi_passes_device(dev, hs, frame, g, hist, surface_light, previous_hdr, optio
     L = dev.L
     T = frame.grid_table_size
     ng = frame.gi_width * frame.gi_height
+    before = {opt: L.rgi_context_get_option(dev.ctx, opt) for opt in (options or {})}
     for opt, value in (options or {}).items():
         ra.check(L.rgi_context_set_option(dev.ctx, opt, value))
     try:
@@ -93,8 +94,8 @@ def gi_passes_device(dev, hs, frame, g, hist, surface_light, previous_hdr, optio
             "specular_gi": dev.get(spec, h4), "filtered": dev.get(filt, h4), "denoised": dev.get(den, h4),
         }
     finally:
-        for opt in (options or {}):
-            ra.check(L.rgi_context_set_option(dev.ctx, opt, 0))
+        for opt, value in before.items():
+            ra.check(L.rgi_context_set_option(dev.ctx, opt, value))
 
 
 KEYS = ("grid", "gi_sample", "gi_receiver", "gi_normal", "color", "moments", "specular_gi", "filtered", "denoised")

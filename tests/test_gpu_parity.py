@@ -442,9 +442,9 @@ GI_CASES = {
     "default": {},
     "default_waves4": {"options": {ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 5}},
     "default_waves4_xcd": {"options": {**XCD_TILES, ra.RGI_OPT_GI_TEMPORAL_WAVES: 4, ra.RGI_OPT_GI_SPATIAL_WAVES: 5}},
-    "default_xcd_tiles": {"options": XCD_TILES},
+    "default_xcd_tiles": {"options": {**XCD_TILES, ra.RGI_OPT_GI_SPATIAL_WAVES: 0}},
     "default_xcd_blocks": {"options": XCD_BLOCKS},
-    "default_wide_tiles": {"options": WIDE_TILES},
+    "default_wide_tiles": {"options": {**WIDE_TILES, ra.RGI_OPT_GI_SPATIAL_WAVES: 0}},
     "default_wide_xcd_blocks": {"options": {**WIDE_TILES, **XCD_BLOCKS}},
     "three_candidates": {"initial_candidates": 3},
     "short_march": {"trace_steps": 5},
@@ -489,14 +489,15 @@ def shadow_atlas(f):
 
 @contextlib.contextmanager
 def options(dev, values):
-    """Context launch variants set for the body of the block, then reset."""
+    """Context launch variants set for the body of the block, then restored."""
+    before = {opt: dev.L.rgi_context_get_option(dev.ctx, opt) for opt in values}
     try:
         for opt, value in values.items():
             ra.check(dev.L.rgi_context_set_option(dev.ctx, opt, value))
         yield
     finally:
-        for opt in values:
-            ra.check(dev.L.rgi_context_set_option(dev.ctx, opt, 0))
+        for opt, value in before.items():
+            ra.check(dev.L.rgi_context_set_option(dev.ctx, opt, value))
 
 
 @pytest.mark.parametrize("case", sorted(GI_CASES))
